@@ -1,0 +1,376 @@
+"""CaMaFlood: the model driver (cmfgpu/models/cama_flood_model.py:46-173).
+
+Holds every module field as a device tensor, owns the libcmfhip network
+context, and advances outer steps either through the fused C-ABI schedule
+(``mode="fused"``, one call per outer step) or through the reference's own
+per-op sequence (``mode="ops"``, the zero-argument registered ops called in
+the order of step_advance, cama_flood_model.py:104-173).
+
+Device layout.  With ``layout="level"`` (the default) catchments are
+renumbered breadth-first from the mouths (network.level_order) so that the
+upstream catchments of every catchment are one contiguous range; with
+``layout="network"`` the reference's own order is kept.  With trials,
+``trial_layout="inner"`` stores element (trial t, catchment i) at i*T + t
+(all members of a catchment adjacent), ``"major"`` at t*N + i (the
+reference's [T, N]).  ``get``/``set_inputs`` always speak the reference's
+order and shapes.
+"""
+from __future__ import annotations
+
+import ctypes
+from datetime import datetime, timedelta
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .. import network as netmod
+from ..phys import (compute_adaptive_time_step, compute_bifurcation_inflow, compute_bifurcation_outflow,
+                    compute_flood_stage, compute_flood_stage_log, compute_inflow,
+                    compute_levee_bifurcation_outflow, compute_levee_stage, compute_levee_stage_log,
+                    compute_outflow, compute_reservoir_outflow)
+from ..phys.registry import bind
+from ..state import assemble
+
+CATCHMENT_SHARED = ("river_width", "river_length", "river_height", "catchment_elevation", "catchment_area",
+                    "downstream_distance", "river_manning", "flood_manning", "flood_depth_table", "runoff",
+                    "is_levee", "is_reservoir", "is_dam_related", "is_dam_upstream")
+CATCHMENT_STATE = ("river_storage", "flood_storage", "protected_storage", "protected_depth", "river_depth",
+                   "flood_depth", "river_outflow", "flood_outflow", "river_cross_section_depth",
+                   "flood_cross_section_depth", "flood_cross_section_area", "flood_fraction", "river_inflow",
+                   "flood_inflow", "outgoing_storage", "total_storage", "water_surface_elevation",
+                   "protected_water_surface_elevation", "global_bifurcation_outflow", "limit_rate",
+                   "reservoir_total_inflow")
+PATH_SHARED = ("bifurcation_manning", "bifurcation_width", "bifurcation_length", "bifurcation_elevation")
+PATH_STATE = ("bifurcation_outflow", "bifurcation_cross_section_depth")
+LEVEE_PARAMS = ("levee_base_height", "levee_crown_height", "levee_fraction")
+RESERVOIR_PARAMS = ("conservation_volume", "emergency_volume", "adjustment_volume", "effective_normal_outflow",
+                    "adjustment_outflow", "flood_control_outflow")
+INDEX_FIELDS = ("downstream_idx", "bifurcation_catchment_idx", "bifurcation_downstream_idx",
+                "levee_catchment_idx", "reservoir_catchment_idx")
+LOG_PTR_NAMES = {name: k for k, name in enumerate(_lib.LOG_FIELDS)}
+
+
+def _torch_dtype(a: np.ndarray):
+    return {np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64,
+            np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64,
+            np.dtype(np.bool_): torch.bool}[a.dtype]
+
+
+class CaMaFlood:
+    def __init__(self, src: dict, *, opened_modules=("base", "adaptive_time"), precision="float32",
+                 device="cuda:0", num_trials: int = 1, batched: dict | None = None,
+                 per_trial_time_step: bool = False, layout: str = "level", trial_layout: str = "inner",
+                 log_buffer_size: int = 20000, step_duration: float = 1800.0, num_sub_steps: int = 1,
+                 start_time: datetime = datetime(2000, 1, 1), log_path: str | Path | None = None,
+                 mode: str = "fused", gravity: float = 9.8, adaptive_time_factor: float = 0.7,
+                 min_kinematic_slope: float = 1e-5):
+        if mode not in ("fused", "ops"):
+            raise ValueError("mode must be 'fused' or 'ops'")
+        if layout not in ("level", "network"):
+            raise ValueError("layout must be 'level' or 'network'")
+        if trial_layout not in ("inner", "major"):
+            raise ValueError("trial_layout must be 'inner' or 'major'")
+        if num_trials < 1:
+            raise ValueError("num_trials must be >= 1")
+        opened = tuple(dict.fromkeys(("base",) + tuple(opened_modules)))
+        if "log" in opened and num_trials > 1:
+            raise ValueError("the log module does not support trials (cama_flood_model.py:65-67)")
+        if mode == "ops" and per_trial_time_step:
+            raise ValueError("per-trial time steps need the fused schedule")
+        self.opened_modules = opened
+        self.mode = mode
+        self.device = torch.device(device)
+        self.T = int(num_trials)
+        self.per_trial = bool(per_trial_time_step)
+        self.trial_layout = trial_layout if self.T > 1 else "major"
+        self.step_duration = float(step_duration)
+        self.num_sub_steps = int(num_sub_steps)
+        self.current_time = start_time
+        self.log_path = Path(log_path) if log_path is not None else None
+        self.host = assemble(src, opened, precision=precision, log_buffer_size=log_buffer_size,
+                             gravity=gravity, adaptive_time_factor=adaptive_time_factor,
+                             min_kinematic_slope=min_kinematic_slope)
+        self.flags = self.host.flags | (_lib.PER_TRIAL_DT if self.per_trial else 0)
+        self.real = self.host.real
+        hf = self.host.fields
+        self.N = self.host.num_catchments
+        self.P = int(hf["bifurcation_catchment_idx"].shape[0]) if "bifurcation_catchment_idx" in hf else 0
+        self.Lb = self.host.num_bifurcation_levels
+        self.V = int(hf["levee_catchment_idx"].shape[0]) if "levee_catchment_idx" in hf else 0
+        self.R = int(hf["reservoir_catchment_idx"].shape[0]) if "reservoir_catchment_idx" in hf else 0
+        self.L = self.host.num_flood_levels
+
+        dn = hf["downstream_idx"]
+        if layout == "level":
+            self.order = netmod.level_order(dn)
+            self.perm = self.order.perm.astype(np.int64)
+            self.inv = self.order.inv.astype(np.int64)
+            dn_int = self.order.dn.astype(np.int32)
+        else:
+            netmod.check_flow_direction(dn)
+            self.order = None
+            self.perm = np.arange(self.N, dtype=np.int64)
+            self.inv = self.perm.copy()
+            dn_int = dn.astype(np.int32)
+        self.layout = layout
+
+        batched = dict(batched or {})
+        self.batched_names = set(batched)
+        bits = 0
+        for name in batched:
+            if name not in _lib.BATCHED_BITS:
+                raise ValueError(f"{name} cannot be batched")
+            bits |= _lib.BATCHED_BITS[name]
+        self.batched_bits = bits
+
+        self.t = {}
+        self.t["downstream_idx"] = torch.from_numpy(dn_int).to(self.device)
+        for name in CATCHMENT_SHARED:
+            if name in hf:
+                src_arr = batched.get(name, hf[name])
+                self.t[name] = self._to_catchment(name, np.asarray(src_arr, dtype=hf[name].dtype),
+                                                  per_trial=name in batched)
+        for name in CATCHMENT_STATE:
+            if name in hf:
+                self.t[name] = self._to_catchment(name, hf[name], per_trial=True, replicate=True)
+        if self.P:
+            ci = self.inv[hf["bifurcation_catchment_idx"]].astype(np.int32)
+            di = self.inv[hf["bifurcation_downstream_idx"]].astype(np.int32)
+            self.t["bifurcation_catchment_idx"] = torch.from_numpy(ci).to(self.device)
+            self.t["bifurcation_downstream_idx"] = torch.from_numpy(di).to(self.device)
+            for name in PATH_SHARED:
+                src_arr = batched.get(name, hf[name])
+                self.t[name] = self._to_path(np.asarray(src_arr, dtype=hf[name].dtype), per_trial=name in batched)
+            for name in PATH_STATE:
+                self.t[name] = self._to_path(hf[name], per_trial=True, replicate=True)
+        else:
+            ci = di = np.zeros(0, np.int32)
+        if self.V:
+            self.t["levee_catchment_idx"] = torch.from_numpy(self.inv[hf["levee_catchment_idx"]].astype(np.int32)).to(self.device)
+            for name in LEVEE_PARAMS:
+                src_arr = np.asarray(batched.get(name, hf[name]), dtype=hf[name].dtype)
+                self.t[name] = torch.from_numpy(np.ascontiguousarray(src_arr)).to(self.device)
+        if self.R:
+            self.t["reservoir_catchment_idx"] = torch.from_numpy(self.inv[hf["reservoir_catchment_idx"]].astype(np.int32)).to(self.device)
+            for name in RESERVOIR_PARAMS:
+                self.t[name] = torch.from_numpy(np.ascontiguousarray(hf[name])).to(self.device)
+        if "log_sums" in hf:
+            self.t["log_sums"] = torch.from_numpy(hf["log_sums"].copy()).to(self.device)
+        tdt = torch.float32 if self.real == np.float32 else torch.float64
+        slots = self.T if self.per_trial else 1
+        self.t["time_step"] = torch.zeros(slots, dtype=tdt, device=self.device)
+        self.t["outer_time_step"] = torch.zeros(1, dtype=tdt, device=self.device)
+        self.t["current_step"] = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self.t["max_sub_steps"] = torch.zeros(slots, dtype=torch.int32, device=self.device)
+
+        # network context (host indices in device order)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().cmf_network_create(
+            self.N, dn_int.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), self.P,
+            ci.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), di.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            self.T, ctypes.byref(h)), "cmf_network_create")
+        self.network_handle = h
+        self.contiguous = bool(_lib.lib().cmf_network_is_contiguous(h))
+        self._table = self._build_table()
+        self.step_count = 0
+        self.last_sub_steps = 0
+
+    # ------------------------------------------------------------ layout --
+    def _to_catchment(self, name, arr, per_trial=False, replicate=False):
+        a = np.asarray(arr)
+        if per_trial and self.T > 1:
+            if replicate:
+                a = np.broadcast_to(a, (self.T,) + a.shape)
+            if a.shape[0] != self.T:
+                raise ValueError(f"{name}: expected a leading trial dimension of {self.T}")
+            a = a[:, self.perm]                      # [T, N, ...]
+            if self.trial_layout == "inner":
+                a = np.swapaxes(a, 0, 1)              # [N, T, ...]
+        else:
+            if a.ndim >= 1 and a.shape[0] == self.T and self.T > 1 and not replicate:
+                raise ValueError(f"{name} has a trial dimension but is not batched")
+            a = a[self.perm]
+        return torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+
+    def _to_path(self, arr, per_trial=False, replicate=False):
+        a = np.asarray(arr)
+        if per_trial and self.T > 1:
+            if replicate:
+                a = np.broadcast_to(a, (self.T,) + a.shape)
+            if self.trial_layout == "inner":
+                a = np.swapaxes(a, 0, 1)              # [P, T, ...]
+        return torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+
+    def _strides(self):
+        if self.T > 1 and self.trial_layout == "inner":
+            return 1, self.T, 1, self.T
+        return self.N, 1, self.P, 1
+
+    def _build_table(self) -> _lib.CmfFields:
+        f = _lib.CmfFields()
+        f.num_catchments = self.N
+        f.num_trials = self.T
+        f.num_flood_levels = self.L
+        f.num_bifurcation_paths = self.P
+        f.num_bifurcation_levels = self.Lb
+        f.num_levees = self.V
+        f.num_reservoirs = self.R
+        f.log_buffer_size = self.host.log_buffer_size
+        f.trial_stride, f.catchment_stride, f.path_trial_stride, f.path_stride = self._strides()
+        f.precision = _lib.PRECISION_F32_F64 if self.real == np.float32 else _lib.PRECISION_F64_F64
+        f.flags = self.flags
+        f.batched = self.batched_bits
+        f.gravity = self.host.gravity
+        f.adaptive_time_factor = self.host.adaptive_time_factor
+        f.min_kinematic_slope = self.host.min_kinematic_slope
+        for name in _lib.POINTER_FIELDS:
+            tensor = self.t.get(name)
+            setattr(f, name, ctypes.c_void_p(tensor.data_ptr()) if tensor is not None else None)
+        if "log_sums" in self.t:
+            ls = self.t["log_sums"]
+            row = ls.stride(0) * ls.element_size()
+            for k in range(len(_lib.LOG_FIELDS)):
+                f.log_sums[k] = ls.data_ptr() + k * row
+        return f
+
+    def field_table_ref(self):
+        return ctypes.byref(self._table)
+
+    # ---------------------------------------------------------- fields ----
+    def get(self, name: str) -> np.ndarray:
+        """A field in the reference's order and shape ([T, ...] for per-trial fields when T > 1)."""
+        x = self.t[name].cpu().numpy()
+        if name in INDEX_FIELDS:
+            if name == "downstream_idx":
+                return self.host.fields["downstream_idx"].copy()
+            return self.perm[x].astype(np.int32)
+        if name in CATCHMENT_STATE or name in CATCHMENT_SHARED:
+            per_trial = name in CATCHMENT_STATE or name in self.batched_names
+            if per_trial and self.T > 1:
+                if self.trial_layout == "inner":
+                    x = np.swapaxes(x, 0, 1)
+                return x[:, self.inv]
+            return x[self.inv]
+        if name in PATH_STATE or name in PATH_SHARED:
+            per_trial = name in PATH_STATE or name in self.batched_names
+            if per_trial and self.T > 1 and self.trial_layout == "inner":
+                x = np.swapaxes(x, 0, 1)
+            return np.ascontiguousarray(x)
+        return x
+
+    def log_sums(self) -> np.ndarray:
+        return self.t["log_sums"].cpu().numpy()
+
+    def set_inputs(self, runoff) -> None:
+        """Stage runoff forcing (cama_flood_model.py:74-102): [N], or [T, N] when runoff is batched."""
+        r = runoff.detach().cpu().numpy() if isinstance(runoff, torch.Tensor) else np.asarray(runoff)
+        r = r.astype(self.real, copy=False)
+        per_trial = "runoff" in self.batched_names
+        expect = (self.T, self.N) if (per_trial and self.T > 1) else (self.N,)
+        if r.shape != expect:
+            raise ValueError(f"runoff must have shape {expect}, got {r.shape}")
+        if per_trial and self.T > 1:
+            r = r[:, self.perm]
+            if self.trial_layout == "inner":
+                r = r.T
+        else:
+            r = r[self.perm]
+        self.t["runoff"].copy_(torch.from_numpy(np.ascontiguousarray(r)))
+
+    def set_runoff_device(self, runoff_internal: torch.Tensor) -> None:
+        """Copy runoff already in the device layout (for benchmarks that keep forcing on the GPU)."""
+        self.t["runoff"].copy_(runoff_internal)
+
+    def check_spec(self, spec) -> None:
+        enabled = set()
+        for feat, (kind, key) in spec.feature_sources.items():
+            if kind == "module_enabled" and key in self.opened_modules:
+                enabled.add(feat)
+            elif kind == "module_flag":
+                base = self.host.modules["base"]
+                if getattr(base, key):
+                    enabled.add(feat)
+        missing = []
+        for name in spec.required_fields(enabled):
+            if name in LOG_PTR_NAMES:
+                if "log_sums" not in self.t:
+                    missing.append(name)
+            elif name not in self.t:
+                missing.append(name)
+        if missing:
+            raise ValueError(f"{spec.name}: model has no field(s) {missing}")
+
+    # ------------------------------------------------------------ step ----
+    def step_advance(self) -> int:
+        """Advance one outer step; returns the number of sub-steps used."""
+        if self.mode == "fused":
+            count = ctypes.c_int32(0)
+            _lib.check(_lib.lib().cmf_hip_advance(self.network_handle, self.field_table_ref(), self.step_duration,
+                                                  self.num_sub_steps, ctypes.byref(count), _lib.current_stream()),
+                       "cmf_hip_advance")
+            n = int(count.value)
+        else:
+            n = self._step_ops()
+        self.last_sub_steps = n
+        log = self.host.modules.get("log")
+        if log is not None:
+            log.set_time(self.step_duration / n, n, self.current_time)
+            if self.log_path is not None:
+                log.write_step(self.log_path, self.log_sums())
+            self.t["log_sums"].zero_()
+        self.current_time = self.current_time + timedelta(seconds=self.step_duration)
+        self.step_count += 1
+        return n
+
+    def _step_ops(self) -> int:
+        """The reference's sub-step sequence, op by op (cama_flood_model.py:104-173)."""
+        has = set(self.opened_modules)
+        with torch.inference_mode(), bind(self):
+            self.t["outer_time_step"].fill_(self.step_duration)
+            if "adaptive_time" in has:
+                self.t["max_sub_steps"].zero_()
+                compute_adaptive_time_step()
+                count = int(self.t["max_sub_steps"].max().item())
+            else:
+                count = self.num_sub_steps
+            self.t["time_step"].fill_(self.step_duration / count)
+            log = "log" in has
+            for k in range(count):
+                if log:
+                    self.t["current_step"].fill_(k)
+                compute_outflow()
+                if "reservoir" in has:
+                    compute_reservoir_outflow()
+                if "bifurcation" in has:
+                    if "levee" in has:
+                        compute_levee_bifurcation_outflow()
+                    else:
+                        compute_bifurcation_outflow()
+                compute_inflow()
+                if "bifurcation" in has:
+                    compute_bifurcation_inflow()
+                if log:
+                    compute_flood_stage_log()
+                else:
+                    compute_flood_stage()
+                if "levee" in has:
+                    if log:
+                        compute_levee_stage_log()
+                    else:
+                        compute_levee_stage()
+            if not log:
+                self.t["current_step"].fill_(count - 1)
+        return count
+
+    def close(self) -> None:
+        if getattr(self, "network_handle", None):
+            _lib.lib().cmf_network_destroy(self.network_handle)
+            self.network_handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

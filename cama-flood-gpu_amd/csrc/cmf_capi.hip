@@ -1,0 +1,664 @@
+// C ABI of libcmfhip: network context, per-op entry points and the fused
+// sub-step schedule.  Declarations and contracts are in include/cmf_hip.h.
+
+#include "cmf_kernels.hip"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace cmf {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define CMF_HIP_CHECK(expr)                                                            \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return set_error(CMF_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+}  // namespace cmf
+
+using namespace cmf;
+
+struct cmf_network {
+    int64_t n = 0, T = 0, P = 0;
+    int contiguous = 0;
+    int device = 0;
+    int32_t* up_ptr = nullptr;
+    int32_t* up_idx = nullptr;
+    int32_t* ev_ptr = nullptr;
+    int32_t* ev_code = nullptr;
+    void* pos_term = nullptr;
+    void* neg_term = nullptr;
+    void* qraw_r = nullptr;
+    void* qraw_f = nullptr;
+    void* lim = nullptr;
+    void* path_pos = nullptr;
+    void* path_neg = nullptr;
+    void* path_net = nullptr;
+    double* partials = nullptr;
+    int64_t partial_blocks = 0;
+    int32_t* host_counts = nullptr;   // pinned, T entries
+};
+
+namespace {
+
+void free_network(cmf_network* net) {
+    if (!net) return;
+    void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->pos_term,
+                    net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
+                    net->path_neg, net->path_net, net->partials};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (net->host_counts) (void)hipHostFree(net->host_counts);
+    delete net;
+}
+
+int dev_alloc(void** p, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_error(CMF_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return CMF_OK;
+}
+
+int upload(void** p, const void* host, size_t bytes) {
+    int rc = dev_alloc(p, bytes);
+    if (rc) return rc;
+    if (bytes) CMF_HIP_CHECK(hipMemcpy(*p, host, bytes, hipMemcpyHostToDevice));
+    return CMF_OK;
+}
+
+int64_t blocks_for(int64_t count) { return (count + kBlock - 1) / kBlock; }
+
+// --------------------------------------------------------------- view ---
+template <class REAL, class STO>
+int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
+    if (!net || !f) return set_error(CMF_ERR_INVALID, "null network or field table");
+    if (f->num_catchments != net->n)
+        return set_error(CMF_ERR_INVALID, "num_catchments %lld does not match the network (%lld)",
+                         (long long)f->num_catchments, (long long)net->n);
+    if (f->num_trials < 1 || f->num_trials > net->T)
+        return set_error(CMF_ERR_INVALID, "num_trials %lld outside [1, %lld] of the network",
+                         (long long)f->num_trials, (long long)net->T);
+    if ((f->flags & CMF_HAS_BIFURCATION) && f->num_bifurcation_paths != net->P)
+        return set_error(CMF_ERR_INVALID, "num_bifurcation_paths does not match the network");
+    std::memset(&v, 0, sizeof(v));
+    v.N = f->num_catchments;
+    v.T = f->num_trials;
+    v.L = f->num_flood_levels;
+    v.P = (f->flags & CMF_HAS_BIFURCATION) ? f->num_bifurcation_paths : 0;
+    v.LB = f->num_bifurcation_levels;
+    v.V = (f->flags & CMF_HAS_LEVEE) ? f->num_levees : 0;
+    v.R = (f->flags & CMF_HAS_RESERVOIR) ? f->num_reservoirs : 0;
+    v.logsize = f->log_buffer_size;
+    v.ts = f->trial_stride ? f->trial_stride : v.N;
+    v.cs = f->catchment_stride ? f->catchment_stride : 1;
+    v.pts = f->path_trial_stride ? f->path_trial_stride : v.P;
+    v.pps = f->path_stride ? f->path_stride : 1;
+    if (v.T > 1) {
+        const bool major = (v.cs == 1 && v.ts == v.N);
+        const bool inner = (v.ts == 1 && v.cs == v.T);
+        if (!major && !inner)
+            return set_error(CMF_ERR_INVALID, "trial layout must be [trials, catchments] or [catchments, trials]");
+        const bool pmajor = (v.pps == 1 && v.pts == v.P);
+        const bool pinner = (v.pts == 1 && v.pps == v.T);
+        if (v.P > 0 && !pmajor && !pinner)
+            return set_error(CMF_ERR_INVALID, "bifurcation trial layout must be [trials, paths] or [paths, trials]");
+    }
+    if (v.L < 1) return set_error(CMF_ERR_INVALID, "num_flood_levels must be >= 1");
+    v.flags = f->flags;
+    v.batched = f->batched;
+    v.gravity = (REAL)f->gravity;
+    v.adaptive_factor = (REAL)f->adaptive_time_factor;
+    v.min_kin = (REAL)f->min_kinematic_slope;
+    v.time_step = static_cast<const REAL*>(f->time_step);
+    v.outer_time_step = static_cast<const REAL*>(f->outer_time_step);
+    v.current_step = f->current_step;
+    v.max_sub_steps = f->max_sub_steps;
+    v.dn = f->downstream_idx;
+#define CMF_SET(dst, src, type) v.dst = static_cast<type>(f->src)
+    CMF_SET(river_inflow, river_inflow, STO*);
+    CMF_SET(flood_inflow, flood_inflow, STO*);
+    CMF_SET(river_outflow, river_outflow, REAL*);
+    CMF_SET(flood_outflow, flood_outflow, REAL*);
+    CMF_SET(river_manning, river_manning, const REAL*);
+    CMF_SET(flood_manning, flood_manning, const REAL*);
+    CMF_SET(river_depth, river_depth, REAL*);
+    CMF_SET(flood_depth, flood_depth, REAL*);
+    CMF_SET(protected_depth, protected_depth, REAL*);
+    CMF_SET(river_width, river_width, const REAL*);
+    CMF_SET(river_length, river_length, const REAL*);
+    CMF_SET(river_height, river_height, const REAL*);
+    CMF_SET(elev, catchment_elevation, const REAL*);
+    CMF_SET(catchment_area, catchment_area, const REAL*);
+    CMF_SET(dist, downstream_distance, const REAL*);
+    CMF_SET(river_storage, river_storage, STO*);
+    CMF_SET(flood_storage, flood_storage, STO*);
+    CMF_SET(protected_storage, protected_storage, STO*);
+    CMF_SET(rcsd, river_cross_section_depth, REAL*);
+    CMF_SET(fcsd, flood_cross_section_depth, REAL*);
+    CMF_SET(fcsa, flood_cross_section_area, REAL*);
+    CMF_SET(flood_fraction, flood_fraction, REAL*);
+    CMF_SET(runoff, runoff, const REAL*);
+    CMF_SET(table, flood_depth_table, const REAL*);
+    CMF_SET(outgoing, outgoing_storage, STO*);
+    CMF_SET(total_storage, total_storage, STO*);
+    CMF_SET(wse, water_surface_elevation, REAL*);
+    CMF_SET(pwse, protected_water_surface_elevation, REAL*);
+    v.is_levee = f->is_levee;
+    v.bif_ci = f->bifurcation_catchment_idx;
+    v.bif_di = f->bifurcation_downstream_idx;
+    CMF_SET(bif_manning, bifurcation_manning, const REAL*);
+    CMF_SET(bif_width, bifurcation_width, const REAL*);
+    CMF_SET(bif_length, bifurcation_length, const REAL*);
+    CMF_SET(bif_elev, bifurcation_elevation, const REAL*);
+    CMF_SET(bif_outflow, bifurcation_outflow, REAL*);
+    CMF_SET(bif_csd, bifurcation_cross_section_depth, REAL*);
+    CMF_SET(gbif, global_bifurcation_outflow, STO*);
+    CMF_SET(limit_rate, limit_rate, REAL*);
+    v.levee_idx = f->levee_catchment_idx;
+    CMF_SET(levee_base, levee_base_height, const REAL*);
+    CMF_SET(levee_crown, levee_crown_height, const REAL*);
+    CMF_SET(levee_frac, levee_fraction, const REAL*);
+    v.res_idx = f->reservoir_catchment_idx;
+    CMF_SET(res_cons, conservation_volume, const REAL*);
+    CMF_SET(res_emer, emergency_volume, const REAL*);
+    CMF_SET(res_adj, adjustment_volume, const REAL*);
+    CMF_SET(res_qn, effective_normal_outflow, const REAL*);
+    CMF_SET(res_qa, adjustment_outflow, const REAL*);
+    CMF_SET(res_qf, flood_control_outflow, const REAL*);
+    CMF_SET(res_inflow, reservoir_total_inflow, STO*);
+#undef CMF_SET
+    v.is_reservoir = f->is_reservoir;
+    v.is_dam_up = f->is_dam_upstream;
+    v.is_dam_related = f->is_dam_related;
+    for (int k = 0; k < CMF_LOG_COUNT; ++k) v.log_sums[k] = f->log_sums[k];
+    v.up_ptr = net->up_ptr;
+    v.up_idx = net->contiguous ? nullptr : net->up_idx;
+    v.ev_ptr = net->ev_ptr;
+    v.ev_code = net->ev_code;
+    v.pos_term = static_cast<REAL*>(net->pos_term);
+    v.neg_term = static_cast<REAL*>(net->neg_term);
+    v.qraw_r = static_cast<REAL*>(net->qraw_r);
+    v.qraw_f = static_cast<REAL*>(net->qraw_f);
+    v.lim = static_cast<REAL*>(net->lim);
+    v.path_pos = static_cast<REAL*>(net->path_pos);
+    v.path_neg = static_cast<REAL*>(net->path_neg);
+    v.path_net = static_cast<REAL*>(net->path_net);
+    v.partials = net->partials;
+    return CMF_OK;
+}
+
+// Required-pointer checks, grouped by the fields each op touches.
+struct Need {
+    const char* name;
+    const void* p;
+};
+int require(std::initializer_list<Need> needs, const char* op) {
+    for (const Need& n : needs)
+        if (!n.p) return set_error(CMF_ERR_INVALID, "%s: field %s is NULL", op, n.name);
+    return CMF_OK;
+}
+#define NEED(field) Need{#field, (const void*)f->field}
+
+int need_base(const cmf_fields* f, const char* op) {
+    return require({NEED(time_step), NEED(downstream_idx), NEED(river_inflow), NEED(flood_inflow),
+                    NEED(river_outflow), NEED(flood_outflow), NEED(river_manning), NEED(flood_manning),
+                    NEED(river_depth), NEED(flood_depth), NEED(protected_depth), NEED(river_width),
+                    NEED(river_length), NEED(river_height), NEED(catchment_elevation),
+                    NEED(catchment_area), NEED(downstream_distance), NEED(river_storage),
+                    NEED(flood_storage), NEED(protected_storage), NEED(river_cross_section_depth),
+                    NEED(flood_cross_section_depth), NEED(flood_cross_section_area),
+                    NEED(flood_fraction), NEED(runoff), NEED(flood_depth_table)},
+                   op);
+}
+int need_features(const cmf_fields* f, const char* op) {
+    int rc;
+    if (f->flags & CMF_HAS_TOTAL_STORAGE) if ((rc = require({NEED(total_storage)}, op))) return rc;
+    if (f->flags & CMF_HAS_WATER_SURFACE) if ((rc = require({NEED(water_surface_elevation)}, op))) return rc;
+    if (f->flags & CMF_HAS_PROTECTED_WATER_SURFACE)
+        if ((rc = require({NEED(protected_water_surface_elevation)}, op))) return rc;
+    if (f->flags & CMF_HAS_LEVEE)
+        if ((rc = require({NEED(is_levee), NEED(levee_catchment_idx), NEED(levee_base_height),
+                           NEED(levee_crown_height), NEED(levee_fraction)}, op))) return rc;
+    if (f->flags & CMF_HAS_BIFURCATION) {
+        if ((rc = require({NEED(bifurcation_catchment_idx), NEED(bifurcation_downstream_idx),
+                           NEED(bifurcation_manning), NEED(bifurcation_outflow), NEED(bifurcation_width),
+                           NEED(bifurcation_length), NEED(bifurcation_elevation),
+                           NEED(bifurcation_cross_section_depth), NEED(global_bifurcation_outflow),
+                           NEED(limit_rate), NEED(water_surface_elevation), NEED(total_storage)}, op)))
+            return rc;
+        if (f->num_bifurcation_levels < 1)
+            return set_error(CMF_ERR_INVALID, "%s: num_bifurcation_levels must be >= 1", op);
+    }
+    if (f->flags & CMF_HAS_RESERVOIR)
+        if ((rc = require({NEED(reservoir_catchment_idx), NEED(conservation_volume), NEED(emergency_volume),
+                           NEED(adjustment_volume), NEED(effective_normal_outflow),
+                           NEED(adjustment_outflow), NEED(flood_control_outflow),
+                           NEED(reservoir_total_inflow), NEED(is_reservoir), NEED(is_dam_upstream),
+                           NEED(is_dam_related), NEED(total_storage)}, op))) return rc;
+    if (f->flags & CMF_HAS_LOG) {
+        for (int k = 0; k < CMF_LOG_COUNT; ++k)
+            if (!f->log_sums[k]) return set_error(CMF_ERR_INVALID, "%s: log_sums[%d] is NULL", op, k);
+        if ((rc = require({NEED(current_step)}, op))) return rc;
+        if (f->num_trials > 1) return set_error(CMF_ERR_UNSUPPORTED, "%s: the log module does not support trials", op);
+    }
+    if ((f->flags & CMF_PER_TRIAL_DT) && !f->max_sub_steps)
+        return set_error(CMF_ERR_INVALID, "%s: per-trial time steps need max_sub_steps", op);
+    return CMF_OK;
+}
+
+#define LAUNCH(kernel, count, stream, ...)                                                   \
+    do {                                                                                     \
+        const int64_t _cnt = (count);                                                        \
+        if (_cnt > 0) {                                                                      \
+            hipLaunchKernelGGL(kernel, dim3((unsigned)blocks_for(_cnt)), dim3(kBlock), 0,    \
+                               (hipStream_t)(stream), __VA_ARGS__);                          \
+            hipError_t _e = hipGetLastError();                                               \
+            if (_e != hipSuccess)                                                            \
+                return set_error(CMF_ERR_HIP, "launch of %s failed: %s", #kernel,            \
+                                 hipGetErrorString(_e));                                     \
+        }                                                                                    \
+    } while (0)
+
+constexpr uint32_t kLogAll = (1u << CMF_LOG_COUNT) - 1;
+constexpr uint32_t kLogStage = (1u << CMF_LOG_STORAGE_STAGE) | (1u << CMF_LOG_RIVER_STORAGE) |
+                               (1u << CMF_LOG_FLOOD_STORAGE) | (1u << CMF_LOG_FLOOD_AREA) |
+                               (1u << CMF_LOG_STAGE_ERROR);
+
+template <class REAL, class STO>
+int log_final(const View<REAL, STO>& v, int64_t nblocks, uint32_t mask, void* stream) {
+    hipLaunchKernelGGL((k_log_final<REAL, STO>), dim3(1), dim3(kBlock), 0, (hipStream_t)stream, v,
+                       nblocks, mask);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(CMF_ERR_HIP, "launch of k_log_final failed: %s", hipGetErrorString(e));
+    return CMF_OK;
+}
+
+// ----------------------------------------------------------- per-op -----
+enum Op {
+    OP_OUTFLOW, OP_INFLOW, OP_STAGE, OP_STAGE_LOG, OP_CFL, OP_BIF_OUT, OP_LEVEE_BIF_OUT,
+    OP_BIF_IN, OP_LEVEE, OP_LEVEE_LOG, OP_RESERVOIR
+};
+
+template <class REAL, class STO>
+int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
+    View<REAL, STO> v;
+    int rc = make_view<REAL, STO>(net, f, v);
+    if (rc) return rc;
+    const int64_t NT = v.N * v.T;
+    const bool contig = net->contiguous != 0;
+    switch (op) {
+        case OP_OUTFLOW:
+            if ((rc = need_base(f, "compute_outflow")) || (rc = need_features(f, "compute_outflow"))) return rc;
+            if (!f->outgoing_storage) return set_error(CMF_ERR_INVALID, "compute_outflow: outgoing_storage is NULL");
+            LAUNCH((k_outflow<REAL, STO, false>), NT, stream, v, -1);
+            if (contig) LAUNCH((k_outgoing_sum<REAL, STO, true>), NT, stream, v);
+            else LAUNCH((k_outgoing_sum<REAL, STO, false>), NT, stream, v);
+            return CMF_OK;
+        case OP_INFLOW:
+            if ((rc = require({NEED(downstream_idx), NEED(river_outflow), NEED(flood_outflow),
+                               NEED(river_storage), NEED(flood_storage), NEED(outgoing_storage),
+                               NEED(river_inflow), NEED(flood_inflow)}, "compute_inflow")))
+                return rc;
+            if ((f->flags & CMF_HAS_BIFURCATION) && !f->limit_rate)
+                return set_error(CMF_ERR_INVALID, "compute_inflow: limit_rate is NULL");
+            if ((f->flags & CMF_HAS_RESERVOIR) && (!f->reservoir_total_inflow || !f->is_reservoir))
+                return set_error(CMF_ERR_INVALID, "compute_inflow: reservoir fields are NULL");
+            LAUNCH((k_inflow_limit<REAL, STO>), NT, stream, v);
+            if (contig) LAUNCH((k_inflow_sum<REAL, STO, true>), NT, stream, v);
+            else LAUNCH((k_inflow_sum<REAL, STO, false>), NT, stream, v);
+            return CMF_OK;
+        case OP_STAGE:
+        case OP_STAGE_LOG: {
+            const bool log = (op == OP_STAGE_LOG);
+            const char* name = log ? "compute_flood_stage_log" : "compute_flood_stage";
+            if ((rc = need_base(f, name))) return rc;
+            if (!f->outgoing_storage) return set_error(CMF_ERR_INVALID, "%s: outgoing_storage is NULL", name);
+            if ((f->flags & CMF_HAS_BIFURCATION) && !f->global_bifurcation_outflow)
+                return set_error(CMF_ERR_INVALID, "%s: global_bifurcation_outflow is NULL", name);
+            if ((f->flags & CMF_HAS_LEVEE) && log && !f->is_levee)
+                return set_error(CMF_ERR_INVALID, "%s: is_levee is NULL", name);
+            if (log) {
+                cmf_fields g = *f;
+                g.flags |= CMF_HAS_LOG;
+                if ((rc = need_features(&g, name))) return rc;
+                if (blocks_for(NT) > net->partial_blocks)
+                    return set_error(CMF_ERR_INTERNAL, "%s: partial buffer too small", name);
+                LAUNCH((k_stage<REAL, STO, false, false, true>), NT, stream, v, -1, 0);
+                return log_final(v, blocks_for(NT), kLogAll, stream);
+            }
+            LAUNCH((k_stage<REAL, STO, false, false, false>), NT, stream, v, -1, 0);
+            return CMF_OK;
+        }
+        case OP_CFL: {
+            if ((rc = require({NEED(river_depth), NEED(downstream_distance), NEED(max_sub_steps),
+                               NEED(outer_time_step)}, "compute_adaptive_time_step")))
+                return rc;
+            if ((f->flags & CMF_HAS_RESERVOIR) && !f->is_dam_related)
+                return set_error(CMF_ERR_INVALID, "compute_adaptive_time_step: is_dam_related is NULL");
+            if (v.N == 0) return CMF_OK;
+            const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 2048);
+            hipLaunchKernelGGL((k_cfl<REAL, STO>), dim3((unsigned)chunks, (unsigned)v.T), dim3(kBlock), 0,
+                               (hipStream_t)stream, v);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return set_error(CMF_ERR_HIP, "launch of k_cfl failed: %s", hipGetErrorString(e));
+            return CMF_OK;
+        }
+        case OP_BIF_OUT:
+        case OP_LEVEE_BIF_OUT: {
+            const char* name = op == OP_BIF_OUT ? "compute_bifurcation_outflow" : "compute_levee_bifurcation_outflow";
+            if (!(f->flags & CMF_HAS_BIFURCATION)) return set_error(CMF_ERR_INVALID, "%s: bifurcation is off", name);
+            if ((rc = need_features(f, name))) return rc;
+            if ((rc = require({NEED(outgoing_storage), NEED(time_step)}, name))) return rc;
+            if (op == OP_LEVEE_BIF_OUT) {
+                if ((rc = require({NEED(protected_water_surface_elevation)}, name))) return rc;
+                LAUNCH((k_bif_outflow<REAL, STO, true>), v.P * v.T, stream, v, -1);
+            } else {
+                LAUNCH((k_bif_outflow<REAL, STO, false>), v.P * v.T, stream, v, -1);
+            }
+            LAUNCH((k_bif_outgoing<REAL, STO>), NT, stream, v);
+            return CMF_OK;
+        }
+        case OP_BIF_IN:
+            if (!(f->flags & CMF_HAS_BIFURCATION))
+                return set_error(CMF_ERR_INVALID, "compute_bifurcation_inflow: bifurcation is off");
+            if ((rc = need_features(f, "compute_bifurcation_inflow"))) return rc;
+            LAUNCH((k_bif_inflow<REAL, STO>), v.P * v.T, stream, v, -1);
+            LAUNCH((k_gbif_sum<REAL, STO>), NT, stream, v);
+            return CMF_OK;
+        case OP_LEVEE:
+        case OP_LEVEE_LOG: {
+            const bool log = (op == OP_LEVEE_LOG);
+            const char* name = log ? "compute_levee_stage_log" : "compute_levee_stage";
+            if (!(f->flags & CMF_HAS_LEVEE)) return set_error(CMF_ERR_INVALID, "%s: levee is off", name);
+            if ((rc = need_base(f, name)) || (rc = need_features(f, name))) return rc;
+            if (log) {
+                cmf_fields g = *f;
+                g.flags |= CMF_HAS_LOG;
+                if ((rc = need_features(&g, name))) return rc;
+                LAUNCH((k_levee<REAL, STO, true>), v.V * v.T, stream, v, -1);
+                if (v.V * v.T > 0) return log_final(v, blocks_for(v.V * v.T), kLogStage, stream);
+                return CMF_OK;
+            }
+            LAUNCH((k_levee<REAL, STO, false>), v.V * v.T, stream, v, -1);
+            return CMF_OK;
+        }
+        case OP_RESERVOIR:
+            if (!(f->flags & CMF_HAS_RESERVOIR))
+                return set_error(CMF_ERR_INVALID, "compute_reservoir_outflow: reservoir is off");
+            if ((rc = need_base(f, "compute_reservoir_outflow")) ||
+                (rc = need_features(f, "compute_reservoir_outflow")))
+                return rc;
+            if (!f->outgoing_storage) return set_error(CMF_ERR_INVALID, "compute_reservoir_outflow: outgoing_storage is NULL");
+            LAUNCH((k_reservoir<REAL, STO, false>), v.R * v.T, stream, v, -1);
+            if (contig) LAUNCH((k_res_regather<REAL, STO, true>), v.R * v.T, stream, v);
+            else LAUNCH((k_res_regather<REAL, STO, false>), v.R * v.T, stream, v);
+            return CMF_OK;
+    }
+    return set_error(CMF_ERR_INVALID, "unknown op");
+}
+
+int dispatch_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
+    if (!f) return set_error(CMF_ERR_INVALID, "null field table");
+    if (f->precision == CMF_PRECISION_F32_F64) return run_op<float, double>(net, f, op, stream);
+    if (f->precision == CMF_PRECISION_F64_F64) return run_op<double, double>(net, f, op, stream);
+    return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
+}
+
+// ------------------------------------------------------------- fused ----
+template <class REAL, class STO>
+int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int substep,
+                  int store_inflow, void* stream) {
+    const int64_t NT = v.N * v.T;
+    const bool contig = net->contiguous != 0;
+    const bool bif = v.has(CMF_HAS_BIFURCATION), levee = v.has(CMF_HAS_LEVEE);
+    const bool log = v.has(CMF_HAS_LOG);
+    LAUNCH((k_outflow<REAL, STO, true>), NT, stream, v, substep);
+    if (v.has(CMF_HAS_RESERVOIR)) LAUNCH((k_reservoir<REAL, STO, true>), v.R * v.T, stream, v, substep);
+    if (bif) {
+        if (levee) LAUNCH((k_bif_outflow<REAL, STO, true>), v.P * v.T, stream, v, substep);
+        else LAUNCH((k_bif_outflow<REAL, STO, false>), v.P * v.T, stream, v, substep);
+    }
+    if (contig) LAUNCH((k_limiter<REAL, STO, true>), NT, stream, v, substep);
+    else LAUNCH((k_limiter<REAL, STO, false>), NT, stream, v, substep);
+    if (bif) LAUNCH((k_bif_inflow<REAL, STO>), v.P * v.T, stream, v, substep);
+    if (log) {
+        if (contig) LAUNCH((k_stage<REAL, STO, true, true, true>), NT, stream, v, substep, store_inflow);
+        else LAUNCH((k_stage<REAL, STO, true, false, true>), NT, stream, v, substep, store_inflow);
+        int rc = log_final(v, blocks_for(NT), kLogAll, stream);
+        if (rc) return rc;
+    } else {
+        if (contig) LAUNCH((k_stage<REAL, STO, true, true, false>), NT, stream, v, substep, store_inflow);
+        else LAUNCH((k_stage<REAL, STO, true, false, false>), NT, stream, v, substep, store_inflow);
+    }
+    if (levee) {
+        if (log) {
+            LAUNCH((k_levee<REAL, STO, true>), v.V * v.T, stream, v, substep);
+            if (v.V * v.T > 0) {
+                int rc = log_final(v, blocks_for(v.V * v.T), kLogStage, stream);
+                if (rc) return rc;
+            }
+        } else {
+            LAUNCH((k_levee<REAL, STO, false>), v.V * v.T, stream, v, substep);
+        }
+    }
+    (void)f;
+    return CMF_OK;
+}
+
+template <class REAL, class STO>
+int run_substep(cmf_network* net, const cmf_fields* f, int substep, int store_inflow, void* stream) {
+    View<REAL, STO> v;
+    int rc = make_view<REAL, STO>(net, f, v);
+    if (rc) return rc;
+    if ((rc = need_base(f, "substep")) || (rc = need_features(f, "substep"))) return rc;
+    if (v.has(CMF_HAS_LOG) && blocks_for(v.N * v.T) > net->partial_blocks)
+        return set_error(CMF_ERR_INTERNAL, "substep: partial buffer too small");
+    return fused_substep(net, v, f, substep, store_inflow, stream);
+}
+
+template <class REAL>
+__global__ void k_prepare(REAL* outer_time_step, double outer) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) outer_time_step[0] = (REAL)outer;
+}
+template <class REAL>
+__global__ void k_set_dt_fixed(REAL* time_step, int64_t n, double value) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) time_step[t] = (REAL)value;
+}
+__global__ void k_fill_int(int32_t* p, int64_t n, int32_t value) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = value;
+}
+
+template <class REAL, class STO>
+int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_count,
+                int32_t* out_count, void* stream) {
+    View<REAL, STO> v;
+    int rc = make_view<REAL, STO>(net, f, v);
+    if (rc) return rc;
+    if ((rc = need_base(f, "advance")) || (rc = need_features(f, "advance"))) return rc;
+    if (!(outer > 0.0)) return set_error(CMF_ERR_INVALID, "advance: outer step must be > 0");
+    hipStream_t s = (hipStream_t)stream;
+    const bool per_trial = v.has(CMF_PER_TRIAL_DT);
+    const int64_t nslots = per_trial ? v.T : 1;
+    int count = fixed_count;
+    if (v.has(CMF_HAS_LOG) && blocks_for(v.N * v.T) > net->partial_blocks)
+        return set_error(CMF_ERR_INTERNAL, "advance: partial buffer too small");
+    if (v.has(CMF_HAS_ADAPTIVE_TIME)) {
+        if ((rc = require({NEED(max_sub_steps), NEED(outer_time_step)}, "advance"))) return rc;
+        hipLaunchKernelGGL((k_prepare<REAL>), dim3(1), dim3(64), 0, s, static_cast<REAL*>(f->outer_time_step), outer);
+        hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, f->max_sub_steps, nslots);
+        if (v.N > 0) {
+            const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 2048);
+            hipLaunchKernelGGL((k_cfl<REAL, STO>), dim3((unsigned)chunks, (unsigned)v.T), dim3(kBlock), 0, s, v);
+        }
+        hipLaunchKernelGGL((k_set_dt<REAL>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
+                           static_cast<REAL*>(f->time_step), f->max_sub_steps, nslots, outer);
+        CMF_HIP_CHECK(hipGetLastError());
+        CMF_HIP_CHECK(hipMemcpyAsync(net->host_counts, f->max_sub_steps, sizeof(int32_t) * nslots,
+                                     hipMemcpyDeviceToHost, s));
+        CMF_HIP_CHECK(hipStreamSynchronize(s));
+        count = 1;
+        for (int64_t t = 0; t < nslots; ++t) count = std::max(count, (int)net->host_counts[t]);
+    } else {
+        if (fixed_count < 1) return set_error(CMF_ERR_INVALID, "advance: fixed_count must be >= 1");
+        if (per_trial) {
+            if (!f->max_sub_steps) return set_error(CMF_ERR_INVALID, "advance: max_sub_steps is NULL");
+            hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
+                               f->max_sub_steps, nslots, (int32_t)fixed_count);
+        }
+        if (f->outer_time_step)
+            hipLaunchKernelGGL((k_prepare<REAL>), dim3(1), dim3(64), 0, s, static_cast<REAL*>(f->outer_time_step), outer);
+        hipLaunchKernelGGL((k_set_dt_fixed<REAL>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
+                           static_cast<REAL*>(f->time_step), nslots, outer / (double)fixed_count);
+        CMF_HIP_CHECK(hipGetLastError());
+    }
+    for (int k = 0; k < count; ++k) {
+        if (v.has(CMF_HAS_LOG)) {
+            hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, f->current_step, (int64_t)1, (int32_t)k);
+        }
+        // with per-trial counts each trial stores its inflow on its own last
+        // sub-step (k_stage checks); a shared count stores on the last one
+        rc = fused_substep(net, v, f, k, (k == count - 1) ? 1 : 0, stream);
+        if (rc) return rc;
+    }
+    if (f->current_step)
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, f->current_step, (int64_t)1, (int32_t)(count - 1));
+    CMF_HIP_CHECK(hipGetLastError());
+    if (out_count) *out_count = count;
+    return CMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cmf_last_error(void) { return g_last_error.c_str(); }
+int cmf_version(void) { return 10000; }
+int64_t cmf_fields_size(void) { return (int64_t)sizeof(cmf_fields); }
+
+int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_paths,
+                       const int32_t* bif_ci, const int32_t* bif_di, int64_t num_trials,
+                       cmf_network** out) {
+    if (!out) return set_error(CMF_ERR_INVALID, "cmf_network_create: out is NULL");
+    *out = nullptr;
+    if (n < 1 || n > INT32_MAX - 1 || !downstream_idx)
+        return set_error(CMF_ERR_INVALID, "cmf_network_create: bad catchment count or downstream_idx");
+    if (num_trials < 1) return set_error(CMF_ERR_INVALID, "cmf_network_create: num_trials must be >= 1");
+    if (num_paths < 0 || (num_paths > 0 && (!bif_ci || !bif_di)))
+        return set_error(CMF_ERR_INVALID, "cmf_network_create: bad bifurcation arguments");
+    int rc = cmf_check_flow_direction(n, downstream_idx);
+    if (rc) return rc;
+    for (int64_t p = 0; p < num_paths; ++p)
+        if (bif_ci[p] < 0 || bif_ci[p] >= n || bif_di[p] < 0 || bif_di[p] >= n)
+            return set_error(CMF_ERR_INVALID, "cmf_network_create: bifurcation index out of range at path %lld",
+                             (long long)p);
+    // upstream lists in ascending network index
+    std::vector<int32_t> up_ptr((size_t)n + 1, 0), up_idx;
+    for (int64_t i = 0; i < n; ++i) if (downstream_idx[i] != i) up_ptr[downstream_idx[i] + 1]++;
+    for (int64_t i = 0; i < n; ++i) up_ptr[i + 1] += up_ptr[i];
+    up_idx.resize((size_t)up_ptr[n]);
+    {
+        std::vector<int32_t> fill(up_ptr.begin(), up_ptr.end() - 1);
+        for (int64_t i = 0; i < n; ++i)
+            if (downstream_idx[i] != i) up_idx[fill[downstream_idx[i]]++] = (int32_t)i;
+    }
+    bool contiguous = true;
+    for (size_t k = 0; k < up_idx.size(); ++k)
+        if (up_idx[k] != (int32_t)k) { contiguous = false; break; }
+    // bifurcation events per catchment, ascending path, upstream end first
+    std::vector<int32_t> ev_ptr((size_t)n + 1, 0), ev_code((size_t)(2 * num_paths));
+    for (int64_t p = 0; p < num_paths; ++p) { ev_ptr[bif_ci[p] + 1]++; ev_ptr[bif_di[p] + 1]++; }
+    for (int64_t i = 0; i < n; ++i) ev_ptr[i + 1] += ev_ptr[i];
+    {
+        std::vector<int32_t> fill(ev_ptr.begin(), ev_ptr.end() - 1);
+        for (int64_t p = 0; p < num_paths; ++p) {
+            ev_code[fill[bif_ci[p]]++] = (int32_t)(2 * p);
+            ev_code[fill[bif_di[p]]++] = (int32_t)(2 * p + 1);
+        }
+    }
+    cmf_network* net = new cmf_network();
+    net->n = n;
+    net->T = num_trials;
+    net->P = num_paths;
+    net->contiguous = contiguous ? 1 : 0;
+    (void)hipGetDevice(&net->device);
+    const size_t el = (size_t)n * (size_t)num_trials * 8;
+    const size_t pel = (size_t)std::max<int64_t>(num_paths, 1) * (size_t)num_trials * 8;
+    net->partial_blocks = blocks_for(n * num_trials) + 1;
+    if ((rc = upload((void**)&net->up_ptr, up_ptr.data(), up_ptr.size() * 4)) ||
+        (rc = upload((void**)&net->up_idx, up_idx.data(), up_idx.size() * 4)) ||
+        (rc = upload((void**)&net->ev_ptr, ev_ptr.data(), ev_ptr.size() * 4)) ||
+        (rc = upload((void**)&net->ev_code, ev_code.data(), ev_code.size() * 4)) ||
+        (rc = dev_alloc(&net->pos_term, el)) || (rc = dev_alloc(&net->neg_term, el)) ||
+        (rc = dev_alloc(&net->qraw_r, el)) || (rc = dev_alloc(&net->qraw_f, el)) ||
+        (rc = dev_alloc(&net->lim, el)) || (rc = dev_alloc(&net->path_pos, pel)) ||
+        (rc = dev_alloc(&net->path_neg, pel)) || (rc = dev_alloc(&net->path_net, pel)) ||
+        (rc = dev_alloc((void**)&net->partials, (size_t)net->partial_blocks * CMF_LOG_COUNT * 8))) {
+        free_network(net);
+        return rc;
+    }
+    hipError_t e = hipHostMalloc((void**)&net->host_counts, sizeof(int32_t) * (size_t)num_trials, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        free_network(net);
+        return set_error(CMF_ERR_NOMEM, "hipHostMalloc failed: %s", hipGetErrorString(e));
+    }
+    *out = net;
+    return CMF_OK;
+}
+
+int cmf_network_destroy(cmf_network* net) {
+    free_network(net);
+    return CMF_OK;
+}
+
+int cmf_network_is_contiguous(const cmf_network* net) { return net ? net->contiguous : 0; }
+
+int cmf_hip_outflow(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_OUTFLOW, s); }
+int cmf_hip_inflow(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_INFLOW, s); }
+int cmf_hip_flood_stage(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_STAGE, s); }
+int cmf_hip_flood_stage_log(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_STAGE_LOG, s); }
+int cmf_hip_adaptive_time_step(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_CFL, s); }
+int cmf_hip_bifurcation_outflow(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_BIF_OUT, s); }
+int cmf_hip_bifurcation_inflow(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_BIF_IN, s); }
+int cmf_hip_levee_stage(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_LEVEE, s); }
+int cmf_hip_levee_stage_log(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_LEVEE_LOG, s); }
+int cmf_hip_levee_bifurcation_outflow(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_LEVEE_BIF_OUT, s); }
+int cmf_hip_reservoir_outflow(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_RESERVOIR, s); }
+
+int cmf_hip_substep(cmf_network* net, const cmf_fields* f, int32_t substep, int32_t store_inflow, void* s) {
+    if (!f) return set_error(CMF_ERR_INVALID, "null field table");
+    if (f->precision == CMF_PRECISION_F32_F64) return run_substep<float, double>(net, f, substep, store_inflow, s);
+    if (f->precision == CMF_PRECISION_F64_F64) return run_substep<double, double>(net, f, substep, store_inflow, s);
+    return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
+}
+
+int cmf_hip_advance(cmf_network* net, const cmf_fields* f, double outer_seconds, int32_t fixed_count,
+                    int32_t* out_count, void* s) {
+    if (!f) return set_error(CMF_ERR_INVALID, "null field table");
+    if (f->precision == CMF_PRECISION_F32_F64)
+        return run_advance<float, double>(net, f, outer_seconds, fixed_count, out_count, s);
+    if (f->precision == CMF_PRECISION_F64_F64)
+        return run_advance<double, double>(net, f, outer_seconds, fixed_count, out_count, s);
+    return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
+}
+
+}  // extern "C"

@@ -1,0 +1,179 @@
+// Device-side building blocks: IEEE min/max, cube root / exp / log built from
+// basic operations, the element view of a cmf_fields table, and the wave64
+// reductions.  Every floating-point function here uses only correctly rounded
+// IEEE operations (with contraction off), so a value computed on the GPU is
+// the value any IEEE host computes with the same sequence.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "cmf_internal.hpp"
+
+namespace cmf {
+
+// ------------------------------------------------------------ min / max --
+// IEEE minNum / maxNum: a NaN operand loses (tl.minimum / tl.maximum).
+template <class T> __device__ __forceinline__ T vmin(T a, T b) { return (a < b || b != b) ? a : b; }
+template <class T> __device__ __forceinline__ T vmax(T a, T b) { return (a > b || b != b) ? a : b; }
+template <class T> __device__ __forceinline__ T vclamp(T x, T lo, T hi) { return vmin(vmax(x, lo), hi); }
+
+// ------------------------------------------------------------ cube root --
+// Exponent-thirds guess on the double bit pattern (relative error < 7%),
+// then Halley steps y <- y (y^3 + 2x) / (2y^3 + x): two steps leave a
+// relative error near 1e-11 before rounding to float, four reach double
+// precision.
+__device__ __forceinline__ double cbrt_pos_d(double x, int steps) {
+    double y = __longlong_as_double(
+        (long long)((unsigned long long)__double_as_longlong(x) / 3ull + 0x2AA0000000000000ull));
+    for (int k = 0; k < steps; ++k) {
+        const double t = y * y * y;
+        y = y * ((t + x + x) / (t + t + x));
+    }
+    return y;
+}
+__device__ __forceinline__ double cbrt_any_d(double x, int steps) {
+    if (x != x || x == 0.0 || __builtin_isinf(x)) return x;
+    if (x < 0.0) return -cbrt_pos_d(-x, steps);
+    if (x < 1e-290) return cbrt_pos_d(x * 0x1p300, steps) * 0x1p-100;
+    return cbrt_pos_d(x, steps);
+}
+__device__ __forceinline__ float dcbrt(float x) { return (float)cbrt_any_d((double)x, 2); }
+__device__ __forceinline__ double dcbrt(double x) { return cbrt_any_d(x, 4); }
+
+// ------------------------------------------------------- log / exp -------
+// log: x = m 2^e, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1),
+// odd series to s^27.  exp: y = k ln2 + r, Taylor to r^22, times 2^k.
+__device__ __forceinline__ double log_core_d(double x) {
+    if (x != x || x < 0.0) return __longlong_as_double(0x7ff8000000000000ll);
+    if (x == 0.0) return -__builtin_inf();
+    if (__builtin_isinf(x)) return x;
+    int escale = 0;
+    if (x < 1e-300) { x *= 0x1p200; escale = -200; }
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    int e = (int)((b >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((long long)((b & 0x000fffffffffffffull) | 0x3ff0000000000000ull));
+    if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double p = 1.0 / 27.0;
+    for (int k = 25; k >= 1; k -= 2) p = p * s2 + 1.0 / (double)k;
+    const double lnm = 2.0 * s * p;
+    const double LN2 = 0.6931471805599453;
+    return (double)(e + escale) * LN2 + lnm;
+}
+__device__ __forceinline__ double exp_core_d(double y) {
+    if (y != y) return y;
+    if (y > 709.0) return __builtin_inf();
+    if (y < -745.0) return 0.0;
+    const double LN2 = 0.6931471805599453;
+    const double kf = floor(y / LN2 + 0.5);
+    const double r = y - kf * LN2;
+    double p = 1.0;
+    for (int k = 22; k >= 1; --k) p = 1.0 + p * r / (double)k;
+    const int k = (int)kf;
+    const int k1 = k / 2, k2 = k - k1;
+    const double s1 = __longlong_as_double((long long)((unsigned long long)(k1 + 1023) << 52));
+    const double s2 = __longlong_as_double((long long)((unsigned long long)(k2 + 1023) << 52));
+    return p * s1 * s2;
+}
+__device__ __forceinline__ float dlog(float x) { return (float)log_core_d((double)x); }
+__device__ __forceinline__ double dlog(double x) { return log_core_d(x); }
+__device__ __forceinline__ float dexp(float x) { return (float)exp_core_d((double)x); }
+__device__ __forceinline__ double dexp(double x) { return exp_core_d(x); }
+
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
+
+// ------------------------------------------------------- wave64 reduce ---
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum_double(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ------------------------------------------------------------ the view --
+// Typed device pointers of a cmf_fields table plus the network scratch.
+template <class REAL, class STO>
+struct View {
+    int64_t N, T, L, P, LB, V, R, logsize;
+    int64_t ts, cs, pts, pps;     // element strides: trial / catchment, path trial / path
+    uint32_t flags, batched;
+    REAL gravity, adaptive_factor, min_kin;
+    const REAL* time_step;
+    const REAL* outer_time_step;
+    const int32_t* current_step;
+    int32_t* max_sub_steps;
+    const int32_t* dn;
+    STO *river_inflow, *flood_inflow;
+    REAL *river_outflow, *flood_outflow;
+    const REAL *river_manning, *flood_manning;
+    REAL *river_depth, *flood_depth, *protected_depth;
+    const REAL *river_width, *river_length, *river_height;
+    const REAL *elev, *catchment_area, *dist;
+    STO *river_storage, *flood_storage, *protected_storage;
+    REAL *rcsd, *fcsd, *fcsa, *flood_fraction;
+    const REAL *runoff, *table;
+    STO *outgoing, *total_storage;
+    REAL *wse, *pwse;
+    const uint8_t* is_levee;
+    const int32_t *bif_ci, *bif_di;
+    const REAL *bif_manning, *bif_width, *bif_length, *bif_elev;
+    REAL *bif_outflow, *bif_csd;
+    STO* gbif;
+    REAL* limit_rate;
+    const int32_t* levee_idx;
+    const REAL *levee_base, *levee_crown, *levee_frac;
+    const int32_t* res_idx;
+    const REAL *res_cons, *res_emer, *res_adj, *res_qn, *res_qa, *res_qf;
+    STO* res_inflow;
+    const uint8_t *is_reservoir, *is_dam_up, *is_dam_related;
+    double* log_sums[CMF_LOG_COUNT];
+    // network
+    const int32_t *up_ptr, *up_idx;     // up_idx == nullptr: contiguous ranges
+    const int32_t *ev_ptr, *ev_code;    // bifurcation events per catchment
+    // scratch
+    REAL *pos_term, *neg_term, *qraw_r, *qraw_f, *lim;
+    REAL *path_pos, *path_neg, *path_net;
+    double* partials;
+
+    __device__ __forceinline__ int64_t el(int64_t t, int64_t i) const { return t * ts + i * cs; }
+    __device__ __forceinline__ int64_t pel(int64_t t, int64_t p) const { return t * pts + p * pps; }
+    __host__ __device__ __forceinline__ bool has(uint32_t bit) const { return (flags & bit) != 0; }
+    __host__ __device__ __forceinline__ bool bat(uint32_t bit) const { return (batched & bit) != 0; }
+    // parameter read: per-trial element when batched, shared otherwise
+    template <class X>
+    __device__ __forceinline__ X prm(const X* p, uint32_t bit, int64_t e, int64_t i) const {
+        return bat(bit) ? p[e] : p[i];
+    }
+    __device__ __forceinline__ REAL dt(int64_t t) const {
+        return has(CMF_PER_TRIAL_DT) ? time_step[t] : time_step[0];
+    }
+    // With per-trial sub-step counts a trial past its count sits out.
+    __device__ __forceinline__ bool active(int64_t t, int substep) const {
+        if (substep < 0 || !has(CMF_PER_TRIAL_DT)) return true;
+        return substep < max_sub_steps[t];
+    }
+    // linear thread index -> (trial, item); `inner` when trials are the
+    // fastest-varying dimension of the layout
+    __device__ __forceinline__ void split(int64_t g, int64_t n, bool inner, int64_t& t, int64_t& i) const {
+        if (T == 1) { t = 0; i = g; }
+        else if (inner) { i = g / T; t = g - i * T; }
+        else { t = g / n; i = g - t * n; }
+    }
+    __device__ __forceinline__ void split_c(int64_t g, int64_t& t, int64_t& i) const {
+        split(g, N, ts == 1, t, i);
+    }
+    __device__ __forceinline__ void split_p(int64_t g, int64_t& t, int64_t& p) const {
+        split(g, P, pts == 1, t, p);
+    }
+};
+
+}  // namespace cmf

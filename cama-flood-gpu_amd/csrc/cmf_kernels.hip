@@ -1,0 +1,916 @@
+// gfx950 kernels of the CaMa-Flood per-sub-step hydrodynamic path.
+//
+// Each kernel is one thread per element (catchment, path, levee or
+// reservoir) of one trial; the arithmetic of every element follows the
+// reference Triton kernel it replaces, operation for operation:
+//   k_outflow            cmfgpu/phys/triton/outflow.py:15-315 (batched :386-717)
+//   k_inflow_limit       cmfgpu/phys/triton/outflow.py:318-371
+//   k_reservoir          cmfgpu/phys/triton/reservoir.py:17-184
+//   k_bif_outflow        cmfgpu/phys/triton/bifurcation.py:13-104, levee.py:565-684
+//   k_bif_inflow         cmfgpu/phys/triton/bifurcation.py:106-151
+//   k_stage              cmfgpu/phys/triton/storage.py:16-193 (log :196-425)
+//   k_levee              cmfgpu/phys/triton/levee.py:18-304 (log :307-562)
+//   k_cfl                cmfgpu/phys/triton/adaptive_time.py:13-103
+// The reference's float atomic scatters (outflow.py:309-314,376-383,
+// bifurcation.py:103-104,150-151, reservoir.py:68-77,175-184) become
+// gathers: every target sums its contributions itself, in a fixed order, from
+// contiguous upstream ranges (k_outgoing_sum, k_inflow_sum, k_limiter,
+// k_stage) and per-catchment bifurcation event lists (k_bif_outgoing,
+// k_gbif_sum).  The CFL maximum is a wave64 shuffle reduction; water-balance
+// sums are FP64 per-block partials folded in a fixed order (k_log_final).
+
+#include "cmf_device.hpp"
+
+namespace cmf {
+
+constexpr int kBlock = 256;
+
+// =========================================================== outflow ====
+// FUSED: the unlimited flows go to the qraw scratch (the limiter and the
+// inflow pass read them); inflows are not zeroed because the fused stage
+// pass writes them whole.  Otherwise flows are written in place, as in the
+// reference, and river/flood inflow and the bifurcation total are zeroed.
+template <class REAL, class STO, bool FUSED>
+__global__ __launch_bounds__(kBlock) void k_outflow(View<REAL, STO> v, int substep) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= v.N * v.T) return;
+    int64_t t, i;
+    v.split_c(g, t, i);
+    if (!v.active(t, substep)) return;
+    const int64_t e = v.el(t, i);
+    const int64_t d = v.dn[i];
+    const int64_t ed = v.el(t, d);
+    const bool mouth = (d == i);
+    const REAL time_step = v.dt(t);
+    const REAL gravity = v.gravity;
+
+    const REAL r_out = v.river_outflow[e];
+    const REAL r_man = v.prm(v.river_manning, CMF_BATCHED_RIVER_MANNING, e, i);
+    const REAL r_dep = v.river_depth[e];
+    const REAL r_wid = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, i);
+    const REAL r_len = v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, i);
+    const REAL r_hgt = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, i);
+    const REAL f_out = v.flood_outflow[e];
+    const REAL f_man = v.prm(v.flood_manning, CMF_BATCHED_FLOOD_MANNING, e, i);
+    const REAL f_dep = v.flood_depth[e];
+    const REAL c_elv = v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, e, i);
+    const REAL dn_dist = v.prm(v.dist, CMF_BATCHED_DOWNSTREAM_DISTANCE, e, i);
+    const REAL rs = (REAL)v.river_storage[e];
+    const REAL fs = (REAL)v.flood_storage[e];
+    const REAL ps = (REAL)v.protected_storage[e];
+    const REAL r_csd = v.rcsd[e], f_csd = v.fcsd[e], f_csa = v.fcsa[e];
+
+    const REAL river_elevation = c_elv - r_hgt;
+    const REAL wse = r_dep + river_elevation;
+    REAL pwse = wse;
+    if (v.has(CMF_HAS_LEVEE) && v.is_levee[i]) pwse = vmin(c_elv + v.protected_depth[e], wse);
+    const REAL total_storage = rs + fs + ps;
+    const REAL r_dep_dn = v.river_depth[ed];
+    const REAL r_hgt_dn = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, ed, d);
+    const REAL c_elv_dn = v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, ed, d);
+    const REAL river_elevation_dn = c_elv_dn - r_hgt_dn;
+    REAL wse_dn = r_dep_dn + river_elevation_dn;
+    if (mouth) wse_dn = c_elv;
+    const REAL max_wse = vmax(wse, wse_dn);
+
+    const REAL river_slope = (wse - wse_dn) / dn_dist;
+    const REAL flood_slope = vclamp(river_slope, (REAL)-0.005, (REAL)0.005);
+
+    const REAL upd_rcsd = mouth ? r_dep : (max_wse - river_elevation);
+    const REAL r_sifd = vmax(dsqrt(upd_rcsd * r_csd), (REAL)1e-6);
+    const REAL flood_surface = mouth ? wse : max_wse;
+    const REAL upd_fcsd = vmax(flood_surface - c_elv, (REAL)0.0);
+    const REAL f_sifd = vmax(dsqrt(upd_fcsd * f_csd), (REAL)1e-6);
+    const REAL upd_fcsa = vmax(fs / r_len - f_dep * r_wid, (REAL)0.0);
+
+    const REAL r_cs_area = upd_rcsd * r_wid;
+    const bool river_cond = (r_sifd > (REAL)1e-5) && (r_cs_area > (REAL)1e-5);
+    REAL q_r = (REAL)0.0;
+    if (river_cond) {
+        const REAL unit_r = r_out / r_wid;
+        const REAL num_r = r_wid * (unit_r + gravity * time_step * r_sifd * river_slope);
+        const REAL den_r = (REAL)1.0 + gravity * time_step * (r_man * r_man) * dabs(unit_r)
+                         * ((REAL)1.0 / (r_sifd * r_sifd * dcbrt(r_sifd)));
+        q_r = num_r / den_r;
+    }
+    const bool flood_cond = (f_sifd > (REAL)1e-5) && (upd_fcsa > (REAL)1e-5);
+    REAL q_f = (REAL)0.0;
+    if (flood_cond) {
+        const REAL f_imp_area = vmax(dsqrt(upd_fcsa * vmax(f_csa, (REAL)1e-6)), (REAL)1e-6);
+        const REAL num_f = f_out + gravity * time_step * f_imp_area * flood_slope;
+        const REAL den_f = (REAL)1.0 + gravity * time_step * (f_man * f_man) * dabs(f_out)
+                         * ((REAL)1.0 / (f_sifd * dcbrt(f_sifd))) / f_imp_area;
+        q_f = num_f / den_f;
+    }
+    if ((q_r * q_f) < (REAL)0.0) q_f = (REAL)0.0;
+    const bool negative = (q_r < (REAL)0.0) && !mouth;
+    if (negative) {
+        const REAL total_neg = (-q_r - q_f) * time_step;
+        const REAL limit = vmin((REAL)0.05 * total_storage / total_neg, (REAL)1.0);
+        q_r = q_r * limit;
+        q_f = q_f * limit;
+    }
+    if (v.has(CMF_HAS_RESERVOIR) && v.is_dam_up[i]) {
+        REAL bed_slope = (c_elv - c_elv_dn) / dn_dist;
+        bed_slope = vmax(bed_slope, v.min_kin);
+        const REAL kin_riv_vel = ((REAL)1.0 / r_man) * dsqrt(bed_slope) * dcbrt(r_dep * r_dep);
+        REAL kin_riv = r_wid * r_dep * kin_riv_vel;
+        kin_riv = vmin(kin_riv, rs / time_step);
+        kin_riv = vmax(kin_riv, (REAL)0.0);
+        const REAL bed_slope_f = vmin(bed_slope, (REAL)0.005);
+        const REAL kin_fld_vel = ((REAL)1.0 / f_man) * dsqrt(bed_slope_f) * dcbrt(f_dep * f_dep);
+        const REAL kin_fld_area = vmax(fs / r_len - f_dep * r_wid, (REAL)0.0);
+        REAL kin_fld = kin_fld_area * kin_fld_vel;
+        kin_fld = vmin(kin_fld, fs / time_step);
+        kin_fld = vmax(kin_fld, (REAL)0.0);
+        q_r = kin_riv;
+        q_f = kin_fld;
+    }
+
+    if (FUSED) {
+        v.qraw_r[e] = q_r;
+        v.qraw_f[e] = q_f;
+    } else {
+        v.river_outflow[e] = q_r;
+        v.flood_outflow[e] = q_f;
+        v.river_inflow[e] = (STO)0;
+        v.flood_inflow[e] = (STO)0;
+        if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = (STO)0;
+    }
+    if (v.has(CMF_HAS_WATER_SURFACE)) v.wse[e] = wse;
+    if (v.has(CMF_HAS_PROTECTED_WATER_SURFACE)) v.pwse[e] = pwse;
+    v.rcsd[e] = upd_rcsd;
+    v.fcsd[e] = upd_fcsd;
+    v.fcsa[e] = upd_fcsa;
+    if (v.has(CMF_HAS_TOTAL_STORAGE)) v.total_storage[e] = (STO)total_storage;
+    const REAL pos = vmax(q_r, (REAL)0.0) + vmax(q_f, (REAL)0.0);
+    const REAL neg = vmin(q_r, (REAL)0.0) + vmin(q_f, (REAL)0.0);
+    v.pos_term[e] = pos * time_step;
+    v.neg_term[e] = mouth ? (REAL)0.0 : -neg * time_step;
+}
+
+// Outgoing total of catchment j from the catchment terms: upstream negative
+// terms in ascending network index, then j's own positive term.
+template <class REAL, class STO, bool CONTIG>
+__device__ __forceinline__ STO outgoing_terms(const View<REAL, STO>& v, int64_t t, int64_t j) {
+    STO acc = (STO)0;
+    const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
+    for (int32_t k = b; k < en; ++k) {
+        const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+        acc += (STO)v.neg_term[v.el(t, u)];
+    }
+    acc += (STO)v.pos_term[v.el(t, j)];
+    return acc;
+}
+
+// per-op: outgoing_storage += catchment terms (compute_outflow's scatter)
+template <class REAL, class STO, bool CONTIG>
+__global__ __launch_bounds__(kBlock) void k_outgoing_sum(View<REAL, STO> v) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= v.N * v.T) return;
+    int64_t t, j;
+    v.split_c(g, t, j);
+    const int64_t e = v.el(t, j);
+    v.outgoing[e] = v.outgoing[e] + outgoing_terms<REAL, STO, CONTIG>(v, t, j);
+}
+
+// ========================================================= reservoir ====
+// Replaces the reservoir catchment's outflow and its outgoing terms.  FUSED
+// writes the flow into the qraw scratch.
+template <class REAL, class STO, bool FUSED>
+__global__ __launch_bounds__(kBlock) void k_reservoir(View<REAL, STO> v, int substep) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= v.R * v.T) return;
+    const int64_t t = g / v.R, r = g - t * v.R;
+    if (!v.active(t, substep)) return;
+    const int64_t c = v.res_idx[r];
+    const int64_t d = v.dn[c];
+    const bool mouth = (d == c);
+    const int64_t e = v.el(t, c);
+    const REAL time_step = v.dt(t);
+    const REAL rs = (REAL)v.river_storage[e];
+    const REAL fs = (REAL)v.flood_storage[e];
+    const REAL total = rs + fs;
+    const REAL total_inflow = (REAL)v.res_inflow[e];
+    v.res_inflow[e] = (STO)0;
+    const REAL runoff = v.prm(v.runoff, CMF_BATCHED_RUNOFF, e, c);
+    const REAL inflow = total_inflow + runoff;
+    const REAL cv = v.res_cons[r], ev = v.res_emer[r], av = v.res_adj[r];
+    const REAL qn = v.res_qn[r], qa = v.res_qa[r], qf = v.res_qf[r];
+    REAL out = (REAL)0;
+    if (total <= cv) out = qn * dsqrt(total / cv);
+    if ((total > cv) && (total <= av)) {
+        const REAL frac2 = (total - cv) / (av - cv);
+        out = qn + dexp((REAL)3.0 * dlog(frac2)) * (qa - qn);
+    }
+    if ((total > av) && (total <= ev)) {
+        const bool flood_period = inflow >= qf;
+        const REAL frac3 = (total - av) / (ev - av);
+        const REAL tmp = qa + dexp((REAL)0.1 * dlog(frac3)) * (qf - qa);
+        if (flood_period) {
+            const REAL o_flood = qn + ((total - cv) / (ev - cv)) * (inflow - qn);
+            out = vmax(o_flood, tmp);
+        } else {
+            out = tmp;
+        }
+    }
+    if (total > ev) out = (inflow >= qf) ? inflow : qf;
+    out = vclamp(out, (REAL)0.0, total / time_step);
+
+    if (FUSED) {
+        v.qraw_r[e] = out;
+        v.qraw_f[e] = (REAL)0.0;
+    } else {
+        v.river_outflow[e] = out;
+        v.flood_outflow[e] = (REAL)0.0;
+    }
+    v.total_storage[e] = (STO)total;
+    const REAL new_pos = vmax(out, (REAL)0.0);
+    const REAL new_neg = vmin(out, (REAL)0.0);
+    v.pos_term[e] = new_pos * time_step;
+    v.neg_term[e] = mouth ? (REAL)0.0 : -(new_neg * time_step);
+}
+
+// per-op: re-sum the outgoing totals touched by a reservoir (its catchment
+// and its downstream catchment).  Two reservoirs may share a target; they
+// write the same value.
+template <class REAL, class STO, bool CONTIG>
+__global__ __launch_bounds__(kBlock) void k_res_regather(View<REAL, STO> v) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= v.R * v.T) return;
+    const int64_t t = g / v.R, r = g - t * v.R;
+    const int64_t c = v.res_idx[r];
+    const int64_t d = v.dn[c];
+    v.outgoing[v.el(t, c)] = outgoing_terms<REAL, STO, CONTIG>(v, t, c);
+    if (d != c) v.outgoing[v.el(t, d)] = outgoing_terms<REAL, STO, CONTIG>(v, t, d);
+}
+
+// ====================================================== bifurcation ======
+template <class REAL, class STO, bool LEVEE>
+__global__ __launch_bounds__(kBlock) void k_bif_outflow(View<REAL, STO> v, int substep) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= v.P * v.T) return;
+    int64_t t, p;
+    v.split_p(g, t, p);
+    if (!v.active(t, substep)) return;
+    const int64_t pe = v.pel(t, p);
+    const int64_t ci = v.bif_ci[p], di = v.bif_di[p];
+    const int64_t eci = v.el(t, ci), edi = v.el(t, di);
+    const REAL time_step = v.dt(t);
+    const REAL gravity = v.gravity;
+    const REAL length = v.prm(v.bif_length, CMF_BATCHED_BIFURCATION_LENGTH, pe, p);
+    const REAL w0 = v.wse[eci], w1 = v.wse[edi];
+    const REAL max_w = vmax(w0, w1);
+    REAL max_pw = (REAL)0;
+    if (LEVEE) max_pw = vmax(v.pwse[eci], v.pwse[edi]);
+    REAL slope = (w0 - w1) / length;
+    slope = vclamp(slope, (REAL)-0.005, (REAL)0.005);
+    const REAL ts0 = (REAL)v.total_storage[eci];
+    const REAL ts1 = (REAL)v.total_storage[edi];
+    const int64_t LB = v.LB;
+    REAL sum = (REAL)0;
+    for (int64_t l = 0; l < LB; ++l) {
+        const int64_t k = pe * LB + l;
+        const int64_t ks = p * LB + l;
+        const REAL csd_old = v.bif_csd[k];
+        const REAL bel = v.prm(v.bif_elev, CMF_BATCHED_BIFURCATION_ELEVATION, k, ks);
+        REAL upd_csd, sifd;
+        if (!LEVEE) {
+            upd_csd = vmax(max_w - bel, (REAL)0.0);
+            sifd = vmax(dsqrt(upd_csd * csd_old), dsqrt(upd_csd * (REAL)0.01));
+        } else if (l == 0) {
+            upd_csd = vmax(max_w - bel, (REAL)0.0);
+            const REAL semi = dsqrt(upd_csd * csd_old);
+            sifd = (semi <= (REAL)0.0) ? upd_csd : semi;
+        } else {
+            upd_csd = vmax(max_pw - bel, (REAL)0.0);
+            sifd = upd_csd;
+        }
+        const REAL w = v.prm(v.bif_width, CMF_BATCHED_BIFURCATION_WIDTH, k, ks);
+        const REAL man = v.prm(v.bif_manning, CMF_BATCHED_BIFURCATION_MANNING, k, ks);
+        REAL q = (REAL)0.0;
+        if (sifd > (REAL)1e-5) {
+            const REAL unit = v.bif_outflow[k] / w;
+            const REAL num = w * (unit + gravity * time_step * sifd * slope);
+            const REAL den = (REAL)1.0 + gravity * time_step * (man * man) * dabs(unit)
+                           * ((REAL)1.0 / (sifd * sifd * dcbrt(sifd)));
+            q = num / den;
+        }
+        sum += q;
+        v.bif_csd[k] = upd_csd;
+        v.bif_outflow[k] = q;
+    }
+    const REAL limit = vmin((REAL)0.05 * vmin(ts0, ts1) / (dabs(sum) * time_step), (REAL)1.0);
+    sum *= limit;
+    for (int64_t l = 0; l < LB; ++l) v.bif_outflow[pe * LB + l] *= limit;
+    const REAL pos = vmax(sum, (REAL)0.0);
+    const REAL neg = vmin(sum, (REAL)0.0);
+    v.path_pos[pe] = pos * time_step;
+    v.path_neg[pe] = -neg * time_step;
+}
+
+// Sum of the bifurcation events of catchment j in ascending path order.
+// code = 2 p + role; role 0: j is the path's upstream end, 1: downstream end.
+template <class REAL, class STO>
+__device__ __forceinline__ STO bif_event_sum(const View<REAL, STO>& v, int64_t t, int64_t j,
+                                             STO acc, bool outgoing_terms_mode) {
+    const int32_t b = v.ev_ptr[j], en = v.ev_ptr[j + 1];
+    for (int32_t k = b; k < en; ++k) {
+        const int32_t code = v.ev_code[k];
+        const int64_t pe = v.pel(t, code >> 1);
+        if (outgoing_terms_mode) {
+            acc += (STO)((code & 1) ? v.path_neg[pe] : v.path_pos[pe]);
+        } else {
+            const REAL net = v.path_net[pe];
+            acc += (code & 1) ? (STO)(-net) : (STO)net;
+        }
+    }
+    return acc;
+}
+
+// per-op: outgoing_storage += bifurcation terms
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_bif_outgoing(View<REAL, STO> v) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= v.N * v.T) return;
+    int64_t t, j;
+    v.split_c(g, t, j);
+    if (v.ev_ptr[j] == v.ev_ptr[j + 1]) return;
+    const int64_t e = v.el(t, j);
+    v.outgoing[e] = bif_event_sum<REAL, STO>(v, t, j, v.outgoing[e], true);
+}
+
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_bif_inflow(View<REAL, STO> v, int substep) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= v.P * v.T) return;
+    int64_t t, p;
+    v.split_p(g, t, p);
+    if (!v.active(t, substep)) return;
+    const int64_t pe = v.pel(t, p);
+    const int64_t ci = v.bif_ci[p], di = v.bif_di[p];
+    const REAL l0 = v.limit_rate[v.el(t, ci)], l1 = v.limit_rate[v.el(t, di)];
+    REAL raw_sum = (REAL)0;
+    for (int64_t l = 0; l < v.LB; ++l) {
+        const int64_t k = pe * v.LB + l;
+        const REAL raw = v.bif_outflow[k];
+        raw_sum += raw;
+        v.bif_outflow[k] = (raw >= (REAL)0.0) ? raw * l0 : raw * l1;
+    }
+    v.path_net[pe] = (raw_sum >= (REAL)0.0) ? raw_sum * l0 : raw_sum * l1;
+}
+
+// per-op: global_bifurcation_outflow += path nets
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_gbif_sum(View<REAL, STO> v) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= v.N * v.T) return;
+    int64_t t, j;
+    v.split_c(g, t, j);
+    if (v.ev_ptr[j] == v.ev_ptr[j + 1]) return;
+    const int64_t e = v.el(t, j);
+    v.gbif[e] = bif_event_sum<REAL, STO>(v, t, j, v.gbif[e], false);
+}
+
+// ============================================================ inflow ====
+template <class REAL, class STO>
+__device__ __forceinline__ REAL limit_of(const View<REAL, STO>& v, STO outgoing, int64_t e) {
+    const REAL og = (REAL)outgoing;
+    const REAL rate = (REAL)(v.river_storage[e] + v.flood_storage[e]);
+    return (og > (REAL)1e-8) ? vmin(rate / og, (REAL)1.0) : (REAL)1.0;
+}
+
+// per-op: apply the storage limiter in place
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_inflow_limit(View<REAL, STO> v) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= v.N * v.T) return;
+    int64_t t, i;
+    v.split_c(g, t, i);
+    const int64_t e = v.el(t, i);
+    const int64_t ed = v.el(t, v.dn[i]);
+    const REAL l_own = limit_of(v, v.outgoing[e], e);
+    const REAL l_dn = limit_of(v, v.outgoing[ed], ed);
+    const REAL qr = v.river_outflow[e], qf = v.flood_outflow[e];
+    v.river_outflow[e] = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
+    v.flood_outflow[e] = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
+    if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e] = l_own;
+}
+
+// per-op: river/flood (and reservoir) inflow += limited upstream flows
+template <class REAL, class STO, bool CONTIG>
+__global__ __launch_bounds__(kBlock) void k_inflow_sum(View<REAL, STO> v) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= v.N * v.T) return;
+    int64_t t, j;
+    v.split_c(g, t, j);
+    const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
+    if (b == en) return;
+    const int64_t e = v.el(t, j);
+    STO ri = v.river_inflow[e], fi = v.flood_inflow[e];
+    const bool res = v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
+    STO rt = res ? v.res_inflow[e] : (STO)0;
+    for (int32_t k = b; k < en; ++k) {
+        const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+        const int64_t eu = v.el(t, u);
+        const REAL ur = v.river_outflow[eu], uf = v.flood_outflow[eu];
+        ri += (STO)ur;
+        fi += (STO)uf;
+        if (res) rt += (STO)(ur + uf);
+    }
+    v.river_inflow[e] = ri;
+    v.flood_inflow[e] = fi;
+    if (res) v.res_inflow[e] = rt;
+}
+
+// fused: storage limiter of every catchment from its outgoing total
+template <class REAL, class STO, bool CONTIG>
+__global__ __launch_bounds__(kBlock) void k_limiter(View<REAL, STO> v, int substep) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= v.N * v.T) return;
+    int64_t t, j;
+    v.split_c(g, t, j);
+    if (!v.active(t, substep)) return;
+    const int64_t e = v.el(t, j);
+    STO og = outgoing_terms<REAL, STO, CONTIG>(v, t, j);
+    if (v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[j] != v.ev_ptr[j + 1])
+        og = bif_event_sum<REAL, STO>(v, t, j, og, true);
+    const REAL l = limit_of(v, og, e);
+    v.lim[e] = l;
+    if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e] = l;
+}
+
+// ======================================================= flood stage ====
+struct LogPart {
+    double q[CMF_LOG_COUNT];
+};
+
+// Storage update and flood-stage diagnosis of one catchment.
+template <class REAL, class STO, bool LOG>
+__device__ __forceinline__ void stage_one(const View<REAL, STO>& v, int64_t e, int64_t i,
+                                          REAL rin, REAL fin, REAL rout, REAL fout, REAL gbif,
+                                          REAL time_step, bool non_levee, LogPart& lp) {
+    const STO rs = v.river_storage[e], fs = v.flood_storage[e], ps = v.protected_storage[e];
+    const REAL ro = v.prm(v.runoff, CMF_BATCHED_RUNOFF, e, i);
+    const REAL inflow_extra = (REAL)0.0;
+
+    STO rsu = rs + (STO)((rin - rout) * time_step);
+    STO fsu = fs + ((rsu < (STO)0.0) ? rsu : (STO)0.0) + (STO)((fin - fout - gbif) * time_step);
+    rsu = vmax(rsu, (STO)0.0);
+    rsu = (fsu < (STO)0.0) ? vmax(rsu + fsu, (STO)0.0) : rsu;
+    fsu = vmax(fsu, (STO)0.0);
+    const STO total_next = rsu + fsu + ps + (STO)(ro * time_step) + (STO)(inflow_extra * time_step);
+    const STO total_hp = vmax(total_next, (STO)0.0);
+    const REAL total = (REAL)total_hp;
+
+    const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, i);
+    const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, i);
+    const REAL rl = v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, i);
+    const REAL river_max_storage = rl * rw * rh;
+    const int L = (int)v.L;
+
+    REAL fdep, ffr, rdep;
+    STO rsf_hp, fsf_hp;
+    REAL ca = (REAL)0;
+    if (!(total > river_max_storage)) {
+        // dry: the table scan leaves level = -1 (storage.py:110,137)
+        rsf_hp = total_hp;
+        fdep = (REAL)0.0;
+        ffr = (REAL)0.0;
+        rdep = (REAL)rsf_hp / (rl * rw);
+        fsf_hp = vmax(total_hp - rsf_hp, (STO)0.0);
+        if (LOG) ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, i);
+    } else {
+        ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, i);
+        const REAL catchment_width = ca / rl;
+        const REAL width_increment = catchment_width / (REAL)L;
+        const REAL* row = v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + i * v.L;
+        int level = 0;
+        REAL S_accum = river_max_storage;
+        REAL prev_H = (REAL)0.0;
+        REAL prev_W = rw;
+        REAL prev_total_storage = river_max_storage;
+        REAL prev_flood_depth = (REAL)0.0;
+        REAL next_flood_depth = (REAL)0.0;
+        // S is non-decreasing, so once total <= S no later level is above:
+        // the scan stops there with the values the full scan would leave.
+        for (int k = 0; k < L; ++k) {
+            const REAL H = row[k];
+            const REAL W = rw + (REAL)(k + 1) * width_increment;
+            const REAL dS = rl * (REAL)0.5 * (prev_W + W) * (H - prev_H);
+            const REAL S = S_accum + dS;
+            next_flood_depth = H;
+            if (!(total > S)) break;
+            level += 1;
+            prev_total_storage = S;
+            prev_flood_depth = H;
+            S_accum = S;
+            prev_H = H;
+            prev_W = W;
+        }
+        const REAL prev_total_width = rw + (REAL)level * width_increment;
+        REAL diff_width;
+        if (level == L) {
+            fdep = prev_flood_depth + (total - prev_total_storage) / (prev_total_width * rl);
+            ffr = (REAL)1.0;
+        } else {
+            const REAL flood_grad = (next_flood_depth - prev_flood_depth) / width_increment;
+            diff_width = dsqrt(prev_total_width * prev_total_width
+                + (REAL)2.0 * (total - prev_total_storage) / (flood_grad * rl)) - prev_total_width;
+            fdep = prev_flood_depth + diff_width * flood_grad;
+            ffr = vclamp((prev_total_width + diff_width - rw) * rl / ca, (REAL)0.0, (REAL)1.0);
+        }
+        const REAL cap = river_max_storage + rl * rw * fdep;
+        rsf_hp = vmin((STO)cap, total_hp);
+        rdep = (REAL)rsf_hp / (rl * rw);
+        fsf_hp = vmax(total_hp - rsf_hp, (STO)0.0);
+    }
+
+    if (LOG) {
+        const STO total_pre = rs + fs + ps;
+        lp.q[CMF_LOG_STORAGE_PRE] = (double)total_pre;
+        lp.q[CMF_LOG_STORAGE_NEXT] = (double)total_next;
+        lp.q[CMF_LOG_STORAGE_NEW] = (double)total_hp;
+        lp.q[CMF_LOG_INFLOW] = (double)((rin + fin + inflow_extra) * time_step);
+        lp.q[CMF_LOG_OUTFLOW] = (double)((rout + fout) * time_step);
+        lp.q[CMF_LOG_INFLOW_ERROR] = (double)(total_pre - total_next
+            + (STO)((rin + fin + ro + inflow_extra - rout - fout - gbif) * time_step));
+        if (non_levee) {
+            const STO stage_new = rsf_hp + fsf_hp;
+            lp.q[CMF_LOG_STORAGE_STAGE] = (double)stage_new;
+            lp.q[CMF_LOG_RIVER_STORAGE] = (double)(REAL)rsf_hp;
+            lp.q[CMF_LOG_FLOOD_STORAGE] = (double)(REAL)fsf_hp;
+            lp.q[CMF_LOG_FLOOD_AREA] = (double)(ffr * ca);
+            lp.q[CMF_LOG_STAGE_ERROR] = (double)(stage_new - total_hp);
+        }
+    }
+    v.river_storage[e] = rsf_hp;
+    v.flood_storage[e] = fsf_hp;
+    v.protected_storage[e] = (STO)0.0;
+    v.river_depth[e] = rdep;
+    v.flood_depth[e] = fdep;
+    v.protected_depth[e] = fdep;
+    v.flood_fraction[e] = ffr;
+}
+
+// Block-level FP64 sums of the log partials in a fixed order.
+__device__ __forceinline__ void block_log_partials(LogPart& lp, double* out_block) {
+    __shared__ double sh[kBlock / 64][CMF_LOG_COUNT];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < CMF_LOG_COUNT; ++k) {
+        const double s = wave_sum_double(lp.q[k]);
+        if (lane == 0) sh[wave][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < CMF_LOG_COUNT) {
+        double acc = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) acc += sh[w][threadIdx.x];
+        out_block[threadIdx.x] = acc;
+    }
+}
+
+// FUSED: inflow from the limited upstream flows and the bifurcation nets is
+// summed here, the own limited flow is formed here, and `store_inflow`
+// writes the inflow totals back to their fields.  Otherwise the fields hold
+// what compute_outflow / compute_inflow / compute_bifurcation_inflow wrote.
+template <class REAL, class STO, bool FUSED, bool CONTIG, bool LOG>
+__global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v, int substep, int store_inflow) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    LogPart lp;
+    if (LOG) {
+#pragma unroll
+        for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
+    }
+    if (g < v.N * v.T) {
+        int64_t t, j;
+        v.split_c(g, t, j);
+        if (v.active(t, substep)) {
+            const int64_t e = v.el(t, j);
+            const REAL time_step = v.dt(t);
+            REAL rin, fin, rout, fout, gb = (REAL)0.0;
+            if (FUSED) {
+                const REAL l_own = v.lim[e];
+                const REAL l_dn = v.lim[v.el(t, v.dn[j])];
+                const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
+                rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
+                fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
+                v.river_outflow[e] = rout;
+                v.flood_outflow[e] = fout;
+                // upstream u has downstream j: negative flow is limited by j
+                const bool res = v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
+                STO ri = (STO)0, fi = (STO)0;
+                STO rt = res ? v.res_inflow[e] : (STO)0;
+                const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
+                for (int32_t k = b; k < en; ++k) {
+                    const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+                    const int64_t eu = v.el(t, u);
+                    const REAL lu = v.lim[eu];
+                    const REAL ur0 = v.qraw_r[eu], uf0 = v.qraw_f[eu];
+                    const REAL ur = (ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own;
+                    const REAL uf = (uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own;
+                    ri += (STO)ur;
+                    fi += (STO)uf;
+                    if (res) rt += (STO)(ur + uf);
+                }
+                if (res) v.res_inflow[e] = rt;
+                STO gsum = (STO)0;
+                if (v.has(CMF_HAS_BIFURCATION)) {
+                    if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_event_sum<REAL, STO>(v, t, j, gsum, false);
+                    gb = (REAL)gsum;
+                }
+                // per-trial counts: each trial stores on its own last sub-step
+                const bool store = (v.has(CMF_PER_TRIAL_DT) && substep >= 0)
+                                       ? (substep == v.max_sub_steps[t] - 1)
+                                       : (store_inflow != 0);
+                if (store) {
+                    v.river_inflow[e] = ri;
+                    v.flood_inflow[e] = fi;
+                    if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = gsum;
+                }
+                rin = (REAL)ri;
+                fin = (REAL)fi;
+            } else {
+                rin = (REAL)v.river_inflow[e];
+                fin = (REAL)v.flood_inflow[e];
+                rout = v.river_outflow[e];
+                fout = v.flood_outflow[e];
+                if (v.has(CMF_HAS_BIFURCATION)) gb = (REAL)v.gbif[e];
+                v.outgoing[e] = (STO)0.0;
+            }
+            const bool non_levee = !(v.has(CMF_HAS_LEVEE) && v.is_levee[j]);
+            stage_one<REAL, STO, LOG>(v, e, j, rin, fin, rout, fout, gb, time_step, non_levee, lp);
+        }
+    }
+    if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
+}
+
+// Fold the per-block partials into log slot current_step: one block, each
+// sum reduced over the blocks in a fixed order.  `mask` selects the sums the
+// launching kernel produced.
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_log_final(View<REAL, STO> v, int64_t nblocks, uint32_t mask) {
+    __shared__ double sh[kBlock];
+    const int step = v.current_step[0];
+    for (int k = 0; k < CMF_LOG_COUNT; ++k) {
+        if (!(mask & (1u << k))) continue;
+        double acc = 0.0;
+        for (int64_t b = threadIdx.x; b < nblocks; b += kBlock) acc += v.partials[b * CMF_LOG_COUNT + k];
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = kBlock / 2; w >= 1; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) v.log_sums[k][step] += sh[0] * 1e-9;
+        __syncthreads();
+    }
+}
+
+// ============================================================= levee ====
+template <class REAL, class STO, bool LOG>
+__global__ __launch_bounds__(kBlock) void k_levee(View<REAL, STO> v, int substep) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    LogPart lp;
+    if (LOG) {
+#pragma unroll
+        for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
+    }
+    if (g < v.V * v.T) {
+        const int64_t t = g / v.V, lv = g - t * v.V;
+        if (v.active(t, substep)) {
+            const int64_t c = v.levee_idx[lv];
+            const int64_t e = v.el(t, c);
+            const int64_t le = t * v.V + lv;
+            const int L = (int)v.L;
+            const REAL rl = v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, c);
+            const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, c);
+            const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, c);
+            const REAL ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, c);
+            const REAL crown = v.prm(v.levee_crown, CMF_BATCHED_LEVEE_CROWN_HEIGHT, le, lv);
+            const REAL lfrac = v.prm(v.levee_frac, CMF_BATCHED_LEVEE_FRACTION, le, lv);
+            const REAL base = v.prm(v.levee_base, CMF_BATCHED_LEVEE_BASE_HEIGHT, le, lv);
+            const STO rs_hp = v.river_storage[e], fs_hp = v.flood_storage[e];
+            const REAL fdep_curr = v.flood_depth[e];
+            const STO total_hp = rs_hp + fs_hp;
+            const REAL rs_c = (REAL)rs_hp, fs_c = (REAL)fs_hp, total = (REAL)total_hp;
+            const REAL* row = v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + c * v.L;
+
+            const REAL river_max_storage = rl * rw * rh;
+            const REAL dwth_inc = (ca / rl) / (REAL)L;
+            const REAL levee_distance = lfrac * (ca / rl);
+            REAL s_curr = river_max_storage;
+            REAL dhgt_pre = (REAL)0.0;
+            REAL dwth_pre = rw;
+            REAL base_sto = river_max_storage, fill_sto = river_max_storage;
+            bool found_base = false, found_fill = false;
+            const int ilev = (int)(lfrac * (REAL)L);
+            REAL dsto_fil_B = (REAL)0.0, dwth_fil_B = (REAL)0.0, ddph_fil_B = (REAL)0.0, gradient_B = (REAL)0.0;
+            bool found_B = false;
+            for (int k = 0; k < L; ++k) {
+                const REAL depth_val = row[k];
+                REAL dhgt_seg = depth_val - dhgt_pre;
+                dhgt_seg = vmax(dhgt_seg, (REAL)1e-6);
+                const REAL dwth_mid = dwth_pre + (REAL)0.5 * dwth_inc;
+                const REAL dsto_seg = rl * dwth_mid * dhgt_seg;
+                const REAL s_next = s_curr + dsto_seg;
+                const REAL gradient = dhgt_seg / dwth_inc;
+                const bool cond_base = (base > dhgt_pre) && (base <= depth_val);
+                if (cond_base) {
+                    const REAL ratio_base = (base - dhgt_pre) / dhgt_seg;
+                    base_sto = s_curr + rl * (dwth_pre + (REAL)0.5 * ratio_base * dwth_inc) * (ratio_base * dhgt_seg);
+                }
+                found_base = found_base || cond_base;
+                const bool cond_fill = (crown > dhgt_pre) && (crown <= depth_val);
+                if (cond_fill) {
+                    const REAL ratio_fill = (crown - dhgt_pre) / dhgt_seg;
+                    fill_sto = s_curr + rl * (dwth_pre + (REAL)0.5 * ratio_fill * dwth_inc) * (ratio_fill * dhgt_seg);
+                }
+                found_fill = found_fill || cond_fill;
+                const bool cond_check = (k >= ilev) && !found_B;
+                if (cond_check) {
+                    const REAL wedge = (levee_distance + rw) * (crown - depth_val) * rl;
+                    const REAL threshold = s_next + wedge;
+                    const bool cond_found = total < threshold;
+                    if (cond_found) {
+                        if (k == ilev) {
+                            const REAL dhgt_dif_loop = crown - base;
+                            dsto_fil_B = base_sto + (levee_distance + rw) * dhgt_dif_loop * rl;
+                        }
+                        gradient_B = gradient;
+                        found_B = true;
+                    } else {
+                        dsto_fil_B = threshold;
+                        dwth_fil_B = dwth_inc * (REAL)(k + 1) - levee_distance;
+                        ddph_fil_B = depth_val - base;
+                    }
+                }
+                s_curr = s_next;
+                dhgt_pre = depth_val;
+                dwth_pre += dwth_inc;
+            }
+            const REAL s_base_extra = s_curr + rl * dwth_pre * (base - dhgt_pre);
+            base_sto = found_base ? base_sto : ((base > dhgt_pre) ? s_base_extra : river_max_storage);
+            const REAL s_fill_extra = s_curr + rl * dwth_pre * (crown - dhgt_pre);
+            fill_sto = found_fill ? fill_sto : ((crown > dhgt_pre) ? s_fill_extra : river_max_storage);
+            const REAL dhgt_dif = crown - base;
+            const REAL s_top = base_sto + (levee_distance + rw) * dhgt_dif * rl;
+            const bool case4 = total >= fill_sto;
+            const bool case3 = !case4 && (total >= s_top);
+            const bool case2 = !case4 && !case3 && (total >= base_sto);
+
+            REAL r_cand = rs_c, f_cand = fs_c, r_dph = v.river_depth[e], f_dph = fdep_curr;
+            REAL p_dph = (REAL)0.0;
+            REAL f_frc = v.flood_fraction[e];
+            if (case2) {
+                const REAL dsto_add_c2 = total - base_sto;
+                const REAL dwth_add_c2 = levee_distance + rw;
+                const REAL f_dph_c2 = base + dsto_add_c2 / dwth_add_c2 / rl;
+                const REAL r_sto_c2 = river_max_storage + rl * rw * f_dph_c2;
+                r_cand = r_sto_c2;
+                f_cand = vmax(total - r_sto_c2, (REAL)0.0);
+                r_dph = r_sto_c2 / rl / rw;
+                f_dph = f_dph_c2;
+                f_frc = lfrac;
+            } else if (case3) {
+                const REAL dsto_add_B = total - dsto_fil_B;
+                REAL p_dph_B, f_frc_B;
+                if (found_B) {
+                    const REAL term_B = dwth_fil_B * dwth_fil_B + (REAL)2.0 * dsto_add_B / rl / (gradient_B + (REAL)1e-9);
+                    const REAL dwth_add_B = -dwth_fil_B + dsqrt(vmax(term_B, (REAL)0.0));
+                    const REAL ddph_add_B = dwth_add_B * gradient_B;
+                    p_dph_B = base + ddph_fil_B + ddph_add_B;
+                    f_frc_B = (dwth_fil_B + levee_distance) / (dwth_inc * (REAL)L);
+                } else {
+                    const REAL ddph_add_B_extra = dsto_add_B / (dwth_fil_B * rl + (REAL)1e-9);
+                    p_dph_B = base + ddph_fil_B + ddph_add_B_extra;
+                    f_frc_B = (REAL)1.0;
+                }
+                const REAL r_sto_c3 = river_max_storage + rl * rw * crown;
+                r_cand = r_sto_c3;
+                f_cand = vmax(s_top - r_sto_c3, (REAL)0.0);
+                r_dph = r_sto_c3 / rl / rw;
+                f_dph = crown;
+                p_dph = p_dph_B;
+                f_frc = vclamp(f_frc_B, (REAL)0.0, (REAL)1.0);
+            } else if (case4) {
+                const REAL dsto_add_c4 = (fdep_curr - crown) * (levee_distance + rw) * rl;
+                r_cand = rs_c;
+                f_cand = vmax(s_top + dsto_add_c4 - rs_c, (REAL)0.0);
+                p_dph = fdep_curr;
+            }
+            const bool none = !(case2 || case3 || case4);
+            STO r_hp = vmin((STO)r_cand, total_hp);
+            r_hp = (case4 || none) ? rs_hp : r_hp;
+            const STO remaining = vmax(total_hp - r_hp, (STO)0.0);
+            STO f_hp = vmin(vmax((STO)f_cand, (STO)0.0), remaining);
+            f_hp = case2 ? remaining : (none ? fs_hp : f_hp);
+            const STO p_hp = vmax(remaining - f_hp, (STO)0.0);
+
+            if (LOG) {
+                const STO stage_new = r_hp + f_hp + p_hp;
+                lp.q[CMF_LOG_STORAGE_STAGE] = (double)stage_new;
+                lp.q[CMF_LOG_RIVER_STORAGE] = (double)r_hp;
+                lp.q[CMF_LOG_FLOOD_STORAGE] = (double)f_hp;
+                lp.q[CMF_LOG_FLOOD_AREA] = (double)(f_frc * ca);
+                lp.q[CMF_LOG_STAGE_ERROR] = (double)(stage_new - total_hp);
+            }
+            v.river_storage[e] = r_hp;
+            v.flood_storage[e] = f_hp;
+            v.protected_storage[e] = p_hp;
+            v.river_depth[e] = r_dph;
+            v.flood_depth[e] = f_dph;
+            v.protected_depth[e] = p_dph;
+            v.flood_fraction[e] = f_frc;
+        }
+    }
+    if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
+}
+
+// =============================================================== CFL ====
+// grid (chunks, trials-or-1): block (bx, t) reduces catchments of trial t.
+// Without per-trial dt every trial folds into slot 0.
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_cfl(View<REAL, STO> v) {
+    __shared__ int sh[kBlock / 64];
+    const int64_t t = blockIdx.y;
+    const REAL outer = v.outer_time_step[0];
+    int best = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < v.N;
+         i += (int64_t)gridDim.x * kBlock) {
+        int steps = 1;
+        if (!(v.has(CMF_HAS_RESERVOIR) && v.is_dam_related[i])) {
+            const int64_t e = v.el(t, i);
+            const REAL depth = vmax(v.river_depth[e], (REAL)0.01);
+            const REAL dist = v.prm(v.dist, CMF_BATCHED_DOWNSTREAM_DISTANCE, e, i);
+            const REAL dt = v.adaptive_factor * dist / dsqrt(v.gravity * depth);
+            const REAL dtc = vmin(dt, outer);
+            const REAL nf = floor(outer / dtc - (REAL)0.01) + (REAL)1.0;
+            steps = (int)nf;
+        }
+        best = max(best, steps);
+    }
+    best = wave_max_int(best);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = sh[0];
+        for (int w = 1; w < kBlock / 64; ++w) m = max(m, sh[w]);
+        atomicMax(&v.max_sub_steps[v.has(CMF_PER_TRIAL_DT) ? t : 0], m);
+    }
+}
+
+// time_step[t] = outer / count[t], in double then rounded (cama_flood_model.py:127,132)
+template <class REAL>
+__global__ void k_set_dt(REAL* time_step, const int32_t* counts, int64_t n, double outer) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int c = counts[t] < 1 ? 1 : counts[t];
+    time_step[t] = (REAL)(outer / (double)c);
+}
+
+__global__ void k_zero_int(int32_t* p, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+
+// ============================================= explicit instantiation ===
+#define CMF_INSTANTIATE(REAL, STO)                                                           \
+    template __global__ void k_outflow<REAL, STO, false>(View<REAL, STO>, int);             \
+    template __global__ void k_outflow<REAL, STO, true>(View<REAL, STO>, int);              \
+    template __global__ void k_outgoing_sum<REAL, STO, false>(View<REAL, STO>);             \
+    template __global__ void k_outgoing_sum<REAL, STO, true>(View<REAL, STO>);              \
+    template __global__ void k_reservoir<REAL, STO, false>(View<REAL, STO>, int);           \
+    template __global__ void k_reservoir<REAL, STO, true>(View<REAL, STO>, int);            \
+    template __global__ void k_res_regather<REAL, STO, false>(View<REAL, STO>);             \
+    template __global__ void k_res_regather<REAL, STO, true>(View<REAL, STO>);              \
+    template __global__ void k_bif_outflow<REAL, STO, false>(View<REAL, STO>, int);         \
+    template __global__ void k_bif_outflow<REAL, STO, true>(View<REAL, STO>, int);          \
+    template __global__ void k_bif_outgoing<REAL, STO>(View<REAL, STO>);                    \
+    template __global__ void k_bif_inflow<REAL, STO>(View<REAL, STO>, int);                 \
+    template __global__ void k_gbif_sum<REAL, STO>(View<REAL, STO>);                        \
+    template __global__ void k_inflow_limit<REAL, STO>(View<REAL, STO>);                    \
+    template __global__ void k_inflow_sum<REAL, STO, false>(View<REAL, STO>);               \
+    template __global__ void k_inflow_sum<REAL, STO, true>(View<REAL, STO>);                \
+    template __global__ void k_limiter<REAL, STO, false>(View<REAL, STO>, int);             \
+    template __global__ void k_limiter<REAL, STO, true>(View<REAL, STO>, int);              \
+    template __global__ void k_levee<REAL, STO, false>(View<REAL, STO>, int);               \
+    template __global__ void k_levee<REAL, STO, true>(View<REAL, STO>, int);                \
+    template __global__ void k_cfl<REAL, STO>(View<REAL, STO>);                             \
+    template __global__ void k_log_final<REAL, STO>(View<REAL, STO>, int64_t, uint32_t);    \
+    template __global__ void k_set_dt<REAL>(REAL*, const int32_t*, int64_t, double);
+
+CMF_INSTANTIATE(float, double)
+CMF_INSTANTIATE(double, double)
+
+#define CMF_INSTANTIATE_STAGE(REAL, STO, FUSED, CONTIG)                                     \
+    template __global__ void k_stage<REAL, STO, FUSED, CONTIG, false>(View<REAL, STO>, int, int); \
+    template __global__ void k_stage<REAL, STO, FUSED, CONTIG, true>(View<REAL, STO>, int, int);
+CMF_INSTANTIATE_STAGE(float, double, false, false)
+CMF_INSTANTIATE_STAGE(float, double, true, false)
+CMF_INSTANTIATE_STAGE(float, double, true, true)
+CMF_INSTANTIATE_STAGE(double, double, false, false)
+CMF_INSTANTIATE_STAGE(double, double, true, false)
+CMF_INSTANTIATE_STAGE(double, double, true, true)
+
+}  // namespace cmf

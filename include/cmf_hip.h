@@ -1,0 +1,302 @@
+/*
+ * cmf_hip.h — C ABI of the MI355X (gfx950) CaMa-Flood hydrodynamic hot path.
+ *
+ * One shared library, libcmfhip.so, built from cama-flood-gpu_amd/csrc.
+ * Every entry point returns a cmf_status (0 = success).  On failure
+ * cmf_last_error() returns a message for the calling thread.
+ *
+ * Buffers are borrowed, never owned: the caller allocates every field (the
+ * reference's module fields, cmfgpu/modules/base.py:92-455) and passes raw
+ * device pointers in a cmf_fields table.  The library owns only the network
+ * scratch it allocates in cmf_network_create and frees in cmf_network_destroy.
+ *
+ * Launches are asynchronous on the caller's HIP stream.  Only
+ * cmf_hip_advance synchronises, once per outer step, to read the CFL
+ * sub-step count back (as CaMaFlood.step_advance does with .item(),
+ * cmfgpu/models/cama_flood_model.py:122).
+ *
+ * Accumulations that the reference performs with float atomics are
+ * deterministic here: each target sums its contributions in a fixed order
+ * (upstream catchments in ascending network index, then its own term, then
+ * bifurcation paths in ascending path index), so results are bitwise
+ * reproducible run to run.
+ */
+#ifndef CMF_HIP_H
+#define CMF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cmf_status {
+    CMF_OK = 0,
+    CMF_ERR_INVALID = 1,     /* bad argument, shape or network */
+    CMF_ERR_HIP = 2,         /* HIP runtime or launch failure */
+    CMF_ERR_NOMEM = 3,       /* device allocation failed */
+    CMF_ERR_UNSUPPORTED = 4, /* feature or dtype combination not built */
+    CMF_ERR_INTERNAL = 5
+} cmf_status;
+
+/* Feature flags: the reference's compile-time HAS_* constants
+ * (cmfgpu/phys/specs.py:28-39), plus the log and adaptive-time modules. */
+enum {
+    CMF_HAS_BIFURCATION = 1u << 0,
+    CMF_HAS_LEVEE = 1u << 1,
+    CMF_HAS_RESERVOIR = 1u << 2,
+    CMF_HAS_LOG = 1u << 3,
+    CMF_HAS_TOTAL_STORAGE = 1u << 4,
+    CMF_HAS_WATER_SURFACE = 1u << 5,
+    CMF_HAS_PROTECTED_WATER_SURFACE = 1u << 6,
+    CMF_HAS_ADAPTIVE_TIME = 1u << 7,
+    /* time_step (and max_sub_steps) hold one entry per trial, so every trial
+     * advances with its own CFL sub-step (an extension of the reference, whose
+     * trials share one time_step, cmfgpu/phys/triton/outflow.py:454). */
+    CMF_PER_TRIAL_DT = 1u << 8
+};
+
+/* Per-trial parameter flags: the reference's batched_* constants
+ * (cmfgpu/phys/specs.py:169-176,199-206,471-477,651-658,338-341,414). */
+enum {
+    CMF_BATCHED_RIVER_MANNING = 1u << 0,
+    CMF_BATCHED_FLOOD_MANNING = 1u << 1,
+    CMF_BATCHED_RIVER_WIDTH = 1u << 2,
+    CMF_BATCHED_RIVER_LENGTH = 1u << 3,
+    CMF_BATCHED_RIVER_HEIGHT = 1u << 4,
+    CMF_BATCHED_CATCHMENT_ELEVATION = 1u << 5,
+    CMF_BATCHED_DOWNSTREAM_DISTANCE = 1u << 6,
+    CMF_BATCHED_RUNOFF = 1u << 7,
+    CMF_BATCHED_FLOOD_DEPTH_TABLE = 1u << 8,
+    CMF_BATCHED_CATCHMENT_AREA = 1u << 9,
+    CMF_BATCHED_LEVEE_CROWN_HEIGHT = 1u << 10,
+    CMF_BATCHED_LEVEE_FRACTION = 1u << 11,
+    CMF_BATCHED_LEVEE_BASE_HEIGHT = 1u << 12,
+    CMF_BATCHED_BIFURCATION_MANNING = 1u << 13,
+    CMF_BATCHED_BIFURCATION_WIDTH = 1u << 14,
+    CMF_BATCHED_BIFURCATION_LENGTH = 1u << 15,
+    CMF_BATCHED_BIFURCATION_ELEVATION = 1u << 16
+};
+
+/* Precision profiles: REAL is the flux/stage dtype, STO the hpfloat storage
+ * dtype (cmfgpu/phys/cuda/outflow.cu:301-366 dispatches the same pairs). */
+enum { CMF_PRECISION_F32_F64 = 0, CMF_PRECISION_F64_F64 = 1 };
+
+/* The 11 water-balance sums, in LogModule field order
+ * (cmfgpu/modules/log.py:207-326). Held in FP64. */
+enum {
+    CMF_LOG_STORAGE_PRE = 0, CMF_LOG_STORAGE_NEXT, CMF_LOG_STORAGE_NEW,
+    CMF_LOG_INFLOW_ERROR, CMF_LOG_INFLOW, CMF_LOG_OUTFLOW, CMF_LOG_STORAGE_STAGE,
+    CMF_LOG_STAGE_ERROR, CMF_LOG_RIVER_STORAGE, CMF_LOG_FLOOD_STORAGE,
+    CMF_LOG_FLOOD_AREA, CMF_LOG_COUNT
+};
+
+/*
+ * Field table.  Names are the reference's module field names (the KernelSpec
+ * parameter names without "_ptr", cmfgpu/phys/specs.py:124-803).  void*
+ * members hold REAL or STO data according to `precision`; the comment gives
+ * which.  Catchment state is laid out [num_trials][num_catchments] (the
+ * reference's trial layout) unless trial_stride/catchment_stride say
+ * otherwise: element (t, i) lives at t * trial_stride + i * catchment_stride.
+ * Per-path bifurcation state uses path_trial_stride/path_stride the same way
+ * (per path and level: element * num_bifurcation_levels + level).
+ * Unused optional fields may be NULL.
+ */
+typedef struct cmf_fields {
+    /* dimensions */
+    int64_t num_catchments;
+    int64_t num_trials;              /* >= 1 */
+    int64_t num_flood_levels;
+    int64_t num_bifurcation_paths;
+    int64_t num_bifurcation_levels;
+    int64_t num_levees;
+    int64_t num_reservoirs;
+    int64_t log_buffer_size;
+    int64_t trial_stride;            /* 0 selects num_catchments */
+    int64_t catchment_stride;        /* 0 selects 1 */
+    int64_t path_trial_stride;       /* 0 selects num_bifurcation_paths */
+    int64_t path_stride;             /* 0 selects 1 */
+    int32_t precision;               /* CMF_PRECISION_* */
+    uint32_t flags;                  /* CMF_HAS_* */
+    uint32_t batched;                /* CMF_BATCHED_* */
+    uint32_t reserved0;
+    /* scalars (base.py:76-87, adaptive_time.py:30-35) */
+    double gravity;
+    double adaptive_time_factor;
+    double min_kinematic_slope;
+    /* shared-state device scalars (base.py:424-455, adaptive_time.py:37-45) */
+    void* time_step;                 /* REAL[1], or REAL[num_trials] with CMF_PER_TRIAL_DT */
+    void* outer_time_step;           /* REAL[1] */
+    int32_t* current_step;           /* int32[1] */
+    int32_t* max_sub_steps;          /* int32[1], or int32[num_trials] with CMF_PER_TRIAL_DT */
+    /* topology */
+    const int32_t* downstream_idx;   /* [N] */
+    /* base module */
+    void* river_inflow;              /* STO */
+    void* flood_inflow;              /* STO */
+    void* river_outflow;             /* REAL */
+    void* flood_outflow;             /* REAL */
+    void* river_manning;             /* REAL */
+    void* flood_manning;             /* REAL */
+    void* river_depth;               /* REAL */
+    void* flood_depth;               /* REAL */
+    void* protected_depth;           /* REAL */
+    void* river_width;               /* REAL */
+    void* river_length;              /* REAL */
+    void* river_height;              /* REAL */
+    void* catchment_elevation;       /* REAL */
+    void* catchment_area;            /* REAL */
+    void* downstream_distance;       /* REAL */
+    void* river_storage;             /* STO */
+    void* flood_storage;             /* STO */
+    void* protected_storage;         /* STO */
+    void* river_cross_section_depth; /* REAL */
+    void* flood_cross_section_depth; /* REAL */
+    void* flood_cross_section_area;  /* REAL */
+    void* flood_fraction;            /* REAL */
+    void* runoff;                    /* REAL */
+    void* flood_depth_table;         /* REAL [N, num_flood_levels] */
+    void* outgoing_storage;          /* STO */
+    void* total_storage;             /* STO, with CMF_HAS_TOTAL_STORAGE */
+    void* water_surface_elevation;   /* REAL, with CMF_HAS_WATER_SURFACE */
+    void* protected_water_surface_elevation; /* REAL */
+    const uint8_t* is_levee;         /* bool [N], with CMF_HAS_LEVEE */
+    /* bifurcation module (bifurcation.py:64-214) */
+    const int32_t* bifurcation_catchment_idx;   /* [P] */
+    const int32_t* bifurcation_downstream_idx;  /* [P] */
+    void* bifurcation_manning;       /* REAL [P, Lb] */
+    void* bifurcation_outflow;       /* REAL [P, Lb] per trial */
+    void* bifurcation_width;         /* REAL [P, Lb] */
+    void* bifurcation_length;        /* REAL [P] */
+    void* bifurcation_elevation;     /* REAL [P, Lb] */
+    void* bifurcation_cross_section_depth; /* REAL [P, Lb] per trial */
+    void* global_bifurcation_outflow;      /* STO [N] per trial */
+    void* limit_rate;                /* REAL [N] per trial */
+    /* levee module (levee.py:73-244) */
+    const int32_t* levee_catchment_idx;     /* [V] */
+    void* levee_base_height;         /* REAL [V] */
+    void* levee_crown_height;        /* REAL [V] */
+    void* levee_fraction;            /* REAL [V] */
+    /* reservoir module (reservoir.py:64-265) */
+    const int32_t* reservoir_catchment_idx; /* [R] */
+    void* conservation_volume;       /* REAL [R] */
+    void* emergency_volume;          /* REAL [R] */
+    void* adjustment_volume;         /* REAL [R] */
+    void* effective_normal_outflow;  /* REAL [R] */
+    void* adjustment_outflow;        /* REAL [R] */
+    void* flood_control_outflow;     /* REAL [R] */
+    void* reservoir_total_inflow;    /* STO [N] per trial */
+    const uint8_t* is_reservoir;     /* bool [N] */
+    const uint8_t* is_dam_upstream;  /* bool [N] */
+    const uint8_t* is_dam_related;   /* bool [N] */
+    /* log module: CMF_LOG_COUNT FP64 buffers of log_buffer_size entries each,
+     * one set per trial (buffer k of trial t at log_sums[k] + t * log_buffer_size
+     * * CMF_LOG_COUNT is not used: trials share one set, as LogModule refuses
+     * trials, cmfgpu/models/cama_flood_model.py:65-67). */
+    double* log_sums[CMF_LOG_COUNT];
+} cmf_fields;
+
+/* Opaque network context: upstream lists, bifurcation event lists and
+ * device scratch, built once per network (and trial count). */
+typedef struct cmf_network cmf_network;
+
+/* ------------------------------------------------------------------ info */
+const char* cmf_last_error(void);
+int cmf_version(void);
+/* Size of cmf_fields as compiled into the library (ABI check). */
+int64_t cmf_fields_size(void);
+
+/* -------------------------------------------------------- host indexing */
+/* cmfgpu/params/utils.py:20-80 */
+int cmf_trace_outlets(int64_t n, const int64_t* catchment_id,
+                      const int64_t* downstream_id, int64_t* out_mouth);
+/* cmfgpu/params/utils.py:83-138; raises on rings like :135-136 */
+int cmf_topological_sort(int64_t n, const int64_t* catchment_id,
+                         const int64_t* downstream_id, int32_t* out_idx);
+/* cmfgpu/params/utils.py:483-503 (ties between equal basin sizes: ascending label) */
+int cmf_reorder_by_basin_size(int64_t n, const int64_t* topo_idx, const int64_t* basin_id,
+                              int64_t* out_order, int64_t* out_basin_sizes,
+                              int64_t* out_num_basins);
+/* hydroforge find_indices_in as used at cmfgpu/params/merit_map.py:450 (-1 = absent) */
+int cmf_find_indices_in(int64_t n_query, const int64_t* query, int64_t n_ref,
+                        const int64_t* ref, int64_t* out);
+/* cmfgpu/params/merit_map.py:1063-1070 */
+int cmf_check_flow_direction(int64_t n, const int32_t* downstream_idx);
+/* cmfgpu/params/estimate_river_geometry.py:76-104 */
+int cmf_accumulate_downstream(int64_t n, const int32_t* downstream_idx,
+                              const double* local, double* out);
+/* Breadth-first level renumbering used for the coalesced device layout
+ * (no reference counterpart; the reference keeps its topological order). */
+int cmf_level_order(int64_t n, const int32_t* downstream_idx, int32_t* out_perm,
+                    int32_t* out_inv, int32_t* out_up_ptr, int32_t* out_dn,
+                    int64_t* out_num_levels, int64_t level_ptr_capacity,
+                    int64_t* out_level_ptr);
+
+/* -------------------------------------------------------------- network */
+/* Build the device network context from host copies of the indices.
+ * `downstream_idx` must satisfy downstream_idx[i] > i for non-mouths and
+ * == i at mouths.  Bifurcation indices may be NULL when num_paths == 0.
+ * `num_trials` sizes the scratch.  Allocates device memory once. */
+int cmf_network_create(int64_t num_catchments, const int32_t* downstream_idx,
+                       int64_t num_paths, const int32_t* bifurcation_catchment_idx,
+                       const int32_t* bifurcation_downstream_idx, int64_t num_trials,
+                       cmf_network** out);
+int cmf_network_destroy(cmf_network* net);
+/* 1 when every upstream range is the identity (breadth-first layout). */
+int cmf_network_is_contiguous(const cmf_network* net);
+
+/* ------------------------------------------------- per-op entry points ---
+ * Each replaces one reference launcher (the .cu files of cmfgpu/phys/cuda, routed at
+ * cmfgpu/phys/cuda/__init__.py:68-155) and one Triton kernel; the fields read
+ * and written are those of the matching KernelSpec.  `stream` is a
+ * hipStream_t (NULL = default stream). */
+
+/* compute_outflow: cuda/outflow.cu:272-298, triton/outflow.py:15-315 (+:386-717 batched) */
+int cmf_hip_outflow(cmf_network* net, const cmf_fields* f, void* stream);
+/* compute_inflow: cuda/outflow.cu:389-399, triton/outflow.py:318-383 (+:720-794) */
+int cmf_hip_inflow(cmf_network* net, const cmf_fields* f, void* stream);
+/* compute_flood_stage: cuda/storage.cu:318-334, triton/storage.py:16-193 */
+int cmf_hip_flood_stage(cmf_network* net, const cmf_fields* f, void* stream);
+/* compute_flood_stage_log: cuda/storage.cu:348-374, triton/storage.py:196-425 */
+int cmf_hip_flood_stage_log(cmf_network* net, const cmf_fields* f, void* stream);
+/* compute_adaptive_time_step: cuda/adaptive_time.cu:50-56, triton/adaptive_time.py:13-103.
+ * Writes max over catchments of the CFL sub-step count into max_sub_steps
+ * (atomic max, as the reference; per trial with CMF_PER_TRIAL_DT). */
+int cmf_hip_adaptive_time_step(cmf_network* net, const cmf_fields* f, void* stream);
+/* compute_bifurcation_outflow: cuda/bifurcation.cu:158-169, triton/bifurcation.py:13-104 */
+int cmf_hip_bifurcation_outflow(cmf_network* net, const cmf_fields* f, void* stream);
+/* compute_bifurcation_inflow: cuda/bifurcation.cu:201-208, triton/bifurcation.py:106-151 */
+int cmf_hip_bifurcation_inflow(cmf_network* net, const cmf_fields* f, void* stream);
+/* compute_levee_stage: cuda/levee.cu:343-353, triton/levee.py:62-304 */
+int cmf_hip_levee_stage(cmf_network* net, const cmf_fields* f, void* stream);
+/* compute_levee_stage_log: cuda/levee.cu:382-395, triton/levee.py:307-562 */
+int cmf_hip_levee_stage_log(cmf_network* net, const cmf_fields* f, void* stream);
+/* compute_levee_bifurcation_outflow: cuda/levee.cu:424-437, triton/levee.py:565-684 */
+int cmf_hip_levee_bifurcation_outflow(cmf_network* net, const cmf_fields* f, void* stream);
+/* compute_reservoir_outflow: cuda/reservoir.cu:96-106, triton/reservoir.py:17-184 */
+int cmf_hip_reservoir_outflow(cmf_network* net, const cmf_fields* f, void* stream);
+
+/* ------------------------------------------------------ fused schedule ---
+ * One sub-step of CaMaFlood.step_advance (cmfgpu/models/cama_flood_model.py:
+ * 134-162) with the ops fused into the fewest passes the dependencies allow:
+ * outflow, [reservoir], [bifurcation outflow], limiter, [bifurcation inflow],
+ * inflow + flood stage [+ levee stage] [+ log].  outgoing_storage is never
+ * materialised (it is zero after every sub-step in the reference too).
+ * `substep` is the sub-step index (log slot); with CMF_PER_TRIAL_DT a trial
+ * whose count max_sub_steps[t] is <= substep is left unchanged.
+ * `store_inflow` != 0 writes river_inflow / flood_inflow (the reference
+ * leaves the last sub-step's inflow there). */
+int cmf_hip_substep(cmf_network* net, const cmf_fields* f, int32_t substep,
+                    int32_t store_inflow, void* stream);
+/* One outer step: CFL (with CMF_HAS_ADAPTIVE_TIME), one host read of the
+ * sub-step count, time_step = outer / count, then `count` fused sub-steps.
+ * `fixed_count` is used without adaptive time.  The count used is written to
+ * *out_count (the maximum over trials with CMF_PER_TRIAL_DT). */
+int cmf_hip_advance(cmf_network* net, const cmf_fields* f, double outer_seconds,
+                    int32_t fixed_count, int32_t* out_count, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CMF_HIP_H */

@@ -1,0 +1,106 @@
+"""Network indexing (integer work): the C-ABI routines vs the pure-Python
+restatement in oracle/ordering.py, bit for bit, plus textbook invariants."""
+import numpy as np
+import pytest
+
+from cmfgpu_amd import network, synthetic
+from oracle import ordering
+
+
+def random_forest(n, seed, p_outlet=0.1):
+    """Generation-order forest: node g links to an earlier node (or is an outlet)."""
+    rng = np.random.default_rng(seed)
+    dn = np.full(n, -1, np.int64)
+    for g in range(1, n):
+        if rng.random() >= p_outlet:
+            dn[g] = rng.integers(max(0, g - 7), g)
+    ids = rng.permutation(n).astype(np.int64) * 3 + 5
+    down_ids = np.where(dn >= 0, ids[np.maximum(dn, 0)], -1)
+    return ids, down_ids
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_trace_topo_reorder_match_restatement(seed):
+    ids, down = random_forest(200, seed)
+    assert np.array_equal(network.trace_outlets(ids, down), ordering.trace_outlets(ids, down))
+    topo = network.topological_sort(ids, down)
+    assert np.array_equal(topo, ordering.topological_sort(ids, down))
+    basin = network.trace_outlets(ids, down)
+    o1, s1 = network.reorder_by_basin_size(topo, basin)
+    o2, s2 = ordering.reorder_by_basin_size(topo, basin)
+    assert np.array_equal(o1, o2) and np.array_equal(s1, s2)
+
+
+def test_topological_sort_is_kahn_fifo_on_a_textbook_graph():
+    # 0->2, 1->2, 2->4, 3->4 (ids = indices); Kahn FIFO from headwaters: 0 1 3 2 4
+    ids = np.arange(5)
+    down = np.array([2, 2, 4, 4, -1])
+    assert network.topological_sort(ids, down).tolist() == [0, 1, 3, 2, 4]
+
+
+def test_ring_raises_like_the_reference():
+    ids = np.array([1, 2, 3])
+    down = np.array([2, 3, 1])
+    with pytest.raises(ValueError, match="Rings exist"):
+        network.topological_sort(ids, down)
+
+
+def test_finalize_connectivity_invariants():
+    ids, down = random_forest(500, 3)
+    c = network.finalize_connectivity(ids, down)
+    n = ids.shape[0]
+    idx = np.arange(n)
+    mouth = c.downstream_idx == idx
+    assert np.all(c.downstream_idx[~mouth] > idx[~mouth])          # merit_map.py:1063-1070
+    assert np.array_equal(np.sort(c.order), np.arange(n))          # permutation
+    assert np.array_equal(c.downstream_id[mouth], c.catchment_id[mouth])
+    # basins are contiguous and sized in descending order
+    b = c.catchment_basin_id
+    assert np.all(np.diff(b) >= 0)
+    assert np.all(np.diff(np.bincount(b)) <= 0)
+
+
+def test_check_flow_direction_raises():
+    with pytest.raises(ValueError, match="Flow direction error"):
+        network.check_flow_direction(np.array([1, 0, 2], np.int32))
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_level_order_matches_restatement(seed):
+    ids, down = random_forest(300, seed, p_outlet=0.05)
+    c = network.finalize_connectivity(ids, down)
+    lo = network.level_order(c.downstream_idx)
+    perm, inv, up_ptr, dn, nl = ordering.level_order(c.downstream_idx.tolist())
+    assert np.array_equal(lo.perm, perm)
+    assert np.array_equal(lo.inv, inv)
+    assert np.array_equal(lo.up_ptr, up_ptr)
+    assert np.array_equal(lo.dn, dn)
+    assert lo.num_levels == nl
+
+
+def test_level_order_properties_at_config0_size():
+    net = synthetic.build_config(0)
+    d = net.fields["downstream_idx"]
+    lo = network.level_order(d)
+    n = d.shape[0]
+    k = np.arange(n)
+    mouth = lo.dn == k
+    assert np.all(lo.dn[~mouth] > k[~mouth])
+    assert np.array_equal(lo.inv[lo.perm], k)
+    # upstream ranges are exactly the catchments whose downstream is k
+    counts = np.bincount(lo.dn[~mouth], minlength=n)
+    assert np.array_equal(np.diff(lo.up_ptr), counts)
+    owner = np.repeat(k, np.diff(lo.up_ptr))
+    assert np.array_equal(lo.dn[:lo.up_ptr[-1]], owner)
+    # inside each range the network order is ascending (accumulation order)
+    up_net = lo.perm[: lo.up_ptr[-1]]
+    same = owner[1:] == owner[:-1]
+    assert np.all(up_net[1:][same] > up_net[:-1][same])
+
+
+def test_find_indices_and_accumulate():
+    ref = np.array([40, 10, 30, 20])
+    assert network.find_indices_in(np.array([10, 20, 99, 40]), ref).tolist() == [1, 3, -1, 0]
+    d = np.array([2, 2, 3, 3], np.int32)
+    out = network.accumulate_downstream(d, np.array([1.0, 2.0, 4.0, 8.0]))
+    assert out.tolist() == [1.0, 2.0, 7.0, 15.0]

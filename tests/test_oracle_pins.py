@@ -1,0 +1,404 @@
+"""Pins of the CPU oracle against what the mathematics fixes.
+
+The reference ships no tests or golden vectors (its tests/ is git-ignored),
+so each oracle kernel is pinned here by closed forms, invariants and
+brute-force sums that a dropped term, a wrong sign or a wrong index would
+break.  Every expected value below is computed independently of the oracle's
+formula (in float64 from the physics), not by retyping it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+G = 9.8
+
+
+def tiny_state(dn, L=10, **over):
+    n = len(dn)
+    s = {
+        "downstream_idx": np.asarray(dn, np.int32),
+        "river_inflow": np.zeros(n), "flood_inflow": np.zeros(n),
+        "river_outflow": np.zeros(n, np.float32), "flood_outflow": np.zeros(n, np.float32),
+        "river_manning": np.full(n, 0.03, np.float32), "flood_manning": np.full(n, 0.1, np.float32),
+        "river_depth": np.zeros(n, np.float32), "flood_depth": np.zeros(n, np.float32),
+        "protected_depth": np.zeros(n, np.float32),
+        "river_width": np.full(n, 50.0, np.float32), "river_length": np.full(n, 10_000.0, np.float32),
+        "river_height": np.full(n, 4.0, np.float32), "catchment_elevation": np.zeros(n, np.float32),
+        "catchment_area": np.full(n, 5e8, np.float32), "downstream_distance": np.full(n, 10_000.0, np.float32),
+        "river_storage": np.zeros(n), "flood_storage": np.zeros(n), "protected_storage": np.zeros(n),
+        "river_cross_section_depth": np.zeros(n, np.float32),
+        "flood_cross_section_depth": np.zeros(n, np.float32),
+        "flood_cross_section_area": np.zeros(n, np.float32),
+        "flood_fraction": np.zeros(n, np.float32), "runoff": np.zeros(n, np.float32),
+        "flood_depth_table": np.tile(np.cumsum(np.full(L, 0.5, np.float32)), (n, 1)).astype(np.float32),
+        "outgoing_storage": np.zeros(n),
+    }
+    for k, v in over.items():
+        s[k] = np.ascontiguousarray(v, dtype=s[k].dtype if k in s else None)
+    return s
+
+
+def model(state, flags=0, L=10, precision=0):
+    return O.OracleModel(state, flags=flags, num_flood_levels=L, precision=precision)
+
+
+# ------------------------------------------------------------ math helpers --
+def test_cbrt_correct_rounding_and_exact_cubes():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.lognormal(0, 8, 20000), [1e-6, 1e-5, 0.125, 8.0, 27.0, 1e30, 3.0]]).astype(np.float32)
+    for x in xs:
+        got = O.cbrtf(float(x))
+        ref = np.float32(np.cbrt(np.float64(x)))   # correctly rounded except in rare hard cases
+        assert abs(int(np.float32(got).view(np.int32)) - int(ref.view(np.int32))) <= 1, (x, got, ref)
+    for k in (1, 2, 3, 10, 100):
+        assert O.cbrtf(float(k ** 3)) == float(k)
+    assert O.cbrtf(0.0) == 0.0
+
+
+def test_log_exp_within_one_ulp():
+    lib = O.lib()
+    rng = np.random.default_rng(1)
+    for x in rng.uniform(1e-4, 1.0, 5000).astype(np.float32):
+        ref = np.float32(np.log(np.float64(x)))
+        got = np.float32(lib.cmf_oracle_logf(float(x)))
+        assert abs(int(got.view(np.int32)) - int(ref.view(np.int32))) <= 1
+    for y in rng.uniform(-20, 5, 5000).astype(np.float32):
+        ref = np.float32(np.exp(np.float64(y)))
+        got = np.float32(lib.cmf_oracle_expf(float(y)))
+        assert abs(int(got.view(np.int32)) - int(ref.view(np.int32))) <= 1
+
+
+# ---------------------------------------------------------------- outflow --
+def test_outflow_steady_uniform_flow_is_a_fixed_point():
+    """Setting q_new = q_old in the semi-implicit update gives Manning's law
+    q = h^(5/3) S^(1/2) / n per unit width (triton/outflow.py:200-216)."""
+    h, S, n_man, W, dx = 2.0, 4e-4, 0.03, 50.0, 10_000.0
+    q_unit = h ** (5 / 3) * math.sqrt(S) / n_man
+    # catchment 0 flows into mouth 1; water surfaces differ by S * dx
+    elev = np.array([10.0 + S * dx, 10.0], np.float32)
+    height = np.array([4.0, 4.0], np.float32)
+    depth = np.array([h, h], np.float32)
+    s = tiny_state([1, 1], river_depth=depth, catchment_elevation=elev, river_height=height,
+                   river_cross_section_depth=np.array([h, h], np.float32),
+                   river_outflow=np.array([W * q_unit, 0.0], np.float32),
+                   river_storage=np.array([h * W * dx, h * W * dx]))
+    m = model(s)
+    m.kernel("outflow", time_step=600.0)
+    assert s["river_outflow"][0] == pytest.approx(W * q_unit, rel=2e-5)
+
+
+def test_outflow_zero_initial_state_gives_zero_flow_and_dry_flood():
+    """Zero cross-section state: semi-implicit depth 1e-6 <= 1e-5, so q = 0
+    exactly on the first step; no floodplain storage, no flood flow."""
+    s = tiny_state([1, 2, 2], river_depth=np.array([1.0, 2.0, 0.5], np.float32),
+                   catchment_elevation=np.array([30.0, 20.0, 10.0], np.float32))
+    m = model(s)
+    m.kernel("outflow", time_step=600.0)
+    assert np.all(s["river_outflow"] == 0.0) and np.all(s["flood_outflow"] == 0.0)
+    # the new cross-section depth is the depth under the higher water surface
+    wse = s["catchment_elevation"] - s["river_height"] + s["river_depth"]
+    expect0 = max(wse[0], wse[1]) - (30.0 - 4.0)
+    assert s["river_cross_section_depth"][0] == pytest.approx(expect0, rel=1e-6)
+    # a mouth uses its own depth (outflow.py:163-167)
+    assert s["river_cross_section_depth"][2] == np.float32(0.5)
+
+
+def test_outflow_sign_follows_slope_and_outgoing_is_positive_part():
+    h, dx, W = 2.0, 10_000.0, 50.0
+    for drop in (+0.5, -0.5):
+        elev = np.array([10.0 + drop, 10.0, 0.0], np.float32)
+        s = tiny_state([1, 2, 2], river_depth=np.full(3, h, np.float32), catchment_elevation=elev,
+                       river_cross_section_depth=np.full(3, h, np.float32),
+                       river_storage=np.full(3, h * W * dx))
+        m = model(s)
+        dt = 60.0
+        m.kernel("outflow", time_step=dt)
+        q0 = float(s["river_outflow"][0])
+        # from rest the denominator is 1: q = W g dt h_s S, with h_s the
+        # semi-implicit depth sqrt(h_new h_old), h_new measured under the
+        # higher of the two water surfaces
+        S = (drop) / dx
+        wse = elev.astype(np.float64) - 4.0 + h
+        h_new = max(wse[0], wse[1]) - (float(elev[0]) - 4.0)
+        h_s = math.sqrt(h_new * h)
+        assert q0 == pytest.approx(W * G * dt * h_s * S, rel=1e-5)
+        og = s["outgoing_storage"]
+        if q0 > 0:
+            assert og[0] == pytest.approx(q0 * dt, rel=1e-6)
+        else:
+            # negative flow leaves catchment 1 (the downstream one): its outgoing holds -q dt
+            assert og[0] == 0.0
+            assert og[1] >= -q0 * dt * (1 - 1e-6)
+
+
+# ----------------------------------------------------------------- inflow --
+def test_inflow_is_the_sum_of_upstream_flows():
+    rng = np.random.default_rng(3)
+    n = 40
+    dn = np.array([min(n - 1, i + 1 + int(rng.integers(0, 3))) for i in range(n)], np.int32)
+    dn[n - 1] = n - 1
+    q = rng.integers(0, 50, n).astype(np.float32)
+    s = tiny_state(dn, river_outflow=q, flood_outflow=q * 0.5,
+                   river_storage=np.full(n, 1e12), outgoing_storage=np.zeros(n))
+    m = model(s)
+    m.kernel("inflow")
+    expect = np.zeros(n)
+    for i in range(n):
+        if dn[i] != i:
+            expect[dn[i]] += q[i]
+    assert np.array_equal(s["river_inflow"], expect)          # integer-valued: exact
+    assert s["river_inflow"].sum() == q[dn != np.arange(n)].sum()
+
+
+def test_inflow_limiter_caps_outgoing_at_storage():
+    # catchment 0 wants to send 10x its storage; the limiter scales to storage
+    s = tiny_state([1, 1], river_outflow=np.array([100.0, 0.0], np.float32),
+                   river_storage=np.array([1000.0, 5e6]), outgoing_storage=np.array([10_000.0, 0.0]))
+    m = model(s)
+    m.kernel("inflow")
+    assert s["river_outflow"][0] == pytest.approx(10.0, rel=1e-6)
+    assert s["river_inflow"][1] == pytest.approx(10.0, rel=1e-6)
+    # outgoing below 1e-8: limit 1
+    s2 = tiny_state([1, 1], river_outflow=np.array([5.0, 0.0], np.float32),
+                    river_storage=np.array([0.0, 0.0]), outgoing_storage=np.array([1e-9, 0.0]))
+    model(s2).kernel("inflow")
+    assert s2["river_outflow"][0] == np.float32(5.0)
+
+
+# ----------------------------------------------------------- flood stage --
+def forward_storage(depth, W, Lr, H, ca, table):
+    """(storage, flooded width) at flood depth `depth`, in float64.
+
+    The water-surface width grows linearly from the channel width W at the
+    bank top to W + k * (catchment width / L) at table depth k, and stays at
+    the full width above the last level; storage is the bankfull channel plus
+    the length times the integral of that width over depth."""
+    L = table.shape[0]
+    wi = ca / Lr / L
+    S = Lr * W * H
+    if depth <= 0:
+        return S, W
+    prevH, prevW = 0.0, W
+    for k in range(L):
+        Hk = table[k]
+        Wk = W + (k + 1) * wi
+        if depth <= Hk:
+            w_at = prevW + (depth - prevH) / (Hk - prevH) * (Wk - prevW)
+            return S + Lr * (depth - prevH) * (prevW + w_at) / 2, w_at
+        S += Lr * 0.5 * (prevW + Wk) * (Hk - prevH)
+        prevH, prevW = Hk, Wk
+    return S + Lr * prevW * (depth - prevH), prevW
+
+
+@pytest.mark.parametrize("depth", [0.0, 0.3, 0.9, 1.7, 3.2, 4.99, 7.5])
+def test_stage_round_trip_on_the_floodplain_profile(depth):
+    """Store the water of a known flood depth; the diagnosis must return it."""
+    W, Lr, H, ca = 80.0, 12_000.0, 3.0, 4e8
+    table = np.cumsum(np.array([0.2, 0.5, 0.1, 0.7, 0.3, 0.6, 0.4, 0.8, 0.5, 0.9])).astype(np.float32)
+    total, width = forward_storage(depth, W, Lr, H, ca, table.astype(np.float64))
+    s = tiny_state([0], river_width=[W], river_length=[Lr], river_height=[H], catchment_area=[ca],
+                   flood_depth_table=table[None, :], river_storage=np.array([total]))
+    model(s).kernel("flood_stage", time_step=1.0)
+    assert s["flood_depth"][0] == pytest.approx(depth, rel=1e-5, abs=1e-5)
+    # river + flood storage conserve the total (FP64)
+    assert s["river_storage"][0] + s["flood_storage"][0] == pytest.approx(total, rel=1e-12)
+    assert s["flood_fraction"][0] == pytest.approx(min((width - W) * Lr / ca, 1.0), rel=1e-4, abs=1e-6)
+    # the channel holds its bank-full volume plus the water above it over the channel width
+    assert s["river_storage"][0] == pytest.approx(Lr * W * (H + depth), rel=1e-5)
+    if depth == 0.0:
+        assert s["flood_fraction"][0] == 0.0
+
+
+def test_stage_dry_depth_is_storage_over_area():
+    s = tiny_state([0], river_storage=np.array([123456.0]))
+    model(s).kernel("flood_stage", time_step=1.0)
+    assert s["river_depth"][0] == pytest.approx(123456.0 / (10_000.0 * 50.0), rel=1e-6)
+    assert s["flood_depth"][0] == 0.0 and s["flood_storage"][0] == 0.0
+
+
+def test_storage_update_global_water_balance():
+    """Sum of new storage = old + dt * (runoff) - dt * (mouth outflow): the
+    reference's own inflow-error check (storage.py:305), no clipping active."""
+    n = 6
+    dn = [1, 2, 5, 4, 5, 5]
+    q = np.array([3.0, 5.0, 2.0, 1.0, 4.0, 6.0], np.float32)
+    inflow = np.zeros(n)
+    for i in range(n):
+        if dn[i] != i:
+            inflow[dn[i]] += q[i]
+    runoff = np.array([0.5, 0.1, 0.2, 0.3, 0.4, 0.6], np.float32)
+    s = tiny_state(dn, river_outflow=q, river_inflow=inflow, runoff=runoff,
+                   river_storage=np.full(n, 1e7))
+    before = s["river_storage"].sum()
+    dt = 100.0
+    model(s).kernel("flood_stage", time_step=dt)
+    after = s["river_storage"].sum() + s["flood_storage"].sum()
+    expect = before + dt * float(runoff.astype(np.float64).sum()) - dt * float(q[5])
+    assert after == pytest.approx(expect, rel=1e-12)
+
+
+# -------------------------------------------------------------------- CFL --
+def test_cfl_closed_form():
+    depth = np.array([0.0, 0.5, 3.0, 12.0], np.float32)
+    dist = np.array([20_000.0, 3_000.0, 15_000.0, 800.0], np.float32)
+    s = tiny_state([1, 2, 3, 3], river_depth=depth, downstream_distance=dist)
+    m = O.OracleModel(s, flags=O.HAS_ADAPTIVE, num_flood_levels=10)
+    T = 1800.0
+    expect = 1
+    for h, dx in zip(depth, dist):
+        dt = 0.7 * float(dx) / math.sqrt(9.8 * max(float(h), 0.01))
+        expect = max(expect, math.floor(T / min(dt, T) - 0.01) + 1)
+    assert m.adaptive_steps(T) == expect
+
+
+# ------------------------------------------------------------ bifurcation --
+def bif_state(P=8, seed=4):
+    rng = np.random.default_rng(seed)
+    n = 12
+    dn = list(range(1, n)) + [n - 1]
+    s = tiny_state(dn, river_depth=rng.uniform(0.5, 3, n).astype(np.float32),
+                   catchment_elevation=np.linspace(30, 0, n).astype(np.float32),
+                   river_storage=rng.uniform(1e5, 1e7, n))
+    ci = rng.integers(0, n, P).astype(np.int32)
+    di = ((ci + 1 + rng.integers(0, n - 1, P)) % n).astype(np.int32)
+    Lb = 5
+    s.update({
+        "water_surface_elevation": np.zeros(n, np.float32), "total_storage": np.zeros(n),
+        "bifurcation_catchment_idx": ci, "bifurcation_downstream_idx": di,
+        "bifurcation_manning": np.full((P, Lb), 0.03, np.float32),
+        "bifurcation_width": rng.uniform(20, 200, (P, Lb)).astype(np.float32),
+        "bifurcation_length": np.full(P, 5000.0, np.float32),
+        "bifurcation_elevation": (np.linspace(30, 0, n)[ci][:, None] - 4.0
+                                  + np.array([0, 0.5, 1, 2, 3])[None, :]).astype(np.float32),
+        "bifurcation_outflow": np.zeros((P, Lb), np.float32),
+        "bifurcation_cross_section_depth": np.zeros((P, Lb), np.float32),
+        "global_bifurcation_outflow": np.zeros(n), "limit_rate": np.ones(n, np.float32),
+    })
+    return s, n, P, Lb
+
+
+def test_bifurcation_net_transfer_sums_to_zero():
+    s, n, P, Lb = bif_state()
+    flags = O.HAS_BIFURCATION | O.HAS_TOTAL_STORAGE | O.HAS_WATER_SURFACE
+    m = O.OracleModel(s, flags=flags, num_flood_levels=10, num_bifurcation_levels=Lb)
+    for _ in range(3):
+        m.kernel("outflow", time_step=300.0)
+        m.kernel("bifurcation_outflow", time_step=300.0)
+        m.kernel("inflow")
+        m.kernel("bifurcation_inflow")
+        assert s["global_bifurcation_outflow"].sum() == 0.0    # antisymmetric pairs
+        m.kernel("flood_stage", time_step=300.0)
+    assert np.abs(s["bifurcation_outflow"]).sum() > 0
+
+
+def test_bifurcation_first_step_closed_form():
+    s, n, P, Lb = bif_state(P=1)
+    flags = O.HAS_BIFURCATION | O.HAS_TOTAL_STORAGE | O.HAS_WATER_SURFACE
+    m = O.OracleModel(s, flags=flags, num_flood_levels=10, num_bifurcation_levels=Lb)
+    dt = 30.0
+    m.kernel("outflow", time_step=dt)
+    wse = s["water_surface_elevation"].astype(np.float64)
+    ci, di = int(s["bifurcation_catchment_idx"][0]), int(s["bifurcation_downstream_idx"][0])
+    slope = np.clip((wse[ci] - wse[di]) / 5000.0, -0.005, 0.005)
+    m.kernel("bifurcation_outflow", time_step=dt)
+    for lev in range(Lb):
+        d = max(max(wse[ci], wse[di]) - float(s["bifurcation_elevation"][0, lev]), 0.0)
+        h = math.sqrt(d * 0.01)  # zero previous depth: the 0.01 floor term
+        q = float(s["bifurcation_width"][0, lev]) * G * dt * h * slope if h > 1e-5 else 0.0
+        lim = min(0.05 * min(s["total_storage"][ci], s["total_storage"][di]) / (abs(q) * dt + 1e-300), 1.0)
+        assert s["bifurcation_cross_section_depth"][0, lev] == pytest.approx(d, rel=1e-5, abs=1e-6)
+        assert abs(s["bifurcation_outflow"][0, lev]) <= abs(q) * (1 + 1e-4) + 1e-6
+        del lim
+
+
+# ------------------------------------------------------------------ levee --
+def levee_state(total, base=1.0, crown=3.0, frac=0.3):
+    s = tiny_state([0], river_storage=np.array([total]))
+    s.update({"is_levee": np.array([True]), "levee_catchment_idx": np.array([0], np.int32),
+              "levee_base_height": np.array([base], np.float32),
+              "levee_crown_height": np.array([crown], np.float32),
+              "levee_fraction": np.array([frac], np.float32)})
+    return s
+
+
+@pytest.mark.parametrize("total", [1e6, 2.5e6, 2e7, 6e7, 2e8, 9e8])
+def test_levee_partition_conserves_the_total(total):
+    s = levee_state(total)
+    m = O.OracleModel(s, flags=O.HAS_LEVEE, num_flood_levels=10)
+    m.kernel("flood_stage", time_step=1.0)
+    before = s["river_storage"][0] + s["flood_storage"][0]
+    rd, fd, ff = s["river_depth"][0], s["flood_depth"][0], s["flood_fraction"][0]
+    m.kernel("levee_stage")
+    after = s["river_storage"][0] + s["flood_storage"][0] + s["protected_storage"][0]
+    assert after == pytest.approx(before, rel=1e-14)
+    assert min(s["river_storage"][0], s["flood_storage"][0], s["protected_storage"][0]) >= 0
+    if before <= 10_000 * 50 * 4.0:   # below bankfull: case 1 leaves the state as the stage left it
+        assert (s["river_depth"][0], s["flood_depth"][0], s["flood_fraction"][0]) == (rd, fd, ff)
+        assert s["protected_storage"][0] == 0.0
+
+
+# -------------------------------------------------------------- reservoir --
+def reservoir_state(total, inflow, cv=4e7, ev=9.5e7, qn=50.0, qf=150.0):
+    n = 3
+    s = tiny_state([1, 2, 2], river_storage=np.array([1e6, total, 1e6]),
+                   runoff=np.zeros(n, np.float32))
+    fv = (ev - cv) / 0.95
+    av = cv + fv * 0.1
+    qa = (qn + qf) * 0.5
+    s.update({
+        "total_storage": np.zeros(n), "reservoir_catchment_idx": np.array([1], np.int32),
+        "conservation_volume": np.array([cv], np.float32), "emergency_volume": np.array([ev], np.float32),
+        "adjustment_volume": np.array([av], np.float32),
+        "effective_normal_outflow": np.array([qn], np.float32),
+        "adjustment_outflow": np.array([qa], np.float32),
+        "flood_control_outflow": np.array([qf], np.float32),
+        "reservoir_total_inflow": np.array([0.0, inflow, 0.0]),
+        "is_reservoir": np.array([False, True, False]), "is_dam_upstream": np.array([True, False, False]),
+        "is_dam_related": np.array([True, True, False]),
+    })
+    return s, dict(cv=cv, ev=ev, av=av, qn=qn, qa=qa, qf=qf)
+
+
+@pytest.mark.parametrize("total,inflow", [(1e7, 10.0), (4.2e7, 10.0), (6e7, 10.0), (6e7, 400.0),
+                                          (9.9e7, 10.0), (9.9e7, 400.0)])
+def test_reservoir_release_regimes(total, inflow):
+    """Piecewise rule of reservoir.py:114-163 evaluated in float64."""
+    s, p = reservoir_state(total, inflow)
+    flags = O.HAS_RESERVOIR | O.HAS_TOTAL_STORAGE
+    m = O.OracleModel(s, flags=flags, num_flood_levels=10)
+    dt = 600.0
+    m.kernel("outflow", time_step=dt)
+    m.kernel("reservoir", time_step=dt)
+    S = total
+    if S <= p["cv"]:
+        q = p["qn"] * math.sqrt(S / p["cv"])
+    elif S <= p["av"]:
+        q = p["qn"] + ((S - p["cv"]) / (p["av"] - p["cv"])) ** 3 * (p["qa"] - p["qn"])
+    elif S <= p["ev"]:
+        tmp = p["qa"] + ((S - p["av"]) / (p["ev"] - p["av"])) ** 0.1 * (p["qf"] - p["qa"])
+        if inflow >= p["qf"]:
+            q = max(p["qn"] + (S - p["cv"]) / (p["ev"] - p["cv"]) * (inflow - p["qn"]), tmp)
+        else:
+            q = tmp
+    else:
+        q = inflow if inflow >= p["qf"] else p["qf"]
+    q = min(max(q, 0.0), S / dt)
+    assert s["river_outflow"][1] == pytest.approx(q, rel=2e-5)
+    assert s["flood_outflow"][1] == 0.0
+    assert s["reservoir_total_inflow"][1] == 0.0           # read and zeroed
+    # outgoing of the dam catchment is exactly its release
+    assert s["outgoing_storage"][1] == pytest.approx(float(s["river_outflow"][1]) * dt, rel=1e-7)
+
+
+def test_reservoir_rule_is_continuous_at_the_thresholds():
+    outs = []
+    for S in (4e7 * (1 - 1e-6), 4e7 * (1 + 1e-6)):
+        s, _ = reservoir_state(S, 10.0)
+        m = O.OracleModel(s, flags=O.HAS_RESERVOIR | O.HAS_TOTAL_STORAGE, num_flood_levels=10)
+        m.kernel("outflow", time_step=600.0)
+        m.kernel("reservoir", time_step=600.0)
+        outs.append(float(s["river_outflow"][1]))
+    assert outs[0] == pytest.approx(outs[1], rel=1e-3)
