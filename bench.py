@@ -1,0 +1,265 @@
+#!/usr/bin/env python3
+"""Benchmark of the HIP hydrodynamic hot path on one MI355X.
+
+Workload (default): the 6-arcmin-scale synthetic network — N = 1,600,000
+catchments, levees on 5 %, 40,000 bifurcation pathways (5 levels), 10 flood
+levels — run with the plain method (one trial), adaptive CFL sub-steps under a
+1800 s outer step, FP32 fluxes and FP64 storage.  One bench "step" is one
+outer step; `value` counts catchment-sub-steps per second (every CFL sub-step
+of every catchment).  Runoff is synthetic (lognormal, mean 1 mm/day) and
+resident in HBM before timing starts.
+
+The same JSON line carries:
+  * batched: the 64-member run of the same network (independent runoff per
+    member, each member with its own CFL sub-step), member-catchment-sub-steps/s,
+    and its ratio to the plain rate (64 sequential plain runs);
+  * roofline: the dominant kernel's algorithmic bytes per launch over its mean
+    duration, timed with HIP events on the launch stream;
+  * cpu_baseline: the CPU oracle (C++/OpenMP restatement of the reference
+    step) on the host cores, on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "cama-flood-gpu_amd"))
+sys.path.insert(0, str(ROOT))
+
+METRIC = "catchment-steps/s on 1 MI355X, plain vs 64-member batched; HBM % of peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# Algorithmic bytes per element of each fused kernel (f32 fluxes, f64
+# storage), counting each field the kernel must read or write once; gathers
+# from the downstream catchment are taken as cache hits.  See DESIGN.md.
+def algorithmic_bytes(kernel: str, model, f_wet: float) -> float:
+    N, T, P, V = model.N, model.T, model.P, model.V
+    lev = V > 0
+    bif = P > 0
+    if kernel.startswith("k_outflow"):
+        per = 84 + 28 + (5 if lev else 0) + (12 if bif else 0) + (4 if lev and bif else 0)
+        return per * N * T
+    if kernel.startswith("k_limiter"):
+        per = 4 + 4 + 4 + 16 + 4 + (8 if bif else 0)
+        return per * N * T
+    if kernel.startswith("k_stage"):
+        per = 12 + 12 + 4 + 24 + 4 + 12 + 48 + 44 * f_wet
+        return per * N * T
+    if kernel.startswith("k_levee"):
+        return (4 + 12 + 16 + 16 + 40 + 4 + 4 + 4 + 24 + 16) * V * T
+    if kernel.startswith("k_bif_outflow"):
+        return (8 + 4 + 8 + 16 + 5 * (4 * 3 + 8 + 8) + 8) * P * T
+    if kernel.startswith("k_bif_inflow"):
+        return (8 + 8 + 5 * 8 + 4) * P * T
+    return 0.0
+
+
+def build_model(net, src, members: int, device: str):
+    from cmfgpu_amd.models import CaMaFlood
+    mods = ["base", "adaptive_time"]
+    if net.num_paths:
+        mods.append("bifurcation")
+    if net.num_levees:
+        mods.append("levee")
+    if members <= 1:
+        return CaMaFlood(src, opened_modules=tuple(mods), device=device, step_duration=1800.0)
+    import numpy as np
+    from cmfgpu_amd import synthetic
+    runoff = np.stack([synthetic.Runoff(src["catchment_area"], 4000 + m).next_day() for m in range(members)])
+    return CaMaFlood(src, opened_modules=tuple(mods), device=device, step_duration=1800.0,
+                     num_trials=members, batched={"runoff": runoff}, per_trial_time_step=True,
+                     trial_layout="inner")
+
+
+def run_steps(model, steps: int, per_trial: bool):
+    """Advance `steps` outer steps; returns member-catchment-sub-steps done."""
+    import torch
+    work = 0
+    for _ in range(steps):
+        n = model.step_advance()
+        if per_trial:
+            counts = model.t["max_sub_steps"].cpu()
+            work += int(counts.sum()) * model.N
+        else:
+            work += n * model.N * model.T
+    torch.cuda.synchronize()
+    return work
+
+
+def kernel_times(model, reps: int):
+    """Mean duration (ms) of each kernel of the fused sub-step, HIP events on the launch stream."""
+    from cmfgpu_amd import _lib
+    cap = 64
+    ms = (ctypes.c_double * cap)()
+    names = (ctypes.c_char_p * cap)()
+    count = ctypes.c_int32(0)
+    acc = {}
+    for r in range(reps):
+        _lib.check(_lib.lib().cmf_hip_substep_timed(model.network_handle, model.field_table_ref(), 0, 0,
+                                                    _lib.current_stream(), ms, names, cap, ctypes.byref(count)),
+                   "cmf_hip_substep_timed")
+        for k in range(count.value):
+            key = names[k].decode()
+            acc.setdefault(key, []).append(ms[k])
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def cpu_baseline(model, net, budget_s: float):
+    """CPU oracle on the GPU model's current state; bounded sample of outer steps."""
+    import numpy as np
+    from cmfgpu_amd.state import assemble
+    from oracle.oracle import OracleModel
+    src = dict(net.fields)
+    src["runoff"] = model.get("runoff")
+    host = assemble(src, model.opened_modules, cold_start=False)
+    state = {k: v.copy() for k, v in host.fields.items()}
+    for name in ("river_storage", "flood_storage", "protected_storage", "protected_depth", "river_depth",
+                 "flood_depth", "river_outflow", "flood_outflow", "river_cross_section_depth",
+                 "flood_cross_section_depth", "flood_cross_section_area", "flood_fraction",
+                 "bifurcation_outflow", "bifurcation_cross_section_depth"):
+        if name in state and name in model.t:
+            state[name] = np.ascontiguousarray(model.get(name), dtype=state[name].dtype)
+    orc = OracleModel(state, flags=host.flags, num_flood_levels=host.num_flood_levels,
+                      num_bifurcation_levels=host.num_bifurcation_levels)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    substeps = 0
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        substeps += int(orc.advance(1800.0, 1)[0])
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": net.num_catchments * substeps / dt, "unit": "catchment-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} outer steps ({substeps} CFL sub-steps) of the same network from the GPU "
+                      f"run's state, CPU oracle (C++ restatement, OpenMP, {threads} threads), {dt:.1f} s"}
+
+
+def read_traffic(kernel: str):
+    path = ROOT / "profiles" / "hbm_traffic.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) >= 2 and not line.startswith("#") and kernel.startswith(parts[0]):
+            try:
+                return float(parts[1])
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=48, help="timed outer steps (1800 s each)")
+    ap.add_argument("--warmup", type=int, default=48, help="untimed outer steps before timing")
+    ap.add_argument("--members", type=int, default=64, help="members of the batched line (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=0, help="timed outer steps of the batched run")
+    ap.add_argument("--scale", type=float, default=1.0, help="network size factor (1.0 = 1.6M catchments)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU basin sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from cmfgpu_amd import synthetic
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a HIP device")
+    device = "cuda:0"
+
+    t0 = time.perf_counter()
+    net = synthetic.build_config(3, scale=args.scale)
+    src = dict(net.fields)
+    src["runoff"] = synthetic.Runoff(src["catchment_area"], 2003).next_day()
+    log(f"network: {net.num_catchments} catchments, {net.num_paths} paths, {net.num_levees} levees "
+        f"({time.perf_counter() - t0:.1f} s)")
+
+    # ---------------------------------------------------------- plain --
+    model = build_model(net, src, 1, device)
+    run_steps(model, args.warmup, per_trial=False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    work = run_steps(model, args.steps, per_trial=False)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    value = work / elapsed
+    substeps = work // model.N
+    log(f"plain: {args.steps} outer steps, {substeps} sub-steps, {elapsed:.3f} s -> {value:.4g} catchment-steps/s")
+
+    f_wet = float((model.t["flood_depth"] > 0).float().mean().item())
+    ktimes = kernel_times(model, args.kernel_reps)
+    dominant = max(ktimes, key=ktimes.get)
+    bytes_dom = algorithmic_bytes(dominant, model, f_wet)
+    achieved = bytes_dom / (ktimes[dominant] * 1e-3) / 1e9
+    substep_ms = sum(ktimes.values())
+    bytes_all = sum(algorithmic_bytes(k, model, f_wet) for k in ktimes)
+    log("kernels (ms): " + ", ".join(f"{k}={v:.4f}" for k, v in sorted(ktimes.items(), key=lambda x: -x[1])))
+    traffic = read_traffic(dominant)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dominant,
+                "algorithmic_bytes": int(bytes_dom), "kernel_ms": round(ktimes[dominant], 5),
+                "substep_kernels_ms": round(substep_ms, 5),
+                "substep_frac": round(bytes_all / (substep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "f_wet": round(f_wet, 4)}
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(model, net, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.4g} catchment-steps/s on {cpu['cores']} threads")
+    model.close()
+    del model
+    torch.cuda.empty_cache()
+
+    # -------------------------------------------------------- batched --
+    batched = None
+    if args.members > 1:
+        M = args.members
+        bm = build_model(net, src, M, device)
+        run_steps(bm, max(args.warmup // 4, 1), per_trial=True)
+        bsteps = args.batched_steps or max(args.steps // 4, 2)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        bwork = run_steps(bm, bsteps, per_trial=True)
+        torch.cuda.synchronize()
+        belapsed = time.perf_counter() - t2
+        bvalue = bwork / belapsed
+        batched = {"members": M, "value": bvalue, "unit": "member-catchment-steps/s", "steps": bsteps,
+                   "vs_plain": round(bvalue / value, 3)}
+        log(f"batched: {M} members, {bsteps} outer steps, {belapsed:.3f} s -> {bvalue:.4g} "
+            f"member-catchment-steps/s ({bvalue / value:.2f}x plain)")
+        bm.close()
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "catchment-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 storage)", "data": "synthetic",
+        "config": {"workload": "6-arcmin synthetic network, plain method, levees + bifurcation, adaptive CFL",
+                   "catchments": net.num_catchments, "bifurcation_paths": net.num_paths,
+                   "levees": net.num_levees, "flood_levels": 10, "outer_step_s": 1800.0,
+                   "substeps_timed": int(substeps), "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -179,29 +179,35 @@ class CaMaFlood:
 
     # ------------------------------------------------------------ layout --
     def _to_catchment(self, name, arr, per_trial=False, replicate=False):
+        """Host array in network order -> device tensor in the device layout.
+
+        replicate: one [N, ...] array copied to every trial (on the device).
+        per_trial without replicate: a [T, N, ...] array of per-trial values.
+        """
         a = np.asarray(arr)
-        if per_trial and self.T > 1:
-            if replicate:
-                a = np.broadcast_to(a, (self.T,) + a.shape)
+        if per_trial and self.T > 1 and not replicate:
             if a.shape[0] != self.T:
                 raise ValueError(f"{name}: expected a leading trial dimension of {self.T}")
-            a = a[:, self.perm]                      # [T, N, ...]
+            x = torch.from_numpy(np.ascontiguousarray(a[:, self.perm])).to(self.device)
+            return x.transpose(0, 1).contiguous() if self.trial_layout == "inner" else x
+        x = torch.from_numpy(np.ascontiguousarray(a[self.perm])).to(self.device)
+        if per_trial and self.T > 1:
             if self.trial_layout == "inner":
-                a = np.swapaxes(a, 0, 1)              # [N, T, ...]
-        else:
-            if a.ndim >= 1 and a.shape[0] == self.T and self.T > 1 and not replicate:
-                raise ValueError(f"{name} has a trial dimension but is not batched")
-            a = a[self.perm]
-        return torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+                return x.unsqueeze(1).expand(x.shape[0], self.T, *x.shape[1:]).contiguous()
+            return x.unsqueeze(0).expand(self.T, *x.shape).contiguous()
+        return x
 
     def _to_path(self, arr, per_trial=False, replicate=False):
         a = np.asarray(arr)
+        if per_trial and self.T > 1 and not replicate:
+            x = torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+            return x.transpose(0, 1).contiguous() if self.trial_layout == "inner" else x
+        x = torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
         if per_trial and self.T > 1:
-            if replicate:
-                a = np.broadcast_to(a, (self.T,) + a.shape)
             if self.trial_layout == "inner":
-                a = np.swapaxes(a, 0, 1)              # [P, T, ...]
-        return torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+                return x.unsqueeze(1).expand(x.shape[0], self.T, *x.shape[1:]).contiguous()
+            return x.unsqueeze(0).expand(self.T, *x.shape).contiguous()
+        return x
 
     def _strides(self):
         if self.T > 1 and self.trial_layout == "inner":

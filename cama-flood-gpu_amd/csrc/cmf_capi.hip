@@ -264,12 +264,36 @@ int need_features(const cmf_fields* f, const char* op) {
     return CMF_OK;
 }
 
+// Kernel timing: while a Timing is installed, every launch is bracketed by
+// a pair of HIP events on its stream (cmf_hip_substep_timed).
+struct Timing {
+    static constexpr int kMax = 32;
+    hipEvent_t ev[2 * kMax];
+    const char* name[kMax];
+    int count = 0;
+};
+thread_local Timing* g_timing = nullptr;
+
+inline void timing_mark(const char* name, hipStream_t s, bool begin) {
+    Timing* t = g_timing;
+    if (!t || t->count >= Timing::kMax) return;
+    if (begin) {
+        t->name[t->count] = name;
+        (void)hipEventRecord(t->ev[2 * t->count], s);
+    } else {
+        (void)hipEventRecord(t->ev[2 * t->count + 1], s);
+        t->count++;
+    }
+}
+
 #define LAUNCH(kernel, count, stream, ...)                                                   \
     do {                                                                                     \
         const int64_t _cnt = (count);                                                        \
         if (_cnt > 0) {                                                                      \
+            timing_mark(#kernel, (hipStream_t)(stream), true);                               \
             hipLaunchKernelGGL(kernel, dim3((unsigned)blocks_for(_cnt)), dim3(kBlock), 0,    \
                                (hipStream_t)(stream), __VA_ARGS__);                          \
+            timing_mark(#kernel, (hipStream_t)(stream), false);                              \
             hipError_t _e = hipGetLastError();                                               \
             if (_e != hipSuccess)                                                            \
                 return set_error(CMF_ERR_HIP, "launch of %s failed: %s", #kernel,            \
@@ -284,8 +308,10 @@ constexpr uint32_t kLogStage = (1u << CMF_LOG_STORAGE_STAGE) | (1u << CMF_LOG_RI
 
 template <class REAL, class STO>
 int log_final(const View<REAL, STO>& v, int64_t nblocks, uint32_t mask, void* stream) {
+    timing_mark("k_log_final", (hipStream_t)stream, true);
     hipLaunchKernelGGL((k_log_final<REAL, STO>), dim3(1), dim3(kBlock), 0, (hipStream_t)stream, v,
                        nblocks, mask);
+    timing_mark("k_log_final", (hipStream_t)stream, false);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(CMF_ERR_HIP, "launch of k_log_final failed: %s", hipGetErrorString(e));
     return CMF_OK;
@@ -649,6 +675,40 @@ int cmf_hip_substep(cmf_network* net, const cmf_fields* f, int32_t substep, int3
     if (f->precision == CMF_PRECISION_F32_F64) return run_substep<float, double>(net, f, substep, store_inflow, s);
     if (f->precision == CMF_PRECISION_F64_F64) return run_substep<double, double>(net, f, substep, store_inflow, s);
     return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
+}
+
+int cmf_hip_substep_timed(cmf_network* net, const cmf_fields* f, int32_t substep, int32_t store_inflow,
+                          void* s, double* out_ms, const char** out_names, int32_t capacity,
+                          int32_t* out_count) {
+    if (!out_ms || !out_count || capacity < 0) return set_error(CMF_ERR_INVALID, "timed substep: bad outputs");
+    Timing t;
+    for (int k = 0; k < 2 * Timing::kMax; ++k) {
+        hipError_t e = hipEventCreate(&t.ev[k]);
+        if (e != hipSuccess) {
+            for (int j = 0; j < k; ++j) (void)hipEventDestroy(t.ev[j]);
+            return set_error(CMF_ERR_HIP, "hipEventCreate failed: %s", hipGetErrorString(e));
+        }
+    }
+    g_timing = &t;
+    int rc = cmf_hip_substep(net, f, substep, store_inflow, s);
+    g_timing = nullptr;
+    if (rc == CMF_OK) {
+        hipError_t e = hipStreamSynchronize((hipStream_t)s);
+        if (e != hipSuccess) rc = set_error(CMF_ERR_HIP, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
+    }
+    int n = 0;
+    if (rc == CMF_OK) {
+        for (int k = 0; k < t.count && k < capacity; ++k) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, t.ev[2 * k], t.ev[2 * k + 1]);
+            out_ms[k] = ms;
+            if (out_names) out_names[k] = t.name[k];
+            ++n;
+        }
+    }
+    for (int k = 0; k < 2 * Timing::kMax; ++k) (void)hipEventDestroy(t.ev[k]);
+    *out_count = n;
+    return rc;
 }
 
 int cmf_hip_advance(cmf_network* net, const cmf_fields* f, double outer_seconds, int32_t fixed_count,

@@ -288,6 +288,13 @@ int cmf_hip_reservoir_outflow(cmf_network* net, const cmf_fields* f, void* strea
  * leaves the last sub-step's inflow there). */
 int cmf_hip_substep(cmf_network* net, const cmf_fields* f, int32_t substep,
                     int32_t store_inflow, void* stream);
+/* cmf_hip_substep with each kernel launch bracketed by HIP events on
+ * `stream`; synchronises, then writes the launch durations (ms) in launch
+ * order to out_ms and the kernel names (static strings) to out_names (may be
+ * NULL).  Instrumentation for measurement; no reference counterpart. */
+int cmf_hip_substep_timed(cmf_network* net, const cmf_fields* f, int32_t substep,
+                          int32_t store_inflow, void* stream, double* out_ms,
+                          const char** out_names, int32_t capacity, int32_t* out_count);
 /* One outer step: CFL (with CMF_HAS_ADAPTIVE_TIME), one host read of the
  * sub-step count, time_step = outer / count, then `count` fused sub-steps.
  * `fixed_count` is used without adaptive time.  The count used is written to
