@@ -110,7 +110,7 @@ def kernel_times(model, reps: int):
                                                     _lib.current_stream(), ms, names, cap, ctypes.byref(count)),
                    "cmf_hip_substep_timed")
         for k in range(count.value):
-            key = names[k].decode()
+            key = names[k].decode().strip("()")
             acc.setdefault(key, []).append(ms[k])
     return {k: sum(v) / len(v) for k, v in acc.items()}
 

@@ -97,7 +97,7 @@ POINTER_FIELDS = (
     "reservoir_catchment_idx", "conservation_volume", "emergency_volume",
     "adjustment_volume", "effective_normal_outflow", "adjustment_outflow",
     "flood_control_outflow", "reservoir_total_inflow", "is_reservoir",
-    "is_dam_upstream", "is_dam_related",
+    "is_dam_upstream", "is_dam_related", "levee_index",
 )
 
 

@@ -148,7 +148,11 @@ class CaMaFlood:
         else:
             ci = di = np.zeros(0, np.int32)
         if self.V:
-            self.t["levee_catchment_idx"] = torch.from_numpy(self.inv[hf["levee_catchment_idx"]].astype(np.int32)).to(self.device)
+            lidx_dev = self.inv[hf["levee_catchment_idx"]].astype(np.int32)
+            self.t["levee_catchment_idx"] = torch.from_numpy(lidx_dev).to(self.device)
+            levee_index = np.full(self.N, -1, np.int32)
+            levee_index[lidx_dev] = np.arange(self.V, dtype=np.int32)
+            self.t["levee_index"] = torch.from_numpy(levee_index).to(self.device)
             for name in LEVEE_PARAMS:
                 src_arr = np.asarray(batched.get(name, hf[name]), dtype=hf[name].dtype)
                 self.t[name] = torch.from_numpy(np.ascontiguousarray(src_arr)).to(self.device)

@@ -188,6 +188,7 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.is_reservoir = f->is_reservoir;
     v.is_dam_up = f->is_dam_upstream;
     v.is_dam_related = f->is_dam_related;
+    v.levee_index = f->levee_index;
     for (int k = 0; k < CMF_LOG_COUNT; ++k) v.log_sums[k] = f->log_sums[k];
     v.up_ptr = net->up_ptr;
     v.up_idx = net->contiguous ? nullptr : net->up_idx;
@@ -367,10 +368,12 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
                 if ((rc = need_features(&g, name))) return rc;
                 if (blocks_for(NT) > net->partial_blocks)
                     return set_error(CMF_ERR_INTERNAL, "%s: partial buffer too small", name);
-                LAUNCH((k_stage<REAL, STO, false, false, true>), NT, stream, v, -1, 0);
+                if (v.L == 10) LAUNCH((k_stage<REAL, STO, false, false, true, 10>), NT, stream, v, -1, 0);
+                else LAUNCH((k_stage<REAL, STO, false, false, true, 0>), NT, stream, v, -1, 0);
                 return log_final(v, blocks_for(NT), kLogAll, stream);
             }
-            LAUNCH((k_stage<REAL, STO, false, false, false>), NT, stream, v, -1, 0);
+            if (v.L == 10) LAUNCH((k_stage<REAL, STO, false, false, false, 10>), NT, stream, v, -1, 0);
+            else LAUNCH((k_stage<REAL, STO, false, false, false, 0>), NT, stream, v, -1, 0);
             return CMF_OK;
         }
         case OP_CFL: {
@@ -419,11 +422,13 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
                 cmf_fields g = *f;
                 g.flags |= CMF_HAS_LOG;
                 if ((rc = need_features(&g, name))) return rc;
-                LAUNCH((k_levee<REAL, STO, true>), v.V * v.T, stream, v, -1);
+                if (v.L == 10) LAUNCH((k_levee<REAL, STO, true, 10>), v.V * v.T, stream, v, -1);
+                else LAUNCH((k_levee<REAL, STO, true, 0>), v.V * v.T, stream, v, -1);
                 if (v.V * v.T > 0) return log_final(v, blocks_for(v.V * v.T), kLogStage, stream);
                 return CMF_OK;
             }
-            LAUNCH((k_levee<REAL, STO, false>), v.V * v.T, stream, v, -1);
+            if (v.L == 10) LAUNCH((k_levee<REAL, STO, false, 10>), v.V * v.T, stream, v, -1);
+            else LAUNCH((k_levee<REAL, STO, false, 0>), v.V * v.T, stream, v, -1);
             return CMF_OK;
         }
         case OP_RESERVOIR:
@@ -465,24 +470,31 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
     if (contig) LAUNCH((k_limiter<REAL, STO, true>), NT, stream, v, substep);
     else LAUNCH((k_limiter<REAL, STO, false>), NT, stream, v, substep);
     if (bif) LAUNCH((k_bif_inflow<REAL, STO>), v.P * v.T, stream, v, substep);
+    // the flood stage (and, with a levee index, the levee stage) in one pass
+#define CMF_STAGE(LOGV, LF)                                                                        \
+    do {                                                                                           \
+        if (contig) LAUNCH((k_stage<REAL, STO, true, true, LOGV, LF>), NT, stream, v, substep, store_inflow); \
+        else LAUNCH((k_stage<REAL, STO, true, false, LOGV, LF>), NT, stream, v, substep, store_inflow);       \
+    } while (0)
     if (log) {
-        if (contig) LAUNCH((k_stage<REAL, STO, true, true, true>), NT, stream, v, substep, store_inflow);
-        else LAUNCH((k_stage<REAL, STO, true, false, true>), NT, stream, v, substep, store_inflow);
+        if (v.L == 10) CMF_STAGE(true, 10); else CMF_STAGE(true, 0);
         int rc = log_final(v, blocks_for(NT), kLogAll, stream);
         if (rc) return rc;
     } else {
-        if (contig) LAUNCH((k_stage<REAL, STO, true, true, false>), NT, stream, v, substep, store_inflow);
-        else LAUNCH((k_stage<REAL, STO, true, false, false>), NT, stream, v, substep, store_inflow);
+        if (v.L == 10) CMF_STAGE(false, 10); else CMF_STAGE(false, 0);
     }
-    if (levee) {
+#undef CMF_STAGE
+    if (levee && !v.levee_index) {
         if (log) {
-            LAUNCH((k_levee<REAL, STO, true>), v.V * v.T, stream, v, substep);
+            if (v.L == 10) LAUNCH((k_levee<REAL, STO, true, 10>), v.V * v.T, stream, v, substep);
+            else LAUNCH((k_levee<REAL, STO, true, 0>), v.V * v.T, stream, v, substep);
             if (v.V * v.T > 0) {
                 int rc = log_final(v, blocks_for(v.V * v.T), kLogStage, stream);
                 if (rc) return rc;
             }
         } else {
-            LAUNCH((k_levee<REAL, STO, false>), v.V * v.T, stream, v, substep);
+            if (v.L == 10) LAUNCH((k_levee<REAL, STO, false, 10>), v.V * v.T, stream, v, substep);
+            else LAUNCH((k_levee<REAL, STO, false, 0>), v.V * v.T, stream, v, substep);
         }
     }
     (void)f;

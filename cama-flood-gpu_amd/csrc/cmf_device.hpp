@@ -130,6 +130,7 @@ struct View {
     STO* gbif;
     REAL* limit_rate;
     const int32_t* levee_idx;
+    const int32_t* levee_index;         // catchment -> levee row, -1 if none (optional)
     const REAL *levee_base, *levee_crown, *levee_frac;
     const int32_t* res_idx;
     const REAL *res_cons, *res_emer, *res_adj, *res_qn, *res_qa, *res_qf;

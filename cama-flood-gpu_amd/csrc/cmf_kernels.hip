@@ -445,11 +445,35 @@ struct LogPart {
     double q[CMF_LOG_COUNT];
 };
 
-// Storage update and flood-stage diagnosis of one catchment.
-template <class REAL, class STO, bool LOG>
-__device__ __forceinline__ void stage_one(const View<REAL, STO>& v, int64_t e, int64_t i,
-                                          REAL rin, REAL fin, REAL rout, REAL fout, REAL gbif,
-                                          REAL time_step, bool non_levee, LogPart& lp) {
+// Floodplain depth table row of one catchment.  With LFIX > 0 (the level
+// count known at compile time) the whole row is loaded into registers up
+// front, one round trip; with LFIX == 0 it is read level by level.
+template <class REAL, int LFIX>
+struct TableRow {
+    const REAL* row;
+    REAL h[LFIX > 0 ? LFIX : 1];
+    __device__ __forceinline__ void load(const REAL* r) {
+        row = r;
+        if (LFIX > 0) {
+#pragma unroll
+            for (int k = 0; k < (LFIX > 0 ? LFIX : 1); ++k) h[k] = r[k];
+        }
+    }
+    __device__ __forceinline__ REAL operator[](int k) const { return LFIX > 0 ? h[k] : row[k]; }
+};
+
+template <class REAL, class STO>
+struct StageOut {
+    STO total_hp, rsf_hp, fsf_hp;
+    REAL rdep, fdep, ffr, ca;
+};
+
+// Storage update and flood-stage diagnosis of one catchment (values only).
+template <class REAL, class STO, int LFIX, bool LOG>
+__device__ __forceinline__ StageOut<REAL, STO> stage_one(
+    const View<REAL, STO>& v, int64_t e, int64_t i, REAL rin, REAL fin, REAL rout, REAL fout, REAL gbif,
+    REAL time_step, bool non_levee, TableRow<REAL, LFIX>& row, bool& row_loaded, LogPart& lp) {
+    StageOut<REAL, STO> o;
     const STO rs = v.river_storage[e], fs = v.flood_storage[e], ps = v.protected_storage[e];
     const REAL ro = v.prm(v.runoff, CMF_BATCHED_RUNOFF, e, i);
     const REAL inflow_extra = (REAL)0.0;
@@ -462,29 +486,31 @@ __device__ __forceinline__ void stage_one(const View<REAL, STO>& v, int64_t e, i
     const STO total_next = rsu + fsu + ps + (STO)(ro * time_step) + (STO)(inflow_extra * time_step);
     const STO total_hp = vmax(total_next, (STO)0.0);
     const REAL total = (REAL)total_hp;
+    o.total_hp = total_hp;
 
     const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, i);
     const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, i);
     const REAL rl = v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, i);
     const REAL river_max_storage = rl * rw * rh;
-    const int L = (int)v.L;
-
-    REAL fdep, ffr, rdep;
-    STO rsf_hp, fsf_hp;
-    REAL ca = (REAL)0;
+    const int L = LFIX > 0 ? LFIX : (int)v.L;
+    o.ca = (REAL)0;
     if (!(total > river_max_storage)) {
         // dry: the table scan leaves level = -1 (storage.py:110,137)
-        rsf_hp = total_hp;
-        fdep = (REAL)0.0;
-        ffr = (REAL)0.0;
-        rdep = (REAL)rsf_hp / (rl * rw);
-        fsf_hp = vmax(total_hp - rsf_hp, (STO)0.0);
-        if (LOG) ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, i);
+        o.rsf_hp = total_hp;
+        o.fdep = (REAL)0.0;
+        o.ffr = (REAL)0.0;
+        o.rdep = (REAL)o.rsf_hp / (rl * rw);
+        o.fsf_hp = vmax(total_hp - o.rsf_hp, (STO)0.0);
+        if (LOG) o.ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, i);
     } else {
-        ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, i);
+        o.ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, i);
+        const REAL ca = o.ca;
+        if (!row_loaded) {
+            row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + i * v.L);
+            row_loaded = true;
+        }
         const REAL catchment_width = ca / rl;
         const REAL width_increment = catchment_width / (REAL)L;
-        const REAL* row = v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + i * v.L;
         int level = 0;
         REAL S_accum = river_max_storage;
         REAL prev_H = (REAL)0.0;
@@ -493,8 +519,10 @@ __device__ __forceinline__ void stage_one(const View<REAL, STO>& v, int64_t e, i
         REAL prev_flood_depth = (REAL)0.0;
         REAL next_flood_depth = (REAL)0.0;
         // S is non-decreasing, so once total <= S no later level is above:
-        // the scan stops there with the values the full scan would leave.
-        for (int k = 0; k < L; ++k) {
+        // stopping there leaves the values the full scan would.
+#pragma unroll
+        for (int k = 0; k < (LFIX > 0 ? LFIX : 64); ++k) {
+            if (LFIX == 0 && k >= L) break;
             const REAL H = row[k];
             const REAL W = rw + (REAL)(k + 1) * width_increment;
             const REAL dS = rl * (REAL)0.5 * (prev_W + W) * (H - prev_H);
@@ -509,48 +537,194 @@ __device__ __forceinline__ void stage_one(const View<REAL, STO>& v, int64_t e, i
             prev_W = W;
         }
         const REAL prev_total_width = rw + (REAL)level * width_increment;
-        REAL diff_width;
         if (level == L) {
-            fdep = prev_flood_depth + (total - prev_total_storage) / (prev_total_width * rl);
-            ffr = (REAL)1.0;
+            o.fdep = prev_flood_depth + (total - prev_total_storage) / (prev_total_width * rl);
+            o.ffr = (REAL)1.0;
         } else {
             const REAL flood_grad = (next_flood_depth - prev_flood_depth) / width_increment;
-            diff_width = dsqrt(prev_total_width * prev_total_width
+            const REAL diff_width = dsqrt(prev_total_width * prev_total_width
                 + (REAL)2.0 * (total - prev_total_storage) / (flood_grad * rl)) - prev_total_width;
-            fdep = prev_flood_depth + diff_width * flood_grad;
-            ffr = vclamp((prev_total_width + diff_width - rw) * rl / ca, (REAL)0.0, (REAL)1.0);
+            o.fdep = prev_flood_depth + diff_width * flood_grad;
+            o.ffr = vclamp((prev_total_width + diff_width - rw) * rl / ca, (REAL)0.0, (REAL)1.0);
         }
-        const REAL cap = river_max_storage + rl * rw * fdep;
-        rsf_hp = vmin((STO)cap, total_hp);
-        rdep = (REAL)rsf_hp / (rl * rw);
-        fsf_hp = vmax(total_hp - rsf_hp, (STO)0.0);
+        const REAL cap = river_max_storage + rl * rw * o.fdep;
+        o.rsf_hp = vmin((STO)cap, total_hp);
+        o.rdep = (REAL)o.rsf_hp / (rl * rw);
+        o.fsf_hp = vmax(total_hp - o.rsf_hp, (STO)0.0);
     }
 
     if (LOG) {
         const STO total_pre = rs + fs + ps;
-        lp.q[CMF_LOG_STORAGE_PRE] = (double)total_pre;
-        lp.q[CMF_LOG_STORAGE_NEXT] = (double)total_next;
-        lp.q[CMF_LOG_STORAGE_NEW] = (double)total_hp;
-        lp.q[CMF_LOG_INFLOW] = (double)((rin + fin + inflow_extra) * time_step);
-        lp.q[CMF_LOG_OUTFLOW] = (double)((rout + fout) * time_step);
-        lp.q[CMF_LOG_INFLOW_ERROR] = (double)(total_pre - total_next
+        lp.q[CMF_LOG_STORAGE_PRE] += (double)total_pre;
+        lp.q[CMF_LOG_STORAGE_NEXT] += (double)total_next;
+        lp.q[CMF_LOG_STORAGE_NEW] += (double)total_hp;
+        lp.q[CMF_LOG_INFLOW] += (double)((rin + fin + inflow_extra) * time_step);
+        lp.q[CMF_LOG_OUTFLOW] += (double)((rout + fout) * time_step);
+        lp.q[CMF_LOG_INFLOW_ERROR] += (double)(total_pre - total_next
             + (STO)((rin + fin + ro + inflow_extra - rout - fout - gbif) * time_step));
         if (non_levee) {
-            const STO stage_new = rsf_hp + fsf_hp;
-            lp.q[CMF_LOG_STORAGE_STAGE] = (double)stage_new;
-            lp.q[CMF_LOG_RIVER_STORAGE] = (double)(REAL)rsf_hp;
-            lp.q[CMF_LOG_FLOOD_STORAGE] = (double)(REAL)fsf_hp;
-            lp.q[CMF_LOG_FLOOD_AREA] = (double)(ffr * ca);
-            lp.q[CMF_LOG_STAGE_ERROR] = (double)(stage_new - total_hp);
+            const STO stage_new = o.rsf_hp + o.fsf_hp;
+            lp.q[CMF_LOG_STORAGE_STAGE] += (double)stage_new;
+            lp.q[CMF_LOG_RIVER_STORAGE] += (double)(REAL)o.rsf_hp;
+            lp.q[CMF_LOG_FLOOD_STORAGE] += (double)(REAL)o.fsf_hp;
+            lp.q[CMF_LOG_FLOOD_AREA] += (double)(o.ffr * o.ca);
+            lp.q[CMF_LOG_STAGE_ERROR] += (double)(stage_new - total_hp);
         }
     }
-    v.river_storage[e] = rsf_hp;
-    v.flood_storage[e] = fsf_hp;
-    v.protected_storage[e] = (STO)0.0;
-    v.river_depth[e] = rdep;
-    v.flood_depth[e] = fdep;
-    v.protected_depth[e] = fdep;
-    v.flood_fraction[e] = ffr;
+    return o;
+}
+
+// ------------------------------------------------------ levee partition --
+template <class REAL, class STO>
+struct LeveeOut {
+    STO r_hp, f_hp, p_hp;
+    REAL r_dph, f_dph, p_dph, f_frc;
+};
+
+// Levee stage of catchment c (levee row lv) from its post-flood-stage state
+// (triton/levee.py:62-304 with the partition of :18-59).
+template <class REAL, class STO, int LFIX>
+__device__ __forceinline__ LeveeOut<REAL, STO> levee_one(
+    const View<REAL, STO>& v, int64_t e, int64_t c, int64_t lv, int64_t le, STO rs_hp, STO fs_hp,
+    REAL fdep_curr, REAL rdep_curr, REAL ffr_curr, const TableRow<REAL, LFIX>& row) {
+    const int L = LFIX > 0 ? LFIX : (int)v.L;
+    const REAL rl = v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, c);
+    const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, c);
+    const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, c);
+    const REAL ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, c);
+    const REAL crown = v.prm(v.levee_crown, CMF_BATCHED_LEVEE_CROWN_HEIGHT, le, lv);
+    const REAL lfrac = v.prm(v.levee_frac, CMF_BATCHED_LEVEE_FRACTION, le, lv);
+    const REAL base = v.prm(v.levee_base, CMF_BATCHED_LEVEE_BASE_HEIGHT, le, lv);
+    const STO total_hp = rs_hp + fs_hp;
+    const REAL rs_c = (REAL)rs_hp, fs_c = (REAL)fs_hp, total = (REAL)total_hp;
+
+    const REAL river_max_storage = rl * rw * rh;
+    const REAL dwth_inc = (ca / rl) / (REAL)L;
+    const REAL levee_distance = lfrac * (ca / rl);
+    REAL s_curr = river_max_storage;
+    REAL dhgt_pre = (REAL)0.0;
+    REAL dwth_pre = rw;
+    REAL base_sto = river_max_storage, fill_sto = river_max_storage;
+    bool found_base = false, found_fill = false;
+    const int ilev = (int)(lfrac * (REAL)L);
+    REAL dsto_fil_B = (REAL)0.0, dwth_fil_B = (REAL)0.0, ddph_fil_B = (REAL)0.0, gradient_B = (REAL)0.0;
+    bool found_B = false;
+#pragma unroll
+    for (int k = 0; k < (LFIX > 0 ? LFIX : 64); ++k) {
+        if (LFIX == 0 && k >= L) break;
+        const REAL depth_val = row[k];
+        REAL dhgt_seg = depth_val - dhgt_pre;
+        dhgt_seg = vmax(dhgt_seg, (REAL)1e-6);
+        const REAL dwth_mid = dwth_pre + (REAL)0.5 * dwth_inc;
+        const REAL dsto_seg = rl * dwth_mid * dhgt_seg;
+        const REAL s_next = s_curr + dsto_seg;
+        const REAL gradient = dhgt_seg / dwth_inc;
+        const bool cond_base = (base > dhgt_pre) && (base <= depth_val);
+        if (cond_base) {
+            const REAL ratio_base = (base - dhgt_pre) / dhgt_seg;
+            base_sto = s_curr + rl * (dwth_pre + (REAL)0.5 * ratio_base * dwth_inc) * (ratio_base * dhgt_seg);
+        }
+        found_base = found_base || cond_base;
+        const bool cond_fill = (crown > dhgt_pre) && (crown <= depth_val);
+        if (cond_fill) {
+            const REAL ratio_fill = (crown - dhgt_pre) / dhgt_seg;
+            fill_sto = s_curr + rl * (dwth_pre + (REAL)0.5 * ratio_fill * dwth_inc) * (ratio_fill * dhgt_seg);
+        }
+        found_fill = found_fill || cond_fill;
+        const bool cond_check = (k >= ilev) && !found_B;
+        if (cond_check) {
+            const REAL wedge = (levee_distance + rw) * (crown - depth_val) * rl;
+            const REAL threshold = s_next + wedge;
+            if (total < threshold) {
+                if (k == ilev) {
+                    const REAL dhgt_dif_loop = crown - base;
+                    dsto_fil_B = base_sto + (levee_distance + rw) * dhgt_dif_loop * rl;
+                }
+                gradient_B = gradient;
+                found_B = true;
+            } else {
+                dsto_fil_B = threshold;
+                dwth_fil_B = dwth_inc * (REAL)(k + 1) - levee_distance;
+                ddph_fil_B = depth_val - base;
+            }
+        }
+        s_curr = s_next;
+        dhgt_pre = depth_val;
+        dwth_pre += dwth_inc;
+    }
+    const REAL s_base_extra = s_curr + rl * dwth_pre * (base - dhgt_pre);
+    base_sto = found_base ? base_sto : ((base > dhgt_pre) ? s_base_extra : river_max_storage);
+    const REAL s_fill_extra = s_curr + rl * dwth_pre * (crown - dhgt_pre);
+    fill_sto = found_fill ? fill_sto : ((crown > dhgt_pre) ? s_fill_extra : river_max_storage);
+    const REAL dhgt_dif = crown - base;
+    const REAL s_top = base_sto + (levee_distance + rw) * dhgt_dif * rl;
+    const bool case4 = total >= fill_sto;
+    const bool case3 = !case4 && (total >= s_top);
+    const bool case2 = !case4 && !case3 && (total >= base_sto);
+
+    LeveeOut<REAL, STO> o;
+    REAL r_cand = rs_c, f_cand = fs_c;
+    o.r_dph = rdep_curr;
+    o.f_dph = fdep_curr;
+    o.p_dph = (REAL)0.0;
+    o.f_frc = ffr_curr;
+    if (case2) {
+        const REAL dsto_add_c2 = total - base_sto;
+        const REAL dwth_add_c2 = levee_distance + rw;
+        const REAL f_dph_c2 = base + dsto_add_c2 / dwth_add_c2 / rl;
+        const REAL r_sto_c2 = river_max_storage + rl * rw * f_dph_c2;
+        r_cand = r_sto_c2;
+        f_cand = vmax(total - r_sto_c2, (REAL)0.0);
+        o.r_dph = r_sto_c2 / rl / rw;
+        o.f_dph = f_dph_c2;
+        o.f_frc = lfrac;
+    } else if (case3) {
+        const REAL dsto_add_B = total - dsto_fil_B;
+        REAL p_dph_B, f_frc_B;
+        if (found_B) {
+            const REAL term_B = dwth_fil_B * dwth_fil_B + (REAL)2.0 * dsto_add_B / rl / (gradient_B + (REAL)1e-9);
+            const REAL dwth_add_B = -dwth_fil_B + dsqrt(vmax(term_B, (REAL)0.0));
+            const REAL ddph_add_B = dwth_add_B * gradient_B;
+            p_dph_B = base + ddph_fil_B + ddph_add_B;
+            f_frc_B = (dwth_fil_B + levee_distance) / (dwth_inc * (REAL)L);
+        } else {
+            const REAL ddph_add_B_extra = dsto_add_B / (dwth_fil_B * rl + (REAL)1e-9);
+            p_dph_B = base + ddph_fil_B + ddph_add_B_extra;
+            f_frc_B = (REAL)1.0;
+        }
+        const REAL r_sto_c3 = river_max_storage + rl * rw * crown;
+        r_cand = r_sto_c3;
+        f_cand = vmax(s_top - r_sto_c3, (REAL)0.0);
+        o.r_dph = r_sto_c3 / rl / rw;
+        o.f_dph = crown;
+        o.p_dph = p_dph_B;
+        o.f_frc = vclamp(f_frc_B, (REAL)0.0, (REAL)1.0);
+    } else if (case4) {
+        const REAL dsto_add_c4 = (fdep_curr - crown) * (levee_distance + rw) * rl;
+        r_cand = rs_c;
+        f_cand = vmax(s_top + dsto_add_c4 - rs_c, (REAL)0.0);
+        o.p_dph = fdep_curr;
+    }
+    const bool none = !(case2 || case3 || case4);
+    STO r_hp = vmin((STO)r_cand, total_hp);
+    r_hp = (case4 || none) ? rs_hp : r_hp;
+    const STO remaining = vmax(total_hp - r_hp, (STO)0.0);
+    STO f_hp = vmin(vmax((STO)f_cand, (STO)0.0), remaining);
+    f_hp = case2 ? remaining : (none ? fs_hp : f_hp);
+    o.r_hp = r_hp;
+    o.f_hp = f_hp;
+    o.p_hp = vmax(remaining - f_hp, (STO)0.0);
+    return o;
+}
+
+template <class REAL, class STO>
+__device__ __forceinline__ void levee_log(const LeveeOut<REAL, STO>& o, STO total_hp, REAL ca, LogPart& lp) {
+    const STO stage_new = o.r_hp + o.f_hp + o.p_hp;
+    lp.q[CMF_LOG_STORAGE_STAGE] += (double)stage_new;
+    lp.q[CMF_LOG_RIVER_STORAGE] += (double)o.r_hp;
+    lp.q[CMF_LOG_FLOOD_STORAGE] += (double)o.f_hp;
+    lp.q[CMF_LOG_FLOOD_AREA] += (double)(o.f_frc * ca);
+    lp.q[CMF_LOG_STAGE_ERROR] += (double)(stage_new - total_hp);
 }
 
 // Block-level FP64 sums of the log partials in a fixed order.
@@ -571,10 +745,12 @@ __device__ __forceinline__ void block_log_partials(LogPart& lp, double* out_bloc
 }
 
 // FUSED: inflow from the limited upstream flows and the bifurcation nets is
-// summed here, the own limited flow is formed here, and `store_inflow`
-// writes the inflow totals back to their fields.  Otherwise the fields hold
-// what compute_outflow / compute_inflow / compute_bifurcation_inflow wrote.
-template <class REAL, class STO, bool FUSED, bool CONTIG, bool LOG>
+// summed here, the own limited flow is formed here, `store_inflow` writes
+// the inflow totals back to their fields, and with a levee index the levee
+// stage of levee catchments runs on the values still in registers.
+// Otherwise the fields hold what compute_outflow / compute_inflow /
+// compute_bifurcation_inflow wrote.
+template <class REAL, class STO, bool FUSED, bool CONTIG, bool LOG, int LFIX>
 __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v, int substep, int store_inflow) {
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     LogPart lp;
@@ -638,8 +814,34 @@ __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v, int substep
                 if (v.has(CMF_HAS_BIFURCATION)) gb = (REAL)v.gbif[e];
                 v.outgoing[e] = (STO)0.0;
             }
-            const bool non_levee = !(v.has(CMF_HAS_LEVEE) && v.is_levee[j]);
-            stage_one<REAL, STO, LOG>(v, e, j, rin, fin, rout, fout, gb, time_step, non_levee, lp);
+            const bool is_lev = v.has(CMF_HAS_LEVEE) && v.is_levee[j];
+            TableRow<REAL, LFIX> row;
+            bool row_loaded = false;
+            const StageOut<REAL, STO> so =
+                stage_one<REAL, STO, LFIX, LOG>(v, e, j, rin, fin, rout, fout, gb, time_step, !is_lev, row, row_loaded, lp);
+            const int32_t lv = (FUSED && is_lev && v.levee_index) ? v.levee_index[j] : -1;
+            if (lv >= 0) {
+                if (!row_loaded)
+                    row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
+                const LeveeOut<REAL, STO> lo = levee_one<REAL, STO, LFIX>(
+                    v, e, j, lv, t * v.V + lv, so.rsf_hp, so.fsf_hp, so.fdep, so.rdep, so.ffr, row);
+                if (LOG) levee_log(lo, so.rsf_hp + so.fsf_hp, v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, j), lp);
+                v.river_storage[e] = lo.r_hp;
+                v.flood_storage[e] = lo.f_hp;
+                v.protected_storage[e] = lo.p_hp;
+                v.river_depth[e] = lo.r_dph;
+                v.flood_depth[e] = lo.f_dph;
+                v.protected_depth[e] = lo.p_dph;
+                v.flood_fraction[e] = lo.f_frc;
+            } else {
+                v.river_storage[e] = so.rsf_hp;
+                v.flood_storage[e] = so.fsf_hp;
+                v.protected_storage[e] = (STO)0.0;
+                v.river_depth[e] = so.rdep;
+                v.flood_depth[e] = so.fdep;
+                v.protected_depth[e] = so.fdep;
+                v.flood_fraction[e] = so.ffr;
+            }
         }
     }
     if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
@@ -668,7 +870,9 @@ __global__ __launch_bounds__(kBlock) void k_log_final(View<REAL, STO> v, int64_t
 }
 
 // ============================================================= levee ====
-template <class REAL, class STO, bool LOG>
+// per-op compute_levee_stage(_log): one thread per levee, on the state the
+// flood stage left.
+template <class REAL, class STO, bool LOG, int LFIX>
 __global__ __launch_bounds__(kBlock) void k_levee(View<REAL, STO> v, int substep) {
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     LogPart lp;
@@ -681,147 +885,20 @@ __global__ __launch_bounds__(kBlock) void k_levee(View<REAL, STO> v, int substep
         if (v.active(t, substep)) {
             const int64_t c = v.levee_idx[lv];
             const int64_t e = v.el(t, c);
-            const int64_t le = t * v.V + lv;
-            const int L = (int)v.L;
-            const REAL rl = v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, c);
-            const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, c);
-            const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, c);
-            const REAL ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, c);
-            const REAL crown = v.prm(v.levee_crown, CMF_BATCHED_LEVEE_CROWN_HEIGHT, le, lv);
-            const REAL lfrac = v.prm(v.levee_frac, CMF_BATCHED_LEVEE_FRACTION, le, lv);
-            const REAL base = v.prm(v.levee_base, CMF_BATCHED_LEVEE_BASE_HEIGHT, le, lv);
+            TableRow<REAL, LFIX> row;
+            row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + c * v.L);
             const STO rs_hp = v.river_storage[e], fs_hp = v.flood_storage[e];
-            const REAL fdep_curr = v.flood_depth[e];
-            const STO total_hp = rs_hp + fs_hp;
-            const REAL rs_c = (REAL)rs_hp, fs_c = (REAL)fs_hp, total = (REAL)total_hp;
-            const REAL* row = v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + c * v.L;
-
-            const REAL river_max_storage = rl * rw * rh;
-            const REAL dwth_inc = (ca / rl) / (REAL)L;
-            const REAL levee_distance = lfrac * (ca / rl);
-            REAL s_curr = river_max_storage;
-            REAL dhgt_pre = (REAL)0.0;
-            REAL dwth_pre = rw;
-            REAL base_sto = river_max_storage, fill_sto = river_max_storage;
-            bool found_base = false, found_fill = false;
-            const int ilev = (int)(lfrac * (REAL)L);
-            REAL dsto_fil_B = (REAL)0.0, dwth_fil_B = (REAL)0.0, ddph_fil_B = (REAL)0.0, gradient_B = (REAL)0.0;
-            bool found_B = false;
-            for (int k = 0; k < L; ++k) {
-                const REAL depth_val = row[k];
-                REAL dhgt_seg = depth_val - dhgt_pre;
-                dhgt_seg = vmax(dhgt_seg, (REAL)1e-6);
-                const REAL dwth_mid = dwth_pre + (REAL)0.5 * dwth_inc;
-                const REAL dsto_seg = rl * dwth_mid * dhgt_seg;
-                const REAL s_next = s_curr + dsto_seg;
-                const REAL gradient = dhgt_seg / dwth_inc;
-                const bool cond_base = (base > dhgt_pre) && (base <= depth_val);
-                if (cond_base) {
-                    const REAL ratio_base = (base - dhgt_pre) / dhgt_seg;
-                    base_sto = s_curr + rl * (dwth_pre + (REAL)0.5 * ratio_base * dwth_inc) * (ratio_base * dhgt_seg);
-                }
-                found_base = found_base || cond_base;
-                const bool cond_fill = (crown > dhgt_pre) && (crown <= depth_val);
-                if (cond_fill) {
-                    const REAL ratio_fill = (crown - dhgt_pre) / dhgt_seg;
-                    fill_sto = s_curr + rl * (dwth_pre + (REAL)0.5 * ratio_fill * dwth_inc) * (ratio_fill * dhgt_seg);
-                }
-                found_fill = found_fill || cond_fill;
-                const bool cond_check = (k >= ilev) && !found_B;
-                if (cond_check) {
-                    const REAL wedge = (levee_distance + rw) * (crown - depth_val) * rl;
-                    const REAL threshold = s_next + wedge;
-                    const bool cond_found = total < threshold;
-                    if (cond_found) {
-                        if (k == ilev) {
-                            const REAL dhgt_dif_loop = crown - base;
-                            dsto_fil_B = base_sto + (levee_distance + rw) * dhgt_dif_loop * rl;
-                        }
-                        gradient_B = gradient;
-                        found_B = true;
-                    } else {
-                        dsto_fil_B = threshold;
-                        dwth_fil_B = dwth_inc * (REAL)(k + 1) - levee_distance;
-                        ddph_fil_B = depth_val - base;
-                    }
-                }
-                s_curr = s_next;
-                dhgt_pre = depth_val;
-                dwth_pre += dwth_inc;
-            }
-            const REAL s_base_extra = s_curr + rl * dwth_pre * (base - dhgt_pre);
-            base_sto = found_base ? base_sto : ((base > dhgt_pre) ? s_base_extra : river_max_storage);
-            const REAL s_fill_extra = s_curr + rl * dwth_pre * (crown - dhgt_pre);
-            fill_sto = found_fill ? fill_sto : ((crown > dhgt_pre) ? s_fill_extra : river_max_storage);
-            const REAL dhgt_dif = crown - base;
-            const REAL s_top = base_sto + (levee_distance + rw) * dhgt_dif * rl;
-            const bool case4 = total >= fill_sto;
-            const bool case3 = !case4 && (total >= s_top);
-            const bool case2 = !case4 && !case3 && (total >= base_sto);
-
-            REAL r_cand = rs_c, f_cand = fs_c, r_dph = v.river_depth[e], f_dph = fdep_curr;
-            REAL p_dph = (REAL)0.0;
-            REAL f_frc = v.flood_fraction[e];
-            if (case2) {
-                const REAL dsto_add_c2 = total - base_sto;
-                const REAL dwth_add_c2 = levee_distance + rw;
-                const REAL f_dph_c2 = base + dsto_add_c2 / dwth_add_c2 / rl;
-                const REAL r_sto_c2 = river_max_storage + rl * rw * f_dph_c2;
-                r_cand = r_sto_c2;
-                f_cand = vmax(total - r_sto_c2, (REAL)0.0);
-                r_dph = r_sto_c2 / rl / rw;
-                f_dph = f_dph_c2;
-                f_frc = lfrac;
-            } else if (case3) {
-                const REAL dsto_add_B = total - dsto_fil_B;
-                REAL p_dph_B, f_frc_B;
-                if (found_B) {
-                    const REAL term_B = dwth_fil_B * dwth_fil_B + (REAL)2.0 * dsto_add_B / rl / (gradient_B + (REAL)1e-9);
-                    const REAL dwth_add_B = -dwth_fil_B + dsqrt(vmax(term_B, (REAL)0.0));
-                    const REAL ddph_add_B = dwth_add_B * gradient_B;
-                    p_dph_B = base + ddph_fil_B + ddph_add_B;
-                    f_frc_B = (dwth_fil_B + levee_distance) / (dwth_inc * (REAL)L);
-                } else {
-                    const REAL ddph_add_B_extra = dsto_add_B / (dwth_fil_B * rl + (REAL)1e-9);
-                    p_dph_B = base + ddph_fil_B + ddph_add_B_extra;
-                    f_frc_B = (REAL)1.0;
-                }
-                const REAL r_sto_c3 = river_max_storage + rl * rw * crown;
-                r_cand = r_sto_c3;
-                f_cand = vmax(s_top - r_sto_c3, (REAL)0.0);
-                r_dph = r_sto_c3 / rl / rw;
-                f_dph = crown;
-                p_dph = p_dph_B;
-                f_frc = vclamp(f_frc_B, (REAL)0.0, (REAL)1.0);
-            } else if (case4) {
-                const REAL dsto_add_c4 = (fdep_curr - crown) * (levee_distance + rw) * rl;
-                r_cand = rs_c;
-                f_cand = vmax(s_top + dsto_add_c4 - rs_c, (REAL)0.0);
-                p_dph = fdep_curr;
-            }
-            const bool none = !(case2 || case3 || case4);
-            STO r_hp = vmin((STO)r_cand, total_hp);
-            r_hp = (case4 || none) ? rs_hp : r_hp;
-            const STO remaining = vmax(total_hp - r_hp, (STO)0.0);
-            STO f_hp = vmin(vmax((STO)f_cand, (STO)0.0), remaining);
-            f_hp = case2 ? remaining : (none ? fs_hp : f_hp);
-            const STO p_hp = vmax(remaining - f_hp, (STO)0.0);
-
-            if (LOG) {
-                const STO stage_new = r_hp + f_hp + p_hp;
-                lp.q[CMF_LOG_STORAGE_STAGE] = (double)stage_new;
-                lp.q[CMF_LOG_RIVER_STORAGE] = (double)r_hp;
-                lp.q[CMF_LOG_FLOOD_STORAGE] = (double)f_hp;
-                lp.q[CMF_LOG_FLOOD_AREA] = (double)(f_frc * ca);
-                lp.q[CMF_LOG_STAGE_ERROR] = (double)(stage_new - total_hp);
-            }
-            v.river_storage[e] = r_hp;
-            v.flood_storage[e] = f_hp;
-            v.protected_storage[e] = p_hp;
-            v.river_depth[e] = r_dph;
-            v.flood_depth[e] = f_dph;
-            v.protected_depth[e] = p_dph;
-            v.flood_fraction[e] = f_frc;
+            const LeveeOut<REAL, STO> lo = levee_one<REAL, STO, LFIX>(
+                v, e, c, lv, t * v.V + lv, rs_hp, fs_hp, v.flood_depth[e], v.river_depth[e],
+                v.flood_fraction[e], row);
+            if (LOG) levee_log(lo, rs_hp + fs_hp, v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, c), lp);
+            v.river_storage[e] = lo.r_hp;
+            v.flood_storage[e] = lo.f_hp;
+            v.protected_storage[e] = lo.p_hp;
+            v.river_depth[e] = lo.r_dph;
+            v.flood_depth[e] = lo.f_dph;
+            v.protected_depth[e] = lo.p_dph;
+            v.flood_fraction[e] = lo.f_frc;
         }
     }
     if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
@@ -894,8 +971,10 @@ __global__ void k_zero_int(int32_t* p, int64_t n) {
     template __global__ void k_inflow_sum<REAL, STO, true>(View<REAL, STO>);                \
     template __global__ void k_limiter<REAL, STO, false>(View<REAL, STO>, int);             \
     template __global__ void k_limiter<REAL, STO, true>(View<REAL, STO>, int);              \
-    template __global__ void k_levee<REAL, STO, false>(View<REAL, STO>, int);               \
-    template __global__ void k_levee<REAL, STO, true>(View<REAL, STO>, int);                \
+    template __global__ void k_levee<REAL, STO, false, 0>(View<REAL, STO>, int);            \
+    template __global__ void k_levee<REAL, STO, true, 0>(View<REAL, STO>, int);             \
+    template __global__ void k_levee<REAL, STO, false, 10>(View<REAL, STO>, int);           \
+    template __global__ void k_levee<REAL, STO, true, 10>(View<REAL, STO>, int);            \
     template __global__ void k_cfl<REAL, STO>(View<REAL, STO>);                             \
     template __global__ void k_log_final<REAL, STO>(View<REAL, STO>, int64_t, uint32_t);    \
     template __global__ void k_set_dt<REAL>(REAL*, const int32_t*, int64_t, double);
@@ -903,14 +982,16 @@ __global__ void k_zero_int(int32_t* p, int64_t n) {
 CMF_INSTANTIATE(float, double)
 CMF_INSTANTIATE(double, double)
 
-#define CMF_INSTANTIATE_STAGE(REAL, STO, FUSED, CONTIG)                                     \
-    template __global__ void k_stage<REAL, STO, FUSED, CONTIG, false>(View<REAL, STO>, int, int); \
-    template __global__ void k_stage<REAL, STO, FUSED, CONTIG, true>(View<REAL, STO>, int, int);
-CMF_INSTANTIATE_STAGE(float, double, false, false)
-CMF_INSTANTIATE_STAGE(float, double, true, false)
-CMF_INSTANTIATE_STAGE(float, double, true, true)
-CMF_INSTANTIATE_STAGE(double, double, false, false)
-CMF_INSTANTIATE_STAGE(double, double, true, false)
-CMF_INSTANTIATE_STAGE(double, double, true, true)
+#define CMF_INSTANTIATE_STAGE(REAL, STO, FUSED, CONTIG, LFIX)                                \
+    template __global__ void k_stage<REAL, STO, FUSED, CONTIG, false, LFIX>(View<REAL, STO>, int, int); \
+    template __global__ void k_stage<REAL, STO, FUSED, CONTIG, true, LFIX>(View<REAL, STO>, int, int);
+#define CMF_INSTANTIATE_STAGE_ALL(REAL, STO, LFIX)     \
+    CMF_INSTANTIATE_STAGE(REAL, STO, false, false, LFIX) \
+    CMF_INSTANTIATE_STAGE(REAL, STO, true, false, LFIX)  \
+    CMF_INSTANTIATE_STAGE(REAL, STO, true, true, LFIX)
+CMF_INSTANTIATE_STAGE_ALL(float, double, 0)
+CMF_INSTANTIATE_STAGE_ALL(float, double, 10)
+CMF_INSTANTIATE_STAGE_ALL(double, double, 0)
+CMF_INSTANTIATE_STAGE_ALL(double, double, 10)
 
 }  // namespace cmf

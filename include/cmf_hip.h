@@ -189,6 +189,10 @@ typedef struct cmf_fields {
     const uint8_t* is_reservoir;     /* bool [N] */
     const uint8_t* is_dam_upstream;  /* bool [N] */
     const uint8_t* is_dam_related;   /* bool [N] */
+    /* levee row of each catchment, -1 where there is none ([N]; optional:
+     * when set, the fused sub-step runs the levee stage inside the flood
+     * stage pass) */
+    const int32_t* levee_index;
     /* log module: CMF_LOG_COUNT FP64 buffers of log_buffer_size entries each,
      * one set per trial (buffer k of trial t at log_sums[k] + t * log_buffer_size
      * * CMF_LOG_COUNT is not used: trials share one set, as LogModule refuses
