@@ -423,19 +423,73 @@ __global__ __launch_bounds__(kBlock) void k_inflow_sum(View<REAL, STO> v) {
     if (res) v.res_inflow[e] = rt;
 }
 
+// ------------------------------------------------- upstream staging ---
+// In the breadth-first layout the upstream catchments of a block's
+// catchments form one contiguous range; the block copies the per-element
+// values it needs from that range into LDS with coalesced loads, and each
+// thread then sums its own segment from LDS in ascending order.  Blocks whose
+// range does not fit read global memory directly (same order, same result).
+constexpr int kStageCap = 1024;
+
+struct UpRange {
+    int64_t first;     // first staged element
+    int64_t count;     // staged elements (0: not staged)
+    int32_t u0;        // first upstream catchment of the block
+};
+
+// Upstream range of this block's elements.  Only for T == 1 or trial-inner
+// layouts, where element (t, u) = u * T + t keeps the range contiguous.
+template <class REAL, class STO>
+__device__ __forceinline__ UpRange block_up_range(const View<REAL, STO>& v, int64_t NT) {
+    UpRange r{0, 0, 0};
+    const int64_t g0 = (int64_t)blockIdx.x * kBlock;
+    const int64_t g1 = min(g0 + kBlock, NT) - 1;
+    if (!(v.T == 1 || v.ts == 1)) return r;
+    const int64_t j0 = g0 / v.T, j1 = g1 / v.T;
+    const int32_t u0 = v.up_ptr[j0], u1 = v.up_ptr[j1 + 1];
+    const int64_t cnt = (int64_t)(u1 - u0) * v.T;
+    if (cnt > kStageCap) return r;
+    r.first = (int64_t)u0 * v.T;
+    r.count = cnt;
+    r.u0 = u0;
+    return r;
+}
+
 // fused: storage limiter of every catchment from its outgoing total
 template <class REAL, class STO, bool CONTIG>
 __global__ __launch_bounds__(kBlock) void k_limiter(View<REAL, STO> v, int substep) {
+    __shared__ REAL s_neg[kStageCap];
+    const int64_t NT = v.N * v.T;
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (g >= v.N * v.T) return;
+    UpRange ur{0, 0, 0};
+    if (CONTIG) {
+        ur = block_up_range(v, NT);
+        for (int64_t k = threadIdx.x; k < ur.count; k += kBlock) s_neg[k] = v.neg_term[ur.first + k];
+        __syncthreads();
+    }
+    if (g >= NT) return;
     int64_t t, j;
     v.split_c(g, t, j);
     if (!v.active(t, substep)) return;
     const int64_t e = v.el(t, j);
-    STO og = outgoing_terms<REAL, STO, CONTIG>(v, t, j);
+    const REAL pos = v.pos_term[e];
+    const STO rate_sto = v.river_storage[e] + v.flood_storage[e];
+    const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
+    STO og = (STO)0;
+    if (CONTIG && ur.count > 0) {
+        for (int32_t k = b; k < en; ++k) og += (STO)s_neg[(int64_t)(k - ur.u0) * v.T + t];
+    } else {
+        for (int32_t k = b; k < en; ++k) {
+            const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+            og += (STO)v.neg_term[v.el(t, u)];
+        }
+    }
+    og += (STO)pos;
     if (v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[j] != v.ev_ptr[j + 1])
         og = bif_event_sum<REAL, STO>(v, t, j, og, true);
-    const REAL l = limit_of(v, og, e);
+    const REAL ogr = (REAL)og;
+    const REAL rate = (REAL)rate_sto;
+    const REAL l = (ogr > (REAL)1e-8) ? vmin(rate / ogr, (REAL)1.0) : (REAL)1.0;
     v.lim[e] = l;
     if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e] = l;
 }
@@ -745,65 +799,80 @@ __device__ __forceinline__ void block_log_partials(LogPart& lp, double* out_bloc
 }
 
 // FUSED: inflow from the limited upstream flows and the bifurcation nets is
-// summed here, the own limited flow is formed here, `store_inflow` writes
-// the inflow totals back to their fields, and with a levee index the levee
-// stage of levee catchments runs on the values still in registers.
-// Otherwise the fields hold what compute_outflow / compute_inflow /
+// summed here (upstream values staged in LDS), the own limited flow is
+// formed here, `store_inflow` writes the inflow totals back to their fields,
+// and with a levee index the levee stage of levee catchments runs on the
+// values still in registers.  All stores come after all loads.  Otherwise
+// (per-op) the fields hold what compute_outflow / compute_inflow /
 // compute_bifurcation_inflow wrote.
 template <class REAL, class STO, bool FUSED, bool CONTIG, bool LOG, int LFIX>
 __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v, int substep, int store_inflow) {
+    __shared__ REAL s_qr[FUSED && CONTIG ? kStageCap : 1];
+    __shared__ REAL s_qf[FUSED && CONTIG ? kStageCap : 1];
+    __shared__ REAL s_lim[FUSED && CONTIG ? kStageCap : 1];
+    const int64_t NT = v.N * v.T;
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     LogPart lp;
     if (LOG) {
 #pragma unroll
         for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
     }
-    if (g < v.N * v.T) {
+    UpRange ur{0, 0, 0};
+    if (FUSED && CONTIG) {
+        ur = block_up_range(v, NT);
+        for (int64_t k = threadIdx.x; k < ur.count; k += kBlock) {
+            s_qr[k] = v.qraw_r[ur.first + k];
+            s_qf[k] = v.qraw_f[ur.first + k];
+            s_lim[k] = v.lim[ur.first + k];
+        }
+        __syncthreads();
+    }
+    if (g < NT) {
         int64_t t, j;
         v.split_c(g, t, j);
         if (v.active(t, substep)) {
             const int64_t e = v.el(t, j);
             const REAL time_step = v.dt(t);
             REAL rin, fin, rout, fout, gb = (REAL)0.0;
+            STO ri = (STO)0, fi = (STO)0, rt = (STO)0, gsum = (STO)0;
+            const bool res = FUSED && v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
+            bool store = false;
             if (FUSED) {
                 const REAL l_own = v.lim[e];
                 const REAL l_dn = v.lim[v.el(t, v.dn[j])];
                 const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
                 rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
                 fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
-                v.river_outflow[e] = rout;
-                v.flood_outflow[e] = fout;
-                // upstream u has downstream j: negative flow is limited by j
-                const bool res = v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
-                STO ri = (STO)0, fi = (STO)0;
-                STO rt = res ? v.res_inflow[e] : (STO)0;
+                // upstream u has downstream j: its negative flow is limited by j
+                if (res) rt = v.res_inflow[e];
                 const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
                 for (int32_t k = b; k < en; ++k) {
-                    const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
-                    const int64_t eu = v.el(t, u);
-                    const REAL lu = v.lim[eu];
-                    const REAL ur0 = v.qraw_r[eu], uf0 = v.qraw_f[eu];
-                    const REAL ur = (ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own;
-                    const REAL uf = (uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own;
-                    ri += (STO)ur;
-                    fi += (STO)uf;
-                    if (res) rt += (STO)(ur + uf);
+                    REAL lu, ur0, uf0;
+                    if (CONTIG && ur.count > 0) {
+                        const int64_t s_idx = (int64_t)(k - ur.u0) * v.T + t;
+                        lu = s_lim[s_idx];
+                        ur0 = s_qr[s_idx];
+                        uf0 = s_qf[s_idx];
+                    } else {
+                        const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+                        const int64_t eu = v.el(t, u);
+                        lu = v.lim[eu];
+                        ur0 = v.qraw_r[eu];
+                        uf0 = v.qraw_f[eu];
+                    }
+                    const REAL ur1 = (ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own;
+                    const REAL uf1 = (uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own;
+                    ri += (STO)ur1;
+                    fi += (STO)uf1;
+                    if (res) rt += (STO)(ur1 + uf1);
                 }
-                if (res) v.res_inflow[e] = rt;
-                STO gsum = (STO)0;
                 if (v.has(CMF_HAS_BIFURCATION)) {
                     if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_event_sum<REAL, STO>(v, t, j, gsum, false);
                     gb = (REAL)gsum;
                 }
                 // per-trial counts: each trial stores on its own last sub-step
-                const bool store = (v.has(CMF_PER_TRIAL_DT) && substep >= 0)
-                                       ? (substep == v.max_sub_steps[t] - 1)
-                                       : (store_inflow != 0);
-                if (store) {
-                    v.river_inflow[e] = ri;
-                    v.flood_inflow[e] = fi;
-                    if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = gsum;
-                }
+                store = (v.has(CMF_PER_TRIAL_DT) && substep >= 0) ? (substep == v.max_sub_steps[t] - 1)
+                                                                 : (store_inflow != 0);
                 rin = (REAL)ri;
                 fin = (REAL)fi;
             } else {
@@ -812,7 +881,6 @@ __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v, int substep
                 rout = v.river_outflow[e];
                 fout = v.flood_outflow[e];
                 if (v.has(CMF_HAS_BIFURCATION)) gb = (REAL)v.gbif[e];
-                v.outgoing[e] = (STO)0.0;
             }
             const bool is_lev = v.has(CMF_HAS_LEVEE) && v.is_levee[j];
             TableRow<REAL, LFIX> row;
@@ -820,28 +888,42 @@ __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v, int substep
             const StageOut<REAL, STO> so =
                 stage_one<REAL, STO, LFIX, LOG>(v, e, j, rin, fin, rout, fout, gb, time_step, !is_lev, row, row_loaded, lp);
             const int32_t lv = (FUSED && is_lev && v.levee_index) ? v.levee_index[j] : -1;
+            STO o_rs = so.rsf_hp, o_fs = so.fsf_hp, o_ps = (STO)0.0;
+            REAL o_rd = so.rdep, o_fd = so.fdep, o_pd = so.fdep, o_ff = so.ffr;
             if (lv >= 0) {
                 if (!row_loaded)
                     row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
                 const LeveeOut<REAL, STO> lo = levee_one<REAL, STO, LFIX>(
                     v, e, j, lv, t * v.V + lv, so.rsf_hp, so.fsf_hp, so.fdep, so.rdep, so.ffr, row);
                 if (LOG) levee_log(lo, so.rsf_hp + so.fsf_hp, v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, j), lp);
-                v.river_storage[e] = lo.r_hp;
-                v.flood_storage[e] = lo.f_hp;
-                v.protected_storage[e] = lo.p_hp;
-                v.river_depth[e] = lo.r_dph;
-                v.flood_depth[e] = lo.f_dph;
-                v.protected_depth[e] = lo.p_dph;
-                v.flood_fraction[e] = lo.f_frc;
-            } else {
-                v.river_storage[e] = so.rsf_hp;
-                v.flood_storage[e] = so.fsf_hp;
-                v.protected_storage[e] = (STO)0.0;
-                v.river_depth[e] = so.rdep;
-                v.flood_depth[e] = so.fdep;
-                v.protected_depth[e] = so.fdep;
-                v.flood_fraction[e] = so.ffr;
+                o_rs = lo.r_hp;
+                o_fs = lo.f_hp;
+                o_ps = lo.p_hp;
+                o_rd = lo.r_dph;
+                o_fd = lo.f_dph;
+                o_pd = lo.p_dph;
+                o_ff = lo.f_frc;
             }
+            // ---- stores
+            if (FUSED) {
+                v.river_outflow[e] = rout;
+                v.flood_outflow[e] = fout;
+                if (res) v.res_inflow[e] = rt;
+                if (store) {
+                    v.river_inflow[e] = ri;
+                    v.flood_inflow[e] = fi;
+                    if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = gsum;
+                }
+            } else {
+                v.outgoing[e] = (STO)0.0;
+            }
+            v.river_storage[e] = o_rs;
+            v.flood_storage[e] = o_fs;
+            v.protected_storage[e] = o_ps;
+            v.river_depth[e] = o_rd;
+            v.flood_depth[e] = o_fd;
+            v.protected_depth[e] = o_pd;
+            v.flood_fraction[e] = o_ff;
         }
     }
     if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
