@@ -172,6 +172,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="network size factor (1.0 = 1.6M catchments)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="run a plain device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -185,6 +187,16 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a HIP device")
     device = "cuda:0"
+
+    if args.calibrate_copy_mb > 0:
+        n = args.calibrate_copy_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device=device)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mb} MiB read + {args.calibrate_copy_mb} MiB written, x3")
+        del a, b
 
     t0 = time.perf_counter()
     net = synthetic.build_config(3, scale=args.scale)

@@ -41,6 +41,8 @@ struct cmf_network {
     int32_t* up_idx = nullptr;
     int32_t* ev_ptr = nullptr;
     int32_t* ev_code = nullptr;
+    int32_t* touched = nullptr;       // catchments that end a bifurcation path
+    int64_t n_touched = 0;
     void* pos_term = nullptr;
     void* neg_term = nullptr;
     void* qraw_r = nullptr;
@@ -58,7 +60,7 @@ namespace {
 
 void free_network(cmf_network* net) {
     if (!net) return;
-    void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->pos_term,
+    void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->touched, net->pos_term,
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
                     net->path_neg, net->path_net, net->partials};
     for (void* p : ptrs)
@@ -368,12 +370,12 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
                 if ((rc = need_features(&g, name))) return rc;
                 if (blocks_for(NT) > net->partial_blocks)
                     return set_error(CMF_ERR_INTERNAL, "%s: partial buffer too small", name);
-                if (v.L == 10) LAUNCH((k_stage<REAL, STO, false, false, true, 10>), NT, stream, v, -1, 0);
-                else LAUNCH((k_stage<REAL, STO, false, false, true, 0>), NT, stream, v, -1, 0);
+                if (v.L == 10) LAUNCH((k_stage<REAL, STO, true, 10>), NT, stream, v);
+                else LAUNCH((k_stage<REAL, STO, true, 0>), NT, stream, v);
                 return log_final(v, blocks_for(NT), kLogAll, stream);
             }
-            if (v.L == 10) LAUNCH((k_stage<REAL, STO, false, false, false, 10>), NT, stream, v, -1, 0);
-            else LAUNCH((k_stage<REAL, STO, false, false, false, 0>), NT, stream, v, -1, 0);
+            if (v.L == 10) LAUNCH((k_stage<REAL, STO, false, 10>), NT, stream, v);
+            else LAUNCH((k_stage<REAL, STO, false, 0>), NT, stream, v);
             return CMF_OK;
         }
         case OP_CFL: {
@@ -409,7 +411,7 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
             if (!(f->flags & CMF_HAS_BIFURCATION))
                 return set_error(CMF_ERR_INVALID, "compute_bifurcation_inflow: bifurcation is off");
             if ((rc = need_features(f, "compute_bifurcation_inflow"))) return rc;
-            LAUNCH((k_bif_inflow<REAL, STO>), v.P * v.T, stream, v, -1);
+            LAUNCH((k_bif_inflow<REAL, STO, false>), v.P * v.T, stream, v, -1);
             LAUNCH((k_gbif_sum<REAL, STO>), NT, stream, v);
             return CMF_OK;
         case OP_LEVEE:
@@ -454,6 +456,24 @@ int dispatch_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
 }
 
 // ------------------------------------------------------------- fused ----
+template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
+int launch_route(const View<REAL, STO>& v, int substep, int store_inflow, void* stream) {
+    LAUNCH((k_route<REAL, STO, CONTIG, LOG, LFIX, LEV>), v.N * v.T, stream, v, substep, store_inflow);
+    return CMF_OK;
+}
+
+template <class REAL, class STO, bool CONTIG, bool LOG>
+int dispatch_route(const View<REAL, STO>& v, bool lev, int substep, int store_inflow, void* stream) {
+    if (v.L == 10) {
+        return lev ? launch_route<REAL, STO, CONTIG, LOG, 10, true>(v, substep, store_inflow, stream)
+                   : launch_route<REAL, STO, CONTIG, LOG, 10, false>(v, substep, store_inflow, stream);
+    }
+    return lev ? launch_route<REAL, STO, CONTIG, LOG, 0, true>(v, substep, store_inflow, stream)
+               : launch_route<REAL, STO, CONTIG, LOG, 0, false>(v, substep, store_inflow, stream);
+}
+
+// One fused sub-step: outflow, [reservoir], [bifurcation outflow, end
+// limiters, bifurcation inflow], route (limiters, inflow, stage [, levee]).
 template <class REAL, class STO>
 int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int substep,
                   int store_inflow, void* stream) {
@@ -461,35 +481,32 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
     const bool contig = net->contiguous != 0;
     const bool bif = v.has(CMF_HAS_BIFURCATION), levee = v.has(CMF_HAS_LEVEE);
     const bool log = v.has(CMF_HAS_LOG);
+    const bool lev_fused = levee && v.levee_index != nullptr;
     LAUNCH((k_outflow<REAL, STO, true>), NT, stream, v, substep);
     if (v.has(CMF_HAS_RESERVOIR)) LAUNCH((k_reservoir<REAL, STO, true>), v.R * v.T, stream, v, substep);
     if (bif) {
         if (levee) LAUNCH((k_bif_outflow<REAL, STO, true>), v.P * v.T, stream, v, substep);
         else LAUNCH((k_bif_outflow<REAL, STO, false>), v.P * v.T, stream, v, substep);
+        if (contig) LAUNCH((k_lim_touched<REAL, STO, true>), net->n_touched * v.T, stream, v, net->touched, net->n_touched, substep);
+        else LAUNCH((k_lim_touched<REAL, STO, false>), net->n_touched * v.T, stream, v, net->touched, net->n_touched, substep);
+        LAUNCH((k_bif_inflow<REAL, STO, true>), v.P * v.T, stream, v, substep);
     }
-    if (contig) LAUNCH((k_limiter<REAL, STO, true>), NT, stream, v, substep);
-    else LAUNCH((k_limiter<REAL, STO, false>), NT, stream, v, substep);
-    if (bif) LAUNCH((k_bif_inflow<REAL, STO>), v.P * v.T, stream, v, substep);
-    // the flood stage (and, with a levee index, the levee stage) in one pass
-#define CMF_STAGE(LOGV, LF)                                                                        \
-    do {                                                                                           \
-        if (contig) LAUNCH((k_stage<REAL, STO, true, true, LOGV, LF>), NT, stream, v, substep, store_inflow); \
-        else LAUNCH((k_stage<REAL, STO, true, false, LOGV, LF>), NT, stream, v, substep, store_inflow);       \
-    } while (0)
+    int rc;
+    if (log) rc = contig ? dispatch_route<REAL, STO, true, true>(v, lev_fused, substep, store_inflow, stream)
+                         : dispatch_route<REAL, STO, false, true>(v, lev_fused, substep, store_inflow, stream);
+    else rc = contig ? dispatch_route<REAL, STO, true, false>(v, lev_fused, substep, store_inflow, stream)
+                     : dispatch_route<REAL, STO, false, false>(v, lev_fused, substep, store_inflow, stream);
+    if (rc) return rc;
     if (log) {
-        if (v.L == 10) CMF_STAGE(true, 10); else CMF_STAGE(true, 0);
-        int rc = log_final(v, blocks_for(NT), kLogAll, stream);
+        rc = log_final(v, blocks_for(NT), kLogAll, stream);
         if (rc) return rc;
-    } else {
-        if (v.L == 10) CMF_STAGE(false, 10); else CMF_STAGE(false, 0);
     }
-#undef CMF_STAGE
-    if (levee && !v.levee_index) {
+    if (levee && !lev_fused) {
         if (log) {
             if (v.L == 10) LAUNCH((k_levee<REAL, STO, true, 10>), v.V * v.T, stream, v, substep);
             else LAUNCH((k_levee<REAL, STO, true, 0>), v.V * v.T, stream, v, substep);
             if (v.V * v.T > 0) {
-                int rc = log_final(v, blocks_for(v.V * v.T), kLogStage, stream);
+                rc = log_final(v, blocks_for(v.V * v.T), kLogStage, stream);
                 if (rc) return rc;
             }
         } else {
@@ -633,7 +650,11 @@ int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_pat
             ev_code[fill[bif_di[p]]++] = (int32_t)(2 * p + 1);
         }
     }
+    std::vector<int32_t> touched;
+    for (int64_t i = 0; i < n; ++i)
+        if (ev_ptr[i + 1] > ev_ptr[i]) touched.push_back((int32_t)i);
     cmf_network* net = new cmf_network();
+    net->n_touched = (int64_t)touched.size();
     net->n = n;
     net->T = num_trials;
     net->P = num_paths;
@@ -646,6 +667,7 @@ int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_pat
         (rc = upload((void**)&net->up_idx, up_idx.data(), up_idx.size() * 4)) ||
         (rc = upload((void**)&net->ev_ptr, ev_ptr.data(), ev_ptr.size() * 4)) ||
         (rc = upload((void**)&net->ev_code, ev_code.data(), ev_code.size() * 4)) ||
+        (rc = upload((void**)&net->touched, touched.data(), touched.size() * 4)) ||
         (rc = dev_alloc(&net->pos_term, el)) || (rc = dev_alloc(&net->neg_term, el)) ||
         (rc = dev_alloc(&net->qraw_r, el)) || (rc = dev_alloc(&net->qraw_f, el)) ||
         (rc = dev_alloc(&net->lim, el)) || (rc = dev_alloc(&net->path_pos, pel)) ||

@@ -25,6 +25,19 @@ namespace cmf {
 
 constexpr int kBlock = 256;
 
+// Store `val` unless the location already holds it bit for bit (saves the
+// write of values that did not change, such as the zero flood state of dry
+// catchments).
+__device__ __forceinline__ unsigned int bits_of(float x) { return __float_as_uint(x); }
+__device__ __forceinline__ unsigned long long bits_of(double x) {
+    return (unsigned long long)__double_as_longlong(x);
+}
+template <class X>
+__device__ __forceinline__ void store_if_changed(X* p, X old, X val) {
+    if (bits_of(old) != bits_of(val)) *p = val;
+}
+
+
 // =========================================================== outflow ====
 // FUSED: the unlimited flows go to the qraw scratch (the limiter and the
 // inflow pass read them); inflows are not zeroed because the fused stage
@@ -140,13 +153,15 @@ __global__ __launch_bounds__(kBlock) void k_outflow(View<REAL, STO> v, int subst
     if (v.has(CMF_HAS_WATER_SURFACE)) v.wse[e] = wse;
     if (v.has(CMF_HAS_PROTECTED_WATER_SURFACE)) v.pwse[e] = pwse;
     v.rcsd[e] = upd_rcsd;
-    v.fcsd[e] = upd_fcsd;
-    v.fcsa[e] = upd_fcsa;
+    store_if_changed(v.fcsd + e, f_csd, upd_fcsd);
+    store_if_changed(v.fcsa + e, f_csa, upd_fcsa);
     if (v.has(CMF_HAS_TOTAL_STORAGE)) v.total_storage[e] = (STO)total_storage;
-    const REAL pos = vmax(q_r, (REAL)0.0) + vmax(q_f, (REAL)0.0);
-    const REAL neg = vmin(q_r, (REAL)0.0) + vmin(q_f, (REAL)0.0);
-    v.pos_term[e] = pos * time_step;
-    v.neg_term[e] = mouth ? (REAL)0.0 : -neg * time_step;
+    if (!FUSED) {
+        const REAL pos = vmax(q_r, (REAL)0.0) + vmax(q_f, (REAL)0.0);
+        const REAL neg = vmin(q_r, (REAL)0.0) + vmin(q_f, (REAL)0.0);
+        v.pos_term[e] = pos * time_step;
+        v.neg_term[e] = mouth ? (REAL)0.0 : -neg * time_step;
+    }
 }
 
 // Outgoing total of catchment j from the catchment terms: upstream negative
@@ -225,10 +240,12 @@ __global__ __launch_bounds__(kBlock) void k_reservoir(View<REAL, STO> v, int sub
         v.flood_outflow[e] = (REAL)0.0;
     }
     v.total_storage[e] = (STO)total;
-    const REAL new_pos = vmax(out, (REAL)0.0);
-    const REAL new_neg = vmin(out, (REAL)0.0);
-    v.pos_term[e] = new_pos * time_step;
-    v.neg_term[e] = mouth ? (REAL)0.0 : -(new_neg * time_step);
+    if (!FUSED) {
+        const REAL new_pos = vmax(out, (REAL)0.0);
+        const REAL new_neg = vmin(out, (REAL)0.0);
+        v.pos_term[e] = new_pos * time_step;
+        v.neg_term[e] = mouth ? (REAL)0.0 : -(new_neg * time_step);
+    }
 }
 
 // per-op: re-sum the outgoing totals touched by a reservoir (its catchment
@@ -340,7 +357,9 @@ __global__ __launch_bounds__(kBlock) void k_bif_outgoing(View<REAL, STO> v) {
     v.outgoing[e] = bif_event_sum<REAL, STO>(v, t, j, v.outgoing[e], true);
 }
 
-template <class REAL, class STO>
+// FUSED reads the end catchments' limiters from the limiter scratch
+// (k_lim_touched); per-op from the limit_rate field compute_inflow wrote.
+template <class REAL, class STO, bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_bif_inflow(View<REAL, STO> v, int substep) {
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (g >= v.P * v.T) return;
@@ -349,7 +368,8 @@ __global__ __launch_bounds__(kBlock) void k_bif_inflow(View<REAL, STO> v, int su
     if (!v.active(t, substep)) return;
     const int64_t pe = v.pel(t, p);
     const int64_t ci = v.bif_ci[p], di = v.bif_di[p];
-    const REAL l0 = v.limit_rate[v.el(t, ci)], l1 = v.limit_rate[v.el(t, di)];
+    const REAL* lr = FUSED ? v.lim : v.limit_rate;
+    const REAL l0 = lr[v.el(t, ci)], l1 = lr[v.el(t, di)];
     REAL raw_sum = (REAL)0;
     for (int64_t l = 0; l < v.LB; ++l) {
         const int64_t k = pe * v.LB + l;
@@ -423,75 +443,54 @@ __global__ __launch_bounds__(kBlock) void k_inflow_sum(View<REAL, STO> v) {
     if (res) v.res_inflow[e] = rt;
 }
 
-// ------------------------------------------------- upstream staging ---
-// In the breadth-first layout the upstream catchments of a block's
-// catchments form one contiguous range; the block copies the per-element
-// values it needs from that range into LDS with coalesced loads, and each
-// thread then sums its own segment from LDS in ascending order.  Blocks whose
-// range does not fit read global memory directly (same order, same result).
-constexpr int kStageCap = 1024;
-
-struct UpRange {
-    int64_t first;     // first staged element
-    int64_t count;     // staged elements (0: not staged)
-    int32_t u0;        // first upstream catchment of the block
-};
-
-// Upstream range of this block's elements.  Only for T == 1 or trial-inner
-// layouts, where element (t, u) = u * T + t keeps the range contiguous.
-template <class REAL, class STO>
-__device__ __forceinline__ UpRange block_up_range(const View<REAL, STO>& v, int64_t NT) {
-    UpRange r{0, 0, 0};
-    const int64_t g0 = (int64_t)blockIdx.x * kBlock;
-    const int64_t g1 = min(g0 + kBlock, NT) - 1;
-    if (!(v.T == 1 || v.ts == 1)) return r;
-    const int64_t j0 = g0 / v.T, j1 = g1 / v.T;
-    const int32_t u0 = v.up_ptr[j0], u1 = v.up_ptr[j1 + 1];
-    const int64_t cnt = (int64_t)(u1 - u0) * v.T;
-    if (cnt > kStageCap) return r;
-    r.first = (int64_t)u0 * v.T;
-    r.count = cnt;
-    r.u0 = u0;
-    return r;
+// ------------------------------------------- limits from raw flows ----
+// The fused schedule keeps only the unlimited flows of the outflow pass and
+// recomputes each storage limiter where it is needed, from the same terms the
+// reference accumulates into outgoing_storage (outflow.py:306-314): the
+// negative-flow terms of the upstream catchments (ascending network index),
+// the catchment's own positive term, then its bifurcation path terms.
+template <class REAL>
+__device__ __forceinline__ REAL pos_term_of(REAL qr, REAL qf, REAL dt) {
+    return (vmax(qr, (REAL)0.0) + vmax(qf, (REAL)0.0)) * dt;
+}
+// negative-flow term of a catchment that has a downstream (not a mouth)
+template <class REAL>
+__device__ __forceinline__ REAL neg_term_of(REAL qr, REAL qf, REAL dt) {
+    return -(vmin(qr, (REAL)0.0) + vmin(qf, (REAL)0.0)) * dt;
 }
 
-// fused: storage limiter of every catchment from its outgoing total
+// Storage limiter of catchment x of trial t (compute_inflow, outflow.py:346-352).
 template <class REAL, class STO, bool CONTIG>
-__global__ __launch_bounds__(kBlock) void k_limiter(View<REAL, STO> v, int substep) {
-    __shared__ REAL s_neg[kStageCap];
-    const int64_t NT = v.N * v.T;
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    UpRange ur{0, 0, 0};
-    if (CONTIG) {
-        ur = block_up_range(v, NT);
-        for (int64_t k = threadIdx.x; k < ur.count; k += kBlock) s_neg[k] = v.neg_term[ur.first + k];
-        __syncthreads();
-    }
-    if (g >= NT) return;
-    int64_t t, j;
-    v.split_c(g, t, j);
-    if (!v.active(t, substep)) return;
-    const int64_t e = v.el(t, j);
-    const REAL pos = v.pos_term[e];
-    const STO rate_sto = v.river_storage[e] + v.flood_storage[e];
-    const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
+__device__ __forceinline__ REAL limit_recomputed(const View<REAL, STO>& v, int64_t t, int64_t x, REAL dt,
+                                                 REAL qr_x, REAL qf_x, STO rate_sto) {
     STO og = (STO)0;
-    if (CONTIG && ur.count > 0) {
-        for (int32_t k = b; k < en; ++k) og += (STO)s_neg[(int64_t)(k - ur.u0) * v.T + t];
-    } else {
-        for (int32_t k = b; k < en; ++k) {
-            const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
-            og += (STO)v.neg_term[v.el(t, u)];
-        }
+    const int32_t b = v.up_ptr[x], en = v.up_ptr[x + 1];
+    for (int32_t k = b; k < en; ++k) {
+        const int64_t w = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+        const int64_t ew = v.el(t, w);
+        og += (STO)neg_term_of(v.qraw_r[ew], v.qraw_f[ew], dt);
     }
-    og += (STO)pos;
-    if (v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[j] != v.ev_ptr[j + 1])
-        og = bif_event_sum<REAL, STO>(v, t, j, og, true);
+    og += (STO)pos_term_of(qr_x, qf_x, dt);
+    if (v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[x] != v.ev_ptr[x + 1])
+        og = bif_event_sum<REAL, STO>(v, t, x, og, true);
     const REAL ogr = (REAL)og;
     const REAL rate = (REAL)rate_sto;
-    const REAL l = (ogr > (REAL)1e-8) ? vmin(rate / ogr, (REAL)1.0) : (REAL)1.0;
-    v.lim[e] = l;
-    if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e] = l;
+    return (ogr > (REAL)1e-8) ? vmin(rate / ogr, (REAL)1.0) : (REAL)1.0;
+}
+
+// fused, with bifurcation: limiters of the path end catchments, for the
+// path nets of k_bif_inflow.
+template <class REAL, class STO, bool CONTIG>
+__global__ __launch_bounds__(kBlock) void k_lim_touched(View<REAL, STO> v, const int32_t* touched,
+                                                        int64_t n_touched, int substep) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= n_touched * v.T) return;
+    const int64_t t = g / n_touched, x = touched[g - t * n_touched];
+    if (!v.active(t, substep)) return;
+    const int64_t e = v.el(t, x);
+    const REAL dt = v.dt(t);
+    v.lim[e] = limit_recomputed<REAL, STO, CONTIG>(v, t, x, dt, v.qraw_r[e], v.qraw_f[e],
+                                                   v.river_storage[e] + v.flood_storage[e]);
 }
 
 // ======================================================= flood stage ====
@@ -798,18 +797,55 @@ __device__ __forceinline__ void block_log_partials(LogPart& lp, double* out_bloc
     }
 }
 
-// FUSED: inflow from the limited upstream flows and the bifurcation nets is
-// summed here (upstream values staged in LDS), the own limited flow is
-// formed here, `store_inflow` writes the inflow totals back to their fields,
-// and with a levee index the levee stage of levee catchments runs on the
-// values still in registers.  All stores come after all loads.  Otherwise
-// (per-op) the fields hold what compute_outflow / compute_inflow /
-// compute_bifurcation_inflow wrote.
-template <class REAL, class STO, bool FUSED, bool CONTIG, bool LOG, int LFIX>
-__global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v, int substep, int store_inflow) {
-    __shared__ REAL s_qr[FUSED && CONTIG ? kStageCap : 1];
-    __shared__ REAL s_qf[FUSED && CONTIG ? kStageCap : 1];
-    __shared__ REAL s_lim[FUSED && CONTIG ? kStageCap : 1];
+// Store the post-stage state of one catchment (and its levee stage when LEV).
+template <class REAL, class STO>
+__device__ __forceinline__ void store_state(const View<REAL, STO>& v, int64_t e, const StageOut<REAL, STO>& so,
+                                            STO fs_old, STO ps_old) {
+    v.river_storage[e] = so.rsf_hp;
+    store_if_changed(v.flood_storage + e, fs_old, so.fsf_hp);
+    store_if_changed(v.protected_storage + e, ps_old, (STO)0.0);
+    v.river_depth[e] = so.rdep;
+    v.flood_depth[e] = so.fdep;
+    v.protected_depth[e] = so.fdep;
+    v.flood_fraction[e] = so.ffr;
+}
+
+// per-op compute_flood_stage(_log): storage update and flood stage from
+// the inflow fields (storage.py:16-193, :196-425).
+template <class REAL, class STO, bool LOG, int LFIX>
+__global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    LogPart lp;
+    if (LOG) {
+#pragma unroll
+        for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
+    }
+    if (g < v.N * v.T) {
+        int64_t t, j;
+        v.split_c(g, t, j);
+        const int64_t e = v.el(t, j);
+        const REAL rin = (REAL)v.river_inflow[e];
+        const REAL fin = (REAL)v.flood_inflow[e];
+        const REAL gb = v.has(CMF_HAS_BIFURCATION) ? (REAL)v.gbif[e] : (REAL)0.0;
+        const bool is_lev = v.has(CMF_HAS_LEVEE) && v.is_levee[j];
+        TableRow<REAL, LFIX> row;
+        bool row_loaded = false;
+        const STO fs_old = v.flood_storage[e], ps_old = v.protected_storage[e];
+        const StageOut<REAL, STO> so = stage_one<REAL, STO, LFIX, LOG>(
+            v, e, j, rin, fin, v.river_outflow[e], v.flood_outflow[e], gb, v.dt(t), !is_lev, row, row_loaded, lp);
+        v.outgoing[e] = (STO)0.0;
+        store_state(v, e, so, fs_old, ps_old);
+    }
+    if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
+}
+
+// fused: storage limiters (recomputed), limited outflow, inflow sums,
+// bifurcation totals, storage update and flood stage, and with LEV the
+// levee stage of levee catchments, in one pass.  All stores follow all loads.
+// `store_inflow` writes river/flood inflow and the bifurcation total back
+// (the reference leaves the last sub-step's values there).
+template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
+__global__ __launch_bounds__(kBlock) void k_route(View<REAL, STO> v, int substep, int store_inflow) {
     const int64_t NT = v.N * v.T;
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     LogPart lp;
@@ -817,113 +853,88 @@ __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v, int substep
 #pragma unroll
         for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
     }
-    UpRange ur{0, 0, 0};
-    if (FUSED && CONTIG) {
-        ur = block_up_range(v, NT);
-        for (int64_t k = threadIdx.x; k < ur.count; k += kBlock) {
-            s_qr[k] = v.qraw_r[ur.first + k];
-            s_qf[k] = v.qraw_f[ur.first + k];
-            s_lim[k] = v.lim[ur.first + k];
-        }
-        __syncthreads();
-    }
     if (g < NT) {
         int64_t t, j;
         v.split_c(g, t, j);
         if (v.active(t, substep)) {
             const int64_t e = v.el(t, j);
-            const REAL time_step = v.dt(t);
-            REAL rin, fin, rout, fout, gb = (REAL)0.0;
-            STO ri = (STO)0, fi = (STO)0, rt = (STO)0, gsum = (STO)0;
-            const bool res = FUSED && v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
-            bool store = false;
-            if (FUSED) {
-                const REAL l_own = v.lim[e];
-                const REAL l_dn = v.lim[v.el(t, v.dn[j])];
-                const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
-                rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
-                fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
-                // upstream u has downstream j: its negative flow is limited by j
-                if (res) rt = v.res_inflow[e];
-                const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
-                for (int32_t k = b; k < en; ++k) {
-                    REAL lu, ur0, uf0;
-                    if (CONTIG && ur.count > 0) {
-                        const int64_t s_idx = (int64_t)(k - ur.u0) * v.T + t;
-                        lu = s_lim[s_idx];
-                        ur0 = s_qr[s_idx];
-                        uf0 = s_qf[s_idx];
-                    } else {
-                        const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
-                        const int64_t eu = v.el(t, u);
-                        lu = v.lim[eu];
-                        ur0 = v.qraw_r[eu];
-                        uf0 = v.qraw_f[eu];
-                    }
-                    const REAL ur1 = (ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own;
-                    const REAL uf1 = (uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own;
-                    ri += (STO)ur1;
-                    fi += (STO)uf1;
-                    if (res) rt += (STO)(ur1 + uf1);
-                }
-                if (v.has(CMF_HAS_BIFURCATION)) {
-                    if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_event_sum<REAL, STO>(v, t, j, gsum, false);
-                    gb = (REAL)gsum;
-                }
-                // per-trial counts: each trial stores on its own last sub-step
-                store = (v.has(CMF_PER_TRIAL_DT) && substep >= 0) ? (substep == v.max_sub_steps[t] - 1)
-                                                                 : (store_inflow != 0);
-                rin = (REAL)ri;
-                fin = (REAL)fi;
-            } else {
-                rin = (REAL)v.river_inflow[e];
-                fin = (REAL)v.flood_inflow[e];
-                rout = v.river_outflow[e];
-                fout = v.flood_outflow[e];
-                if (v.has(CMF_HAS_BIFURCATION)) gb = (REAL)v.gbif[e];
+            const REAL dt = v.dt(t);
+            const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
+            const STO rs_old = v.river_storage[e], fs_old = v.flood_storage[e];
+            const REAL l_own = limit_recomputed<REAL, STO, CONTIG>(v, t, j, dt, qr, qf, rs_old + fs_old);
+            // upstream u has downstream j: positive flow limited by u, negative by j
+            const bool res = v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
+            STO ri = (STO)0, fi = (STO)0;
+            STO rt = res ? v.res_inflow[e] : (STO)0;
+            const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
+            for (int32_t k = b; k < en; ++k) {
+                const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+                const int64_t eu = v.el(t, u);
+                const REAL ur0 = v.qraw_r[eu], uf0 = v.qraw_f[eu];
+                REAL lu = (REAL)1.0;
+                if (ur0 >= (REAL)0.0 || uf0 >= (REAL)0.0)
+                    lu = limit_recomputed<REAL, STO, CONTIG>(v, t, u, dt, ur0, uf0,
+                                                             v.river_storage[eu] + v.flood_storage[eu]);
+                const REAL ur1 = (ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own;
+                const REAL uf1 = (uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own;
+                ri += (STO)ur1;
+                fi += (STO)uf1;
+                if (res) rt += (STO)(ur1 + uf1);
             }
-            const bool is_lev = v.has(CMF_HAS_LEVEE) && v.is_levee[j];
+            // own limited flow: negative flow is limited by the downstream catchment
+            REAL l_dn = l_own;
+            if (qr < (REAL)0.0 || qf < (REAL)0.0) {
+                const int64_t d = v.dn[j];
+                if (d != j) {
+                    const int64_t ed = v.el(t, d);
+                    l_dn = limit_recomputed<REAL, STO, CONTIG>(v, t, d, dt, v.qraw_r[ed], v.qraw_f[ed],
+                                                               v.river_storage[ed] + v.flood_storage[ed]);
+                }
+            }
+            const REAL rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
+            const REAL fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
+            STO gsum = (STO)0;
+            REAL gb = (REAL)0.0;
+            if (v.has(CMF_HAS_BIFURCATION)) {
+                if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_event_sum<REAL, STO>(v, t, j, gsum, false);
+                gb = (REAL)gsum;
+            }
+            // per-trial counts: each trial stores on its own last sub-step
+            const bool store = (v.has(CMF_PER_TRIAL_DT) && substep >= 0) ? (substep == v.max_sub_steps[t] - 1)
+                                                                         : (store_inflow != 0);
+            const bool is_lev = LEV && v.is_levee[j];
             TableRow<REAL, LFIX> row;
             bool row_loaded = false;
-            const StageOut<REAL, STO> so =
-                stage_one<REAL, STO, LFIX, LOG>(v, e, j, rin, fin, rout, fout, gb, time_step, !is_lev, row, row_loaded, lp);
-            const int32_t lv = (FUSED && is_lev && v.levee_index) ? v.levee_index[j] : -1;
-            STO o_rs = so.rsf_hp, o_fs = so.fsf_hp, o_ps = (STO)0.0;
-            REAL o_rd = so.rdep, o_fd = so.fdep, o_pd = so.fdep, o_ff = so.ffr;
-            if (lv >= 0) {
-                if (!row_loaded)
-                    row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
+            const STO ps_old = v.protected_storage[e];
+            const StageOut<REAL, STO> so = stage_one<REAL, STO, LFIX, LOG>(
+                v, e, j, (REAL)ri, (REAL)fi, rout, fout, gb, dt, !(v.has(CMF_HAS_LEVEE) && v.is_levee[j]),
+                row, row_loaded, lp);
+            // ---- stores
+            v.river_outflow[e] = rout;
+            v.flood_outflow[e] = fout;
+            if (res) v.res_inflow[e] = rt;
+            if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e] = l_own;
+            if (store) {
+                v.river_inflow[e] = ri;
+                v.flood_inflow[e] = fi;
+                if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = gsum;
+            }
+            if (is_lev) {
+                const int32_t lv = v.levee_index[j];
+                if (!row_loaded) row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
                 const LeveeOut<REAL, STO> lo = levee_one<REAL, STO, LFIX>(
                     v, e, j, lv, t * v.V + lv, so.rsf_hp, so.fsf_hp, so.fdep, so.rdep, so.ffr, row);
                 if (LOG) levee_log(lo, so.rsf_hp + so.fsf_hp, v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, j), lp);
-                o_rs = lo.r_hp;
-                o_fs = lo.f_hp;
-                o_ps = lo.p_hp;
-                o_rd = lo.r_dph;
-                o_fd = lo.f_dph;
-                o_pd = lo.p_dph;
-                o_ff = lo.f_frc;
-            }
-            // ---- stores
-            if (FUSED) {
-                v.river_outflow[e] = rout;
-                v.flood_outflow[e] = fout;
-                if (res) v.res_inflow[e] = rt;
-                if (store) {
-                    v.river_inflow[e] = ri;
-                    v.flood_inflow[e] = fi;
-                    if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = gsum;
-                }
+                v.river_storage[e] = lo.r_hp;
+                v.flood_storage[e] = lo.f_hp;
+                v.protected_storage[e] = lo.p_hp;
+                v.river_depth[e] = lo.r_dph;
+                v.flood_depth[e] = lo.f_dph;
+                v.protected_depth[e] = lo.p_dph;
+                v.flood_fraction[e] = lo.f_frc;
             } else {
-                v.outgoing[e] = (STO)0.0;
+                store_state(v, e, so, fs_old, ps_old);
             }
-            v.river_storage[e] = o_rs;
-            v.flood_storage[e] = o_fs;
-            v.protected_storage[e] = o_ps;
-            v.river_depth[e] = o_rd;
-            v.flood_depth[e] = o_fd;
-            v.protected_depth[e] = o_pd;
-            v.flood_fraction[e] = o_ff;
         }
     }
     if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
@@ -1034,46 +1045,6 @@ __global__ void k_zero_int(int32_t* p, int64_t n) {
 }
 
 // ============================================= explicit instantiation ===
-#define CMF_INSTANTIATE(REAL, STO)                                                           \
-    template __global__ void k_outflow<REAL, STO, false>(View<REAL, STO>, int);             \
-    template __global__ void k_outflow<REAL, STO, true>(View<REAL, STO>, int);              \
-    template __global__ void k_outgoing_sum<REAL, STO, false>(View<REAL, STO>);             \
-    template __global__ void k_outgoing_sum<REAL, STO, true>(View<REAL, STO>);              \
-    template __global__ void k_reservoir<REAL, STO, false>(View<REAL, STO>, int);           \
-    template __global__ void k_reservoir<REAL, STO, true>(View<REAL, STO>, int);            \
-    template __global__ void k_res_regather<REAL, STO, false>(View<REAL, STO>);             \
-    template __global__ void k_res_regather<REAL, STO, true>(View<REAL, STO>);              \
-    template __global__ void k_bif_outflow<REAL, STO, false>(View<REAL, STO>, int);         \
-    template __global__ void k_bif_outflow<REAL, STO, true>(View<REAL, STO>, int);          \
-    template __global__ void k_bif_outgoing<REAL, STO>(View<REAL, STO>);                    \
-    template __global__ void k_bif_inflow<REAL, STO>(View<REAL, STO>, int);                 \
-    template __global__ void k_gbif_sum<REAL, STO>(View<REAL, STO>);                        \
-    template __global__ void k_inflow_limit<REAL, STO>(View<REAL, STO>);                    \
-    template __global__ void k_inflow_sum<REAL, STO, false>(View<REAL, STO>);               \
-    template __global__ void k_inflow_sum<REAL, STO, true>(View<REAL, STO>);                \
-    template __global__ void k_limiter<REAL, STO, false>(View<REAL, STO>, int);             \
-    template __global__ void k_limiter<REAL, STO, true>(View<REAL, STO>, int);              \
-    template __global__ void k_levee<REAL, STO, false, 0>(View<REAL, STO>, int);            \
-    template __global__ void k_levee<REAL, STO, true, 0>(View<REAL, STO>, int);             \
-    template __global__ void k_levee<REAL, STO, false, 10>(View<REAL, STO>, int);           \
-    template __global__ void k_levee<REAL, STO, true, 10>(View<REAL, STO>, int);            \
-    template __global__ void k_cfl<REAL, STO>(View<REAL, STO>);                             \
-    template __global__ void k_log_final<REAL, STO>(View<REAL, STO>, int64_t, uint32_t);    \
-    template __global__ void k_set_dt<REAL>(REAL*, const int32_t*, int64_t, double);
-
-CMF_INSTANTIATE(float, double)
-CMF_INSTANTIATE(double, double)
-
-#define CMF_INSTANTIATE_STAGE(REAL, STO, FUSED, CONTIG, LFIX)                                \
-    template __global__ void k_stage<REAL, STO, FUSED, CONTIG, false, LFIX>(View<REAL, STO>, int, int); \
-    template __global__ void k_stage<REAL, STO, FUSED, CONTIG, true, LFIX>(View<REAL, STO>, int, int);
-#define CMF_INSTANTIATE_STAGE_ALL(REAL, STO, LFIX)     \
-    CMF_INSTANTIATE_STAGE(REAL, STO, false, false, LFIX) \
-    CMF_INSTANTIATE_STAGE(REAL, STO, true, false, LFIX)  \
-    CMF_INSTANTIATE_STAGE(REAL, STO, true, true, LFIX)
-CMF_INSTANTIATE_STAGE_ALL(float, double, 0)
-CMF_INSTANTIATE_STAGE_ALL(float, double, 10)
-CMF_INSTANTIATE_STAGE_ALL(double, double, 0)
-CMF_INSTANTIATE_STAGE_ALL(double, double, 10)
+// (the C ABI in cmf_capi.hip includes this file and instantiates what it launches)
 
 }  // namespace cmf
