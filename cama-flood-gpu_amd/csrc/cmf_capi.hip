@@ -337,7 +337,7 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
         case OP_OUTFLOW:
             if ((rc = need_base(f, "compute_outflow")) || (rc = need_features(f, "compute_outflow"))) return rc;
             if (!f->outgoing_storage) return set_error(CMF_ERR_INVALID, "compute_outflow: outgoing_storage is NULL");
-            LAUNCH((k_outflow<REAL, STO, false>), NT, stream, v, -1);
+            LAUNCH((k_outflow<REAL, STO, false, true>), NT, stream, v, -1);
             if (contig) LAUNCH((k_outgoing_sum<REAL, STO, true>), NT, stream, v);
             else LAUNCH((k_outgoing_sum<REAL, STO, false>), NT, stream, v);
             return CMF_OK;
@@ -456,20 +456,27 @@ int dispatch_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
 }
 
 // ------------------------------------------------------------- fused ----
+// MERGED: limiters recomputed inside the route pass (k_route), used for the
+// trial-inner batched layout; otherwise k_limiter then k_stage_fused.
 template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
-int launch_route(const View<REAL, STO>& v, int substep, int store_inflow, void* stream) {
-    LAUNCH((k_route<REAL, STO, CONTIG, LOG, LFIX, LEV>), v.N * v.T, stream, v, substep, store_inflow);
+int launch_route(const View<REAL, STO>& v, bool merged, int substep, int store_inflow, void* stream) {
+    if (merged) {
+        LAUNCH((k_route<REAL, STO, CONTIG, LOG, LFIX, LEV>), v.N * v.T, stream, v, substep, store_inflow);
+    } else {
+        LAUNCH((k_limiter<REAL, STO, CONTIG>), v.N * v.T, stream, v, substep);
+        LAUNCH((k_stage_fused<REAL, STO, CONTIG, LOG, LFIX, LEV>), v.N * v.T, stream, v, substep, store_inflow);
+    }
     return CMF_OK;
 }
 
 template <class REAL, class STO, bool CONTIG, bool LOG>
-int dispatch_route(const View<REAL, STO>& v, bool lev, int substep, int store_inflow, void* stream) {
+int dispatch_route(const View<REAL, STO>& v, bool merged, bool lev, int substep, int store_inflow, void* stream) {
     if (v.L == 10) {
-        return lev ? launch_route<REAL, STO, CONTIG, LOG, 10, true>(v, substep, store_inflow, stream)
-                   : launch_route<REAL, STO, CONTIG, LOG, 10, false>(v, substep, store_inflow, stream);
+        return lev ? launch_route<REAL, STO, CONTIG, LOG, 10, true>(v, merged, substep, store_inflow, stream)
+                   : launch_route<REAL, STO, CONTIG, LOG, 10, false>(v, merged, substep, store_inflow, stream);
     }
-    return lev ? launch_route<REAL, STO, CONTIG, LOG, 0, true>(v, substep, store_inflow, stream)
-               : launch_route<REAL, STO, CONTIG, LOG, 0, false>(v, substep, store_inflow, stream);
+    return lev ? launch_route<REAL, STO, CONTIG, LOG, 0, true>(v, merged, substep, store_inflow, stream)
+               : launch_route<REAL, STO, CONTIG, LOG, 0, false>(v, merged, substep, store_inflow, stream);
 }
 
 // One fused sub-step: outflow, [reservoir], [bifurcation outflow, end
@@ -482,20 +489,49 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
     const bool bif = v.has(CMF_HAS_BIFURCATION), levee = v.has(CMF_HAS_LEVEE);
     const bool log = v.has(CMF_HAS_LOG);
     const bool lev_fused = levee && v.levee_index != nullptr;
-    LAUNCH((k_outflow<REAL, STO, true>), NT, stream, v, substep);
+    // trials adjacent in memory: the merged route pass (uniform loops per wave)
+    const bool merged = v.T > 1 && v.ts == 1;
+    if (merged) LAUNCH((k_outflow<REAL, STO, true, false>), NT, stream, v, substep);
+    else LAUNCH((k_outflow<REAL, STO, true, true>), NT, stream, v, substep);
     if (v.has(CMF_HAS_RESERVOIR)) LAUNCH((k_reservoir<REAL, STO, true>), v.R * v.T, stream, v, substep);
+    int rc;
     if (bif) {
         if (levee) LAUNCH((k_bif_outflow<REAL, STO, true>), v.P * v.T, stream, v, substep);
         else LAUNCH((k_bif_outflow<REAL, STO, false>), v.P * v.T, stream, v, substep);
-        if (contig) LAUNCH((k_lim_touched<REAL, STO, true>), net->n_touched * v.T, stream, v, net->touched, net->n_touched, substep);
-        else LAUNCH((k_lim_touched<REAL, STO, false>), net->n_touched * v.T, stream, v, net->touched, net->n_touched, substep);
-        LAUNCH((k_bif_inflow<REAL, STO, true>), v.P * v.T, stream, v, substep);
+        if (merged) {
+            if (contig) LAUNCH((k_lim_touched<REAL, STO, true>), net->n_touched * v.T, stream, v, net->touched, net->n_touched, substep);
+            else LAUNCH((k_lim_touched<REAL, STO, false>), net->n_touched * v.T, stream, v, net->touched, net->n_touched, substep);
+            LAUNCH((k_bif_inflow<REAL, STO, true>), v.P * v.T, stream, v, substep);
+            rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(v, true, lev_fused, substep, store_inflow, stream)
+                               : dispatch_route<REAL, STO, false, true>(v, true, lev_fused, substep, store_inflow, stream))
+                     : (contig ? dispatch_route<REAL, STO, true, false>(v, true, lev_fused, substep, store_inflow, stream)
+                               : dispatch_route<REAL, STO, false, false>(v, true, lev_fused, substep, store_inflow, stream));
+        } else {
+            // the path nets need every end limiter: full limiter pass first
+            if (contig) LAUNCH((k_limiter<REAL, STO, true>), NT, stream, v, substep);
+            else LAUNCH((k_limiter<REAL, STO, false>), NT, stream, v, substep);
+            LAUNCH((k_bif_inflow<REAL, STO, true>), v.P * v.T, stream, v, substep);
+            rc = CMF_OK;
+#define CMF_SF(C, LG, LF, LV) LAUNCH((k_stage_fused<REAL, STO, C, LG, LF, LV>), NT, stream, v, substep, store_inflow)
+            if (contig) {
+                if (log) { if (v.L == 10) { if (lev_fused) CMF_SF(true, true, 10, true); else CMF_SF(true, true, 10, false); }
+                           else { if (lev_fused) CMF_SF(true, true, 0, true); else CMF_SF(true, true, 0, false); } }
+                else { if (v.L == 10) { if (lev_fused) CMF_SF(true, false, 10, true); else CMF_SF(true, false, 10, false); }
+                       else { if (lev_fused) CMF_SF(true, false, 0, true); else CMF_SF(true, false, 0, false); } }
+            } else {
+                if (log) { if (v.L == 10) { if (lev_fused) CMF_SF(false, true, 10, true); else CMF_SF(false, true, 10, false); }
+                           else { if (lev_fused) CMF_SF(false, true, 0, true); else CMF_SF(false, true, 0, false); } }
+                else { if (v.L == 10) { if (lev_fused) CMF_SF(false, false, 10, true); else CMF_SF(false, false, 10, false); }
+                       else { if (lev_fused) CMF_SF(false, false, 0, true); else CMF_SF(false, false, 0, false); } }
+            }
+#undef CMF_SF
+        }
+    } else {
+        rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(v, merged, lev_fused, substep, store_inflow, stream)
+                           : dispatch_route<REAL, STO, false, true>(v, merged, lev_fused, substep, store_inflow, stream))
+                 : (contig ? dispatch_route<REAL, STO, true, false>(v, merged, lev_fused, substep, store_inflow, stream)
+                           : dispatch_route<REAL, STO, false, false>(v, merged, lev_fused, substep, store_inflow, stream));
     }
-    int rc;
-    if (log) rc = contig ? dispatch_route<REAL, STO, true, true>(v, lev_fused, substep, store_inflow, stream)
-                         : dispatch_route<REAL, STO, false, true>(v, lev_fused, substep, store_inflow, stream);
-    else rc = contig ? dispatch_route<REAL, STO, true, false>(v, lev_fused, substep, store_inflow, stream)
-                     : dispatch_route<REAL, STO, false, false>(v, lev_fused, substep, store_inflow, stream);
     if (rc) return rc;
     if (log) {
         rc = log_final(v, blocks_for(NT), kLogAll, stream);

@@ -43,7 +43,9 @@ __device__ __forceinline__ void store_if_changed(X* p, X old, X val) {
 // inflow pass read them); inflows are not zeroed because the fused stage
 // pass writes them whole.  Otherwise flows are written in place, as in the
 // reference, and river/flood inflow and the bifurcation total are zeroed.
-template <class REAL, class STO, bool FUSED>
+// TERMS: write the outgoing-storage terms (pos_term / neg_term) of each
+// catchment, for the passes that sum them (k_outgoing_sum, k_limiter).
+template <class REAL, class STO, bool FUSED, bool TERMS>
 __global__ __launch_bounds__(kBlock) void k_outflow(View<REAL, STO> v, int substep) {
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (g >= v.N * v.T) return;
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_outflow(View<REAL, STO> v, int subst
     store_if_changed(v.fcsd + e, f_csd, upd_fcsd);
     store_if_changed(v.fcsa + e, f_csa, upd_fcsa);
     if (v.has(CMF_HAS_TOTAL_STORAGE)) v.total_storage[e] = (STO)total_storage;
-    if (!FUSED) {
+    if (TERMS) {
         const REAL pos = vmax(q_r, (REAL)0.0) + vmax(q_f, (REAL)0.0);
         const REAL neg = vmin(q_r, (REAL)0.0) + vmin(q_f, (REAL)0.0);
         v.pos_term[e] = pos * time_step;
@@ -240,12 +242,10 @@ __global__ __launch_bounds__(kBlock) void k_reservoir(View<REAL, STO> v, int sub
         v.flood_outflow[e] = (REAL)0.0;
     }
     v.total_storage[e] = (STO)total;
-    if (!FUSED) {
-        const REAL new_pos = vmax(out, (REAL)0.0);
-        const REAL new_neg = vmin(out, (REAL)0.0);
-        v.pos_term[e] = new_pos * time_step;
-        v.neg_term[e] = mouth ? (REAL)0.0 : -(new_neg * time_step);
-    }
+    const REAL new_pos = vmax(out, (REAL)0.0);
+    const REAL new_neg = vmin(out, (REAL)0.0);
+    v.pos_term[e] = new_pos * time_step;
+    v.neg_term[e] = mouth ? (REAL)0.0 : -(new_neg * time_step);
 }
 
 // per-op: re-sum the outgoing totals touched by a reservoir (its catchment
@@ -835,6 +835,106 @@ __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v) {
             v, e, j, rin, fin, v.river_outflow[e], v.flood_outflow[e], gb, v.dt(t), !is_lev, row, row_loaded, lp);
         v.outgoing[e] = (STO)0.0;
         store_state(v, e, so, fs_old, ps_old);
+    }
+    if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
+}
+
+// fused, single trial: storage limiter of every catchment from the terms of
+// the outflow pass (outgoing_storage of compute_outflow, then compute_inflow's
+// limiter, outflow.py:346-352).
+template <class REAL, class STO, bool CONTIG>
+__global__ __launch_bounds__(kBlock) void k_limiter(View<REAL, STO> v, int substep) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= v.N * v.T) return;
+    int64_t t, j;
+    v.split_c(g, t, j);
+    if (!v.active(t, substep)) return;
+    const int64_t e = v.el(t, j);
+    STO og = outgoing_terms<REAL, STO, CONTIG>(v, t, j);
+    if (v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[j] != v.ev_ptr[j + 1])
+        og = bif_event_sum<REAL, STO>(v, t, j, og, true);
+    const REAL l = limit_of(v, og, e);
+    v.lim[e] = l;
+    if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e] = l;
+}
+
+// fused, after k_limiter: limited outflow, inflow sums, bifurcation totals,
+// storage update and flood stage, and with LEV the levee stage of levee
+// catchments, in one pass.
+template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
+__global__ __launch_bounds__(kBlock) void k_stage_fused(View<REAL, STO> v, int substep, int store_inflow) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    LogPart lp;
+    if (LOG) {
+#pragma unroll
+        for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
+    }
+    if (g < v.N * v.T) {
+        int64_t t, j;
+        v.split_c(g, t, j);
+        if (v.active(t, substep)) {
+            const int64_t e = v.el(t, j);
+            const REAL dt = v.dt(t);
+            const REAL l_own = v.lim[e];
+            const REAL l_dn = v.lim[v.el(t, v.dn[j])];
+            const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
+            const REAL rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
+            const REAL fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
+            v.river_outflow[e] = rout;
+            v.flood_outflow[e] = fout;
+            // upstream u has downstream j: its negative flow is limited by j
+            const bool res = v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
+            STO ri = (STO)0, fi = (STO)0;
+            STO rt = res ? v.res_inflow[e] : (STO)0;
+            const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
+            for (int32_t k = b; k < en; ++k) {
+                const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+                const int64_t eu = v.el(t, u);
+                const REAL lu = v.lim[eu];
+                const REAL ur0 = v.qraw_r[eu], uf0 = v.qraw_f[eu];
+                const REAL ur1 = (ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own;
+                const REAL uf1 = (uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own;
+                ri += (STO)ur1;
+                fi += (STO)uf1;
+                if (res) rt += (STO)(ur1 + uf1);
+            }
+            if (res) v.res_inflow[e] = rt;
+            STO gsum = (STO)0;
+            REAL gb = (REAL)0.0;
+            if (v.has(CMF_HAS_BIFURCATION)) {
+                if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_event_sum<REAL, STO>(v, t, j, gsum, false);
+                gb = (REAL)gsum;
+            }
+            const bool store = (v.has(CMF_PER_TRIAL_DT) && substep >= 0) ? (substep == v.max_sub_steps[t] - 1)
+                                                                         : (store_inflow != 0);
+            if (store) {
+                v.river_inflow[e] = ri;
+                v.flood_inflow[e] = fi;
+                if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = gsum;
+            }
+            const bool lev_cell = v.has(CMF_HAS_LEVEE) && v.is_levee[j];
+            TableRow<REAL, LFIX> row;
+            bool row_loaded = false;
+            const STO fs_old = v.flood_storage[e], ps_old = v.protected_storage[e];
+            const StageOut<REAL, STO> so = stage_one<REAL, STO, LFIX, LOG>(
+                v, e, j, (REAL)ri, (REAL)fi, rout, fout, gb, dt, !lev_cell, row, row_loaded, lp);
+            if (LEV && lev_cell) {
+                const int32_t lv = v.levee_index[j];
+                if (!row_loaded) row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
+                const LeveeOut<REAL, STO> lo = levee_one<REAL, STO, LFIX>(
+                    v, e, j, lv, t * v.V + lv, so.rsf_hp, so.fsf_hp, so.fdep, so.rdep, so.ffr, row);
+                if (LOG) levee_log(lo, so.rsf_hp + so.fsf_hp, v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, j), lp);
+                v.river_storage[e] = lo.r_hp;
+                v.flood_storage[e] = lo.f_hp;
+                v.protected_storage[e] = lo.p_hp;
+                v.river_depth[e] = lo.r_dph;
+                v.flood_depth[e] = lo.f_dph;
+                v.protected_depth[e] = lo.p_dph;
+                v.flood_fraction[e] = lo.f_frc;
+            } else {
+                store_state(v, e, so, fs_old, ps_old);
+            }
+        }
     }
     if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
 }

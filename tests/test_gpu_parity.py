@@ -85,3 +85,63 @@ def test_log_sums():
     scale = np.maximum(np.abs(c), 1e-9)
     assert np.all(np.abs(g - c) <= 1e-12 * scale + 1e-15), (g, c)
     del sums_gpu
+
+
+def _member_sources(src, members, seed0=4000):
+    from cmfgpu_amd import synthetic
+    return [synthetic.Runoff(src["catchment_area"], seed0 + m).next_day() for m in range(members)]
+
+
+@pytest.mark.parametrize("trial_layout", ["inner", "major"])
+@pytest.mark.parametrize("modules", ["plain", "full"])
+def test_batched_members_match_standalone_runs(trial_layout, modules):
+    """Each member of a batched run with per-member CFL steps is bit-exact
+    with a standalone oracle run on that member's runoff."""
+    from cmfgpu_amd import _lib
+    mods = ("base", "adaptive_time") if modules == "plain" else FULL
+    net, src = small_source() if modules == "full" else small_source(paths=0, levee=0.0, reservoirs=0)
+    M = 4
+    runoffs = _member_sources(src, M)
+    model = _model(src, mods, num_trials=M, batched={"runoff": np.stack(runoffs)},
+                   per_trial_time_step=True, trial_layout=trial_layout)
+    oracles = []
+    for m in range(M):
+        s = dict(src, runoff=runoffs[m])
+        orc, _ = oracle_for(s, mods)
+        oracles.append(orc)
+    for step in range(5):
+        model.step_advance()
+        counts = model.t["max_sub_steps"].cpu().numpy()
+        for m in range(M):
+            assert int(orc_adv := oracles[m].advance(1800.0, 1)[0]) == counts[m], (step, m, orc_adv, counts[m])
+    import torch
+    torch.cuda.synchronize()
+    for name in STATE_FIELDS:
+        got = model.get(name)
+        for m in range(M):
+            msg = mismatch(got[m], oracles[m].state[name])
+            assert msg is None, f"member {m} {name}: {msg}"
+    del _lib
+
+
+def test_batched_shared_time_step():
+    """Trials sharing one time step (the reference's batched semantics): every
+    member advances the maximum sub-step count; compare with fixed-count runs."""
+    from oracle.oracle import HAS_ADAPTIVE
+    net, src = small_source(paths=0, levee=0.0, reservoirs=0)
+    M = 3
+    runoffs = _member_sources(src, M)
+    mods = ("base", "adaptive_time")
+    model = _model(src, mods, num_trials=M, batched={"runoff": np.stack(runoffs)})
+    oracles = []
+    for m in range(M):
+        orc, _ = oracle_for(dict(src, runoff=runoffs[m]), mods)
+        orc.flags &= ~HAS_ADAPTIVE
+        oracles.append(orc)
+    for step in range(4):
+        n = model.step_advance()
+        for m in range(M):
+            oracles[m].advance(1800.0, 1, fixed_count=n)
+    got = model.get("river_outflow")
+    for m in range(M):
+        assert mismatch(got[m], oracles[m].state["river_outflow"]) is None
