@@ -400,9 +400,11 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
             if ((rc = require({NEED(outgoing_storage), NEED(time_step)}, name))) return rc;
             if (op == OP_LEVEE_BIF_OUT) {
                 if ((rc = require({NEED(protected_water_surface_elevation)}, name))) return rc;
-                LAUNCH((k_bif_outflow<REAL, STO, true>), v.P * v.T, stream, v, -1);
+                if (v.LB == 5) LAUNCH((k_bif_outflow<REAL, STO, true, 5>), v.P * v.T, stream, v, -1);
+                else LAUNCH((k_bif_outflow<REAL, STO, true, 0>), v.P * v.T, stream, v, -1);
             } else {
-                LAUNCH((k_bif_outflow<REAL, STO, false>), v.P * v.T, stream, v, -1);
+                if (v.LB == 5) LAUNCH((k_bif_outflow<REAL, STO, false, 5>), v.P * v.T, stream, v, -1);
+                else LAUNCH((k_bif_outflow<REAL, STO, false, 0>), v.P * v.T, stream, v, -1);
             }
             LAUNCH((k_bif_outgoing<REAL, STO>), NT, stream, v);
             return CMF_OK;
@@ -496,12 +498,16 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
     if (v.has(CMF_HAS_RESERVOIR)) LAUNCH((k_reservoir<REAL, STO, true>), v.R * v.T, stream, v, substep);
     int rc;
     if (bif) {
-        if (levee) LAUNCH((k_bif_outflow<REAL, STO, true>), v.P * v.T, stream, v, substep);
-        else LAUNCH((k_bif_outflow<REAL, STO, false>), v.P * v.T, stream, v, substep);
+        if (levee) {
+            if (v.LB == 5) LAUNCH((k_bif_outflow<REAL, STO, true, 5>), v.P * v.T, stream, v, substep);
+            else LAUNCH((k_bif_outflow<REAL, STO, true, 0>), v.P * v.T, stream, v, substep);
+        } else {
+            if (v.LB == 5) LAUNCH((k_bif_outflow<REAL, STO, false, 5>), v.P * v.T, stream, v, substep);
+            else LAUNCH((k_bif_outflow<REAL, STO, false, 0>), v.P * v.T, stream, v, substep);
+        }
         if (merged) {
             if (contig) LAUNCH((k_lim_touched<REAL, STO, true>), net->n_touched * v.T, stream, v, net->touched, net->n_touched, substep);
             else LAUNCH((k_lim_touched<REAL, STO, false>), net->n_touched * v.T, stream, v, net->touched, net->n_touched, substep);
-            LAUNCH((k_bif_inflow<REAL, STO, true>), v.P * v.T, stream, v, substep);
             rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(v, true, lev_fused, substep, store_inflow, stream)
                                : dispatch_route<REAL, STO, false, true>(v, true, lev_fused, substep, store_inflow, stream))
                      : (contig ? dispatch_route<REAL, STO, true, false>(v, true, lev_fused, substep, store_inflow, stream)
@@ -510,7 +516,6 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
             // the path nets need every end limiter: full limiter pass first
             if (contig) LAUNCH((k_limiter<REAL, STO, true>), NT, stream, v, substep);
             else LAUNCH((k_limiter<REAL, STO, false>), NT, stream, v, substep);
-            LAUNCH((k_bif_inflow<REAL, STO, true>), v.P * v.T, stream, v, substep);
             rc = CMF_OK;
 #define CMF_SF(C, LG, LF, LV) LAUNCH((k_stage_fused<REAL, STO, C, LG, LF, LV>), NT, stream, v, substep, store_inflow)
             if (contig) {

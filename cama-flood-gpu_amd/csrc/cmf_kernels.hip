@@ -25,6 +25,21 @@ namespace cmf {
 
 constexpr int kBlock = 256;
 
+// Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Renumbering them so that every XCD walks one contiguous stretch of the
+// breadth-first layout keeps a block's upstream range and downstream gathers
+// in the L2 of the XCD that reads them.  A bijection on [0, gridDim.x).
+#ifndef CMF_XCD_REMAP
+#define CMF_XCD_REMAP 1
+#endif
+constexpr int64_t kNumXcd = 8;
+__device__ __forceinline__ int64_t xcd_block() {
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    if (!CMF_XCD_REMAP || nb < kNumXcd) return b;
+    const int64_t q = nb / kNumXcd, r = nb % kNumXcd, x = b % kNumXcd, i = b / kNumXcd;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 // Store `val` unless the location already holds it bit for bit (saves the
 // write of values that did not change, such as the zero flood state of dry
 // catchments).
@@ -47,7 +62,7 @@ __device__ __forceinline__ void store_if_changed(X* p, X old, X val) {
 // catchment, for the passes that sum them (k_outgoing_sum, k_limiter).
 template <class REAL, class STO, bool FUSED, bool TERMS>
 __global__ __launch_bounds__(kBlock) void k_outflow(View<REAL, STO> v, int substep) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = xcd_block() * kBlock + threadIdx.x;
     if (g >= v.N * v.T) return;
     int64_t t, i;
     v.split_c(g, t, i);
@@ -263,7 +278,11 @@ __global__ __launch_bounds__(kBlock) void k_res_regather(View<REAL, STO> v) {
 }
 
 // ====================================================== bifurcation ======
-template <class REAL, class STO, bool LEVEE>
+// LBFIX > 0: the level count known at compile time (loop unrolled, the
+// per-level flows kept in registers).  Besides the per-level state and the
+// outgoing terms, writes path_raw: the sum of the limited per-level flows,
+// in level order, as compute_bifurcation_inflow forms it (bifurcation.py:132-137).
+template <class REAL, class STO, bool LEVEE, int LBFIX>
 __global__ __launch_bounds__(kBlock) void k_bif_outflow(View<REAL, STO> v, int substep) {
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (g >= v.P * v.T) return;
@@ -284,9 +303,12 @@ __global__ __launch_bounds__(kBlock) void k_bif_outflow(View<REAL, STO> v, int s
     slope = vclamp(slope, (REAL)-0.005, (REAL)0.005);
     const REAL ts0 = (REAL)v.total_storage[eci];
     const REAL ts1 = (REAL)v.total_storage[edi];
-    const int64_t LB = v.LB;
+    const int LB = LBFIX > 0 ? LBFIX : (int)v.LB;
+    REAL qs[LBFIX > 0 ? LBFIX : 1];
     REAL sum = (REAL)0;
-    for (int64_t l = 0; l < LB; ++l) {
+#pragma unroll
+    for (int l = 0; l < (LBFIX > 0 ? LBFIX : 64); ++l) {
+        if (LBFIX == 0 && l >= LB) break;
         const int64_t k = pe * LB + l;
         const int64_t ks = p * LB + l;
         const REAL csd_old = v.bif_csd[k];
@@ -305,25 +327,64 @@ __global__ __launch_bounds__(kBlock) void k_bif_outflow(View<REAL, STO> v, int s
         }
         const REAL w = v.prm(v.bif_width, CMF_BATCHED_BIFURCATION_WIDTH, k, ks);
         const REAL man = v.prm(v.bif_manning, CMF_BATCHED_BIFURCATION_MANNING, k, ks);
+        const REAL q_old = v.bif_outflow[k];
         REAL q = (REAL)0.0;
         if (sifd > (REAL)1e-5) {
-            const REAL unit = v.bif_outflow[k] / w;
+            const REAL unit = q_old / w;
             const REAL num = w * (unit + gravity * time_step * sifd * slope);
             const REAL den = (REAL)1.0 + gravity * time_step * (man * man) * dabs(unit)
                            * ((REAL)1.0 / (sifd * sifd * dcbrt(sifd)));
             q = num / den;
         }
         sum += q;
-        v.bif_csd[k] = upd_csd;
-        v.bif_outflow[k] = q;
+        store_if_changed(v.bif_csd + k, csd_old, upd_csd);
+        if (LBFIX > 0) qs[LBFIX > 0 ? l : 0] = q;
+        else v.bif_outflow[k] = q;
     }
     const REAL limit = vmin((REAL)0.05 * vmin(ts0, ts1) / (dabs(sum) * time_step), (REAL)1.0);
     sum *= limit;
-    for (int64_t l = 0; l < LB; ++l) v.bif_outflow[pe * LB + l] *= limit;
+    REAL raw = (REAL)0;
+#pragma unroll
+    for (int l = 0; l < (LBFIX > 0 ? LBFIX : 64); ++l) {
+        if (LBFIX == 0 && l >= LB) break;
+        const int64_t k = pe * LB + l;
+        const REAL q = (LBFIX > 0 ? qs[LBFIX > 0 ? l : 0] : v.bif_outflow[k]) * limit;
+        v.bif_outflow[k] = q;
+        raw += q;
+    }
     const REAL pos = vmax(sum, (REAL)0.0);
     const REAL neg = vmin(sum, (REAL)0.0);
     v.path_pos[pe] = pos * time_step;
     v.path_neg[pe] = -neg * time_step;
+    v.path_net[pe] = raw;
+}
+
+// fused: bifurcation total of catchment j (compute_bifurcation_inflow,
+// bifurcation.py:128-151) from the path sums and the end limiters, in
+// ascending path order; the upstream end also applies the limiters to the
+// path's per-level flows.
+template <class REAL, class STO>
+__device__ __forceinline__ STO bif_total_fused(const View<REAL, STO>& v, int64_t t, int64_t j) {
+    STO acc = (STO)0;
+    const int32_t b = v.ev_ptr[j], en = v.ev_ptr[j + 1];
+    for (int32_t k = b; k < en; ++k) {
+        const int32_t code = v.ev_code[k];
+        const int64_t p = code >> 1;
+        const int64_t pe = v.pel(t, p);
+        const REAL l0 = v.lim[v.el(t, v.bif_ci[p])];
+        const REAL l1 = v.lim[v.el(t, v.bif_di[p])];
+        const REAL raw = v.path_net[pe];
+        const REAL net = (raw >= (REAL)0.0) ? raw * l0 : raw * l1;
+        acc += (code & 1) ? (STO)(-net) : (STO)net;
+        if (!(code & 1)) {
+            for (int64_t l = 0; l < v.LB; ++l) {
+                const int64_t kk = pe * v.LB + l;
+                const REAL q = v.bif_outflow[kk];
+                v.bif_outflow[kk] = (q >= (REAL)0.0) ? q * l0 : q * l1;
+            }
+        }
+    }
+    return acc;
 }
 
 // Sum of the bifurcation events of catchment j in ascending path order.
@@ -844,7 +905,7 @@ __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v) {
 // limiter, outflow.py:346-352).
 template <class REAL, class STO, bool CONTIG>
 __global__ __launch_bounds__(kBlock) void k_limiter(View<REAL, STO> v, int substep) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = xcd_block() * kBlock + threadIdx.x;
     if (g >= v.N * v.T) return;
     int64_t t, j;
     v.split_c(g, t, j);
@@ -863,7 +924,7 @@ __global__ __launch_bounds__(kBlock) void k_limiter(View<REAL, STO> v, int subst
 // catchments, in one pass.
 template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
 __global__ __launch_bounds__(kBlock) void k_stage_fused(View<REAL, STO> v, int substep, int store_inflow) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = xcd_block() * kBlock + threadIdx.x;
     LogPart lp;
     if (LOG) {
 #pragma unroll
@@ -902,7 +963,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_fused(View<REAL, STO> v, int s
             STO gsum = (STO)0;
             REAL gb = (REAL)0.0;
             if (v.has(CMF_HAS_BIFURCATION)) {
-                if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_event_sum<REAL, STO>(v, t, j, gsum, false);
+                if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_total_fused<REAL, STO>(v, t, j);
                 gb = (REAL)gsum;
             }
             const bool store = (v.has(CMF_PER_TRIAL_DT) && substep >= 0) ? (substep == v.max_sub_steps[t] - 1)
@@ -947,7 +1008,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_fused(View<REAL, STO> v, int s
 template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
 __global__ __launch_bounds__(kBlock) void k_route(View<REAL, STO> v, int substep, int store_inflow) {
     const int64_t NT = v.N * v.T;
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = xcd_block() * kBlock + threadIdx.x;
     LogPart lp;
     if (LOG) {
 #pragma unroll
@@ -996,7 +1057,7 @@ __global__ __launch_bounds__(kBlock) void k_route(View<REAL, STO> v, int substep
             STO gsum = (STO)0;
             REAL gb = (REAL)0.0;
             if (v.has(CMF_HAS_BIFURCATION)) {
-                if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_event_sum<REAL, STO>(v, t, j, gsum, false);
+                if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_total_fused<REAL, STO>(v, t, j);
                 gb = (REAL)gsum;
             }
             // per-trial counts: each trial stores on its own last sub-step
