@@ -51,6 +51,7 @@ struct cmf_network {
     void* path_pos = nullptr;
     void* path_neg = nullptr;
     void* path_net = nullptr;
+    void* path_raw = nullptr;
     double* partials = nullptr;
     int64_t partial_blocks = 0;
     int32_t* host_counts = nullptr;   // pinned, T entries
@@ -62,7 +63,7 @@ void free_network(cmf_network* net) {
     if (!net) return;
     void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->touched, net->pos_term,
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
-                    net->path_neg, net->path_net, net->partials};
+                    net->path_neg, net->path_net, net->path_raw, net->partials};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (net->host_counts) (void)hipHostFree(net->host_counts);
@@ -204,6 +205,7 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.path_pos = static_cast<REAL*>(net->path_pos);
     v.path_neg = static_cast<REAL*>(net->path_neg);
     v.path_net = static_cast<REAL*>(net->path_net);
+    v.path_raw = static_cast<REAL*>(net->path_raw);
     v.partials = net->partials;
     return CMF_OK;
 }
@@ -516,6 +518,7 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
             // the path nets need every end limiter: full limiter pass first
             if (contig) LAUNCH((k_limiter<REAL, STO, true>), NT, stream, v, substep);
             else LAUNCH((k_limiter<REAL, STO, false>), NT, stream, v, substep);
+            LAUNCH((k_bif_inflow<REAL, STO, true>), v.P * v.T, stream, v, substep);
             rc = CMF_OK;
 #define CMF_SF(C, LG, LF, LV) LAUNCH((k_stage_fused<REAL, STO, C, LG, LF, LV>), NT, stream, v, substep, store_inflow)
             if (contig) {
@@ -713,6 +716,7 @@ int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_pat
         (rc = dev_alloc(&net->qraw_r, el)) || (rc = dev_alloc(&net->qraw_f, el)) ||
         (rc = dev_alloc(&net->lim, el)) || (rc = dev_alloc(&net->path_pos, pel)) ||
         (rc = dev_alloc(&net->path_neg, pel)) || (rc = dev_alloc(&net->path_net, pel)) ||
+        (rc = dev_alloc(&net->path_raw, pel)) ||
         (rc = dev_alloc((void**)&net->partials, (size_t)net->partial_blocks * CMF_LOG_COUNT * 8))) {
         free_network(net);
         return rc;

@@ -142,7 +142,7 @@ struct View {
     const int32_t *ev_ptr, *ev_code;    // bifurcation events per catchment
     // scratch
     REAL *pos_term, *neg_term, *qraw_r, *qraw_f, *lim;
-    REAL *path_pos, *path_neg, *path_net;
+    REAL *path_pos, *path_neg, *path_net, *path_raw;
     double* partials;
 
     __device__ __forceinline__ int64_t el(int64_t t, int64_t i) const { return t * ts + i * cs; }

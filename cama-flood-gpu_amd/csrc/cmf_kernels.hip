@@ -26,11 +26,12 @@ namespace cmf {
 constexpr int kBlock = 256;
 
 // Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
-// Renumbering them so that every XCD walks one contiguous stretch of the
-// breadth-first layout keeps a block's upstream range and downstream gathers
-// in the L2 of the XCD that reads them.  A bijection on [0, gridDim.x).
+// With CMF_XCD_REMAP the blocks are renumbered so that every XCD walks one
+// contiguous stretch of the breadth-first layout (a bijection on
+// [0, gridDim.x)).  Measured slower on the 6-arcmin workload (the plain
+// rate drops about 5 %), so it is off by default.
 #ifndef CMF_XCD_REMAP
-#define CMF_XCD_REMAP 1
+#define CMF_XCD_REMAP 0
 #endif
 constexpr int64_t kNumXcd = 8;
 __device__ __forceinline__ int64_t xcd_block() {
@@ -356,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_bif_outflow(View<REAL, STO> v, int s
     const REAL neg = vmin(sum, (REAL)0.0);
     v.path_pos[pe] = pos * time_step;
     v.path_neg[pe] = -neg * time_step;
-    v.path_net[pe] = raw;
+    v.path_raw[pe] = raw;
 }
 
 // fused: bifurcation total of catchment j (compute_bifurcation_inflow,
@@ -373,7 +374,7 @@ __device__ __forceinline__ STO bif_total_fused(const View<REAL, STO>& v, int64_t
         const int64_t pe = v.pel(t, p);
         const REAL l0 = v.lim[v.el(t, v.bif_ci[p])];
         const REAL l1 = v.lim[v.el(t, v.bif_di[p])];
-        const REAL raw = v.path_net[pe];
+        const REAL raw = v.path_raw[pe];
         const REAL net = (raw >= (REAL)0.0) ? raw * l0 : raw * l1;
         acc += (code & 1) ? (STO)(-net) : (STO)net;
         if (!(code & 1)) {
@@ -963,7 +964,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_fused(View<REAL, STO> v, int s
             STO gsum = (STO)0;
             REAL gb = (REAL)0.0;
             if (v.has(CMF_HAS_BIFURCATION)) {
-                if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_total_fused<REAL, STO>(v, t, j);
+                if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_event_sum<REAL, STO>(v, t, j, gsum, false);
                 gb = (REAL)gsum;
             }
             const bool store = (v.has(CMF_PER_TRIAL_DT) && substep >= 0) ? (substep == v.max_sub_steps[t] - 1)
