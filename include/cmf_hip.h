@@ -193,10 +193,10 @@ typedef struct cmf_fields {
      * when set, the fused sub-step runs the levee stage inside the flood
      * stage pass) */
     const int32_t* levee_index;
-    /* log module: CMF_LOG_COUNT FP64 buffers of log_buffer_size entries each,
-     * one set per trial (buffer k of trial t at log_sums[k] + t * log_buffer_size
-     * * CMF_LOG_COUNT is not used: trials share one set, as LogModule refuses
-     * trials, cmfgpu/models/cama_flood_model.py:65-67). */
+    /* log module: CMF_LOG_COUNT FP64 buffers of log_buffer_size entries each;
+     * sum k of sub-step s accumulates into log_sums[k][s].  All trials add
+     * into the one set (the reference's LogModule supports one trial,
+     * cmfgpu/models/cama_flood_model.py:65-67). */
     double* log_sums[CMF_LOG_COUNT];
 } cmf_fields;
 
