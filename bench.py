@@ -65,7 +65,7 @@ def algorithmic_bytes(kernel: str, model, f_wet: float) -> float:
     return 0.0
 
 
-def build_model(net, src, members: int, device: str):
+def build_model(net, src, members: int, device: str, layout: str):
     from cmfgpu_amd.models import CaMaFlood
     mods = ["base", "adaptive_time"]
     if net.num_paths:
@@ -73,13 +73,13 @@ def build_model(net, src, members: int, device: str):
     if net.num_levees:
         mods.append("levee")
     if members <= 1:
-        return CaMaFlood(src, opened_modules=tuple(mods), device=device, step_duration=1800.0)
+        return CaMaFlood(src, opened_modules=tuple(mods), device=device, step_duration=1800.0, layout=layout)
     import numpy as np
     from cmfgpu_amd import synthetic
     runoff = np.stack([synthetic.Runoff(src["catchment_area"], 4000 + m).next_day() for m in range(members)])
     return CaMaFlood(src, opened_modules=tuple(mods), device=device, step_duration=1800.0,
                      num_trials=members, batched={"runoff": runoff}, per_trial_time_step=True,
-                     trial_layout="inner")
+                     trial_layout="inner", layout=layout)
 
 
 def run_steps(model, steps: int, per_trial: bool):
@@ -172,6 +172,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="network size factor (1.0 = 1.6M catchments)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--layout", default="basin", choices=("basin", "level", "network"),
+                    help="device catchment order")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
@@ -206,7 +208,7 @@ def main():
         f"({time.perf_counter() - t0:.1f} s)")
 
     # ---------------------------------------------------------- plain --
-    model = build_model(net, src, 1, device)
+    model = build_model(net, src, 1, device, args.layout)
     run_steps(model, args.warmup, per_trial=False)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -245,7 +247,7 @@ def main():
     batched = None
     if args.members > 1:
         M = args.members
-        bm = build_model(net, src, M, device)
+        bm = build_model(net, src, M, device, args.layout)
         run_steps(bm, max(args.warmup // 4, 1), per_trial=True)
         bsteps = args.batched_steps or max(args.steps // 4, 2)
         torch.cuda.synchronize()

@@ -6,9 +6,11 @@ context, and advances outer steps either through the fused C-ABI schedule
 per-op sequence (``mode="ops"``, the zero-argument registered ops called in
 the order of step_advance, cama_flood_model.py:104-173).
 
-Device layout.  With ``layout="level"`` (the default) catchments are
-renumbered breadth-first from the mouths (network.level_order) so that the
-upstream catchments of every catchment are one contiguous range; with
+Device layout.  With ``layout="basin"`` (the default) catchments are
+renumbered breadth-first from the mouths inside each basin, basin after basin
+(network.basin_level_order), so that the upstream catchments of every
+catchment are one contiguous range next to it; ``layout="level"`` uses the
+breadth-first levels of the whole forest (network.level_order); with
 ``layout="network"`` the reference's own order is kept.  With trials,
 ``trial_layout="inner"`` stores element (trial t, catchment i) at i*T + t
 (all members of a catchment adjacent), ``"major"`` at t*N + i (the
@@ -61,15 +63,15 @@ def _torch_dtype(a: np.ndarray):
 class CaMaFlood:
     def __init__(self, src: dict, *, opened_modules=("base", "adaptive_time"), precision="float32",
                  device="cuda:0", num_trials: int = 1, batched: dict | None = None,
-                 per_trial_time_step: bool = False, layout: str = "level", trial_layout: str = "inner",
+                 per_trial_time_step: bool = False, layout: str = "basin", trial_layout: str = "inner",
                  log_buffer_size: int = 20000, step_duration: float = 1800.0, num_sub_steps: int = 1,
                  start_time: datetime = datetime(2000, 1, 1), log_path: str | Path | None = None,
                  mode: str = "fused", gravity: float = 9.8, adaptive_time_factor: float = 0.7,
                  min_kinematic_slope: float = 1e-5):
         if mode not in ("fused", "ops"):
             raise ValueError("mode must be 'fused' or 'ops'")
-        if layout not in ("level", "network"):
-            raise ValueError("layout must be 'level' or 'network'")
+        if layout not in ("basin", "level", "network"):
+            raise ValueError("layout must be 'basin', 'level' or 'network'")
         if trial_layout not in ("inner", "major"):
             raise ValueError("trial_layout must be 'inner' or 'major'")
         if num_trials < 1:
@@ -103,7 +105,12 @@ class CaMaFlood:
         self.L = self.host.num_flood_levels
 
         dn = hf["downstream_idx"]
-        if layout == "level":
+        if layout == "basin":
+            self.order = netmod.basin_level_order(dn)
+            self.perm = self.order.perm.astype(np.int64)
+            self.inv = self.order.inv.astype(np.int64)
+            dn_int = self.order.dn.astype(np.int32)
+        elif layout == "level":
             self.order = netmod.level_order(dn)
             self.perm = self.order.perm.astype(np.int64)
             self.inv = self.order.inv.astype(np.int64)

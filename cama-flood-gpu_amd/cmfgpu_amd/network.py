@@ -133,6 +133,43 @@ def level_order(downstream_idx) -> LevelOrder:
 
 
 @dataclass
+class BasinLevelOrder:
+    """Breadth-first levels laid out basin by basin (cmf_basin_level_order).
+
+    Same arrays as LevelOrder; basin_ptr[b]..basin_ptr[b+1] is basin b (the tree
+    of one mouth, mouths in ascending network index) in internal order.
+    """
+
+    perm: np.ndarray
+    inv: np.ndarray
+    up_ptr: np.ndarray
+    dn: np.ndarray
+    basin_ptr: np.ndarray
+
+    @property
+    def num_basins(self) -> int:
+        return self.basin_ptr.shape[0] - 1
+
+
+def basin_level_order(downstream_idx) -> BasinLevelOrder:
+    d = _i32(downstream_idx)
+    n = d.shape[0]
+    perm = np.empty(n, np.int32)
+    inv = np.empty(n, np.int32)
+    up_ptr = np.empty(n + 1, np.int32)
+    dn = np.empty(n, np.int32)
+    nb = ctypes.c_int64(0)
+    cap = n + 1
+    basin_ptr = np.empty(cap, np.int64)
+    _lib.check(_lib.lib().cmf_basin_level_order(n, _p(d, ctypes.c_int32), _p(perm, ctypes.c_int32),
+                                                _p(inv, ctypes.c_int32), _p(up_ptr, ctypes.c_int32),
+                                                _p(dn, ctypes.c_int32), ctypes.byref(nb), cap,
+                                                _p(basin_ptr, ctypes.c_int64)),
+               "basin_level_order")
+    return BasinLevelOrder(perm, inv, up_ptr, dn, basin_ptr[: nb.value + 1].copy())
+
+
+@dataclass
 class Connectivity:
     """Result of finalize_connectivity: the reference order of the catchments."""
 

@@ -680,9 +680,25 @@ int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_pat
         for (int64_t i = 0; i < n; ++i)
             if (downstream_idx[i] != i) up_idx[fill[downstream_idx[i]]++] = (int32_t)i;
     }
+    // Contiguous layout: walking the catchments in order, the upstream
+    // catchments of each are the next ones in index order.  A mouth whose own
+    // index comes next claims itself as well (the basin-by-basin level order
+    // puts each mouth right after its upstream range); kernels skip the owner.
+    std::vector<int32_t> cup((size_t)n + 1, 0);
     bool contiguous = true;
-    for (size_t k = 0; k < up_idx.size(); ++k)
-        if (up_idx[k] != (int32_t)k) { contiguous = false; break; }
+    {
+        int64_t pos = 0;
+        for (int64_t k = 0; k < n && contiguous; ++k) {
+            cup[k] = (int32_t)pos;
+            for (int32_t q = up_ptr[k]; q < up_ptr[k + 1]; ++q) {
+                if (up_idx[q] != pos) { contiguous = false; break; }
+                ++pos;
+            }
+            if (downstream_idx[k] == k && pos == k) ++pos;
+        }
+        cup[n] = (int32_t)pos;
+    }
+    if (contiguous) up_ptr.swap(cup);
     // bifurcation events per catchment, ascending path, upstream end first
     std::vector<int32_t> ev_ptr((size_t)n + 1, 0), ev_code((size_t)(2 * num_paths));
     for (int64_t p = 0; p < num_paths; ++p) { ev_ptr[bif_ci[p] + 1]++; ev_ptr[bif_di[p] + 1]++; }

@@ -138,7 +138,10 @@ struct View {
     const uint8_t *is_reservoir, *is_dam_up, *is_dam_related;
     double* log_sums[CMF_LOG_COUNT];
     // network
-    const int32_t *up_ptr, *up_idx;     // up_idx == nullptr: contiguous ranges
+    // Upstream catchments of j: [up_ptr[j], up_ptr[j+1]) directly (up_idx ==
+    // nullptr: contiguous ranges, where a mouth's range may end with the mouth
+    // itself) or through up_idx.
+    const int32_t *up_ptr, *up_idx;
     const int32_t *ev_ptr, *ev_code;    // bifurcation events per catchment
     // scratch
     REAL *pos_term, *neg_term, *qraw_r, *qraw_f, *lim;

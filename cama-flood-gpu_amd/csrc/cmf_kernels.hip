@@ -190,6 +190,7 @@ __device__ __forceinline__ STO outgoing_terms(const View<REAL, STO>& v, int64_t 
     const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
     for (int32_t k = b; k < en; ++k) {
         const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+        if (CONTIG && u == j) continue;  // a mouth's range ends with the mouth
         acc += (STO)v.neg_term[v.el(t, u)];
     }
     acc += (STO)v.pos_term[v.el(t, j)];
@@ -494,6 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_inflow_sum(View<REAL, STO> v) {
     STO rt = res ? v.res_inflow[e] : (STO)0;
     for (int32_t k = b; k < en; ++k) {
         const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+        if (CONTIG && u == j) continue;
         const int64_t eu = v.el(t, u);
         const REAL ur = v.river_outflow[eu], uf = v.flood_outflow[eu];
         ri += (STO)ur;
@@ -529,6 +531,7 @@ __device__ __forceinline__ REAL limit_recomputed(const View<REAL, STO>& v, int64
     const int32_t b = v.up_ptr[x], en = v.up_ptr[x + 1];
     for (int32_t k = b; k < en; ++k) {
         const int64_t w = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+        if (CONTIG && w == x) continue;
         const int64_t ew = v.el(t, w);
         og += (STO)neg_term_of(v.qraw_r[ew], v.qraw_f[ew], dt);
     }
@@ -716,14 +719,22 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one(
     const REAL river_max_storage = rl * rw * rh;
     const REAL dwth_inc = (ca / rl) / (REAL)L;
     const REAL levee_distance = lfrac * (ca / rl);
+    const int ilev = (int)(lfrac * (REAL)L);
+    // The scan of levee.py:149-211.  The quotients it takes at one level only
+    // (the base and crown interpolation ratios, the gradient of the level where
+    // the protected side stops filling) are formed after the loop from the
+    // operands recorded at that level: the same operations on the same values,
+    // one division each instead of one per level.
     REAL s_curr = river_max_storage;
     REAL dhgt_pre = (REAL)0.0;
     REAL dwth_pre = rw;
-    REAL base_sto = river_max_storage, fill_sto = river_max_storage;
     bool found_base = false, found_fill = false;
-    const int ilev = (int)(lfrac * (REAL)L);
-    REAL dsto_fil_B = (REAL)0.0, dwth_fil_B = (REAL)0.0, ddph_fil_B = (REAL)0.0, gradient_B = (REAL)0.0;
-    bool found_B = false;
+    REAL b_s = (REAL)0.0, b_pre = (REAL)0.0, b_seg = (REAL)1.0, b_w = (REAL)0.0;   // base level operands
+    REAL f_s = (REAL)0.0, f_pre = (REAL)0.0, f_seg = (REAL)1.0, f_w = (REAL)0.0;   // crown level operands
+    bool ib_found = false;                                                         // base found by level ilev
+    REAL ib_s = (REAL)0.0, ib_pre = (REAL)0.0, ib_seg = (REAL)1.0, ib_w = (REAL)0.0;
+    REAL dsto_fil_B = (REAL)0.0, dwth_fil_B = (REAL)0.0, ddph_fil_B = (REAL)0.0, seg_B = (REAL)0.0;
+    bool found_B = false, B_at_ilev = false;
 #pragma unroll
     for (int k = 0; k < (LFIX > 0 ? LFIX : 64); ++k) {
         if (LFIX == 0 && k >= L) break;
@@ -733,29 +744,23 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one(
         const REAL dwth_mid = dwth_pre + (REAL)0.5 * dwth_inc;
         const REAL dsto_seg = rl * dwth_mid * dhgt_seg;
         const REAL s_next = s_curr + dsto_seg;
-        const REAL gradient = dhgt_seg / dwth_inc;
-        const bool cond_base = (base > dhgt_pre) && (base <= depth_val);
-        if (cond_base) {
-            const REAL ratio_base = (base - dhgt_pre) / dhgt_seg;
-            base_sto = s_curr + rl * (dwth_pre + (REAL)0.5 * ratio_base * dwth_inc) * (ratio_base * dhgt_seg);
+        if ((base > dhgt_pre) && (base <= depth_val)) {
+            b_s = s_curr; b_pre = dhgt_pre; b_seg = dhgt_seg; b_w = dwth_pre;
+            found_base = true;
         }
-        found_base = found_base || cond_base;
-        const bool cond_fill = (crown > dhgt_pre) && (crown <= depth_val);
-        if (cond_fill) {
-            const REAL ratio_fill = (crown - dhgt_pre) / dhgt_seg;
-            fill_sto = s_curr + rl * (dwth_pre + (REAL)0.5 * ratio_fill * dwth_inc) * (ratio_fill * dhgt_seg);
+        if ((crown > dhgt_pre) && (crown <= depth_val)) {
+            f_s = s_curr; f_pre = dhgt_pre; f_seg = dhgt_seg; f_w = dwth_pre;
+            found_fill = true;
         }
-        found_fill = found_fill || cond_fill;
-        const bool cond_check = (k >= ilev) && !found_B;
-        if (cond_check) {
+        if (k == ilev) {
+            ib_found = found_base; ib_s = b_s; ib_pre = b_pre; ib_seg = b_seg; ib_w = b_w;
+        }
+        if ((k >= ilev) && !found_B) {
             const REAL wedge = (levee_distance + rw) * (crown - depth_val) * rl;
             const REAL threshold = s_next + wedge;
             if (total < threshold) {
-                if (k == ilev) {
-                    const REAL dhgt_dif_loop = crown - base;
-                    dsto_fil_B = base_sto + (levee_distance + rw) * dhgt_dif_loop * rl;
-                }
-                gradient_B = gradient;
+                B_at_ilev = (k == ilev);
+                seg_B = dhgt_seg;
                 found_B = true;
             } else {
                 dsto_fil_B = threshold;
@@ -767,10 +772,22 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one(
         dhgt_pre = depth_val;
         dwth_pre += dwth_inc;
     }
+    // storage at a height inside level (s, pre, seg, w): levee.py:163-170,178-185
+    auto interp_sto = [&](REAL h, REAL s, REAL pre, REAL seg, REAL w) -> REAL {
+        const REAL ratio = (h - pre) / seg;
+        return s + rl * (w + (REAL)0.5 * ratio * dwth_inc) * (ratio * seg);
+    };
     const REAL s_base_extra = s_curr + rl * dwth_pre * (base - dhgt_pre);
-    base_sto = found_base ? base_sto : ((base > dhgt_pre) ? s_base_extra : river_max_storage);
+    const REAL base_sto = found_base ? interp_sto(base, b_s, b_pre, b_seg, b_w)
+                                     : ((base > dhgt_pre) ? s_base_extra : river_max_storage);
     const REAL s_fill_extra = s_curr + rl * dwth_pre * (crown - dhgt_pre);
-    fill_sto = found_fill ? fill_sto : ((crown > dhgt_pre) ? s_fill_extra : river_max_storage);
+    const REAL fill_sto = found_fill ? interp_sto(crown, f_s, f_pre, f_seg, f_w)
+                                     : ((crown > dhgt_pre) ? s_fill_extra : river_max_storage);
+    const REAL gradient_B = found_B ? seg_B / dwth_inc : (REAL)0.0;
+    if (B_at_ilev) {
+        const REAL base_sto_ilev = ib_found ? interp_sto(base, ib_s, ib_pre, ib_seg, ib_w) : river_max_storage;
+        dsto_fil_B = base_sto_ilev + (levee_distance + rw) * (crown - base) * rl;
+    }
     const REAL dhgt_dif = crown - base;
     const REAL s_top = base_sto + (levee_distance + rw) * dhgt_dif * rl;
     const bool case4 = total >= fill_sto;
@@ -951,6 +968,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_fused(View<REAL, STO> v, int s
             const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
             for (int32_t k = b; k < en; ++k) {
                 const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+                if (CONTIG && u == j) continue;
                 const int64_t eu = v.el(t, u);
                 const REAL lu = v.lim[eu];
                 const REAL ur0 = v.qraw_r[eu], uf0 = v.qraw_f[eu];
@@ -1031,6 +1049,7 @@ __global__ __launch_bounds__(kBlock) void k_route(View<REAL, STO> v, int substep
             const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
             for (int32_t k = b; k < en; ++k) {
                 const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+                if (CONTIG && u == j) continue;
                 const int64_t eu = v.el(t, u);
                 const REAL ur0 = v.qraw_r[eu], uf0 = v.qraw_f[eu];
                 REAL lu = (REAL)1.0;

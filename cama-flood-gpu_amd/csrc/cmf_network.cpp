@@ -10,6 +10,7 @@
 // and the build's own renumbering:
 //   cmf_level_order            breadth-first levels from the mouths; see the
 //                              comment on the function.
+//   cmf_basin_level_order      the same levels laid out basin by basin.
 
 #include "cmf_internal.hpp"
 
@@ -267,6 +268,94 @@ int cmf_level_order(int64_t n, const int32_t* downstream_idx, int32_t* out_perm,
         if (level_ptr_capacity < num_levels + 1)
             return set_error(CMF_ERR_INVALID, "cmf_level_order: level_ptr buffer too small");
         for (int64_t d = 0; d <= num_levels; ++d) out_level_ptr[d] = ilevel_start[d];
+    }
+    return CMF_OK;
+}
+
+// Breadth-first levels inside each basin.
+//
+// A basin is the tree that drains to one mouth.  Basins are laid out one after
+// another in ascending network index of their mouths; inside a basin the
+// levels of that tree follow deepest first, each built as in cmf_level_order
+// (upstream catchments of consecutive catchments are consecutive, in
+// ascending network index).  So downstream(k) > k, upstream ranges stay
+// contiguous with up_ptr monotone over the whole network, and a catchment and
+// its downstream neighbour lie one tree level apart instead of one level of the
+// whole forest apart.
+//
+//   out_basin_ptr     start of each basin in internal order (num_basins + 1
+//                     entries, capacity basin_ptr_capacity; NULL to skip)
+int cmf_basin_level_order(int64_t n, const int32_t* downstream_idx, int32_t* out_perm,
+                          int32_t* out_inv, int32_t* out_up_ptr, int32_t* out_dn,
+                          int64_t* out_num_basins, int64_t basin_ptr_capacity,
+                          int64_t* out_basin_ptr) {
+    if (n < 0 || n > INT32_MAX - 1 ||
+        (n > 0 && (!downstream_idx || !out_perm || !out_inv || !out_up_ptr || !out_dn)) ||
+        !out_num_basins)
+        return set_error(CMF_ERR_INVALID, "cmf_basin_level_order: bad arguments");
+    int rc = cmf_check_flow_direction(n, downstream_idx);
+    if (rc != CMF_OK) return rc;
+    std::vector<int32_t> cptr((size_t)n + 1, 0), cidx;
+    for (int64_t i = 0; i < n; ++i) if (downstream_idx[i] != i) cptr[downstream_idx[i] + 1]++;
+    for (int64_t i = 0; i < n; ++i) cptr[i + 1] += cptr[i];
+    cidx.resize((size_t)cptr[n]);
+    {
+        std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
+        for (int64_t i = 0; i < n; ++i)
+            if (downstream_idx[i] != i) cidx[fill[downstream_idx[i]]++] = (int32_t)i;
+    }
+    std::vector<int32_t> bfs;
+    std::vector<int64_t> lstart;
+    std::vector<int64_t> bstart;
+    int64_t pos = 0;
+    for (int64_t m = 0; m < n; ++m) {
+        if (downstream_idx[m] != m) continue;
+        bstart.push_back(pos);
+        bfs.clear();
+        lstart.clear();
+        bfs.push_back((int32_t)m);
+        lstart.push_back(0);
+        int64_t head = 0;
+        while (head < (int64_t)bfs.size()) {
+            const int64_t end = (int64_t)bfs.size();
+            lstart.push_back(end);
+            for (int64_t k = head; k < end; ++k) {
+                const int32_t u = bfs[k];
+                for (int32_t c = cptr[u]; c < cptr[u + 1]; ++c) bfs.push_back(cidx[c]);
+            }
+            head = end;
+        }
+        if (pos + (int64_t)bfs.size() > n)
+            return set_error(CMF_ERR_INVALID, "cmf_basin_level_order: network is not a forest");
+        // lstart[d]..lstart[d+1] is level d; emit deepest first
+        for (int64_t d = (int64_t)lstart.size() - 2; d >= 0; --d)
+            for (int64_t k = lstart[d]; k < lstart[d + 1]; ++k) out_perm[pos++] = bfs[k];
+    }
+    if (pos != n)
+        return set_error(CMF_ERR_INVALID, "cmf_basin_level_order: network is not a forest");
+    const int64_t nb = (int64_t)bstart.size();
+    *out_num_basins = nb;
+    for (int64_t k = 0; k < n; ++k) out_inv[out_perm[k]] = (int32_t)k;
+    for (int64_t k = 0; k < n; ++k) out_dn[k] = out_inv[downstream_idx[out_perm[k]]];
+    // A mouth sits right after its upstream range, so its range also covers
+    // the mouth itself (the catchment after it starts the next basin); users
+    // of the ranges skip the owner.
+    int64_t acc = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        out_up_ptr[k] = (int32_t)acc;
+        const int32_t u = out_perm[k];
+        const int32_t cnt = cptr[u + 1] - cptr[u];
+        if (cnt > 0 && out_inv[cidx[cptr[u]]] != acc)
+            return set_error(CMF_ERR_INTERNAL, "cmf_basin_level_order: upstream ranges are not contiguous");
+        acc += cnt;
+        if (out_dn[k] == k && acc == k) acc += 1;
+    }
+    out_up_ptr[n] = (int32_t)acc;
+    if (out_basin_ptr) {
+        if (basin_ptr_capacity < nb + 1)
+            return set_error(CMF_ERR_INVALID, "cmf_basin_level_order: basin_ptr buffer too small");
+        for (int64_t b = 0; b < nb; ++b) out_basin_ptr[b] = bstart[b];
+        out_basin_ptr[nb] = n;
     }
     return CMF_OK;
 }

@@ -235,6 +235,15 @@ int cmf_level_order(int64_t n, const int32_t* downstream_idx, int32_t* out_perm,
                     int32_t* out_inv, int32_t* out_up_ptr, int32_t* out_dn,
                     int64_t* out_num_levels, int64_t level_ptr_capacity,
                     int64_t* out_level_ptr);
+/* Breadth-first levels laid out basin by basin (each tree draining to one
+ * mouth is contiguous, its levels deepest first); same outputs as
+ * cmf_level_order plus the basin boundaries.  No reference counterpart: the
+ * reference groups basins the same way (cmfgpu/params/utils.py:483-503) but
+ * keeps its topological order inside them. */
+int cmf_basin_level_order(int64_t n, const int32_t* downstream_idx, int32_t* out_perm,
+                          int32_t* out_inv, int32_t* out_up_ptr, int32_t* out_dn,
+                          int64_t* out_num_basins, int64_t basin_ptr_capacity,
+                          int64_t* out_basin_ptr);
 
 /* -------------------------------------------------------------- network */
 /* Build the device network context from host copies of the indices.

@@ -8,6 +8,7 @@ TEST INFRASTRUCTURE ONLY (the checker for cmfgpu_amd.network).  Small inputs.
                          ascending label order: the reference's np.argsort
                          default kind leaves that tie order unspecified)
   level_order            breadth-first levels from the mouths (the device layout)
+  basin_level_order      the same levels laid out basin by basin
 """
 from __future__ import annotations
 
@@ -91,3 +92,38 @@ def level_order(downstream_idx):
     dn = [inv[downstream_idx[perm[k]]] for k in range(n)]
     return (np.array(perm, np.int32), np.array(inv, np.int32), np.array(up_ptr, np.int32),
             np.array(dn, np.int32), len(levels))
+
+
+def basin_level_order(downstream_idx):
+    """Breadth-first levels inside each basin, basins by ascending mouth index,
+    deepest level first inside a basin (the layout of cmf_basin_level_order)."""
+    n = len(downstream_idx)
+    children = [[] for _ in range(n)]
+    for i, d in enumerate(downstream_idx):
+        if d != i:
+            children[d].append(i)
+    perm, basin_ptr = [], [0]
+    for m in range(n):
+        if downstream_idx[m] != m:
+            continue
+        levels = [[m]]
+        while True:
+            nxt = [c for u in levels[-1] for c in children[u]]
+            if not nxt:
+                break
+            levels.append(nxt)
+        perm += [i for lev in reversed(levels) for i in lev]
+        basin_ptr.append(len(perm))
+    inv = [0] * n
+    for k, i in enumerate(perm):
+        inv[i] = k
+    dn = [inv[downstream_idx[perm[k]]] for k in range(n)]
+    # a mouth's range ends with the mouth itself when the mouth comes next
+    up_ptr = [0]
+    for k in range(n):
+        end = up_ptr[-1] + len(children[perm[k]])
+        if dn[k] == k and end == k:
+            end += 1
+        up_ptr.append(end)
+    return (np.array(perm, np.int32), np.array(inv, np.int32), np.array(up_ptr, np.int32),
+            np.array(dn, np.int32), np.array(basin_ptr, np.int64))

@@ -27,7 +27,7 @@ def _compare(model, orc, fields):
         assert msg is None, f"{name}: {msg}"
 
 
-@pytest.mark.parametrize("layout", ["level", "network"])
+@pytest.mark.parametrize("layout", ["basin", "level", "network"])
 @pytest.mark.parametrize("mode", ["fused", "ops"])
 def test_plain_bitexact(layout, mode):
     net, src = small_source(paths=0, levee=0.0, reservoirs=0)
@@ -41,7 +41,7 @@ def test_plain_bitexact(layout, mode):
     _compare(model, orc, STATE_FIELDS)
 
 
-@pytest.mark.parametrize("layout", ["level", "network"])
+@pytest.mark.parametrize("layout", ["basin", "level", "network"])
 @pytest.mark.parametrize("mode", ["fused", "ops"])
 def test_full_physics_bitexact(layout, mode):
     net, src = small_source()

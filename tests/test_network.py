@@ -98,6 +98,49 @@ def test_level_order_properties_at_config0_size():
     assert np.all(up_net[1:][same] > up_net[:-1][same])
 
 
+@pytest.mark.parametrize("seed", range(5))
+def test_basin_level_order_matches_restatement(seed):
+    ids, down = random_forest(300, seed, p_outlet=0.05)
+    c = network.finalize_connectivity(ids, down)
+    bo = network.basin_level_order(c.downstream_idx)
+    perm, inv, up_ptr, dn, basin_ptr = ordering.basin_level_order(c.downstream_idx.tolist())
+    assert np.array_equal(bo.perm, perm)
+    assert np.array_equal(bo.inv, inv)
+    assert np.array_equal(bo.up_ptr, up_ptr)
+    assert np.array_equal(bo.dn, dn)
+    assert np.array_equal(bo.basin_ptr, basin_ptr)
+
+
+def test_basin_level_order_properties_at_config0_size():
+    net = synthetic.build_config(0)
+    d = net.fields["downstream_idx"]
+    bo = network.basin_level_order(d)
+    n = d.shape[0]
+    k = np.arange(n)
+    mouth = bo.dn == k
+    assert np.all(bo.dn[~mouth] > k[~mouth])
+    assert np.array_equal(bo.inv[bo.perm], k)
+    # each basin is contiguous and ends with its mouth
+    assert bo.basin_ptr[0] == 0 and bo.basin_ptr[-1] == n
+    ends = bo.basin_ptr[1:] - 1
+    assert np.array_equal(np.flatnonzero(mouth), ends)
+    basin = np.repeat(np.arange(bo.num_basins), np.diff(bo.basin_ptr))
+    assert np.array_equal(basin[bo.dn], basin)
+    # ranges tile [0, up_ptr[n]): the catchments draining to k, plus k itself
+    # at the end of a mouth's range
+    assert bo.up_ptr[0] == 0 and np.all(np.diff(bo.up_ptr) >= 0)
+    owner = np.repeat(k, np.diff(bo.up_ptr))
+    members = np.arange(bo.up_ptr[-1])
+    own = members == owner
+    assert np.all(mouth[owner[own]])
+    assert np.all(bo.up_ptr[owner[own] + 1] - 1 == members[own])
+    assert np.array_equal(bo.dn[members[~own]], owner[~own])
+    assert np.array_equal(np.bincount(owner[~own], minlength=n), np.bincount(bo.dn[~mouth], minlength=n))
+    up_net = bo.perm[members[~own]]
+    same = owner[~own][1:] == owner[~own][:-1]
+    assert np.all(up_net[1:][same] > up_net[:-1][same])
+
+
 def test_find_indices_and_accumulate():
     ref = np.array([40, 10, 30, 20])
     assert network.find_indices_in(np.array([10, 20, 99, 40]), ref).tolist() == [1, 3, -1, 0]
