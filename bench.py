@@ -82,6 +82,44 @@ def build_model(net, src, members: int, device: str, layout: str):
                      trial_layout="inner", layout=layout)
 
 
+ERROR_FIELDS = ("river_outflow", "flood_outflow", "river_storage", "flood_storage", "river_depth",
+                "flood_depth", "flood_fraction")
+
+
+def fp64_errors(net, src, device: str, layout: str, days: int) -> dict:
+    """Relative L2 error of each output field of the FP32 (FP64 storage) run
+    against the (FP64, FP64) run of the same kernels, after `days` simulated
+    days (48 outer steps each, fresh daily runoff) from the same state.  The
+    FP64 profile is bit-identical to the FP64 CPU oracle (parity tests), so
+    this is the error against an FP64 oracle run at full size."""
+    import numpy as np
+    from cmfgpu_amd import synthetic
+    from cmfgpu_amd.models import CaMaFlood
+    mods = ["base", "adaptive_time"] + (["bifurcation"] if net.num_paths else []) \
+        + (["levee"] if net.num_levees else [])
+    models = [CaMaFlood(src, opened_modules=tuple(mods), device=device, step_duration=1800.0, layout=layout,
+                        precision=p) for p in ("float32", "float64")]
+    forcing = synthetic.Runoff(src["catchment_area"], 2003)
+    count_mismatch = 0
+    for _ in range(days):
+        r = forcing.next_day()
+        for m in models:
+            m.set_inputs(r)
+        for _ in range(48):
+            a, b = (m.step_advance() for m in models)
+            count_mismatch += int(a != b)
+    out = {}
+    for name in ERROR_FIELDS:
+        x, y = (m.get(name).astype(np.float64) for m in models)
+        out[name] = float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+    q32, q64 = ((m.get("river_outflow").astype(np.float64) + m.get("flood_outflow")) for m in models)
+    for m in models:
+        m.close()
+    return {"days": days, "outer_steps": 48 * days, "substep_count_mismatches": count_mismatch,
+            "discharge_rel_l2": float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64)),
+            "fields_rel_l2": {k: float(f"{v:.4g}") for k, v in out.items()}}
+
+
 def run_steps(model, steps: int, per_trial: bool):
     """Advance `steps` outer steps; returns member-catchment-sub-steps done."""
     import torch
@@ -172,6 +210,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="network size factor (1.0 = 1.6M catchments)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--error-days", type=int, default=1,
+                    help="simulated days of the FP32-vs-FP64 error run (0: skip)")
     ap.add_argument("--layout", default="basin", choices=("basin", "level", "network"),
                     help="device catchment order")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -262,6 +302,11 @@ def main():
             f"member-catchment-steps/s ({bvalue / value:.2f}x plain)")
         bm.close()
 
+    errors = None
+    if args.error_days > 0:
+        errors = fp64_errors(net, src, device, args.layout, args.error_days)
+        log(f"FP32 vs FP64 after {args.error_days} day(s): discharge rel L2 {errors['discharge_rel_l2']:.3g}")
+
     line = {
         "metric": METRIC, "value": value, "unit": "catchment-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
@@ -270,7 +315,7 @@ def main():
                    "catchments": net.num_catchments, "bifurcation_paths": net.num_paths,
                    "levees": net.num_levees, "flood_levels": 10, "outer_step_s": 1800.0,
                    "substeps_timed": int(substeps), "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "error_vs_fp64": errors,
     }
     print(json.dumps(line), flush=True)
 

@@ -12,8 +12,13 @@ member m = 4000 + m.
 Builder choices (the configurations leave them open):
   * the downstream link of catchment g (generation order, outlets first) is
     g - k with k ~ Geometric(0.01) folded into [1, min(2000, g)];
-  * downstream_distance = river_length; outlet bed elevation 0 m;
-    catchment_elevation = bed elevation + bank depth;
+  * downstream_distance = river_length; the elevation profile (slope
+    U(1e-4, 1e-3) times river length, outlets at 0 m) is applied to
+    catchment_elevation, the bank-top elevation water-surface slopes are
+    taken from; the bed lies a bank depth below it.  Applied to the bed
+    instead, bank tops rise downstream wherever the downstream channel is
+    deeper, water ponds behind those steps, and FP32 and FP64 runs of such a
+    network drift apart by several percent of discharge within a day;
   * levee crown U(1, 5) m above bank; where the derived levee base is not
     below it, the crown is lifted to base + U(1, 5) m;
   * bifurcation length = mean river length of the two ends, width
@@ -138,7 +143,7 @@ def generate_network(n: int, seed: int, *, levee: float = 0.0, paths: int = 0,
     area = rng.lognormal(np.log(600e6), 0.7, n)
     slope = np.maximum(rng.uniform(1e-4, 1e-3, n), 1e-5)
     layers = rng.uniform(0.0, 2.0, (n, NUM_FLOOD_LEVELS))
-    bed = _pointer_jump_sum(dn_gen, np.where(outlet, 0.0, slope * length))
+    top = _pointer_jump_sum(dn_gen, np.where(outlet, 0.0, slope * length))
 
     ids = rng.permutation(n).astype(np.int64) * 7 + 11
     down_ids = np.where(outlet, -1, ids[np.maximum(dn_gen, 0)])
@@ -155,7 +160,7 @@ def generate_network(n: int, seed: int, *, levee: float = 0.0, paths: int = 0,
     f["river_height"] = height[o].astype(np.float32)
     f["catchment_area"] = area[o].astype(np.float32)
     f["downstream_distance"] = f["river_length"].copy()
-    f["catchment_elevation"] = (bed[o] + height[o]).astype(np.float32)
+    f["catchment_elevation"] = top[o].astype(np.float32)
     f["river_manning"] = np.full(n, 0.03, np.float32)
     f["flood_manning"] = np.full(n, 0.10, np.float32)
     f["flood_depth_table"] = np.cumsum(layers[o], axis=1).astype(np.float32)
