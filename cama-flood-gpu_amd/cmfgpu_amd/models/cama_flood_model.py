@@ -60,6 +60,30 @@ def _torch_dtype(a: np.ndarray):
             np.dtype(np.bool_): torch.bool}[a.dtype]
 
 
+def _group_basins(order, groups: np.ndarray, dn: np.ndarray):
+    """Order the basins of a basin layout group by group (ascending label,
+    basins keeping their order inside a group).  Returns the new permutation,
+    the group boundaries in internal order and the sorted labels.  Every
+    catchment of a basin must carry its basin's label."""
+    n = dn.shape[0]
+    if groups.shape != (n,):
+        raise ValueError(f"groups must have one label per catchment ({n}), got shape {groups.shape}")
+    if np.any(groups[dn] != groups):
+        bad = int(np.flatnonzero(groups[dn] != groups)[0])
+        raise ValueError(f"catchment {bad} drains into another group: a group must be a union of basins")
+    bptr = order.basin_ptr
+    mouths = order.perm[bptr[1:] - 1]
+    blabel = groups[mouths]
+    labels, gidx = np.unique(blabel, return_inverse=True)
+    border = np.argsort(gidx, kind="stable")
+    sizes = np.diff(bptr)
+    perm = np.concatenate([order.perm[bptr[b]:bptr[b + 1]] for b in border]).astype(np.int64)
+    gsize = np.bincount(gidx, weights=sizes, minlength=labels.shape[0]).astype(np.int64)
+    group_ptr = np.zeros(labels.shape[0] + 1, np.int64)
+    group_ptr[1:] = np.cumsum(gsize)
+    return perm, group_ptr, labels
+
+
 class CaMaFlood:
     def __init__(self, src: dict, *, opened_modules=("base", "adaptive_time"), precision="float32",
                  device="cuda:0", num_trials: int = 1, batched: dict | None = None,
@@ -67,7 +91,7 @@ class CaMaFlood:
                  log_buffer_size: int = 20000, step_duration: float = 1800.0, num_sub_steps: int = 1,
                  start_time: datetime = datetime(2000, 1, 1), log_path: str | Path | None = None,
                  mode: str = "fused", gravity: float = 9.8, adaptive_time_factor: float = 0.7,
-                 min_kinematic_slope: float = 1e-5):
+                 min_kinematic_slope: float = 1e-5, groups=None):
         if mode not in ("fused", "ops"):
             raise ValueError("mode must be 'fused' or 'ops'")
         if layout not in ("basin", "level", "network"):
@@ -81,6 +105,10 @@ class CaMaFlood:
             raise ValueError("the log module does not support trials (cama_flood_model.py:65-67)")
         if mode == "ops" and per_trial_time_step:
             raise ValueError("per-trial time steps need the fused schedule")
+        if groups is not None and (mode != "fused" or num_trials != 1 or layout != "basin"):
+            raise ValueError("groups need the fused schedule, one trial and the basin layout")
+        if groups is not None and log_path is not None:
+            raise ValueError("the text log is global; read per-group sums with log_sums()")
         self.opened_modules = opened
         self.mode = mode
         self.device = torch.device(device)
@@ -105,11 +133,21 @@ class CaMaFlood:
         self.L = self.host.num_flood_levels
 
         dn = hf["downstream_idx"]
+        self.G = 0
+        self.group_labels = None
+        group_ptr = None
         if layout == "basin":
             self.order = netmod.basin_level_order(dn)
             self.perm = self.order.perm.astype(np.int64)
-            self.inv = self.order.inv.astype(np.int64)
-            dn_int = self.order.dn.astype(np.int32)
+            if groups is not None:
+                self.perm, group_ptr, self.group_labels = _group_basins(self.order, np.asarray(groups), dn)
+                self.G = int(group_ptr.shape[0] - 1)
+                self.inv = np.empty_like(self.perm)
+                self.inv[self.perm] = np.arange(self.N, dtype=np.int64)
+                dn_int = self.inv[dn[self.perm]].astype(np.int32)
+            else:
+                self.inv = self.order.inv.astype(np.int64)
+                dn_int = self.order.dn.astype(np.int32)
         elif layout == "level":
             self.order = netmod.level_order(dn)
             self.perm = self.order.perm.astype(np.int64)
@@ -168,9 +206,12 @@ class CaMaFlood:
             for name in RESERVOIR_PARAMS:
                 self.t[name] = torch.from_numpy(np.ascontiguousarray(hf[name])).to(self.device)
         if "log_sums" in hf:
-            self.t["log_sums"] = torch.from_numpy(hf["log_sums"].copy()).to(self.device)
+            ls = hf["log_sums"]
+            if self.G:
+                ls = np.zeros((ls.shape[0], self.G * ls.shape[1]), ls.dtype)
+            self.t["log_sums"] = torch.from_numpy(ls.copy()).to(self.device)
         tdt = torch.float32 if self.real == np.float32 else torch.float64
-        slots = self.T if self.per_trial else 1
+        slots = self.G if self.G else (self.T if self.per_trial else 1)
         self.t["time_step"] = torch.zeros(slots, dtype=tdt, device=self.device)
         self.t["outer_time_step"] = torch.zeros(1, dtype=tdt, device=self.device)
         self.t["current_step"] = torch.zeros(1, dtype=torch.int32, device=self.device)
@@ -178,10 +219,12 @@ class CaMaFlood:
 
         # network context (host indices in device order)
         h = ctypes.c_void_p()
-        _lib.check(_lib.lib().cmf_network_create(
+        gp = group_ptr if group_ptr is not None else np.zeros(1, np.int64)
+        _lib.check(_lib.lib().cmf_network_create_grouped(
             self.N, dn_int.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), self.P,
             ci.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), di.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-            self.T, ctypes.byref(h)), "cmf_network_create")
+            self.T, self.G, gp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(h)),
+            "cmf_network_create_grouped")
         self.network_handle = h
         self.contiguous = bool(_lib.lib().cmf_network_is_contiguous(h))
         self._table = self._build_table()
@@ -278,7 +321,15 @@ class CaMaFlood:
         return x
 
     def log_sums(self) -> np.ndarray:
-        return self.t["log_sums"].cpu().numpy()
+        """The water-balance sums, [11, log_buffer_size] (with groups [11, G, log_buffer_size])."""
+        x = self.t["log_sums"].cpu().numpy()
+        if self.G:
+            return x.reshape(x.shape[0], self.G, -1)
+        return x
+
+    def group_sub_steps(self) -> np.ndarray:
+        """Sub-step count of each group in the last outer step (group order = sorted labels)."""
+        return self.t["max_sub_steps"].cpu().numpy()
 
     def set_inputs(self, runoff) -> None:
         """Stage runoff forcing (cama_flood_model.py:74-102): [N], or [T, N] when runoff is batched."""
@@ -332,7 +383,11 @@ class CaMaFlood:
             n = self._step_ops()
         self.last_sub_steps = n
         log = self.host.modules.get("log")
-        if log is not None:
+        if log is not None and self.G:
+            # per-group sums of this outer step: [11, G, log_buffer_size]
+            self.last_group_log = self.log_sums()
+            self.t["log_sums"].zero_()
+        elif log is not None:
             log.set_time(self.step_duration / n, n, self.current_time)
             if self.log_path is not None:
                 log.write_step(self.log_path, self.log_sums())

@@ -261,6 +261,43 @@ def basin_sizes(count: int = 400, lo: int = 500, hi: int = 400_000, seed: int = 
     return np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
 
 
+@dataclass
+class BasinSet:
+    """Independent basins concatenated into one network (reference order of
+    each basin, basins one after another); ``group`` labels each catchment
+    with its basin and ``basin_ptr`` bounds the basins."""
+
+    network: Network
+    group: np.ndarray
+    basin_ptr: np.ndarray
+
+
+def build_basins(count: int = 400, lo: int = 500, hi: int = 400_000, scale: float = 1.0) -> BasinSet:
+    """The uneven-basin workload: `count` basins with sizes log-uniform in
+    [lo, hi] (seed 3000), basin b generated as the plain network with seed
+    3000 + b.  `scale` multiplies every size (at least 50 catchments)."""
+    sizes = np.maximum(np.round(basin_sizes(count, lo, hi) * scale), 50).astype(np.int64)
+    parts = [generate_network(int(sizes[b]), 3000 + b) for b in range(count)]
+    ptr = np.zeros(count + 1, np.int64)
+    ptr[1:] = np.cumsum([p.num_catchments for p in parts])
+    f = {}
+    for name in parts[0].fields:
+        arrs = []
+        for b, p in enumerate(parts):
+            a = p.fields[name]
+            if name == "downstream_idx":
+                a = (a.astype(np.int64) + ptr[b]).astype(np.int32)
+            elif name in ("catchment_id", "downstream_id", "catchment_basin_id"):
+                a = a.astype(np.int64) + ptr[b] * 8 + 16 * b
+            arrs.append(a)
+        f[name] = np.concatenate(arrs)
+    net = Network(name=f"basins{count}" + ("" if scale == 1.0 else f"@{scale:g}"), fields=f)
+    net.levels = np.concatenate([p.levels for p in parts])
+    net.upstream_area = np.concatenate([p.upstream_area for p in parts])
+    group = np.repeat(np.arange(count, dtype=np.int32), np.diff(ptr))
+    return BasinSet(network=net, group=group, basin_ptr=ptr)
+
+
 class Runoff:
     """Daily runoff forcing (m^3/s per catchment, float32).
 

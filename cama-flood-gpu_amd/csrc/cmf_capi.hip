@@ -54,7 +54,16 @@ struct cmf_network {
     void* path_raw = nullptr;
     double* partials = nullptr;
     int64_t partial_blocks = 0;
-    int32_t* host_counts = nullptr;   // pinned, T entries
+    int32_t* host_counts = nullptr;   // pinned, max(T, G) entries
+    // groups (G > 0): group of each catchment and the block table of the
+    // catchment kernels (blocks never straddle two groups)
+    int64_t G = 0;
+    int64_t nblk = 0;
+    int32_t* cgroup = nullptr;
+    int32_t* blk_first = nullptr;
+    int32_t* blk_end = nullptr;
+    int32_t* blk_group = nullptr;
+    int32_t* gblk_ptr = nullptr;      // first block of each group, G + 1 entries
 };
 
 namespace {
@@ -63,7 +72,8 @@ void free_network(cmf_network* net) {
     if (!net) return;
     void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->touched, net->pos_term,
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
-                    net->path_neg, net->path_net, net->path_raw, net->partials};
+                    net->path_neg, net->path_net, net->path_raw, net->partials, net->cgroup,
+                    net->blk_first, net->blk_end, net->blk_group, net->gblk_ptr};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (net->host_counts) (void)hipHostFree(net->host_counts);
@@ -193,6 +203,10 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.is_dam_related = f->is_dam_related;
     v.levee_index = f->levee_index;
     for (int k = 0; k < CMF_LOG_COUNT; ++k) v.log_sums[k] = f->log_sums[k];
+    v.G = net->G;
+    v.cgroup = net->cgroup;
+    v.blk_first = net->blk_first;
+    v.blk_end = net->blk_end;
     v.up_ptr = net->up_ptr;
     v.up_idx = net->contiguous ? nullptr : net->up_idx;
     v.ev_ptr = net->ev_ptr;
@@ -333,6 +347,9 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
     View<REAL, STO> v;
     int rc = make_view<REAL, STO>(net, f, v);
     if (rc) return rc;
+    if (net->G > 0)
+        return set_error(CMF_ERR_UNSUPPORTED,
+                         "per-op entry points take ungrouped networks; advance grouped networks with cmf_hip_advance");
     const int64_t NT = v.N * v.T;
     const bool contig = net->contiguous != 0;
     switch (op) {
@@ -462,25 +479,27 @@ int dispatch_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
 // ------------------------------------------------------------- fused ----
 // MERGED: limiters recomputed inside the route pass (k_route), used for the
 // trial-inner batched layout; otherwise k_limiter then k_stage_fused.
+// `nt`: threads of a catchment kernel (N*T, or the group block table)
 template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
-int launch_route(const View<REAL, STO>& v, bool merged, int substep, int store_inflow, void* stream) {
+int launch_route(const View<REAL, STO>& v, int64_t nt, bool merged, int substep, int store_inflow, void* stream) {
     if (merged) {
-        LAUNCH((k_route<REAL, STO, CONTIG, LOG, LFIX, LEV>), v.N * v.T, stream, v, substep, store_inflow);
+        LAUNCH((k_route<REAL, STO, CONTIG, LOG, LFIX, LEV>), nt, stream, v, substep, store_inflow);
     } else {
-        LAUNCH((k_limiter<REAL, STO, CONTIG>), v.N * v.T, stream, v, substep);
-        LAUNCH((k_stage_fused<REAL, STO, CONTIG, LOG, LFIX, LEV>), v.N * v.T, stream, v, substep, store_inflow);
+        LAUNCH((k_limiter<REAL, STO, CONTIG>), nt, stream, v, substep);
+        LAUNCH((k_stage_fused<REAL, STO, CONTIG, LOG, LFIX, LEV>), nt, stream, v, substep, store_inflow);
     }
     return CMF_OK;
 }
 
 template <class REAL, class STO, bool CONTIG, bool LOG>
-int dispatch_route(const View<REAL, STO>& v, bool merged, bool lev, int substep, int store_inflow, void* stream) {
+int dispatch_route(const View<REAL, STO>& v, int64_t nt, bool merged, bool lev, int substep, int store_inflow,
+                   void* stream) {
     if (v.L == 10) {
-        return lev ? launch_route<REAL, STO, CONTIG, LOG, 10, true>(v, merged, substep, store_inflow, stream)
-                   : launch_route<REAL, STO, CONTIG, LOG, 10, false>(v, merged, substep, store_inflow, stream);
+        return lev ? launch_route<REAL, STO, CONTIG, LOG, 10, true>(v, nt, merged, substep, store_inflow, stream)
+                   : launch_route<REAL, STO, CONTIG, LOG, 10, false>(v, nt, merged, substep, store_inflow, stream);
     }
-    return lev ? launch_route<REAL, STO, CONTIG, LOG, 0, true>(v, merged, substep, store_inflow, stream)
-               : launch_route<REAL, STO, CONTIG, LOG, 0, false>(v, merged, substep, store_inflow, stream);
+    return lev ? launch_route<REAL, STO, CONTIG, LOG, 0, true>(v, nt, merged, substep, store_inflow, stream)
+               : launch_route<REAL, STO, CONTIG, LOG, 0, false>(v, nt, merged, substep, store_inflow, stream);
 }
 
 // One fused sub-step: outflow, [reservoir], [bifurcation outflow, end
@@ -488,7 +507,8 @@ int dispatch_route(const View<REAL, STO>& v, bool merged, bool lev, int substep,
 template <class REAL, class STO>
 int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int substep,
                   int store_inflow, void* stream) {
-    const int64_t NT = v.N * v.T;
+    // catchment kernels: N*T threads, or one block per group table entry
+    const int64_t NT = net->G > 0 ? net->nblk * kBlock : v.N * v.T;
     const bool contig = net->contiguous != 0;
     const bool bif = v.has(CMF_HAS_BIFURCATION), levee = v.has(CMF_HAS_LEVEE);
     const bool log = v.has(CMF_HAS_LOG);
@@ -510,10 +530,10 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
         if (merged) {
             if (contig) LAUNCH((k_lim_touched<REAL, STO, true>), net->n_touched * v.T, stream, v, net->touched, net->n_touched, substep);
             else LAUNCH((k_lim_touched<REAL, STO, false>), net->n_touched * v.T, stream, v, net->touched, net->n_touched, substep);
-            rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(v, true, lev_fused, substep, store_inflow, stream)
-                               : dispatch_route<REAL, STO, false, true>(v, true, lev_fused, substep, store_inflow, stream))
-                     : (contig ? dispatch_route<REAL, STO, true, false>(v, true, lev_fused, substep, store_inflow, stream)
-                               : dispatch_route<REAL, STO, false, false>(v, true, lev_fused, substep, store_inflow, stream));
+            rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(v, NT, true, lev_fused, substep, store_inflow, stream)
+                               : dispatch_route<REAL, STO, false, true>(v, NT, true, lev_fused, substep, store_inflow, stream))
+                     : (contig ? dispatch_route<REAL, STO, true, false>(v, NT, true, lev_fused, substep, store_inflow, stream)
+                               : dispatch_route<REAL, STO, false, false>(v, NT, true, lev_fused, substep, store_inflow, stream));
         } else {
             // the path nets need every end limiter: full limiter pass first
             if (contig) LAUNCH((k_limiter<REAL, STO, true>), NT, stream, v, substep);
@@ -535,15 +555,23 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
 #undef CMF_SF
         }
     } else {
-        rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(v, merged, lev_fused, substep, store_inflow, stream)
-                           : dispatch_route<REAL, STO, false, true>(v, merged, lev_fused, substep, store_inflow, stream))
-                 : (contig ? dispatch_route<REAL, STO, true, false>(v, merged, lev_fused, substep, store_inflow, stream)
-                           : dispatch_route<REAL, STO, false, false>(v, merged, lev_fused, substep, store_inflow, stream));
+        rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(v, NT, merged, lev_fused, substep, store_inflow, stream)
+                           : dispatch_route<REAL, STO, false, true>(v, NT, merged, lev_fused, substep, store_inflow, stream))
+                 : (contig ? dispatch_route<REAL, STO, true, false>(v, NT, merged, lev_fused, substep, store_inflow, stream)
+                           : dispatch_route<REAL, STO, false, false>(v, NT, merged, lev_fused, substep, store_inflow, stream));
     }
     if (rc) return rc;
     if (log) {
-        rc = log_final(v, blocks_for(NT), kLogAll, stream);
-        if (rc) return rc;
+        if (net->G > 0) {
+            timing_mark("k_log_final_grouped", (hipStream_t)stream, true);
+            hipLaunchKernelGGL((k_log_final_grouped<REAL, STO>), dim3((unsigned)net->G), dim3(kBlock), 0,
+                               (hipStream_t)stream, v, net->gblk_ptr, kLogAll, substep);
+            timing_mark("k_log_final_grouped", (hipStream_t)stream, false);
+            CMF_HIP_CHECK(hipGetLastError());
+        } else {
+            rc = log_final(v, blocks_for(NT), kLogAll, stream);
+            if (rc) return rc;
+        }
     }
     if (levee && !lev_fused) {
         if (log) {
@@ -562,13 +590,26 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
     return CMF_OK;
 }
 
+// Grouped networks: one trial; with the log module the levee stage must run
+// inside the stage pass (levee_index), whose blocks follow the group table.
+int need_grouped(const cmf_network* net, const cmf_fields* f, const char* name) {
+    if (net->G == 0) return CMF_OK;
+    if (f->num_trials != 1) return set_error(CMF_ERR_UNSUPPORTED, "%s: grouped networks take one trial", name);
+    if ((f->flags & CMF_HAS_LOG) && (f->flags & CMF_HAS_LEVEE) && !f->levee_index)
+        return set_error(CMF_ERR_UNSUPPORTED, "%s: grouped networks with log and levee need levee_index", name);
+    if (!f->max_sub_steps || !f->time_step)
+        return set_error(CMF_ERR_INVALID, "%s: grouped networks need max_sub_steps and time_step (one per group)", name);
+    return CMF_OK;
+}
+
 template <class REAL, class STO>
 int run_substep(cmf_network* net, const cmf_fields* f, int substep, int store_inflow, void* stream) {
     View<REAL, STO> v;
     int rc = make_view<REAL, STO>(net, f, v);
     if (rc) return rc;
-    if ((rc = need_base(f, "substep")) || (rc = need_features(f, "substep"))) return rc;
-    if (v.has(CMF_HAS_LOG) && blocks_for(v.N * v.T) > net->partial_blocks)
+    if ((rc = need_base(f, "substep")) || (rc = need_features(f, "substep")) || (rc = need_grouped(net, f, "substep")))
+        return rc;
+    if (v.has(CMF_HAS_LOG) && (net->G > 0 ? net->nblk : blocks_for(v.N * v.T)) > net->partial_blocks)
         return set_error(CMF_ERR_INTERNAL, "substep: partial buffer too small");
     return fused_substep(net, v, f, substep, store_inflow, stream);
 }
@@ -593,19 +634,23 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
     View<REAL, STO> v;
     int rc = make_view<REAL, STO>(net, f, v);
     if (rc) return rc;
-    if ((rc = need_base(f, "advance")) || (rc = need_features(f, "advance"))) return rc;
+    if ((rc = need_base(f, "advance")) || (rc = need_features(f, "advance")) || (rc = need_grouped(net, f, "advance")))
+        return rc;
     if (!(outer > 0.0)) return set_error(CMF_ERR_INVALID, "advance: outer step must be > 0");
     hipStream_t s = (hipStream_t)stream;
-    const bool per_trial = v.has(CMF_PER_TRIAL_DT);
-    const int64_t nslots = per_trial ? v.T : 1;
+    const bool per_trial = v.has(CMF_PER_TRIAL_DT) || net->G > 0;
+    const int64_t nslots = net->G > 0 ? net->G : (v.has(CMF_PER_TRIAL_DT) ? v.T : 1);
     int count = fixed_count;
-    if (v.has(CMF_HAS_LOG) && blocks_for(v.N * v.T) > net->partial_blocks)
+    if (v.has(CMF_HAS_LOG) && (net->G > 0 ? net->nblk : blocks_for(v.N * v.T)) > net->partial_blocks)
         return set_error(CMF_ERR_INTERNAL, "advance: partial buffer too small");
     if (v.has(CMF_HAS_ADAPTIVE_TIME)) {
         if ((rc = require({NEED(max_sub_steps), NEED(outer_time_step)}, "advance"))) return rc;
         hipLaunchKernelGGL((k_prepare<REAL>), dim3(1), dim3(64), 0, s, static_cast<REAL*>(f->outer_time_step), outer);
         hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, f->max_sub_steps, nslots);
-        if (v.N > 0) {
+        if (net->G > 0) {
+            hipLaunchKernelGGL((k_cfl_grouped<REAL, STO>), dim3((unsigned)net->nblk), dim3(kBlock), 0, s, v,
+                               net->blk_group);
+        } else if (v.N > 0) {
             const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 2048);
             hipLaunchKernelGGL((k_cfl<REAL, STO>), dim3((unsigned)chunks, (unsigned)v.T), dim3(kBlock), 0, s, v);
         }
@@ -654,9 +699,9 @@ const char* cmf_last_error(void) { return g_last_error.c_str(); }
 int cmf_version(void) { return 10000; }
 int64_t cmf_fields_size(void) { return (int64_t)sizeof(cmf_fields); }
 
-int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_paths,
-                       const int32_t* bif_ci, const int32_t* bif_di, int64_t num_trials,
-                       cmf_network** out) {
+int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t num_paths,
+                               const int32_t* bif_ci, const int32_t* bif_di, int64_t num_trials,
+                               int64_t num_groups, const int64_t* group_ptr, cmf_network** out) {
     if (!out) return set_error(CMF_ERR_INVALID, "cmf_network_create: out is NULL");
     *out = nullptr;
     if (n < 1 || n > INT32_MAX - 1 || !downstream_idx)
@@ -670,6 +715,38 @@ int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_pat
         if (bif_ci[p] < 0 || bif_ci[p] >= n || bif_di[p] < 0 || bif_di[p] >= n)
             return set_error(CMF_ERR_INVALID, "cmf_network_create: bifurcation index out of range at path %lld",
                              (long long)p);
+    // groups: contiguous ranges closed under downstream links and paths
+    std::vector<int32_t> cgroup, blk_first, blk_end, blk_group, gblk_ptr;
+    if (num_groups < 0 || (num_groups > 0 && !group_ptr))
+        return set_error(CMF_ERR_INVALID, "cmf_network_create_grouped: bad group arguments");
+    if (num_groups > 0) {
+        if (num_trials != 1)
+            return set_error(CMF_ERR_UNSUPPORTED, "cmf_network_create_grouped: grouped networks take one trial");
+        if (group_ptr[0] != 0 || group_ptr[num_groups] != n)
+            return set_error(CMF_ERR_INVALID, "cmf_network_create_grouped: group_ptr must run from 0 to n");
+        cgroup.resize((size_t)n);
+        for (int64_t g = 0; g < num_groups; ++g) {
+            if (group_ptr[g + 1] <= group_ptr[g])
+                return set_error(CMF_ERR_INVALID, "cmf_network_create_grouped: group %lld is empty or reversed",
+                                 (long long)g);
+            gblk_ptr.push_back((int32_t)blk_first.size());
+            for (int64_t i = group_ptr[g]; i < group_ptr[g + 1]; ++i) cgroup[i] = (int32_t)g;
+            for (int64_t b = group_ptr[g]; b < group_ptr[g + 1]; b += kBlock) {
+                blk_first.push_back((int32_t)b);
+                blk_end.push_back((int32_t)std::min<int64_t>(b + kBlock, group_ptr[g + 1]));
+                blk_group.push_back((int32_t)g);
+            }
+        }
+        gblk_ptr.push_back((int32_t)blk_first.size());
+        for (int64_t i = 0; i < n; ++i)
+            if (cgroup[downstream_idx[i]] != cgroup[i])
+                return set_error(CMF_ERR_INVALID, "cmf_network_create_grouped: catchment %lld drains into another group",
+                                 (long long)i);
+        for (int64_t p = 0; p < num_paths; ++p)
+            if (cgroup[bif_ci[p]] != cgroup[bif_di[p]])
+                return set_error(CMF_ERR_INVALID, "cmf_network_create_grouped: bifurcation path %lld joins two groups",
+                                 (long long)p);
+    }
     // upstream lists in ascending network index
     std::vector<int32_t> up_ptr((size_t)n + 1, 0), up_idx;
     for (int64_t i = 0; i < n; ++i) if (downstream_idx[i] != i) up_ptr[downstream_idx[i] + 1]++;
@@ -722,7 +799,9 @@ int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_pat
     (void)hipGetDevice(&net->device);
     const size_t el = (size_t)n * (size_t)num_trials * 8;
     const size_t pel = (size_t)std::max<int64_t>(num_paths, 1) * (size_t)num_trials * 8;
-    net->partial_blocks = blocks_for(n * num_trials) + 1;
+    net->G = num_groups;
+    net->nblk = (int64_t)blk_first.size();
+    net->partial_blocks = std::max<int64_t>(blocks_for(n * num_trials), net->nblk) + 1;
     if ((rc = upload((void**)&net->up_ptr, up_ptr.data(), up_ptr.size() * 4)) ||
         (rc = upload((void**)&net->up_idx, up_idx.data(), up_idx.size() * 4)) ||
         (rc = upload((void**)&net->ev_ptr, ev_ptr.data(), ev_ptr.size() * 4)) ||
@@ -737,7 +816,17 @@ int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_pat
         free_network(net);
         return rc;
     }
-    hipError_t e = hipHostMalloc((void**)&net->host_counts, sizeof(int32_t) * (size_t)num_trials, hipHostMallocDefault);
+    if (num_groups > 0 &&
+        ((rc = upload((void**)&net->cgroup, cgroup.data(), cgroup.size() * 4)) ||
+         (rc = upload((void**)&net->blk_first, blk_first.data(), blk_first.size() * 4)) ||
+         (rc = upload((void**)&net->blk_end, blk_end.data(), blk_end.size() * 4)) ||
+         (rc = upload((void**)&net->blk_group, blk_group.data(), blk_group.size() * 4)) ||
+         (rc = upload((void**)&net->gblk_ptr, gblk_ptr.data(), gblk_ptr.size() * 4)))) {
+        free_network(net);
+        return rc;
+    }
+    const size_t slots = (size_t)std::max<int64_t>(num_trials, num_groups);
+    hipError_t e = hipHostMalloc((void**)&net->host_counts, sizeof(int32_t) * slots, hipHostMallocDefault);
     if (e != hipSuccess) {
         free_network(net);
         return set_error(CMF_ERR_NOMEM, "hipHostMalloc failed: %s", hipGetErrorString(e));
@@ -745,6 +834,14 @@ int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_pat
     *out = net;
     return CMF_OK;
 }
+
+int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_paths,
+                       const int32_t* bif_ci, const int32_t* bif_di, int64_t num_trials,
+                       cmf_network** out) {
+    return cmf_network_create_grouped(n, downstream_idx, num_paths, bif_ci, bif_di, num_trials, 0, nullptr, out);
+}
+
+int64_t cmf_network_num_groups(const cmf_network* net) { return net ? net->G : 0; }
 
 int cmf_network_destroy(cmf_network* net) {
     free_network(net);

@@ -147,6 +147,12 @@ struct View {
     REAL *pos_term, *neg_term, *qraw_r, *qraw_f, *lim;
     REAL *path_pos, *path_neg, *path_net, *path_raw;
     double* partials;
+    // groups (G > 0): independent sets of basins, each with its own sub-step
+    // count and time step (slot = group of the catchment).  Catchment kernels
+    // then run one block per table entry: block b covers catchments
+    // [blk_first[b], blk_end[b]) of a single group.
+    int64_t G;
+    const int32_t *cgroup, *blk_first, *blk_end;
 
     __device__ __forceinline__ int64_t el(int64_t t, int64_t i) const { return t * ts + i * cs; }
     __device__ __forceinline__ int64_t pel(int64_t t, int64_t p) const { return t * pts + p * pps; }
@@ -157,13 +163,21 @@ struct View {
     __device__ __forceinline__ X prm(const X* p, uint32_t bit, int64_t e, int64_t i) const {
         return bat(bit) ? p[e] : p[i];
     }
-    __device__ __forceinline__ REAL dt(int64_t t) const {
-        return has(CMF_PER_TRIAL_DT) ? time_step[t] : time_step[0];
+    // time-step slot of (trial t, catchment c): its group, its trial with
+    // per-trial dt, else the shared slot 0
+    __device__ __forceinline__ int64_t slot(int64_t t, int64_t c) const {
+        return G > 0 ? (int64_t)cgroup[c] : (has(CMF_PER_TRIAL_DT) ? t : 0);
     }
-    // With per-trial sub-step counts a trial past its count sits out.
-    __device__ __forceinline__ bool active(int64_t t, int substep) const {
-        if (substep < 0 || !has(CMF_PER_TRIAL_DT)) return true;
-        return substep < max_sub_steps[t];
+    __device__ __forceinline__ bool per_slot() const { return G > 0 || has(CMF_PER_TRIAL_DT); }
+    __device__ __forceinline__ REAL dt(int64_t t, int64_t c) const { return time_step[slot(t, c)]; }
+    // With per-trial or per-group sub-step counts a slot past its count sits out.
+    __device__ __forceinline__ bool active(int64_t t, int64_t c, int substep) const {
+        if (substep < 0 || !per_slot()) return true;
+        return substep < max_sub_steps[slot(t, c)];
+    }
+    // the last sub-step of (t, c)'s slot, or `shared_last` with one count for all
+    __device__ __forceinline__ bool last_substep(int64_t t, int64_t c, int substep, int shared_last) const {
+        return (per_slot() && substep >= 0) ? (substep == max_sub_steps[slot(t, c)] - 1) : (shared_last != 0);
     }
     // linear thread index -> (trial, item); `inner` when trials are the
     // fastest-varying dimension of the layout
@@ -174,6 +188,19 @@ struct View {
     }
     __device__ __forceinline__ void split_c(int64_t g, int64_t& t, int64_t& i) const {
         split(g, N, ts == 1, t, i);
+    }
+    // catchment of this thread in a catchment kernel (linear index `g` of the
+    // plain grid, or the group block table); false past the end
+    __device__ __forceinline__ bool catchment_of(int64_t g, int64_t& t, int64_t& i) const {
+        if (G > 0) {
+            const int64_t j = (int64_t)blk_first[blockIdx.x] + threadIdx.x;
+            t = 0;
+            i = j;
+            return j < blk_end[blockIdx.x];
+        }
+        if (g >= N * T) return false;
+        split_c(g, t, i);
+        return true;
     }
     __device__ __forceinline__ void split_p(int64_t g, int64_t& t, int64_t& p) const {
         split(g, P, pts == 1, t, p);

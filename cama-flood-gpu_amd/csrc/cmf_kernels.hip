@@ -63,16 +63,14 @@ __device__ __forceinline__ void store_if_changed(X* p, X old, X val) {
 // catchment, for the passes that sum them (k_outgoing_sum, k_limiter).
 template <class REAL, class STO, bool FUSED, bool TERMS>
 __global__ __launch_bounds__(kBlock) void k_outflow(View<REAL, STO> v, int substep) {
-    const int64_t g = xcd_block() * kBlock + threadIdx.x;
-    if (g >= v.N * v.T) return;
     int64_t t, i;
-    v.split_c(g, t, i);
-    if (!v.active(t, substep)) return;
+    if (!v.catchment_of(xcd_block() * kBlock + threadIdx.x, t, i)) return;
+    if (!v.active(t, i, substep)) return;
     const int64_t e = v.el(t, i);
     const int64_t d = v.dn[i];
     const int64_t ed = v.el(t, d);
     const bool mouth = (d == i);
-    const REAL time_step = v.dt(t);
+    const REAL time_step = v.dt(t, i);
     const REAL gravity = v.gravity;
 
     const REAL r_out = v.river_outflow[e];
@@ -216,12 +214,12 @@ __global__ __launch_bounds__(kBlock) void k_reservoir(View<REAL, STO> v, int sub
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (g >= v.R * v.T) return;
     const int64_t t = g / v.R, r = g - t * v.R;
-    if (!v.active(t, substep)) return;
     const int64_t c = v.res_idx[r];
+    if (!v.active(t, c, substep)) return;
     const int64_t d = v.dn[c];
     const bool mouth = (d == c);
     const int64_t e = v.el(t, c);
-    const REAL time_step = v.dt(t);
+    const REAL time_step = v.dt(t, c);
     const REAL rs = (REAL)v.river_storage[e];
     const REAL fs = (REAL)v.flood_storage[e];
     const REAL total = rs + fs;
@@ -290,11 +288,11 @@ __global__ __launch_bounds__(kBlock) void k_bif_outflow(View<REAL, STO> v, int s
     if (g >= v.P * v.T) return;
     int64_t t, p;
     v.split_p(g, t, p);
-    if (!v.active(t, substep)) return;
-    const int64_t pe = v.pel(t, p);
     const int64_t ci = v.bif_ci[p], di = v.bif_di[p];
+    if (!v.active(t, ci, substep)) return;
+    const int64_t pe = v.pel(t, p);
     const int64_t eci = v.el(t, ci), edi = v.el(t, di);
-    const REAL time_step = v.dt(t);
+    const REAL time_step = v.dt(t, ci);
     const REAL gravity = v.gravity;
     const REAL length = v.prm(v.bif_length, CMF_BATCHED_BIFURCATION_LENGTH, pe, p);
     const REAL w0 = v.wse[eci], w1 = v.wse[edi];
@@ -428,9 +426,9 @@ __global__ __launch_bounds__(kBlock) void k_bif_inflow(View<REAL, STO> v, int su
     if (g >= v.P * v.T) return;
     int64_t t, p;
     v.split_p(g, t, p);
-    if (!v.active(t, substep)) return;
-    const int64_t pe = v.pel(t, p);
     const int64_t ci = v.bif_ci[p], di = v.bif_di[p];
+    if (!v.active(t, ci, substep)) return;
+    const int64_t pe = v.pel(t, p);
     const REAL* lr = FUSED ? v.lim : v.limit_rate;
     const REAL l0 = lr[v.el(t, ci)], l1 = lr[v.el(t, di)];
     REAL raw_sum = (REAL)0;
@@ -551,9 +549,9 @@ __global__ __launch_bounds__(kBlock) void k_lim_touched(View<REAL, STO> v, const
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (g >= n_touched * v.T) return;
     const int64_t t = g / n_touched, x = touched[g - t * n_touched];
-    if (!v.active(t, substep)) return;
+    if (!v.active(t, x, substep)) return;
     const int64_t e = v.el(t, x);
-    const REAL dt = v.dt(t);
+    const REAL dt = v.dt(t, x);
     v.lim[e] = limit_recomputed<REAL, STO, CONTIG>(v, t, x, dt, v.qraw_r[e], v.qraw_f[e],
                                                    v.river_storage[e] + v.flood_storage[e]);
 }
@@ -911,7 +909,7 @@ __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v) {
         bool row_loaded = false;
         const STO fs_old = v.flood_storage[e], ps_old = v.protected_storage[e];
         const StageOut<REAL, STO> so = stage_one<REAL, STO, LFIX, LOG>(
-            v, e, j, rin, fin, v.river_outflow[e], v.flood_outflow[e], gb, v.dt(t), !is_lev, row, row_loaded, lp);
+            v, e, j, rin, fin, v.river_outflow[e], v.flood_outflow[e], gb, v.dt(t, j), !is_lev, row, row_loaded, lp);
         v.outgoing[e] = (STO)0.0;
         store_state(v, e, so, fs_old, ps_old);
     }
@@ -923,11 +921,9 @@ __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v) {
 // limiter, outflow.py:346-352).
 template <class REAL, class STO, bool CONTIG>
 __global__ __launch_bounds__(kBlock) void k_limiter(View<REAL, STO> v, int substep) {
-    const int64_t g = xcd_block() * kBlock + threadIdx.x;
-    if (g >= v.N * v.T) return;
     int64_t t, j;
-    v.split_c(g, t, j);
-    if (!v.active(t, substep)) return;
+    if (!v.catchment_of(xcd_block() * kBlock + threadIdx.x, t, j)) return;
+    if (!v.active(t, j, substep)) return;
     const int64_t e = v.el(t, j);
     STO og = outgoing_terms<REAL, STO, CONTIG>(v, t, j);
     if (v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[j] != v.ev_ptr[j + 1])
@@ -948,12 +944,11 @@ __global__ __launch_bounds__(kBlock) void k_stage_fused(View<REAL, STO> v, int s
 #pragma unroll
         for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
     }
-    if (g < v.N * v.T) {
-        int64_t t, j;
-        v.split_c(g, t, j);
-        if (v.active(t, substep)) {
+    int64_t t, j;
+    if (v.catchment_of(g, t, j)) {
+        if (v.active(t, j, substep)) {
             const int64_t e = v.el(t, j);
-            const REAL dt = v.dt(t);
+            const REAL dt = v.dt(t, j);
             const REAL l_own = v.lim[e];
             const REAL l_dn = v.lim[v.el(t, v.dn[j])];
             const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
@@ -985,8 +980,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_fused(View<REAL, STO> v, int s
                 if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_event_sum<REAL, STO>(v, t, j, gsum, false);
                 gb = (REAL)gsum;
             }
-            const bool store = (v.has(CMF_PER_TRIAL_DT) && substep >= 0) ? (substep == v.max_sub_steps[t] - 1)
-                                                                         : (store_inflow != 0);
+            const bool store = v.last_substep(t, j, substep, store_inflow);
             if (store) {
                 v.river_inflow[e] = ri;
                 v.flood_inflow[e] = fi;
@@ -1033,12 +1027,11 @@ __global__ __launch_bounds__(kBlock) void k_route(View<REAL, STO> v, int substep
 #pragma unroll
         for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
     }
-    if (g < NT) {
-        int64_t t, j;
-        v.split_c(g, t, j);
-        if (v.active(t, substep)) {
+    int64_t t, j;
+    if (v.catchment_of(g, t, j)) {
+        if (v.active(t, j, substep)) {
             const int64_t e = v.el(t, j);
-            const REAL dt = v.dt(t);
+            const REAL dt = v.dt(t, j);
             const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
             const STO rs_old = v.river_storage[e], fs_old = v.flood_storage[e];
             const REAL l_own = limit_recomputed<REAL, STO, CONTIG>(v, t, j, dt, qr, qf, rs_old + fs_old);
@@ -1081,8 +1074,7 @@ __global__ __launch_bounds__(kBlock) void k_route(View<REAL, STO> v, int substep
                 gb = (REAL)gsum;
             }
             // per-trial counts: each trial stores on its own last sub-step
-            const bool store = (v.has(CMF_PER_TRIAL_DT) && substep >= 0) ? (substep == v.max_sub_steps[t] - 1)
-                                                                         : (store_inflow != 0);
+            const bool store = v.last_substep(t, j, substep, store_inflow);
             const bool is_lev = LEV && v.is_levee[j];
             TableRow<REAL, LFIX> row;
             bool row_loaded = false;
@@ -1143,6 +1135,32 @@ __global__ __launch_bounds__(kBlock) void k_log_final(View<REAL, STO> v, int64_t
     }
 }
 
+// grouped networks: one block per group g sums the partials of the group's
+// blocks [gblk[g], gblk[g+1]) in a fixed order into log_sums[k][g *
+// log_buffer_size + step]; groups past their sub-step count are skipped.
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_log_final_grouped(View<REAL, STO> v, const int32_t* gblk,
+                                                              uint32_t mask, int substep) {
+    __shared__ double sh[kBlock];
+    const int64_t g = blockIdx.x;
+    if (substep >= 0 && substep >= v.max_sub_steps[g]) return;
+    const int step = v.current_step[0];
+    const int64_t b0 = gblk[g], b1 = gblk[g + 1];
+    for (int k = 0; k < CMF_LOG_COUNT; ++k) {
+        if (!(mask & (1u << k))) continue;
+        double acc = 0.0;
+        for (int64_t b = b0 + threadIdx.x; b < b1; b += kBlock) acc += v.partials[b * CMF_LOG_COUNT + k];
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = kBlock / 2; w >= 1; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) v.log_sums[k][g * v.logsize + step] += sh[0] * 1e-9;
+        __syncthreads();
+    }
+}
+
 // ============================================================= levee ====
 // per-op compute_levee_stage(_log): one thread per levee, on the state the
 // flood stage left.
@@ -1156,8 +1174,8 @@ __global__ __launch_bounds__(kBlock) void k_levee(View<REAL, STO> v, int substep
     }
     if (g < v.V * v.T) {
         const int64_t t = g / v.V, lv = g - t * v.V;
-        if (v.active(t, substep)) {
-            const int64_t c = v.levee_idx[lv];
+        const int64_t c = v.levee_idx[lv];
+        if (v.active(t, c, substep)) {
             const int64_t e = v.el(t, c);
             TableRow<REAL, LFIX> row;
             row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + c * v.L);
@@ -1179,36 +1197,53 @@ __global__ __launch_bounds__(kBlock) void k_levee(View<REAL, STO> v, int substep
 }
 
 // =============================================================== CFL ====
+// Sub-step count of one catchment (adaptive_time.py:30-48): dam-related
+// catchments count 1.
+template <class REAL, class STO>
+__device__ __forceinline__ int cfl_steps(const View<REAL, STO>& v, int64_t t, int64_t i, REAL outer) {
+    if (v.has(CMF_HAS_RESERVOIR) && v.is_dam_related[i]) return 1;
+    const int64_t e = v.el(t, i);
+    const REAL depth = vmax(v.river_depth[e], (REAL)0.01);
+    const REAL dist = v.prm(v.dist, CMF_BATCHED_DOWNSTREAM_DISTANCE, e, i);
+    const REAL dt = v.adaptive_factor * dist / dsqrt(v.gravity * depth);
+    const REAL dtc = vmin(dt, outer);
+    const REAL nf = floor(outer / dtc - (REAL)0.01) + (REAL)1.0;
+    return (int)nf;
+}
+
+__device__ __forceinline__ int block_max_int(int best) {
+    __shared__ int sh[kBlock / 64];
+    best = wave_max_int(best);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = best;
+    __syncthreads();
+    int m = sh[0];
+    for (int w = 1; w < kBlock / 64; ++w) m = max(m, sh[w]);
+    return m;
+}
+
 // grid (chunks, trials-or-1): block (bx, t) reduces catchments of trial t.
 // Without per-trial dt every trial folds into slot 0.
 template <class REAL, class STO>
 __global__ __launch_bounds__(kBlock) void k_cfl(View<REAL, STO> v) {
-    __shared__ int sh[kBlock / 64];
     const int64_t t = blockIdx.y;
     const REAL outer = v.outer_time_step[0];
     int best = 0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < v.N;
-         i += (int64_t)gridDim.x * kBlock) {
-        int steps = 1;
-        if (!(v.has(CMF_HAS_RESERVOIR) && v.is_dam_related[i])) {
-            const int64_t e = v.el(t, i);
-            const REAL depth = vmax(v.river_depth[e], (REAL)0.01);
-            const REAL dist = v.prm(v.dist, CMF_BATCHED_DOWNSTREAM_DISTANCE, e, i);
-            const REAL dt = v.adaptive_factor * dist / dsqrt(v.gravity * depth);
-            const REAL dtc = vmin(dt, outer);
-            const REAL nf = floor(outer / dtc - (REAL)0.01) + (REAL)1.0;
-            steps = (int)nf;
-        }
-        best = max(best, steps);
-    }
-    best = wave_max_int(best);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = best;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int m = sh[0];
-        for (int w = 1; w < kBlock / 64; ++w) m = max(m, sh[w]);
-        atomicMax(&v.max_sub_steps[v.has(CMF_PER_TRIAL_DT) ? t : 0], m);
-    }
+         i += (int64_t)gridDim.x * kBlock)
+        best = max(best, cfl_steps(v, t, i, outer));
+    const int m = block_max_int(best);
+    if (threadIdx.x == 0) atomicMax(&v.max_sub_steps[v.has(CMF_PER_TRIAL_DT) ? t : 0], m);
+}
+
+// grouped networks: one block per table entry (a single group), one
+// integer atomic max per block into the group's count
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_cfl_grouped(View<REAL, STO> v, const int32_t* blk_group) {
+    const REAL outer = v.outer_time_step[0];
+    int64_t t, i;
+    const int best = v.catchment_of(0, t, i) ? cfl_steps(v, t, i, outer) : 0;
+    const int m = block_max_int(best);
+    if (threadIdx.x == 0) atomicMax(&v.max_sub_steps[blk_group[blockIdx.x]], m);
 }
 
 // time_step[t] = outer / count[t], in double then rounded (cama_flood_model.py:127,132)

@@ -254,6 +254,24 @@ int cmf_network_create(int64_t num_catchments, const int32_t* downstream_idx,
                        int64_t num_paths, const int32_t* bifurcation_catchment_idx,
                        const int32_t* bifurcation_downstream_idx, int64_t num_trials,
                        cmf_network** out);
+/* The same, with the catchments partitioned into num_groups groups that each
+ * advance with their own CFL sub-step count and time step (independent basins
+ * of uneven sizes in one launch per pass).  Group g is the internal catchments
+ * [group_ptr[g], group_ptr[g + 1]) (host array, num_groups + 1 entries, from 0
+ * to num_catchments, no empty group); no downstream link or bifurcation path
+ * may cross groups.  One trial only.  With a grouped network, time_step and
+ * max_sub_steps hold one entry per group, and with the log module
+ * log_sums[k] holds num_groups * log_buffer_size entries (group g's sub-step s
+ * at g * log_buffer_size + s).  Only cmf_hip_advance and cmf_hip_substep take
+ * grouped networks; the per-op entries return CMF_ERR_UNSUPPORTED.  No
+ * reference counterpart: the reference's trials share one time step
+ * (cmfgpu/phys/triton/outflow.py:454) and its ranks one sub-step count
+ * (cmfgpu/models/cama_flood_model.py:119-120). */
+int cmf_network_create_grouped(int64_t num_catchments, const int32_t* downstream_idx,
+                               int64_t num_paths, const int32_t* bifurcation_catchment_idx,
+                               const int32_t* bifurcation_downstream_idx, int64_t num_trials,
+                               int64_t num_groups, const int64_t* group_ptr, cmf_network** out);
+int64_t cmf_network_num_groups(const cmf_network* net);
 int cmf_network_destroy(cmf_network* net);
 /* 1 when every upstream range is the identity (breadth-first layout). */
 int cmf_network_is_contiguous(const cmf_network* net);

@@ -132,3 +132,41 @@ def test_assemble_flags_follow_opened_modules():
     assert h.flags == want
     with pytest.raises(ValueError, match="unsupported modules"):
         assemble(src, ("base", "inflow"))
+
+
+def test_grouped_network_rejects_links_across_groups():
+    lib = _lib.lib()
+    P32 = ctypes.POINTER(ctypes.c_int32)
+    P64 = ctypes.POINTER(ctypes.c_int64)
+    dn = np.array([1, 1, 3, 3], np.int32)            # basins {0, 1} and {2, 3}
+    h = ctypes.c_void_p()
+    bad = np.array([0, 3, 4], np.int64)               # splits basin {0, 1}
+    rc = lib.cmf_network_create_grouped(4, dn.ctypes.data_as(P32), 0, None, None, 1, 2,
+                                        bad.ctypes.data_as(P64), ctypes.byref(h))
+    assert rc == _lib.CMF_ERR_INVALID and b"drains into another group" in lib.cmf_last_error()
+    empty = np.array([0, 0, 4], np.int64)
+    rc = lib.cmf_network_create_grouped(4, dn.ctypes.data_as(P32), 0, None, None, 1, 2,
+                                        empty.ctypes.data_as(P64), ctypes.byref(h))
+    assert rc == _lib.CMF_ERR_INVALID and b"empty" in lib.cmf_last_error()
+    ci, di = np.array([0], np.int32), np.array([2], np.int32)
+    good = np.array([0, 2, 4], np.int64)
+    rc = lib.cmf_network_create_grouped(4, dn.ctypes.data_as(P32), 1, ci.ctypes.data_as(P32),
+                                        di.ctypes.data_as(P32), 1, 2, good.ctypes.data_as(P64), ctypes.byref(h))
+    assert rc == _lib.CMF_ERR_INVALID and b"joins two groups" in lib.cmf_last_error()
+    rc = lib.cmf_network_create_grouped(4, dn.ctypes.data_as(P32), 0, None, None, 2, 2,
+                                        good.ctypes.data_as(P64), ctypes.byref(h))
+    assert rc == _lib.CMF_ERR_UNSUPPORTED
+
+
+def test_group_basins_orders_groups_contiguously():
+    from cmfgpu_amd.models.cama_flood_model import _group_basins
+    from cmfgpu_amd import network
+    dn = np.array([1, 1, 3, 3, 5, 5], np.int32)      # three basins
+    order = network.basin_level_order(dn)
+    labels = np.array([7, 7, 3, 3, 7, 7])
+    perm, gptr, lab = _group_basins(order, labels, dn)
+    assert lab.tolist() == [3, 7]
+    assert gptr.tolist() == [0, 2, 6]
+    assert perm.tolist() == [2, 3, 0, 1, 4, 5]
+    with pytest.raises(ValueError, match="union of basins"):
+        _group_basins(order, np.array([1, 2, 3, 3, 4, 4]), dn)
