@@ -120,6 +120,54 @@ def fp64_errors(net, src, device: str, layout: str, days: int) -> dict:
             "fields_rel_l2": {k: float(f"{v:.4g}") for k, v in out.items()}}
 
 
+def basins_line(args, device: str) -> dict:
+    """Uneven basins (log-uniform sizes 500..400,000, plain physics): every
+    basin advances with its own CFL sub-step count and dt in one launch per
+    pass (groups), against the same network with one shared sub-step count
+    (the reference's semantics).  `value` counts the sub-steps each basin
+    really takes."""
+    import numpy as np
+    import torch
+    from cmfgpu_amd import synthetic
+    from cmfgpu_amd.models import CaMaFlood
+    t0 = time.perf_counter()
+    bs = synthetic.build_basins(args.basins, scale=args.basins_scale)
+    src = dict(bs.network.fields)
+    src["runoff"] = synthetic.Runoff(src["catchment_area"], 2002).next_day()
+    n = bs.network.num_catchments
+    log(f"basins: {args.basins} basins, {n} catchments ({time.perf_counter() - t0:.1f} s)")
+    sizes = np.diff(bs.basin_ptr)
+    out = {"basins": args.basins, "catchments": int(n)}
+    for grouped in (True, False):
+        m = CaMaFlood(src, opened_modules=("base", "adaptive_time"), device=device, step_duration=1800.0,
+                      groups=bs.group if grouped else None)
+        for _ in range(max(args.warmup // 4, 2)):
+            m.step_advance()
+        torch.cuda.synchronize()
+        steps = max(args.steps // 4, 4)
+        work = 0
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            cnt = m.step_advance()
+            if grouped:
+                work += int((sizes * m.group_sub_steps()).sum())
+            else:
+                work += n * cnt
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t1
+        key = "per_basin_dt" if grouped else "shared_dt"
+        out[key] = {"outer_steps_per_s": steps / el, "catchment_steps_per_s": work / el}
+        m.close()
+        del m
+        torch.cuda.empty_cache()
+    out["value"] = out["per_basin_dt"]["catchment_steps_per_s"]
+    out["unit"] = "catchment-steps/s"
+    out["speedup_vs_shared_dt"] = round(out["per_basin_dt"]["outer_steps_per_s"] / out["shared_dt"]["outer_steps_per_s"], 3)
+    log(f"basins: per-basin dt {out['per_basin_dt']['outer_steps_per_s']:.1f} outer steps/s, shared dt "
+        f"{out['shared_dt']['outer_steps_per_s']:.1f} ({out['speedup_vs_shared_dt']}x)")
+    return out
+
+
 def run_steps(model, steps: int, per_trial: bool):
     """Advance `steps` outer steps; returns member-catchment-sub-steps done."""
     import torch
@@ -210,6 +258,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="network size factor (1.0 = 1.6M catchments)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--basins", type=int, default=400, help="uneven basins of the grouped line (0: skip)")
+    ap.add_argument("--basins-scale", type=float, default=1.0, help="basin size factor (1.0: about 20M catchments)")
     ap.add_argument("--error-days", type=int, default=1,
                     help="simulated days of the FP32-vs-FP64 error run (0: skip)")
     ap.add_argument("--layout", default="basin", choices=("basin", "level", "network"),
@@ -302,6 +352,10 @@ def main():
             f"member-catchment-steps/s ({bvalue / value:.2f}x plain)")
         bm.close()
 
+    basins = None
+    if args.basins > 0:
+        basins = basins_line(args, device)
+
     errors = None
     if args.error_days > 0:
         errors = fp64_errors(net, src, device, args.layout, args.error_days)
@@ -315,7 +369,8 @@ def main():
                    "catchments": net.num_catchments, "bifurcation_paths": net.num_paths,
                    "levees": net.num_levees, "flood_levels": 10, "outer_step_s": 1800.0,
                    "substeps_timed": int(substeps), "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "error_vs_fp64": errors,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "basins": basins,
+        "error_vs_fp64": errors,
     }
     print(json.dumps(line), flush=True)
 

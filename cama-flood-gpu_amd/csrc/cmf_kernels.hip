@@ -584,14 +584,21 @@ struct StageOut {
     REAL rdep, fdep, ffr, ca;
 };
 
-// Storage update and flood-stage diagnosis of one catchment (values only).
-template <class REAL, class STO, int LFIX, bool LOG>
-__device__ __forceinline__ StageOut<REAL, STO> stage_one(
-    const View<REAL, STO>& v, int64_t e, int64_t i, REAL rin, REAL fin, REAL rout, REAL fout, REAL gbif,
-    REAL time_step, bool non_levee, TableRow<REAL, LFIX>& row, bool& row_loaded, LogPart& lp) {
-    StageOut<REAL, STO> o;
-    const STO rs = v.river_storage[e], fs = v.flood_storage[e], ps = v.protected_storage[e];
-    const REAL ro = v.prm(v.runoff, CMF_BATCHED_RUNOFF, e, i);
+// Storage update of one catchment (storage.py:81-94): the new total storage
+// and, with LOG, the sums that depend only on it.
+template <class REAL, class STO>
+struct StagePre {
+    STO total_hp;
+    STO fs_old, ps_old;
+};
+
+template <class REAL, class STO, bool LOG>
+__device__ __forceinline__ StagePre<REAL, STO> stage_update_p(
+    STO rs, STO fs, STO ps, REAL ro, REAL rin, REAL fin, REAL rout, REAL fout, REAL gbif, REAL time_step,
+    LogPart& lp) {
+    StagePre<REAL, STO> o;
+    o.fs_old = fs;
+    o.ps_old = ps;
     const REAL inflow_extra = (REAL)0.0;
 
     STO rsu = rs + (STO)((rin - rout) * time_step);
@@ -600,13 +607,48 @@ __device__ __forceinline__ StageOut<REAL, STO> stage_one(
     rsu = (fsu < (STO)0.0) ? vmax(rsu + fsu, (STO)0.0) : rsu;
     fsu = vmax(fsu, (STO)0.0);
     const STO total_next = rsu + fsu + ps + (STO)(ro * time_step) + (STO)(inflow_extra * time_step);
-    const STO total_hp = vmax(total_next, (STO)0.0);
-    const REAL total = (REAL)total_hp;
-    o.total_hp = total_hp;
+    o.total_hp = vmax(total_next, (STO)0.0);
+    if (LOG) {
+        const STO total_pre = rs + fs + ps;
+        lp.q[CMF_LOG_STORAGE_PRE] += (double)total_pre;
+        lp.q[CMF_LOG_STORAGE_NEXT] += (double)total_next;
+        lp.q[CMF_LOG_STORAGE_NEW] += (double)o.total_hp;
+        lp.q[CMF_LOG_INFLOW] += (double)((rin + fin + inflow_extra) * time_step);
+        lp.q[CMF_LOG_OUTFLOW] += (double)((rout + fout) * time_step);
+        lp.q[CMF_LOG_INFLOW_ERROR] += (double)(total_pre - total_next
+            + (STO)((rin + fin + ro + inflow_extra - rout - fout - gbif) * time_step));
+    }
+    return o;
+}
 
+template <class REAL, class STO, bool LOG>
+__device__ __forceinline__ StagePre<REAL, STO> stage_update(
+    const View<REAL, STO>& v, int64_t e, int64_t i, REAL rin, REAL fin, REAL rout, REAL fout, REAL gbif,
+    REAL time_step, LogPart& lp) {
+    return stage_update_p<REAL, STO, LOG>(v.river_storage[e], v.flood_storage[e], v.protected_storage[e],
+                                          v.prm(v.runoff, CMF_BATCHED_RUNOFF, e, i), rin, fin, rout, fout, gbif,
+                                          time_step, lp);
+}
+
+// Whether the new total exceeds the channel (the table scan is needed).
+template <class REAL, class STO>
+__device__ __forceinline__ bool stage_is_wet(const View<REAL, STO>& v, int64_t e, int64_t i, STO total_hp) {
     const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, i);
     const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, i);
     const REAL rl = v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, i);
+    return (REAL)total_hp > rl * rw * rh;
+}
+
+// Flood-stage diagnosis of one catchment from its new total storage
+// (storage.py:105-181), with LOG the sums of the stage (non-levee catchments).
+// `ca_pre` >= 0 is the catchment area already read (else read when needed).
+template <class REAL, class STO, int LFIX, bool LOG>
+__device__ __forceinline__ StageOut<REAL, STO> stage_diag_p(
+    const View<REAL, STO>& v, int64_t e, int64_t i, STO total_hp, REAL rl, REAL rw, REAL rh, REAL ca_pre,
+    bool non_levee, TableRow<REAL, LFIX>& row, bool& row_loaded, LogPart& lp) {
+    StageOut<REAL, STO> o;
+    const REAL total = (REAL)total_hp;
+    o.total_hp = total_hp;
     const REAL river_max_storage = rl * rw * rh;
     const int L = LFIX > 0 ? LFIX : (int)v.L;
     o.ca = (REAL)0;
@@ -617,9 +659,9 @@ __device__ __forceinline__ StageOut<REAL, STO> stage_one(
         o.ffr = (REAL)0.0;
         o.rdep = (REAL)o.rsf_hp / (rl * rw);
         o.fsf_hp = vmax(total_hp - o.rsf_hp, (STO)0.0);
-        if (LOG) o.ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, i);
+        if (LOG) o.ca = ca_pre >= (REAL)0 ? ca_pre : v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, i);
     } else {
-        o.ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, i);
+        o.ca = ca_pre >= (REAL)0 ? ca_pre : v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, i);
         const REAL ca = o.ca;
         if (!row_loaded) {
             row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + i * v.L);
@@ -669,25 +711,34 @@ __device__ __forceinline__ StageOut<REAL, STO> stage_one(
         o.fsf_hp = vmax(total_hp - o.rsf_hp, (STO)0.0);
     }
 
-    if (LOG) {
-        const STO total_pre = rs + fs + ps;
-        lp.q[CMF_LOG_STORAGE_PRE] += (double)total_pre;
-        lp.q[CMF_LOG_STORAGE_NEXT] += (double)total_next;
-        lp.q[CMF_LOG_STORAGE_NEW] += (double)total_hp;
-        lp.q[CMF_LOG_INFLOW] += (double)((rin + fin + inflow_extra) * time_step);
-        lp.q[CMF_LOG_OUTFLOW] += (double)((rout + fout) * time_step);
-        lp.q[CMF_LOG_INFLOW_ERROR] += (double)(total_pre - total_next
-            + (STO)((rin + fin + ro + inflow_extra - rout - fout - gbif) * time_step));
-        if (non_levee) {
-            const STO stage_new = o.rsf_hp + o.fsf_hp;
-            lp.q[CMF_LOG_STORAGE_STAGE] += (double)stage_new;
-            lp.q[CMF_LOG_RIVER_STORAGE] += (double)(REAL)o.rsf_hp;
-            lp.q[CMF_LOG_FLOOD_STORAGE] += (double)(REAL)o.fsf_hp;
-            lp.q[CMF_LOG_FLOOD_AREA] += (double)(o.ffr * o.ca);
-            lp.q[CMF_LOG_STAGE_ERROR] += (double)(stage_new - total_hp);
-        }
+    if (LOG && non_levee) {
+        const STO stage_new = o.rsf_hp + o.fsf_hp;
+        lp.q[CMF_LOG_STORAGE_STAGE] += (double)stage_new;
+        lp.q[CMF_LOG_RIVER_STORAGE] += (double)(REAL)o.rsf_hp;
+        lp.q[CMF_LOG_FLOOD_STORAGE] += (double)(REAL)o.fsf_hp;
+        lp.q[CMF_LOG_FLOOD_AREA] += (double)(o.ffr * o.ca);
+        lp.q[CMF_LOG_STAGE_ERROR] += (double)(stage_new - total_hp);
     }
     return o;
+}
+
+template <class REAL, class STO, int LFIX, bool LOG>
+__device__ __forceinline__ StageOut<REAL, STO> stage_diag(
+    const View<REAL, STO>& v, int64_t e, int64_t i, STO total_hp, bool non_levee, TableRow<REAL, LFIX>& row,
+    bool& row_loaded, LogPart& lp) {
+    return stage_diag_p<REAL, STO, LFIX, LOG>(
+        v, e, i, total_hp, v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, i),
+        v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, i), v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, i),
+        (REAL)-1, non_levee, row, row_loaded, lp);
+}
+
+// Storage update and flood-stage diagnosis of one catchment (values only).
+template <class REAL, class STO, int LFIX, bool LOG>
+__device__ __forceinline__ StageOut<REAL, STO> stage_one(
+    const View<REAL, STO>& v, int64_t e, int64_t i, REAL rin, REAL fin, REAL rout, REAL fout, REAL gbif,
+    REAL time_step, bool non_levee, TableRow<REAL, LFIX>& row, bool& row_loaded, LogPart& lp) {
+    const StagePre<REAL, STO> pre = stage_update<REAL, STO, LOG>(v, e, i, rin, fin, rout, fout, gbif, time_step, lp);
+    return stage_diag<REAL, STO, LFIX, LOG>(v, e, i, pre.total_hp, non_levee, row, row_loaded, lp);
 }
 
 // ------------------------------------------------------ levee partition --
@@ -700,17 +751,10 @@ struct LeveeOut {
 // Levee stage of catchment c (levee row lv) from its post-flood-stage state
 // (triton/levee.py:62-304 with the partition of :18-59).
 template <class REAL, class STO, int LFIX>
-__device__ __forceinline__ LeveeOut<REAL, STO> levee_one(
-    const View<REAL, STO>& v, int64_t e, int64_t c, int64_t lv, int64_t le, STO rs_hp, STO fs_hp,
-    REAL fdep_curr, REAL rdep_curr, REAL ffr_curr, const TableRow<REAL, LFIX>& row) {
+__device__ __forceinline__ LeveeOut<REAL, STO> levee_one_p(
+    const View<REAL, STO>& v, REAL rl, REAL rw, REAL rh, REAL ca, REAL crown, REAL lfrac, REAL base,
+    STO rs_hp, STO fs_hp, REAL fdep_curr, REAL rdep_curr, REAL ffr_curr, const TableRow<REAL, LFIX>& row) {
     const int L = LFIX > 0 ? LFIX : (int)v.L;
-    const REAL rl = v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, c);
-    const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, c);
-    const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, c);
-    const REAL ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, c);
-    const REAL crown = v.prm(v.levee_crown, CMF_BATCHED_LEVEE_CROWN_HEIGHT, le, lv);
-    const REAL lfrac = v.prm(v.levee_frac, CMF_BATCHED_LEVEE_FRACTION, le, lv);
-    const REAL base = v.prm(v.levee_base, CMF_BATCHED_LEVEE_BASE_HEIGHT, le, lv);
     const STO total_hp = rs_hp + fs_hp;
     const REAL rs_c = (REAL)rs_hp, fs_c = (REAL)fs_hp, total = (REAL)total_hp;
 
@@ -847,6 +891,18 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one(
     return o;
 }
 
+// levee_one_p with the catchment and levee parameters read from the fields
+template <class REAL, class STO, int LFIX>
+__device__ __forceinline__ LeveeOut<REAL, STO> levee_one(
+    const View<REAL, STO>& v, int64_t e, int64_t c, int64_t lv, int64_t le, STO rs_hp, STO fs_hp,
+    REAL fdep_curr, REAL rdep_curr, REAL ffr_curr, const TableRow<REAL, LFIX>& row) {
+    return levee_one_p<REAL, STO, LFIX>(
+        v, v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, c), v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, c),
+        v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, c), v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, c),
+        v.prm(v.levee_crown, CMF_BATCHED_LEVEE_CROWN_HEIGHT, le, lv), v.prm(v.levee_frac, CMF_BATCHED_LEVEE_FRACTION, le, lv),
+        v.prm(v.levee_base, CMF_BATCHED_LEVEE_BASE_HEIGHT, le, lv), rs_hp, fs_hp, fdep_curr, rdep_curr, ffr_curr, row);
+}
+
 template <class REAL, class STO>
 __device__ __forceinline__ void levee_log(const LeveeOut<REAL, STO>& o, STO total_hp, REAL ca, LogPart& lp) {
     const STO stage_new = o.r_hp + o.f_hp + o.p_hp;
@@ -936,6 +992,12 @@ __global__ __launch_bounds__(kBlock) void k_limiter(View<REAL, STO> v, int subst
 // fused, after k_limiter: limited outflow, inflow sums, bifurcation totals,
 // storage update and flood stage, and with LEV the levee stage of levee
 // catchments, in one pass.
+//
+// The pass is bound by memory latency: every field of the catchment that
+// does not depend on another load is read up front (one round trip), then
+// the downstream limiter, the upstream ranges, the bifurcation events and,
+// for catchments near bankfull or with a levee, the floodplain row and levee
+// parameters (a second round trip); the arithmetic follows.
 template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
 __global__ __launch_bounds__(kBlock) void k_stage_fused(View<REAL, STO> v, int substep, int store_inflow) {
     const int64_t g = xcd_block() * kBlock + threadIdx.x;
@@ -945,69 +1007,91 @@ __global__ __launch_bounds__(kBlock) void k_stage_fused(View<REAL, STO> v, int s
         for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
     }
     int64_t t, j;
-    if (v.catchment_of(g, t, j)) {
-        if (v.active(t, j, substep)) {
-            const int64_t e = v.el(t, j);
-            const REAL dt = v.dt(t, j);
-            const REAL l_own = v.lim[e];
-            const REAL l_dn = v.lim[v.el(t, v.dn[j])];
-            const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
-            const REAL rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
-            const REAL fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
-            v.river_outflow[e] = rout;
-            v.flood_outflow[e] = fout;
-            // upstream u has downstream j: its negative flow is limited by j
-            const bool res = v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
-            STO ri = (STO)0, fi = (STO)0;
-            STO rt = res ? v.res_inflow[e] : (STO)0;
-            const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
-            for (int32_t k = b; k < en; ++k) {
-                const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
-                if (CONTIG && u == j) continue;
-                const int64_t eu = v.el(t, u);
-                const REAL lu = v.lim[eu];
-                const REAL ur0 = v.qraw_r[eu], uf0 = v.qraw_f[eu];
-                const REAL ur1 = (ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own;
-                const REAL uf1 = (uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own;
-                ri += (STO)ur1;
-                fi += (STO)uf1;
-                if (res) rt += (STO)(ur1 + uf1);
-            }
-            if (res) v.res_inflow[e] = rt;
-            STO gsum = (STO)0;
-            REAL gb = (REAL)0.0;
-            if (v.has(CMF_HAS_BIFURCATION)) {
-                if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_event_sum<REAL, STO>(v, t, j, gsum, false);
-                gb = (REAL)gsum;
-            }
-            const bool store = v.last_substep(t, j, substep, store_inflow);
-            if (store) {
-                v.river_inflow[e] = ri;
-                v.flood_inflow[e] = fi;
-                if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = gsum;
-            }
-            const bool lev_cell = v.has(CMF_HAS_LEVEE) && v.is_levee[j];
-            TableRow<REAL, LFIX> row;
-            bool row_loaded = false;
-            const STO fs_old = v.flood_storage[e], ps_old = v.protected_storage[e];
-            const StageOut<REAL, STO> so = stage_one<REAL, STO, LFIX, LOG>(
-                v, e, j, (REAL)ri, (REAL)fi, rout, fout, gb, dt, !lev_cell, row, row_loaded, lp);
-            if (LEV && lev_cell) {
-                const int32_t lv = v.levee_index[j];
-                if (!row_loaded) row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
-                const LeveeOut<REAL, STO> lo = levee_one<REAL, STO, LFIX>(
-                    v, e, j, lv, t * v.V + lv, so.rsf_hp, so.fsf_hp, so.fdep, so.rdep, so.ffr, row);
-                if (LOG) levee_log(lo, so.rsf_hp + so.fsf_hp, v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, j), lp);
-                v.river_storage[e] = lo.r_hp;
-                v.flood_storage[e] = lo.f_hp;
-                v.protected_storage[e] = lo.p_hp;
-                v.river_depth[e] = lo.r_dph;
-                v.flood_depth[e] = lo.f_dph;
-                v.protected_depth[e] = lo.p_dph;
-                v.flood_fraction[e] = lo.f_frc;
-            } else {
-                store_state(v, e, so, fs_old, ps_old);
-            }
+    if (v.catchment_of(g, t, j) && v.active(t, j, substep)) {
+        const int64_t e = v.el(t, j);
+        const REAL dt = v.dt(t, j);
+        // round trip 1
+        const int64_t d = v.dn[j];
+        const REAL l_own = v.lim[e];
+        const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
+        const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
+        const STO rs = v.river_storage[e], fs = v.flood_storage[e], ps = v.protected_storage[e];
+        const REAL ro = v.prm(v.runoff, CMF_BATCHED_RUNOFF, e, j);
+        const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, j);
+        const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, j);
+        const REAL rl = v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, j);
+        const int32_t lv = LEV ? v.levee_index[j] : -1;
+        const bool lev_cell = LEV ? (lv >= 0) : (v.has(CMF_HAS_LEVEE) && v.is_levee[j]);
+        const bool res = v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
+        const bool events = v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[j] != v.ev_ptr[j + 1];
+        // round trip 2
+        const REAL l_dn = v.lim[v.el(t, d)];
+        TableRow<REAL, LFIX> row;
+        bool row_loaded = false;
+        REAL ca = (REAL)-1;
+        const bool near = lev_cell || (REAL)(rs + fs + ps) > (REAL)0.9 * (rl * rw * rh);
+        if (LFIX > 0 && near) {
+            row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
+            row_loaded = true;
+            ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, j);
+        }
+        REAL crown = (REAL)0, lfrac = (REAL)0, lbase = (REAL)0;
+        if (LEV && lev_cell) {
+            const int64_t le = t * v.V + lv;
+            crown = v.prm(v.levee_crown, CMF_BATCHED_LEVEE_CROWN_HEIGHT, le, lv);
+            lfrac = v.prm(v.levee_frac, CMF_BATCHED_LEVEE_FRACTION, le, lv);
+            lbase = v.prm(v.levee_base, CMF_BATCHED_LEVEE_BASE_HEIGHT, le, lv);
+        }
+        const REAL rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
+        const REAL fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
+        v.river_outflow[e] = rout;
+        v.flood_outflow[e] = fout;
+        // upstream u has downstream j: its negative flow is limited by j
+        STO ri = (STO)0, fi = (STO)0;
+        STO rt = res ? v.res_inflow[e] : (STO)0;
+        for (int32_t k = b; k < en; ++k) {
+            const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+            if (CONTIG && u == j) continue;
+            const int64_t eu = v.el(t, u);
+            const REAL lu = v.lim[eu];
+            const REAL ur0 = v.qraw_r[eu], uf0 = v.qraw_f[eu];
+            const REAL ur1 = (ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own;
+            const REAL uf1 = (uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own;
+            ri += (STO)ur1;
+            fi += (STO)uf1;
+            if (res) rt += (STO)(ur1 + uf1);
+        }
+        if (res) v.res_inflow[e] = rt;
+        STO gsum = (STO)0;
+        REAL gb = (REAL)0.0;
+        if (v.has(CMF_HAS_BIFURCATION)) {
+            if (events) gsum = bif_event_sum<REAL, STO>(v, t, j, gsum, false);
+            gb = (REAL)gsum;
+        }
+        if (v.last_substep(t, j, substep, store_inflow)) {
+            v.river_inflow[e] = ri;
+            v.flood_inflow[e] = fi;
+            if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = gsum;
+        }
+        const StagePre<REAL, STO> pre = stage_update_p<REAL, STO, LOG>(rs, fs, ps, ro, (REAL)ri, (REAL)fi, rout, fout,
+                                                                       gb, dt, lp);
+        const StageOut<REAL, STO> so = stage_diag_p<REAL, STO, LFIX, LOG>(v, e, j, pre.total_hp, rl, rw, rh, ca,
+                                                                         !lev_cell, row, row_loaded, lp);
+        if (LEV && lev_cell) {
+            if (!row_loaded) row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
+            const REAL cav = ca >= (REAL)0 ? ca : v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, j);
+            const LeveeOut<REAL, STO> lo = levee_one_p<REAL, STO, LFIX>(
+                v, rl, rw, rh, cav, crown, lfrac, lbase, so.rsf_hp, so.fsf_hp, so.fdep, so.rdep, so.ffr, row);
+            if (LOG) levee_log(lo, so.rsf_hp + so.fsf_hp, cav, lp);
+            v.river_storage[e] = lo.r_hp;
+            v.flood_storage[e] = lo.f_hp;
+            v.protected_storage[e] = lo.p_hp;
+            v.river_depth[e] = lo.r_dph;
+            v.flood_depth[e] = lo.f_dph;
+            v.protected_depth[e] = lo.p_dph;
+            v.flood_fraction[e] = lo.f_frc;
+        } else {
+            store_state(v, e, so, pre.fs_old, pre.ps_old);
         }
     }
     if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
