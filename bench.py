@@ -168,6 +168,37 @@ def basins_line(args, device: str) -> dict:
     return out
 
 
+def graph_line(args, device: str) -> dict:
+    """Launch-bound case: the 60,000-catchment plain network, sub-step loops
+    launched eagerly vs replayed from captured HIP graphs."""
+    import torch
+    from cmfgpu_amd import synthetic
+    from cmfgpu_amd.models import CaMaFlood
+    net = synthetic.build_config(0)
+    src = dict(net.fields)
+    src["runoff"] = synthetic.Runoff(src["catchment_area"], 2000).next_day()
+    out = {"workload": "60,000-catchment plain network", "catchments": net.num_catchments}
+    for use_graph in (False, True):
+        m = CaMaFlood(src, opened_modules=("base", "adaptive_time"), device=device, step_duration=1800.0,
+                      use_graph=use_graph)
+        for _ in range(8):
+            m.step_advance()
+        torch.cuda.synchronize()
+        steps = max(args.steps, 24)
+        work = 0
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            work += m.step_advance() * net.num_catchments
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t1
+        out["graph" if use_graph else "eager"] = {"outer_steps_per_s": steps / el, "catchment_steps_per_s": work / el}
+        m.close()
+    out["speedup"] = round(out["graph"]["outer_steps_per_s"] / out["eager"]["outer_steps_per_s"], 3)
+    log(f"graph: {out['graph']['outer_steps_per_s']:.0f} vs eager {out['eager']['outer_steps_per_s']:.0f} "
+        f"outer steps/s ({out['speedup']}x) on {net.num_catchments} catchments")
+    return out
+
+
 def run_steps(model, steps: int, per_trial: bool):
     """Advance `steps` outer steps; returns member-catchment-sub-steps done."""
     import torch
@@ -260,6 +291,7 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--basins", type=int, default=400, help="uneven basins of the grouped line (0: skip)")
     ap.add_argument("--basins-scale", type=float, default=1.0, help="basin size factor (1.0: about 20M catchments)")
+    ap.add_argument("--graph", type=int, default=1, help="eager vs HIP-graph line on the small network (0: skip)")
     ap.add_argument("--error-days", type=int, default=1,
                     help="simulated days of the FP32-vs-FP64 error run (0: skip)")
     ap.add_argument("--layout", default="basin", choices=("basin", "level", "network"),
@@ -355,6 +387,7 @@ def main():
     basins = None
     if args.basins > 0:
         basins = basins_line(args, device)
+    graph = graph_line(args, device) if args.graph else None
 
     errors = None
     if args.error_days > 0:
@@ -369,7 +402,7 @@ def main():
                    "catchments": net.num_catchments, "bifurcation_paths": net.num_paths,
                    "levees": net.num_levees, "flood_levels": 10, "outer_step_s": 1800.0,
                    "substeps_timed": int(substeps), "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "basins": basins,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "basins": basins, "graph": graph,
         "error_vs_fp64": errors,
     }
     print(json.dumps(line), flush=True)

@@ -165,6 +165,8 @@ def _declare(lib):
         "cmf_hip_substep": (ctypes.c_int, [_vp, p(CmfFields), ctypes.c_int32, ctypes.c_int32, _vp]),
         "cmf_hip_advance": (ctypes.c_int, [_vp, p(CmfFields), ctypes.c_double, ctypes.c_int32,
                                            i32p, _vp]),
+        "cmf_hip_advance_graph": (ctypes.c_int, [_vp, p(CmfFields), ctypes.c_double, ctypes.c_int32,
+                                                 i32p, _vp]),
         "cmf_hip_substep_timed": (ctypes.c_int, [_vp, p(CmfFields), ctypes.c_int32, ctypes.c_int32, _vp,
                                                  f64p, p(ctypes.c_char_p), ctypes.c_int32, i32p]),
     }

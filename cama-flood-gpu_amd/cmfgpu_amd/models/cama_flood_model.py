@@ -91,7 +91,7 @@ class CaMaFlood:
                  log_buffer_size: int = 20000, step_duration: float = 1800.0, num_sub_steps: int = 1,
                  start_time: datetime = datetime(2000, 1, 1), log_path: str | Path | None = None,
                  mode: str = "fused", gravity: float = 9.8, adaptive_time_factor: float = 0.7,
-                 min_kinematic_slope: float = 1e-5, groups=None):
+                 min_kinematic_slope: float = 1e-5, groups=None, use_graph: bool = False):
         if mode not in ("fused", "ops"):
             raise ValueError("mode must be 'fused' or 'ops'")
         if layout not in ("basin", "level", "network"):
@@ -111,6 +111,9 @@ class CaMaFlood:
             raise ValueError("the text log is global; read per-group sums with log_sums()")
         self.opened_modules = opened
         self.mode = mode
+        if use_graph and mode != "fused":
+            raise ValueError("graph replay needs the fused schedule")
+        self.use_graph = bool(use_graph)
         self.device = torch.device(device)
         self.T = int(num_trials)
         self.per_trial = bool(per_trial_time_step)
@@ -375,9 +378,10 @@ class CaMaFlood:
         """Advance one outer step; returns the number of sub-steps used."""
         if self.mode == "fused":
             count = ctypes.c_int32(0)
-            _lib.check(_lib.lib().cmf_hip_advance(self.network_handle, self.field_table_ref(), self.step_duration,
-                                                  self.num_sub_steps, ctypes.byref(count), _lib.current_stream()),
-                       "cmf_hip_advance")
+            fn = _lib.lib().cmf_hip_advance_graph if self.use_graph else _lib.lib().cmf_hip_advance
+            _lib.check(fn(self.network_handle, self.field_table_ref(), self.step_duration,
+                          self.num_sub_steps, ctypes.byref(count), _lib.current_stream()),
+                       "cmf_hip_advance_graph" if self.use_graph else "cmf_hip_advance")
             n = int(count.value)
         else:
             n = self._step_ops()

@@ -64,6 +64,12 @@ struct cmf_network {
     int32_t* blk_end = nullptr;
     int32_t* blk_group = nullptr;
     int32_t* gblk_ptr = nullptr;      // first block of each group, G + 1 entries
+    // captured sub-step loops (cmf_hip_advance_graph): one executable graph
+    // per sub-step count, valid for the field table they were captured with
+    hipStream_t capture_stream = nullptr;
+    cmf_fields graph_fields;
+    bool graph_fields_set = false;
+    std::vector<std::pair<int, hipGraphExec_t>> graphs;
 };
 
 namespace {
@@ -77,6 +83,8 @@ void free_network(cmf_network* net) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (net->host_counts) (void)hipHostFree(net->host_counts);
+    for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
+    if (net->capture_stream) (void)hipStreamDestroy(net->capture_stream);
     delete net;
 }
 
@@ -628,9 +636,65 @@ __global__ void k_fill_int(int32_t* p, int64_t n, int32_t value) {
     if (t < n) p[t] = value;
 }
 
+// The sub-step loop of one outer step: `count` fused sub-steps, the inflow
+// stored on the last, current_step left at count - 1.
+template <class REAL, class STO>
+int substep_loop(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int count, hipStream_t s) {
+    for (int k = 0; k < count; ++k) {
+        if (v.has(CMF_HAS_LOG)) {
+            hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, f->current_step, (int64_t)1, (int32_t)k);
+        }
+        // with per-trial counts each trial stores its inflow on its own last
+        // sub-step (k_stage checks); a shared count stores on the last one
+        int rc = fused_substep(net, v, f, k, (k == count - 1) ? 1 : 0, s);
+        if (rc) return rc;
+    }
+    if (f->current_step)
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, f->current_step, (int64_t)1, (int32_t)(count - 1));
+    CMF_HIP_CHECK(hipGetLastError());
+    return CMF_OK;
+}
+
+// The same loop replayed from a graph captured once per sub-step count (on a
+// private stream) and launched on `s`.  A changed field table drops the cache.
+template <class REAL, class STO>
+int substep_loop_graph(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int count, hipStream_t s) {
+    if (!net->graph_fields_set || std::memcmp(&net->graph_fields, f, sizeof(cmf_fields)) != 0) {
+        for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
+        net->graphs.clear();
+        std::memcpy(&net->graph_fields, f, sizeof(cmf_fields));
+        net->graph_fields_set = true;
+    }
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : net->graphs)
+        if (g.first == count) exec = g.second;
+    if (!exec) {
+        if (!net->capture_stream) CMF_HIP_CHECK(hipStreamCreateWithFlags(&net->capture_stream, hipStreamNonBlocking));
+        CMF_HIP_CHECK(hipStreamBeginCapture(net->capture_stream, hipStreamCaptureModeThreadLocal));
+        int rc = substep_loop(net, v, f, count, net->capture_stream);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(net->capture_stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return set_error(CMF_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return set_error(CMF_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        if (net->graphs.size() >= 64) {
+            (void)hipGraphExecDestroy(net->graphs.front().second);
+            net->graphs.erase(net->graphs.begin());
+        }
+        net->graphs.emplace_back(count, exec);
+    }
+    CMF_HIP_CHECK(hipGraphLaunch(exec, s));
+    return CMF_OK;
+}
+
 template <class REAL, class STO>
 int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_count,
-                int32_t* out_count, void* stream) {
+                int32_t* out_count, void* stream, bool use_graph) {
     View<REAL, STO> v;
     int rc = make_view<REAL, STO>(net, f, v);
     if (rc) return rc;
@@ -675,18 +739,8 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
                            static_cast<REAL*>(f->time_step), nslots, outer / (double)fixed_count);
         CMF_HIP_CHECK(hipGetLastError());
     }
-    for (int k = 0; k < count; ++k) {
-        if (v.has(CMF_HAS_LOG)) {
-            hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, f->current_step, (int64_t)1, (int32_t)k);
-        }
-        // with per-trial counts each trial stores its inflow on its own last
-        // sub-step (k_stage checks); a shared count stores on the last one
-        rc = fused_substep(net, v, f, k, (k == count - 1) ? 1 : 0, stream);
-        if (rc) return rc;
-    }
-    if (f->current_step)
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, f->current_step, (int64_t)1, (int32_t)(count - 1));
-    CMF_HIP_CHECK(hipGetLastError());
+    rc = use_graph ? substep_loop_graph(net, v, f, count, s) : substep_loop(net, v, f, count, s);
+    if (rc) return rc;
     if (out_count) *out_count = count;
     return CMF_OK;
 }
@@ -907,9 +961,19 @@ int cmf_hip_advance(cmf_network* net, const cmf_fields* f, double outer_seconds,
                     int32_t* out_count, void* s) {
     if (!f) return set_error(CMF_ERR_INVALID, "null field table");
     if (f->precision == CMF_PRECISION_F32_F64)
-        return run_advance<float, double>(net, f, outer_seconds, fixed_count, out_count, s);
+        return run_advance<float, double>(net, f, outer_seconds, fixed_count, out_count, s, false);
     if (f->precision == CMF_PRECISION_F64_F64)
-        return run_advance<double, double>(net, f, outer_seconds, fixed_count, out_count, s);
+        return run_advance<double, double>(net, f, outer_seconds, fixed_count, out_count, s, false);
+    return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
+}
+
+int cmf_hip_advance_graph(cmf_network* net, const cmf_fields* f, double outer_seconds, int32_t fixed_count,
+                          int32_t* out_count, void* s) {
+    if (!f) return set_error(CMF_ERR_INVALID, "null field table");
+    if (f->precision == CMF_PRECISION_F32_F64)
+        return run_advance<float, double>(net, f, outer_seconds, fixed_count, out_count, s, true);
+    if (f->precision == CMF_PRECISION_F64_F64)
+        return run_advance<double, double>(net, f, outer_seconds, fixed_count, out_count, s, true);
     return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
 }
 

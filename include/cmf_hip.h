@@ -332,6 +332,15 @@ int cmf_hip_substep_timed(cmf_network* net, const cmf_fields* f, int32_t substep
  * *out_count (the maximum over trials with CMF_PER_TRIAL_DT). */
 int cmf_hip_advance(cmf_network* net, const cmf_fields* f, double outer_seconds,
                     int32_t fixed_count, int32_t* out_count, void* stream);
+/* cmf_hip_advance with the sub-step loop replayed from a HIP graph: the loop
+ * for a given sub-step count is captured once (on a private stream; kernel
+ * nodes only) and relaunched on `stream` in later outer steps while the field
+ * table holds the same pointers and sizes (a changed table drops the cached
+ * graphs).  Results are identical to cmf_hip_advance.  No reference
+ * counterpart (the reference launches every op from Python each sub-step,
+ * cmfgpu/models/cama_flood_model.py:134-162). */
+int cmf_hip_advance_graph(cmf_network* net, const cmf_fields* fields, double outer_seconds,
+                          int32_t fixed_count, int32_t* out_count, void* stream);
 
 #ifdef __cplusplus
 }
