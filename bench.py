@@ -216,20 +216,8 @@ def run_steps(model, steps: int, per_trial: bool):
 
 def kernel_times(model, reps: int):
     """Mean duration (ms) of each kernel of the fused sub-step, HIP events on the launch stream."""
-    from cmfgpu_amd import _lib
-    cap = 64
-    ms = (ctypes.c_double * cap)()
-    names = (ctypes.c_char_p * cap)()
-    count = ctypes.c_int32(0)
-    acc = {}
-    for r in range(reps):
-        _lib.check(_lib.lib().cmf_hip_substep_timed(model.network_handle, model.field_table_ref(), 0, 0,
-                                                    _lib.current_stream(), ms, names, cap, ctypes.byref(count)),
-                   "cmf_hip_substep_timed")
-        for k in range(count.value):
-            key = names[k].decode().strip("()")
-            acc.setdefault(key, []).append(ms[k])
-    return {k: sum(v) / len(v) for k, v in acc.items()}
+    from cmfgpu_amd.autotune import kernel_times as kt
+    return kt(model, reps)
 
 
 def cpu_baseline(model, net, budget_s: float):
@@ -291,6 +279,7 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--basins", type=int, default=400, help="uneven basins of the grouped line (0: skip)")
     ap.add_argument("--basins-scale", type=float, default=1.0, help="basin size factor (1.0: about 20M catchments)")
+    ap.add_argument("--autotune", type=int, default=1, help="pick kernel block sizes by timing first (0: 256)")
     ap.add_argument("--graph", type=int, default=1, help="eager vs HIP-graph line on the small network (0: skip)")
     ap.add_argument("--error-days", type=int, default=1,
                     help="simulated days of the FP32-vs-FP64 error run (0: skip)")
@@ -331,6 +320,9 @@ def main():
 
     # ---------------------------------------------------------- plain --
     model = build_model(net, src, 1, device, args.layout)
+    tuned = model.autotune(use_cache=False) if args.autotune else None
+    if tuned:
+        log("autotune: " + ", ".join(f"{k}={v['block']}" for k, v in sorted(tuned.items())))
     run_steps(model, args.warmup, per_trial=False)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -370,6 +362,8 @@ def main():
     if args.members > 1:
         M = args.members
         bm = build_model(net, src, M, device, args.layout)
+        if args.autotune:
+            bm.autotune(use_cache=False)
         run_steps(bm, max(args.warmup // 4, 1), per_trial=True)
         bsteps = args.batched_steps or max(args.steps // 4, 2)
         torch.cuda.synchronize()
@@ -403,6 +397,7 @@ def main():
                    "levees": net.num_levees, "flood_levels": 10, "outer_step_s": 1800.0,
                    "substeps_timed": int(substeps), "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "basins": basins, "graph": graph,
+        "block_sizes": {k: v["block"] for k, v in tuned.items()} if tuned else None,
         "error_vs_fp64": errors,
     }
     print(json.dumps(line), flush=True)

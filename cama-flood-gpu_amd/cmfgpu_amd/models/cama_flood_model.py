@@ -330,6 +330,11 @@ class CaMaFlood:
             return x.reshape(x.shape[0], self.G, -1)
         return x
 
+    def autotune(self, **kw) -> dict:
+        """Choose the block size of each fused-schedule kernel by timing (cmfgpu_amd.autotune.tune)."""
+        from ..autotune import tune
+        return tune(self, **kw)
+
     def group_sub_steps(self) -> np.ndarray:
         """Sub-step count of each group in the last outer step (group order = sorted labels)."""
         return self.t["max_sub_steps"].cpu().numpy()

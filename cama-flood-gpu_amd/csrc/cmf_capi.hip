@@ -55,6 +55,8 @@ struct cmf_network {
     double* partials = nullptr;
     int64_t partial_blocks = 0;
     int32_t* host_counts = nullptr;   // pinned, max(T, G) entries
+    // block size of each tunable kernel of the fused schedule (CMF_TUNE_*)
+    int32_t blocks[CMF_TUNE_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256};
     // groups (G > 0): group of each catchment and the block table of the
     // catchment kernels (blocks never straddle two groups)
     int64_t G = 0;
@@ -344,6 +346,29 @@ int log_final(const View<REAL, STO>& v, int64_t nblocks, uint32_t mask, void* st
     return CMF_OK;
 }
 
+// Launch with an explicit block size (a multiple of 64, at most kMaxBlock).
+#define LAUNCH_B(kernel, count, block, stream, ...)                                         \
+    do {                                                                                     \
+        const int64_t _cnt = (count);                                                        \
+        const int64_t _blk = (block);                                                        \
+        if (_cnt > 0) {                                                                      \
+            timing_mark(#kernel, (hipStream_t)(stream), true);                               \
+            hipLaunchKernelGGL(kernel, dim3((unsigned)((_cnt + _blk - 1) / _blk)), dim3((unsigned)_blk), 0, \
+                               (hipStream_t)(stream), __VA_ARGS__);                          \
+            timing_mark(#kernel, (hipStream_t)(stream), false);                              \
+            hipError_t _e = hipGetLastError();                                               \
+            if (_e != hipSuccess)                                                            \
+                return set_error(CMF_ERR_HIP, "launch of %s failed: %s", #kernel,            \
+                                 hipGetErrorString(_e));                                     \
+        }                                                                                    \
+    } while (0)
+
+// Block size of a tunable kernel; catchment kernels of a grouped network run
+// the group block table (kBlock threads per entry).
+inline int64_t block_of(const cmf_network* net, int id, bool catchment_kernel) {
+    return (catchment_kernel && net->G > 0) ? kBlock : net->blocks[id];
+}
+
 // ----------------------------------------------------------- per-op -----
 enum Op {
     OP_OUTFLOW, OP_INFLOW, OP_STAGE, OP_STAGE_LOG, OP_CFL, OP_BIF_OUT, OP_LEVEE_BIF_OUT,
@@ -489,25 +514,28 @@ int dispatch_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
 // trial-inner batched layout; otherwise k_limiter then k_stage_fused.
 // `nt`: threads of a catchment kernel (N*T, or the group block table)
 template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
-int launch_route(const View<REAL, STO>& v, int64_t nt, bool merged, int substep, int store_inflow, void* stream) {
+int launch_route(const cmf_network* net, const View<REAL, STO>& v, int64_t nt, bool merged, int substep,
+                 int store_inflow, void* stream) {
     if (merged) {
-        LAUNCH((k_route<REAL, STO, CONTIG, LOG, LFIX, LEV>), nt, stream, v, substep, store_inflow);
+        LAUNCH_B((k_route<REAL, STO, CONTIG, LOG, LFIX, LEV>), nt, block_of(net, CMF_TUNE_ROUTE, true), stream, v,
+                 substep, store_inflow);
     } else {
-        LAUNCH((k_limiter<REAL, STO, CONTIG>), nt, stream, v, substep);
-        LAUNCH((k_stage_fused<REAL, STO, CONTIG, LOG, LFIX, LEV>), nt, stream, v, substep, store_inflow);
+        LAUNCH_B((k_limiter<REAL, STO, CONTIG>), nt, block_of(net, CMF_TUNE_LIMITER, true), stream, v, substep);
+        LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, LFIX, LEV>), nt, block_of(net, CMF_TUNE_STAGE, true), stream,
+                 v, substep, store_inflow);
     }
     return CMF_OK;
 }
 
 template <class REAL, class STO, bool CONTIG, bool LOG>
-int dispatch_route(const View<REAL, STO>& v, int64_t nt, bool merged, bool lev, int substep, int store_inflow,
-                   void* stream) {
+int dispatch_route(const cmf_network* net, const View<REAL, STO>& v, int64_t nt, bool merged, bool lev, int substep,
+                   int store_inflow, void* stream) {
     if (v.L == 10) {
-        return lev ? launch_route<REAL, STO, CONTIG, LOG, 10, true>(v, nt, merged, substep, store_inflow, stream)
-                   : launch_route<REAL, STO, CONTIG, LOG, 10, false>(v, nt, merged, substep, store_inflow, stream);
+        return lev ? launch_route<REAL, STO, CONTIG, LOG, 10, true>(net, v, nt, merged, substep, store_inflow, stream)
+                   : launch_route<REAL, STO, CONTIG, LOG, 10, false>(net, v, nt, merged, substep, store_inflow, stream);
     }
-    return lev ? launch_route<REAL, STO, CONTIG, LOG, 0, true>(v, nt, merged, substep, store_inflow, stream)
-               : launch_route<REAL, STO, CONTIG, LOG, 0, false>(v, nt, merged, substep, store_inflow, stream);
+    return lev ? launch_route<REAL, STO, CONTIG, LOG, 0, true>(net, v, nt, merged, substep, store_inflow, stream)
+               : launch_route<REAL, STO, CONTIG, LOG, 0, false>(net, v, nt, merged, substep, store_inflow, stream);
 }
 
 // One fused sub-step: outflow, [reservoir], [bifurcation outflow, end
@@ -523,32 +551,36 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
     const bool lev_fused = levee && v.levee_index != nullptr;
     // trials adjacent in memory: the merged route pass (uniform loops per wave)
     const bool merged = v.T > 1 && v.ts == 1;
-    if (merged) LAUNCH((k_outflow<REAL, STO, true, false>), NT, stream, v, substep);
-    else LAUNCH((k_outflow<REAL, STO, true, true>), NT, stream, v, substep);
-    if (v.has(CMF_HAS_RESERVOIR)) LAUNCH((k_reservoir<REAL, STO, true>), v.R * v.T, stream, v, substep);
+    const int64_t b_out = block_of(net, CMF_TUNE_OUTFLOW, true), b_bo = block_of(net, CMF_TUNE_BIF_OUTFLOW, false);
+    const int64_t b_lim = block_of(net, CMF_TUNE_LIMITER, true), b_stage = block_of(net, CMF_TUNE_STAGE, true);
+    const int64_t b_bi = block_of(net, CMF_TUNE_BIF_INFLOW, false), b_lt = block_of(net, CMF_TUNE_LIM_TOUCHED, false);
+    if (merged) LAUNCH_B((k_outflow<REAL, STO, true, false>), NT, b_out, stream, v, substep);
+    else LAUNCH_B((k_outflow<REAL, STO, true, true>), NT, b_out, stream, v, substep);
+    if (v.has(CMF_HAS_RESERVOIR))
+        LAUNCH_B((k_reservoir<REAL, STO, true>), v.R * v.T, block_of(net, CMF_TUNE_RESERVOIR, false), stream, v, substep);
     int rc;
     if (bif) {
         if (levee) {
-            if (v.LB == 5) LAUNCH((k_bif_outflow<REAL, STO, true, 5>), v.P * v.T, stream, v, substep);
-            else LAUNCH((k_bif_outflow<REAL, STO, true, 0>), v.P * v.T, stream, v, substep);
+            if (v.LB == 5) LAUNCH_B((k_bif_outflow<REAL, STO, true, 5>), v.P * v.T, b_bo, stream, v, substep);
+            else LAUNCH_B((k_bif_outflow<REAL, STO, true, 0>), v.P * v.T, b_bo, stream, v, substep);
         } else {
-            if (v.LB == 5) LAUNCH((k_bif_outflow<REAL, STO, false, 5>), v.P * v.T, stream, v, substep);
-            else LAUNCH((k_bif_outflow<REAL, STO, false, 0>), v.P * v.T, stream, v, substep);
+            if (v.LB == 5) LAUNCH_B((k_bif_outflow<REAL, STO, false, 5>), v.P * v.T, b_bo, stream, v, substep);
+            else LAUNCH_B((k_bif_outflow<REAL, STO, false, 0>), v.P * v.T, b_bo, stream, v, substep);
         }
         if (merged) {
-            if (contig) LAUNCH((k_lim_touched<REAL, STO, true>), net->n_touched * v.T, stream, v, net->touched, net->n_touched, substep);
-            else LAUNCH((k_lim_touched<REAL, STO, false>), net->n_touched * v.T, stream, v, net->touched, net->n_touched, substep);
-            rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(v, NT, true, lev_fused, substep, store_inflow, stream)
-                               : dispatch_route<REAL, STO, false, true>(v, NT, true, lev_fused, substep, store_inflow, stream))
-                     : (contig ? dispatch_route<REAL, STO, true, false>(v, NT, true, lev_fused, substep, store_inflow, stream)
-                               : dispatch_route<REAL, STO, false, false>(v, NT, true, lev_fused, substep, store_inflow, stream));
+            if (contig) LAUNCH_B((k_lim_touched<REAL, STO, true>), net->n_touched * v.T, b_lt, stream, v, net->touched, net->n_touched, substep);
+            else LAUNCH_B((k_lim_touched<REAL, STO, false>), net->n_touched * v.T, b_lt, stream, v, net->touched, net->n_touched, substep);
+            rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(net, v, NT, true, lev_fused, substep, store_inflow, stream)
+                               : dispatch_route<REAL, STO, false, true>(net, v, NT, true, lev_fused, substep, store_inflow, stream))
+                     : (contig ? dispatch_route<REAL, STO, true, false>(net, v, NT, true, lev_fused, substep, store_inflow, stream)
+                               : dispatch_route<REAL, STO, false, false>(net, v, NT, true, lev_fused, substep, store_inflow, stream));
         } else {
             // the path nets need every end limiter: full limiter pass first
-            if (contig) LAUNCH((k_limiter<REAL, STO, true>), NT, stream, v, substep);
-            else LAUNCH((k_limiter<REAL, STO, false>), NT, stream, v, substep);
-            LAUNCH((k_bif_inflow<REAL, STO, true>), v.P * v.T, stream, v, substep);
+            if (contig) LAUNCH_B((k_limiter<REAL, STO, true>), NT, b_lim, stream, v, substep);
+            else LAUNCH_B((k_limiter<REAL, STO, false>), NT, b_lim, stream, v, substep);
+            LAUNCH_B((k_bif_inflow<REAL, STO, true>), v.P * v.T, b_bi, stream, v, substep);
             rc = CMF_OK;
-#define CMF_SF(C, LG, LF, LV) LAUNCH((k_stage_fused<REAL, STO, C, LG, LF, LV>), NT, stream, v, substep, store_inflow)
+#define CMF_SF(C, LG, LF, LV) LAUNCH_B((k_stage_fused<REAL, STO, C, LG, LF, LV>), NT, b_stage, stream, v, substep, store_inflow)
             if (contig) {
                 if (log) { if (v.L == 10) { if (lev_fused) CMF_SF(true, true, 10, true); else CMF_SF(true, true, 10, false); }
                            else { if (lev_fused) CMF_SF(true, true, 0, true); else CMF_SF(true, true, 0, false); } }
@@ -563,10 +595,10 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
 #undef CMF_SF
         }
     } else {
-        rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(v, NT, merged, lev_fused, substep, store_inflow, stream)
-                           : dispatch_route<REAL, STO, false, true>(v, NT, merged, lev_fused, substep, store_inflow, stream))
-                 : (contig ? dispatch_route<REAL, STO, true, false>(v, NT, merged, lev_fused, substep, store_inflow, stream)
-                           : dispatch_route<REAL, STO, false, false>(v, NT, merged, lev_fused, substep, store_inflow, stream));
+        rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(net, v, NT, merged, lev_fused, substep, store_inflow, stream)
+                           : dispatch_route<REAL, STO, false, true>(net, v, NT, merged, lev_fused, substep, store_inflow, stream))
+                 : (contig ? dispatch_route<REAL, STO, true, false>(net, v, NT, merged, lev_fused, substep, store_inflow, stream)
+                           : dispatch_route<REAL, STO, false, false>(net, v, NT, merged, lev_fused, substep, store_inflow, stream));
     }
     if (rc) return rc;
     if (log) {
@@ -577,7 +609,9 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
             timing_mark("k_log_final_grouped", (hipStream_t)stream, false);
             CMF_HIP_CHECK(hipGetLastError());
         } else {
-            rc = log_final(v, blocks_for(NT), kLogAll, stream);
+            // partials: one per block of the pass that produced them
+            const int64_t b_log = merged ? block_of(net, CMF_TUNE_ROUTE, true) : b_stage;
+            rc = log_final(v, (NT + b_log - 1) / b_log, kLogAll, stream);
             if (rc) return rc;
         }
     }
@@ -617,7 +651,7 @@ int run_substep(cmf_network* net, const cmf_fields* f, int substep, int store_in
     if (rc) return rc;
     if ((rc = need_base(f, "substep")) || (rc = need_features(f, "substep")) || (rc = need_grouped(net, f, "substep")))
         return rc;
-    if (v.has(CMF_HAS_LOG) && (net->G > 0 ? net->nblk : blocks_for(v.N * v.T)) > net->partial_blocks)
+    if (v.has(CMF_HAS_LOG) && (net->G > 0 ? net->nblk : (v.N * v.T + 63) / 64) > net->partial_blocks)
         return set_error(CMF_ERR_INTERNAL, "substep: partial buffer too small");
     return fused_substep(net, v, f, substep, store_inflow, stream);
 }
@@ -705,7 +739,7 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
     const bool per_trial = v.has(CMF_PER_TRIAL_DT) || net->G > 0;
     const int64_t nslots = net->G > 0 ? net->G : (v.has(CMF_PER_TRIAL_DT) ? v.T : 1);
     int count = fixed_count;
-    if (v.has(CMF_HAS_LOG) && (net->G > 0 ? net->nblk : blocks_for(v.N * v.T)) > net->partial_blocks)
+    if (v.has(CMF_HAS_LOG) && (net->G > 0 ? net->nblk : (v.N * v.T + 63) / 64) > net->partial_blocks)
         return set_error(CMF_ERR_INTERNAL, "advance: partial buffer too small");
     if (v.has(CMF_HAS_ADAPTIVE_TIME)) {
         if ((rc = require({NEED(max_sub_steps), NEED(outer_time_step)}, "advance"))) return rc;
@@ -855,7 +889,8 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     const size_t pel = (size_t)std::max<int64_t>(num_paths, 1) * (size_t)num_trials * 8;
     net->G = num_groups;
     net->nblk = (int64_t)blk_first.size();
-    net->partial_blocks = std::max<int64_t>(blocks_for(n * num_trials), net->nblk) + 1;
+    // partials for the smallest block size a tunable pass may use (64)
+    net->partial_blocks = std::max<int64_t>((n * num_trials + 63) / 64, net->nblk) + 1;
     if ((rc = upload((void**)&net->up_ptr, up_ptr.data(), up_ptr.size() * 4)) ||
         (rc = upload((void**)&net->up_idx, up_idx.data(), up_idx.size() * 4)) ||
         (rc = upload((void**)&net->ev_ptr, ev_ptr.data(), ev_ptr.size() * 4)) ||
@@ -896,6 +931,27 @@ int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_pat
 }
 
 int64_t cmf_network_num_groups(const cmf_network* net) { return net ? net->G : 0; }
+
+int cmf_network_set_block_sizes(cmf_network* net, const int32_t* sizes, int32_t count) {
+    if (!net || !sizes || count < 0 || count > CMF_TUNE_COUNT)
+        return set_error(CMF_ERR_INVALID, "cmf_network_set_block_sizes: bad arguments");
+    for (int32_t k = 0; k < count; ++k)
+        if (sizes[k] < 64 || sizes[k] > kMaxBlock || sizes[k] % 64 != 0)
+            return set_error(CMF_ERR_INVALID, "cmf_network_set_block_sizes: size %d of kernel %d is not a multiple "
+                             "of 64 in [64, %d]", sizes[k], k, kMaxBlock);
+    for (int32_t k = 0; k < count; ++k) net->blocks[k] = sizes[k];
+    // captured sub-step loops hold the old launch shapes
+    for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
+    net->graphs.clear();
+    return CMF_OK;
+}
+
+int cmf_network_get_block_sizes(const cmf_network* net, int32_t* sizes, int32_t count) {
+    if (!net || !sizes || count < 0) return set_error(CMF_ERR_INVALID, "cmf_network_get_block_sizes: bad arguments");
+    const int32_t n = std::min<int32_t>(count, CMF_TUNE_COUNT);
+    for (int32_t k = 0; k < n; ++k) sizes[k] = net->blocks[k];
+    return CMF_OK;
+}
 
 int cmf_network_destroy(cmf_network* net) {
     free_network(net);

@@ -23,7 +23,10 @@
 
 namespace cmf {
 
+// Default block size, and the largest one the tunable kernels accept (their
+// block size is a launch parameter, a multiple of 64).
 constexpr int kBlock = 256;
+constexpr int kMaxBlock = 512;
 
 // Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
 // With CMF_XCD_REMAP the blocks are renumbered so that every XCD walks one
@@ -62,9 +65,9 @@ __device__ __forceinline__ void store_if_changed(X* p, X old, X val) {
 // TERMS: write the outgoing-storage terms (pos_term / neg_term) of each
 // catchment, for the passes that sum them (k_outgoing_sum, k_limiter).
 template <class REAL, class STO, bool FUSED, bool TERMS>
-__global__ __launch_bounds__(kBlock) void k_outflow(View<REAL, STO> v, int substep) {
+__global__ __launch_bounds__(kMaxBlock) void k_outflow(View<REAL, STO> v, int substep) {
     int64_t t, i;
-    if (!v.catchment_of(xcd_block() * kBlock + threadIdx.x, t, i)) return;
+    if (!v.catchment_of(xcd_block() * blockDim.x + threadIdx.x, t, i)) return;
     if (!v.active(t, i, substep)) return;
     const int64_t e = v.el(t, i);
     const int64_t d = v.dn[i];
@@ -198,7 +201,7 @@ __device__ __forceinline__ STO outgoing_terms(const View<REAL, STO>& v, int64_t 
 // per-op: outgoing_storage += catchment terms (compute_outflow's scatter)
 template <class REAL, class STO, bool CONTIG>
 __global__ __launch_bounds__(kBlock) void k_outgoing_sum(View<REAL, STO> v) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= v.N * v.T) return;
     int64_t t, j;
     v.split_c(g, t, j);
@@ -210,8 +213,8 @@ __global__ __launch_bounds__(kBlock) void k_outgoing_sum(View<REAL, STO> v) {
 // Replaces the reservoir catchment's outflow and its outgoing terms.  FUSED
 // writes the flow into the qraw scratch.
 template <class REAL, class STO, bool FUSED>
-__global__ __launch_bounds__(kBlock) void k_reservoir(View<REAL, STO> v, int substep) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kMaxBlock) void k_reservoir(View<REAL, STO> v, int substep) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= v.R * v.T) return;
     const int64_t t = g / v.R, r = g - t * v.R;
     const int64_t c = v.res_idx[r];
@@ -268,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_reservoir(View<REAL, STO> v, int sub
 // write the same value.
 template <class REAL, class STO, bool CONTIG>
 __global__ __launch_bounds__(kBlock) void k_res_regather(View<REAL, STO> v) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= v.R * v.T) return;
     const int64_t t = g / v.R, r = g - t * v.R;
     const int64_t c = v.res_idx[r];
@@ -283,8 +286,8 @@ __global__ __launch_bounds__(kBlock) void k_res_regather(View<REAL, STO> v) {
 // outgoing terms, writes path_raw: the sum of the limited per-level flows,
 // in level order, as compute_bifurcation_inflow forms it (bifurcation.py:132-137).
 template <class REAL, class STO, bool LEVEE, int LBFIX>
-__global__ __launch_bounds__(kBlock) void k_bif_outflow(View<REAL, STO> v, int substep) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kMaxBlock) void k_bif_outflow(View<REAL, STO> v, int substep) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= v.P * v.T) return;
     int64_t t, p;
     v.split_p(g, t, p);
@@ -409,7 +412,7 @@ __device__ __forceinline__ STO bif_event_sum(const View<REAL, STO>& v, int64_t t
 // per-op: outgoing_storage += bifurcation terms
 template <class REAL, class STO>
 __global__ __launch_bounds__(kBlock) void k_bif_outgoing(View<REAL, STO> v) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= v.N * v.T) return;
     int64_t t, j;
     v.split_c(g, t, j);
@@ -421,8 +424,8 @@ __global__ __launch_bounds__(kBlock) void k_bif_outgoing(View<REAL, STO> v) {
 // FUSED reads the end catchments' limiters from the limiter scratch
 // (k_lim_touched); per-op from the limit_rate field compute_inflow wrote.
 template <class REAL, class STO, bool FUSED>
-__global__ __launch_bounds__(kBlock) void k_bif_inflow(View<REAL, STO> v, int substep) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kMaxBlock) void k_bif_inflow(View<REAL, STO> v, int substep) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= v.P * v.T) return;
     int64_t t, p;
     v.split_p(g, t, p);
@@ -444,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_bif_inflow(View<REAL, STO> v, int su
 // per-op: global_bifurcation_outflow += path nets
 template <class REAL, class STO>
 __global__ __launch_bounds__(kBlock) void k_gbif_sum(View<REAL, STO> v) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= v.N * v.T) return;
     int64_t t, j;
     v.split_c(g, t, j);
@@ -464,7 +467,7 @@ __device__ __forceinline__ REAL limit_of(const View<REAL, STO>& v, STO outgoing,
 // per-op: apply the storage limiter in place
 template <class REAL, class STO>
 __global__ __launch_bounds__(kBlock) void k_inflow_limit(View<REAL, STO> v) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= v.N * v.T) return;
     int64_t t, i;
     v.split_c(g, t, i);
@@ -481,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_inflow_limit(View<REAL, STO> v) {
 // per-op: river/flood (and reservoir) inflow += limited upstream flows
 template <class REAL, class STO, bool CONTIG>
 __global__ __launch_bounds__(kBlock) void k_inflow_sum(View<REAL, STO> v) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= v.N * v.T) return;
     int64_t t, j;
     v.split_c(g, t, j);
@@ -544,9 +547,9 @@ __device__ __forceinline__ REAL limit_recomputed(const View<REAL, STO>& v, int64
 // fused, with bifurcation: limiters of the path end catchments, for the
 // path nets of k_bif_inflow.
 template <class REAL, class STO, bool CONTIG>
-__global__ __launch_bounds__(kBlock) void k_lim_touched(View<REAL, STO> v, const int32_t* touched,
+__global__ __launch_bounds__(kMaxBlock) void k_lim_touched(View<REAL, STO> v, const int32_t* touched,
                                                         int64_t n_touched, int substep) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n_touched * v.T) return;
     const int64_t t = g / n_touched, x = touched[g - t * n_touched];
     if (!v.active(t, x, substep)) return;
@@ -915,7 +918,7 @@ __device__ __forceinline__ void levee_log(const LeveeOut<REAL, STO>& o, STO tota
 
 // Block-level FP64 sums of the log partials in a fixed order.
 __device__ __forceinline__ void block_log_partials(LogPart& lp, double* out_block) {
-    __shared__ double sh[kBlock / 64][CMF_LOG_COUNT];
+    __shared__ double sh[kMaxBlock / 64][CMF_LOG_COUNT];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < CMF_LOG_COUNT; ++k) {
@@ -925,7 +928,7 @@ __device__ __forceinline__ void block_log_partials(LogPart& lp, double* out_bloc
     __syncthreads();
     if (threadIdx.x < CMF_LOG_COUNT) {
         double acc = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w) acc += sh[w][threadIdx.x];
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) acc += sh[w][threadIdx.x];
         out_block[threadIdx.x] = acc;
     }
 }
@@ -947,7 +950,7 @@ __device__ __forceinline__ void store_state(const View<REAL, STO>& v, int64_t e,
 // the inflow fields (storage.py:16-193, :196-425).
 template <class REAL, class STO, bool LOG, int LFIX>
 __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     LogPart lp;
     if (LOG) {
 #pragma unroll
@@ -976,9 +979,9 @@ __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v) {
 // the outflow pass (outgoing_storage of compute_outflow, then compute_inflow's
 // limiter, outflow.py:346-352).
 template <class REAL, class STO, bool CONTIG>
-__global__ __launch_bounds__(kBlock) void k_limiter(View<REAL, STO> v, int substep) {
+__global__ __launch_bounds__(kMaxBlock) void k_limiter(View<REAL, STO> v, int substep) {
     int64_t t, j;
-    if (!v.catchment_of(xcd_block() * kBlock + threadIdx.x, t, j)) return;
+    if (!v.catchment_of(xcd_block() * blockDim.x + threadIdx.x, t, j)) return;
     if (!v.active(t, j, substep)) return;
     const int64_t e = v.el(t, j);
     STO og = outgoing_terms<REAL, STO, CONTIG>(v, t, j);
@@ -999,8 +1002,8 @@ __global__ __launch_bounds__(kBlock) void k_limiter(View<REAL, STO> v, int subst
 // for catchments near bankfull or with a levee, the floodplain row and levee
 // parameters (a second round trip); the arithmetic follows.
 template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
-__global__ __launch_bounds__(kBlock) void k_stage_fused(View<REAL, STO> v, int substep, int store_inflow) {
-    const int64_t g = xcd_block() * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kMaxBlock) void k_stage_fused(View<REAL, STO> v, int substep, int store_inflow) {
+    const int64_t g = xcd_block() * blockDim.x + threadIdx.x;
     LogPart lp;
     if (LOG) {
 #pragma unroll
@@ -1103,9 +1106,9 @@ __global__ __launch_bounds__(kBlock) void k_stage_fused(View<REAL, STO> v, int s
 // `store_inflow` writes river/flood inflow and the bifurcation total back
 // (the reference leaves the last sub-step's values there).
 template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
-__global__ __launch_bounds__(kBlock) void k_route(View<REAL, STO> v, int substep, int store_inflow) {
+__global__ __launch_bounds__(kMaxBlock) void k_route(View<REAL, STO> v, int substep, int store_inflow) {
     const int64_t NT = v.N * v.T;
-    const int64_t g = xcd_block() * kBlock + threadIdx.x;
+    const int64_t g = xcd_block() * blockDim.x + threadIdx.x;
     LogPart lp;
     if (LOG) {
 #pragma unroll
@@ -1250,7 +1253,7 @@ __global__ __launch_bounds__(kBlock) void k_log_final_grouped(View<REAL, STO> v,
 // flood stage left.
 template <class REAL, class STO, bool LOG, int LFIX>
 __global__ __launch_bounds__(kBlock) void k_levee(View<REAL, STO> v, int substep) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     LogPart lp;
     if (LOG) {
 #pragma unroll
@@ -1296,12 +1299,12 @@ __device__ __forceinline__ int cfl_steps(const View<REAL, STO>& v, int64_t t, in
 }
 
 __device__ __forceinline__ int block_max_int(int best) {
-    __shared__ int sh[kBlock / 64];
+    __shared__ int sh[kMaxBlock / 64];
     best = wave_max_int(best);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = best;
     __syncthreads();
     int m = sh[0];
-    for (int w = 1; w < kBlock / 64; ++w) m = max(m, sh[w]);
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = max(m, sh[w]);
     return m;
 }
 
@@ -1312,7 +1315,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl(View<REAL, STO> v) {
     const int64_t t = blockIdx.y;
     const REAL outer = v.outer_time_step[0];
     int best = 0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < v.N;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.N;
          i += (int64_t)gridDim.x * kBlock)
         best = max(best, cfl_steps(v, t, i, outer));
     const int m = block_max_int(best);

@@ -82,6 +82,13 @@ enum {
  * dtype (cmfgpu/phys/cuda/outflow.cu:301-366 dispatches the same pairs). */
 enum { CMF_PRECISION_F32_F64 = 0, CMF_PRECISION_F64_F64 = 1 };
 
+/* Tunable kernels of the fused schedule (block sizes: multiples of 64 up to
+ * 512, see cmf_network_set_block_sizes). */
+enum {
+    CMF_TUNE_OUTFLOW = 0, CMF_TUNE_RESERVOIR, CMF_TUNE_BIF_OUTFLOW, CMF_TUNE_BIF_INFLOW,
+    CMF_TUNE_LIM_TOUCHED, CMF_TUNE_LIMITER, CMF_TUNE_STAGE, CMF_TUNE_ROUTE, CMF_TUNE_COUNT
+};
+
 /* The 11 water-balance sums, in LogModule field order
  * (cmfgpu/modules/log.py:207-326). Held in FP64. */
 enum {
@@ -272,6 +279,13 @@ int cmf_network_create_grouped(int64_t num_catchments, const int32_t* downstream
                                const int32_t* bifurcation_downstream_idx, int64_t num_trials,
                                int64_t num_groups, const int64_t* group_ptr, cmf_network** out);
 int64_t cmf_network_num_groups(const cmf_network* net);
+/* Block sizes of the fused schedule's kernels, indexed by CMF_TUNE_* (count
+ * entries; default 256 each).  Grouped networks keep 256 for the catchment
+ * passes.  Setting them drops the graphs cached by cmf_hip_advance_graph.
+ * Chosen by cmfgpu_amd.autotune; no reference counterpart (the reference
+ * passes a fixed BLOCK_SIZE, cmfgpu/phys/specs.py:10-26). */
+int cmf_network_set_block_sizes(cmf_network* net, const int32_t* sizes, int32_t count);
+int cmf_network_get_block_sizes(const cmf_network* net, int32_t* sizes, int32_t count);
 int cmf_network_destroy(cmf_network* net);
 /* 1 when every upstream range is the identity (breadth-first layout). */
 int cmf_network_is_contiguous(const cmf_network* net);
