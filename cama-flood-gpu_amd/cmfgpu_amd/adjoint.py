@@ -1,0 +1,239 @@
+"""Gradients of a gauge-discharge loss with respect to river Manning's n and
+river (bank) height, by the hand-written adjoint of the fused sub-step
+(cmf_hip_substep_adjoint, csrc/cmf_adjoint.hip).
+
+Loss over S outer steps and G gauge catchments, Q = river + flood outflow of
+the gauge after each outer step:
+
+    L = 1/(S G) * sum_{s,g} (log(max(Q_sg, 0) + eps) - log(max(Q_obs_sg, 0) + eps))^2
+
+The forward run advances the model with its own adaptive CFL sub-steps and
+records the sub-step count of every outer step; the backward holds those
+counts (dt = outer step / count) fixed.  The forward keeps a checkpoint of the
+state every K outer steps; the backward walks the segments from the last one,
+re-runs each segment sub-step by sub-step from its checkpoint keeping every
+sub-step's input state on the device, then sweeps the segment in reverse.
+The initial state and every other parameter are held fixed.
+
+``GaugeLoss.apply(manning, height, problem)`` wraps this in a
+torch.autograd.Function.  Only the base physics (no levee, bifurcation or
+reservoir module), one trial and an ungrouped network are differentiated.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+STATE = ("river_outflow", "flood_outflow", "river_cross_section_depth", "flood_cross_section_depth",
+         "flood_cross_section_area", "river_storage", "flood_storage", "river_depth", "flood_depth")
+QR, QF = 0, 1
+
+
+class CmfAdjoint(ctypes.Structure):
+    """Mirror of struct cmf_adjoint."""
+
+    _fields_ = [(name, ctypes.c_void_p) for name in STATE] + [
+        ("adj_out", ctypes.c_void_p * len(STATE)),
+        ("adj_in", ctypes.c_void_p * len(STATE)),
+        ("grad_river_manning", ctypes.c_void_p),
+        ("grad_river_height", ctypes.c_void_p),
+        ("time_step", ctypes.c_double),
+    ]
+
+
+_SIG_DONE = False
+
+
+def _declare():
+    global _SIG_DONE
+    if _SIG_DONE:
+        return
+    lib = _lib.lib()
+    lib.cmf_hip_substep_adjoint.restype = ctypes.c_int
+    lib.cmf_hip_substep_adjoint.argtypes = [ctypes.c_void_p, ctypes.POINTER(_lib.CmfFields),
+                                            ctypes.POINTER(CmfAdjoint), ctypes.c_void_p]
+    _SIG_DONE = True
+
+
+class GaugeProblem:
+    """A calibration problem on a built model: gauges (network order), the
+    observed discharge [S, G], the runoff of each outer step (one array, or a
+    callable step -> array) and the checkpoint interval K."""
+
+    def __init__(self, model, gauges, observed, runoff=None, checkpoint_every: int = 48, eps: float = 1e-3):
+        extra = set(model.opened_modules) - {"base", "adaptive_time"}
+        if extra:
+            raise ValueError(f"the adjoint covers the base physics only; modules {sorted(extra)} are open")
+        if model.T != 1 or model.G or model.batched_names or model.mode != "fused":
+            raise ValueError("the adjoint needs one trial, an ungrouped network, no per-trial parameters and "
+                             "the fused schedule")
+        self.model = model
+        self.gauges = np.asarray(gauges, dtype=np.int64)
+        self.obs = torch.as_tensor(np.asarray(observed, dtype=np.float64), device=model.device)
+        if self.obs.ndim != 2 or self.obs.shape[1] != self.gauges.shape[0]:
+            raise ValueError("observed must be [steps, gauges]")
+        self.steps = int(self.obs.shape[0])
+        self.K = int(checkpoint_every)
+        self.eps = float(eps)
+        self.runoff = runoff
+        self.g_dev = torch.as_tensor(model.inv[self.gauges], device=model.device)
+        self.initial = self._snapshot()
+        self.counts: list[int] = []
+        self.checkpoints: dict[int, list] = {}
+        self.last_loss = None
+        _declare()
+
+    # ------------------------------------------------------------ state --
+    def _state(self):
+        return [self.model.t[name] for name in STATE]
+
+    def _snapshot(self):
+        return [t.clone() for t in self._state()]
+
+    def _restore(self, snap):
+        for t, s in zip(self._state(), snap):
+            t.copy_(s)
+
+    def _apply_runoff(self, step: int):
+        if self.runoff is None:
+            return
+        r = self.runoff(step) if callable(self.runoff) else self.runoff
+        if r is not None:
+            self.model.set_inputs(r)
+
+    def set_parameters(self, manning=None, height=None):
+        """Per-catchment river Manning's n and river height, network order."""
+        m = self.model
+        perm = torch.as_tensor(m.perm, device=m.device)
+        for name, x in (("river_manning", manning), ("river_height", height)):
+            if x is not None:
+                x = torch.as_tensor(x).to(m.device)
+                m.t[name].copy_(x[perm].to(m.t[name].dtype))
+
+    def parameters(self):
+        m = self.model
+        return m.get("river_manning"), m.get("river_height")
+
+    # ------------------------------------------------------------- loss --
+    def _gauge_q(self):
+        t = self.model.t
+        return (t["river_outflow"][self.g_dev].double() + t["flood_outflow"][self.g_dev].double())
+
+    def _loss_term(self, step, q):
+        qc = q.clamp_min(0.0)
+        return ((torch.log(qc + self.eps) - self._log_obs(step)) ** 2).sum()
+
+    def _log_obs(self, step):
+        return torch.log(self.obs[step].clamp_min(0.0) + self.eps)
+
+    def _loss_seed(self, step, q):
+        qc = q.clamp_min(0.0)
+        r = torch.log(qc + self.eps) - self._log_obs(step)
+        return torch.where(q > 0, 2.0 * r / (qc + self.eps), torch.zeros_like(q)) / (self.steps * self.obs.shape[1])
+
+    def _substep(self, k: int, last: bool):
+        m = self.model
+        _lib.check(_lib.lib().cmf_hip_substep(m.network_handle, m.field_table_ref(), k, 1 if last else 0,
+                                              _lib.current_stream()), "cmf_hip_substep")
+
+    def _step_fixed(self, count: int):
+        m = self.model
+        m.t["time_step"].fill_(m.step_duration / count)
+        for k in range(count):
+            self._substep(k, k == count - 1)
+
+    def run(self, fixed_counts=None) -> float:
+        """Forward from the initial state; returns the loss.  With
+        `fixed_counts` the sub-step counts are prescribed (dt held)."""
+        self._restore(self.initial)
+        self.counts = []
+        self.checkpoints = {}
+        total = torch.zeros((), dtype=torch.float64, device=self.model.device)
+        for step in range(self.steps):
+            if step % self.K == 0:
+                self.checkpoints[step] = self._snapshot()
+            self._apply_runoff(step)
+            if fixed_counts is None:
+                n = self.model.step_advance()
+            else:
+                n = int(fixed_counts[step])
+                self._step_fixed(n)
+            self.counts.append(n)
+            total += self._loss_term(step, self._gauge_q())
+        self.final = self._snapshot()
+        self.last_loss = float(total.item()) / (self.steps * self.obs.shape[1])
+        return self.last_loss
+
+    def gradient(self):
+        """dL/d(river_manning), dL/d(river_height) (float64, network order)
+        for the last `run`."""
+        if not self.counts:
+            raise RuntimeError("run() first")
+        m = self.model
+        dev = m.device
+        n = m.N
+        lam = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in STATE]
+        lam_new = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in STATE]
+        g_n = torch.zeros(n, dtype=torch.float64, device=dev)
+        g_h = torch.zeros(n, dtype=torch.float64, device=dev)
+        adj = CmfAdjoint()
+        adj.grad_river_manning = g_n.data_ptr()
+        adj.grad_river_height = g_h.data_ptr()
+        starts = sorted(self.checkpoints)
+        for s0 in reversed(starts):
+            s1 = min(s0 + self.K, self.steps)
+            self._restore(self.checkpoints[s0])
+            tape = []
+            for step in range(s0, s1):
+                self._apply_runoff(step)
+                cnt = self.counts[step]
+                m.t["time_step"].fill_(m.step_duration / cnt)
+                for k in range(cnt):
+                    tape.append((step, k, self._snapshot()))
+                    self._substep(k, k == cnt - 1)
+                # end of the outer step: the loss term's discharge
+                tape.append((step, -1, self._gauge_q()))
+            for step, k, item in reversed(tape):
+                if k < 0:
+                    seed = self._loss_seed(step, item)
+                    lam[QR].index_add_(0, self.g_dev, seed)
+                    lam[QF].index_add_(0, self.g_dev, seed)
+                    self._apply_runoff(step)
+                    continue
+                cnt = self.counts[step]
+                for f, t in zip(STATE, item):
+                    setattr(adj, f, t.data_ptr())
+                for x in range(len(STATE)):
+                    adj.adj_out[x] = lam[x].data_ptr()
+                    adj.adj_in[x] = lam_new[x].data_ptr()
+                adj.time_step = m.step_duration / cnt
+                _lib.check(_lib.lib().cmf_hip_substep_adjoint(m.network_handle, m.field_table_ref(),
+                                                              ctypes.byref(adj), _lib.current_stream()),
+                           "cmf_hip_substep_adjoint")
+                lam, lam_new = lam_new, lam
+            del tape
+        self._restore(self.final)
+        inv = torch.as_tensor(m.inv, device=dev)
+        return g_n[inv], g_h[inv]
+
+
+class GaugeLoss(torch.autograd.Function):
+    """loss = GaugeLoss.apply(manning, height, problem): the gauge-discharge
+    loss of `problem` as a function of per-catchment river Manning's n and
+    river height (network order); backward runs the adjoint."""
+
+    @staticmethod
+    def forward(ctx, manning, height, problem):
+        problem.set_parameters(manning.detach(), height.detach())
+        loss = problem.run()
+        ctx.problem = problem
+        return torch.tensor(loss, dtype=torch.float64, device=manning.device)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        g_n, g_h = ctx.problem.gradient()
+        return (grad_out * g_n).to(grad_out.device), (grad_out * g_h).to(grad_out.device), None

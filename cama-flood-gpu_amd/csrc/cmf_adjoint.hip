@@ -1,0 +1,491 @@
+// Hand-written adjoint (reverse mode) of one fused sub-step of the base
+// physics: local-inertial river and floodplain outflow
+// (cmfgpu/phys/triton/outflow.py:15-315), the storage limiter and inflow sums
+// (:318-383), the storage update and flood-stage diagnosis
+// (cmfgpu/phys/triton/storage.py:16-193).  Levee, bifurcation and reservoir
+// modules are not differentiated.
+//
+// The sub-step maps the state x = (river/flood outflow, river/flood/flood-area
+// cross-section depth, river/flood storage, river/flood depth) and the
+// parameters (river Manning n, river height H) to the next state.  Given the
+// adjoint of the next state, the passes below give the adjoint of x and add
+// the parameter gradients, with dt held at its forward value.  Branch decisions
+// (max/min/clamp picks, wet or dry, floodplain level, limiter regimes) are
+// those of the forward evaluation, recomputed bit for bit from the saved input
+// state; derivatives are formed in double precision.
+//
+// Passes (each over catchments, one grid-wide boundary between them):
+//   A0  recompute the unlimited outflows and outgoing terms
+//   A1  recompute the outgoing totals and storage limiters
+//   A2  reverse of the storage update and stage diagnosis
+//   A3  reverse of the limited flows and inflow sums, then of the limiter
+//   A4  reverse of the outflow momentum update
+//   A5  gather of the downstream water-surface adjoints
+
+namespace cmf {
+
+enum {
+    ADJ_QR = 0, ADJ_QF, ADJ_RCSD, ADJ_FCSD, ADJ_FCSA, ADJ_RS, ADJ_FS, ADJ_RD, ADJ_FD, ADJ_COUNT
+};
+
+template <class REAL>
+struct AdjView {
+    // saved input state of the sub-step
+    const REAL *q_r, *q_f, *rcsd, *fcsd, *fcsa, *rd, *fd;
+    const void *rs_raw, *fs_raw;
+    const double* lout[ADJ_COUNT];   // adjoint of the sub-step outputs
+    double* lin[ADJ_COUNT];          // adjoint of the sub-step inputs (written)
+    double *g_n, *g_h;               // gradient accumulators
+    // scratch
+    REAL *qr, *qf, *pos, *neg, *lim, *og;
+    double *a_rout, *a_fout, *a_ri, *a_fi, *a_qr, *a_qf, *a_og, *d_wd;
+    REAL dt;
+};
+
+template <class STO>
+__device__ __forceinline__ STO adj_sto(const void* p, int64_t i) { return static_cast<const STO*>(p)[i]; }
+
+// ---------------------------------------------------- outflow trace ----
+// The arithmetic of k_outflow (FUSED, no reservoir override), same
+// operations in the same order, keeping what the reverse needs.
+template <class REAL>
+struct OutTrace {
+    bool mouth, river_cond, flood_cond, killed, negative, limited, mx_own, slope_in;
+    REAL wse, wse_dn, slope, fslope, upd_rcsd, r_sifd_raw, r_sifd, fsurf, upd_fcsd, f_sifd_raw, f_sifd;
+    REAL upd_fcsa_raw, upd_fcsa, f_csa_m, f_imp_raw, f_imp, unit_r, den_r, qr0, den_f, qf0, qf1, tot, total_neg;
+    REAL ratio, limit, qr, qf;
+};
+
+template <class REAL, class STO>
+__device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v, const AdjView<REAL>& a,
+                                                        int64_t i, REAL time_step) {
+    OutTrace<REAL> o;
+    const int64_t d = v.dn[i];
+    o.mouth = (d == i);
+    const REAL gravity = v.gravity;
+    const REAL r_out = a.q_r[i];
+    const REAL r_man = v.river_manning[i];
+    const REAL r_dep = a.rd[i];
+    const REAL r_wid = v.river_width[i];
+    const REAL r_len = v.river_length[i];
+    const REAL r_hgt = v.river_height[i];
+    const REAL f_out = a.q_f[i];
+    const REAL f_man = v.flood_manning[i];
+    const REAL f_dep = a.fd[i];
+    const REAL c_elv = v.elev[i];
+    const REAL dn_dist = v.dist[i];
+    const REAL rs = (REAL)adj_sto<STO>(a.rs_raw, i);
+    const REAL fs = (REAL)adj_sto<STO>(a.fs_raw, i);
+    const REAL ps = (REAL)v.protected_storage[i];
+    const REAL r_csd = a.rcsd[i], f_csd = a.fcsd[i], f_csa = a.fcsa[i];
+
+    const REAL river_elevation = c_elv - r_hgt;
+    o.wse = r_dep + river_elevation;
+    o.tot = rs + fs + ps;
+    const REAL r_dep_dn = a.rd[d];
+    const REAL r_hgt_dn = v.river_height[d];
+    const REAL c_elv_dn = v.elev[d];
+    const REAL river_elevation_dn = c_elv_dn - r_hgt_dn;
+    o.wse_dn = r_dep_dn + river_elevation_dn;
+    if (o.mouth) o.wse_dn = c_elv;
+    o.mx_own = (o.wse > o.wse_dn || o.wse_dn != o.wse_dn);
+    const REAL max_wse = vmax(o.wse, o.wse_dn);
+    o.slope = (o.wse - o.wse_dn) / dn_dist;
+    o.fslope = vclamp(o.slope, (REAL)-0.005, (REAL)0.005);
+    o.slope_in = (o.slope > (REAL)-0.005) && (o.slope < (REAL)0.005);
+    o.upd_rcsd = o.mouth ? r_dep : (max_wse - river_elevation);
+    o.r_sifd_raw = dsqrt(o.upd_rcsd * r_csd);
+    o.r_sifd = vmax(o.r_sifd_raw, (REAL)1e-6);
+    o.fsurf = o.mouth ? o.wse : max_wse;
+    o.upd_fcsd = vmax(o.fsurf - c_elv, (REAL)0.0);
+    o.f_sifd_raw = dsqrt(o.upd_fcsd * f_csd);
+    o.f_sifd = vmax(o.f_sifd_raw, (REAL)1e-6);
+    o.upd_fcsa_raw = fs / r_len - f_dep * r_wid;
+    o.upd_fcsa = vmax(o.upd_fcsa_raw, (REAL)0.0);
+
+    const REAL r_cs_area = o.upd_rcsd * r_wid;
+    o.river_cond = (o.r_sifd > (REAL)1e-5) && (r_cs_area > (REAL)1e-5);
+    REAL q_r = (REAL)0.0;
+    o.unit_r = r_out / r_wid;
+    o.den_r = (REAL)1;
+    if (o.river_cond) {
+        const REAL num_r = r_wid * (o.unit_r + gravity * time_step * o.r_sifd * o.slope);
+        o.den_r = (REAL)1.0 + gravity * time_step * (r_man * r_man) * dabs(o.unit_r)
+                * ((REAL)1.0 / (o.r_sifd * o.r_sifd * dcbrt(o.r_sifd)));
+        q_r = num_r / o.den_r;
+    }
+    o.qr0 = q_r;
+    o.flood_cond = (o.f_sifd > (REAL)1e-5) && (o.upd_fcsa > (REAL)1e-5);
+    REAL q_f = (REAL)0.0;
+    o.f_csa_m = vmax(f_csa, (REAL)1e-6);
+    o.f_imp_raw = (REAL)0;
+    o.f_imp = (REAL)1;
+    o.den_f = (REAL)1;
+    if (o.flood_cond) {
+        o.f_imp_raw = dsqrt(o.upd_fcsa * o.f_csa_m);
+        o.f_imp = vmax(o.f_imp_raw, (REAL)1e-6);
+        const REAL num_f = f_out + gravity * time_step * o.f_imp * o.fslope;
+        o.den_f = (REAL)1.0 + gravity * time_step * (f_man * f_man) * dabs(f_out)
+                * ((REAL)1.0 / (o.f_sifd * dcbrt(o.f_sifd))) / o.f_imp;
+        q_f = num_f / o.den_f;
+    }
+    o.qf0 = q_f;
+    o.killed = (q_r * q_f) < (REAL)0.0;
+    if (o.killed) q_f = (REAL)0.0;
+    o.qf1 = q_f;
+    o.negative = (q_r < (REAL)0.0) && !o.mouth;
+    o.limited = false;
+    o.limit = (REAL)1;
+    o.total_neg = (REAL)1;
+    o.ratio = (REAL)1;
+    if (o.negative) {
+        o.total_neg = (-q_r - q_f) * time_step;
+        o.ratio = (REAL)0.05 * o.tot / o.total_neg;
+        o.limited = (o.ratio < (REAL)1.0);
+        o.limit = vmin(o.ratio, (REAL)1.0);
+        q_r = q_r * o.limit;
+        q_f = q_f * o.limit;
+    }
+    o.qr = q_r;
+    o.qf = q_f;
+    return o;
+}
+
+// A0: unlimited outflows and outgoing terms of the saved state
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_adj_outflow_fwd(View<REAL, STO> v, AdjView<REAL> a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.N) return;
+    const OutTrace<REAL> o = outflow_trace(v, a, i, a.dt);
+    a.qr[i] = o.qr;
+    a.qf[i] = o.qf;
+    const REAL pos = vmax(o.qr, (REAL)0.0) + vmax(o.qf, (REAL)0.0);
+    const REAL neg = vmin(o.qr, (REAL)0.0) + vmin(o.qf, (REAL)0.0);
+    a.pos[i] = pos * a.dt;
+    a.neg[i] = o.mouth ? (REAL)0.0 : -neg * a.dt;
+}
+
+// A1: outgoing totals (upstream negative terms in order, then the own
+// positive term) and storage limiters, as k_limiter
+template <class REAL, class STO, bool CONTIG>
+__global__ __launch_bounds__(kBlock) void k_adj_limiter_fwd(View<REAL, STO> v, AdjView<REAL> a) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= v.N) return;
+    STO acc = (STO)0;
+    for (int32_t k = v.up_ptr[j]; k < v.up_ptr[j + 1]; ++k) {
+        const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+        if (CONTIG && u == j) continue;
+        acc += (STO)a.neg[u];
+    }
+    acc += (STO)a.pos[j];
+    const REAL og = (REAL)acc;
+    const REAL rate = (REAL)(adj_sto<STO>(a.rs_raw, j) + adj_sto<STO>(a.fs_raw, j));
+    a.og[j] = og;
+    a.lim[j] = (og > (REAL)1e-8) ? vmin(rate / og, (REAL)1.0) : (REAL)1.0;
+}
+
+// ------------------------------------------------------ stage reverse ----
+// A2: recompute the limited flows, inflow sums, storage update and stage of
+// catchment j (k_stage_fused), then reverse them.  Writes the adjoints of the
+// limited own flows (a_rout/a_fout, including the next state's direct use),
+// of the inflow sums (a_ri/a_fi), of the old storages, and dJ/dH (stage part).
+template <class REAL, class STO, bool CONTIG, int LFIX>
+__global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView<REAL> a) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= v.N) return;
+    const REAL dt = a.dt;
+    const int64_t d = v.dn[j];
+    const REAL l_own = a.lim[j], l_dn = a.lim[d];
+    const REAL qr = a.qr[j], qf = a.qf[j];
+    const REAL rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
+    const REAL fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
+    STO ri = (STO)0, fi = (STO)0;
+    for (int32_t k = v.up_ptr[j]; k < v.up_ptr[j + 1]; ++k) {
+        const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+        if (CONTIG && u == j) continue;
+        const REAL lu = a.lim[u];
+        const REAL ur0 = a.qr[u], uf0 = a.qf[u];
+        ri += (STO)((ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own);
+        fi += (STO)((uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own);
+    }
+    const REAL rin = (REAL)ri, fin = (REAL)fi;
+    const STO rs = adj_sto<STO>(a.rs_raw, j), fs = adj_sto<STO>(a.fs_raw, j);
+    const STO ps = v.protected_storage[j];
+    const REAL ro = v.runoff[j];
+    // storage update (storage.py:81-94)
+    const STO sa = rs + (STO)((rin - rout) * dt);
+    const bool carry = sa < (STO)0.0;
+    const STO sb = fs + (carry ? sa : (STO)0.0) + (STO)((fin - fout - (REAL)0.0) * dt);
+    const STO rsu1 = vmax(sa, (STO)0.0);
+    const bool b_neg = sb < (STO)0.0;
+    const STO rsu2 = b_neg ? vmax(rsu1 + sb, (STO)0.0) : rsu1;
+    const STO fsu1 = vmax(sb, (STO)0.0);
+    const STO total_next = rsu2 + fsu1 + ps + (STO)(ro * dt) + (STO)((REAL)0.0 * dt);
+    const STO total_hp = vmax(total_next, (STO)0.0);
+    const REAL total = (REAL)total_hp;
+    // stage diagnosis (storage.py:105-181)
+    const REAL rh = v.river_height[j], rw = v.river_width[j], rl = v.river_length[j];
+    const REAL rms = rl * rw * rh;
+    const int L = LFIX > 0 ? LFIX : (int)v.L;
+    double Ltot = 0.0, LH = 0.0;
+    const double Lrs_out = a.lout[ADJ_RS][j], Lfs_out = a.lout[ADJ_FS][j];
+    const double Lrd_out = a.lout[ADJ_RD][j], Lfd_out = a.lout[ADJ_FD][j];
+    double Lrsf = Lrs_out + Lrd_out / ((double)rl * (double)rw);
+    if (!(total > rms)) {
+        // dry: rsf = total, flood depth 0, flood storage max(0, 0)
+        Ltot += Lrsf;
+    } else {
+        const REAL ca = v.catchment_area[j];
+        const REAL* row = v.table + j * v.L;
+        const REAL catchment_width = ca / rl;
+        const REAL winc = catchment_width / (REAL)L;
+        int level = 0;
+        REAL S_accum = rms, prev_H = (REAL)0.0, prev_W = rw, pts = rms, pfd = (REAL)0.0, nfd = (REAL)0.0;
+        for (int k = 0; k < L; ++k) {
+            const REAL H = row[k];
+            const REAL W = rw + (REAL)(k + 1) * winc;
+            const REAL dS = rl * (REAL)0.5 * (prev_W + W) * (H - prev_H);
+            const REAL S = S_accum + dS;
+            nfd = H;
+            if (!(total > S)) break;
+            level += 1;
+            pts = S;
+            pfd = H;
+            S_accum = S;
+            prev_H = H;
+            prev_W = W;
+        }
+        const REAL ptw = rw + (REAL)level * winc;
+        REAL fdep;
+        double dfd_dtot, dfd_dpts;
+        if (level == L) {
+            fdep = pfd + (total - pts) / (ptw * rl);
+            dfd_dtot = 1.0 / ((double)ptw * (double)rl);
+            dfd_dpts = -dfd_dtot;
+        } else {
+            const REAL fg = (nfd - pfd) / winc;
+            const REAL A = ptw * ptw + (REAL)2.0 * (total - pts) / (fg * rl);
+            const REAL dw = dsqrt(A) - ptw;
+            fdep = pfd + dw * fg;
+            const double sq = sqrt((double)A);
+            dfd_dtot = 1.0 / (sq * (double)rl);
+            dfd_dpts = -dfd_dtot;
+        }
+        const REAL cap = rms + rl * rw * fdep;
+        const bool cap_pick = ((STO)cap < total_hp);
+        const STO rsf = cap_pick ? (STO)cap : total_hp;
+        if (total_hp - rsf > (STO)0.0) {
+            Ltot += Lfs_out;
+            Lrsf -= Lfs_out;
+        }
+        double Lcap = 0.0;
+        if (cap_pick) Lcap = Lrsf;
+        else Ltot += Lrsf;
+        const double rlrw = (double)rl * (double)rw;
+        const double Lfdep = Lfd_out + Lcap * rlrw;
+        LH += Lcap * rlrw;
+        Ltot += Lfdep * dfd_dtot;
+        LH += Lfdep * dfd_dpts * rlrw;
+    }
+    // total_hp = max(total_next, 0); total_next = rsu2 + fsu1 + ...
+    const double Ltn = (total_next > (STO)0.0) ? Ltot : 0.0;
+    double Lrsu1 = 0.0, Lb = 0.0, La = 0.0;
+    if (sb > (STO)0.0) Lb += Ltn;
+    if (b_neg) {
+        if (rsu1 + sb > (STO)0.0) { Lrsu1 += Ltn; Lb += Ltn; }
+    } else {
+        Lrsu1 += Ltn;
+    }
+    if (sa > (STO)0.0) La += Lrsu1;
+    double Lfs = Lb, Lfin = Lb * (double)dt, Lfout = -Lb * (double)dt;
+    if (carry) La += Lb;
+    double Lrs = La, Lrin = La * (double)dt, Lrout = -La * (double)dt;
+    a.a_rout[j] = Lrout + a.lout[ADJ_QR][j];
+    a.a_fout[j] = Lfout + a.lout[ADJ_QF][j];
+    a.a_ri[j] = Lrin;
+    a.a_fi[j] = Lfin;
+    a.lin[ADJ_RS][j] = Lrs;
+    a.lin[ADJ_FS][j] = Lfs;
+    a.g_h[j] += LH;
+}
+
+// A3: adjoints of the unlimited flows and of the limiter of catchment k.
+// The limited outflow of k enters its own storage update and the inflow sum
+// of its downstream catchment; positive flow is limited by k, negative flow
+// by the downstream catchment (outflow.py:364-365).  Then the limiter
+// reverse (outflow.py:346-352) gives the adjoint of the outgoing total.
+template <class REAL, class STO, bool CONTIG>
+__global__ __launch_bounds__(kBlock) void k_adj_route(View<REAL, STO> v, AdjView<REAL> a) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= v.N) return;
+    const int64_t d = v.dn[k];
+    const bool mouth = (d == k);
+    const REAL qr = a.qr[k], qf = a.qf[k];
+    const REAL l_own = a.lim[k], l_dn = a.lim[d];
+    const double LR = a.a_rout[k] + (mouth ? 0.0 : a.a_ri[d]);
+    const double LF = a.a_fout[k] + (mouth ? 0.0 : a.a_fi[d]);
+    a.a_qr[k] = LR * (double)((qr >= (REAL)0.0) ? l_own : l_dn);
+    a.a_qf[k] = LF * (double)((qf >= (REAL)0.0) ? l_own : l_dn);
+    double Llim = 0.0;
+    if (qr >= (REAL)0.0 || mouth) Llim += LR * (double)qr;
+    if (qf >= (REAL)0.0 || mouth) Llim += LF * (double)qf;
+    const double Lri_k = a.a_ri[k], Lfi_k = a.a_fi[k];
+    for (int32_t p = v.up_ptr[k]; p < v.up_ptr[k + 1]; ++p) {
+        const int64_t u = CONTIG ? (int64_t)p : (int64_t)v.up_idx[p];
+        if (CONTIG && u == k) continue;
+        const REAL uq = a.qr[u], uf = a.qf[u];
+        if (uq < (REAL)0.0) Llim += (a.a_rout[u] + Lri_k) * (double)uq;
+        if (uf < (REAL)0.0) Llim += (a.a_fout[u] + Lfi_k) * (double)uf;
+    }
+    // limiter: lim = og > 1e-8 ? min(rate / og, 1) : 1, rate = rs + fs
+    const REAL og = a.og[k];
+    const REAL rate = (REAL)(adj_sto<STO>(a.rs_raw, k) + adj_sto<STO>(a.fs_raw, k));
+    double Log = 0.0;
+    if (og > (REAL)1e-8 && (rate / og) < (REAL)1.0) {
+        const double Lrate = Llim / (double)og;
+        Log = -Llim * (double)rate / ((double)og * (double)og);
+        a.lin[ADJ_RS][k] += Lrate;
+        a.lin[ADJ_FS][k] += Lrate;
+    }
+    a.a_og[k] = Log;
+}
+
+// A4: reverse of the outflow momentum update of catchment i.  The final
+// flows enter the limited flows (a_qr/a_qf) and the outgoing terms (own total
+// for the positive part, the downstream total for the negative part); the
+// cross-section depths enter the next sub-step.
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjView<REAL> a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.N) return;
+    const REAL dtr = a.dt;
+    const OutTrace<REAL> o = outflow_trace(v, a, i, dtr);
+    const double dt = (double)dtr, g = (double)v.gravity;
+    const int64_t d = v.dn[i];
+    const double og_own = a.a_og[i], og_dn = o.mouth ? 0.0 : a.a_og[d];
+    double Lqr = a.a_qr[i], Lqf = a.a_qf[i];
+    if (o.qr > (REAL)0.0) Lqr += og_own * dt;
+    if (o.qf > (REAL)0.0) Lqf += og_own * dt;
+    if (o.qr < (REAL)0.0) Lqr -= og_dn * dt;
+    if (o.qf < (REAL)0.0) Lqf -= og_dn * dt;
+    // negative-flow limiter at 5 % of the storage
+    double Lqr0, Lqf1, Ltot = 0.0;
+    if (o.negative) {
+        Lqr0 = Lqr * (double)o.limit;
+        Lqf1 = Lqf * (double)o.limit;
+        if (o.limited) {
+            const double Llimit = Lqr * (double)o.qr0 + Lqf * (double)o.qf1;
+            Ltot += Llimit * 0.05 / (double)o.total_neg;
+            const double Ltn = -Llimit * (double)o.ratio / (double)o.total_neg;
+            Lqr0 -= Ltn * dt;
+            Lqf1 -= Ltn * dt;
+        }
+    } else {
+        Lqr0 = Lqr;
+        Lqf1 = Lqf;
+    }
+    const double Lqf0 = o.killed ? 0.0 : Lqf1;
+    double Lro = 0.0, Lfo = 0.0, Lr_sifd = 0.0, Lf_sifd = 0.0, Lf_imp = 0.0, Lslope = 0.0, Lfslope = 0.0;
+    double gn = 0.0;
+    const double fo = (double)a.q_f[i];
+    if (o.flood_cond) {
+        const double fm = (double)v.flood_manning[i];
+        const double Lnum = Lqf0 / (double)o.den_f, Lden = -Lqf0 * (double)o.qf0 / (double)o.den_f;
+        Lfo += Lnum;
+        Lf_imp += Lnum * g * dt * (double)o.fslope;
+        Lfslope += Lnum * g * dt * (double)o.f_imp;
+        const double cf = g * dt * fm * fm;
+        const double fs_ = (double)o.f_sifd, fimp = (double)o.f_imp;
+        const double P = fs_ * cbrt(fs_);
+        const double sgn = fo > 0.0 ? 1.0 : (fo < 0.0 ? -1.0 : 0.0);
+        Lfo += Lden * cf * sgn / (P * fimp);
+        Lf_sifd += Lden * (-cf * fabs(fo) / fimp) * (4.0 / 3.0) / (P * fs_);
+        Lf_imp += Lden * (-cf * fabs(fo) / (P * fimp * fimp));
+    }
+    double Lupd_fcsa = a.lout[ADJ_FCSA][i], Lfcsa = 0.0;
+    if (o.flood_cond && o.f_imp_raw > (REAL)1e-6) {
+        const double fr = (double)o.f_imp_raw;
+        Lupd_fcsa += Lf_imp * 0.5 * (double)o.f_csa_m / fr;
+        if (a.fcsa[i] > (REAL)1e-6) Lfcsa += Lf_imp * 0.5 * (double)o.upd_fcsa / fr;
+    }
+    if (o.river_cond) {
+        const double rw = (double)v.river_width[i], n = (double)v.river_manning[i];
+        const double Lnum = Lqr0 / (double)o.den_r, Lden = -Lqr0 * (double)o.qr0 / (double)o.den_r;
+        const double rsf = (double)o.r_sifd;
+        Lro += Lnum;
+        Lr_sifd += Lnum * rw * g * dt * (double)o.slope;
+        Lslope += Lnum * rw * g * dt * rsf;
+        const double Q = 1.0 / (rsf * rsf * cbrt(rsf));
+        const double unit = (double)o.unit_r, au = fabs(unit);
+        const double sgn = unit > 0.0 ? 1.0 : (unit < 0.0 ? -1.0 : 0.0);
+        gn += Lden * 2.0 * g * dt * n * au * Q;
+        Lro += Lden * g * dt * n * n * sgn / rw * Q;
+        Lr_sifd += Lden * g * dt * n * n * au * (-7.0 / 3.0) * Q / rsf;
+    }
+    double Lupd_rcsd = a.lout[ADJ_RCSD][i], Lupd_fcsd = a.lout[ADJ_FCSD][i], Lrcsd = 0.0, Lfcsd = 0.0;
+    if (o.f_sifd_raw > (REAL)1e-6) {
+        const double fr = (double)o.f_sifd_raw;
+        Lupd_fcsd += Lf_sifd * 0.5 * (double)a.fcsd[i] / fr;
+        Lfcsd += Lf_sifd * 0.5 * (double)o.upd_fcsd / fr;
+    }
+    if (o.r_sifd_raw > (REAL)1e-6) {
+        const double rr = (double)o.r_sifd_raw;
+        Lupd_rcsd += Lr_sifd * 0.5 * (double)a.rcsd[i] / rr;
+        Lrcsd += Lr_sifd * 0.5 * (double)o.upd_rcsd / rr;
+    }
+    double Lfs = 0.0, Lfd = 0.0, Lrd = 0.0, Lwse = 0.0, Lwse_dn = 0.0, Lmx = 0.0, Lre = 0.0;
+    if (o.upd_fcsa_raw > (REAL)0.0) {
+        Lfs += Lupd_fcsa / (double)v.river_length[i];
+        Lfd -= Lupd_fcsa * (double)v.river_width[i];
+    }
+    const REAL c_elv = v.elev[i];
+    if (o.fsurf - c_elv > (REAL)0.0) {
+        if (o.mouth) Lwse += Lupd_fcsd;
+        else Lmx += Lupd_fcsd;
+    }
+    if (o.mouth) {
+        Lrd += Lupd_rcsd;
+    } else {
+        Lmx += Lupd_rcsd;
+        Lre -= Lupd_rcsd;
+    }
+    if (o.slope_in) Lslope += Lfslope;
+    const double dist = (double)v.dist[i];
+    Lwse += Lslope / dist;
+    Lwse_dn -= Lslope / dist;
+    if (o.mx_own) Lwse += Lmx;
+    else Lwse_dn += Lmx;
+    Lrd += Lwse;
+    Lre += Lwse;
+    Ltot += 0.0;
+    a.lin[ADJ_QR][i] = Lro;
+    a.lin[ADJ_QF][i] = Lfo;
+    a.lin[ADJ_RCSD][i] = Lrcsd;
+    a.lin[ADJ_FCSD][i] = Lfcsd;
+    a.lin[ADJ_FCSA][i] = Lfcsa;
+    a.lin[ADJ_RS][i] += Ltot;
+    a.lin[ADJ_FS][i] += Ltot + Lfs;
+    a.lin[ADJ_RD][i] = Lrd;
+    a.lin[ADJ_FD][i] = Lfd;
+    a.g_n[i] += gn;
+    a.g_h[i] += -Lre;
+    a.d_wd[i] = o.mouth ? 0.0 : Lwse_dn;
+}
+
+// A5: downstream water surface: wse_dn = river_depth[d] + elevation[d] -
+// height[d] for every upstream catchment of d.
+template <class REAL, class STO, bool CONTIG>
+__global__ __launch_bounds__(kBlock) void k_adj_gather(View<REAL, STO> v, AdjView<REAL> a) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= v.N) return;
+    double s = 0.0;
+    for (int32_t p = v.up_ptr[k]; p < v.up_ptr[k + 1]; ++p) {
+        const int64_t u = CONTIG ? (int64_t)p : (int64_t)v.up_idx[p];
+        if (CONTIG && u == k) continue;
+        s += a.d_wd[u];
+    }
+    a.lin[ADJ_RD][k] += s;
+    a.g_h[k] -= s;
+}
+
+}  // namespace cmf

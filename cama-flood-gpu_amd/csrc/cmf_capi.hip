@@ -2,6 +2,7 @@
 // sub-step schedule.  Declarations and contracts are in include/cmf_hip.h.
 
 #include "cmf_kernels.hip"
+#include "cmf_adjoint.hip"
 
 #include <algorithm>
 #include <cstring>
@@ -72,6 +73,9 @@ struct cmf_network {
     cmf_fields graph_fields;
     bool graph_fields_set = false;
     std::vector<std::pair<int, hipGraphExec_t>> graphs;
+    // adjoint scratch (cmf_hip_substep_adjoint), allocated on first use
+    void* adj_real = nullptr;          // 6 REAL-width arrays of N (8 bytes each slot)
+    double* adj_dbl = nullptr;         // 8 double arrays of N
 };
 
 namespace {
@@ -87,6 +91,8 @@ void free_network(cmf_network* net) {
     if (net->host_counts) (void)hipHostFree(net->host_counts);
     for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
     if (net->capture_stream) (void)hipStreamDestroy(net->capture_stream);
+    if (net->adj_real) (void)hipFree(net->adj_real);
+    if (net->adj_dbl) (void)hipFree(net->adj_dbl);
     delete net;
 }
 
@@ -779,6 +785,77 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
     return CMF_OK;
 }
 
+// Reverse sweep of one fused sub-step (base physics, one trial, ungrouped).
+template <class REAL, class STO>
+int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, void* stream) {
+    View<REAL, STO> v;
+    int rc = make_view<REAL, STO>(net, f, v);
+    if (rc) return rc;
+    if (!adj) return set_error(CMF_ERR_INVALID, "substep_adjoint: null adjoint table");
+    if (f->flags & (CMF_HAS_LEVEE | CMF_HAS_BIFURCATION | CMF_HAS_RESERVOIR))
+        return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: levee, bifurcation and reservoir are not differentiated");
+    if (v.T != 1 || net->G > 0 || f->batched)
+        return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: one trial, ungrouped, no per-trial parameters");
+    if ((rc = require({NEED(downstream_idx), NEED(river_manning), NEED(flood_manning), NEED(river_width),
+                       NEED(river_length), NEED(river_height), NEED(catchment_elevation), NEED(catchment_area),
+                       NEED(downstream_distance), NEED(protected_storage), NEED(runoff), NEED(flood_depth_table)},
+                      "substep_adjoint")))
+        return rc;
+    const void* state[ADJ_COUNT] = {adj->river_outflow, adj->flood_outflow, adj->river_cross_section_depth,
+                                    adj->flood_cross_section_depth, adj->flood_cross_section_area,
+                                    adj->river_storage, adj->flood_storage, adj->river_depth, adj->flood_depth};
+    for (int k = 0; k < ADJ_COUNT; ++k)
+        if (!state[k] || !adj->adj_out[k] || !adj->adj_in[k])
+            return set_error(CMF_ERR_INVALID, "substep_adjoint: state, adj_out and adj_in need all %d fields", ADJ_COUNT);
+    if (!adj->grad_river_manning || !adj->grad_river_height)
+        return set_error(CMF_ERR_INVALID, "substep_adjoint: gradient accumulators are NULL");
+    if (!(adj->time_step > 0.0)) return set_error(CMF_ERR_INVALID, "substep_adjoint: time_step must be > 0");
+    const int64_t n = v.N;
+    if (!net->adj_real && (rc = dev_alloc(&net->adj_real, (size_t)n * 8 * 6))) return rc;
+    if (!net->adj_dbl && (rc = dev_alloc((void**)&net->adj_dbl, (size_t)n * 8 * 8))) return rc;
+    AdjView<REAL> a;
+    a.q_r = static_cast<const REAL*>(adj->river_outflow);
+    a.q_f = static_cast<const REAL*>(adj->flood_outflow);
+    a.rcsd = static_cast<const REAL*>(adj->river_cross_section_depth);
+    a.fcsd = static_cast<const REAL*>(adj->flood_cross_section_depth);
+    a.fcsa = static_cast<const REAL*>(adj->flood_cross_section_area);
+    a.rs_raw = adj->river_storage;
+    a.fs_raw = adj->flood_storage;
+    a.rd = static_cast<const REAL*>(adj->river_depth);
+    a.fd = static_cast<const REAL*>(adj->flood_depth);
+    for (int k = 0; k < ADJ_COUNT; ++k) {
+        a.lout[k] = adj->adj_out[k];
+        a.lin[k] = adj->adj_in[k];
+    }
+    a.g_n = adj->grad_river_manning;
+    a.g_h = adj->grad_river_height;
+    REAL* r = static_cast<REAL*>(net->adj_real);
+    const int64_t stride = n * (int64_t)(8 / sizeof(REAL));
+    a.qr = r; a.qf = r + stride; a.pos = r + 2 * stride; a.neg = r + 3 * stride; a.lim = r + 4 * stride;
+    a.og = r + 5 * stride;
+    double* q = net->adj_dbl;
+    a.a_rout = q; a.a_fout = q + n; a.a_ri = q + 2 * n; a.a_fi = q + 3 * n; a.a_qr = q + 4 * n;
+    a.a_qf = q + 5 * n; a.a_og = q + 6 * n; a.d_wd = q + 7 * n;
+    a.dt = (REAL)adj->time_step;
+    const bool contig = net->contiguous != 0;
+    LAUNCH((k_adj_outflow_fwd<REAL, STO>), n, stream, v, a);
+    if (contig) LAUNCH((k_adj_limiter_fwd<REAL, STO, true>), n, stream, v, a);
+    else LAUNCH((k_adj_limiter_fwd<REAL, STO, false>), n, stream, v, a);
+    if (contig) {
+        if (v.L == 10) LAUNCH((k_adj_stage<REAL, STO, true, 10>), n, stream, v, a);
+        else LAUNCH((k_adj_stage<REAL, STO, true, 0>), n, stream, v, a);
+        LAUNCH((k_adj_route<REAL, STO, true>), n, stream, v, a);
+    } else {
+        if (v.L == 10) LAUNCH((k_adj_stage<REAL, STO, false, 10>), n, stream, v, a);
+        else LAUNCH((k_adj_stage<REAL, STO, false, 0>), n, stream, v, a);
+        LAUNCH((k_adj_route<REAL, STO, false>), n, stream, v, a);
+    }
+    LAUNCH((k_adj_outflow<REAL, STO>), n, stream, v, a);
+    if (contig) LAUNCH((k_adj_gather<REAL, STO, true>), n, stream, v, a);
+    else LAUNCH((k_adj_gather<REAL, STO, false>), n, stream, v, a);
+    return CMF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1020,6 +1097,13 @@ int cmf_hip_advance(cmf_network* net, const cmf_fields* f, double outer_seconds,
         return run_advance<float, double>(net, f, outer_seconds, fixed_count, out_count, s, false);
     if (f->precision == CMF_PRECISION_F64_F64)
         return run_advance<double, double>(net, f, outer_seconds, fixed_count, out_count, s, false);
+    return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
+}
+
+int cmf_hip_substep_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* a, void* s) {
+    if (!f) return set_error(CMF_ERR_INVALID, "null field table");
+    if (f->precision == CMF_PRECISION_F32_F64) return run_adjoint<float, double>(net, f, a, s);
+    if (f->precision == CMF_PRECISION_F64_F64) return run_adjoint<double, double>(net, f, a, s);
     return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
 }
 

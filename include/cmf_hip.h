@@ -346,6 +346,39 @@ int cmf_hip_substep_timed(cmf_network* net, const cmf_fields* f, int32_t substep
  * *out_count (the maximum over trials with CMF_PER_TRIAL_DT). */
 int cmf_hip_advance(cmf_network* net, const cmf_fields* f, double outer_seconds,
                     int32_t fixed_count, int32_t* out_count, void* stream);
+/* ------------------------------------------------------------- adjoint */
+/* Reverse mode of one fused sub-step of the base physics (outflow, limiter
+ * and inflow sums, storage update, flood stage; cmfgpu/phys/triton/outflow.py
+ * :15-383 and storage.py:16-193): given the adjoint of the state after the
+ * sub-step (adj_out) it writes the adjoint of the state before it (adj_in)
+ * and adds d/d(river_manning) and d/d(river_height) into the gradient
+ * accumulators, with the time step held at its forward value.  State order:
+ * river_outflow, flood_outflow, river/flood cross-section depth, flood
+ * cross-section area, river/flood storage, river/flood depth.  The state
+ * pointers give the sub-step's saved input state (dtypes of the field table);
+ * the field table supplies parameters, runoff and the flood table.  Levee,
+ * bifurcation and reservoir modules, trials, groups and per-trial parameters
+ * return CMF_ERR_UNSUPPORTED.  No reference counterpart (the reference has no
+ * differentiation). */
+typedef struct cmf_adjoint {
+    const void* river_outflow;
+    const void* flood_outflow;
+    const void* river_cross_section_depth;
+    const void* flood_cross_section_depth;
+    const void* flood_cross_section_area;
+    const void* river_storage;
+    const void* flood_storage;
+    const void* river_depth;
+    const void* flood_depth;
+    const double* adj_out[9];
+    double* adj_in[9];
+    double* grad_river_manning;
+    double* grad_river_height;
+    double time_step;
+} cmf_adjoint;
+int cmf_hip_substep_adjoint(cmf_network* net, const cmf_fields* fields, const cmf_adjoint* adjoint,
+                            void* stream);
+
 /* cmf_hip_advance with the sub-step loop replayed from a HIP graph: the loop
  * for a given sub-step count is captured once (on a private stream; kernel
  * nodes only) and relaunched on `stream` in later outer steps while the field

@@ -1,0 +1,140 @@
+"""Hand-written adjoint of the fused sub-step (cmf_hip_substep_adjoint) against
+FP64 torch autograd through a plain PyTorch-op transcription of the reference
+step (oracle/torch_step.py) and against central finite differences.
+
+Problem: plain network of 3,000 catchments, 16 gauges (largest upstream
+area), loss = mean squared log-discharge error over 8 outer steps against a
+run whose Manning's n and river height were scaled by lognormal(0, 0.2)
+factors; checkpoints every 3 outer steps (so the backward re-runs segments).
+
+Tolerances: FP64 adjoint vs FP64 autograd, relative L2 of each gradient 1e-6
+(the two sum floats in different orders); central differences (h = 1e-3
+relative, sub-step counts held) relative 2e-2 per component; the FP32 adjoint
+against the FP64 one, relative L2 5e-2.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+PLAIN = ("base", "adaptive_time")
+STEPS = 8
+
+
+def _setup(precision):
+    from cmfgpu_amd import synthetic
+    from cmfgpu_amd.adjoint import GaugeProblem
+    from cmfgpu_amd.models import CaMaFlood
+    net = synthetic.generate_network(3000, 1003)
+    src = dict(net.fields)
+    src["runoff"] = synthetic.Runoff(src["catchment_area"], 2003).next_day() * np.float32(4.0)
+    gauges = np.argsort(net.upstream_area)[-16:]
+    model = CaMaFlood(src, opened_modules=PLAIN, device="cuda:0", step_duration=1800.0, precision=precision)
+    rng = np.random.default_rng(7)
+    n0, h0 = src["river_manning"].astype(np.float64), src["river_height"].astype(np.float64)
+    n_true = n0 * rng.lognormal(0.0, 0.2, n0.shape[0])
+    h_true = h0 * rng.lognormal(0.0, 0.2, h0.shape[0])
+    problem = GaugeProblem(model, gauges, np.ones((STEPS, 16)), checkpoint_every=3)
+    # observations: the discharge of a run with the perturbed parameters
+    problem.set_parameters(n_true, h_true)
+    obs = []
+    for _ in range(STEPS):
+        model.step_advance()
+        obs.append(problem._gauge_q().cpu().numpy())
+    problem.set_parameters(n0, h0)
+    problem._restore(problem.initial)
+    problem.obs = torch.as_tensor(np.array(obs), dtype=torch.float64, device=model.device)
+    return net, src, model, problem, gauges
+
+
+def _torch_gradient(model, problem, gauges, n0, h0):
+    from oracle.torch_step import substep
+    problem._restore(problem.initial)
+    st = {k: torch.tensor(model.get(k), dtype=torch.float64) for k in (
+        "river_outflow", "flood_outflow", "river_cross_section_depth", "flood_cross_section_depth",
+        "flood_cross_section_area", "river_storage", "flood_storage", "river_depth", "flood_depth")}
+    prm = {k: torch.tensor(model.get(k), dtype=torch.float64) for k in (
+        "flood_manning", "river_width", "river_length", "catchment_elevation", "catchment_area",
+        "downstream_distance", "runoff", "flood_depth_table")}
+    prm["downstream_idx"] = torch.tensor(model.get("downstream_idx"), dtype=torch.int64)
+    n = torch.tensor(n0, dtype=torch.float64, requires_grad=True)
+    h = torch.tensor(h0, dtype=torch.float64, requires_grad=True)
+    prm["river_manning"], prm["river_height"] = n, h
+    obs = problem.obs.cpu()
+    g = torch.tensor(gauges)
+    loss = torch.zeros((), dtype=torch.float64)
+    for step, cnt in enumerate(problem.counts):
+        dt = 1800.0 / cnt
+        for _ in range(cnt):
+            st = substep(st, prm, dt)
+        q = st["river_outflow"][g] + st["flood_outflow"][g]
+        qc = torch.where(q > 0, q, torch.zeros_like(q))
+        lo = torch.log(obs[step].clamp_min(0.0) + problem.eps)
+        loss = loss + ((torch.log(qc + problem.eps) - lo) ** 2).sum()
+    loss = loss / (len(problem.counts) * g.shape[0])
+    loss.backward()
+    return float(loss), n.grad.numpy(), h.grad.numpy()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_adjoint_matches_fp64_autograd():
+    net, src, model, problem, gauges = _setup("float64")
+    loss = problem.run()
+    assert loss > 0 and len(problem.checkpoints) == 3
+    g_n, g_h = (x.cpu().numpy() for x in problem.gradient())
+    n0, h0 = (x.astype(np.float64) for x in problem.parameters())
+    t_loss, t_n, t_h = _torch_gradient(model, problem, gauges, n0, h0)
+    assert abs(t_loss - loss) <= 1e-9 * abs(loss)
+    assert np.count_nonzero(t_n) > 50
+    print(f"adjoint vs FP64 autograd: manning {_rel(g_n, t_n):.2e}, height {_rel(g_h, t_h):.2e}", flush=True)
+    assert _rel(g_n, t_n) <= 1e-6, _rel(g_n, t_n)
+    assert _rel(g_h, t_h) <= 1e-6, _rel(g_h, t_h)
+    model.close()
+
+
+def test_adjoint_against_central_differences():
+    net, src, model, problem, gauges = _setup("float64")
+    problem.run()
+    counts = list(problem.counts)
+    g_n, g_h = (x.cpu().numpy() for x in problem.gradient())
+    n0, h0 = (x.astype(np.float64) for x in problem.parameters())
+    rng = np.random.default_rng(11)
+    cand_n = np.flatnonzero(np.abs(g_n) > 1e-3 * np.abs(g_n).max())
+    cand_h = np.flatnonzero(np.abs(g_h) > 1e-3 * np.abs(g_h).max())
+    checks = [("n", i) for i in rng.choice(cand_n, 8, replace=False)] + \
+             [("h", i) for i in rng.choice(cand_h, 8, replace=False)]
+    for kind, i in checks:
+        base = n0 if kind == "n" else h0
+        step = 1e-3 * base[i]
+        vals = []
+        for sgn in (1.0, -1.0):
+            x = base.copy()
+            x[i] += sgn * step
+            if kind == "n":
+                problem.set_parameters(x, h0)
+            else:
+                problem.set_parameters(n0, x)
+            vals.append(problem.run(fixed_counts=counts))
+        problem.set_parameters(n0, h0)
+        fd = (vals[0] - vals[1]) / (2.0 * step)
+        ad = (g_n if kind == "n" else g_h)[i]
+        print(f"central difference {kind}[{i}]: fd {fd:.6e} adjoint {ad:.6e} rel {abs(fd - ad) / abs(ad):.2e}",
+              flush=True)
+        assert abs(fd - ad) <= 2e-2 * abs(ad), (kind, i, fd, ad)
+    model.close()
+
+
+def test_fp32_adjoint_close_to_fp64():
+    grads = {}
+    for prec in ("float64", "float32"):
+        net, src, model, problem, gauges = _setup(prec)
+        problem.run()
+        grads[prec] = [x.cpu().numpy() for x in problem.gradient()]
+        model.close()
+    for a, b in zip(grads["float32"], grads["float64"]):
+        print(f"FP32 adjoint vs FP64 adjoint: {_rel(a, b):.2e}", flush=True)
+        assert _rel(a, b) <= 5e-2, _rel(a, b)
