@@ -199,6 +199,50 @@ def graph_line(args, device: str) -> dict:
     return out
 
 
+def adjoint_line(args, device: str) -> dict:
+    """Cost of the adjoint on the 6-arcmin network with the base physics (the
+    differentiated modules): 2,000 gauges with upstream area > 5,000 km^2,
+    `--adjoint-steps` outer steps, checkpoints every 48.  Forward time per
+    sub-step against backward time (segment re-run + reverse sweep) per
+    sub-step."""
+    import numpy as np
+    import torch
+    from cmfgpu_amd import synthetic
+    from cmfgpu_amd.adjoint import GaugeProblem
+    from cmfgpu_amd.models import CaMaFlood
+    net = synthetic.build_config(3, scale=args.scale)
+    src = {k: v for k, v in net.fields.items()
+           if not k.startswith(("levee_", "bifurcation_"))}
+    src["runoff"] = synthetic.Runoff(src["catchment_area"], 2003).next_day()
+    model = CaMaFlood(src, opened_modules=("base", "adaptive_time"), device=device, step_duration=1800.0)
+    gauges = net.gauges
+    steps = args.adjoint_steps
+    problem = GaugeProblem(model, gauges, np.ones((steps, gauges.shape[0])), checkpoint_every=48)
+    # warm-up: one short forward and backward (first use of every kernel)
+    warm = GaugeProblem(model, gauges, np.ones((2, gauges.shape[0])), checkpoint_every=48)
+    warm.run()
+    warm.gradient()
+    problem._restore(problem.initial)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    problem.run()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g_n, g_h = problem.gradient()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    sub = sum(problem.counts)
+    out = {"workload": f"{net.num_catchments} catchments, base physics, {gauges.shape[0]} gauges",
+           "outer_steps": steps, "substeps": sub, "forward_ms_per_substep": t_fwd / sub * 1e3,
+           "backward_ms_per_substep": t_bwd / sub * 1e3, "ratio": round(t_bwd / t_fwd, 3),
+           "gradient_finite": bool(torch.isfinite(g_n).all().item() and torch.isfinite(g_h).all().item())}
+    log(f"adjoint: forward {out['forward_ms_per_substep']:.3f} ms/sub-step, backward "
+        f"{out['backward_ms_per_substep']:.3f} ms/sub-step ({out['ratio']}x)")
+    model.close()
+    return out
+
+
 def run_steps(model, steps: int, per_trial: bool):
     """Advance `steps` outer steps; returns member-catchment-sub-steps done."""
     import torch
@@ -280,6 +324,7 @@ def main():
     ap.add_argument("--basins", type=int, default=400, help="uneven basins of the grouped line (0: skip)")
     ap.add_argument("--basins-scale", type=float, default=1.0, help="basin size factor (1.0: about 20M catchments)")
     ap.add_argument("--autotune", type=int, default=1, help="pick kernel block sizes by timing first (0: 256)")
+    ap.add_argument("--adjoint-steps", type=int, default=48, help="outer steps of the adjoint line (0: skip)")
     ap.add_argument("--graph", type=int, default=1, help="eager vs HIP-graph line on the small network (0: skip)")
     ap.add_argument("--error-days", type=int, default=1,
                     help="simulated days of the FP32-vs-FP64 error run (0: skip)")
@@ -382,6 +427,7 @@ def main():
     if args.basins > 0:
         basins = basins_line(args, device)
     graph = graph_line(args, device) if args.graph else None
+    adjoint = adjoint_line(args, device) if args.adjoint_steps > 0 else None
 
     errors = None
     if args.error_days > 0:
@@ -397,6 +443,7 @@ def main():
                    "levees": net.num_levees, "flood_levels": 10, "outer_step_s": 1800.0,
                    "substeps_timed": int(substeps), "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "basins": basins, "graph": graph,
+        "adjoint": adjoint,
         "block_sizes": {k: v["block"] for k, v in tuned.items()} if tuned else None,
         "error_vs_fp64": errors,
     }

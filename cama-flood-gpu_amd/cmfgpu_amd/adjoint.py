@@ -33,16 +33,29 @@ STATE = ("river_outflow", "flood_outflow", "river_cross_section_depth", "flood_c
 QR, QF = 0, 1
 
 
+INTERMEDIATES = ("qraw_r", "qraw_f", "lim")
+# tape slot layout: REAL rows (state without storages, then intermediates), STO rows
+TAPE_REAL = ("river_outflow", "flood_outflow", "river_cross_section_depth", "flood_cross_section_depth",
+             "flood_cross_section_area", "river_depth", "flood_depth") + INTERMEDIATES
+TAPE_STO = ("river_storage", "flood_storage")
+
+
 class CmfAdjoint(ctypes.Structure):
     """Mirror of struct cmf_adjoint."""
 
-    _fields_ = [(name, ctypes.c_void_p) for name in STATE] + [
+    _fields_ = [(name, ctypes.c_void_p) for name in STATE + INTERMEDIATES] + [
         ("adj_out", ctypes.c_void_p * len(STATE)),
         ("adj_in", ctypes.c_void_p * len(STATE)),
         ("grad_river_manning", ctypes.c_void_p),
         ("grad_river_height", ctypes.c_void_p),
         ("time_step", ctypes.c_double),
     ]
+
+
+class CmfTape(ctypes.Structure):
+    """Mirror of struct cmf_tape."""
+
+    _fields_ = [(name, ctypes.c_void_p) for name in STATE + INTERMEDIATES]
 
 
 _SIG_DONE = False
@@ -56,6 +69,9 @@ def _declare():
     lib.cmf_hip_substep_adjoint.restype = ctypes.c_int
     lib.cmf_hip_substep_adjoint.argtypes = [ctypes.c_void_p, ctypes.POINTER(_lib.CmfFields),
                                             ctypes.POINTER(CmfAdjoint), ctypes.c_void_p]
+    lib.cmf_hip_substep_taped.restype = ctypes.c_int
+    lib.cmf_hip_substep_taped.argtypes = [ctypes.c_void_p, ctypes.POINTER(_lib.CmfFields), ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.POINTER(CmfTape), ctypes.c_void_p]
     _SIG_DONE = True
 
 
@@ -176,46 +192,62 @@ class GaugeProblem:
         m = self.model
         dev = m.device
         n = m.N
-        lam = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in STATE]
-        lam_new = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in STATE]
+        real_t = m.t["river_outflow"].dtype
+        sto_t = m.t["river_storage"].dtype
+        lam = [[torch.zeros(n, dtype=real_t, device=dev) for _ in STATE] for _ in range(2)]
         g_n = torch.zeros(n, dtype=torch.float64, device=dev)
         g_h = torch.zeros(n, dtype=torch.float64, device=dev)
-        adj = CmfAdjoint()
-        adj.grad_river_manning = g_n.data_ptr()
-        adj.grad_river_height = g_h.data_ptr()
         starts = sorted(self.checkpoints)
+        slots = max(sum(self.counts[s0:min(s0 + self.K, self.steps)]) for s0 in starts)
+        tape_r = torch.empty((slots, len(TAPE_REAL), n), dtype=real_t, device=dev)
+        tape_s = torch.empty((slots, len(TAPE_STO), n), dtype=sto_t, device=dev)
+        tapes, adjs = [], []
+        for k in range(slots):
+            ptrs = {name: tape_r[k, x].data_ptr() for x, name in enumerate(TAPE_REAL)}
+            ptrs.update({name: tape_s[k, x].data_ptr() for x, name in enumerate(TAPE_STO)})
+            tp = CmfTape(**ptrs)
+            pair = []
+            for parity in range(2):
+                a = CmfAdjoint(**ptrs)
+                for x in range(len(STATE)):
+                    a.adj_out[x] = lam[parity][x].data_ptr()
+                    a.adj_in[x] = lam[1 - parity][x].data_ptr()
+                a.grad_river_manning = g_n.data_ptr()
+                a.grad_river_height = g_h.data_ptr()
+                pair.append(a)
+            tapes.append(tp)
+            adjs.append(pair)
+        lib = _lib.lib()
+        parity = 0
         for s0 in reversed(starts):
             s1 = min(s0 + self.K, self.steps)
             self._restore(self.checkpoints[s0])
-            tape = []
+            marks = []
+            slot = 0
             for step in range(s0, s1):
                 self._apply_runoff(step)
                 cnt = self.counts[step]
                 m.t["time_step"].fill_(m.step_duration / cnt)
                 for k in range(cnt):
-                    tape.append((step, k, self._snapshot()))
-                    self._substep(k, k == cnt - 1)
-                # end of the outer step: the loss term's discharge
-                tape.append((step, -1, self._gauge_q()))
-            for step, k, item in reversed(tape):
-                if k < 0:
-                    seed = self._loss_seed(step, item)
-                    lam[QR].index_add_(0, self.g_dev, seed)
-                    lam[QF].index_add_(0, self.g_dev, seed)
-                    self._apply_runoff(step)
-                    continue
+                    _lib.check(lib.cmf_hip_substep_taped(m.network_handle, m.field_table_ref(), k,
+                                                         1 if k == cnt - 1 else 0, ctypes.byref(tapes[slot]),
+                                                         _lib.current_stream()), "cmf_hip_substep_taped")
+                    slot += 1
+                marks.append((step, slot, self._gauge_q()))
+            for step, end, q in reversed(marks):
                 cnt = self.counts[step]
-                for f, t in zip(STATE, item):
-                    setattr(adj, f, t.data_ptr())
-                for x in range(len(STATE)):
-                    adj.adj_out[x] = lam[x].data_ptr()
-                    adj.adj_in[x] = lam_new[x].data_ptr()
-                adj.time_step = m.step_duration / cnt
-                _lib.check(_lib.lib().cmf_hip_substep_adjoint(m.network_handle, m.field_table_ref(),
-                                                              ctypes.byref(adj), _lib.current_stream()),
-                           "cmf_hip_substep_adjoint")
-                lam, lam_new = lam_new, lam
-            del tape
+                seed = self._loss_seed(step, q).to(real_t)
+                lam[parity][QR].index_add_(0, self.g_dev, seed)
+                lam[parity][QF].index_add_(0, self.g_dev, seed)
+                self._apply_runoff(step)
+                dt = m.step_duration / cnt
+                for slot in range(end - 1, end - 1 - cnt, -1):
+                    a = adjs[slot][parity]
+                    a.time_step = dt
+                    _lib.check(lib.cmf_hip_substep_adjoint(m.network_handle, m.field_table_ref(), ctypes.byref(a),
+                                                           _lib.current_stream()), "cmf_hip_substep_adjoint")
+                    parity = 1 - parity
+        del tape_r, tape_s
         self._restore(self.final)
         inv = torch.as_tensor(m.inv, device=dev)
         return g_n[inv], g_h[inv]

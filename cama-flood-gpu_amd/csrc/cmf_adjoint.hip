@@ -15,8 +15,8 @@
 // state; derivatives are formed in double precision.
 //
 // Passes (each over catchments, one grid-wide boundary between them):
-//   A0  recompute the unlimited outflows and outgoing terms
-//   A1  recompute the outgoing totals and storage limiters
+//   A0  recompute the unlimited outflows and outgoing terms  } skipped when the
+//   A1  recompute the outgoing totals and storage limiters   } forward taped them
 //   A2  reverse of the storage update and stage diagnosis
 //   A3  reverse of the limited flows and inflow sums, then of the limiter
 //   A4  reverse of the outflow momentum update
@@ -28,17 +28,21 @@ enum {
     ADJ_QR = 0, ADJ_QF, ADJ_RCSD, ADJ_FCSD, ADJ_FCSA, ADJ_RS, ADJ_FS, ADJ_RD, ADJ_FD, ADJ_COUNT
 };
 
+// Adjoint arrays are held in REAL (the profile's flux precision); the
+// arithmetic of every pass is double; the gradients accumulate in double.
 template <class REAL>
 struct AdjView {
     // saved input state of the sub-step
     const REAL *q_r, *q_f, *rcsd, *fcsd, *fcsa, *rd, *fd;
     const void *rs_raw, *fs_raw;
-    const double* lout[ADJ_COUNT];   // adjoint of the sub-step outputs
-    double* lin[ADJ_COUNT];          // adjoint of the sub-step inputs (written)
+    const REAL* lout[ADJ_COUNT];     // adjoint of the sub-step outputs
+    REAL* lin[ADJ_COUNT];            // adjoint of the sub-step inputs (written)
     double *g_n, *g_h;               // gradient accumulators
-    // scratch
-    REAL *qr, *qf, *pos, *neg, *lim, *og;
-    double *a_rout, *a_fout, *a_ri, *a_fi, *a_qr, *a_qf, *a_og, *d_wd;
+    // forward intermediates: unlimited flows and limiters (taped, or
+    // recomputed into scratch by A0/A1), outgoing terms (A0 scratch)
+    const REAL *qr, *qf, *lim;
+    REAL *w_qr, *w_qf, *w_lim, *pos, *neg;
+    REAL *a_rout, *a_fout, *a_ri, *a_fi, *a_qr, *a_qf, *a_og, *d_wd;
     REAL dt;
 };
 
@@ -157,8 +161,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow_fwd(View<REAL, STO> v, A
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.N) return;
     const OutTrace<REAL> o = outflow_trace(v, a, i, a.dt);
-    a.qr[i] = o.qr;
-    a.qf[i] = o.qf;
+    a.w_qr[i] = o.qr;
+    a.w_qf[i] = o.qf;
     const REAL pos = vmax(o.qr, (REAL)0.0) + vmax(o.qf, (REAL)0.0);
     const REAL neg = vmin(o.qr, (REAL)0.0) + vmin(o.qf, (REAL)0.0);
     a.pos[i] = pos * a.dt;
@@ -180,8 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_limiter_fwd(View<REAL, STO> v, A
     acc += (STO)a.pos[j];
     const REAL og = (REAL)acc;
     const REAL rate = (REAL)(adj_sto<STO>(a.rs_raw, j) + adj_sto<STO>(a.fs_raw, j));
-    a.og[j] = og;
-    a.lim[j] = (og > (REAL)1e-8) ? vmin(rate / og, (REAL)1.0) : (REAL)1.0;
+    a.w_lim[j] = (og > (REAL)1e-8) ? vmin(rate / og, (REAL)1.0) : (REAL)1.0;
 }
 
 // ------------------------------------------------------ stage reverse ----
@@ -228,8 +231,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     const REAL rms = rl * rw * rh;
     const int L = LFIX > 0 ? LFIX : (int)v.L;
     double Ltot = 0.0, LH = 0.0;
-    const double Lrs_out = a.lout[ADJ_RS][j], Lfs_out = a.lout[ADJ_FS][j];
-    const double Lrd_out = a.lout[ADJ_RD][j], Lfd_out = a.lout[ADJ_FD][j];
+    const double Lrs_out = (double)a.lout[ADJ_RS][j], Lfs_out = (double)a.lout[ADJ_FS][j];
+    const double Lrd_out = (double)a.lout[ADJ_RD][j], Lfd_out = (double)a.lout[ADJ_FD][j];
     double Lrsf = Lrs_out + Lrd_out / ((double)rl * (double)rw);
     if (!(total > rms)) {
         // dry: rsf = total, flood depth 0, flood storage max(0, 0)
@@ -300,12 +303,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     double Lfs = Lb, Lfin = Lb * (double)dt, Lfout = -Lb * (double)dt;
     if (carry) La += Lb;
     double Lrs = La, Lrin = La * (double)dt, Lrout = -La * (double)dt;
-    a.a_rout[j] = Lrout + a.lout[ADJ_QR][j];
-    a.a_fout[j] = Lfout + a.lout[ADJ_QF][j];
-    a.a_ri[j] = Lrin;
-    a.a_fi[j] = Lfin;
-    a.lin[ADJ_RS][j] = Lrs;
-    a.lin[ADJ_FS][j] = Lfs;
+    a.a_rout[j] = (REAL)(Lrout + (double)a.lout[ADJ_QR][j]);
+    a.a_fout[j] = (REAL)(Lfout + (double)a.lout[ADJ_QF][j]);
+    a.a_ri[j] = (REAL)Lrin;
+    a.a_fi[j] = (REAL)Lfin;
+    a.lin[ADJ_RS][j] = (REAL)Lrs;
+    a.lin[ADJ_FS][j] = (REAL)Lfs;
     a.g_h[j] += LH;
 }
 
@@ -322,32 +325,38 @@ __global__ __launch_bounds__(kBlock) void k_adj_route(View<REAL, STO> v, AdjView
     const bool mouth = (d == k);
     const REAL qr = a.qr[k], qf = a.qf[k];
     const REAL l_own = a.lim[k], l_dn = a.lim[d];
-    const double LR = a.a_rout[k] + (mouth ? 0.0 : a.a_ri[d]);
-    const double LF = a.a_fout[k] + (mouth ? 0.0 : a.a_fi[d]);
-    a.a_qr[k] = LR * (double)((qr >= (REAL)0.0) ? l_own : l_dn);
-    a.a_qf[k] = LF * (double)((qf >= (REAL)0.0) ? l_own : l_dn);
+    const double LR = (double)a.a_rout[k] + (mouth ? 0.0 : (double)a.a_ri[d]);
+    const double LF = (double)a.a_fout[k] + (mouth ? 0.0 : (double)a.a_fi[d]);
+    a.a_qr[k] = (REAL)(LR * (double)((qr >= (REAL)0.0) ? l_own : l_dn));
+    a.a_qf[k] = (REAL)(LF * (double)((qf >= (REAL)0.0) ? l_own : l_dn));
     double Llim = 0.0;
     if (qr >= (REAL)0.0 || mouth) Llim += LR * (double)qr;
     if (qf >= (REAL)0.0 || mouth) Llim += LF * (double)qf;
-    const double Lri_k = a.a_ri[k], Lfi_k = a.a_fi[k];
+    const double Lri_k = (double)a.a_ri[k], Lfi_k = (double)a.a_fi[k];
+    // the outgoing total of k, as the forward formed it: upstream negative
+    // terms in order, then k's own positive term
+    const REAL dt = a.dt;
+    STO og_acc = (STO)0;
     for (int32_t p = v.up_ptr[k]; p < v.up_ptr[k + 1]; ++p) {
         const int64_t u = CONTIG ? (int64_t)p : (int64_t)v.up_idx[p];
         if (CONTIG && u == k) continue;
         const REAL uq = a.qr[u], uf = a.qf[u];
-        if (uq < (REAL)0.0) Llim += (a.a_rout[u] + Lri_k) * (double)uq;
-        if (uf < (REAL)0.0) Llim += (a.a_fout[u] + Lfi_k) * (double)uf;
+        og_acc += (STO)(-(vmin(uq, (REAL)0.0) + vmin(uf, (REAL)0.0)) * dt);
+        if (uq < (REAL)0.0) Llim += ((double)a.a_rout[u] + Lri_k) * (double)uq;
+        if (uf < (REAL)0.0) Llim += ((double)a.a_fout[u] + Lfi_k) * (double)uf;
     }
+    og_acc += (STO)((vmax(qr, (REAL)0.0) + vmax(qf, (REAL)0.0)) * dt);
     // limiter: lim = og > 1e-8 ? min(rate / og, 1) : 1, rate = rs + fs
-    const REAL og = a.og[k];
+    const REAL og = (REAL)og_acc;
     const REAL rate = (REAL)(adj_sto<STO>(a.rs_raw, k) + adj_sto<STO>(a.fs_raw, k));
     double Log = 0.0;
     if (og > (REAL)1e-8 && (rate / og) < (REAL)1.0) {
         const double Lrate = Llim / (double)og;
         Log = -Llim * (double)rate / ((double)og * (double)og);
-        a.lin[ADJ_RS][k] += Lrate;
-        a.lin[ADJ_FS][k] += Lrate;
+        a.lin[ADJ_RS][k] = (REAL)((double)a.lin[ADJ_RS][k] + Lrate);
+        a.lin[ADJ_FS][k] = (REAL)((double)a.lin[ADJ_FS][k] + Lrate);
     }
-    a.a_og[k] = Log;
+    a.a_og[k] = (REAL)Log;
 }
 
 // A4: reverse of the outflow momentum update of catchment i.  The final
@@ -362,8 +371,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
     const OutTrace<REAL> o = outflow_trace(v, a, i, dtr);
     const double dt = (double)dtr, g = (double)v.gravity;
     const int64_t d = v.dn[i];
-    const double og_own = a.a_og[i], og_dn = o.mouth ? 0.0 : a.a_og[d];
-    double Lqr = a.a_qr[i], Lqf = a.a_qf[i];
+    const double og_own = (double)a.a_og[i], og_dn = o.mouth ? 0.0 : (double)a.a_og[d];
+    double Lqr = (double)a.a_qr[i], Lqf = (double)a.a_qf[i];
     if (o.qr > (REAL)0.0) Lqr += og_own * dt;
     if (o.qf > (REAL)0.0) Lqf += og_own * dt;
     if (o.qr < (REAL)0.0) Lqr -= og_dn * dt;
@@ -402,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
         Lf_sifd += Lden * (-cf * fabs(fo) / fimp) * (4.0 / 3.0) / (P * fs_);
         Lf_imp += Lden * (-cf * fabs(fo) / (P * fimp * fimp));
     }
-    double Lupd_fcsa = a.lout[ADJ_FCSA][i], Lfcsa = 0.0;
+    double Lupd_fcsa = (double)a.lout[ADJ_FCSA][i], Lfcsa = 0.0;
     if (o.flood_cond && o.f_imp_raw > (REAL)1e-6) {
         const double fr = (double)o.f_imp_raw;
         Lupd_fcsa += Lf_imp * 0.5 * (double)o.f_csa_m / fr;
@@ -422,7 +431,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
         Lro += Lden * g * dt * n * n * sgn / rw * Q;
         Lr_sifd += Lden * g * dt * n * n * au * (-7.0 / 3.0) * Q / rsf;
     }
-    double Lupd_rcsd = a.lout[ADJ_RCSD][i], Lupd_fcsd = a.lout[ADJ_FCSD][i], Lrcsd = 0.0, Lfcsd = 0.0;
+    double Lupd_rcsd = (double)a.lout[ADJ_RCSD][i], Lupd_fcsd = (double)a.lout[ADJ_FCSD][i], Lrcsd = 0.0,
+           Lfcsd = 0.0;
     if (o.f_sifd_raw > (REAL)1e-6) {
         const double fr = (double)o.f_sifd_raw;
         Lupd_fcsd += Lf_sifd * 0.5 * (double)a.fcsd[i] / fr;
@@ -457,19 +467,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
     else Lwse_dn += Lmx;
     Lrd += Lwse;
     Lre += Lwse;
-    Ltot += 0.0;
-    a.lin[ADJ_QR][i] = Lro;
-    a.lin[ADJ_QF][i] = Lfo;
-    a.lin[ADJ_RCSD][i] = Lrcsd;
-    a.lin[ADJ_FCSD][i] = Lfcsd;
-    a.lin[ADJ_FCSA][i] = Lfcsa;
-    a.lin[ADJ_RS][i] += Ltot;
-    a.lin[ADJ_FS][i] += Ltot + Lfs;
-    a.lin[ADJ_RD][i] = Lrd;
-    a.lin[ADJ_FD][i] = Lfd;
+    a.lin[ADJ_QR][i] = (REAL)Lro;
+    a.lin[ADJ_QF][i] = (REAL)Lfo;
+    a.lin[ADJ_RCSD][i] = (REAL)Lrcsd;
+    a.lin[ADJ_FCSD][i] = (REAL)Lfcsd;
+    a.lin[ADJ_FCSA][i] = (REAL)Lfcsa;
+    a.lin[ADJ_RS][i] = (REAL)((double)a.lin[ADJ_RS][i] + Ltot);
+    a.lin[ADJ_FS][i] = (REAL)((double)a.lin[ADJ_FS][i] + Ltot + Lfs);
+    a.lin[ADJ_RD][i] = (REAL)Lrd;
+    a.lin[ADJ_FD][i] = (REAL)Lfd;
     a.g_n[i] += gn;
     a.g_h[i] += -Lre;
-    a.d_wd[i] = o.mouth ? 0.0 : Lwse_dn;
+    a.d_wd[i] = o.mouth ? (REAL)0.0 : (REAL)Lwse_dn;
 }
 
 // A5: downstream water surface: wse_dn = river_depth[d] + elevation[d] -
@@ -482,9 +491,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(View<REAL, STO> v, AdjVie
     for (int32_t p = v.up_ptr[k]; p < v.up_ptr[k + 1]; ++p) {
         const int64_t u = CONTIG ? (int64_t)p : (int64_t)v.up_idx[p];
         if (CONTIG && u == k) continue;
-        s += a.d_wd[u];
+        s += (double)a.d_wd[u];
     }
-    a.lin[ADJ_RD][k] += s;
+    a.lin[ADJ_RD][k] = (REAL)((double)a.lin[ADJ_RD][k] + s);
     a.g_h[k] -= s;
 }
 

@@ -73,9 +73,9 @@ struct cmf_network {
     cmf_fields graph_fields;
     bool graph_fields_set = false;
     std::vector<std::pair<int, hipGraphExec_t>> graphs;
-    // adjoint scratch (cmf_hip_substep_adjoint), allocated on first use
-    void* adj_real = nullptr;          // 6 REAL-width arrays of N (8 bytes each slot)
-    double* adj_dbl = nullptr;         // 8 double arrays of N
+    // adjoint scratch (cmf_hip_substep_adjoint), allocated on first use:
+    // 13 arrays of N elements of up to 8 bytes
+    void* adj_scratch = nullptr;
 };
 
 namespace {
@@ -91,8 +91,7 @@ void free_network(cmf_network* net) {
     if (net->host_counts) (void)hipHostFree(net->host_counts);
     for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
     if (net->capture_stream) (void)hipStreamDestroy(net->capture_stream);
-    if (net->adj_real) (void)hipFree(net->adj_real);
-    if (net->adj_dbl) (void)hipFree(net->adj_dbl);
+    if (net->adj_scratch) (void)hipFree(net->adj_scratch);
     delete net;
 }
 
@@ -785,6 +784,39 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
     return CMF_OK;
 }
 
+// One fused sub-step that also records the adjoint's tape: the outflow pass
+// writes the input state, the unlimited flows and limiters go to the tape
+// instead of the network scratch.
+template <class REAL, class STO>
+int run_substep_taped(cmf_network* net, const cmf_fields* f, int substep, int store_inflow, const cmf_tape* tp,
+                      void* stream) {
+    View<REAL, STO> v;
+    int rc = make_view<REAL, STO>(net, f, v);
+    if (rc) return rc;
+    if ((rc = need_base(f, "substep_taped")) || (rc = need_features(f, "substep_taped"))) return rc;
+    if (!tp) return set_error(CMF_ERR_INVALID, "substep_taped: null tape");
+    if (v.T != 1 || net->G > 0 || (f->flags & (CMF_HAS_LEVEE | CMF_HAS_BIFURCATION | CMF_HAS_RESERVOIR)))
+        return set_error(CMF_ERR_UNSUPPORTED, "substep_taped: base physics, one trial, ungrouped");
+    void* ptrs[] = {tp->river_outflow, tp->flood_outflow, tp->river_cross_section_depth,
+                    tp->flood_cross_section_depth, tp->flood_cross_section_area, tp->river_storage,
+                    tp->flood_storage, tp->river_depth, tp->flood_depth, tp->qraw_r, tp->qraw_f, tp->lim};
+    for (void* p : ptrs)
+        if (!p) return set_error(CMF_ERR_INVALID, "substep_taped: every tape slot is needed");
+    v.tape_q_r = static_cast<REAL*>(tp->river_outflow);
+    v.tape_q_f = static_cast<REAL*>(tp->flood_outflow);
+    v.tape_rcsd = static_cast<REAL*>(tp->river_cross_section_depth);
+    v.tape_fcsd = static_cast<REAL*>(tp->flood_cross_section_depth);
+    v.tape_fcsa = static_cast<REAL*>(tp->flood_cross_section_area);
+    v.tape_rs = static_cast<STO*>(tp->river_storage);
+    v.tape_fs = static_cast<STO*>(tp->flood_storage);
+    v.tape_rd = static_cast<REAL*>(tp->river_depth);
+    v.tape_fd = static_cast<REAL*>(tp->flood_depth);
+    v.qraw_r = static_cast<REAL*>(tp->qraw_r);
+    v.qraw_f = static_cast<REAL*>(tp->qraw_f);
+    v.lim = static_cast<REAL*>(tp->lim);
+    return fused_substep(net, v, f, substep, store_inflow, stream);
+}
+
 // Reverse sweep of one fused sub-step (base physics, one trial, ungrouped).
 template <class REAL, class STO>
 int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, void* stream) {
@@ -811,8 +843,7 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
         return set_error(CMF_ERR_INVALID, "substep_adjoint: gradient accumulators are NULL");
     if (!(adj->time_step > 0.0)) return set_error(CMF_ERR_INVALID, "substep_adjoint: time_step must be > 0");
     const int64_t n = v.N;
-    if (!net->adj_real && (rc = dev_alloc(&net->adj_real, (size_t)n * 8 * 6))) return rc;
-    if (!net->adj_dbl && (rc = dev_alloc((void**)&net->adj_dbl, (size_t)n * 8 * 8))) return rc;
+    if (!net->adj_scratch && (rc = dev_alloc(&net->adj_scratch, (size_t)n * 8 * 13))) return rc;
     AdjView<REAL> a;
     a.q_r = static_cast<const REAL*>(adj->river_outflow);
     a.q_f = static_cast<const REAL*>(adj->flood_outflow);
@@ -824,23 +855,29 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
     a.rd = static_cast<const REAL*>(adj->river_depth);
     a.fd = static_cast<const REAL*>(adj->flood_depth);
     for (int k = 0; k < ADJ_COUNT; ++k) {
-        a.lout[k] = adj->adj_out[k];
-        a.lin[k] = adj->adj_in[k];
+        a.lout[k] = static_cast<const REAL*>(adj->adj_out[k]);
+        a.lin[k] = static_cast<REAL*>(adj->adj_in[k]);
     }
     a.g_n = adj->grad_river_manning;
     a.g_h = adj->grad_river_height;
-    REAL* r = static_cast<REAL*>(net->adj_real);
-    const int64_t stride = n * (int64_t)(8 / sizeof(REAL));
-    a.qr = r; a.qf = r + stride; a.pos = r + 2 * stride; a.neg = r + 3 * stride; a.lim = r + 4 * stride;
-    a.og = r + 5 * stride;
-    double* q = net->adj_dbl;
-    a.a_rout = q; a.a_fout = q + n; a.a_ri = q + 2 * n; a.a_fi = q + 3 * n; a.a_qr = q + 4 * n;
-    a.a_qf = q + 5 * n; a.a_og = q + 6 * n; a.d_wd = q + 7 * n;
+    REAL* r = static_cast<REAL*>(net->adj_scratch);
+    const int64_t st = n * (int64_t)(8 / sizeof(REAL));
+    REAL* w[13];
+    for (int k = 0; k < 13; ++k) w[k] = r + k * st;
+    a.w_qr = w[0]; a.w_qf = w[1]; a.w_lim = w[2]; a.pos = w[3]; a.neg = w[4];
+    a.a_rout = w[5]; a.a_fout = w[6]; a.a_ri = w[7]; a.a_fi = w[8]; a.a_qr = w[9]; a.a_qf = w[10];
+    a.a_og = w[11]; a.d_wd = w[12];
     a.dt = (REAL)adj->time_step;
+    const bool taped = adj->qraw_r && adj->qraw_f && adj->lim;
+    a.qr = taped ? static_cast<const REAL*>(adj->qraw_r) : a.w_qr;
+    a.qf = taped ? static_cast<const REAL*>(adj->qraw_f) : a.w_qf;
+    a.lim = taped ? static_cast<const REAL*>(adj->lim) : a.w_lim;
     const bool contig = net->contiguous != 0;
-    LAUNCH((k_adj_outflow_fwd<REAL, STO>), n, stream, v, a);
-    if (contig) LAUNCH((k_adj_limiter_fwd<REAL, STO, true>), n, stream, v, a);
-    else LAUNCH((k_adj_limiter_fwd<REAL, STO, false>), n, stream, v, a);
+    if (!taped) {
+        LAUNCH((k_adj_outflow_fwd<REAL, STO>), n, stream, v, a);
+        if (contig) LAUNCH((k_adj_limiter_fwd<REAL, STO, true>), n, stream, v, a);
+        else LAUNCH((k_adj_limiter_fwd<REAL, STO, false>), n, stream, v, a);
+    }
     if (contig) {
         if (v.L == 10) LAUNCH((k_adj_stage<REAL, STO, true, 10>), n, stream, v, a);
         else LAUNCH((k_adj_stage<REAL, STO, true, 0>), n, stream, v, a);
@@ -1097,6 +1134,14 @@ int cmf_hip_advance(cmf_network* net, const cmf_fields* f, double outer_seconds,
         return run_advance<float, double>(net, f, outer_seconds, fixed_count, out_count, s, false);
     if (f->precision == CMF_PRECISION_F64_F64)
         return run_advance<double, double>(net, f, outer_seconds, fixed_count, out_count, s, false);
+    return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
+}
+
+int cmf_hip_substep_taped(cmf_network* net, const cmf_fields* f, int32_t substep, int32_t store_inflow,
+                          const cmf_tape* tape, void* s) {
+    if (!f) return set_error(CMF_ERR_INVALID, "null field table");
+    if (f->precision == CMF_PRECISION_F32_F64) return run_substep_taped<float, double>(net, f, substep, store_inflow, tape, s);
+    if (f->precision == CMF_PRECISION_F64_F64) return run_substep_taped<double, double>(net, f, substep, store_inflow, tape, s);
     return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
 }
 

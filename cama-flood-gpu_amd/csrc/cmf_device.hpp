@@ -153,6 +153,10 @@ struct View {
     // [blk_first[b], blk_end[b]) of a single group.
     int64_t G;
     const int32_t *cgroup, *blk_first, *blk_end;
+    // taping (cmf_hip_substep_taped): the outflow pass also writes the
+    // sub-step's input state here (nullptr: no tape)
+    REAL *tape_q_r, *tape_q_f, *tape_rcsd, *tape_fcsd, *tape_fcsa, *tape_rd, *tape_fd;
+    STO *tape_rs, *tape_fs;
 
     __device__ __forceinline__ int64_t el(int64_t t, int64_t i) const { return t * ts + i * cs; }
     __device__ __forceinline__ int64_t pel(int64_t t, int64_t p) const { return t * pts + p * pps; }

@@ -159,6 +159,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_outflow(View<REAL, STO> v, int su
         q_f = kin_fld;
     }
 
+    if (v.tape_q_r) {
+        v.tape_q_r[e] = r_out;
+        v.tape_q_f[e] = f_out;
+        v.tape_rcsd[e] = r_csd;
+        v.tape_fcsd[e] = f_csd;
+        v.tape_fcsa[e] = f_csa;
+        v.tape_rd[e] = r_dep;
+        v.tape_fd[e] = f_dep;
+        v.tape_rs[e] = v.river_storage[e];
+        v.tape_fs[e] = v.flood_storage[e];
+    }
     if (FUSED) {
         v.qraw_r[e] = q_r;
         v.qraw_f[e] = q_f;

@@ -355,11 +355,14 @@ int cmf_hip_advance(cmf_network* net, const cmf_fields* f, double outer_seconds,
  * accumulators, with the time step held at its forward value.  State order:
  * river_outflow, flood_outflow, river/flood cross-section depth, flood
  * cross-section area, river/flood storage, river/flood depth.  The state
- * pointers give the sub-step's saved input state (dtypes of the field table);
- * the field table supplies parameters, runoff and the flood table.  Levee,
- * bifurcation and reservoir modules, trials, groups and per-trial parameters
- * return CMF_ERR_UNSUPPORTED.  No reference counterpart (the reference has no
- * differentiation). */
+ * pointers give the sub-step's saved input state (dtypes of the field table:
+ * REAL, storages STO); adj_out/adj_in hold REAL, the gradients double.
+ * qraw_r/qraw_f/lim (REAL) may give the unlimited flows and limiters the
+ * forward sub-step produced (cmf_hip_substep_taped); when NULL they are
+ * recomputed from the state.  The field table supplies parameters, runoff and
+ * the flood table.  Levee, bifurcation and reservoir modules, trials, groups
+ * and per-trial parameters return CMF_ERR_UNSUPPORTED.  No reference
+ * counterpart (the reference has no differentiation). */
 typedef struct cmf_adjoint {
     const void* river_outflow;
     const void* flood_outflow;
@@ -370,14 +373,37 @@ typedef struct cmf_adjoint {
     const void* flood_storage;
     const void* river_depth;
     const void* flood_depth;
-    const double* adj_out[9];
-    double* adj_in[9];
+    const void* qraw_r;
+    const void* qraw_f;
+    const void* lim;
+    const void* adj_out[9];
+    void* adj_in[9];
     double* grad_river_manning;
     double* grad_river_height;
     double time_step;
 } cmf_adjoint;
 int cmf_hip_substep_adjoint(cmf_network* net, const cmf_fields* fields, const cmf_adjoint* adjoint,
                             void* stream);
+
+/* One fused sub-step (as cmf_hip_substep) that records what the adjoint needs:
+ * its input state (same nine fields and order as cmf_adjoint) and the
+ * unlimited flows and limiters it forms.  Base physics, one trial. */
+typedef struct cmf_tape {
+    void* river_outflow;
+    void* flood_outflow;
+    void* river_cross_section_depth;
+    void* flood_cross_section_depth;
+    void* flood_cross_section_area;
+    void* river_storage;
+    void* flood_storage;
+    void* river_depth;
+    void* flood_depth;
+    void* qraw_r;
+    void* qraw_f;
+    void* lim;
+} cmf_tape;
+int cmf_hip_substep_taped(cmf_network* net, const cmf_fields* fields, int32_t substep, int32_t store_inflow,
+                          const cmf_tape* tape, void* stream);
 
 /* cmf_hip_advance with the sub-step loop replayed from a HIP graph: the loop
  * for a given sub-step count is captured once (on a private stream; kernel
