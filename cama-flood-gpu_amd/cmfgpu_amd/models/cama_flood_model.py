@@ -330,6 +330,30 @@ class CaMaFlood:
             return x.reshape(x.shape[0], self.G, -1)
         return x
 
+    def enable_outflow_mean(self) -> None:
+        """Accumulate the time integral of total outflow (river + flood) over
+        every sub-step on the device (the reference's "mean" statistics of
+        total_outflow); read it with outflow_mean()."""
+        self.t["acc_outflow"] = torch.zeros(self.N * self.T, dtype=torch.float64, device=self.device)
+        self._acc_seconds = 0.0
+        _lib.check(_lib.lib().cmf_network_set_outflow_accumulator(self.network_handle,
+                                                                  _lib.ptr(self.t["acc_outflow"])),
+                   "cmf_network_set_outflow_accumulator")
+
+    def outflow_mean(self, reset: bool = True) -> np.ndarray:
+        """Time-mean total outflow since the last reset (m^3/s), reference order
+        ([T, N] with trials)."""
+        acc = self.t["acc_outflow"]
+        secs = self._acc_seconds
+        x = (acc / secs).cpu().numpy() if secs > 0 else np.zeros(acc.shape[0])
+        if reset:
+            acc.zero_()
+            self._acc_seconds = 0.0
+        if self.T > 1:
+            x = x.reshape(self.N, self.T).T if self.trial_layout == "inner" else x.reshape(self.T, self.N)
+            return x[:, self.inv]
+        return x[self.inv]
+
     def autotune(self, **kw) -> dict:
         """Choose the block size of each fused-schedule kernel by timing (cmfgpu_amd.autotune.tune)."""
         from ..autotune import tune
@@ -401,6 +425,8 @@ class CaMaFlood:
             if self.log_path is not None:
                 log.write_step(self.log_path, self.log_sums())
             self.t["log_sums"].zero_()
+        if "acc_outflow" in self.t:
+            self._acc_seconds += self.step_duration
         self.current_time = self.current_time + timedelta(seconds=self.step_duration)
         self.step_count += 1
         return n

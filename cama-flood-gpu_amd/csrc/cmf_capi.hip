@@ -73,6 +73,8 @@ struct cmf_network {
     cmf_fields graph_fields;
     bool graph_fields_set = false;
     std::vector<std::pair<int, hipGraphExec_t>> graphs;
+    // output statistics accumulator (cmf_network_set_outflow_accumulator)
+    double* acc_outflow = nullptr;
     // adjoint scratch (cmf_hip_substep_adjoint), allocated on first use:
     // 13 arrays of N elements of up to 8 bytes
     void* adj_scratch = nullptr;
@@ -218,6 +220,7 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.is_dam_related = f->is_dam_related;
     v.levee_index = f->levee_index;
     for (int k = 0; k < CMF_LOG_COUNT; ++k) v.log_sums[k] = f->log_sums[k];
+    v.acc_outflow = net->acc_outflow;
     v.G = net->G;
     v.cgroup = net->cgroup;
     v.blk_first = net->blk_first;
@@ -1045,6 +1048,28 @@ int cmf_network_create(int64_t n, const int32_t* downstream_idx, int64_t num_pat
 }
 
 int64_t cmf_network_num_groups(const cmf_network* net) { return net ? net->G : 0; }
+
+int cmf_network_set_outflow_accumulator(cmf_network* net, double* acc) {
+    if (!net) return set_error(CMF_ERR_INVALID, "cmf_network_set_outflow_accumulator: null network");
+    net->acc_outflow = acc;
+    for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
+    net->graphs.clear();
+    return CMF_OK;
+}
+
+int cmf_hip_map_runoff(int64_t rows, const int64_t* indptr, const int32_t* indices, const float* weights,
+                       const float* grid, double scale, int32_t precision, void* out, void* stream) {
+    if (rows < 0 || (rows > 0 && (!indptr || !indices || !weights || !grid || !out)))
+        return set_error(CMF_ERR_INVALID, "cmf_hip_map_runoff: bad arguments");
+    if (rows == 0) return CMF_OK;
+    if (precision == CMF_PRECISION_F32_F64)
+        LAUNCH((k_map_runoff<float>), rows, stream, rows, indptr, indices, weights, grid, scale, static_cast<float*>(out));
+    else if (precision == CMF_PRECISION_F64_F64)
+        LAUNCH((k_map_runoff<double>), rows, stream, rows, indptr, indices, weights, grid, scale, static_cast<double*>(out));
+    else
+        return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", precision);
+    return CMF_OK;
+}
 
 int cmf_network_set_block_sizes(cmf_network* net, const int32_t* sizes, int32_t count) {
     if (!net || !sizes || count < 0 || count > CMF_TUNE_COUNT)

@@ -157,6 +157,9 @@ struct View {
     // sub-step's input state here (nullptr: no tape)
     REAL *tape_q_r, *tape_q_f, *tape_rcsd, *tape_fcsd, *tape_fcsa, *tape_rd, *tape_fd;
     STO *tape_rs, *tape_fs;
+    // output statistics: sum over sub-steps of (river + flood outflow) * dt
+    // per element (nullptr: off)
+    double* acc_outflow;
 
     __device__ __forceinline__ int64_t el(int64_t t, int64_t i) const { return t * ts + i * cs; }
     __device__ __forceinline__ int64_t pel(int64_t t, int64_t p) const { return t * pts + p * pps; }

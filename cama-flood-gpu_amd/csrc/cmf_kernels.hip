@@ -1060,6 +1060,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_fused(View<REAL, STO> v, in
         const REAL fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
         v.river_outflow[e] = rout;
         v.flood_outflow[e] = fout;
+        if (v.acc_outflow) v.acc_outflow[e] += (double)(rout + fout) * (double)dt;
         // upstream u has downstream j: its negative flow is limited by j
         STO ri = (STO)0, fi = (STO)0;
         STO rt = res ? v.res_inflow[e] : (STO)0;
@@ -1183,6 +1184,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_route(View<REAL, STO> v, int subs
             // ---- stores
             v.river_outflow[e] = rout;
             v.flood_outflow[e] = fout;
+            if (v.acc_outflow) v.acc_outflow[e] += (double)(rout + fout) * (double)dt;
             if (res) v.res_inflow[e] = rt;
             if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e] = l_own;
             if (store) {
@@ -1356,6 +1358,23 @@ __global__ void k_set_dt(REAL* time_step, const int32_t* counts, int64_t n, doub
 __global__ void k_zero_int(int32_t* p, int64_t n) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = 0;
+}
+
+// ==================================================== runoff mapping ===
+// Catchment runoff from a gridded field through the catchment x grid-cell
+// area-weight matrix (CSR, rows in the caller's catchment order):
+// out[r] = scale * sum_k w[k] * grid[idx[k]], summed in double in index order.
+template <class REAL>
+__global__ __launch_bounds__(kBlock) void k_map_runoff(int64_t rows, const int64_t* __restrict__ indptr,
+                                                       const int32_t* __restrict__ idx,
+                                                       const float* __restrict__ w,
+                                                       const float* __restrict__ grid, double scale,
+                                                       REAL* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    double acc = 0.0;
+    for (int64_t k = indptr[r], e = indptr[r + 1]; k < e; ++k) acc += (double)w[k] * (double)grid[idx[k]];
+    out[r] = (REAL)(scale * acc);
 }
 
 // ============================================= explicit instantiation ===

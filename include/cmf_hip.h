@@ -346,6 +346,23 @@ int cmf_hip_substep_timed(cmf_network* net, const cmf_fields* f, int32_t substep
  * *out_count (the maximum over trials with CMF_PER_TRIAL_DT). */
 int cmf_hip_advance(cmf_network* net, const cmf_fields* f, double outer_seconds,
                     int32_t fixed_count, int32_t* out_count, void* stream);
+/* ----------------------------------------------------- forcing, output */
+/* Catchment runoff from a gridded field: out[r] = scale * sum_k weights[k] *
+ * grid[indices[k]] over k in [indptr[r], indptr[r + 1]) (sums in double, in
+ * index order), out in the profile's REAL.  The matrix is the reference's
+ * runoff mapping (catchments x grid cells, area weights; the arrays of the
+ * mapping .npz read at cmfgpu/params/export_bin.py:963-975), applied where
+ * the reference applies it in hydroforge's shard_forcing
+ * (scripts/run_daily_bin.py:114-119); rows in the caller's catchment order. */
+int cmf_hip_map_runoff(int64_t rows, const int64_t* indptr, const int32_t* indices, const float* weights,
+                       const float* grid, double scale, int32_t precision, void* out, void* stream);
+/* With a non-NULL accumulator (double, one entry per catchment element) the
+ * fused stage pass adds (river_outflow + flood_outflow) * dt every sub-step:
+ * the time integral behind the reference's "mean" statistics of
+ * total_outflow (cmfgpu/modules/base.py:412-419, scripts/run_daily_bin.py:28).
+ * NULL turns it off. */
+int cmf_network_set_outflow_accumulator(cmf_network* net, double* accumulator);
+
 /* ------------------------------------------------------------- adjoint */
 /* Reverse mode of one fused sub-step of the base physics (outflow, limiter
  * and inflow sums, storage update, flood stage; cmfgpu/phys/triton/outflow.py
