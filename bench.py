@@ -243,6 +243,40 @@ def adjoint_line(args, device: str) -> dict:
     return out
 
 
+def forcing_line(model, reps: int = 20) -> dict:
+    """Gridded runoff mapped onto the model's catchments (cmf_hip_map_runoff):
+    a 0.1-degree grid (3600 x 1800 cells), 1-4 cells per catchment."""
+    import numpy as np
+    import torch
+    from cmfgpu_amd.forcing import RunoffMapping
+    rng = np.random.default_rng(12)
+    n, cells = model.N, 3600 * 1800
+    counts = rng.integers(1, 5, n)
+    indptr = np.zeros(n + 1, np.int64)
+    indptr[1:] = np.cumsum(counts)
+    # neighbouring catchments sit on neighbouring cells, as on a real map
+    base = (np.sort(rng.integers(0, cells - 8, n))).astype(np.int64)
+    indices = (np.repeat(base, counts) + (np.arange(indptr[-1]) - np.repeat(indptr[:-1], counts))).astype(np.int32)
+    weights = rng.uniform(1e6, 1e8, indptr[-1]).astype(np.float32)
+    rm = RunoffMapping(indptr, indices, weights, cells, 1e-3 / 86400.0).bind(model)
+    grid = torch.rand(cells, device=model.device, dtype=torch.float32)
+    rm.apply(grid)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        rm.apply(grid)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nnz = int(indptr[-1])
+    algo = n * (8 + 4) + nnz * (4 + 4 + 4)
+    out = {"catchments": n, "grid_cells": cells, "nonzeros": nnz, "ms": round(ms, 5),
+           "algorithmic_bytes": algo, "GBps": round(algo / (ms * 1e-3) / 1e9, 1)}
+    log(f"forcing: {nnz} weights onto {n} catchments in {ms * 1e3:.1f} us ({out['GBps']} GB/s)")
+    return out
+
+
 def run_steps(model, steps: int, per_trial: bool):
     """Advance `steps` outer steps; returns member-catchment-sub-steps done."""
     import torch
@@ -398,6 +432,7 @@ def main():
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(model, net, args.cpu_seconds)
         log(f"cpu baseline: {cpu['value']:.4g} catchment-steps/s on {cpu['cores']} threads")
+    forcing = forcing_line(model)
     model.close()
     del model
     torch.cuda.empty_cache()
@@ -443,7 +478,7 @@ def main():
                    "levees": net.num_levees, "flood_levels": 10, "outer_step_s": 1800.0,
                    "substeps_timed": int(substeps), "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "basins": basins, "graph": graph,
-        "adjoint": adjoint,
+        "adjoint": adjoint, "forcing": forcing,
         "block_sizes": {k: v["block"] for k, v in tuned.items()} if tuned else None,
         "error_vs_fp64": errors,
     }

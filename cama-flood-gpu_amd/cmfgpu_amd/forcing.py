@@ -96,8 +96,11 @@ class RunoffMapping:
         if self.model is None:
             raise RuntimeError("bind(model) first")
         m = self.model
-        g = grid if isinstance(grid, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(grid))
-        g = g.to(device=m.device, dtype=torch.float32).reshape(-1).contiguous()
+        if not (isinstance(grid, torch.Tensor) and grid.device == m.device and grid.dtype == torch.float32
+                and grid.is_contiguous()):
+            g = grid if isinstance(grid, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(grid))
+            grid = g.to(device=m.device, dtype=torch.float32).contiguous()
+        g = grid.reshape(-1)
         if g.shape[0] != self.num_cells:
             raise ValueError(f"grid has {g.shape[0]} cells, mapping {self.num_cells}")
         out = m.t["runoff"]
