@@ -575,21 +575,38 @@ struct LogPart {
     double q[CMF_LOG_COUNT];
 };
 
-// Floodplain depth table row of one catchment.  With LFIX > 0 (the level
-// count known at compile time) the whole row is loaded into registers up
-// front, one round trip; with LFIX == 0 it is read level by level.
+// Floodplain depth table row of one catchment.  With LFIX == 10 (the level
+// count of every configuration, known at compile time) the whole row is
+// loaded into ten named registers up front, one round trip, and read through
+// a switch that folds to a register once the level loops are unrolled (an
+// array here would be placed in LDS or scratch, since the array promotion
+// runs before unrolling); other counts read the row level by level.
 template <class REAL, int LFIX>
 struct TableRow {
     const REAL* row;
-    REAL h[LFIX > 0 ? LFIX : 1];
+    REAL h0, h1, h2, h3, h4, h5, h6, h7, h8, h9;
     __device__ __forceinline__ void load(const REAL* r) {
         row = r;
-        if (LFIX > 0) {
-#pragma unroll
-            for (int k = 0; k < (LFIX > 0 ? LFIX : 1); ++k) h[k] = r[k];
+        if (LFIX == 10) {
+            h0 = r[0]; h1 = r[1]; h2 = r[2]; h3 = r[3]; h4 = r[4];
+            h5 = r[5]; h6 = r[6]; h7 = r[7]; h8 = r[8]; h9 = r[9];
         }
     }
-    __device__ __forceinline__ REAL operator[](int k) const { return LFIX > 0 ? h[k] : row[k]; }
+    __device__ __forceinline__ REAL operator[](int k) const {
+        if (LFIX != 10) return row[k];
+        switch (k) {
+            case 0: return h0;
+            case 1: return h1;
+            case 2: return h2;
+            case 3: return h3;
+            case 4: return h4;
+            case 5: return h5;
+            case 6: return h6;
+            case 7: return h7;
+            case 8: return h8;
+            default: return h9;
+        }
+    }
 };
 
 template <class REAL, class STO>
