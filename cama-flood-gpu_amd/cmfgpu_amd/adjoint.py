@@ -49,7 +49,12 @@ class CmfAdjoint(ctypes.Structure):
         ("grad_river_manning", ctypes.c_void_p),
         ("grad_river_height", ctypes.c_void_p),
         ("time_step", ctypes.c_double),
+        ("mode", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
+
+
+ADJ_DEFER, ADJ_PENDING, ADJ_FLUSH = 1, 2, 4
 
 
 class CmfTape(ctypes.Structure):
@@ -72,6 +77,12 @@ def _declare():
     lib.cmf_hip_substep_taped.restype = ctypes.c_int
     lib.cmf_hip_substep_taped.argtypes = [ctypes.c_void_p, ctypes.POINTER(_lib.CmfFields), ctypes.c_int32,
                                           ctypes.c_int32, ctypes.POINTER(CmfTape), ctypes.c_void_p]
+    lib.cmf_hip_advance_taped.restype = ctypes.c_int
+    lib.cmf_hip_advance_taped.argtypes = [ctypes.c_void_p, ctypes.POINTER(_lib.CmfFields), ctypes.c_double,
+                                          ctypes.c_int32, ctypes.POINTER(CmfTape), ctypes.c_void_p]
+    lib.cmf_hip_adjoint_steps.restype = ctypes.c_int
+    lib.cmf_hip_adjoint_steps.argtypes = [ctypes.c_void_p, ctypes.POINTER(_lib.CmfFields),
+                                          ctypes.POINTER(CmfAdjoint), ctypes.c_int32, ctypes.c_void_p]
     _SIG_DONE = True
 
 
@@ -219,6 +230,7 @@ class GaugeProblem:
             adjs.append(pair)
         lib = _lib.lib()
         parity = 0
+        first_done = False
         for s0 in reversed(starts):
             s1 = min(s0 + self.K, self.steps)
             self._restore(self.checkpoints[s0])
@@ -227,12 +239,10 @@ class GaugeProblem:
             for step in range(s0, s1):
                 self._apply_runoff(step)
                 cnt = self.counts[step]
-                m.t["time_step"].fill_(m.step_duration / cnt)
-                for k in range(cnt):
-                    _lib.check(lib.cmf_hip_substep_taped(m.network_handle, m.field_table_ref(), k,
-                                                         1 if k == cnt - 1 else 0, ctypes.byref(tapes[slot]),
-                                                         _lib.current_stream()), "cmf_hip_substep_taped")
-                    slot += 1
+                arr = (CmfTape * cnt)(*tapes[slot:slot + cnt])
+                _lib.check(lib.cmf_hip_advance_taped(m.network_handle, m.field_table_ref(), m.step_duration, cnt,
+                                                     arr, _lib.current_stream()), "cmf_hip_advance_taped")
+                slot += cnt
                 marks.append((step, slot, self._gauge_q()))
             for step, end, q in reversed(marks):
                 cnt = self.counts[step]
@@ -241,12 +251,22 @@ class GaugeProblem:
                 lam[parity][QF].index_add_(0, self.g_dev, seed)
                 self._apply_runoff(step)
                 dt = m.step_duration / cnt
-                for slot in range(end - 1, end - 1 - cnt, -1):
-                    a = adjs[slot][parity]
-                    a.time_step = dt
-                    _lib.check(lib.cmf_hip_substep_adjoint(m.network_handle, m.field_table_ref(), ctypes.byref(a),
-                                                           _lib.current_stream()), "cmf_hip_substep_adjoint")
-                    parity = 1 - parity
+                # sub-step k of this outer step runs after the (cnt - 1 - k) later ones
+                arr = (CmfAdjoint * cnt)(*[adjs[end - cnt + k][(parity + cnt - 1 - k) % 2] for k in range(cnt)])
+                for k in range(cnt):
+                    arr[k].time_step = dt
+                    # the gather of each call is folded into the next one
+                    arr[k].mode = ADJ_DEFER | (ADJ_PENDING if (first_done or k < cnt - 1) else 0)
+                first_done = True
+                _lib.check(lib.cmf_hip_adjoint_steps(m.network_handle, m.field_table_ref(), arr, cnt,
+                                                     _lib.current_stream()), "cmf_hip_adjoint_steps")
+                parity = (parity + cnt) % 2
+        # the deferred gather of the last call (the first sub-step)
+        fl = CmfAdjoint()
+        ctypes.memmove(ctypes.byref(fl), ctypes.byref(adjs[0][(parity + 1) % 2]), ctypes.sizeof(CmfAdjoint))
+        fl.mode = ADJ_FLUSH
+        _lib.check(lib.cmf_hip_substep_adjoint(m.network_handle, m.field_table_ref(), ctypes.byref(fl),
+                                               _lib.current_stream()), "cmf_hip_substep_adjoint")
         del tape_r, tape_s
         self._restore(self.final)
         inv = torch.as_tensor(m.inv, device=dev)

@@ -20,7 +20,9 @@
 //   A2  reverse of the storage update and stage diagnosis
 //   A3  reverse of the limited flows and inflow sums, then of the limiter
 //   A4  reverse of the outflow momentum update
-//   A5  gather of the downstream water-surface adjoints
+//   A5  gather of the downstream water-surface adjoints (or deferred into the
+//       A2 pass of the next, earlier sub-step, which loops over the same
+//       upstream range anyway)
 
 namespace cmf {
 
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_limiter_fwd(View<REAL, STO> v, A
 // catchment j (k_stage_fused), then reverse them.  Writes the adjoints of the
 // limited own flows (a_rout/a_fout, including the next state's direct use),
 // of the inflow sums (a_ri/a_fi), of the old storages, and dJ/dH (stage part).
-template <class REAL, class STO, bool CONTIG, int LFIX>
+template <class REAL, class STO, bool CONTIG, int LFIX, bool PENDING>
 __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView<REAL> a) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= v.N) return;
@@ -203,6 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     const REAL rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
     const REAL fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
     STO ri = (STO)0, fi = (STO)0;
+    double pend = 0.0;   // PENDING: downstream water-surface adjoints of the later sub-step
     for (int32_t k = v.up_ptr[j]; k < v.up_ptr[j + 1]; ++k) {
         const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
         if (CONTIG && u == j) continue;
@@ -210,6 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
         const REAL ur0 = a.qr[u], uf0 = a.qf[u];
         ri += (STO)((ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own);
         fi += (STO)((uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own);
+        if (PENDING) pend += (double)a.d_wd[u];
     }
     const REAL rin = (REAL)ri, fin = (REAL)fi;
     const STO rs = adj_sto<STO>(a.rs_raw, j), fs = adj_sto<STO>(a.fs_raw, j);
@@ -232,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     const int L = LFIX > 0 ? LFIX : (int)v.L;
     double Ltot = 0.0, LH = 0.0;
     const double Lrs_out = (double)a.lout[ADJ_RS][j], Lfs_out = (double)a.lout[ADJ_FS][j];
-    const double Lrd_out = (double)a.lout[ADJ_RD][j], Lfd_out = (double)a.lout[ADJ_FD][j];
+    const double Lrd_out = (double)a.lout[ADJ_RD][j] + pend, Lfd_out = (double)a.lout[ADJ_FD][j];
     double Lrsf = Lrs_out + Lrd_out / ((double)rl * (double)rw);
     if (!(total > rms)) {
         // dry: rsf = total, flood depth 0, flood storage max(0, 0)
@@ -309,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     a.a_fi[j] = (REAL)Lfin;
     a.lin[ADJ_RS][j] = (REAL)Lrs;
     a.lin[ADJ_FS][j] = (REAL)Lfs;
-    a.g_h[j] += LH;
+    a.g_h[j] += LH - pend;
 }
 
 // A3: adjoints of the unlimited flows and of the limiter of catchment k.

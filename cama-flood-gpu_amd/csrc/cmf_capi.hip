@@ -844,7 +844,8 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
             return set_error(CMF_ERR_INVALID, "substep_adjoint: state, adj_out and adj_in need all %d fields", ADJ_COUNT);
     if (!adj->grad_river_manning || !adj->grad_river_height)
         return set_error(CMF_ERR_INVALID, "substep_adjoint: gradient accumulators are NULL");
-    if (!(adj->time_step > 0.0)) return set_error(CMF_ERR_INVALID, "substep_adjoint: time_step must be > 0");
+    if (!(adj->time_step > 0.0) && !(adj->mode & CMF_ADJ_FLUSH))
+        return set_error(CMF_ERR_INVALID, "substep_adjoint: time_step must be > 0");
     const int64_t n = v.N;
     if (!net->adj_scratch && (rc = dev_alloc(&net->adj_scratch, (size_t)n * 8 * 13))) return rc;
     AdjView<REAL> a;
@@ -876,23 +877,35 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
     a.qf = taped ? static_cast<const REAL*>(adj->qraw_f) : a.w_qf;
     a.lim = taped ? static_cast<const REAL*>(adj->lim) : a.w_lim;
     const bool contig = net->contiguous != 0;
+    if (adj->mode & CMF_ADJ_FLUSH) {
+        // the deferred gather of the last call only
+        if (contig) LAUNCH((k_adj_gather<REAL, STO, true>), n, stream, v, a);
+        else LAUNCH((k_adj_gather<REAL, STO, false>), n, stream, v, a);
+        return CMF_OK;
+    }
     if (!taped) {
         LAUNCH((k_adj_outflow_fwd<REAL, STO>), n, stream, v, a);
         if (contig) LAUNCH((k_adj_limiter_fwd<REAL, STO, true>), n, stream, v, a);
         else LAUNCH((k_adj_limiter_fwd<REAL, STO, false>), n, stream, v, a);
     }
+    const bool pending = (adj->mode & CMF_ADJ_PENDING) != 0;
+    const bool defer = (adj->mode & CMF_ADJ_DEFER) != 0;
+#define CMF_AS(C, LF, PD) LAUNCH((k_adj_stage<REAL, STO, C, LF, PD>), n, stream, v, a)
     if (contig) {
-        if (v.L == 10) LAUNCH((k_adj_stage<REAL, STO, true, 10>), n, stream, v, a);
-        else LAUNCH((k_adj_stage<REAL, STO, true, 0>), n, stream, v, a);
+        if (v.L == 10) { if (pending) CMF_AS(true, 10, true); else CMF_AS(true, 10, false); }
+        else { if (pending) CMF_AS(true, 0, true); else CMF_AS(true, 0, false); }
         LAUNCH((k_adj_route<REAL, STO, true>), n, stream, v, a);
     } else {
-        if (v.L == 10) LAUNCH((k_adj_stage<REAL, STO, false, 10>), n, stream, v, a);
-        else LAUNCH((k_adj_stage<REAL, STO, false, 0>), n, stream, v, a);
+        if (v.L == 10) { if (pending) CMF_AS(false, 10, true); else CMF_AS(false, 10, false); }
+        else { if (pending) CMF_AS(false, 0, true); else CMF_AS(false, 0, false); }
         LAUNCH((k_adj_route<REAL, STO, false>), n, stream, v, a);
     }
+#undef CMF_AS
     LAUNCH((k_adj_outflow<REAL, STO>), n, stream, v, a);
-    if (contig) LAUNCH((k_adj_gather<REAL, STO, true>), n, stream, v, a);
-    else LAUNCH((k_adj_gather<REAL, STO, false>), n, stream, v, a);
+    if (!defer) {
+        if (contig) LAUNCH((k_adj_gather<REAL, STO, true>), n, stream, v, a);
+        else LAUNCH((k_adj_gather<REAL, STO, false>), n, stream, v, a);
+    }
     return CMF_OK;
 }
 
@@ -1168,6 +1181,35 @@ int cmf_hip_substep_taped(cmf_network* net, const cmf_fields* f, int32_t substep
     if (f->precision == CMF_PRECISION_F32_F64) return run_substep_taped<float, double>(net, f, substep, store_inflow, tape, s);
     if (f->precision == CMF_PRECISION_F64_F64) return run_substep_taped<double, double>(net, f, substep, store_inflow, tape, s);
     return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
+}
+
+int cmf_hip_advance_taped(cmf_network* net, const cmf_fields* f, double outer_seconds, int32_t count,
+                          const cmf_tape* tapes, void* s) {
+    if (!f || !tapes || count < 1 || !(outer_seconds > 0.0))
+        return set_error(CMF_ERR_INVALID, "cmf_hip_advance_taped: bad arguments");
+    if (!f->time_step) return set_error(CMF_ERR_INVALID, "cmf_hip_advance_taped: time_step is NULL");
+    const double dt = outer_seconds / (double)count;
+    if (f->precision == CMF_PRECISION_F32_F64)
+        hipLaunchKernelGGL((k_set_dt_fixed<float>), dim3(1), dim3(64), 0, (hipStream_t)s,
+                           static_cast<float*>(f->time_step), (int64_t)1, dt);
+    else
+        hipLaunchKernelGGL((k_set_dt_fixed<double>), dim3(1), dim3(64), 0, (hipStream_t)s,
+                           static_cast<double*>(f->time_step), (int64_t)1, dt);
+    for (int32_t k = 0; k < count; ++k) {
+        const int rc = cmf_hip_substep_taped(net, f, k, k == count - 1 ? 1 : 0, tapes + k, s);
+        if (rc) return rc;
+    }
+    return CMF_OK;
+}
+
+int cmf_hip_adjoint_steps(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adjoints, int32_t count,
+                          void* s) {
+    if (!adjoints || count < 1) return set_error(CMF_ERR_INVALID, "cmf_hip_adjoint_steps: bad arguments");
+    for (int32_t k = count - 1; k >= 0; --k) {
+        const int rc = cmf_hip_substep_adjoint(net, f, adjoints + k, s);
+        if (rc) return rc;
+    }
+    return CMF_OK;
 }
 
 int cmf_hip_substep_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* a, void* s) {

@@ -398,7 +398,16 @@ typedef struct cmf_adjoint {
     double* grad_river_manning;
     double* grad_river_height;
     double time_step;
+    int32_t mode;        /* CMF_ADJ_* bits, 0 for a standalone sub-step */
+    int32_t reserved;
 } cmf_adjoint;
+/* mode bits for a sweep over consecutive sub-steps on one network:
+ * DEFER leaves the last pass (the gather of downstream water-surface
+ * adjoints) pending in the network scratch, PENDING folds the pending gather
+ * of the previous call (the later sub-step) into this call, FLUSH runs only
+ * the pending gather (into adj_in and the height gradient) after the last
+ * call.  Results equal those of standalone calls. */
+enum { CMF_ADJ_DEFER = 1, CMF_ADJ_PENDING = 2, CMF_ADJ_FLUSH = 4 };
 int cmf_hip_substep_adjoint(cmf_network* net, const cmf_fields* fields, const cmf_adjoint* adjoint,
                             void* stream);
 
@@ -421,6 +430,14 @@ typedef struct cmf_tape {
 } cmf_tape;
 int cmf_hip_substep_taped(cmf_network* net, const cmf_fields* fields, int32_t substep, int32_t store_inflow,
                           const cmf_tape* tape, void* stream);
+/* One outer step of `count` taped sub-steps (dt = outer_seconds / count,
+ * written to time_step; tapes[k] records sub-step k), and the reverse sweep
+ * of `count` sub-steps (adjoints[k] for sub-step k, run from the last to the
+ * first): the per-outer-step loops of the adjoint driver in one call each. */
+int cmf_hip_advance_taped(cmf_network* net, const cmf_fields* fields, double outer_seconds, int32_t count,
+                          const cmf_tape* tapes, void* stream);
+int cmf_hip_adjoint_steps(cmf_network* net, const cmf_fields* fields, const cmf_adjoint* adjoints,
+                          int32_t count, void* stream);
 
 /* cmf_hip_advance with the sub-step loop replayed from a HIP graph: the loop
  * for a given sub-step count is captured once (on a private stream; kernel
