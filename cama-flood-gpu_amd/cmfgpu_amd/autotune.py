@@ -24,9 +24,9 @@ from . import _lib
 CANDIDATES = (64, 128, 256, 512)
 KERNELS = {
     "k_outflow": 0, "k_reservoir": 1, "k_bif_outflow": 2, "k_bif_inflow": 3,
-    "k_lim_touched": 4, "k_limiter": 5, "k_stage_fused": 6, "k_route": 7,
+    "k_limiter": 4, "k_stage_fused": 5,
 }
-NUM_TUNABLE = 8
+NUM_TUNABLE = 6
 
 
 def kernel_times(model, reps: int) -> dict:
@@ -65,7 +65,7 @@ def _key(model) -> str:
     dev = torch.cuda.get_device_name(model.device) if model.device.type == "cuda" else str(model.device)
     return (f"{dev}|N={model.N}|T={model.T}|P={model.P}|V={model.V}|R={model.R}|G={model.G}|"
             f"flags={model.flags}|prec={model.real.__name__}|layout={model.layout}|"
-            f"trials={model.trial_layout}")
+            f"trials={model.trial_layout}|kernels={','.join(sorted(KERNELS, key=KERNELS.get))}")
 
 
 def default_cache_path() -> Path:
@@ -85,7 +85,7 @@ def tune(model, *, reps: int = 8, candidates=CANDIDATES, cache_path: str | Path 
             cache = json.loads(path.read_text())
         except (OSError, ValueError):
             cache = {}
-        if key in cache:
+        if key in cache and len(cache[key].get("sizes", ())) == NUM_TUNABLE:
             set_block_sizes(model, cache[key]["sizes"])
             return cache[key]["report"]
     saved = {name: t.clone() for name, t in model.t.items()}

@@ -42,8 +42,6 @@ struct cmf_network {
     int32_t* up_idx = nullptr;
     int32_t* ev_ptr = nullptr;
     int32_t* ev_code = nullptr;
-    int32_t* touched = nullptr;       // catchments that end a bifurcation path
-    int64_t n_touched = 0;
     void* pos_term = nullptr;
     void* neg_term = nullptr;
     void* qraw_r = nullptr;
@@ -57,7 +55,7 @@ struct cmf_network {
     int64_t partial_blocks = 0;
     int32_t* host_counts = nullptr;   // pinned, max(T, G) entries
     // block size of each tunable kernel of the fused schedule (CMF_TUNE_*)
-    int32_t blocks[CMF_TUNE_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256};
+    int32_t blocks[CMF_TUNE_COUNT] = {256, 256, 256, 256, 256, 256};
     // groups (G > 0): group of each catchment and the block table of the
     // catchment kernels (blocks never straddle two groups)
     int64_t G = 0;
@@ -84,7 +82,7 @@ namespace {
 
 void free_network(cmf_network* net) {
     if (!net) return;
-    void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->touched, net->pos_term,
+    void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->pos_term,
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
                     net->path_neg, net->path_net, net->path_raw, net->partials, net->cgroup,
                     net->blk_first, net->blk_end, net->blk_group, net->gblk_ptr};
@@ -397,7 +395,7 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
         case OP_OUTFLOW:
             if ((rc = need_base(f, "compute_outflow")) || (rc = need_features(f, "compute_outflow"))) return rc;
             if (!f->outgoing_storage) return set_error(CMF_ERR_INVALID, "compute_outflow: outgoing_storage is NULL");
-            LAUNCH((k_outflow<REAL, STO, false, true>), NT, stream, v, -1);
+            LAUNCH((k_outflow<REAL, STO, false>), NT, stream, v, -1);
             if (contig) LAUNCH((k_outgoing_sum<REAL, STO, true>), NT, stream, v);
             else LAUNCH((k_outgoing_sum<REAL, STO, false>), NT, stream, v);
             return CMF_OK;
@@ -518,36 +516,26 @@ int dispatch_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
 }
 
 // ------------------------------------------------------------- fused ----
-// MERGED: limiters recomputed inside the route pass (k_route), used for the
-// trial-inner batched layout; otherwise k_limiter then k_stage_fused.
-// `nt`: threads of a catchment kernel (N*T, or the group block table)
-template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
-int launch_route(const cmf_network* net, const View<REAL, STO>& v, int64_t nt, bool merged, int substep,
-                 int store_inflow, void* stream) {
-    if (merged) {
-        LAUNCH_B((k_route<REAL, STO, CONTIG, LOG, LFIX, LEV>), nt, block_of(net, CMF_TUNE_ROUTE, true), stream, v,
-                 substep, store_inflow);
+// The stage pass k_stage_fused, specialised on the table level count and the
+// fused levee stage.  `nt`: threads of a catchment kernel (N*T, or the group
+// block table).
+template <class REAL, class STO, bool CONTIG, bool LOG>
+int launch_stage(const cmf_network* net, const View<REAL, STO>& v, int64_t nt, bool lev, int substep,
+                  int store_inflow, void* stream) {
+    const int64_t b = block_of(net, CMF_TUNE_STAGE, true);
+    if (v.L == 10) {
+        if (lev) LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, 10, true>), nt, b, stream, v, substep, store_inflow);
+        else LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, 10, false>), nt, b, stream, v, substep, store_inflow);
     } else {
-        LAUNCH_B((k_limiter<REAL, STO, CONTIG>), nt, block_of(net, CMF_TUNE_LIMITER, true), stream, v, substep);
-        LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, LFIX, LEV>), nt, block_of(net, CMF_TUNE_STAGE, true), stream,
-                 v, substep, store_inflow);
+        if (lev) LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, 0, true>), nt, b, stream, v, substep, store_inflow);
+        else LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, 0, false>), nt, b, stream, v, substep, store_inflow);
     }
     return CMF_OK;
 }
 
-template <class REAL, class STO, bool CONTIG, bool LOG>
-int dispatch_route(const cmf_network* net, const View<REAL, STO>& v, int64_t nt, bool merged, bool lev, int substep,
-                   int store_inflow, void* stream) {
-    if (v.L == 10) {
-        return lev ? launch_route<REAL, STO, CONTIG, LOG, 10, true>(net, v, nt, merged, substep, store_inflow, stream)
-                   : launch_route<REAL, STO, CONTIG, LOG, 10, false>(net, v, nt, merged, substep, store_inflow, stream);
-    }
-    return lev ? launch_route<REAL, STO, CONTIG, LOG, 0, true>(net, v, nt, merged, substep, store_inflow, stream)
-               : launch_route<REAL, STO, CONTIG, LOG, 0, false>(net, v, nt, merged, substep, store_inflow, stream);
-}
-
-// One fused sub-step: outflow, [reservoir], [bifurcation outflow, end
-// limiters, bifurcation inflow], route (limiters, inflow, stage [, levee]).
+// One fused sub-step: outflow, [reservoir], [bifurcation outflow], limiters,
+// [bifurcation inflow], stage (inflow sums, limited flows, storage, flood
+// stage [, levee stage]).
 template <class REAL, class STO>
 int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int substep,
                   int store_inflow, void* stream) {
@@ -557,16 +545,12 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
     const bool bif = v.has(CMF_HAS_BIFURCATION), levee = v.has(CMF_HAS_LEVEE);
     const bool log = v.has(CMF_HAS_LOG);
     const bool lev_fused = levee && v.levee_index != nullptr;
-    // trials adjacent in memory: the merged route pass (uniform loops per wave)
-    const bool merged = v.T > 1 && v.ts == 1;
     const int64_t b_out = block_of(net, CMF_TUNE_OUTFLOW, true), b_bo = block_of(net, CMF_TUNE_BIF_OUTFLOW, false);
     const int64_t b_lim = block_of(net, CMF_TUNE_LIMITER, true), b_stage = block_of(net, CMF_TUNE_STAGE, true);
-    const int64_t b_bi = block_of(net, CMF_TUNE_BIF_INFLOW, false), b_lt = block_of(net, CMF_TUNE_LIM_TOUCHED, false);
-    if (merged) LAUNCH_B((k_outflow<REAL, STO, true, false>), NT, b_out, stream, v, substep);
-    else LAUNCH_B((k_outflow<REAL, STO, true, true>), NT, b_out, stream, v, substep);
+    const int64_t b_bi = block_of(net, CMF_TUNE_BIF_INFLOW, false);
+    LAUNCH_B((k_outflow<REAL, STO, true>), NT, b_out, stream, v, substep);
     if (v.has(CMF_HAS_RESERVOIR))
         LAUNCH_B((k_reservoir<REAL, STO, true>), v.R * v.T, block_of(net, CMF_TUNE_RESERVOIR, false), stream, v, substep);
-    int rc;
     if (bif) {
         if (levee) {
             if (v.LB == 5) LAUNCH_B((k_bif_outflow<REAL, STO, true, 5>), v.P * v.T, b_bo, stream, v, substep);
@@ -575,39 +559,16 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
             if (v.LB == 5) LAUNCH_B((k_bif_outflow<REAL, STO, false, 5>), v.P * v.T, b_bo, stream, v, substep);
             else LAUNCH_B((k_bif_outflow<REAL, STO, false, 0>), v.P * v.T, b_bo, stream, v, substep);
         }
-        if (merged) {
-            if (contig) LAUNCH_B((k_lim_touched<REAL, STO, true>), net->n_touched * v.T, b_lt, stream, v, net->touched, net->n_touched, substep);
-            else LAUNCH_B((k_lim_touched<REAL, STO, false>), net->n_touched * v.T, b_lt, stream, v, net->touched, net->n_touched, substep);
-            rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(net, v, NT, true, lev_fused, substep, store_inflow, stream)
-                               : dispatch_route<REAL, STO, false, true>(net, v, NT, true, lev_fused, substep, store_inflow, stream))
-                     : (contig ? dispatch_route<REAL, STO, true, false>(net, v, NT, true, lev_fused, substep, store_inflow, stream)
-                               : dispatch_route<REAL, STO, false, false>(net, v, NT, true, lev_fused, substep, store_inflow, stream));
-        } else {
-            // the path nets need every end limiter: full limiter pass first
-            if (contig) LAUNCH_B((k_limiter<REAL, STO, true>), NT, b_lim, stream, v, substep);
-            else LAUNCH_B((k_limiter<REAL, STO, false>), NT, b_lim, stream, v, substep);
-            LAUNCH_B((k_bif_inflow<REAL, STO, true>), v.P * v.T, b_bi, stream, v, substep);
-            rc = CMF_OK;
-#define CMF_SF(C, LG, LF, LV) LAUNCH_B((k_stage_fused<REAL, STO, C, LG, LF, LV>), NT, b_stage, stream, v, substep, store_inflow)
-            if (contig) {
-                if (log) { if (v.L == 10) { if (lev_fused) CMF_SF(true, true, 10, true); else CMF_SF(true, true, 10, false); }
-                           else { if (lev_fused) CMF_SF(true, true, 0, true); else CMF_SF(true, true, 0, false); } }
-                else { if (v.L == 10) { if (lev_fused) CMF_SF(true, false, 10, true); else CMF_SF(true, false, 10, false); }
-                       else { if (lev_fused) CMF_SF(true, false, 0, true); else CMF_SF(true, false, 0, false); } }
-            } else {
-                if (log) { if (v.L == 10) { if (lev_fused) CMF_SF(false, true, 10, true); else CMF_SF(false, true, 10, false); }
-                           else { if (lev_fused) CMF_SF(false, true, 0, true); else CMF_SF(false, true, 0, false); } }
-                else { if (v.L == 10) { if (lev_fused) CMF_SF(false, false, 10, true); else CMF_SF(false, false, 10, false); }
-                       else { if (lev_fused) CMF_SF(false, false, 0, true); else CMF_SF(false, false, 0, false); } }
-            }
-#undef CMF_SF
-        }
-    } else {
-        rc = log ? (contig ? dispatch_route<REAL, STO, true, true>(net, v, NT, merged, lev_fused, substep, store_inflow, stream)
-                           : dispatch_route<REAL, STO, false, true>(net, v, NT, merged, lev_fused, substep, store_inflow, stream))
-                 : (contig ? dispatch_route<REAL, STO, true, false>(net, v, NT, merged, lev_fused, substep, store_inflow, stream)
-                           : dispatch_route<REAL, STO, false, false>(net, v, NT, merged, lev_fused, substep, store_inflow, stream));
     }
+    // limiters of every catchment (the path nets of k_bif_inflow need the
+    // limiters of both path ends), then the path nets, then the stage pass
+    if (contig) LAUNCH_B((k_limiter<REAL, STO, true>), NT, b_lim, stream, v, substep);
+    else LAUNCH_B((k_limiter<REAL, STO, false>), NT, b_lim, stream, v, substep);
+    if (bif) LAUNCH_B((k_bif_inflow<REAL, STO, true>), v.P * v.T, b_bi, stream, v, substep);
+    int rc = log ? (contig ? launch_stage<REAL, STO, true, true>(net, v, NT, lev_fused, substep, store_inflow, stream)
+                           : launch_stage<REAL, STO, false, true>(net, v, NT, lev_fused, substep, store_inflow, stream))
+                 : (contig ? launch_stage<REAL, STO, true, false>(net, v, NT, lev_fused, substep, store_inflow, stream)
+                           : launch_stage<REAL, STO, false, false>(net, v, NT, lev_fused, substep, store_inflow, stream));
     if (rc) return rc;
     if (log) {
         if (net->G > 0) {
@@ -617,10 +578,9 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
             timing_mark("k_log_final_grouped", (hipStream_t)stream, false);
             CMF_HIP_CHECK(hipGetLastError());
         } else {
-            // partials: one per block of the pass that produced them
-            const int64_t b_log = merged ? block_of(net, CMF_TUNE_ROUTE, true) : b_stage;
-            rc = log_final(v, (NT + b_log - 1) / b_log, kLogAll, stream);
-            if (rc) return rc;
+            // partials: one per block of the stage pass
+            const int rf = log_final(v, (NT + b_stage - 1) / b_stage, kLogAll, stream);
+            if (rf) return rf;
         }
     }
     if (levee && !lev_fused) {
@@ -1005,11 +965,7 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
             ev_code[fill[bif_di[p]]++] = (int32_t)(2 * p + 1);
         }
     }
-    std::vector<int32_t> touched;
-    for (int64_t i = 0; i < n; ++i)
-        if (ev_ptr[i + 1] > ev_ptr[i]) touched.push_back((int32_t)i);
     cmf_network* net = new cmf_network();
-    net->n_touched = (int64_t)touched.size();
     net->n = n;
     net->T = num_trials;
     net->P = num_paths;
@@ -1025,7 +981,6 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
         (rc = upload((void**)&net->up_idx, up_idx.data(), up_idx.size() * 4)) ||
         (rc = upload((void**)&net->ev_ptr, ev_ptr.data(), ev_ptr.size() * 4)) ||
         (rc = upload((void**)&net->ev_code, ev_code.data(), ev_code.size() * 4)) ||
-        (rc = upload((void**)&net->touched, touched.data(), touched.size() * 4)) ||
         (rc = dev_alloc(&net->pos_term, el)) || (rc = dev_alloc(&net->neg_term, el)) ||
         (rc = dev_alloc(&net->qraw_r, el)) || (rc = dev_alloc(&net->qraw_f, el)) ||
         (rc = dev_alloc(&net->lim, el)) || (rc = dev_alloc(&net->path_pos, pel)) ||

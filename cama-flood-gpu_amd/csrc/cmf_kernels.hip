@@ -62,9 +62,9 @@ __device__ __forceinline__ void store_if_changed(X* p, X old, X val) {
 // inflow pass read them); inflows are not zeroed because the fused stage
 // pass writes them whole.  Otherwise flows are written in place, as in the
 // reference, and river/flood inflow and the bifurcation total are zeroed.
-// TERMS: write the outgoing-storage terms (pos_term / neg_term) of each
+// Both write the outgoing-storage terms (pos_term / neg_term) of each
 // catchment, for the passes that sum them (k_outgoing_sum, k_limiter).
-template <class REAL, class STO, bool FUSED, bool TERMS>
+template <class REAL, class STO, bool FUSED>
 __global__ __launch_bounds__(kMaxBlock) void k_outflow(View<REAL, STO> v, int substep) {
     int64_t t, i;
     if (!v.catchment_of(xcd_block() * blockDim.x + threadIdx.x, t, i)) return;
@@ -186,12 +186,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_outflow(View<REAL, STO> v, int su
     store_if_changed(v.fcsd + e, f_csd, upd_fcsd);
     store_if_changed(v.fcsa + e, f_csa, upd_fcsa);
     if (v.has(CMF_HAS_TOTAL_STORAGE)) v.total_storage[e] = (STO)total_storage;
-    if (TERMS) {
-        const REAL pos = vmax(q_r, (REAL)0.0) + vmax(q_f, (REAL)0.0);
-        const REAL neg = vmin(q_r, (REAL)0.0) + vmin(q_f, (REAL)0.0);
-        v.pos_term[e] = pos * time_step;
-        v.neg_term[e] = mouth ? (REAL)0.0 : -neg * time_step;
-    }
+    const REAL pos = vmax(q_r, (REAL)0.0) + vmax(q_f, (REAL)0.0);
+    const REAL neg = vmin(q_r, (REAL)0.0) + vmin(q_f, (REAL)0.0);
+    v.pos_term[e] = pos * time_step;
+    v.neg_term[e] = mouth ? (REAL)0.0 : -neg * time_step;
 }
 
 // Outgoing total of catchment j from the catchment terms: upstream negative
@@ -373,34 +371,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_bif_outflow(View<REAL, STO> v, in
     v.path_raw[pe] = raw;
 }
 
-// fused: bifurcation total of catchment j (compute_bifurcation_inflow,
-// bifurcation.py:128-151) from the path sums and the end limiters, in
-// ascending path order; the upstream end also applies the limiters to the
-// path's per-level flows.
-template <class REAL, class STO>
-__device__ __forceinline__ STO bif_total_fused(const View<REAL, STO>& v, int64_t t, int64_t j) {
-    STO acc = (STO)0;
-    const int32_t b = v.ev_ptr[j], en = v.ev_ptr[j + 1];
-    for (int32_t k = b; k < en; ++k) {
-        const int32_t code = v.ev_code[k];
-        const int64_t p = code >> 1;
-        const int64_t pe = v.pel(t, p);
-        const REAL l0 = v.lim[v.el(t, v.bif_ci[p])];
-        const REAL l1 = v.lim[v.el(t, v.bif_di[p])];
-        const REAL raw = v.path_raw[pe];
-        const REAL net = (raw >= (REAL)0.0) ? raw * l0 : raw * l1;
-        acc += (code & 1) ? (STO)(-net) : (STO)net;
-        if (!(code & 1)) {
-            for (int64_t l = 0; l < v.LB; ++l) {
-                const int64_t kk = pe * v.LB + l;
-                const REAL q = v.bif_outflow[kk];
-                v.bif_outflow[kk] = (q >= (REAL)0.0) ? q * l0 : q * l1;
-            }
-        }
-    }
-    return acc;
-}
-
 // Sum of the bifurcation events of catchment j in ascending path order.
 // code = 2 p + role; role 0: j is the path's upstream end, 1: downstream end.
 template <class REAL, class STO>
@@ -433,7 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_bif_outgoing(View<REAL, STO> v) {
 }
 
 // FUSED reads the end catchments' limiters from the limiter scratch
-// (k_lim_touched); per-op from the limit_rate field compute_inflow wrote.
+// (k_limiter); per-op from the limit_rate field compute_inflow wrote.
 template <class REAL, class STO, bool FUSED>
 __global__ __launch_bounds__(kMaxBlock) void k_bif_inflow(View<REAL, STO> v, int substep) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -517,57 +487,6 @@ __global__ __launch_bounds__(kBlock) void k_inflow_sum(View<REAL, STO> v) {
     v.river_inflow[e] = ri;
     v.flood_inflow[e] = fi;
     if (res) v.res_inflow[e] = rt;
-}
-
-// ------------------------------------------- limits from raw flows ----
-// The fused schedule keeps only the unlimited flows of the outflow pass and
-// recomputes each storage limiter where it is needed, from the same terms the
-// reference accumulates into outgoing_storage (outflow.py:306-314): the
-// negative-flow terms of the upstream catchments (ascending network index),
-// the catchment's own positive term, then its bifurcation path terms.
-template <class REAL>
-__device__ __forceinline__ REAL pos_term_of(REAL qr, REAL qf, REAL dt) {
-    return (vmax(qr, (REAL)0.0) + vmax(qf, (REAL)0.0)) * dt;
-}
-// negative-flow term of a catchment that has a downstream (not a mouth)
-template <class REAL>
-__device__ __forceinline__ REAL neg_term_of(REAL qr, REAL qf, REAL dt) {
-    return -(vmin(qr, (REAL)0.0) + vmin(qf, (REAL)0.0)) * dt;
-}
-
-// Storage limiter of catchment x of trial t (compute_inflow, outflow.py:346-352).
-template <class REAL, class STO, bool CONTIG>
-__device__ __forceinline__ REAL limit_recomputed(const View<REAL, STO>& v, int64_t t, int64_t x, REAL dt,
-                                                 REAL qr_x, REAL qf_x, STO rate_sto) {
-    STO og = (STO)0;
-    const int32_t b = v.up_ptr[x], en = v.up_ptr[x + 1];
-    for (int32_t k = b; k < en; ++k) {
-        const int64_t w = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
-        if (CONTIG && w == x) continue;
-        const int64_t ew = v.el(t, w);
-        og += (STO)neg_term_of(v.qraw_r[ew], v.qraw_f[ew], dt);
-    }
-    og += (STO)pos_term_of(qr_x, qf_x, dt);
-    if (v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[x] != v.ev_ptr[x + 1])
-        og = bif_event_sum<REAL, STO>(v, t, x, og, true);
-    const REAL ogr = (REAL)og;
-    const REAL rate = (REAL)rate_sto;
-    return (ogr > (REAL)1e-8) ? vmin(rate / ogr, (REAL)1.0) : (REAL)1.0;
-}
-
-// fused, with bifurcation: limiters of the path end catchments, for the
-// path nets of k_bif_inflow.
-template <class REAL, class STO, bool CONTIG>
-__global__ __launch_bounds__(kMaxBlock) void k_lim_touched(View<REAL, STO> v, const int32_t* touched,
-                                                        int64_t n_touched, int substep) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n_touched * v.T) return;
-    const int64_t t = g / n_touched, x = touched[g - t * n_touched];
-    if (!v.active(t, x, substep)) return;
-    const int64_t e = v.el(t, x);
-    const REAL dt = v.dt(t, x);
-    v.lim[e] = limit_recomputed<REAL, STO, CONTIG>(v, t, x, dt, v.qraw_r[e], v.qraw_f[e],
-                                                   v.river_storage[e] + v.flood_storage[e]);
 }
 
 // ======================================================= flood stage ====
@@ -1124,107 +1043,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_fused(View<REAL, STO> v, in
             v.flood_fraction[e] = lo.f_frc;
         } else {
             store_state(v, e, so, pre.fs_old, pre.ps_old);
-        }
-    }
-    if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
-}
-
-// fused: storage limiters (recomputed), limited outflow, inflow sums,
-// bifurcation totals, storage update and flood stage, and with LEV the
-// levee stage of levee catchments, in one pass.  All stores follow all loads.
-// `store_inflow` writes river/flood inflow and the bifurcation total back
-// (the reference leaves the last sub-step's values there).
-template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
-__global__ __launch_bounds__(kMaxBlock) void k_route(View<REAL, STO> v, int substep, int store_inflow) {
-    const int64_t NT = v.N * v.T;
-    const int64_t g = xcd_block() * blockDim.x + threadIdx.x;
-    LogPart lp;
-    if (LOG) {
-#pragma unroll
-        for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
-    }
-    int64_t t, j;
-    if (v.catchment_of(g, t, j)) {
-        if (v.active(t, j, substep)) {
-            const int64_t e = v.el(t, j);
-            const REAL dt = v.dt(t, j);
-            const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
-            const STO rs_old = v.river_storage[e], fs_old = v.flood_storage[e];
-            const REAL l_own = limit_recomputed<REAL, STO, CONTIG>(v, t, j, dt, qr, qf, rs_old + fs_old);
-            // upstream u has downstream j: positive flow limited by u, negative by j
-            const bool res = v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
-            STO ri = (STO)0, fi = (STO)0;
-            STO rt = res ? v.res_inflow[e] : (STO)0;
-            const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
-            for (int32_t k = b; k < en; ++k) {
-                const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
-                if (CONTIG && u == j) continue;
-                const int64_t eu = v.el(t, u);
-                const REAL ur0 = v.qraw_r[eu], uf0 = v.qraw_f[eu];
-                REAL lu = (REAL)1.0;
-                if (ur0 >= (REAL)0.0 || uf0 >= (REAL)0.0)
-                    lu = limit_recomputed<REAL, STO, CONTIG>(v, t, u, dt, ur0, uf0,
-                                                             v.river_storage[eu] + v.flood_storage[eu]);
-                const REAL ur1 = (ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own;
-                const REAL uf1 = (uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own;
-                ri += (STO)ur1;
-                fi += (STO)uf1;
-                if (res) rt += (STO)(ur1 + uf1);
-            }
-            // own limited flow: negative flow is limited by the downstream catchment
-            REAL l_dn = l_own;
-            if (qr < (REAL)0.0 || qf < (REAL)0.0) {
-                const int64_t d = v.dn[j];
-                if (d != j) {
-                    const int64_t ed = v.el(t, d);
-                    l_dn = limit_recomputed<REAL, STO, CONTIG>(v, t, d, dt, v.qraw_r[ed], v.qraw_f[ed],
-                                                               v.river_storage[ed] + v.flood_storage[ed]);
-                }
-            }
-            const REAL rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
-            const REAL fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
-            STO gsum = (STO)0;
-            REAL gb = (REAL)0.0;
-            if (v.has(CMF_HAS_BIFURCATION)) {
-                if (v.ev_ptr[j] != v.ev_ptr[j + 1]) gsum = bif_total_fused<REAL, STO>(v, t, j);
-                gb = (REAL)gsum;
-            }
-            // per-trial counts: each trial stores on its own last sub-step
-            const bool store = v.last_substep(t, j, substep, store_inflow);
-            const bool is_lev = LEV && v.is_levee[j];
-            TableRow<REAL, LFIX> row;
-            bool row_loaded = false;
-            const STO ps_old = v.protected_storage[e];
-            const StageOut<REAL, STO> so = stage_one<REAL, STO, LFIX, LOG>(
-                v, e, j, (REAL)ri, (REAL)fi, rout, fout, gb, dt, !(v.has(CMF_HAS_LEVEE) && v.is_levee[j]),
-                row, row_loaded, lp);
-            // ---- stores
-            v.river_outflow[e] = rout;
-            v.flood_outflow[e] = fout;
-            if (v.acc_outflow) v.acc_outflow[e] += (double)(rout + fout) * (double)dt;
-            if (res) v.res_inflow[e] = rt;
-            if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e] = l_own;
-            if (store) {
-                v.river_inflow[e] = ri;
-                v.flood_inflow[e] = fi;
-                if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = gsum;
-            }
-            if (is_lev) {
-                const int32_t lv = v.levee_index[j];
-                if (!row_loaded) row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
-                const LeveeOut<REAL, STO> lo = levee_one<REAL, STO, LFIX>(
-                    v, e, j, lv, t * v.V + lv, so.rsf_hp, so.fsf_hp, so.fdep, so.rdep, so.ffr, row);
-                if (LOG) levee_log(lo, so.rsf_hp + so.fsf_hp, v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, j), lp);
-                v.river_storage[e] = lo.r_hp;
-                v.flood_storage[e] = lo.f_hp;
-                v.protected_storage[e] = lo.p_hp;
-                v.river_depth[e] = lo.r_dph;
-                v.flood_depth[e] = lo.f_dph;
-                v.protected_depth[e] = lo.p_dph;
-                v.flood_fraction[e] = lo.f_frc;
-            } else {
-                store_state(v, e, so, fs_old, ps_old);
-            }
         }
     }
     if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
