@@ -47,15 +47,21 @@ def algorithmic_bytes(kernel: str, model, f_wet: float) -> float:
     N, T, P, V = model.N, model.T, model.P, model.V
     lev = V > 0
     bif = P > 0
+    # the path pass that shares the outflow launch: ends 8, length 4, five
+    # levels of manning/width/elevation/depth/flow read and depth/flow written,
+    # path terms 12 (end-catchment state gathers taken as cache hits)
+    paths = (8 + 4 + 5 * (4 * 3 + 8 + 8) + 12) * P * T
     if kernel.startswith("k_outflow"):
         per = 84 + 28 + (5 if lev else 0) + (12 if bif else 0) + (4 if lev and bif else 0)
-        return per * N * T
+        return per * N * T + (paths if kernel.startswith("k_outflow_paths") else 0)
     if kernel.startswith("k_limiter"):
         per = 4 + 4 + 4 + 16 + 4 + (8 if bif else 0)
         return per * N * T
     if kernel.startswith("k_stage"):
         per = 12 + 12 + 4 + 24 + 4 + 12 + 48 + 44 * f_wet
-        return per * N * T
+        # path nets: events 8 + other end 4 + level sum 4 + limiter 4 at each
+        # end, the five limited level flows read and written once
+        return per * N * T + ((2 * (8 + 4 + 4 + 4) + 5 * 8) * P * T if bif else 0)
     if kernel.startswith("k_levee"):
         return (4 + 12 + 16 + 16 + 40 + 4 + 4 + 4 + 24 + 16) * V * T
     if kernel.startswith("k_bif_outflow"):
@@ -337,7 +343,7 @@ def read_traffic(kernel: str):
         return None
     for line in path.read_text().splitlines():
         parts = line.split()
-        if len(parts) >= 2 and not line.startswith("#") and kernel.startswith(parts[0]):
+        if len(parts) >= 2 and not line.startswith("#") and kernel.split("<")[0] == parts[0]:
             try:
                 return float(parts[1])
             except ValueError:

@@ -23,10 +23,9 @@ from . import _lib
 
 CANDIDATES = (64, 128, 256, 512)
 KERNELS = {
-    "k_outflow": 0, "k_reservoir": 1, "k_bif_outflow": 2, "k_bif_inflow": 3,
-    "k_limiter": 4, "k_stage_fused": 5,
+    "k_outflow": 0, "k_outflow_paths": 0, "k_reservoir": 1, "k_limiter": 2, "k_stage_fused": 3,
 }
-NUM_TUNABLE = 6
+NUM_TUNABLE = 4
 
 
 def kernel_times(model, reps: int) -> dict:
@@ -72,7 +71,7 @@ def default_cache_path() -> Path:
     return Path(os.environ.get("CMF_TUNE_CACHE", Path.home() / ".cache" / "cmfgpu_amd" / "block_sizes.json"))
 
 
-def tune(model, *, reps: int = 8, candidates=CANDIDATES, cache_path: str | Path | None = None,
+def tune(model, *, reps: int = 16, candidates=CANDIDATES, cache_path: str | Path | None = None,
          use_cache: bool = True) -> dict:
     """Pick and apply the block size of each tunable kernel; returns
     {kernel name: {"block": size, "ms": {size: mean ms}}}."""
@@ -90,19 +89,28 @@ def tune(model, *, reps: int = 8, candidates=CANDIDATES, cache_path: str | Path 
             return cache[key]["report"]
     saved = {name: t.clone() for name, t in model.t.items()}
     original = get_block_sizes(model)
+
+    def restore():
+        for name, t in model.t.items():
+            t.copy_(saved[name])
+        torch.cuda.synchronize()
+
     times = {}
     try:
+        # every candidate runs the same sub-steps from the same state: the
+        # stage pass's cost follows the wet fraction, which drifts as the
+        # state advances
         for size in candidates:
             set_block_sizes(model, [size] * NUM_TUNABLE)
+            restore()
             kernel_times(model, 2)               # warm-up
+            restore()
             for name, ms in kernel_times(model, reps).items():
                 base = name.split("<")[0]
                 if base in KERNELS:
                     times.setdefault(base, {})[size] = ms
     finally:
-        for name, t in model.t.items():
-            t.copy_(saved[name])
-        torch.cuda.synchronize()
+        restore()
     sizes = list(original)
     report = {}
     for base, per in times.items():

@@ -42,6 +42,7 @@ struct cmf_network {
     int32_t* up_idx = nullptr;
     int32_t* ev_ptr = nullptr;
     int32_t* ev_code = nullptr;
+    int32_t* ev_other = nullptr;
     void* pos_term = nullptr;
     void* neg_term = nullptr;
     void* qraw_r = nullptr;
@@ -55,7 +56,7 @@ struct cmf_network {
     int64_t partial_blocks = 0;
     int32_t* host_counts = nullptr;   // pinned, max(T, G) entries
     // block size of each tunable kernel of the fused schedule (CMF_TUNE_*)
-    int32_t blocks[CMF_TUNE_COUNT] = {256, 256, 256, 256, 256, 256};
+    int32_t blocks[CMF_TUNE_COUNT] = {256, 256, 256, 256};
     // groups (G > 0): group of each catchment and the block table of the
     // catchment kernels (blocks never straddle two groups)
     int64_t G = 0;
@@ -82,7 +83,7 @@ namespace {
 
 void free_network(cmf_network* net) {
     if (!net) return;
-    void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->pos_term,
+    void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->ev_other, net->pos_term,
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
                     net->path_neg, net->path_net, net->path_raw, net->partials, net->cgroup,
                     net->blk_first, net->blk_end, net->blk_group, net->gblk_ptr};
@@ -227,6 +228,7 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.up_idx = net->contiguous ? nullptr : net->up_idx;
     v.ev_ptr = net->ev_ptr;
     v.ev_code = net->ev_code;
+    v.ev_other = net->ev_other;
     v.pos_term = static_cast<REAL*>(net->pos_term);
     v.neg_term = static_cast<REAL*>(net->neg_term);
     v.qraw_r = static_cast<REAL*>(net->qraw_r);
@@ -471,7 +473,7 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
             if (!(f->flags & CMF_HAS_BIFURCATION))
                 return set_error(CMF_ERR_INVALID, "compute_bifurcation_inflow: bifurcation is off");
             if ((rc = need_features(f, "compute_bifurcation_inflow"))) return rc;
-            LAUNCH((k_bif_inflow<REAL, STO, false>), v.P * v.T, stream, v, -1);
+            LAUNCH((k_bif_inflow<REAL, STO>), v.P * v.T, stream, v, -1);
             LAUNCH((k_gbif_sum<REAL, STO>), NT, stream, v);
             return CMF_OK;
         case OP_LEVEE:
@@ -533,9 +535,9 @@ int launch_stage(const cmf_network* net, const View<REAL, STO>& v, int64_t nt, b
     return CMF_OK;
 }
 
-// One fused sub-step: outflow, [reservoir], [bifurcation outflow], limiters,
-// [bifurcation inflow], stage (inflow sums, limited flows, storage, flood
-// stage [, levee stage]).
+// One fused sub-step: outflow [+ bifurcation paths], [reservoir], limiters,
+// stage (inflow sums, limited flows, bifurcation nets, storage, flood stage
+// [, levee stage]).
 template <class REAL, class STO>
 int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int substep,
                   int store_inflow, void* stream) {
@@ -545,26 +547,28 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
     const bool bif = v.has(CMF_HAS_BIFURCATION), levee = v.has(CMF_HAS_LEVEE);
     const bool log = v.has(CMF_HAS_LOG);
     const bool lev_fused = levee && v.levee_index != nullptr;
-    const int64_t b_out = block_of(net, CMF_TUNE_OUTFLOW, true), b_bo = block_of(net, CMF_TUNE_BIF_OUTFLOW, false);
+    const int64_t b_out = block_of(net, CMF_TUNE_OUTFLOW, true);
     const int64_t b_lim = block_of(net, CMF_TUNE_LIMITER, true), b_stage = block_of(net, CMF_TUNE_STAGE, true);
-    const int64_t b_bi = block_of(net, CMF_TUNE_BIF_INFLOW, false);
-    LAUNCH_B((k_outflow<REAL, STO, true>), NT, b_out, stream, v, substep);
+    if (bif) {
+        // the path pass rides in the outflow launch, in blocks of its own first
+        const int64_t pb = (v.P * v.T + b_out - 1) / b_out;
+        const int64_t cnt = (pb + (NT + b_out - 1) / b_out) * b_out;
+        if (levee) {
+            if (v.LB == 5) LAUNCH_B((k_outflow_paths<REAL, STO, true, 5>), cnt, b_out, stream, v, substep, pb);
+            else LAUNCH_B((k_outflow_paths<REAL, STO, true, 0>), cnt, b_out, stream, v, substep, pb);
+        } else {
+            if (v.LB == 5) LAUNCH_B((k_outflow_paths<REAL, STO, false, 5>), cnt, b_out, stream, v, substep, pb);
+            else LAUNCH_B((k_outflow_paths<REAL, STO, false, 0>), cnt, b_out, stream, v, substep, pb);
+        }
+    } else {
+        LAUNCH_B((k_outflow<REAL, STO, true>), NT, b_out, stream, v, substep);
+    }
     if (v.has(CMF_HAS_RESERVOIR))
         LAUNCH_B((k_reservoir<REAL, STO, true>), v.R * v.T, block_of(net, CMF_TUNE_RESERVOIR, false), stream, v, substep);
-    if (bif) {
-        if (levee) {
-            if (v.LB == 5) LAUNCH_B((k_bif_outflow<REAL, STO, true, 5>), v.P * v.T, b_bo, stream, v, substep);
-            else LAUNCH_B((k_bif_outflow<REAL, STO, true, 0>), v.P * v.T, b_bo, stream, v, substep);
-        } else {
-            if (v.LB == 5) LAUNCH_B((k_bif_outflow<REAL, STO, false, 5>), v.P * v.T, b_bo, stream, v, substep);
-            else LAUNCH_B((k_bif_outflow<REAL, STO, false, 0>), v.P * v.T, b_bo, stream, v, substep);
-        }
-    }
-    // limiters of every catchment (the path nets of k_bif_inflow need the
-    // limiters of both path ends), then the path nets, then the stage pass
+    // limiters of every catchment, then the stage pass (which also forms the
+    // path nets from the limiters of both path ends)
     if (contig) LAUNCH_B((k_limiter<REAL, STO, true>), NT, b_lim, stream, v, substep);
     else LAUNCH_B((k_limiter<REAL, STO, false>), NT, b_lim, stream, v, substep);
-    if (bif) LAUNCH_B((k_bif_inflow<REAL, STO, true>), v.P * v.T, b_bi, stream, v, substep);
     int rc = log ? (contig ? launch_stage<REAL, STO, true, true>(net, v, NT, lev_fused, substep, store_inflow, stream)
                            : launch_stage<REAL, STO, false, true>(net, v, NT, lev_fused, substep, store_inflow, stream))
                  : (contig ? launch_stage<REAL, STO, true, false>(net, v, NT, lev_fused, substep, store_inflow, stream)
@@ -955,13 +959,15 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     }
     if (contiguous) up_ptr.swap(cup);
     // bifurcation events per catchment, ascending path, upstream end first
-    std::vector<int32_t> ev_ptr((size_t)n + 1, 0), ev_code((size_t)(2 * num_paths));
+    std::vector<int32_t> ev_ptr((size_t)n + 1, 0), ev_code((size_t)(2 * num_paths)), ev_other((size_t)(2 * num_paths));
     for (int64_t p = 0; p < num_paths; ++p) { ev_ptr[bif_ci[p] + 1]++; ev_ptr[bif_di[p] + 1]++; }
     for (int64_t i = 0; i < n; ++i) ev_ptr[i + 1] += ev_ptr[i];
     {
         std::vector<int32_t> fill(ev_ptr.begin(), ev_ptr.end() - 1);
         for (int64_t p = 0; p < num_paths; ++p) {
+            ev_other[fill[bif_ci[p]]] = bif_di[p];
             ev_code[fill[bif_ci[p]]++] = (int32_t)(2 * p);
+            ev_other[fill[bif_di[p]]] = bif_ci[p];
             ev_code[fill[bif_di[p]]++] = (int32_t)(2 * p + 1);
         }
     }
@@ -981,6 +987,7 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
         (rc = upload((void**)&net->up_idx, up_idx.data(), up_idx.size() * 4)) ||
         (rc = upload((void**)&net->ev_ptr, ev_ptr.data(), ev_ptr.size() * 4)) ||
         (rc = upload((void**)&net->ev_code, ev_code.data(), ev_code.size() * 4)) ||
+        (rc = upload((void**)&net->ev_other, ev_other.data(), ev_other.size() * 4)) ||
         (rc = dev_alloc(&net->pos_term, el)) || (rc = dev_alloc(&net->neg_term, el)) ||
         (rc = dev_alloc(&net->qraw_r, el)) || (rc = dev_alloc(&net->qraw_f, el)) ||
         (rc = dev_alloc(&net->lim, el)) || (rc = dev_alloc(&net->path_pos, pel)) ||

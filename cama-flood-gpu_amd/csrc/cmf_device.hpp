@@ -142,7 +142,10 @@ struct View {
     // nullptr: contiguous ranges, where a mouth's range may end with the mouth
     // itself) or through up_idx.
     const int32_t *up_ptr, *up_idx;
-    const int32_t *ev_ptr, *ev_code;    // bifurcation events per catchment
+    // bifurcation events per catchment: ev_code = 2 path + role (0: the
+    // catchment is the path's upstream end, 1: its downstream end), ev_other
+    // the path's other end
+    const int32_t *ev_ptr, *ev_code, *ev_other;
     // scratch
     REAL *pos_term, *neg_term, *qraw_r, *qraw_f, *lim;
     REAL *path_pos, *path_neg, *path_net, *path_raw;
@@ -208,6 +211,16 @@ struct View {
         if (g >= N * T) return false;
         split_c(g, t, i);
         return true;
+    }
+    // the same for block `blk` of the catchment grid
+    __device__ __forceinline__ bool catchment_in(int64_t blk, int64_t& t, int64_t& i) const {
+        if (G > 0) {
+            const int64_t j = (int64_t)blk_first[blk] + threadIdx.x;
+            t = 0;
+            i = j;
+            return j < blk_end[blk];
+        }
+        return catchment_of(blk * blockDim.x + threadIdx.x, t, i);
     }
     __device__ __forceinline__ void split_p(int64_t g, int64_t& t, int64_t& p) const {
         split(g, P, pts == 1, t, p);

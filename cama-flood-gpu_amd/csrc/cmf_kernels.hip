@@ -64,10 +64,11 @@ __device__ __forceinline__ void store_if_changed(X* p, X old, X val) {
 // reference, and river/flood inflow and the bifurcation total are zeroed.
 // Both write the outgoing-storage terms (pos_term / neg_term) of each
 // catchment, for the passes that sum them (k_outgoing_sum, k_limiter).
+// `blk`: the block's index in the catchment grid.
 template <class REAL, class STO, bool FUSED>
-__global__ __launch_bounds__(kMaxBlock) void k_outflow(View<REAL, STO> v, int substep) {
+__device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int substep, int64_t blk) {
     int64_t t, i;
-    if (!v.catchment_of(xcd_block() * blockDim.x + threadIdx.x, t, i)) return;
+    if (!v.catchment_in(blk, t, i)) return;
     if (!v.active(t, i, substep)) return;
     const int64_t e = v.el(t, i);
     const int64_t d = v.dn[i];
@@ -192,6 +193,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_outflow(View<REAL, STO> v, int su
     v.neg_term[e] = mouth ? (REAL)0.0 : -neg * time_step;
 }
 
+template <class REAL, class STO, bool FUSED>
+__global__ __launch_bounds__(kMaxBlock) void k_outflow(View<REAL, STO> v, int substep) {
+    outflow_one<REAL, STO, FUSED>(v, substep, xcd_block());
+}
+
 // Outgoing total of catchment j from the catchment terms: upstream negative
 // terms in ascending network index, then j's own positive term.
 template <class REAL, class STO, bool CONTIG>
@@ -290,13 +296,37 @@ __global__ __launch_bounds__(kBlock) void k_res_regather(View<REAL, STO> v) {
 }
 
 // ====================================================== bifurcation ======
-// LBFIX > 0: the level count known at compile time (loop unrolled, the
-// per-level flows kept in registers).  Besides the per-level state and the
-// outgoing terms, writes path_raw: the sum of the limited per-level flows,
-// in level order, as compute_bifurcation_inflow forms it (bifurcation.py:132-137).
-template <class REAL, class STO, bool LEVEE, int LBFIX>
-__global__ __launch_bounds__(kMaxBlock) void k_bif_outflow(View<REAL, STO> v, int substep) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Water surface, protected water surface and total storage of catchment i
+// from the state the stage pass left, by the expressions of the outflow pass
+// (outflow.py:185-200, levee.py:565-600); bit-identical to the values that
+// pass stores in the water_surface_elevation, protected_water_surface_elevation
+// and total_storage fields.
+template <class REAL, class STO>
+__device__ __forceinline__ REAL surface_of(const View<REAL, STO>& v, int64_t e, int64_t i) {
+    const REAL c_elv = v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, e, i);
+    const REAL r_hgt = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, i);
+    return v.river_depth[e] + (c_elv - r_hgt);
+}
+template <class REAL, class STO>
+__device__ __forceinline__ REAL protected_surface_of(const View<REAL, STO>& v, int64_t e, int64_t i, REAL wse) {
+    if (!(v.has(CMF_HAS_LEVEE) && v.is_levee[i])) return wse;
+    return vmin(v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, e, i) + v.protected_depth[e], wse);
+}
+template <class REAL, class STO>
+__device__ __forceinline__ REAL total_of(const View<REAL, STO>& v, int64_t e) {
+    return (REAL)v.river_storage[e] + (REAL)v.flood_storage[e] + (REAL)v.protected_storage[e];
+}
+
+// One path-trial element `g`.  LBFIX > 0: the level count known at compile
+// time (loop unrolled, the per-level flows kept in registers).  FROM_STATE:
+// the end catchments' surfaces and total storages are formed from the state
+// (surface_of, total_of), so the path pass can share the launch of the
+// catchment outflow pass; otherwise they are read from the fields that pass
+// wrote.  Besides the per-level state and the outgoing terms, writes path_raw:
+// the sum of the limited per-level flows, in level order, as
+// compute_bifurcation_inflow forms it (bifurcation.py:132-137).
+template <class REAL, class STO, bool LEVEE, int LBFIX, bool FROM_STATE>
+__device__ __forceinline__ void bif_path(const View<REAL, STO>& v, int substep, int64_t g) {
     if (g >= v.P * v.T) return;
     int64_t t, p;
     v.split_p(g, t, p);
@@ -307,14 +337,23 @@ __global__ __launch_bounds__(kMaxBlock) void k_bif_outflow(View<REAL, STO> v, in
     const REAL time_step = v.dt(t, ci);
     const REAL gravity = v.gravity;
     const REAL length = v.prm(v.bif_length, CMF_BATCHED_BIFURCATION_LENGTH, pe, p);
-    const REAL w0 = v.wse[eci], w1 = v.wse[edi];
+    REAL w0, w1, ts0, ts1, max_pw = (REAL)0;
+    if (FROM_STATE) {
+        w0 = surface_of(v, eci, ci);
+        w1 = surface_of(v, edi, di);
+        if (LEVEE) max_pw = vmax(protected_surface_of(v, eci, ci, w0), protected_surface_of(v, edi, di, w1));
+        ts0 = total_of(v, eci);
+        ts1 = total_of(v, edi);
+    } else {
+        w0 = v.wse[eci];
+        w1 = v.wse[edi];
+        if (LEVEE) max_pw = vmax(v.pwse[eci], v.pwse[edi]);
+        ts0 = (REAL)v.total_storage[eci];
+        ts1 = (REAL)v.total_storage[edi];
+    }
     const REAL max_w = vmax(w0, w1);
-    REAL max_pw = (REAL)0;
-    if (LEVEE) max_pw = vmax(v.pwse[eci], v.pwse[edi]);
     REAL slope = (w0 - w1) / length;
     slope = vclamp(slope, (REAL)-0.005, (REAL)0.005);
-    const REAL ts0 = (REAL)v.total_storage[eci];
-    const REAL ts1 = (REAL)v.total_storage[edi];
     const int LB = LBFIX > 0 ? LBFIX : (int)v.LB;
     REAL qs[LBFIX > 0 ? LBFIX : 1];
     REAL sum = (REAL)0;
@@ -371,6 +410,58 @@ __global__ __launch_bounds__(kMaxBlock) void k_bif_outflow(View<REAL, STO> v, in
     v.path_raw[pe] = raw;
 }
 
+// per-op: compute_bifurcation_outflow
+template <class REAL, class STO, bool LEVEE, int LBFIX>
+__global__ __launch_bounds__(kMaxBlock) void k_bif_outflow(View<REAL, STO> v, int substep) {
+    bif_path<REAL, STO, LEVEE, LBFIX, false>(v, substep, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// fused, with bifurcation: the catchment outflow pass and the path pass in
+// one launch.  Blocks [0, path_blocks) run the paths, first, so that their
+// longer threads overlap the catchment blocks; the rest run the catchments.
+template <class REAL, class STO, bool LEVEE, int LBFIX>
+__global__ __launch_bounds__(kMaxBlock) void k_outflow_paths(View<REAL, STO> v, int substep, int64_t path_blocks) {
+    const int64_t b = blockIdx.x;
+    if (b < path_blocks) bif_path<REAL, STO, LEVEE, LBFIX, true>(v, substep, b * blockDim.x + threadIdx.x);
+    else outflow_one<REAL, STO, true>(v, substep, b - path_blocks);
+}
+
+// fused: bifurcation total of catchment j (compute_bifurcation_inflow,
+// bifurcation.py:128-151) in ascending path order, each path net formed from
+// the path's level sum and the limiters of its two ends (l_own: j's); the
+// upstream end also writes the path's limited per-level flows.
+template <class REAL, class STO>
+__device__ __forceinline__ STO bif_nets(const View<REAL, STO>& v, int64_t t, int64_t j, REAL l_own) {
+    STO acc = (STO)0;
+    const int32_t b = v.ev_ptr[j], en = v.ev_ptr[j + 1];
+    for (int32_t k = b; k < en; ++k) {
+        const int32_t code = v.ev_code[k];
+        const int64_t pe = v.pel(t, code >> 1);
+        const REAL l_other = v.lim[v.el(t, v.ev_other[k])];
+        const bool down_end = (code & 1) != 0;
+        const REAL l0 = down_end ? l_other : l_own;   // limiter of the upstream end
+        const REAL l1 = down_end ? l_own : l_other;   // of the downstream end
+        const REAL raw = v.path_raw[pe];
+        const REAL net = (raw >= (REAL)0.0) ? raw * l0 : raw * l1;
+        acc += down_end ? (STO)(-net) : (STO)net;
+        if (!down_end) {
+            REAL* q = v.bif_outflow + pe * v.LB;
+            if (v.LB == 5) {
+                // all five loads in flight together, then the stores
+                const REAL q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4];
+                q[0] = (q0 >= (REAL)0.0) ? q0 * l0 : q0 * l1;
+                q[1] = (q1 >= (REAL)0.0) ? q1 * l0 : q1 * l1;
+                q[2] = (q2 >= (REAL)0.0) ? q2 * l0 : q2 * l1;
+                q[3] = (q3 >= (REAL)0.0) ? q3 * l0 : q3 * l1;
+                q[4] = (q4 >= (REAL)0.0) ? q4 * l0 : q4 * l1;
+            } else {
+                for (int64_t l = 0; l < v.LB; ++l) q[l] = (q[l] >= (REAL)0.0) ? q[l] * l0 : q[l] * l1;
+            }
+        }
+    }
+    return acc;
+}
+
 // Sum of the bifurcation events of catchment j in ascending path order.
 // code = 2 p + role; role 0: j is the path's upstream end, 1: downstream end.
 template <class REAL, class STO>
@@ -402,9 +493,8 @@ __global__ __launch_bounds__(kBlock) void k_bif_outgoing(View<REAL, STO> v) {
     v.outgoing[e] = bif_event_sum<REAL, STO>(v, t, j, v.outgoing[e], true);
 }
 
-// FUSED reads the end catchments' limiters from the limiter scratch
-// (k_limiter); per-op from the limit_rate field compute_inflow wrote.
-template <class REAL, class STO, bool FUSED>
+// per-op: path nets from the limit_rate field compute_inflow wrote
+template <class REAL, class STO>
 __global__ __launch_bounds__(kMaxBlock) void k_bif_inflow(View<REAL, STO> v, int substep) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= v.P * v.T) return;
@@ -413,8 +503,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_bif_inflow(View<REAL, STO> v, int
     const int64_t ci = v.bif_ci[p], di = v.bif_di[p];
     if (!v.active(t, ci, substep)) return;
     const int64_t pe = v.pel(t, p);
-    const REAL* lr = FUSED ? v.lim : v.limit_rate;
-    const REAL l0 = lr[v.el(t, ci)], l1 = lr[v.el(t, di)];
+    const REAL l0 = v.limit_rate[v.el(t, ci)], l1 = v.limit_rate[v.el(t, di)];
     REAL raw_sum = (REAL)0;
     for (int64_t l = 0; l < v.LB; ++l) {
         const int64_t k = pe * v.LB + l;
@@ -1016,7 +1105,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_fused(View<REAL, STO> v, in
         STO gsum = (STO)0;
         REAL gb = (REAL)0.0;
         if (v.has(CMF_HAS_BIFURCATION)) {
-            if (events) gsum = bif_event_sum<REAL, STO>(v, t, j, gsum, false);
+            if (events) gsum = bif_nets<REAL, STO>(v, t, j, l_own);
             gb = (REAL)gsum;
         }
         if (v.last_substep(t, j, substep, store_inflow)) {

@@ -85,8 +85,7 @@ enum { CMF_PRECISION_F32_F64 = 0, CMF_PRECISION_F64_F64 = 1 };
 /* Tunable kernels of the fused schedule (block sizes: multiples of 64 up to
  * 512, see cmf_network_set_block_sizes). */
 enum {
-    CMF_TUNE_OUTFLOW = 0, CMF_TUNE_RESERVOIR, CMF_TUNE_BIF_OUTFLOW, CMF_TUNE_BIF_INFLOW,
-    CMF_TUNE_LIMITER, CMF_TUNE_STAGE, CMF_TUNE_COUNT
+    CMF_TUNE_OUTFLOW = 0, CMF_TUNE_RESERVOIR, CMF_TUNE_LIMITER, CMF_TUNE_STAGE, CMF_TUNE_COUNT
 };
 
 /* The 11 water-balance sums, in LogModule field order
