@@ -23,7 +23,8 @@ from . import _lib
 
 CANDIDATES = (64, 128, 256, 512)
 KERNELS = {
-    "k_outflow": 0, "k_outflow_paths": 0, "k_reservoir": 1, "k_limiter": 2, "k_stage_fused": 3,
+    "k_outflow": 0, "k_outflow_paths": 0, "k_reservoir": 1, "k_limiter": 2, "k_limiter_pair": 2,
+    "k_stage_fused": 3,
 }
 NUM_TUNABLE = 4
 

@@ -1028,6 +1028,78 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter(View<REAL, STO> v, int su
     if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e] = l;
 }
 
+// The same for ungrouped networks, two catchments per thread (g and g +
+// blockDim.x of a block's 2 * blockDim.x) with their loads interleaved: the
+// pass is bound by dependent round trips (upstream range, then its terms), so
+// two independent chains per thread keep twice the loads in flight.
+template <class REAL, class STO, bool CONTIG>
+__global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(View<REAL, STO> v, int substep) {
+    const int64_t NT = v.N * v.T;
+    const int64_t g0 = (int64_t)blockIdx.x * (2 * (int64_t)blockDim.x) + threadIdx.x;
+    const int64_t g1 = g0 + blockDim.x;
+    int64_t t0 = 0, j0 = 0, t1 = 0, j1 = 0;
+    bool a0 = g0 < NT, a1 = g1 < NT;
+    if (a0) {
+        v.split_c(g0, t0, j0);
+        a0 = v.active(t0, j0, substep);
+    }
+    if (a1) {
+        v.split_c(g1, t1, j1);
+        a1 = v.active(t1, j1, substep);
+    }
+    if (!a0 && !a1) return;
+    const int64_t e0 = v.el(t0, j0), e1 = v.el(t1, j1);
+    int32_t b0 = 0, n0 = 0, b1 = 0, n1 = 0;
+    REAL p0 = (REAL)0, p1 = (REAL)0;
+    STO rs0 = (STO)0, fs0 = (STO)0, rs1 = (STO)0, fs1 = (STO)0;
+    if (a0) {
+        b0 = v.up_ptr[j0];
+        n0 = v.up_ptr[j0 + 1] - b0;
+        p0 = v.pos_term[e0];
+        rs0 = v.river_storage[e0];
+        fs0 = v.flood_storage[e0];
+    }
+    if (a1) {
+        b1 = v.up_ptr[j1];
+        n1 = v.up_ptr[j1 + 1] - b1;
+        p1 = v.pos_term[e1];
+        rs1 = v.river_storage[e1];
+        fs1 = v.flood_storage[e1];
+    }
+    // outgoing_terms of both, one loop so that both ranges' loads issue together
+    STO acc0 = (STO)0, acc1 = (STO)0;
+    const int32_t nmax = n0 > n1 ? n0 : n1;
+    for (int32_t k = 0; k < nmax; ++k) {
+        if (k < n0) {
+            const int64_t u = CONTIG ? (int64_t)(b0 + k) : (int64_t)v.up_idx[b0 + k];
+            if (!(CONTIG && u == j0)) acc0 += (STO)v.neg_term[v.el(t0, u)];
+        }
+        if (k < n1) {
+            const int64_t u = CONTIG ? (int64_t)(b1 + k) : (int64_t)v.up_idx[b1 + k];
+            if (!(CONTIG && u == j1)) acc1 += (STO)v.neg_term[v.el(t1, u)];
+        }
+    }
+    acc0 += (STO)p0;
+    acc1 += (STO)p1;
+    if (v.has(CMF_HAS_BIFURCATION)) {
+        if (a0 && v.ev_ptr[j0] != v.ev_ptr[j0 + 1]) acc0 = bif_event_sum<REAL, STO>(v, t0, j0, acc0, true);
+        if (a1 && v.ev_ptr[j1] != v.ev_ptr[j1 + 1]) acc1 = bif_event_sum<REAL, STO>(v, t1, j1, acc1, true);
+    }
+    // limit_of (outflow.py:346-352) of both
+    const REAL og0 = (REAL)acc0, og1 = (REAL)acc1;
+    const REAL r0 = (REAL)(rs0 + fs0), r1 = (REAL)(rs1 + fs1);
+    const REAL l0 = (og0 > (REAL)1e-8) ? vmin(r0 / og0, (REAL)1.0) : (REAL)1.0;
+    const REAL l1 = (og1 > (REAL)1e-8) ? vmin(r1 / og1, (REAL)1.0) : (REAL)1.0;
+    if (a0) {
+        v.lim[e0] = l0;
+        if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e0] = l0;
+    }
+    if (a1) {
+        v.lim[e1] = l1;
+        if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e1] = l1;
+    }
+}
+
 // fused, after k_limiter: limited outflow, inflow sums, bifurcation totals,
 // storage update and flood stage, and with LEV the levee stage of levee
 // catchments, in one pass.
