@@ -43,6 +43,7 @@ struct cmf_network {
     int32_t* ev_ptr = nullptr;
     int32_t* ev_code = nullptr;
     int32_t* ev_other = nullptr;
+    int2* ev1 = nullptr;
     void* pos_term = nullptr;
     void* neg_term = nullptr;
     void* qraw_r = nullptr;
@@ -83,7 +84,7 @@ namespace {
 
 void free_network(cmf_network* net) {
     if (!net) return;
-    void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->ev_other, net->pos_term,
+    void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->ev_other, net->ev1, net->pos_term,
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
                     net->path_neg, net->path_net, net->path_raw, net->partials, net->cgroup,
                     net->blk_first, net->blk_end, net->blk_group, net->gblk_ptr};
@@ -229,6 +230,7 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.ev_ptr = net->ev_ptr;
     v.ev_code = net->ev_code;
     v.ev_other = net->ev_other;
+    v.ev1 = net->ev1;
     v.pos_term = static_cast<REAL*>(net->pos_term);
     v.neg_term = static_cast<REAL*>(net->neg_term);
     v.qraw_r = static_cast<REAL*>(net->qraw_r);
@@ -978,6 +980,17 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
             ev_code[fill[bif_di[p]]++] = (int32_t)(2 * p + 1);
         }
     }
+    std::vector<int2> ev1((size_t)std::max<int64_t>(n, 1));
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t b = ev_ptr[i], e = ev_ptr[i + 1];
+        if (e == b) {
+            ev1[i].x = -1;
+            ev1[i].y = 0;
+        } else {
+            ev1[i].x = ev_code[b];
+            ev1[i].y = (e - b > 1) ? -(ev_other[b] + 1) : ev_other[b];
+        }
+    }
     cmf_network* net = new cmf_network();
     net->n = n;
     net->T = num_trials;
@@ -995,6 +1008,7 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
         (rc = upload((void**)&net->ev_ptr, ev_ptr.data(), ev_ptr.size() * 4)) ||
         (rc = upload((void**)&net->ev_code, ev_code.data(), ev_code.size() * 4)) ||
         (rc = upload((void**)&net->ev_other, ev_other.data(), ev_other.size() * 4)) ||
+        (rc = upload((void**)&net->ev1, ev1.data(), (size_t)n * sizeof(int2))) ||
         (rc = dev_alloc(&net->pos_term, el)) || (rc = dev_alloc(&net->neg_term, el)) ||
         (rc = dev_alloc(&net->qraw_r, el)) || (rc = dev_alloc(&net->qraw_f, el)) ||
         (rc = dev_alloc(&net->lim, el)) || (rc = dev_alloc(&net->path_pos, pel)) ||

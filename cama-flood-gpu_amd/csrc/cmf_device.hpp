@@ -146,6 +146,10 @@ struct View {
     // catchment is the path's upstream end, 1: its downstream end), ev_other
     // the path's other end
     const int32_t *ev_ptr, *ev_code, *ev_other;
+    // the first event of each catchment in one 8-byte record for the limiter
+    // pass: {code, other end}; code -1: no event; other end -(other + 1):
+    // more events follow (the ev_ptr list from its second entry)
+    const int2* ev1;
     // scratch
     REAL *pos_term, *neg_term, *qraw_r, *qraw_f, *lim;
     REAL *path_pos, *path_neg, *path_net, *path_raw;

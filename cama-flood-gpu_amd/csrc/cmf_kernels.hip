@@ -481,6 +481,23 @@ __device__ __forceinline__ STO bif_event_sum(const View<REAL, STO>& v, int64_t t
     return acc;
 }
 
+// The same from the first-event record `ev` of j (ev.x >= 0), the further
+// events (if any) from the list.
+template <class REAL, class STO>
+__device__ __forceinline__ STO bif_terms_from(const View<REAL, STO>& v, int64_t t, int64_t j, int2 ev, STO acc) {
+    const int64_t pe = v.pel(t, ev.x >> 1);
+    acc += (STO)((ev.x & 1) ? v.path_neg[pe] : v.path_pos[pe]);
+    if (ev.y < 0) {
+        const int32_t en = v.ev_ptr[j + 1];
+        for (int32_t k = v.ev_ptr[j] + 1; k < en; ++k) {
+            const int32_t code = v.ev_code[k];
+            const int64_t pk = v.pel(t, code >> 1);
+            acc += (STO)((code & 1) ? v.path_neg[pk] : v.path_pos[pk]);
+        }
+    }
+    return acc;
+}
+
 // per-op: outgoing_storage += bifurcation terms
 template <class REAL, class STO>
 __global__ __launch_bounds__(kBlock) void k_bif_outgoing(View<REAL, STO> v) {
@@ -1020,9 +1037,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter(View<REAL, STO> v, int su
     if (!v.catchment_of(xcd_block() * blockDim.x + threadIdx.x, t, j)) return;
     if (!v.active(t, j, substep)) return;
     const int64_t e = v.el(t, j);
+    const int2 ev = v.has(CMF_HAS_BIFURCATION) ? v.ev1[j] : make_int2(-1, 0);
     STO og = outgoing_terms<REAL, STO, CONTIG>(v, t, j);
-    if (v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[j] != v.ev_ptr[j + 1])
-        og = bif_event_sum<REAL, STO>(v, t, j, og, true);
+    if (ev.x >= 0) og = bif_terms_from<REAL, STO>(v, t, j, ev, og);
     const REAL l = limit_of(v, og, e);
     v.lim[e] = l;
     if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e] = l;
@@ -1052,6 +1069,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(View<REAL, STO> v, i
     int32_t b0 = 0, n0 = 0, b1 = 0, n1 = 0;
     REAL p0 = (REAL)0, p1 = (REAL)0;
     STO rs0 = (STO)0, fs0 = (STO)0, rs1 = (STO)0, fs1 = (STO)0;
+    int2 ev0 = make_int2(-1, 0), ev1 = make_int2(-1, 0);
+    if (v.has(CMF_HAS_BIFURCATION)) {
+        if (a0) ev0 = v.ev1[j0];
+        if (a1) ev1 = v.ev1[j1];
+    }
     if (a0) {
         b0 = v.up_ptr[j0];
         n0 = v.up_ptr[j0 + 1] - b0;
@@ -1081,10 +1103,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(View<REAL, STO> v, i
     }
     acc0 += (STO)p0;
     acc1 += (STO)p1;
-    if (v.has(CMF_HAS_BIFURCATION)) {
-        if (a0 && v.ev_ptr[j0] != v.ev_ptr[j0 + 1]) acc0 = bif_event_sum<REAL, STO>(v, t0, j0, acc0, true);
-        if (a1 && v.ev_ptr[j1] != v.ev_ptr[j1 + 1]) acc1 = bif_event_sum<REAL, STO>(v, t1, j1, acc1, true);
-    }
+    if (ev0.x >= 0) acc0 = bif_terms_from<REAL, STO>(v, t0, j0, ev0, acc0);
+    if (ev1.x >= 0) acc1 = bif_terms_from<REAL, STO>(v, t1, j1, ev1, acc1);
     // limit_of (outflow.py:346-352) of both
     const REAL og0 = (REAL)acc0, og1 = (REAL)acc1;
     const REAL r0 = (REAL)(rs0 + fs0), r1 = (REAL)(rs1 + fs1);
