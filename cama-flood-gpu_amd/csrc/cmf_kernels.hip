@@ -815,6 +815,25 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one_p(
     const REAL rs_c = (REAL)rs_hp, fs_c = (REAL)fs_hp, total = (REAL)total_hp;
 
     const REAL river_max_storage = rl * rw * rh;
+    // Below bankfull none of the cases applies: the base and crown storages
+    // formed below are river_max_storage plus non-negative terms, and with
+    // crown >= base the protected-side threshold s_top is not below the base
+    // storage, so all three tests are false and the partition keeps the state
+    // (the "none" branch at the end).  With negative geometry the full scan
+    // decides.
+    if (total < river_max_storage && crown >= base && rl > (REAL)0 && rw >= (REAL)0 && ca >= (REAL)0 &&
+        lfrac >= (REAL)0) {
+        LeveeOut<REAL, STO> o;
+        o.r_dph = rdep_curr;
+        o.f_dph = fdep_curr;
+        o.p_dph = (REAL)0.0;
+        o.f_frc = ffr_curr;
+        const STO remaining = vmax(total_hp - rs_hp, (STO)0.0);
+        o.r_hp = rs_hp;
+        o.f_hp = fs_hp;
+        o.p_hp = vmax(remaining - fs_hp, (STO)0.0);
+        return o;
+    }
     const REAL dwth_inc = (ca / rl) / (REAL)L;
     const REAL levee_distance = lfrac * (ca / rl);
     const int ilev = (int)(lfrac * (REAL)L);
