@@ -51,14 +51,17 @@ def algorithmic_bytes(kernel: str, model, f_wet: float) -> float:
     # levels of manning/width/elevation/depth/flow read and depth/flow written,
     # path terms 12 (end-catchment state gathers taken as cache hits)
     paths = (8 + 4 + 5 * (4 * 3 + 8 + 8) + 12) * P * T
+    # (the timed sub-step is not the last of its outer step: the water surfaces,
+    # total storage, flood fraction and protected depth, written on the last
+    # one only, are not counted)
     if kernel.startswith("k_outflow"):
-        per = 84 + 28 + (5 if lev else 0) + (12 if bif else 0) + (4 if lev and bif else 0)
+        per = 84 + 28 + (5 if lev else 0)
         return per * N * T + (paths if kernel.startswith("k_outflow_paths") else 0)
     if kernel.startswith("k_limiter"):
         per = 4 + 4 + 4 + 16 + 4 + (8 if bif else 0)
         return per * N * T
     if kernel.startswith("k_stage"):
-        per = 12 + 12 + 4 + 24 + 4 + 12 + 48 + 44 * f_wet
+        per = 12 + 12 + 4 + 24 + 4 + 12 + 40 + 44 * f_wet
         # path nets: events 8 + other end 4 + level sum 4 + limiter 4 at each
         # end, the five limited level flows read and written once
         return per * N * T + ((2 * (8 + 4 + 4 + 4) + 5 * 8) * P * T if bif else 0)

@@ -399,7 +399,7 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
         case OP_OUTFLOW:
             if ((rc = need_base(f, "compute_outflow")) || (rc = need_features(f, "compute_outflow"))) return rc;
             if (!f->outgoing_storage) return set_error(CMF_ERR_INVALID, "compute_outflow: outgoing_storage is NULL");
-            LAUNCH((k_outflow<REAL, STO, false>), NT, stream, v, -1);
+            LAUNCH((k_outflow<REAL, STO, false>), NT, stream, v, -1, 1);
             if (contig) LAUNCH((k_outgoing_sum<REAL, STO, true>), NT, stream, v);
             else LAUNCH((k_outgoing_sum<REAL, STO, false>), NT, stream, v);
             return CMF_OK;
@@ -556,14 +556,14 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
         const int64_t pb = (v.P * v.T + b_out - 1) / b_out;
         const int64_t cnt = (pb + (NT + b_out - 1) / b_out) * b_out;
         if (levee) {
-            if (v.LB == 5) LAUNCH_B((k_outflow_paths<REAL, STO, true, 5>), cnt, b_out, stream, v, substep, pb);
-            else LAUNCH_B((k_outflow_paths<REAL, STO, true, 0>), cnt, b_out, stream, v, substep, pb);
+            if (v.LB == 5) LAUNCH_B((k_outflow_paths<REAL, STO, true, 5>), cnt, b_out, stream, v, substep, pb, store_inflow);
+            else LAUNCH_B((k_outflow_paths<REAL, STO, true, 0>), cnt, b_out, stream, v, substep, pb, store_inflow);
         } else {
-            if (v.LB == 5) LAUNCH_B((k_outflow_paths<REAL, STO, false, 5>), cnt, b_out, stream, v, substep, pb);
-            else LAUNCH_B((k_outflow_paths<REAL, STO, false, 0>), cnt, b_out, stream, v, substep, pb);
+            if (v.LB == 5) LAUNCH_B((k_outflow_paths<REAL, STO, false, 5>), cnt, b_out, stream, v, substep, pb, store_inflow);
+            else LAUNCH_B((k_outflow_paths<REAL, STO, false, 0>), cnt, b_out, stream, v, substep, pb, store_inflow);
         }
     } else {
-        LAUNCH_B((k_outflow<REAL, STO, true>), NT, b_out, stream, v, substep);
+        LAUNCH_B((k_outflow<REAL, STO, true>), NT, b_out, stream, v, substep, store_inflow);
     }
     if (v.has(CMF_HAS_RESERVOIR))
         LAUNCH_B((k_reservoir<REAL, STO, true>), v.R * v.T, block_of(net, CMF_TUNE_RESERVOIR, false), stream, v, substep);

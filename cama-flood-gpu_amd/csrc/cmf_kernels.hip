@@ -64,9 +64,12 @@ __device__ __forceinline__ void store_if_changed(X* p, X old, X val) {
 // reference, and river/flood inflow and the bifurcation total are zeroed.
 // Both write the outgoing-storage terms (pos_term / neg_term) of each
 // catchment, for the passes that sum them (k_outgoing_sum, k_limiter).
-// `blk`: the block's index in the catchment grid.
+// FUSED writes the water surfaces and the total storage, which no later pass
+// of the fused schedule reads, on the catchment's last sub-step only
+// (`store_last`: that flag for one shared count).  `blk`: the block's index
+// in the catchment grid.
 template <class REAL, class STO, bool FUSED>
-__device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int substep, int64_t blk) {
+__device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int substep, int64_t blk, int store_last) {
     int64_t t, i;
     if (!v.catchment_in(blk, t, i)) return;
     if (!v.active(t, i, substep)) return;
@@ -181,12 +184,13 @@ __device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int subste
         v.flood_inflow[e] = (STO)0;
         if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = (STO)0;
     }
-    if (v.has(CMF_HAS_WATER_SURFACE)) v.wse[e] = wse;
-    if (v.has(CMF_HAS_PROTECTED_WATER_SURFACE)) v.pwse[e] = pwse;
+    const bool fin = !FUSED || v.last_substep(t, i, substep, store_last);
+    if (fin && v.has(CMF_HAS_WATER_SURFACE)) v.wse[e] = wse;
+    if (fin && v.has(CMF_HAS_PROTECTED_WATER_SURFACE)) v.pwse[e] = pwse;
     v.rcsd[e] = upd_rcsd;
     store_if_changed(v.fcsd + e, f_csd, upd_fcsd);
     store_if_changed(v.fcsa + e, f_csa, upd_fcsa);
-    if (v.has(CMF_HAS_TOTAL_STORAGE)) v.total_storage[e] = (STO)total_storage;
+    if (fin && v.has(CMF_HAS_TOTAL_STORAGE)) v.total_storage[e] = (STO)total_storage;
     const REAL pos = vmax(q_r, (REAL)0.0) + vmax(q_f, (REAL)0.0);
     const REAL neg = vmin(q_r, (REAL)0.0) + vmin(q_f, (REAL)0.0);
     v.pos_term[e] = pos * time_step;
@@ -194,8 +198,8 @@ __device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int subste
 }
 
 template <class REAL, class STO, bool FUSED>
-__global__ __launch_bounds__(kMaxBlock) void k_outflow(View<REAL, STO> v, int substep) {
-    outflow_one<REAL, STO, FUSED>(v, substep, xcd_block());
+__global__ __launch_bounds__(kMaxBlock) void k_outflow(View<REAL, STO> v, int substep, int store_last) {
+    outflow_one<REAL, STO, FUSED>(v, substep, xcd_block(), store_last);
 }
 
 // Outgoing total of catchment j from the catchment terms: upstream negative
@@ -420,10 +424,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_bif_outflow(View<REAL, STO> v, in
 // one launch.  Blocks [0, path_blocks) run the paths, first, so that their
 // longer threads overlap the catchment blocks; the rest run the catchments.
 template <class REAL, class STO, bool LEVEE, int LBFIX>
-__global__ __launch_bounds__(kMaxBlock) void k_outflow_paths(View<REAL, STO> v, int substep, int64_t path_blocks) {
+__global__ __launch_bounds__(kMaxBlock) void k_outflow_paths(View<REAL, STO> v, int substep, int64_t path_blocks,
+                                                         int store_last) {
     const int64_t b = blockIdx.x;
     if (b < path_blocks) bif_path<REAL, STO, LEVEE, LBFIX, true>(v, substep, b * blockDim.x + threadIdx.x);
-    else outflow_one<REAL, STO, true>(v, substep, b - path_blocks);
+    else outflow_one<REAL, STO, true>(v, substep, b - path_blocks, store_last);
 }
 
 // fused: bifurcation total of catchment j (compute_bifurcation_inflow,
@@ -1005,17 +1010,22 @@ __device__ __forceinline__ void block_log_partials(LogPart& lp, double* out_bloc
     }
 }
 
-// Store the post-stage state of one catchment (and its levee stage when LEV).
+// Store the post-stage state of one catchment.  `fin`: also the fields no
+// later pass of the sub-step loop reads for a catchment without levee (flood
+// fraction, protected depth), which the fused schedule writes on the
+// catchment's last sub-step only.
 template <class REAL, class STO>
 __device__ __forceinline__ void store_state(const View<REAL, STO>& v, int64_t e, const StageOut<REAL, STO>& so,
-                                            STO fs_old, STO ps_old) {
+                                            STO fs_old, STO ps_old, bool fin = true) {
     v.river_storage[e] = so.rsf_hp;
     store_if_changed(v.flood_storage + e, fs_old, so.fsf_hp);
     store_if_changed(v.protected_storage + e, ps_old, (STO)0.0);
     v.river_depth[e] = so.rdep;
     v.flood_depth[e] = so.fdep;
-    v.protected_depth[e] = so.fdep;
-    v.flood_fraction[e] = so.ffr;
+    if (fin) {
+        v.protected_depth[e] = so.fdep;
+        v.flood_fraction[e] = so.ffr;
+    }
 }
 
 // per-op compute_flood_stage(_log): storage update and flood stage from
@@ -1219,7 +1229,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_fused(View<REAL, STO> v, in
             if (events) gsum = bif_nets<REAL, STO>(v, t, j, l_own);
             gb = (REAL)gsum;
         }
-        if (v.last_substep(t, j, substep, store_inflow)) {
+        const bool last = v.last_substep(t, j, substep, store_inflow);
+        if (last) {
             v.river_inflow[e] = ri;
             v.flood_inflow[e] = fi;
             if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = gsum;
@@ -1242,7 +1253,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_fused(View<REAL, STO> v, in
             v.protected_depth[e] = lo.p_dph;
             v.flood_fraction[e] = lo.f_frc;
         } else {
-            store_state(v, e, so, pre.fs_old, pre.ps_old);
+            // a separate levee pass (k_levee) reads the flood fraction
+            store_state(v, e, so, pre.fs_old, pre.ps_old, last || (v.has(CMF_HAS_LEVEE) && !LEV));
         }
     }
     if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);

@@ -328,8 +328,13 @@ int cmf_hip_reservoir_outflow(cmf_network* net, const cmf_fields* f, void* strea
  * materialised (it is zero after every sub-step in the reference too).
  * `substep` is the sub-step index (log slot); with CMF_PER_TRIAL_DT a trial
  * whose count max_sub_steps[t] is <= substep is left unchanged.
- * `store_inflow` != 0 writes river_inflow / flood_inflow (the reference
- * leaves the last sub-step's inflow there). */
+ * `store_inflow` != 0 marks the last sub-step of the outer step: it writes
+ * river_inflow / flood_inflow (the reference leaves the last sub-step's
+ * inflow there) and the fields no later pass reads, which the reference
+ * rewrites every sub-step and which therefore hold the last sub-step's values
+ * after it: water_surface_elevation, protected_water_surface_elevation,
+ * total_storage, and flood_fraction / protected_depth of catchments without
+ * a levee (every sub-step when a separate levee pass reads them). */
 int cmf_hip_substep(cmf_network* net, const cmf_fields* f, int32_t substep,
                     int32_t store_inflow, void* stream);
 /* cmf_hip_substep with each kernel launch bracketed by HIP events on
