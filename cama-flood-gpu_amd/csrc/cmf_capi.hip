@@ -571,12 +571,12 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
     // path nets from the limiters of both path ends)
     if (net->G > 0) {
         // the group block table maps one catchment per thread
-        if (contig) LAUNCH_B((k_limiter<REAL, STO, true>), NT, b_lim, stream, v, substep);
-        else LAUNCH_B((k_limiter<REAL, STO, false>), NT, b_lim, stream, v, substep);
+        if (contig) LAUNCH_B((k_limiter<REAL, STO, true>), NT, b_lim, stream, v, substep, store_inflow);
+        else LAUNCH_B((k_limiter<REAL, STO, false>), NT, b_lim, stream, v, substep, store_inflow);
     } else {
         const int64_t threads = ((NT + 2 * b_lim - 1) / (2 * b_lim)) * b_lim;
-        if (contig) LAUNCH_B((k_limiter_pair<REAL, STO, true>), threads, b_lim, stream, v, substep);
-        else LAUNCH_B((k_limiter_pair<REAL, STO, false>), threads, b_lim, stream, v, substep);
+        if (contig) LAUNCH_B((k_limiter_pair<REAL, STO, true>), threads, b_lim, stream, v, substep, store_inflow);
+        else LAUNCH_B((k_limiter_pair<REAL, STO, false>), threads, b_lim, stream, v, substep, store_inflow);
     }
     int rc = log ? (contig ? launch_stage<REAL, STO, true, true>(net, v, NT, lev_fused, substep, store_inflow, stream)
                            : launch_stage<REAL, STO, false, true>(net, v, NT, lev_fused, substep, store_inflow, stream))

@@ -1057,11 +1057,12 @@ __global__ __launch_bounds__(kBlock) void k_stage(View<REAL, STO> v) {
     if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
 }
 
-// fused, single trial: storage limiter of every catchment from the terms of
-// the outflow pass (outgoing_storage of compute_outflow, then compute_inflow's
-// limiter, outflow.py:346-352).
+// fused: storage limiter of every catchment from the terms of the outflow
+// pass (outgoing_storage of compute_outflow, then compute_inflow's limiter,
+// outflow.py:346-352) into the limiter scratch; the limit_rate field, which no
+// later pass reads, on the catchment's last sub-step only (`store_last`).
 template <class REAL, class STO, bool CONTIG>
-__global__ __launch_bounds__(kMaxBlock) void k_limiter(View<REAL, STO> v, int substep) {
+__global__ __launch_bounds__(kMaxBlock) void k_limiter(View<REAL, STO> v, int substep, int store_last) {
     int64_t t, j;
     if (!v.catchment_of(xcd_block() * blockDim.x + threadIdx.x, t, j)) return;
     if (!v.active(t, j, substep)) return;
@@ -1071,7 +1072,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter(View<REAL, STO> v, int su
     if (ev.x >= 0) og = bif_terms_from<REAL, STO>(v, t, j, ev, og);
     const REAL l = limit_of(v, og, e);
     v.lim[e] = l;
-    if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e] = l;
+    if (v.has(CMF_HAS_BIFURCATION) && v.last_substep(t, j, substep, store_last)) v.limit_rate[e] = l;
 }
 
 // The same for ungrouped networks, two catchments per thread (g and g +
@@ -1079,7 +1080,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter(View<REAL, STO> v, int su
 // pass is bound by dependent round trips (upstream range, then its terms), so
 // two independent chains per thread keep twice the loads in flight.
 template <class REAL, class STO, bool CONTIG>
-__global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(View<REAL, STO> v, int substep) {
+__global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(View<REAL, STO> v, int substep, int store_last) {
     const int64_t NT = v.N * v.T;
     const int64_t g0 = (int64_t)blockIdx.x * (2 * (int64_t)blockDim.x) + threadIdx.x;
     const int64_t g1 = g0 + blockDim.x;
@@ -1141,11 +1142,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(View<REAL, STO> v, i
     const REAL l1 = (og1 > (REAL)1e-8) ? vmin(r1 / og1, (REAL)1.0) : (REAL)1.0;
     if (a0) {
         v.lim[e0] = l0;
-        if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e0] = l0;
+        if (v.has(CMF_HAS_BIFURCATION) && v.last_substep(t0, j0, substep, store_last)) v.limit_rate[e0] = l0;
     }
     if (a1) {
         v.lim[e1] = l1;
-        if (v.has(CMF_HAS_BIFURCATION)) v.limit_rate[e1] = l1;
+        if (v.has(CMF_HAS_BIFURCATION) && v.last_substep(t1, j1, substep, store_last)) v.limit_rate[e1] = l1;
     }
 }
 

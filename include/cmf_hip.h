@@ -333,8 +333,9 @@ int cmf_hip_reservoir_outflow(cmf_network* net, const cmf_fields* f, void* strea
  * inflow there) and the fields no later pass reads, which the reference
  * rewrites every sub-step and which therefore hold the last sub-step's values
  * after it: water_surface_elevation, protected_water_surface_elevation,
- * total_storage, and flood_fraction / protected_depth of catchments without
- * a levee (every sub-step when a separate levee pass reads them). */
+ * total_storage, limit_rate, and flood_fraction / protected_depth of
+ * catchments without a levee (every sub-step when a separate levee pass
+ * reads them). */
 int cmf_hip_substep(cmf_network* net, const cmf_fields* f, int32_t substep,
                     int32_t store_inflow, void* stream);
 /* cmf_hip_substep with each kernel launch bracketed by HIP events on
