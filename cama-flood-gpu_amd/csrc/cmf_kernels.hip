@@ -93,13 +93,17 @@ __device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int subste
     const REAL dn_dist = v.prm(v.dist, CMF_BATCHED_DOWNSTREAM_DISTANCE, e, i);
     const REAL rs = (REAL)v.river_storage[e];
     const REAL fs = (REAL)v.flood_storage[e];
-    const REAL ps = (REAL)v.protected_storage[e];
+    // A stage pass folds protected storage into the total and leaves +0.0 in
+    // every catchment without a levee, so after the first sub-step of an
+    // outer step only levee catchments hold any.
+    const bool lev_i = v.has(CMF_HAS_LEVEE) && v.is_levee[i];
+    const REAL ps = (FUSED && substep > 0 && !lev_i) ? (REAL)0.0 : (REAL)v.protected_storage[e];
     const REAL r_csd = v.rcsd[e], f_csd = v.fcsd[e], f_csa = v.fcsa[e];
 
     const REAL river_elevation = c_elv - r_hgt;
     const REAL wse = r_dep + river_elevation;
     REAL pwse = wse;
-    if (v.has(CMF_HAS_LEVEE) && v.is_levee[i]) pwse = vmin(c_elv + v.protected_depth[e], wse);
+    if (lev_i) pwse = vmin(c_elv + v.protected_depth[e], wse);
     const REAL total_storage = rs + fs + ps;
     const REAL r_dep_dn = v.river_depth[ed];
     const REAL r_hgt_dn = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, ed, d);
@@ -1176,7 +1180,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_fused(View<REAL, STO> v, in
         const REAL l_own = v.lim[e];
         const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
         const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
-        const STO rs = v.river_storage[e], fs = v.flood_storage[e], ps = v.protected_storage[e];
+        const STO rs = v.river_storage[e], fs = v.flood_storage[e];
+        // without the levee module every catchment holds +0.0 protected
+        // storage after the first sub-step's stage pass
+        const STO ps = (!v.has(CMF_HAS_LEVEE) && substep > 0) ? (STO)0.0 : v.protected_storage[e];
         const REAL ro = v.prm(v.runoff, CMF_BATCHED_RUNOFF, e, j);
         const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, j);
         const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, j);

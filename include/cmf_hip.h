@@ -327,7 +327,10 @@ int cmf_hip_reservoir_outflow(cmf_network* net, const cmf_fields* f, void* strea
  * inflow + flood stage [+ levee stage] [+ log].  outgoing_storage is never
  * materialised (it is zero after every sub-step in the reference too).
  * `substep` is the sub-step index (log slot); with CMF_PER_TRIAL_DT a trial
- * whose count max_sub_steps[t] is <= substep is left unchanged.
+ * whose count max_sub_steps[t] is <= substep is left unchanged.  Sub-steps
+ * after the first of an outer step follow a stage pass, which leaves zero
+ * protected storage in every catchment without a levee: they read
+ * protected_storage only for levee catchments.
  * `store_inflow` != 0 marks the last sub-step of the outer step: it writes
  * river_inflow / flood_inflow (the reference leaves the last sub-step's
  * inflow there) and the fields no later pass reads, which the reference
