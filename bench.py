@@ -165,7 +165,15 @@ def basins_line(args, device: str) -> dict:
         torch.cuda.synchronize()
         el = time.perf_counter() - t1
         key = "per_basin_dt" if grouped else "shared_dt"
-        out[key] = {"outer_steps_per_s": steps / el, "catchment_steps_per_s": work / el}
+        # whole-run HBM rate: the fused passes' algorithmic bytes per
+        # catchment-sub-step times the measured rate (wall clock, CFL pass and
+        # launch gaps included)
+        f_wet = float((m.t["flood_depth"] > 0).float().mean().item())
+        per = sum(algorithmic_bytes(k, m, f_wet) for k in ("k_outflow", "k_limiter_pair", "k_stage_fused")) / n
+        gbs = work / el * per / 1e9
+        out[key] = {"outer_steps_per_s": steps / el, "catchment_steps_per_s": work / el,
+                    "bytes_per_catchment_step": round(per, 1), "GBps": round(gbs, 1),
+                    "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4), "f_wet": round(f_wet, 4)}
         m.close()
         del m
         torch.cuda.empty_cache()
@@ -173,7 +181,8 @@ def basins_line(args, device: str) -> dict:
     out["unit"] = "catchment-steps/s"
     out["speedup_vs_shared_dt"] = round(out["per_basin_dt"]["outer_steps_per_s"] / out["shared_dt"]["outer_steps_per_s"], 3)
     log(f"basins: per-basin dt {out['per_basin_dt']['outer_steps_per_s']:.1f} outer steps/s, shared dt "
-        f"{out['shared_dt']['outer_steps_per_s']:.1f} ({out['speedup_vs_shared_dt']}x)")
+        f"{out['shared_dt']['outer_steps_per_s']:.1f} ({out['speedup_vs_shared_dt']}x); HBM "
+        f"{out['per_basin_dt']['GBps']:.0f} / {out['shared_dt']['GBps']:.0f} GB/s algorithmic")
     return out
 
 
