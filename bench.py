@@ -236,10 +236,12 @@ def adjoint_line(args, device: str) -> dict:
     gauges = net.gauges
     steps = args.adjoint_steps
     problem = GaugeProblem(model, gauges, np.ones((steps, gauges.shape[0])), checkpoint_every=48)
-    # warm-up: one short forward and backward (first use of every kernel)
-    warm = GaugeProblem(model, gauges, np.ones((2, gauges.shape[0])), checkpoint_every=48)
-    warm.run()
-    warm.gradient()
+    # warm-up: the same forward and backward once (first use of every kernel,
+    # and the tape of a full checkpoint segment, tens of GB at this size, is
+    # allocated once by the caching allocator, as in a calibration loop that
+    # takes many gradients); then the timed pass from the same initial state
+    problem.run()
+    problem.gradient()
     problem._restore(problem.initial)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
