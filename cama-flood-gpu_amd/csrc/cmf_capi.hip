@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -67,6 +68,14 @@ struct cmf_network {
     int32_t* blk_end = nullptr;
     int32_t* blk_group = nullptr;
     int32_t* gblk_ptr = nullptr;      // first block of each group, G + 1 entries
+    // adaptive grouped advance: the block table entries ordered by their
+    // group's sub-step count (descending), and per sub-step the number of
+    // leading entries whose group is still sub-stepping
+    std::vector<int32_t> gblk_host;
+    int32_t* blk_order = nullptr;     // device
+    int32_t* host_order = nullptr;    // pinned staging of blk_order
+    hipEvent_t order_copied = nullptr;
+    std::vector<int64_t> sub_blocks;
     // captured sub-step loops (cmf_hip_advance_graph): one executable graph
     // per sub-step count, valid for the field table they were captured with
     hipStream_t capture_stream = nullptr;
@@ -87,10 +96,12 @@ void free_network(cmf_network* net) {
     void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->ev_other, net->ev1, net->pos_term,
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
                     net->path_neg, net->path_net, net->path_raw, net->partials, net->cgroup,
-                    net->blk_first, net->blk_end, net->blk_group, net->gblk_ptr};
+                    net->blk_first, net->blk_end, net->blk_group, net->gblk_ptr, net->blk_order};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (net->host_counts) (void)hipHostFree(net->host_counts);
+    if (net->host_order) (void)hipHostFree(net->host_order);
+    if (net->order_copied) (void)hipEventDestroy(net->order_copied);
     for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
     if (net->capture_stream) (void)hipStreamDestroy(net->capture_stream);
     if (net->adj_scratch) (void)hipFree(net->adj_scratch);
@@ -225,6 +236,7 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.cgroup = net->cgroup;
     v.blk_first = net->blk_first;
     v.blk_end = net->blk_end;
+    v.blk_order = nullptr;
     v.up_ptr = net->up_ptr;
     v.up_idx = net->contiguous ? nullptr : net->up_idx;
     v.ev_ptr = net->ev_ptr;
@@ -544,7 +556,8 @@ template <class REAL, class STO>
 int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int substep,
                   int store_inflow, void* stream) {
     // catchment kernels: N*T threads, or one block per group table entry
-    const int64_t NT = net->G > 0 ? net->nblk * kBlock : v.N * v.T;
+    const bool ordered = net->G > 0 && v.blk_order && substep >= 0 && substep < (int)net->sub_blocks.size();
+    const int64_t NT = net->G > 0 ? (ordered ? net->sub_blocks[substep] : net->nblk) * kBlock : v.N * v.T;
     const bool contig = net->contiguous != 0;
     const bool bif = v.has(CMF_HAS_BIFURCATION), levee = v.has(CMF_HAS_LEVEE);
     const bool log = v.has(CMF_HAS_LOG);
@@ -753,6 +766,34 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
         hipLaunchKernelGGL((k_set_dt_fixed<REAL>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
                            static_cast<REAL*>(f->time_step), nslots, outer / (double)fixed_count);
         CMF_HIP_CHECK(hipGetLastError());
+    }
+    if (net->G > 0 && v.has(CMF_HAS_ADAPTIVE_TIME) && !use_graph) {
+        // Order the block table by group sub-step count, descending, so that
+        // sub-step k launches only the leading blocks of the groups with
+        // count > k.  The staging buffer is rewritten only after the previous
+        // copy from it has completed.
+        const int64_t G = net->G;
+        std::vector<int32_t> gs((size_t)G);
+        std::iota(gs.begin(), gs.end(), 0);
+        const int32_t* hc = net->host_counts;
+        std::stable_sort(gs.begin(), gs.end(), [hc](int32_t a, int32_t b) { return hc[a] > hc[b]; });
+        CMF_HIP_CHECK(hipEventSynchronize(net->order_copied));
+        std::vector<int64_t> lead((size_t)G + 1, 0);
+        int64_t nb = 0;
+        for (int64_t q = 0; q < G; ++q) {
+            const int32_t g = gs[q];
+            for (int32_t b = net->gblk_host[g]; b < net->gblk_host[g + 1]; ++b) net->host_order[nb++] = b;
+            lead[q + 1] = nb;
+        }
+        net->sub_blocks.assign((size_t)count, 0);
+        int64_t active = G;
+        for (int k = 0; k < count; ++k) {
+            while (active > 0 && hc[gs[active - 1]] <= k) --active;
+            net->sub_blocks[k] = lead[active];
+        }
+        CMF_HIP_CHECK(hipMemcpyAsync(net->blk_order, net->host_order, (size_t)nb * 4, hipMemcpyHostToDevice, s));
+        CMF_HIP_CHECK(hipEventRecord(net->order_copied, s));
+        v.blk_order = net->blk_order;
     }
     rc = use_graph ? substep_loop_graph(net, v, f, count, s) : substep_loop(net, v, f, count, s);
     if (rc) return rc;
@@ -1032,6 +1073,15 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     if (e != hipSuccess) {
         free_network(net);
         return set_error(CMF_ERR_NOMEM, "hipHostMalloc failed: %s", hipGetErrorString(e));
+    }
+    if (num_groups > 0) {
+        net->gblk_host = gblk_ptr;
+        if ((rc = dev_alloc((void**)&net->blk_order, (size_t)net->nblk * 4)) ||
+            (e = hipHostMalloc((void**)&net->host_order, (size_t)net->nblk * 4, hipHostMallocDefault)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&net->order_copied, hipEventDisableTiming)) != hipSuccess) {
+            free_network(net);
+            return rc ? rc : set_error(CMF_ERR_NOMEM, "block order buffers: %s", hipGetErrorString(e));
+        }
     }
     *out = net;
     return CMF_OK;

@@ -160,6 +160,9 @@ struct View {
     // [blk_first[b], blk_end[b]) of a single group.
     int64_t G;
     const int32_t *cgroup, *blk_first, *blk_end;
+    // grouped: grid block -> block table entry, the blocks of the groups
+    // still sub-stepping first (nullptr: the same index)
+    const int32_t* blk_order;
     // taping (cmf_hip_substep_taped): the outflow pass also writes the
     // sub-step's input state here (nullptr: no tape)
     REAL *tape_q_r, *tape_q_f, *tape_rcsd, *tape_fcsd, *tape_fcsa, *tape_rd, *tape_fd;
@@ -203,14 +206,22 @@ struct View {
     __device__ __forceinline__ void split_c(int64_t g, int64_t& t, int64_t& i) const {
         split(g, N, ts == 1, t, i);
     }
+    __device__ __forceinline__ int64_t table_block(int64_t blk) const {
+        return blk_order ? (int64_t)blk_order[blk] : blk;
+    }
+    // log partial slot of this block (its block table entry when grouped)
+    __device__ __forceinline__ int64_t partial_slot() const {
+        return G > 0 ? table_block(blockIdx.x) : (int64_t)blockIdx.x;
+    }
     // catchment of this thread in a catchment kernel (linear index `g` of the
     // plain grid, or the group block table); false past the end
     __device__ __forceinline__ bool catchment_of(int64_t g, int64_t& t, int64_t& i) const {
         if (G > 0) {
-            const int64_t j = (int64_t)blk_first[blockIdx.x] + threadIdx.x;
+            const int64_t b = table_block(blockIdx.x);
+            const int64_t j = (int64_t)blk_first[b] + threadIdx.x;
             t = 0;
             i = j;
-            return j < blk_end[blockIdx.x];
+            return j < blk_end[b];
         }
         if (g >= N * T) return false;
         split_c(g, t, i);
@@ -219,10 +230,11 @@ struct View {
     // the same for block `blk` of the catchment grid
     __device__ __forceinline__ bool catchment_in(int64_t blk, int64_t& t, int64_t& i) const {
         if (G > 0) {
-            const int64_t j = (int64_t)blk_first[blk] + threadIdx.x;
+            const int64_t b = table_block(blk);
+            const int64_t j = (int64_t)blk_first[b] + threadIdx.x;
             t = 0;
             i = j;
-            return j < blk_end[blk];
+            return j < blk_end[b];
         }
         return catchment_of(blk * blockDim.x + threadIdx.x, t, i);
     }

@@ -1265,7 +1265,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_fused(View<REAL, STO> v, in
             store_state(v, e, so, pre.fs_old, pre.ps_old, last || (v.has(CMF_HAS_LEVEE) && !LEV));
         }
     }
-    if (LOG) block_log_partials(lp, v.partials + (int64_t)blockIdx.x * CMF_LOG_COUNT);
+    if (LOG) block_log_partials(lp, v.partials + v.partial_slot() * CMF_LOG_COUNT);
 }
 
 // Fold the per-block partials into log slot current_step: one block, each
