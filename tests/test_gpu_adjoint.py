@@ -2,15 +2,21 @@
 FP64 torch autograd through a plain PyTorch-op transcription of the reference
 step (oracle/torch_step.py) and against central finite differences.
 
-Problem: plain network of 3,000 catchments, 16 gauges (largest upstream
-area), loss = mean squared log-discharge error over 8 outer steps against a
-run whose Manning's n and river height were scaled by lognormal(0, 0.2)
-factors; checkpoints every 3 outer steps (so the backward re-runs segments).
+Problem: plain network (3,000 catchments; the autograd check on a
+20,000-catchment network), 16 gauges (largest upstream area), loss = mean
+squared log-discharge error over 8 outer steps against a run whose Manning's
+n and river height were scaled by lognormal(0, 0.2) factors; checkpoints
+every 3 outer steps (so the backward re-runs segments).
 
 Tolerances: FP64 adjoint vs FP64 autograd, relative L2 of each gradient 1e-6
 (the two sum floats in different orders); central differences (h = 1e-3
-relative, sub-step counts held) relative 2e-2 per component; the FP32 adjoint
-against the FP64 one, relative L2 5e-2.
+relative, sub-step counts held) on 64 random parameters (32 n, 32 H, among
+those whose gradient exceeds 1e-4 of the largest): relative 2e-2 for every
+component where the loss is smooth inside +-h (the forward and backward
+one-sided quotients agree within 10 %; where a branch of the step, such as a
+wet/dry or floodplain-level switch, flips inside +-h the quotient is no
+derivative), at most 8 of the 64 at such kinks, median relative error 1e-3
+over all 64; the FP32 adjoint against the FP64 one, relative L2 5e-2.
 """
 import numpy as np
 import pytest
@@ -22,11 +28,11 @@ PLAIN = ("base", "adaptive_time")
 STEPS = 8
 
 
-def _setup(precision):
+def _setup(precision, n=3000):
     from cmfgpu_amd import synthetic
     from cmfgpu_amd.adjoint import GaugeProblem
     from cmfgpu_amd.models import CaMaFlood
-    net = synthetic.generate_network(3000, 1003)
+    net = synthetic.generate_network(n, 1003)
     src = dict(net.fields)
     src["runoff"] = synthetic.Runoff(src["catchment_area"], 2003).next_day() * np.float32(4.0)
     gauges = np.argsort(net.upstream_area)[-16:]
@@ -82,7 +88,7 @@ def _rel(a, b):
 
 
 def test_adjoint_matches_fp64_autograd():
-    net, src, model, problem, gauges = _setup("float64")
+    net, src, model, problem, gauges = _setup("float64", n=20000)
     loss = problem.run()
     assert loss > 0 and len(problem.checkpoints) == 3
     g_n, g_h = (x.cpu().numpy() for x in problem.gradient())
@@ -103,10 +109,13 @@ def test_adjoint_against_central_differences():
     g_n, g_h = (x.cpu().numpy() for x in problem.gradient())
     n0, h0 = (x.astype(np.float64) for x in problem.parameters())
     rng = np.random.default_rng(11)
-    cand_n = np.flatnonzero(np.abs(g_n) > 1e-3 * np.abs(g_n).max())
-    cand_h = np.flatnonzero(np.abs(g_h) > 1e-3 * np.abs(g_h).max())
-    checks = [("n", i) for i in rng.choice(cand_n, 8, replace=False)] + \
-             [("h", i) for i in rng.choice(cand_h, 8, replace=False)]
+    cand_n = np.flatnonzero(np.abs(g_n) > 1e-4 * np.abs(g_n).max())
+    cand_h = np.flatnonzero(np.abs(g_h) > 1e-4 * np.abs(g_h).max())
+    checks = [("n", i) for i in rng.choice(cand_n, min(32, cand_n.size), replace=False)] + \
+             [("h", i) for i in rng.choice(cand_h, min(32, cand_h.size), replace=False)]
+    assert len(checks) == 64, (cand_n.size, cand_h.size)
+    loss0 = problem.run(fixed_counts=counts)
+    rel, kink = [], []
     for kind, i in checks:
         base = n0 if kind == "n" else h0
         step = 1e-3 * base[i]
@@ -122,9 +131,19 @@ def test_adjoint_against_central_differences():
         problem.set_parameters(n0, h0)
         fd = (vals[0] - vals[1]) / (2.0 * step)
         ad = (g_n if kind == "n" else g_h)[i]
-        print(f"central difference {kind}[{i}]: fd {fd:.6e} adjoint {ad:.6e} rel {abs(fd - ad) / abs(ad):.2e}",
-              flush=True)
-        assert abs(fd - ad) <= 2e-2 * abs(ad), (kind, i, fd, ad)
+        # one-sided quotients: disagreeing by more than 10 % marks a kink of
+        # the loss inside +-h (a branch of the step switching there)
+        fwd, bwd = (vals[0] - loss0) / step, (loss0 - vals[1]) / step
+        kink.append(abs(fwd - bwd) > 0.1 * max(abs(fwd), abs(bwd)))
+        rel.append(abs(fd - ad) / abs(ad))
+        print(f"central difference {kind}[{i}]: fd {fd:.6e} adjoint {ad:.6e} rel {rel[-1]:.2e}"
+              f"{' (kink inside +-h)' if kink[-1] else ''}", flush=True)
+    rel, kink = np.array(rel), np.array(kink)
+    print(f"central differences: median rel {np.median(rel):.2e}, max over smooth points "
+          f"{rel[~kink].max():.2e}, {int(kink.sum())} of {rel.size} at kinks", flush=True)
+    assert np.all(rel[~kink] <= 2e-2), rel[~kink]
+    assert int(kink.sum()) <= 8, int(kink.sum())
+    assert np.median(rel) <= 1e-3, np.median(rel)
     model.close()
 
 
