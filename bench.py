@@ -186,6 +186,107 @@ def basins_line(args, device: str) -> dict:
     return out
 
 
+def batched_full_bytes(N: int, M: int, P: int, f_wet: float) -> float:
+    """Algorithmic bytes of one fused sub-step of M trial-inner members with
+    full physics, per-member data counted once per member and the shared
+    parameters once per catchment (a sub-step after the first: the write-only
+    fields are not written):
+      per member-catchment 184 B: outflow reads flows, depths, river and
+      flood storage, three cross-section fields (44 B) and writes two flows,
+      three cross-section fields and two terms (28 B); the limiter reads two
+      terms and two storages and writes the limiter (28 B); the stage reads
+      own and upstream limiter and flows (24 B), three storages (24 B) and
+      runoff (4 B) and writes two flows, two storages and two depths (32 B);
+      per catchment 69 B + 44 f_wet: eight channel fields and the levee flag
+      (33 B), upstream ranges and event records (16 B), channel geometry,
+      levee row and event range (20 B), the table row and area where wet;
+      per path-member 112 B: five levels of flow and depth (60 B), three path
+      terms (12 B), the nets' rescale of the five flows (40 B);
+      per path 72 B: five levels of Manning's n, width and elevation, length
+      and the two ends."""
+    return N * M * 184.0 + N * (69.0 + 44.0 * f_wet) + P * M * 112.0 + P * 72.0
+
+
+def batched_full_line(args, device: str) -> dict:
+    """Batched full physics: the 3-arcmin-scale network (6,000,000
+    catchments, 150,000 bifurcation paths, levees on 5 %, 7,000 reservoirs)
+    with `--full-members` runoff members (seeds 4000 + m), each with its own
+    CFL count, trial-inner.  Member-catchment-steps/s over timed outer steps,
+    the whole-run algorithmic HBM rate (batched_full_bytes), and after
+    `--full-days` simulated days the relative L2 error of each member's
+    discharge, storages and flood depth against the FP64 profile of the same
+    batch (bit-identical to the FP64 oracle)."""
+    import numpy as np
+    import torch
+    from cmfgpu_amd import synthetic
+    from cmfgpu_amd.models import CaMaFlood
+    t0 = time.perf_counter()
+    net = synthetic.build_config(4, scale=args.full_scale)
+    src = dict(net.fields)
+    M = args.full_members
+    forcing = [synthetic.Runoff(src["catchment_area"], 4000 + m) for m in range(M)]
+    runoff = np.stack([f.next_day() for f in forcing])
+    N, P = net.num_catchments, net.num_paths
+    log(f"batched full: {N} catchments, {P} paths, {net.num_levees} levees, {net.num_reservoirs} reservoirs, "
+        f"{M} members ({time.perf_counter() - t0:.1f} s)")
+    mods = ("base", "adaptive_time", "bifurcation", "levee", "reservoir")
+
+    def build(precision):
+        return CaMaFlood(src, opened_modules=mods, device=device, step_duration=1800.0, num_trials=M,
+                         batched={"runoff": runoff}, per_trial_time_step=True, trial_layout="inner",
+                         precision=precision)
+
+    # error run: FP32 and FP64 profiles from the same state, one day each
+    models = [build(p) for p in ("float32", "float64")]
+    for _ in range(args.full_days):
+        for _ in range(48):
+            for m in models:
+                m.step_advance()
+        if args.full_days > 1:
+            r = np.stack([f.next_day() for f in forcing])
+            for m in models:
+                m.set_inputs(r)
+    errors = {}
+    for name in ("river_storage", "flood_storage", "flood_depth"):
+        x, y = (m.get(name).astype(np.float64) for m in models)
+        errors[name] = [float(f"{np.linalg.norm(x[t] - y[t]) / max(np.linalg.norm(y[t]), 1e-300):.4g}")
+                        for t in range(M)]
+    q32, q64 = ((m.get("river_outflow").astype(np.float64) + m.get("flood_outflow")) for m in models)
+    errors["discharge"] = [float(f"{np.linalg.norm(q32[t] - q64[t]) / np.linalg.norm(q64[t]):.4g}")
+                           for t in range(M)]
+    # the same without the reservoir catchments, whose release switches
+    # regime at storage and inflow thresholds (a flip there is not rounding)
+    keep = np.ones(N, bool)
+    keep[np.asarray(src["reservoir_catchment_idx"], np.int64)] = False
+    errors["discharge_without_reservoirs"] = [
+        float(f"{np.linalg.norm((q32[t] - q64[t])[keep]) / np.linalg.norm(q64[t][keep]):.4g}") for t in range(M)]
+    del q32, q64
+    models[1].close()
+    # throughput: continue the FP32 model
+    bm = models[0]
+    run_steps(bm, 2, per_trial=True)
+    steps = args.full_steps
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    work = run_steps(bm, steps, per_trial=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    subs = work / (N * M)                       # mean sub-steps per member over the timed steps
+    f_wet = float((bm.t["flood_depth"] > 0).float().mean().item())
+    gbs = batched_full_bytes(N, M, P, f_wet) * subs / el / 1e9
+    bm.close()
+    out = {"workload": f"{N} catchments, {P} paths, {net.num_levees} levees, {net.num_reservoirs} reservoirs, "
+                       f"{M} members with their own CFL counts, trial-inner",
+           "members": M, "value": work / el, "unit": "member-catchment-steps/s", "steps": steps,
+           "GBps_algorithmic": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4), "f_wet": round(f_wet, 4),
+           "error_days": args.full_days, "rel_l2_per_member": errors,
+           "max_discharge_rel_l2": max(errors["discharge"]),
+           "max_discharge_rel_l2_without_reservoirs": max(errors["discharge_without_reservoirs"])}
+    log(f"batched full: {work / el:.4g} member-catchment-steps/s, {gbs:.0f} GB/s algorithmic; discharge error "
+        f"after {args.full_days} day(s) {min(errors['discharge']):.3g}..{max(errors['discharge']):.3g} over members")
+    return out
+
+
 def graph_line(args, device: str) -> dict:
     """Launch-bound case: the 60,000-catchment plain network, sub-step loops
     launched eagerly vs replayed from captured HIP graphs."""
@@ -377,6 +478,11 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--basins", type=int, default=400, help="uneven basins of the grouped line (0: skip)")
     ap.add_argument("--basins-scale", type=float, default=1.0, help="basin size factor (1.0: about 20M catchments)")
+    ap.add_argument("--full-members", type=int, default=16,
+                    help="members of the batched full-physics line on 6M catchments (0: skip)")
+    ap.add_argument("--full-scale", type=float, default=1.0, help="size factor of that network")
+    ap.add_argument("--full-steps", type=int, default=4, help="timed outer steps of that line")
+    ap.add_argument("--full-days", type=int, default=1, help="simulated days of its FP32-vs-FP64 error run")
     ap.add_argument("--autotune", type=int, default=1, help="pick kernel block sizes by timing first (0: 256)")
     ap.add_argument("--adjoint-steps", type=int, default=48, help="outer steps of the adjoint line (0: skip)")
     ap.add_argument("--graph", type=int, default=1, help="eager vs HIP-graph line on the small network (0: skip)")
@@ -482,6 +588,7 @@ def main():
     if args.basins > 0:
         basins = basins_line(args, device)
     graph = graph_line(args, device) if args.graph else None
+    batched_full = batched_full_line(args, device) if args.full_members > 0 else None
     adjoint = adjoint_line(args, device) if args.adjoint_steps > 0 else None
 
     errors = None
@@ -498,7 +605,7 @@ def main():
                    "levees": net.num_levees, "flood_levels": 10, "outer_step_s": 1800.0,
                    "substeps_timed": int(substeps), "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "basins": basins, "graph": graph,
-        "adjoint": adjoint, "forcing": forcing,
+        "adjoint": adjoint, "forcing": forcing, "batched_full": batched_full,
         "block_sizes": {k: v["block"] for k, v in tuned.items()} if tuned else None,
         "error_vs_fp64": errors,
     }
