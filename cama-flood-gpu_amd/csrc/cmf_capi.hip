@@ -459,7 +459,7 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
             if ((f->flags & CMF_HAS_RESERVOIR) && !f->is_dam_related)
                 return set_error(CMF_ERR_INVALID, "compute_adaptive_time_step: is_dam_related is NULL");
             if (v.N == 0) return CMF_OK;
-            const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 2048);
+            const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 512);
             hipLaunchKernelGGL((k_cfl<REAL, STO>), dim3((unsigned)chunks, (unsigned)v.T), dim3(kBlock), 0,
                                (hipStream_t)stream, v);
             hipError_t e = hipGetLastError();
@@ -743,7 +743,7 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
             hipLaunchKernelGGL((k_cfl_grouped<REAL, STO>), dim3((unsigned)net->nblk), dim3(kBlock), 0, s, v,
                                net->blk_group);
         } else if (v.N > 0) {
-            const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 2048);
+            const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 512);
             hipLaunchKernelGGL((k_cfl<REAL, STO>), dim3((unsigned)chunks, (unsigned)v.T), dim3(kBlock), 0, s, v);
         }
         hipLaunchKernelGGL((k_set_dt<REAL>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,

@@ -1376,16 +1376,23 @@ __device__ __forceinline__ int block_max_int(int best) {
     return m;
 }
 
-// grid (chunks, trials-or-1): block (bx, t) reduces catchments of trial t.
-// Without per-trial dt every trial folds into slot 0.
+// grid (chunks, trials-or-1): block (bx, t) reduces catchments of trial t,
+// four independent grid strides per iteration (their loads in flight
+// together); one atomic per block, so few blocks.  Without per-trial dt every
+// trial folds into slot 0.
 template <class REAL, class STO>
 __global__ __launch_bounds__(kBlock) void k_cfl(View<REAL, STO> v) {
     const int64_t t = blockIdx.y;
     const REAL outer = v.outer_time_step[0];
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
     int best = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.N;
-         i += (int64_t)gridDim.x * kBlock)
-        best = max(best, cfl_steps(v, t, i, outer));
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.N; i += 4 * stride) {
+        const int c0 = cfl_steps(v, t, i, outer);
+        const int c1 = (i + stride < v.N) ? cfl_steps(v, t, i + stride, outer) : 0;
+        const int c2 = (i + 2 * stride < v.N) ? cfl_steps(v, t, i + 2 * stride, outer) : 0;
+        const int c3 = (i + 3 * stride < v.N) ? cfl_steps(v, t, i + 3 * stride, outer) : 0;
+        best = max(best, max(max(c0, c1), max(c2, c3)));
+    }
     const int m = block_max_int(best);
     if (threadIdx.x == 0) atomicMax(&v.max_sub_steps[v.has(CMF_PER_TRIAL_DT) ? t : 0], m);
 }
