@@ -64,16 +64,16 @@ struct cmf_network {
     int64_t G = 0;
     int64_t nblk = 0;
     int32_t* cgroup = nullptr;
-    int32_t* blk_first = nullptr;
-    int32_t* blk_end = nullptr;
-    int32_t* blk_group = nullptr;
+    // block records {first catchment, end, table entry, group}, in table
+    // order (rec_id) and, for an adaptive advance, ordered by group count
+    int4* blk_rec_id = nullptr;
+    int4* blk_rec_ord = nullptr;
+    std::vector<int4> rec_host;
     int32_t* gblk_ptr = nullptr;      // first block of each group, G + 1 entries
-    // adaptive grouped advance: the block table entries ordered by their
-    // group's sub-step count (descending), and per sub-step the number of
-    // leading entries whose group is still sub-stepping
+    // adaptive grouped advance: per sub-step the number of leading ordered
+    // records whose group is still sub-stepping
     std::vector<int32_t> gblk_host;
-    int32_t* blk_order = nullptr;     // device
-    int32_t* host_order = nullptr;    // pinned staging of blk_order
+    int4* host_rec = nullptr;         // pinned staging of blk_rec_ord
     hipEvent_t order_copied = nullptr;
     std::vector<int64_t> sub_blocks;
     // captured sub-step loops (cmf_hip_advance_graph): one executable graph
@@ -96,11 +96,11 @@ void free_network(cmf_network* net) {
     void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->ev_other, net->ev1, net->pos_term,
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
                     net->path_neg, net->path_net, net->path_raw, net->partials, net->cgroup,
-                    net->blk_first, net->blk_end, net->blk_group, net->gblk_ptr, net->blk_order};
+                    net->blk_rec_id, net->blk_rec_ord, net->gblk_ptr};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (net->host_counts) (void)hipHostFree(net->host_counts);
-    if (net->host_order) (void)hipHostFree(net->host_order);
+    if (net->host_rec) (void)hipHostFree(net->host_rec);
     if (net->order_copied) (void)hipEventDestroy(net->order_copied);
     for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
     if (net->capture_stream) (void)hipStreamDestroy(net->capture_stream);
@@ -234,9 +234,8 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.acc_outflow = net->acc_outflow;
     v.G = net->G;
     v.cgroup = net->cgroup;
-    v.blk_first = net->blk_first;
-    v.blk_end = net->blk_end;
-    v.blk_order = nullptr;
+    v.blk_rec = net->blk_rec_id;
+    v.ordered = false;
     v.up_ptr = net->up_ptr;
     v.up_idx = net->contiguous ? nullptr : net->up_idx;
     v.ev_ptr = net->ev_ptr;
@@ -556,7 +555,7 @@ template <class REAL, class STO>
 int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int substep,
                   int store_inflow, void* stream) {
     // catchment kernels: N*T threads, or one block per group table entry
-    const bool ordered = net->G > 0 && v.blk_order && substep >= 0 && substep < (int)net->sub_blocks.size();
+    const bool ordered = net->G > 0 && v.ordered && substep >= 0 && substep < (int)net->sub_blocks.size();
     const int64_t NT = net->G > 0 ? (ordered ? net->sub_blocks[substep] : net->nblk) * kBlock : v.N * v.T;
     const bool contig = net->contiguous != 0;
     const bool bif = v.has(CMF_HAS_BIFURCATION), levee = v.has(CMF_HAS_LEVEE);
@@ -740,8 +739,7 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
         hipLaunchKernelGGL((k_prepare<REAL>), dim3(1), dim3(64), 0, s, static_cast<REAL*>(f->outer_time_step), outer);
         hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, f->max_sub_steps, nslots);
         if (net->G > 0) {
-            hipLaunchKernelGGL((k_cfl_grouped<REAL, STO>), dim3((unsigned)net->nblk), dim3(kBlock), 0, s, v,
-                               net->blk_group);
+            hipLaunchKernelGGL((k_cfl_grouped<REAL, STO>), dim3((unsigned)net->nblk), dim3(kBlock), 0, s, v);
         } else if (v.N > 0) {
             const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 512);
             hipLaunchKernelGGL((k_cfl<REAL, STO>), dim3((unsigned)chunks, (unsigned)v.T), dim3(kBlock), 0, s, v);
@@ -782,7 +780,7 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
         int64_t nb = 0;
         for (int64_t q = 0; q < G; ++q) {
             const int32_t g = gs[q];
-            for (int32_t b = net->gblk_host[g]; b < net->gblk_host[g + 1]; ++b) net->host_order[nb++] = b;
+            for (int32_t b = net->gblk_host[g]; b < net->gblk_host[g + 1]; ++b) net->host_rec[nb++] = net->rec_host[b];
             lead[q + 1] = nb;
         }
         net->sub_blocks.assign((size_t)count, 0);
@@ -791,9 +789,11 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
             while (active > 0 && hc[gs[active - 1]] <= k) --active;
             net->sub_blocks[k] = lead[active];
         }
-        CMF_HIP_CHECK(hipMemcpyAsync(net->blk_order, net->host_order, (size_t)nb * 4, hipMemcpyHostToDevice, s));
+        CMF_HIP_CHECK(hipMemcpyAsync(net->blk_rec_ord, net->host_rec, (size_t)nb * sizeof(int4),
+                                     hipMemcpyHostToDevice, s));
         CMF_HIP_CHECK(hipEventRecord(net->order_copied, s));
-        v.blk_order = net->blk_order;
+        v.blk_rec = net->blk_rec_ord;
+        v.ordered = true;
     }
     rc = use_graph ? substep_loop_graph(net, v, f, count, s) : substep_loop(net, v, f, count, s);
     if (rc) return rc;
@@ -948,7 +948,8 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
             return set_error(CMF_ERR_INVALID, "cmf_network_create: bifurcation index out of range at path %lld",
                              (long long)p);
     // groups: contiguous ranges closed under downstream links and paths
-    std::vector<int32_t> cgroup, blk_first, blk_end, blk_group, gblk_ptr;
+    std::vector<int32_t> cgroup, gblk_ptr;
+    std::vector<int4> rec;
     if (num_groups < 0 || (num_groups > 0 && !group_ptr))
         return set_error(CMF_ERR_INVALID, "cmf_network_create_grouped: bad group arguments");
     if (num_groups > 0) {
@@ -961,15 +962,18 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
             if (group_ptr[g + 1] <= group_ptr[g])
                 return set_error(CMF_ERR_INVALID, "cmf_network_create_grouped: group %lld is empty or reversed",
                                  (long long)g);
-            gblk_ptr.push_back((int32_t)blk_first.size());
+            gblk_ptr.push_back((int32_t)rec.size());
             for (int64_t i = group_ptr[g]; i < group_ptr[g + 1]; ++i) cgroup[i] = (int32_t)g;
             for (int64_t b = group_ptr[g]; b < group_ptr[g + 1]; b += kBlock) {
-                blk_first.push_back((int32_t)b);
-                blk_end.push_back((int32_t)std::min<int64_t>(b + kBlock, group_ptr[g + 1]));
-                blk_group.push_back((int32_t)g);
+                int4 r;
+                r.x = (int32_t)b;
+                r.y = (int32_t)std::min<int64_t>(b + kBlock, group_ptr[g + 1]);
+                r.z = (int32_t)rec.size();
+                r.w = (int32_t)g;
+                rec.push_back(r);
             }
         }
-        gblk_ptr.push_back((int32_t)blk_first.size());
+        gblk_ptr.push_back((int32_t)rec.size());
         for (int64_t i = 0; i < n; ++i)
             if (cgroup[downstream_idx[i]] != cgroup[i])
                 return set_error(CMF_ERR_INVALID, "cmf_network_create_grouped: catchment %lld drains into another group",
@@ -1041,7 +1045,7 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     const size_t el = (size_t)n * (size_t)num_trials * 8;
     const size_t pel = (size_t)std::max<int64_t>(num_paths, 1) * (size_t)num_trials * 8;
     net->G = num_groups;
-    net->nblk = (int64_t)blk_first.size();
+    net->nblk = (int64_t)rec.size();
     // partials for the smallest block size a tunable pass may use (64)
     net->partial_blocks = std::max<int64_t>((n * num_trials + 63) / 64, net->nblk) + 1;
     if ((rc = upload((void**)&net->up_ptr, up_ptr.data(), up_ptr.size() * 4)) ||
@@ -1061,9 +1065,7 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     }
     if (num_groups > 0 &&
         ((rc = upload((void**)&net->cgroup, cgroup.data(), cgroup.size() * 4)) ||
-         (rc = upload((void**)&net->blk_first, blk_first.data(), blk_first.size() * 4)) ||
-         (rc = upload((void**)&net->blk_end, blk_end.data(), blk_end.size() * 4)) ||
-         (rc = upload((void**)&net->blk_group, blk_group.data(), blk_group.size() * 4)) ||
+         (rc = upload((void**)&net->blk_rec_id, rec.data(), rec.size() * sizeof(int4))) ||
          (rc = upload((void**)&net->gblk_ptr, gblk_ptr.data(), gblk_ptr.size() * 4)))) {
         free_network(net);
         return rc;
@@ -1076,8 +1078,9 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     }
     if (num_groups > 0) {
         net->gblk_host = gblk_ptr;
-        if ((rc = dev_alloc((void**)&net->blk_order, (size_t)net->nblk * 4)) ||
-            (e = hipHostMalloc((void**)&net->host_order, (size_t)net->nblk * 4, hipHostMallocDefault)) != hipSuccess ||
+        net->rec_host = rec;
+        if ((rc = dev_alloc((void**)&net->blk_rec_ord, (size_t)net->nblk * sizeof(int4))) ||
+            (e = hipHostMalloc((void**)&net->host_rec, (size_t)net->nblk * sizeof(int4), hipHostMallocDefault)) != hipSuccess ||
             (e = hipEventCreateWithFlags(&net->order_copied, hipEventDisableTiming)) != hipSuccess) {
             free_network(net);
             return rc ? rc : set_error(CMF_ERR_NOMEM, "block order buffers: %s", hipGetErrorString(e));

@@ -156,13 +156,14 @@ struct View {
     double* partials;
     // groups (G > 0): independent sets of basins, each with its own sub-step
     // count and time step (slot = group of the catchment).  Catchment kernels
-    // then run one block per table entry: block b covers catchments
-    // [blk_first[b], blk_end[b]) of a single group.
+    // then run one block per record of blk_rec: {first catchment, end, block
+    // table entry (the log partial slot), group}, a single group's
+    // catchments [first, end).  `ordered`: the records of this launch belong
+    // to groups still sub-stepping only (the table ordered by group count).
     int64_t G;
-    const int32_t *cgroup, *blk_first, *blk_end;
-    // grouped: grid block -> block table entry, the blocks of the groups
-    // still sub-stepping first (nullptr: the same index)
-    const int32_t* blk_order;
+    const int32_t* cgroup;
+    const int4* blk_rec;
+    bool ordered;
     // taping (cmf_hip_substep_taped): the outflow pass also writes the
     // sub-step's input state here (nullptr: no tape)
     REAL *tape_q_r, *tape_q_f, *tape_rcsd, *tape_fcsd, *tape_fcsa, *tape_rd, *tape_fd;
@@ -192,6 +193,12 @@ struct View {
         if (substep < 0 || !per_slot()) return true;
         return substep < max_sub_steps[slot(t, c)];
     }
+    // the same in a catchment kernel, whose ordered launches hold active
+    // groups only
+    __device__ __forceinline__ bool active_in_block(int64_t t, int64_t c, int substep) const {
+        if (G > 0 && ordered) return true;
+        return active(t, c, substep);
+    }
     // the last sub-step of (t, c)'s slot, or `shared_last` with one count for all
     __device__ __forceinline__ bool last_substep(int64_t t, int64_t c, int substep, int shared_last) const {
         return (per_slot() && substep >= 0) ? (substep == max_sub_steps[slot(t, c)] - 1) : (shared_last != 0);
@@ -206,22 +213,19 @@ struct View {
     __device__ __forceinline__ void split_c(int64_t g, int64_t& t, int64_t& i) const {
         split(g, N, ts == 1, t, i);
     }
-    __device__ __forceinline__ int64_t table_block(int64_t blk) const {
-        return blk_order ? (int64_t)blk_order[blk] : blk;
-    }
     // log partial slot of this block (its block table entry when grouped)
     __device__ __forceinline__ int64_t partial_slot() const {
-        return G > 0 ? table_block(blockIdx.x) : (int64_t)blockIdx.x;
+        return G > 0 ? (int64_t)blk_rec[blockIdx.x].z : (int64_t)blockIdx.x;
     }
     // catchment of this thread in a catchment kernel (linear index `g` of the
     // plain grid, or the group block table); false past the end
     __device__ __forceinline__ bool catchment_of(int64_t g, int64_t& t, int64_t& i) const {
         if (G > 0) {
-            const int64_t b = table_block(blockIdx.x);
-            const int64_t j = (int64_t)blk_first[b] + threadIdx.x;
+            const int4 r = blk_rec[blockIdx.x];
+            const int64_t j = (int64_t)r.x + threadIdx.x;
             t = 0;
             i = j;
-            return j < blk_end[b];
+            return j < r.y;
         }
         if (g >= N * T) return false;
         split_c(g, t, i);
@@ -230,11 +234,11 @@ struct View {
     // the same for block `blk` of the catchment grid
     __device__ __forceinline__ bool catchment_in(int64_t blk, int64_t& t, int64_t& i) const {
         if (G > 0) {
-            const int64_t b = table_block(blk);
-            const int64_t j = (int64_t)blk_first[b] + threadIdx.x;
+            const int4 r = blk_rec[blk];
+            const int64_t j = (int64_t)r.x + threadIdx.x;
             t = 0;
             i = j;
-            return j < blk_end[b];
+            return j < r.y;
         }
         return catchment_of(blk * blockDim.x + threadIdx.x, t, i);
     }

@@ -72,7 +72,7 @@ template <class REAL, class STO, bool FUSED>
 __device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int substep, int64_t blk, int store_last) {
     int64_t t, i;
     if (!v.catchment_in(blk, t, i)) return;
-    if (!v.active(t, i, substep)) return;
+    if (!v.active_in_block(t, i, substep)) return;
     const int64_t e = v.el(t, i);
     const int64_t d = v.dn[i];
     const int64_t ed = v.el(t, d);
@@ -1069,7 +1069,7 @@ template <class REAL, class STO, bool CONTIG>
 __global__ __launch_bounds__(kMaxBlock) void k_limiter(View<REAL, STO> v, int substep, int store_last) {
     int64_t t, j;
     if (!v.catchment_of(xcd_block() * blockDim.x + threadIdx.x, t, j)) return;
-    if (!v.active(t, j, substep)) return;
+    if (!v.active_in_block(t, j, substep)) return;
     const int64_t e = v.el(t, j);
     const int2 ev = v.has(CMF_HAS_BIFURCATION) ? v.ev1[j] : make_int2(-1, 0);
     STO og = outgoing_terms<REAL, STO, CONTIG>(v, t, j);
@@ -1172,7 +1172,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_fused(View<REAL, STO> v, in
         for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
     }
     int64_t t, j;
-    if (v.catchment_of(g, t, j) && v.active(t, j, substep)) {
+    if (v.catchment_of(g, t, j) && v.active_in_block(t, j, substep)) {
         const int64_t e = v.el(t, j);
         const REAL dt = v.dt(t, j);
         // round trip 1
@@ -1400,12 +1400,12 @@ __global__ __launch_bounds__(kBlock) void k_cfl(View<REAL, STO> v) {
 // grouped networks: one block per table entry (a single group), one
 // integer atomic max per block into the group's count
 template <class REAL, class STO>
-__global__ __launch_bounds__(kBlock) void k_cfl_grouped(View<REAL, STO> v, const int32_t* blk_group) {
+__global__ __launch_bounds__(kBlock) void k_cfl_grouped(View<REAL, STO> v) {
     const REAL outer = v.outer_time_step[0];
     int64_t t, i;
     const int best = v.catchment_of(0, t, i) ? cfl_steps(v, t, i, outer) : 0;
     const int m = block_max_int(best);
-    if (threadIdx.x == 0) atomicMax(&v.max_sub_steps[blk_group[blockIdx.x]], m);
+    if (threadIdx.x == 0) atomicMax(&v.max_sub_steps[v.blk_rec[blockIdx.x].w], m);
 }
 
 // time_step[t] = outer / count[t], in double then rounded (cama_flood_model.py:127,132)
