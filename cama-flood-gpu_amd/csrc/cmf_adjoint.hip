@@ -2,8 +2,11 @@
 // physics: local-inertial river and floodplain outflow
 // (cmfgpu/phys/triton/outflow.py:15-315), the storage limiter and inflow sums
 // (:318-383), the storage update and flood-stage diagnosis
-// (cmfgpu/phys/triton/storage.py:16-193).  Levee, bifurcation and reservoir
-// modules are not differentiated.
+// (cmfgpu/phys/triton/storage.py:16-193), and of the bifurcation paths
+// (cmfgpu/phys/triton/bifurcation.py:13-151: per-level local-inertial flow,
+// the 5 % storage limiter of a path, the path terms of the outgoing totals and
+// the path nets limited by the end catchments).  Levee and reservoir modules
+// are not differentiated.
 //
 // The sub-step maps the state x = (river/flood outflow, river/flood/flood-area
 // cross-section depth, river/flood storage, river/flood depth) and the
@@ -14,15 +17,22 @@
 // those of the forward evaluation, recomputed bit for bit from the saved input
 // state; derivatives are formed in double precision.
 //
-// Passes (each over catchments, one grid-wide boundary between them):
+// Passes (each over catchments or paths, one grid-wide boundary between them):
+//   B0  (paths) recompute the limited level flows, level sums and outgoing terms
 //   A0  recompute the unlimited outflows and outgoing terms  } skipped when the
 //   A1  recompute the outgoing totals and storage limiters   } forward taped them
 //   A2  reverse of the storage update and stage diagnosis
-//   A3  reverse of the limited flows and inflow sums, then of the limiter
+//   A3  reverse of the limited flows, inflow sums and path nets, then of the
+//       limiter
+//   B2  (paths) reverse of the path nets, the path terms, the path limiter
+//       and the per-level momentum update
 //   A4  reverse of the outflow momentum update
 //   A5  gather of the downstream water-surface adjoints (or deferred into the
 //       A2 pass of the next, earlier sub-step, which loops over the same
 //       upstream range anyway)
+// A path pass writes what belongs to its end catchments into per-path slots;
+// the next catchment pass gathers them over the catchment's events in path
+// order, so no pass adds into another element's memory.
 
 namespace cmf {
 
@@ -46,7 +56,20 @@ struct AdjView {
     REAL *w_qr, *w_qf, *w_lim, *pos, *neg;
     REAL *a_rout, *a_fout, *a_ri, *a_fi, *a_qr, *a_qf, *a_og, *d_wd;
     REAL dt;
+    // bifurcation paths (bif: the module is open): saved input path state
+    // [P, LB] and its adjoints after (lout_b*) and before (lin_b*) the sub-step
+    bool bif;
+    const REAL *bq, *bcsd, *lout_bq, *lout_bcsd;
+    REAL *lin_bq, *lin_bcsd;
+    // B0: limited level flows [P, LB], their sum, the outgoing terms at the
+    // upstream (pos) and downstream (neg) end
+    REAL *p_qlim, *p_raw, *p_pos, *p_neg;
+    // A2: adjoint of each catchment's bifurcation total; B2: adjoint of the
+    // end water surfaces and total storages
+    REAL *a_gb, *p_w0, *p_w1, *p_t0, *p_t1;
 };
+
+constexpr int kAdjMaxLevels = 8;
 
 template <class STO>
 __device__ __forceinline__ STO adj_sto(const void* p, int64_t i) { return static_cast<const STO*>(p)[i]; }
@@ -157,6 +180,103 @@ __device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v
     return o;
 }
 
+// ------------------------------------------------------- path trace ----
+// The arithmetic of bif_path (non-levee variant, end surfaces and totals
+// formed from the state) for path p on the saved input state, same
+// operations in the same order, keeping what the reverse needs.
+template <class REAL>
+struct PathTrace {
+    int64_t ci, di;
+    bool max_own, slope_in, min_own, limited;
+    REAL w0, w1, max_w, slope, ts0, ts1, sum, ratio, limit, sum_l;
+    REAL upd[kAdjMaxLevels], sq_a[kAdjMaxLevels], sq_b[kAdjMaxLevels], sifd[kAdjMaxLevels];
+    REAL unit[kAdjMaxLevels], den[kAdjMaxLevels], q[kAdjMaxLevels];
+};
+
+template <class REAL, class STO>
+__device__ __forceinline__ void path_trace(const View<REAL, STO>& v, const AdjView<REAL>& a, int64_t p,
+                                           REAL time_step, PathTrace<REAL>& o) {
+    const int LB = (int)v.LB;
+    o.ci = v.bif_ci[p];
+    o.di = v.bif_di[p];
+    const int64_t ci = o.ci, di = o.di;
+    const REAL gravity = v.gravity;
+    const REAL length = v.bif_length[p];
+    o.w0 = a.rd[ci] + (v.elev[ci] - v.river_height[ci]);
+    o.w1 = a.rd[di] + (v.elev[di] - v.river_height[di]);
+    o.ts0 = (REAL)adj_sto<STO>(a.rs_raw, ci) + (REAL)adj_sto<STO>(a.fs_raw, ci) + (REAL)v.protected_storage[ci];
+    o.ts1 = (REAL)adj_sto<STO>(a.rs_raw, di) + (REAL)adj_sto<STO>(a.fs_raw, di) + (REAL)v.protected_storage[di];
+    o.max_own = (o.w0 > o.w1 || o.w1 != o.w1);
+    o.max_w = vmax(o.w0, o.w1);
+    const REAL sx = (o.w0 - o.w1) / length;
+    o.slope_in = (sx > (REAL)-0.005) && (sx < (REAL)0.005);
+    o.slope = vclamp(sx, (REAL)-0.005, (REAL)0.005);
+    REAL sum = (REAL)0;
+#pragma unroll
+    for (int l = 0; l < kAdjMaxLevels; ++l) {
+        if (l >= LB) break;
+        const int64_t k = p * LB + l;
+        const REAL csd_old = a.bcsd[k];
+        const REAL upd = vmax(o.max_w - v.bif_elev[k], (REAL)0.0);
+        const REAL sa = dsqrt(upd * csd_old), sb = dsqrt(upd * (REAL)0.01);
+        const REAL sifd = vmax(sa, sb);
+        const REAL w = v.bif_width[k], man = v.bif_manning[k];
+        const REAL unit = a.bq[k] / w;
+        REAL q = (REAL)0.0, den = (REAL)1;
+        if (sifd > (REAL)1e-5) {
+            const REAL num = w * (unit + gravity * time_step * sifd * o.slope);
+            den = (REAL)1.0 + gravity * time_step * (man * man) * dabs(unit)
+                * ((REAL)1.0 / (sifd * sifd * dcbrt(sifd)));
+            q = num / den;
+        }
+        sum += q;
+        o.upd[l] = upd;
+        o.sq_a[l] = sa;
+        o.sq_b[l] = sb;
+        o.sifd[l] = sifd;
+        o.unit[l] = unit;
+        o.den[l] = den;
+        o.q[l] = q;
+    }
+    o.sum = sum;
+    o.min_own = (o.ts0 < o.ts1 || o.ts1 != o.ts1);
+    o.ratio = (REAL)0.05 * vmin(o.ts0, o.ts1) / (dabs(sum) * time_step);
+    o.limited = o.ratio < (REAL)1.0;
+    o.limit = vmin(o.ratio, (REAL)1.0);
+    o.sum_l = sum * o.limit;
+}
+
+// B0: limited level flows, their sum and the outgoing terms of every path
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_adj_path_fwd(View<REAL, STO> v, AdjView<REAL> a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= v.P) return;
+    PathTrace<REAL> o;
+    path_trace(v, a, p, a.dt, o);
+    const int LB = (int)v.LB;
+    REAL raw = (REAL)0;
+#pragma unroll
+    for (int l = 0; l < kAdjMaxLevels; ++l) {
+        if (l >= LB) break;
+        const REAL q = o.q[l] * o.limit;
+        a.p_qlim[p * LB + l] = q;
+        raw += q;
+    }
+    a.p_raw[p] = raw;
+    a.p_pos[p] = vmax(o.sum_l, (REAL)0.0) * a.dt;
+    a.p_neg[p] = -vmin(o.sum_l, (REAL)0.0) * a.dt;
+}
+
+// Path terms of catchment j's outgoing total, in the order of bif_terms_from.
+template <class REAL, class STO>
+__device__ __forceinline__ STO adj_path_terms(const View<REAL, STO>& v, const AdjView<REAL>& a, int64_t j, STO acc) {
+    for (int32_t k = v.ev_ptr[j]; k < v.ev_ptr[j + 1]; ++k) {
+        const int32_t code = v.ev_code[k];
+        acc += (STO)((code & 1) ? a.p_neg[code >> 1] : a.p_pos[code >> 1]);
+    }
+    return acc;
+}
+
 // A0: unlimited outflows and outgoing terms of the saved state
 template <class REAL, class STO>
 __global__ __launch_bounds__(kBlock) void k_adj_outflow_fwd(View<REAL, STO> v, AdjView<REAL> a) {
@@ -184,6 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_limiter_fwd(View<REAL, STO> v, A
         acc += (STO)a.neg[u];
     }
     acc += (STO)a.pos[j];
+    if (a.bif) acc = adj_path_terms(v, a, j, acc);
     const REAL og = (REAL)acc;
     const REAL rate = (REAL)(adj_sto<STO>(a.rs_raw, j) + adj_sto<STO>(a.fs_raw, j));
     a.w_lim[j] = (og > (REAL)1e-8) ? vmin(rate / og, (REAL)1.0) : (REAL)1.0;
@@ -216,13 +337,27 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
         if (PENDING) pend += (double)a.d_wd[u];
     }
     const REAL rin = (REAL)ri, fin = (REAL)fi;
+    // bifurcation total: the path nets in path order, as bif_nets forms them
+    STO gsum = (STO)0;
+    if (a.bif) {
+        for (int32_t k = v.ev_ptr[j]; k < v.ev_ptr[j + 1]; ++k) {
+            const int32_t code = v.ev_code[k];
+            const REAL l_other = a.lim[v.ev_other[k]];
+            const bool down_end = (code & 1) != 0;
+            const REAL l0 = down_end ? l_other : l_own, l1 = down_end ? l_own : l_other;
+            const REAL raw = a.p_raw[code >> 1];
+            const REAL net = (raw >= (REAL)0.0) ? raw * l0 : raw * l1;
+            gsum += down_end ? (STO)(-net) : (STO)net;
+        }
+    }
+    const REAL gb = (REAL)gsum;
     const STO rs = adj_sto<STO>(a.rs_raw, j), fs = adj_sto<STO>(a.fs_raw, j);
     const STO ps = v.protected_storage[j];
     const REAL ro = v.runoff[j];
     // storage update (storage.py:81-94)
     const STO sa = rs + (STO)((rin - rout) * dt);
     const bool carry = sa < (STO)0.0;
-    const STO sb = fs + (carry ? sa : (STO)0.0) + (STO)((fin - fout - (REAL)0.0) * dt);
+    const STO sb = fs + (carry ? sa : (STO)0.0) + (STO)((fin - fout - gb) * dt);
     const STO rsu1 = vmax(sa, (STO)0.0);
     const bool b_neg = sb < (STO)0.0;
     const STO rsu2 = b_neg ? vmax(rsu1 + sb, (STO)0.0) : rsu1;
@@ -311,6 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     a.a_fout[j] = (REAL)(Lfout + (double)a.lout[ADJ_QF][j]);
     a.a_ri[j] = (REAL)Lrin;
     a.a_fi[j] = (REAL)Lfin;
+    if (a.bif) a.a_gb[j] = (REAL)(-Lb * (double)dt);
     a.lin[ADJ_RS][j] = (REAL)Lrs;
     a.lin[ADJ_FS][j] = (REAL)Lfs;
     a.g_h[j] += LH - pend;
@@ -350,17 +486,183 @@ __global__ __launch_bounds__(kBlock) void k_adj_route(View<REAL, STO> v, AdjView
         if (uf < (REAL)0.0) Llim += ((double)a.a_fout[u] + Lfi_k) * (double)uf;
     }
     og_acc += (STO)((vmax(qr, (REAL)0.0) + vmax(qf, (REAL)0.0)) * dt);
+    if (a.bif) {
+        og_acc = adj_path_terms(v, a, k, og_acc);
+    }
     // limiter: lim = og > 1e-8 ? min(rate / og, 1) : 1, rate = rs + fs
     const REAL og = (REAL)og_acc;
     const REAL rate = (REAL)(adj_sto<STO>(a.rs_raw, k) + adj_sto<STO>(a.fs_raw, k));
     double Log = 0.0;
     if (og > (REAL)1e-8 && (rate / og) < (REAL)1.0) {
+        if (a.bif) {
+            // the limiter of k also scales the nets of its paths
+            // (bifurcation.py:137-151: the level sum limited by the upstream
+            // end when non-negative, else by the downstream end, added to the
+            // upstream end's total and subtracted from the downstream end's)
+            // and their limited next level flows (:132-143) on the same rule
+            const int LB = (int)v.LB;
+            for (int32_t e = v.ev_ptr[k]; e < v.ev_ptr[k + 1]; ++e) {
+                const int32_t code = v.ev_code[e];
+                const int64_t p = code >> 1;
+                const bool down_end = (code & 1) != 0;
+                const int64_t other = v.ev_other[e];
+                const double Lnet = down_end ? (double)a.a_gb[other] - (double)a.a_gb[k]
+                                             : (double)a.a_gb[k] - (double)a.a_gb[other];
+                const REAL raw = a.p_raw[p];
+                if ((raw >= (REAL)0.0) != down_end) Llim += Lnet * (double)raw;
+                for (int l = 0; l < LB; ++l) {
+                    const REAL q = a.p_qlim[p * LB + l];
+                    if ((q >= (REAL)0.0) != down_end) Llim += (double)a.lout_bq[p * LB + l] * (double)q;
+                }
+            }
+        }
         const double Lrate = Llim / (double)og;
         Log = -Llim * (double)rate / ((double)og * (double)og);
         a.lin[ADJ_RS][k] = (REAL)((double)a.lin[ADJ_RS][k] + Lrate);
         a.lin[ADJ_FS][k] = (REAL)((double)a.lin[ADJ_FS][k] + Lrate);
     }
     a.a_og[k] = (REAL)Log;
+}
+
+// B2: reverse of a path's net and limited next level flows (as in A3, their
+// share of the adjoint of the limited level flows), its outgoing terms, its
+// 5 % storage limiter (bifurcation.py:93-104) and the per-level momentum
+// update (:49-89).  Gives
+// the adjoint of the input path state and, per end, of the end's water
+// surface and total storage.  Eight lanes per path, lane l on level l: the
+// level sums pass between the lanes of the group by shuffles, in level order
+// (the forward's order for its sum), so the recomputed values are the
+// forward's and every result is independent of the launch.
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<REAL> a) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t p = gid >> 3;
+    const int l = (int)(gid & 7);
+    if (p >= v.P) return;   // whole groups of eight leave together
+    const int LB = (int)v.LB;
+    const bool lv = l < LB;
+    const int64_t ci = v.bif_ci[p], di = v.bif_di[p];
+    const REAL time_step = a.dt;
+    const REAL gravity = v.gravity;
+    const REAL length = v.bif_length[p];
+    // path values (every lane of the group forms the same ones)
+    const REAL w0 = a.rd[ci] + (v.elev[ci] - v.river_height[ci]);
+    const REAL w1 = a.rd[di] + (v.elev[di] - v.river_height[di]);
+    const REAL ts0 = (REAL)adj_sto<STO>(a.rs_raw, ci) + (REAL)adj_sto<STO>(a.fs_raw, ci) + (REAL)v.protected_storage[ci];
+    const REAL ts1 = (REAL)adj_sto<STO>(a.rs_raw, di) + (REAL)adj_sto<STO>(a.fs_raw, di) + (REAL)v.protected_storage[di];
+    const bool max_own = (w0 > w1 || w1 != w1);
+    const REAL max_w = vmax(w0, w1);
+    const REAL sx = (w0 - w1) / length;
+    const bool slope_in = (sx > (REAL)-0.005) && (sx < (REAL)0.005);
+    const REAL slope = vclamp(sx, (REAL)-0.005, (REAL)0.005);
+    // level l of the forward
+    const int64_t k = p * LB + (lv ? l : 0);
+    REAL upd = (REAL)0, sa = (REAL)0, sb = (REAL)0, sifd = (REAL)0, unit = (REAL)0, den = (REAL)1, q = (REAL)0;
+    REAL csd_old = (REAL)0;
+    if (lv) {
+        csd_old = a.bcsd[k];
+        upd = vmax(max_w - v.bif_elev[k], (REAL)0.0);
+        sa = dsqrt(upd * csd_old);
+        sb = dsqrt(upd * (REAL)0.01);
+        sifd = vmax(sa, sb);
+        const REAL w = v.bif_width[k], man = v.bif_manning[k];
+        unit = a.bq[k] / w;
+        if (sifd > (REAL)1e-5) {
+            const REAL num = w * (unit + gravity * time_step * sifd * slope);
+            den = (REAL)1.0 + gravity * time_step * (man * man) * dabs(unit)
+                * ((REAL)1.0 / (sifd * sifd * dcbrt(sifd)));
+            q = num / den;
+        }
+    }
+    REAL sum = (REAL)0;
+    for (int m = 0; m < LB; ++m) sum += __shfl(q, m, 8);
+    const REAL ratio = (REAL)0.05 * vmin(ts0, ts1) / (dabs(sum) * time_step);
+    const bool limited = ratio < (REAL)1.0;
+    const REAL limit = vmin(ratio, (REAL)1.0);
+    const REAL sum_l = sum * limit;
+    const double dt = (double)time_step, g = (double)gravity;
+    // outgoing terms: max(sum_l, 0) dt at the upstream end, -min(sum_l, 0) dt
+    // at the downstream end
+    double Lsum_l = 0.0;
+    if (sum_l > (REAL)0.0) Lsum_l += (double)a.a_og[ci] * dt;
+    if (sum_l < (REAL)0.0) Lsum_l -= (double)a.a_og[di] * dt;
+    // the limited level flow q_l * limit enters the path net (level sum raw)
+    // and the next level flow, each limited by one end
+    double Llq = 0.0;
+    if (lv) {
+        const REAL l0 = a.lim[ci], l1 = a.lim[di];
+        const double Lnet = (double)a.a_gb[ci] - (double)a.a_gb[di];
+        const REAL raw = a.p_raw[p], ql = a.p_qlim[k];
+        Llq = Lnet * (double)(raw >= (REAL)0.0 ? l0 : l1)
+            + (double)a.lout_bq[k] * (double)(ql >= (REAL)0.0 ? l0 : l1);
+    }
+    // sum_l = sum * limit, limited level flow q_l * limit
+    const double mine = Llq * (double)q;
+    double Llimit = 0.0;
+    for (int m = 0; m < LB; ++m) Llimit += __shfl(mine, m, 8);
+    Llimit += Lsum_l * (double)sum;
+    // limit = min(0.05 min(ts0, ts1) / (|sum| dt), 1)
+    double Lts0 = 0.0, Lts1 = 0.0, Lsum = 0.0;
+    if (limited) {
+        const double as = fabs((double)sum);
+        const double Lm = Llimit * 0.05 / (as * dt);
+        if (ts0 < ts1 || ts1 != ts1) Lts0 += Lm;
+        else Lts1 += Lm;
+        const double sgn = sum > (REAL)0.0 ? 1.0 : (sum < (REAL)0.0 ? -1.0 : 0.0);
+        Lsum = -Llimit * (double)ratio / as * sgn;
+    }
+    const double Lql = (Llq + Lsum_l) * (double)limit + Lsum;
+    double Lqold = 0.0, Lsifd = 0.0, Lcsd = 0.0, Lslope = 0.0, Lmax_w = 0.0;
+    if (lv) {
+        if (sifd > (REAL)1e-5) {
+            const double w = (double)v.bif_width[k], man = (double)v.bif_manning[k];
+            const double s = (double)sifd, dd = (double)den;
+            const double Lnum = Lql / dd, Lden = -Lql * (double)q / dd;
+            const double Q = 1.0 / (s * s * cbrt(s));
+            const double u = (double)unit, au = fabs(u);
+            const double sgn = u > 0.0 ? 1.0 : (u < 0.0 ? -1.0 : 0.0);
+            // num = w (q_old / w + g dt sifd slope)
+            Lqold += Lnum;
+            Lsifd += Lnum * w * g * dt * (double)slope;
+            Lslope += Lnum * w * g * dt * s;
+            // den = 1 + g dt n^2 |q_old / w| sifd^(-7/3)
+            Lqold += Lden * g * dt * man * man * sgn * Q / w;
+            Lsifd += Lden * g * dt * man * man * au * (-7.0 / 3.0) * Q / s;
+        }
+        // sifd = max(sqrt(upd csd_old), sqrt(upd 0.01)); the next depth is upd
+        double Lupd = (double)a.lout_bcsd[k];
+        if (sa > sb || sb != sb) {
+            if (sa > (REAL)0.0) {
+                Lupd += Lsifd * 0.5 * (double)csd_old / (double)sa;
+                Lcsd += Lsifd * 0.5 * (double)upd / (double)sa;
+            }
+        } else if (sb > (REAL)0.0) {
+            Lupd += Lsifd * 0.5 * 0.01 / (double)sb;
+        }
+        // upd = max(max_w - elevation, 0)
+        if (max_w - v.bif_elev[k] > (REAL)0.0) Lmax_w += Lupd;
+        a.lin_bq[k] = (REAL)Lqold;
+        a.lin_bcsd[k] = (REAL)Lcsd;
+    }
+    double Ls = 0.0, Lx = 0.0;
+    for (int m = 0; m < LB; ++m) {
+        Ls += __shfl(Lslope, m, 8);
+        Lx += __shfl(Lmax_w, m, 8);
+    }
+    if (l != 0) return;
+    // slope = clamp((w0 - w1) / length, -0.005, 0.005); max_w = max(w0, w1)
+    double Lw0 = 0.0, Lw1 = 0.0;
+    if (slope_in) {
+        const double len = (double)length;
+        Lw0 += Ls / len;
+        Lw1 -= Ls / len;
+    }
+    if (max_own) Lw0 += Lx;
+    else Lw1 += Lx;
+    a.p_w0[p] = (REAL)Lw0;
+    a.p_w1[p] = (REAL)Lw1;
+    a.p_t0[p] = (REAL)Lts0;
+    a.p_t1[p] = (REAL)Lts1;
 }
 
 // A4: reverse of the outflow momentum update of catchment i.  The final
@@ -464,6 +766,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
         Lre -= Lupd_rcsd;
     }
     if (o.slope_in) Lslope += Lfslope;
+    if (a.bif) {
+        // the paths of i read its water surface and total storage (B2)
+        for (int32_t e = v.ev_ptr[i]; e < v.ev_ptr[i + 1]; ++e) {
+            const int32_t code = v.ev_code[e];
+            const int64_t p = code >> 1;
+            Lwse += (double)((code & 1) ? a.p_w1[p] : a.p_w0[p]);
+            Ltot += (double)((code & 1) ? a.p_t1[p] : a.p_t0[p]);
+        }
+    }
     const double dist = (double)v.dist[i];
     Lwse += Lslope / dist;
     Lwse_dn -= Lslope / dist;

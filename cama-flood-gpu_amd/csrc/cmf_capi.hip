@@ -85,8 +85,9 @@ struct cmf_network {
     // output statistics accumulator (cmf_network_set_outflow_accumulator)
     double* acc_outflow = nullptr;
     // adjoint scratch (cmf_hip_substep_adjoint), allocated on first use:
-    // 13 arrays of N elements of up to 8 bytes
+    // 14 arrays of N elements of up to 8 bytes
     void* adj_scratch = nullptr;
+    void* adj_path_scratch = nullptr;   // with paths: [P, LB] + [P] x 7
 };
 
 namespace {
@@ -105,6 +106,7 @@ void free_network(cmf_network* net) {
     for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
     if (net->capture_stream) (void)hipStreamDestroy(net->capture_stream);
     if (net->adj_scratch) (void)hipFree(net->adj_scratch);
+    if (net->adj_path_scratch) (void)hipFree(net->adj_path_scratch);
     delete net;
 }
 
@@ -812,8 +814,8 @@ int run_substep_taped(cmf_network* net, const cmf_fields* f, int substep, int st
     if (rc) return rc;
     if ((rc = need_base(f, "substep_taped")) || (rc = need_features(f, "substep_taped"))) return rc;
     if (!tp) return set_error(CMF_ERR_INVALID, "substep_taped: null tape");
-    if (v.T != 1 || net->G > 0 || (f->flags & (CMF_HAS_LEVEE | CMF_HAS_BIFURCATION | CMF_HAS_RESERVOIR)))
-        return set_error(CMF_ERR_UNSUPPORTED, "substep_taped: base physics, one trial, ungrouped");
+    if (v.T != 1 || net->G > 0 || (f->flags & (CMF_HAS_LEVEE | CMF_HAS_RESERVOIR)))
+        return set_error(CMF_ERR_UNSUPPORTED, "substep_taped: base physics and bifurcation, one trial, ungrouped");
     void* ptrs[] = {tp->river_outflow, tp->flood_outflow, tp->river_cross_section_depth,
                     tp->flood_cross_section_depth, tp->flood_cross_section_area, tp->river_storage,
                     tp->flood_storage, tp->river_depth, tp->flood_depth, tp->qraw_r, tp->qraw_f, tp->lim};
@@ -831,6 +833,19 @@ int run_substep_taped(cmf_network* net, const cmf_fields* f, int substep, int st
     v.qraw_r = static_cast<REAL*>(tp->qraw_r);
     v.qraw_f = static_cast<REAL*>(tp->qraw_f);
     v.lim = static_cast<REAL*>(tp->lim);
+    if (v.has(CMF_HAS_BIFURCATION) && v.P > 0) {
+        if (!tp->bifurcation_outflow || !tp->bifurcation_cross_section_depth)
+            return set_error(CMF_ERR_INVALID, "substep_taped: the path state slots are needed with bifurcation");
+        v.tape_bq = static_cast<REAL*>(tp->bifurcation_outflow);
+        v.tape_bcsd = static_cast<REAL*>(tp->bifurcation_cross_section_depth);
+        if (tp->path_terms) {
+            REAL* pt = static_cast<REAL*>(tp->path_terms);
+            v.tape_bqlim = pt;
+            v.path_raw = pt + v.P * v.LB;
+            v.path_pos = v.path_raw + v.P;
+            v.path_neg = v.path_pos + v.P;
+        }
+    }
     return fused_substep(net, v, f, substep, store_inflow, stream);
 }
 
@@ -841,8 +856,8 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
     int rc = make_view<REAL, STO>(net, f, v);
     if (rc) return rc;
     if (!adj) return set_error(CMF_ERR_INVALID, "substep_adjoint: null adjoint table");
-    if (f->flags & (CMF_HAS_LEVEE | CMF_HAS_BIFURCATION | CMF_HAS_RESERVOIR))
-        return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: levee, bifurcation and reservoir are not differentiated");
+    if (f->flags & (CMF_HAS_LEVEE | CMF_HAS_RESERVOIR))
+        return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: levee and reservoir are not differentiated");
     if (v.T != 1 || net->G > 0 || f->batched)
         return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: one trial, ungrouped, no per-trial parameters");
     if ((rc = require({NEED(downstream_idx), NEED(river_manning), NEED(flood_manning), NEED(river_width),
@@ -861,7 +876,24 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
     if (!(adj->time_step > 0.0) && !(adj->mode & CMF_ADJ_FLUSH))
         return set_error(CMF_ERR_INVALID, "substep_adjoint: time_step must be > 0");
     const int64_t n = v.N;
-    if (!net->adj_scratch && (rc = dev_alloc(&net->adj_scratch, (size_t)n * 8 * 13))) return rc;
+    const bool bif = v.has(CMF_HAS_BIFURCATION) && v.P > 0;
+    if (bif) {
+        if ((rc = require({NEED(bifurcation_catchment_idx), NEED(bifurcation_downstream_idx),
+                           NEED(bifurcation_manning), NEED(bifurcation_width), NEED(bifurcation_length),
+                           NEED(bifurcation_elevation)}, "substep_adjoint")))
+            return rc;
+        if (v.LB < 1 || v.LB > kAdjMaxLevels)
+            return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: 1 to %d bifurcation levels", kAdjMaxLevels);
+        if (!(adj->mode & CMF_ADJ_FLUSH) &&
+            (!adj->bifurcation_outflow || !adj->bifurcation_cross_section_depth || !adj->adj_out_path[0] ||
+             !adj->adj_out_path[1] || !adj->adj_in_path[0] || !adj->adj_in_path[1]))
+            return set_error(CMF_ERR_INVALID, "substep_adjoint: the path state and its adjoints are needed with "
+                                              "bifurcation");
+    }
+    if (!net->adj_scratch && (rc = dev_alloc(&net->adj_scratch, (size_t)n * 8 * 14))) return rc;
+    if (bif && !net->adj_path_scratch &&
+        (rc = dev_alloc(&net->adj_path_scratch, (size_t)v.P * 8 * (size_t)(v.LB + 7))))
+        return rc;
     AdjView<REAL> a;
     a.q_r = static_cast<const REAL*>(adj->river_outflow);
     a.q_f = static_cast<const REAL*>(adj->flood_outflow);
@@ -880,11 +912,34 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
     a.g_h = adj->grad_river_height;
     REAL* r = static_cast<REAL*>(net->adj_scratch);
     const int64_t st = n * (int64_t)(8 / sizeof(REAL));
-    REAL* w[13];
-    for (int k = 0; k < 13; ++k) w[k] = r + k * st;
+    REAL* w[14];
+    for (int k = 0; k < 14; ++k) w[k] = r + k * st;
     a.w_qr = w[0]; a.w_qf = w[1]; a.w_lim = w[2]; a.pos = w[3]; a.neg = w[4];
     a.a_rout = w[5]; a.a_fout = w[6]; a.a_ri = w[7]; a.a_fi = w[8]; a.a_qr = w[9]; a.a_qf = w[10];
-    a.a_og = w[11]; a.d_wd = w[12];
+    a.a_og = w[11]; a.d_wd = w[12]; a.a_gb = w[13];
+    a.bif = bif;
+    if (bif) {
+        a.bq = static_cast<const REAL*>(adj->bifurcation_outflow);
+        a.bcsd = static_cast<const REAL*>(adj->bifurcation_cross_section_depth);
+        a.lout_bq = static_cast<const REAL*>(adj->adj_out_path[0]);
+        a.lout_bcsd = static_cast<const REAL*>(adj->adj_out_path[1]);
+        a.lin_bq = static_cast<REAL*>(adj->adj_in_path[0]);
+        a.lin_bcsd = static_cast<REAL*>(adj->adj_in_path[1]);
+        REAL* pr = static_cast<REAL*>(net->adj_path_scratch);
+        const int64_t pst = v.P * (int64_t)(8 / sizeof(REAL));
+        a.p_qlim = pr;
+        REAL* q = pr + v.LB * pst;
+        REAL** one[] = {&a.p_raw, &a.p_pos, &a.p_neg, &a.p_w0, &a.p_w1, &a.p_t0, &a.p_t1};
+        for (int k = 0; k < 7; ++k) *one[k] = q + k * pst;
+        if (adj->path_terms) {
+            // the forward's own limited level flows, level sums and terms
+            REAL* pt = const_cast<REAL*>(static_cast<const REAL*>(adj->path_terms));
+            a.p_qlim = pt;
+            a.p_raw = pt + v.P * v.LB;
+            a.p_pos = a.p_raw + v.P;
+            a.p_neg = a.p_pos + v.P;
+        }
+    }
     a.dt = (REAL)adj->time_step;
     const bool taped = adj->qraw_r && adj->qraw_f && adj->lim;
     a.qr = taped ? static_cast<const REAL*>(adj->qraw_r) : a.w_qr;
@@ -897,6 +952,7 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
         else LAUNCH((k_adj_gather<REAL, STO, false>), n, stream, v, a);
         return CMF_OK;
     }
+    if (bif && !adj->path_terms) LAUNCH_B((k_adj_path_fwd<REAL, STO>), v.P, 64, stream, v, a);
     if (!taped) {
         LAUNCH((k_adj_outflow_fwd<REAL, STO>), n, stream, v, a);
         if (contig) LAUNCH((k_adj_limiter_fwd<REAL, STO, true>), n, stream, v, a);
@@ -915,6 +971,7 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
         LAUNCH((k_adj_route<REAL, STO, false>), n, stream, v, a);
     }
 #undef CMF_AS
+    if (bif) LAUNCH((k_adj_path<REAL, STO>), v.P * 8, stream, v, a);   // eight lanes per path
     LAUNCH((k_adj_outflow<REAL, STO>), n, stream, v, a);
     if (!defer) {
         if (contig) LAUNCH((k_adj_gather<REAL, STO, true>), n, stream, v, a);

@@ -168,6 +168,8 @@ struct View {
     // sub-step's input state here (nullptr: no tape)
     REAL *tape_q_r, *tape_q_f, *tape_rcsd, *tape_fcsd, *tape_fcsa, *tape_rd, *tape_fd;
     STO *tape_rs, *tape_fs;
+    REAL *tape_bq, *tape_bcsd;    // the path pass: input path flows and depths
+    REAL* tape_bqlim;             // and its limited level flows
     // output statistics: sum over sub-steps of (river + flood outflow) * dt
     // per element (nullptr: off)
     double* acc_outflow;

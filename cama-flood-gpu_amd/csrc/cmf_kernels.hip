@@ -387,6 +387,10 @@ __device__ __forceinline__ void bif_path(const View<REAL, STO>& v, int substep, 
         const REAL w = v.prm(v.bif_width, CMF_BATCHED_BIFURCATION_WIDTH, k, ks);
         const REAL man = v.prm(v.bif_manning, CMF_BATCHED_BIFURCATION_MANNING, k, ks);
         const REAL q_old = v.bif_outflow[k];
+        if (v.tape_bq) {
+            v.tape_bq[k] = q_old;
+            v.tape_bcsd[k] = csd_old;
+        }
         REAL q = (REAL)0.0;
         if (sifd > (REAL)1e-5) {
             const REAL unit = q_old / w;
@@ -409,6 +413,7 @@ __device__ __forceinline__ void bif_path(const View<REAL, STO>& v, int substep, 
         const int64_t k = pe * LB + l;
         const REAL q = (LBFIX > 0 ? qs[LBFIX > 0 ? l : 0] : v.bif_outflow[k]) * limit;
         v.bif_outflow[k] = q;
+        if (v.tape_bqlim) v.tape_bqlim[k] = q;
         raw += q;
     }
     const REAL pos = vmax(sum, (REAL)0.0);

@@ -374,7 +374,8 @@ int cmf_network_set_outflow_accumulator(cmf_network* net, double* accumulator);
 /* ------------------------------------------------------------- adjoint */
 /* Reverse mode of one fused sub-step of the base physics (outflow, limiter
  * and inflow sums, storage update, flood stage; cmfgpu/phys/triton/outflow.py
- * :15-383 and storage.py:16-193): given the adjoint of the state after the
+ * :15-383 and storage.py:16-193) and of the bifurcation paths
+ * (cmfgpu/phys/triton/bifurcation.py:13-151): given the adjoint of the state after the
  * sub-step (adj_out) it writes the adjoint of the state before it (adj_in)
  * and adds d/d(river_manning) and d/d(river_height) into the gradient
  * accumulators, with the time step held at its forward value.  State order:
@@ -385,9 +386,13 @@ int cmf_network_set_outflow_accumulator(cmf_network* net, double* accumulator);
  * qraw_r/qraw_f/lim (REAL) may give the unlimited flows and limiters the
  * forward sub-step produced (cmf_hip_substep_taped); when NULL they are
  * recomputed from the state.  The field table supplies parameters, runoff and
- * the flood table.  Levee, bifurcation and reservoir modules, trials, groups
- * and per-trial parameters return CMF_ERR_UNSUPPORTED.  No reference
- * counterpart (the reference has no differentiation). */
+ * the flood table.  With the bifurcation module open the path state
+ * (bifurcation_outflow, bifurcation_cross_section_depth: REAL [P, levels],
+ * at most 8 levels) is part of the state: its saved input values and its
+ * adjoints after (adj_out_path) and before (adj_in_path) the sub-step.
+ * Levee and reservoir modules, trials, groups and per-trial parameters
+ * return CMF_ERR_UNSUPPORTED.  No reference counterpart (the reference has
+ * no differentiation). */
 typedef struct cmf_adjoint {
     const void* river_outflow;
     const void* flood_outflow;
@@ -408,6 +413,11 @@ typedef struct cmf_adjoint {
     double time_step;
     int32_t mode;        /* CMF_ADJ_* bits, 0 for a standalone sub-step */
     int32_t reserved;
+    const void* bifurcation_outflow;
+    const void* bifurcation_cross_section_depth;
+    const void* adj_out_path[2];
+    void* adj_in_path[2];
+    const void* path_terms;   /* the forward's path intermediates (cmf_tape), or NULL */
 } cmf_adjoint;
 /* mode bits for a sweep over consecutive sub-steps on one network:
  * DEFER leaves the last pass (the gather of downstream water-surface
@@ -421,7 +431,9 @@ int cmf_hip_substep_adjoint(cmf_network* net, const cmf_fields* fields, const cm
 
 /* One fused sub-step (as cmf_hip_substep) that records what the adjoint needs:
  * its input state (same nine fields and order as cmf_adjoint) and the
- * unlimited flows and limiters it forms.  Base physics, one trial. */
+ * unlimited flows and limiters it forms; with the bifurcation module open
+ * also the input path state [P, levels].  Base physics (and bifurcation),
+ * one trial. */
 typedef struct cmf_tape {
     void* river_outflow;
     void* flood_outflow;
@@ -435,6 +447,12 @@ typedef struct cmf_tape {
     void* qraw_r;
     void* qraw_f;
     void* lim;
+    void* bifurcation_outflow;
+    void* bifurcation_cross_section_depth;
+    /* optional, REAL [P (LB + 3)]: the limited level flows [P, LB], their
+     * sums, and the outgoing terms at the upstream and downstream ends, as the
+     * sub-step formed them (NULL: the adjoint recomputes them) */
+    void* path_terms;
 } cmf_tape;
 int cmf_hip_substep_taped(cmf_network* net, const cmf_fields* fields, int32_t substep, int32_t store_inflow,
                           const cmf_tape* tape, void* stream);

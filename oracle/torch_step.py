@@ -1,5 +1,5 @@
-"""Plain PyTorch-op transcription of one sub-step of the base physics, for
-FP64 autograd (the checker of the hand-written adjoint).
+"""Plain PyTorch-op transcription of one sub-step of the base physics and the
+bifurcation paths, for FP64 autograd (the checker of the hand-written adjoint).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, never by the product package.
 
@@ -11,10 +11,13 @@ Follows the same reference arithmetic as cmf_oracle.cpp, with vector ops:
   storage    cmfgpu/phys/triton/storage.py:81-94 (update with negative carry)
   stage      cmfgpu/phys/triton/storage.py:105-181 (floodplain table scan,
              quadratic or linear inversion, river/flood split)
+  paths      cmfgpu/phys/triton/bifurcation.py:13-104 (per-level flow between
+             the end water surfaces, 5 % storage limiter, outgoing terms) and
+             :106-151 (level flows and the path net limited by the ends)
 Branch picks follow the forward kernels (max/min pick the first operand only
 when it is strictly larger/smaller), and every expression that is not taken
 is evaluated on a harmless stand-in so that its gradient is zero, not NaN.
-No levee, bifurcation or reservoir.
+No levee or reservoir.
 """
 from __future__ import annotations
 
@@ -41,13 +44,56 @@ def _maxz(x):
     return _pick(x > 0, x, torch.zeros_like(x))
 
 
+def _sqrt0(x):
+    """sqrt(x) for x >= 0 with a zero (not infinite) gradient at x = 0."""
+    pos = x > 0
+    return _pick(pos, torch.sqrt(_safe(pos, x)), torch.zeros_like(x))
+
+
+def _paths(st, prm, wse, tot, dt, g):
+    """Per-level path flows limited at 5 % of the smaller end storage, and the
+    outgoing terms at the upstream (pos) and downstream (neg) ends."""
+    ci, di = prm["bifurcation_catchment_idx"], prm["bifurcation_downstream_idx"]
+    bq, bcsd = st["bifurcation_outflow"], st["bifurcation_cross_section_depth"]
+    bel, bw, bman = prm["bifurcation_elevation"], prm["bifurcation_width"], prm["bifurcation_manning"]
+    w0, w1 = wse[ci], wse[di]
+    max_w = _pick(w0 > w1, w0, w1)
+    sx = (w0 - w1) / prm["bifurcation_length"]
+    inside = (sx > -0.005) & (sx < 0.005)
+    slope = _pick(inside, sx, torch.clamp(sx.detach(), -0.005, 0.005))
+    upd = _maxz(max_w[:, None] - bel)
+    a = _sqrt0(upd * bcsd)
+    b = _sqrt0(upd * 0.01)
+    sifd = _pick(a > b, a, b)
+    cond = sifd > 1e-5
+    s_s = _safe(cond, sifd)
+    unit = bq / bw
+    num = bw * (unit + g * dt * s_s * slope[:, None])
+    den = 1.0 + g * dt * (bman * bman) * torch.abs(unit) * (1.0 / (s_s * s_s * torch.pow(s_s, 1.0 / 3.0)))
+    q = _pick(cond, num / den, torch.zeros_like(num))
+    total = q.sum(dim=1)
+    t0, t1 = tot[ci], tot[di]
+    m = _pick(t0 < t1, t0, t1)
+    nz = total != 0
+    ratio = 0.05 * m / (torch.abs(_safe(nz, total)) * dt)
+    limit = _pick(nz & (ratio < 1.0), ratio, torch.ones_like(ratio))
+    qlim = q * limit[:, None]
+    total_l = total * limit
+    pos = _maxz(total_l) * dt
+    neg = -_pick(total_l < 0, total_l, torch.zeros_like(total_l)) * dt
+    return qlim, upd, pos, neg
+
+
 def substep(st: dict, prm: dict, dt: float, gravity: float = 9.8) -> dict:
     """One sub-step.  st: river_outflow, flood_outflow, river_cross_section_depth,
     flood_cross_section_depth, flood_cross_section_area, river_storage,
     flood_storage, river_depth, flood_depth.  prm: river_manning, river_height
     (may require grad), flood_manning, river_width, river_length,
     catchment_elevation, catchment_area, downstream_distance, runoff,
-    flood_depth_table [N, L], downstream_idx (int64)."""
+    flood_depth_table [N, L], downstream_idx (int64).  With paths, prm also
+    holds bifurcation_catchment_idx / bifurcation_downstream_idx (int64),
+    bifurcation_length [P] and bifurcation_elevation / _width / _manning
+    [P, LB], and st bifurcation_outflow / bifurcation_cross_section_depth."""
     dn = prm["downstream_idx"]
     n = dn.shape[0]
     idx = torch.arange(n, device=dn.device)
@@ -103,12 +149,19 @@ def substep(st: dict, prm: dict, dt: float, gravity: float = 9.8) -> dict:
     q_r = q_r * limit
     q_f = q_f * limit
 
+    paths = "bifurcation_catchment_idx" in prm
+    if paths:
+        qlim, b_upd, b_pos, b_neg = _paths(st, prm, wse, tot, dt, g)
+        ci, di = prm["bifurcation_catchment_idx"], prm["bifurcation_downstream_idx"]
+
     # ------------------------------------------------- limiter and inflow --
     pos = (_maxz(q_r) + _maxz(q_f)) * dt
     neg_part = _pick(q_r < 0, q_r, torch.zeros_like(q_r)) + _pick(q_f < 0, q_f, torch.zeros_like(q_f))
     neg = _pick(mouth, torch.zeros_like(q_r), -neg_part * dt)
     nm = ~mouth
     og = pos.index_add(0, dn[nm], neg[nm])
+    if paths:
+        og = og.index_add(0, ci, b_pos).index_add(0, di, b_neg)
     rate = rs + fs
     og_ok = og > 1e-8
     ratio_l = rate / _safe(og_ok, og)
@@ -119,11 +172,18 @@ def substep(st: dict, prm: dict, dt: float, gravity: float = 9.8) -> dict:
     zero = torch.zeros_like(rout)
     rin = zero.index_add(0, dn[nm], rout[nm])
     fin = zero.index_add(0, dn[nm], fout[nm])
+    gbif = zero
+    if paths:
+        l0, l1 = lim[ci], lim[di]
+        raw = qlim.sum(dim=1)
+        net = raw * _pick(raw >= 0, l0, l1)
+        gbif = zero.index_add(0, ci, net).index_add(0, di, -net)
+        b_next = qlim * _pick(qlim >= 0, l0[:, None], l1[:, None])
 
     # ---------------------------------------------------- storage update --
     sa = rs + (rin - rout) * dt
     carry = sa < 0
-    sb = fs + _pick(carry, sa, zero) + (fin - fout) * dt
+    sb = fs + _pick(carry, sa, zero) + (fin - fout - gbif) * dt
     rsu1 = _maxz(sa)
     rsu2 = _pick(sb < 0, _maxz(rsu1 + sb), rsu1)
     fsu1 = _maxz(sb)
@@ -163,7 +223,11 @@ def substep(st: dict, prm: dict, dt: float, gravity: float = 9.8) -> dict:
     rsf = _pick(wet & (cap < total), cap, total)
     rdep = rsf / (r_len * r_wid)
     fsf = _maxz(total - rsf)
-    return {"river_outflow": rout, "flood_outflow": fout,
-            "river_cross_section_depth": upd_rcsd, "flood_cross_section_depth": upd_fcsd,
-            "flood_cross_section_area": upd_fcsa, "river_storage": rsf, "flood_storage": fsf,
-            "river_depth": rdep, "flood_depth": fdep}
+    out = {"river_outflow": rout, "flood_outflow": fout,
+           "river_cross_section_depth": upd_rcsd, "flood_cross_section_depth": upd_fcsd,
+           "flood_cross_section_area": upd_fcsa, "river_storage": rsf, "flood_storage": fsf,
+           "river_depth": rdep, "flood_depth": fdep}
+    if paths:
+        out["bifurcation_outflow"] = b_next
+        out["bifurcation_cross_section_depth"] = b_upd
+    return out

@@ -6,7 +6,9 @@ Problem: plain network (3,000 catchments; the autograd check on a
 20,000-catchment network), 16 gauges (largest upstream area), loss = mean
 squared log-discharge error over 8 outer steps against a run whose Manning's
 n and river height were scaled by lognormal(0, 0.2) factors; checkpoints
-every 3 outer steps (so the backward re-runs segments).
+every 3 outer steps (so the backward re-runs segments).  The same checks run
+with bifurcation paths open (100 paths on 3,000 catchments, 600 on 20,000);
+the central differences then take half their parameters at path ends.
 
 Tolerances: FP64 adjoint vs FP64 autograd, relative L2 of each gradient 1e-6
 (the two sum floats in different orders); central differences (h = 1e-3
@@ -28,15 +30,16 @@ PLAIN = ("base", "adaptive_time")
 STEPS = 8
 
 
-def _setup(precision, n=3000):
+def _setup(precision, n=3000, paths=0):
     from cmfgpu_amd import synthetic
     from cmfgpu_amd.adjoint import GaugeProblem
     from cmfgpu_amd.models import CaMaFlood
-    net = synthetic.generate_network(n, 1003)
+    net = synthetic.generate_network(n, 1003, paths=paths)
     src = dict(net.fields)
     src["runoff"] = synthetic.Runoff(src["catchment_area"], 2003).next_day() * np.float32(4.0)
     gauges = np.argsort(net.upstream_area)[-16:]
-    model = CaMaFlood(src, opened_modules=PLAIN, device="cuda:0", step_duration=1800.0, precision=precision)
+    mods = PLAIN + (("bifurcation",) if paths else ())
+    model = CaMaFlood(src, opened_modules=mods, device="cuda:0", step_duration=1800.0, precision=precision)
     rng = np.random.default_rng(7)
     n0, h0 = src["river_manning"].astype(np.float64), src["river_height"].astype(np.float64)
     n_true = n0 * rng.lognormal(0.0, 0.2, n0.shape[0])
@@ -64,6 +67,13 @@ def _torch_gradient(model, problem, gauges, n0, h0):
         "flood_manning", "river_width", "river_length", "catchment_elevation", "catchment_area",
         "downstream_distance", "runoff", "flood_depth_table")}
     prm["downstream_idx"] = torch.tensor(model.get("downstream_idx"), dtype=torch.int64)
+    if problem.paths:
+        for k in ("bifurcation_outflow", "bifurcation_cross_section_depth"):
+            st[k] = torch.tensor(model.get(k), dtype=torch.float64)
+        for k in ("bifurcation_length", "bifurcation_elevation", "bifurcation_width", "bifurcation_manning"):
+            prm[k] = torch.tensor(model.get(k), dtype=torch.float64)
+        for k in ("bifurcation_catchment_idx", "bifurcation_downstream_idx"):
+            prm[k] = torch.tensor(model.get(k), dtype=torch.int64)
     n = torch.tensor(n0, dtype=torch.float64, requires_grad=True)
     h = torch.tensor(h0, dtype=torch.float64, requires_grad=True)
     prm["river_manning"], prm["river_height"] = n, h
@@ -87,8 +97,9 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def test_adjoint_matches_fp64_autograd():
-    net, src, model, problem, gauges = _setup("float64", n=20000)
+@pytest.mark.parametrize("paths", [0, 600])
+def test_adjoint_matches_fp64_autograd(paths):
+    net, src, model, problem, gauges = _setup("float64", n=20000, paths=paths)
     loss = problem.run()
     assert loss > 0 and len(problem.checkpoints) == 3
     g_n, g_h = (x.cpu().numpy() for x in problem.gradient())
@@ -102,18 +113,32 @@ def test_adjoint_matches_fp64_autograd():
     model.close()
 
 
-def test_adjoint_against_central_differences():
-    net, src, model, problem, gauges = _setup("float64")
+def _pick_checks(rng, g, ends, count):
+    """`count` indices whose gradient exceeds 1e-4 of the largest; with path
+    ends given, half of them among the path ends."""
+    cand = np.flatnonzero(np.abs(g) > 1e-4 * np.abs(g).max())
+    if ends is None:
+        return list(rng.choice(cand, min(count, cand.size), replace=False))
+    at_ends = np.intersect1d(cand, ends)
+    first = rng.choice(at_ends, min(count // 2, at_ends.size), replace=False)
+    rest = np.setdiff1d(cand, first)
+    return list(first) + list(rng.choice(rest, count - first.size, replace=False))
+
+
+@pytest.mark.parametrize("paths", [0, 100])
+def test_adjoint_against_central_differences(paths):
+    net, src, model, problem, gauges = _setup("float64", paths=paths)
     problem.run()
     counts = list(problem.counts)
     g_n, g_h = (x.cpu().numpy() for x in problem.gradient())
     n0, h0 = (x.astype(np.float64) for x in problem.parameters())
     rng = np.random.default_rng(11)
-    cand_n = np.flatnonzero(np.abs(g_n) > 1e-4 * np.abs(g_n).max())
-    cand_h = np.flatnonzero(np.abs(g_h) > 1e-4 * np.abs(g_h).max())
-    checks = [("n", i) for i in rng.choice(cand_n, min(32, cand_n.size), replace=False)] + \
-             [("h", i) for i in rng.choice(cand_h, min(32, cand_h.size), replace=False)]
-    assert len(checks) == 64, (cand_n.size, cand_h.size)
+    ends = None
+    if paths:
+        ends = np.union1d(src["bifurcation_catchment_idx"], src["bifurcation_downstream_idx"])
+    checks = [("n", i) for i in _pick_checks(rng, g_n, ends, 32)] + \
+             [("h", i) for i in _pick_checks(rng, g_h, ends, 32)]
+    assert len(checks) == 64, len(checks)
     loss0 = problem.run(fixed_counts=counts)
     rel, kink = [], []
     for kind, i in checks:
@@ -147,13 +172,74 @@ def test_adjoint_against_central_differences():
     model.close()
 
 
-def test_fp32_adjoint_close_to_fp64():
+@pytest.mark.parametrize("paths", [0, 100])
+def test_fp32_adjoint_close_to_fp64(paths):
     grads = {}
     for prec in ("float64", "float32"):
-        net, src, model, problem, gauges = _setup(prec)
+        net, src, model, problem, gauges = _setup(prec, paths=paths)
         problem.run()
         grads[prec] = [x.cpu().numpy() for x in problem.gradient()]
         model.close()
     for a, b in zip(grads["float32"], grads["float64"]):
         print(f"FP32 adjoint vs FP64 adjoint: {_rel(a, b):.2e}", flush=True)
         assert _rel(a, b) <= 5e-2, _rel(a, b)
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_recomputed_intermediates_match_the_tape(precision):
+    """One sub-step's adjoint from the taped forward intermediates (unlimited
+    flows, limiters, path terms) and again with them recomputed from the saved
+    state (passes A0, A1, B0): bit-identical adjoints and gradients, so the
+    reverse re-forms the forward's values exactly (network with paths)."""
+    import ctypes
+    from cmfgpu_amd import _lib
+    from cmfgpu_amd.adjoint import (CmfAdjoint, CmfTape, PATH_STATE, STATE, TAPE_REAL, TAPE_STO, _declare)
+    net, src, model, problem, gauges = _setup(precision, paths=100)
+    for _ in range(3):
+        model.step_advance()
+    _declare()
+    lib = _lib.lib()
+    dev, n, P = model.device, model.N, model.P
+    real_t, sto_t = model.t["river_outflow"].dtype, model.t["river_storage"].dtype
+    pe = P * model.t["bifurcation_outflow"].shape[-1]
+    tr = torch.empty((len(TAPE_REAL), n), dtype=real_t, device=dev)
+    ts = torch.empty((len(TAPE_STO), n), dtype=sto_t, device=dev)
+    tp = torch.empty((len(PATH_STATE), pe), dtype=real_t, device=dev)
+    tt = torch.empty(pe + 3 * P, dtype=real_t, device=dev)
+    ptrs = {name: tr[x].data_ptr() for x, name in enumerate(TAPE_REAL)}
+    ptrs.update({name: ts[x].data_ptr() for x, name in enumerate(TAPE_STO)})
+    ptrs.update({name: tp[x].data_ptr() for x, name in enumerate(PATH_STATE)})
+    ptrs["path_terms"] = tt.data_ptr()
+    count = 6
+    model.t["time_step"].fill_(model.step_duration / count)
+    _lib.check(lib.cmf_hip_substep_taped(model.network_handle, model.field_table_ref(), 2, 0,
+                                         ctypes.byref(CmfTape(**ptrs)), _lib.current_stream()), "taped")
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    lout = [torch.randn(n, generator=gen, dtype=torch.float64).to(dev, real_t) for _ in STATE]
+    lout_p = [torch.randn(pe, generator=gen, dtype=torch.float64).to(dev, real_t) for _ in PATH_STATE]
+    results = []
+    for taped in (True, False):
+        lin = [torch.zeros(n, dtype=real_t, device=dev) for _ in STATE]
+        lin_p = [torch.zeros(pe, dtype=real_t, device=dev) for _ in PATH_STATE]
+        g_n = torch.zeros(n, dtype=torch.float64, device=dev)
+        g_h = torch.zeros(n, dtype=torch.float64, device=dev)
+        p = dict(ptrs)
+        if not taped:
+            for k in ("qraw_r", "qraw_f", "lim", "path_terms"):
+                p[k] = None
+        a = CmfAdjoint(**p)
+        for x in range(len(STATE)):
+            a.adj_out[x] = lout[x].data_ptr()
+            a.adj_in[x] = lin[x].data_ptr()
+        for x in range(len(PATH_STATE)):
+            a.adj_out_path[x] = lout_p[x].data_ptr()
+            a.adj_in_path[x] = lin_p[x].data_ptr()
+        a.grad_river_manning, a.grad_river_height = g_n.data_ptr(), g_h.data_ptr()
+        a.time_step = model.step_duration / count
+        _lib.check(lib.cmf_hip_substep_adjoint(model.network_handle, model.field_table_ref(), ctypes.byref(a),
+                                               _lib.current_stream()), "adjoint")
+        results.append([x.cpu().numpy() for x in lin + lin_p + [g_n, g_h]])
+    for x, y in zip(*results):
+        assert np.array_equal(x, y)
+    assert np.count_nonzero(results[0][len(STATE)]) > 10   # the path flows' adjoint
+    model.close()
