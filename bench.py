@@ -342,9 +342,9 @@ def graph_line(args, device: str) -> dict:
 
 
 def adjoint_line(args, device: str) -> dict:
-    """Cost of the adjoint on the 6-arcmin network with the differentiated
-    modules, the base physics and its 40,000 bifurcation paths (its levees
-    left closed): 2,000 gauges with upstream area > 5,000 km^2,
+    """Cost of the adjoint on the 6-arcmin network with all its modules: the
+    base physics, 40,000 bifurcation paths and levees on 5 % of the
+    catchments: 2,000 gauges with upstream area > 5,000 km^2,
     `--adjoint-steps` outer steps, checkpoints every 48.  Forward time per
     sub-step against backward time (segment re-run + reverse sweep) per
     sub-step."""
@@ -354,9 +354,9 @@ def adjoint_line(args, device: str) -> dict:
     from cmfgpu_amd.adjoint import GaugeProblem
     from cmfgpu_amd.models import CaMaFlood
     net = synthetic.build_config(3, scale=args.scale)
-    src = {k: v for k, v in net.fields.items() if not k.startswith("levee_")}
+    src = dict(net.fields)
     src["runoff"] = synthetic.Runoff(src["catchment_area"], 2003).next_day()
-    model = CaMaFlood(src, opened_modules=("base", "adaptive_time", "bifurcation"), device=device,
+    model = CaMaFlood(src, opened_modules=("base", "adaptive_time", "bifurcation", "levee"), device=device,
                       step_duration=1800.0)
     gauges = net.gauges
     steps = args.adjoint_steps
@@ -378,8 +378,8 @@ def adjoint_line(args, device: str) -> dict:
     torch.cuda.synchronize()
     t_bwd = time.perf_counter() - t0
     sub = sum(problem.counts)
-    out = {"workload": f"{net.num_catchments} catchments, base physics and {net.num_paths} bifurcation paths, "
-                       f"{gauges.shape[0]} gauges",
+    out = {"workload": f"{net.num_catchments} catchments, {net.num_paths} bifurcation paths, {net.num_levees} "
+                       f"levees, {gauges.shape[0]} gauges",
            "outer_steps": steps, "substeps": sub, "forward_ms_per_substep": t_fwd / sub * 1e3,
            "backward_ms_per_substep": t_bwd / sub * 1e3, "ratio": round(t_bwd / t_fwd, 3),
            "gradient_finite": bool(torch.isfinite(g_n).all().item() and torch.isfinite(g_h).all().item())}

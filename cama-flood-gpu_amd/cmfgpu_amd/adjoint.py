@@ -16,9 +16,10 @@ sub-step's input state on the device, then sweeps the segment in reverse.
 The initial state and every other parameter are held fixed.
 
 ``GaugeLoss.apply(manning, height, problem)`` wraps this in a
-torch.autograd.Function.  The base physics and the bifurcation paths (whose
-per-level flows and depths join the state) are differentiated; levee and
-reservoir modules, trials and grouped networks are refused.
+torch.autograd.Function.  The base physics, the bifurcation paths (whose
+per-level flows and depths join the state) and the levee stage (whose
+protected storage and depth join the state) are differentiated; the
+reservoir module, trials and grouped networks are refused.
 """
 from __future__ import annotations
 
@@ -32,6 +33,7 @@ from . import _lib
 STATE = ("river_outflow", "flood_outflow", "river_cross_section_depth", "flood_cross_section_depth",
          "flood_cross_section_area", "river_storage", "flood_storage", "river_depth", "flood_depth")
 PATH_STATE = ("bifurcation_outflow", "bifurcation_cross_section_depth")
+LEVEE_STATE = ("protected_storage", "protected_depth")
 QR, QF = 0, 1
 
 
@@ -57,6 +59,9 @@ class CmfAdjoint(ctypes.Structure):
         ("adj_out_path", ctypes.c_void_p * 2),
         ("adj_in_path", ctypes.c_void_p * 2),
         ("path_terms", ctypes.c_void_p),
+    ] + [(name, ctypes.c_void_p) for name in LEVEE_STATE] + [
+        ("adj_out_levee", ctypes.c_void_p * 2),
+        ("adj_in_levee", ctypes.c_void_p * 2),
     ]
 
 
@@ -66,7 +71,8 @@ ADJ_DEFER, ADJ_PENDING, ADJ_FLUSH = 1, 2, 4
 class CmfTape(ctypes.Structure):
     """Mirror of struct cmf_tape."""
 
-    _fields_ = [(name, ctypes.c_void_p) for name in STATE + INTERMEDIATES + PATH_STATE + ("path_terms",)]
+    _fields_ = [(name, ctypes.c_void_p) for name in STATE + INTERMEDIATES + PATH_STATE + ("path_terms",)
+                + LEVEE_STATE]
 
 
 _SIG_DONE = False
@@ -98,14 +104,16 @@ class GaugeProblem:
     callable step -> array) and the checkpoint interval K."""
 
     def __init__(self, model, gauges, observed, runoff=None, checkpoint_every: int = 48, eps: float = 1e-3):
-        extra = set(model.opened_modules) - {"base", "adaptive_time", "bifurcation"}
+        extra = set(model.opened_modules) - {"base", "adaptive_time", "bifurcation", "levee"}
         if extra:
-            raise ValueError(f"the adjoint covers the base physics and bifurcation; modules {sorted(extra)} are open")
+            raise ValueError(f"the adjoint covers the base physics, bifurcation and levee; modules {sorted(extra)} "
+                             "are open")
         if model.T != 1 or model.G or model.batched_names or model.mode != "fused":
             raise ValueError("the adjoint needs one trial, an ungrouped network, no per-trial parameters and "
                              "the fused schedule")
         self.model = model
         self.paths = "bifurcation" in model.opened_modules and model.P > 0
+        self.levees = "levee" in model.opened_modules
         self.gauges = np.asarray(gauges, dtype=np.int64)
         self.obs = torch.as_tensor(np.asarray(observed, dtype=np.float64), device=model.device)
         if self.obs.ndim != 2 or self.obs.shape[1] != self.gauges.shape[0]:
@@ -123,7 +131,7 @@ class GaugeProblem:
 
     # ------------------------------------------------------------ state --
     def _state(self):
-        names = STATE + (PATH_STATE if self.paths else ())
+        names = STATE + (PATH_STATE if self.paths else ()) + (LEVEE_STATE if self.levees else ())
         return [self.model.t[name] for name in names]
 
     def _snapshot(self):
@@ -216,6 +224,8 @@ class GaugeProblem:
         lam = [[torch.zeros(n, dtype=real_t, device=dev) for _ in STATE] for _ in range(2)]
         pe = m.P * m.t["bifurcation_outflow"].shape[-1] if self.paths else 0
         lam_p = [[torch.zeros(pe, dtype=real_t, device=dev) for _ in PATH_STATE] for _ in range(2)]
+        lam_l = [[torch.zeros(n if self.levees else 0, dtype=real_t, device=dev) for _ in LEVEE_STATE]
+                 for _ in range(2)]
         g_n = torch.zeros(n, dtype=torch.float64, device=dev)
         g_h = torch.zeros(n, dtype=torch.float64, device=dev)
         starts = sorted(self.checkpoints)
@@ -226,6 +236,9 @@ class GaugeProblem:
         # the path intermediates of each sub-step: limited level flows, level
         # sums, outgoing terms at the two ends
         tape_pt = torch.empty((slots, pe + 3 * m.P), dtype=real_t, device=dev) if self.paths else None
+        # with levees the input protected storage (STO) and depth (REAL)
+        tape_ls = torch.empty((slots, n), dtype=sto_t, device=dev) if self.levees else None
+        tape_ld = torch.empty((slots, n), dtype=real_t, device=dev) if self.levees else None
         tapes, adjs = [], []
         for k in range(slots):
             ptrs = {name: tape_r[k, x].data_ptr() for x, name in enumerate(TAPE_REAL)}
@@ -233,6 +246,9 @@ class GaugeProblem:
             if self.paths:
                 ptrs.update({name: tape_p[k, x].data_ptr() for x, name in enumerate(PATH_STATE)})
                 ptrs["path_terms"] = tape_pt[k].data_ptr()
+            if self.levees:
+                ptrs["protected_storage"] = tape_ls[k].data_ptr()
+                ptrs["protected_depth"] = tape_ld[k].data_ptr()
             tp = CmfTape(**ptrs)
             pair = []
             for parity in range(2):
@@ -244,6 +260,10 @@ class GaugeProblem:
                     for x in range(len(PATH_STATE)):
                         a.adj_out_path[x] = lam_p[parity][x].data_ptr()
                         a.adj_in_path[x] = lam_p[1 - parity][x].data_ptr()
+                if self.levees:
+                    for x in range(len(LEVEE_STATE)):
+                        a.adj_out_levee[x] = lam_l[parity][x].data_ptr()
+                        a.adj_in_levee[x] = lam_l[1 - parity][x].data_ptr()
                 a.grad_river_manning = g_n.data_ptr()
                 a.grad_river_height = g_h.data_ptr()
                 pair.append(a)
@@ -288,7 +308,7 @@ class GaugeProblem:
         fl.mode = ADJ_FLUSH
         _lib.check(lib.cmf_hip_substep_adjoint(m.network_handle, m.field_table_ref(), ctypes.byref(fl),
                                                _lib.current_stream()), "cmf_hip_substep_adjoint")
-        del tape_r, tape_s, tape_p, tape_pt
+        del tape_r, tape_s, tape_p, tape_pt, tape_ls, tape_ld
         self._restore(self.final)
         inv = torch.as_tensor(m.inv, device=dev)
         return g_n[inv], g_h[inv]

@@ -2,11 +2,13 @@
 // physics: local-inertial river and floodplain outflow
 // (cmfgpu/phys/triton/outflow.py:15-315), the storage limiter and inflow sums
 // (:318-383), the storage update and flood-stage diagnosis
-// (cmfgpu/phys/triton/storage.py:16-193), and of the bifurcation paths
+// (cmfgpu/phys/triton/storage.py:16-193), of the bifurcation paths
 // (cmfgpu/phys/triton/bifurcation.py:13-151: per-level local-inertial flow,
 // the 5 % storage limiter of a path, the path terms of the outgoing totals and
-// the path nets limited by the end catchments).  Levee and reservoir modules
-// are not differentiated.
+// the path nets limited by the end catchments; with levees the variant of
+// cmfgpu/phys/triton/levee.py:565-684) and of the levee stage
+// (cmfgpu/phys/triton/levee.py:62-304 with the partition :18-59).  The
+// reservoir module is not differentiated.
 //
 // The sub-step maps the state x = (river/flood outflow, river/flood/flood-area
 // cross-section depth, river/flood storage, river/flood depth) and the
@@ -65,8 +67,14 @@ struct AdjView {
     // upstream (pos) and downstream (neg) end
     REAL *p_qlim, *p_raw, *p_pos, *p_neg;
     // A2: adjoint of each catchment's bifurcation total; B2: adjoint of the
-    // end water surfaces and total storages
-    REAL *a_gb, *p_w0, *p_w1, *p_t0, *p_t1;
+    // end water surfaces, total storages and (levee ends) protected depths
+    REAL *a_gb, *p_w0, *p_w1, *p_t0, *p_t1, *p_d0, *p_d1;
+    // levees (lev: the module is open): protected storage (STO) and
+    // protected depth join the state
+    bool lev;
+    const void* ps_raw;
+    const REAL *pd, *lout_ps, *lout_pd;
+    REAL *lin_ps, *lin_pd;
 };
 
 constexpr int kAdjMaxLevels = 8;
@@ -105,7 +113,7 @@ __device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v
     const REAL dn_dist = v.dist[i];
     const REAL rs = (REAL)adj_sto<STO>(a.rs_raw, i);
     const REAL fs = (REAL)adj_sto<STO>(a.fs_raw, i);
-    const REAL ps = (REAL)v.protected_storage[i];
+    const REAL ps = a.lev ? (REAL)adj_sto<STO>(a.ps_raw, i) : (REAL)v.protected_storage[i];
     const REAL r_csd = a.rcsd[i], f_csd = a.fcsd[i], f_csa = a.fcsa[i];
 
     const REAL river_elevation = c_elv - r_hgt;
@@ -180,6 +188,20 @@ __device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v
     return o;
 }
 
+// protected storage of catchment i in the saved state (with levees) or the
+// field (without: +0.0 after every stage pass)
+template <class REAL, class STO>
+__device__ __forceinline__ REAL adj_ps(const View<REAL, STO>& v, const AdjView<REAL>& a, int64_t i) {
+    return a.lev ? (REAL)adj_sto<STO>(a.ps_raw, i) : (REAL)v.protected_storage[i];
+}
+// protected water surface of catchment i from its river surface wse, as
+// protected_surface_of forms it (levee.py:565-600)
+template <class REAL, class STO>
+__device__ __forceinline__ REAL adj_pwse(const View<REAL, STO>& v, const AdjView<REAL>& a, int64_t i, REAL wse) {
+    if (!v.is_levee[i]) return wse;
+    return vmin(v.elev[i] + a.pd[i], wse);
+}
+
 // ------------------------------------------------------- path trace ----
 // The arithmetic of bif_path (non-levee variant, end surfaces and totals
 // formed from the state) for path p on the saved input state, same
@@ -204,10 +226,12 @@ __device__ __forceinline__ void path_trace(const View<REAL, STO>& v, const AdjVi
     const REAL length = v.bif_length[p];
     o.w0 = a.rd[ci] + (v.elev[ci] - v.river_height[ci]);
     o.w1 = a.rd[di] + (v.elev[di] - v.river_height[di]);
-    o.ts0 = (REAL)adj_sto<STO>(a.rs_raw, ci) + (REAL)adj_sto<STO>(a.fs_raw, ci) + (REAL)v.protected_storage[ci];
-    o.ts1 = (REAL)adj_sto<STO>(a.rs_raw, di) + (REAL)adj_sto<STO>(a.fs_raw, di) + (REAL)v.protected_storage[di];
+    o.ts0 = (REAL)adj_sto<STO>(a.rs_raw, ci) + (REAL)adj_sto<STO>(a.fs_raw, ci) + adj_ps(v, a, ci);
+    o.ts1 = (REAL)adj_sto<STO>(a.rs_raw, di) + (REAL)adj_sto<STO>(a.fs_raw, di) + adj_ps(v, a, di);
     o.max_own = (o.w0 > o.w1 || o.w1 != o.w1);
     o.max_w = vmax(o.w0, o.w1);
+    REAL max_pw = (REAL)0;
+    if (a.lev) max_pw = vmax(adj_pwse(v, a, ci, o.w0), adj_pwse(v, a, di, o.w1));
     const REAL sx = (o.w0 - o.w1) / length;
     o.slope_in = (sx > (REAL)-0.005) && (sx < (REAL)0.005);
     o.slope = vclamp(sx, (REAL)-0.005, (REAL)0.005);
@@ -217,9 +241,21 @@ __device__ __forceinline__ void path_trace(const View<REAL, STO>& v, const AdjVi
         if (l >= LB) break;
         const int64_t k = p * LB + l;
         const REAL csd_old = a.bcsd[k];
-        const REAL upd = vmax(o.max_w - v.bif_elev[k], (REAL)0.0);
-        const REAL sa = dsqrt(upd * csd_old), sb = dsqrt(upd * (REAL)0.01);
-        const REAL sifd = vmax(sa, sb);
+        REAL upd, sa, sb = (REAL)0, sifd;
+        if (!a.lev) {
+            upd = vmax(o.max_w - v.bif_elev[k], (REAL)0.0);
+            sa = dsqrt(upd * csd_old);
+            sb = dsqrt(upd * (REAL)0.01);
+            sifd = vmax(sa, sb);
+        } else if (l == 0) {
+            upd = vmax(o.max_w - v.bif_elev[k], (REAL)0.0);
+            sa = dsqrt(upd * csd_old);
+            sifd = (sa <= (REAL)0.0) ? upd : sa;
+        } else {
+            upd = vmax(max_pw - v.bif_elev[k], (REAL)0.0);
+            sa = (REAL)0;
+            sifd = upd;
+        }
         const REAL w = v.bif_width[k], man = v.bif_manning[k];
         const REAL unit = a.bq[k] / w;
         REAL q = (REAL)0.0, den = (REAL)1;
@@ -275,6 +311,107 @@ __device__ __forceinline__ STO adj_path_terms(const View<REAL, STO>& v, const Ad
         acc += (STO)((code & 1) ? a.p_neg[code >> 1] : a.p_pos[code >> 1]);
     }
     return acc;
+}
+
+// ---------------------------------------------------- levee reverse ----
+// Reverse of levee_one_p (levee.py:62-304, partition :18-59) from its trace.
+// On entry Lr, Lf, Lrd, Lfd hold the adjoints of the levee stage's river and
+// flood storage and depth (Lp, Lpd: protected storage and depth); on return
+// Lr, Lf, Lrd, Lfd hold the adjoints of its inputs (the flood stage's river
+// and flood storage, river and flood depth) and Lrms that of the bankfull
+// storage.  Every storage the cases compare against (base, crown, protected
+// threshold, case-3 fill) is the bankfull storage plus terms of the fixed
+// profile, so each moves one for one with it.
+template <class REAL, class STO>
+__device__ __forceinline__ void levee_reverse(const LeveeTrace<REAL>& t, STO rs_hp, STO fs_hp, REAL rl, REAL rw,
+                                              REAL crown, double& Lr, double& Lf, double Lp, double& Lrd,
+                                              double& Lfd, double Lpd, double& Lrms) {
+    (void)crown;
+    const STO total_hp = rs_hp + fs_hp;
+    // the partition's picks
+    const bool keep_r = t.case4 || t.none;
+    const bool r_cand_pick = !keep_r && ((STO)t.r_cand < total_hp || total_hp != total_hp);
+    const STO r_hp = keep_r ? rs_hp : vmin((STO)t.r_cand, total_hp);
+    const bool rem_pos = total_hp - r_hp > (STO)0.0;
+    const STO remaining = vmax(total_hp - r_hp, (STO)0.0);
+    const STO f_cl = vmax((STO)t.f_cand, (STO)0.0);
+    const bool f_cl_pick = (f_cl < remaining || remaining != remaining);
+    const STO f_hp = t.case2 ? remaining : (t.none ? fs_hp : vmin(f_cl, remaining));
+    const bool p_pos = remaining - f_hp > (STO)0.0;
+    // p = max(remaining - f, 0)
+    double Lrem = 0.0, Lfh = Lf;
+    if (p_pos) {
+        Lrem += Lp;
+        Lfh -= Lp;
+    }
+    // f = case2 ? remaining : none ? fs : min(max(f_cand, 0), remaining)
+    double Lfc = 0.0, Lrs_in = 0.0, Lfs_in = 0.0;
+    if (t.case2) Lrem += Lfh;
+    else if (t.none) Lfs_in += Lfh;
+    else if (f_cl_pick) {
+        if ((STO)t.f_cand > (STO)0.0) Lfc += Lfh;
+    } else {
+        Lrem += Lfh;
+    }
+    // remaining = max(total - r, 0)
+    double Ltot_hp = 0.0, Lrh = Lr;
+    if (rem_pos) {
+        Ltot_hp += Lrem;
+        Lrh -= Lrem;
+    }
+    // r = case4 or none ? rs : min(r_cand, total)
+    double Lrc = 0.0;
+    if (keep_r) Lrs_in += Lrh;
+    else if (r_cand_pick) Lrc += Lrh;
+    else Ltot_hp += Lrh;
+    // f_cand = max(f_cand_raw, 0)
+    const double Lfr = (t.f_cand_raw > (REAL)0.0) ? Lfc : 0.0;
+    const double drl = (double)rl, drw = (double)rw;
+    double Ltotal = 0.0, Lrdep = 0.0, Lfdep = 0.0;
+    Lrms = 0.0;
+    if (t.case2) {
+        // f_dph = base + (total - base_sto) / (ld + rw) / rl;
+        // r_sto = rms + rl rw f_dph; f_cand_raw = total - r_sto; r_dph = r_sto / rl / rw
+        const double Lrsto = Lrc + Lrd / (drl * drw) - Lfr;
+        Ltotal += Lfr;
+        Lrms += Lrsto;
+        const double Lfd2 = Lfd + Lrsto * drl * drw;
+        const double c = 1.0 / (((double)t.levee_distance + drw) * drl);
+        Ltotal += Lfd2 * c;
+        Lrms -= Lfd2 * c;
+    } else if (t.case3) {
+        // r_sto = rms + rl rw crown; f_cand_raw = s_top - r_sto; r_dph = r_sto / rl / rw;
+        // f_dph = crown; p_dph from total - fill storage
+        const double Lrsto = Lrc + Lrd / (drl * drw) - Lfr;
+        Lrms += Lrsto + Lfr;
+        double Ladd = 0.0;
+        if (t.found_B) {
+            if (t.term_B > (REAL)0.0) {
+                const double g = (double)t.gradient_B;
+                Ladd = Lpd * g * 0.5 / sqrt((double)t.term_B) * 2.0 / drl / (g + 1e-9);
+            }
+        } else {
+            Ladd = Lpd / ((double)t.dwth_fil_B * drl + 1e-9);
+        }
+        Ltotal += Ladd;
+        if (t.fil_moves) Lrms -= Ladd;
+    } else if (t.case4) {
+        // f_cand_raw = s_top + (f_dph - crown) (ld + rw) rl - rs; depths: the flood
+        // stage's river and flood depth, protected depth = flood depth
+        Lrms += Lfr;
+        Lfdep += Lfr * ((double)t.levee_distance + drw) * drl;
+        Lrs_in -= Lfr;
+        Lrdep += Lrd;
+        Lfdep += Lfd + Lpd;
+    } else {
+        Lrdep += Lrd;
+        Lfdep += Lfd;
+    }
+    Ltot_hp += Ltotal;
+    Lr = Lrs_in + Ltot_hp;
+    Lf = Lfs_in + Ltot_hp;
+    Lrd = Lrdep;
+    Lfd = Lfdep;
 }
 
 // A0: unlimited outflows and outgoing terms of the saved state
@@ -352,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     }
     const REAL gb = (REAL)gsum;
     const STO rs = adj_sto<STO>(a.rs_raw, j), fs = adj_sto<STO>(a.fs_raw, j);
-    const STO ps = v.protected_storage[j];
+    const STO ps = a.lev ? adj_sto<STO>(a.ps_raw, j) : v.protected_storage[j];
     const REAL ro = v.runoff[j];
     // storage update (storage.py:81-94)
     const STO sa = rs + (STO)((rin - rout) * dt);
@@ -370,13 +507,16 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     const REAL rms = rl * rw * rh;
     const int L = LFIX > 0 ? LFIX : (int)v.L;
     double Ltot = 0.0, LH = 0.0;
-    const double Lrs_out = (double)a.lout[ADJ_RS][j], Lfs_out = (double)a.lout[ADJ_FS][j];
-    const double Lrd_out = (double)a.lout[ADJ_RD][j] + pend, Lfd_out = (double)a.lout[ADJ_FD][j];
-    double Lrsf = Lrs_out + Lrd_out / ((double)rl * (double)rw);
-    if (!(total > rms)) {
-        // dry: rsf = total, flood depth 0, flood storage max(0, 0)
-        Ltot += Lrsf;
-    } else {
+    double Lrs_out = (double)a.lout[ADJ_RS][j], Lfs_out = (double)a.lout[ADJ_FS][j];
+    double Lrd_out = (double)a.lout[ADJ_RD][j] + pend, Lfd_out = (double)a.lout[ADJ_FD][j];
+    // forward of the stage diagnosis: flood depth and river storage
+    const bool wet = total > rms;
+    TableRow<REAL, LFIX> lrow;
+    REAL fdep = (REAL)0.0;
+    double dfd_dtot = 0.0, dfd_dpts = 0.0;
+    bool cap_pick = false;
+    STO rsf = total_hp;
+    if (wet) {
         const REAL ca = v.catchment_area[j];
         const REAL* row = v.table + j * v.L;
         const REAL catchment_width = ca / rl;
@@ -398,8 +538,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
             prev_W = W;
         }
         const REAL ptw = rw + (REAL)level * winc;
-        REAL fdep;
-        double dfd_dtot, dfd_dpts;
         if (level == L) {
             fdep = pfd + (total - pts) / (ptw * rl);
             dfd_dtot = 1.0 / ((double)ptw * (double)rl);
@@ -414,8 +552,31 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
             dfd_dpts = -dfd_dtot;
         }
         const REAL cap = rms + rl * rw * fdep;
-        const bool cap_pick = ((STO)cap < total_hp);
-        const STO rsf = cap_pick ? (STO)cap : total_hp;
+        cap_pick = ((STO)cap < total_hp);
+        rsf = cap_pick ? (STO)cap : total_hp;
+    }
+    const int32_t lv = a.lev ? v.levee_index[j] : -1;
+    if (lv >= 0) {
+        // levee stage on the stage's output, then its reverse: the adjoints of
+        // the stage's river / flood storage and depths, and of the bankfull
+        // storage (so of the river height)
+        const STO fsf = vmax(total_hp - rsf, (STO)0.0);
+        const REAL rdep = (REAL)rsf / (rl * rw);
+        lrow.load(v.table + j * v.L);
+        LeveeTrace<REAL> tr;
+        (void)levee_one_p<REAL, STO, LFIX, true>(v, rl, rw, rh, v.catchment_area[j], v.levee_crown[lv],
+                                                 v.levee_frac[lv], v.levee_base[lv], rsf, fsf, fdep, rdep,
+                                                 (REAL)0.0, lrow, &tr);
+        double Lrms = 0.0;
+        levee_reverse<REAL, STO>(tr, rsf, fsf, rl, rw, v.levee_crown[lv], Lrs_out, Lfs_out,
+                                 (double)a.lout_ps[j], Lrd_out, Lfd_out, (double)a.lout_pd[j], Lrms);
+        LH += Lrms * (double)rl * (double)rw;
+    }
+    double Lrsf = Lrs_out + Lrd_out / ((double)rl * (double)rw);
+    if (!wet) {
+        // dry: rsf = total, flood depth 0, flood storage max(0, 0)
+        Ltot += Lrsf;
+    } else {
         if (total_hp - rsf > (STO)0.0) {
             Ltot += Lfs_out;
             Lrsf -= Lfs_out;
@@ -431,6 +592,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     }
     // total_hp = max(total_next, 0); total_next = rsu2 + fsu1 + ...
     const double Ltn = (total_next > (STO)0.0) ? Ltot : 0.0;
+    if (a.lev) a.lin_ps[j] = (REAL)Ltn;
     double Lrsu1 = 0.0, Lb = 0.0, La = 0.0;
     if (sb > (STO)0.0) Lb += Ltn;
     if (b_neg) {
@@ -548,10 +710,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
     // path values (every lane of the group forms the same ones)
     const REAL w0 = a.rd[ci] + (v.elev[ci] - v.river_height[ci]);
     const REAL w1 = a.rd[di] + (v.elev[di] - v.river_height[di]);
-    const REAL ts0 = (REAL)adj_sto<STO>(a.rs_raw, ci) + (REAL)adj_sto<STO>(a.fs_raw, ci) + (REAL)v.protected_storage[ci];
-    const REAL ts1 = (REAL)adj_sto<STO>(a.rs_raw, di) + (REAL)adj_sto<STO>(a.fs_raw, di) + (REAL)v.protected_storage[di];
+    const REAL ts0 = (REAL)adj_sto<STO>(a.rs_raw, ci) + (REAL)adj_sto<STO>(a.fs_raw, ci) + adj_ps(v, a, ci);
+    const REAL ts1 = (REAL)adj_sto<STO>(a.rs_raw, di) + (REAL)adj_sto<STO>(a.fs_raw, di) + adj_ps(v, a, di);
     const bool max_own = (w0 > w1 || w1 != w1);
     const REAL max_w = vmax(w0, w1);
+    // with levees the upper levels run on the protected surfaces
+    REAL pw0 = w0, pw1 = w1, max_pw = (REAL)0;
+    if (a.lev) {
+        pw0 = adj_pwse(v, a, ci, w0);
+        pw1 = adj_pwse(v, a, di, w1);
+        max_pw = vmax(pw0, pw1);
+    }
+    const bool upper = a.lev && l > 0;   // this lane's level on the protected surface
     const REAL sx = (w0 - w1) / length;
     const bool slope_in = (sx > (REAL)-0.005) && (sx < (REAL)0.005);
     const REAL slope = vclamp(sx, (REAL)-0.005, (REAL)0.005);
@@ -561,10 +731,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
     REAL csd_old = (REAL)0;
     if (lv) {
         csd_old = a.bcsd[k];
-        upd = vmax(max_w - v.bif_elev[k], (REAL)0.0);
-        sa = dsqrt(upd * csd_old);
-        sb = dsqrt(upd * (REAL)0.01);
-        sifd = vmax(sa, sb);
+        if (!a.lev) {
+            upd = vmax(max_w - v.bif_elev[k], (REAL)0.0);
+            sa = dsqrt(upd * csd_old);
+            sb = dsqrt(upd * (REAL)0.01);
+            sifd = vmax(sa, sb);
+        } else if (!upper) {
+            upd = vmax(max_w - v.bif_elev[k], (REAL)0.0);
+            sa = dsqrt(upd * csd_old);
+            sifd = (sa <= (REAL)0.0) ? upd : sa;
+        } else {
+            upd = vmax(max_pw - v.bif_elev[k], (REAL)0.0);
+            sifd = upd;
+        }
         const REAL w = v.bif_width[k], man = v.bif_manning[k];
         unit = a.bq[k] / w;
         if (sifd > (REAL)1e-5) {
@@ -612,7 +791,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
         Lsum = -Llimit * (double)ratio / as * sgn;
     }
     const double Lql = (Llq + Lsum_l) * (double)limit + Lsum;
-    double Lqold = 0.0, Lsifd = 0.0, Lcsd = 0.0, Lslope = 0.0, Lmax_w = 0.0;
+    double Lqold = 0.0, Lsifd = 0.0, Lcsd = 0.0, Lslope = 0.0, Lmax_w = 0.0, Lmax_pw = 0.0;
     if (lv) {
         if (sifd > (REAL)1e-5) {
             const double w = (double)v.bif_width[k], man = (double)v.bif_manning[k];
@@ -629,9 +808,20 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
             Lqold += Lden * g * dt * man * man * sgn * Q / w;
             Lsifd += Lden * g * dt * man * man * au * (-7.0 / 3.0) * Q / s;
         }
-        // sifd = max(sqrt(upd csd_old), sqrt(upd 0.01)); the next depth is upd
+        // the next depth is upd; sifd = max(sqrt(upd csd_old), sqrt(upd 0.01)),
+        // with levees sqrt(upd csd_old) (upd where that is not positive) at
+        // level 0 and upd above
         double Lupd = (double)a.lout_bcsd[k];
-        if (sa > sb || sb != sb) {
+        if (upper) {
+            Lupd += Lsifd;
+        } else if (a.lev) {
+            if (sa > (REAL)0.0) {
+                Lupd += Lsifd * 0.5 * (double)csd_old / (double)sa;
+                Lcsd += Lsifd * 0.5 * (double)upd / (double)sa;
+            } else {
+                Lupd += Lsifd;
+            }
+        } else if (sa > sb || sb != sb) {
             if (sa > (REAL)0.0) {
                 Lupd += Lsifd * 0.5 * (double)csd_old / (double)sa;
                 Lcsd += Lsifd * 0.5 * (double)upd / (double)sa;
@@ -639,15 +829,20 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
         } else if (sb > (REAL)0.0) {
             Lupd += Lsifd * 0.5 * 0.01 / (double)sb;
         }
-        // upd = max(max_w - elevation, 0)
-        if (max_w - v.bif_elev[k] > (REAL)0.0) Lmax_w += Lupd;
+        // upd = max(surface - elevation, 0)
+        if (upper) {
+            if (max_pw - v.bif_elev[k] > (REAL)0.0) Lmax_pw += Lupd;
+        } else if (max_w - v.bif_elev[k] > (REAL)0.0) {
+            Lmax_w += Lupd;
+        }
         a.lin_bq[k] = (REAL)Lqold;
         a.lin_bcsd[k] = (REAL)Lcsd;
     }
-    double Ls = 0.0, Lx = 0.0;
+    double Ls = 0.0, Lx = 0.0, Lxp = 0.0;
     for (int m = 0; m < LB; ++m) {
         Ls += __shfl(Lslope, m, 8);
         Lx += __shfl(Lmax_w, m, 8);
+        Lxp += __shfl(Lmax_pw, m, 8);
     }
     if (l != 0) return;
     // slope = clamp((w0 - w1) / length, -0.005, 0.005); max_w = max(w0, w1)
@@ -659,6 +854,20 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
     }
     if (max_own) Lw0 += Lx;
     else Lw1 += Lx;
+    if (a.lev) {
+        // max_pw = max(pw0, pw1); pw = min(elevation + protected depth, wse)
+        // at a levee end, wse elsewhere
+        double Lp0 = 0.0, Lp1 = 0.0, Ld0 = 0.0, Ld1 = 0.0;
+        if (pw0 > pw1 || pw1 != pw1) Lp0 = Lxp;
+        else Lp1 = Lxp;
+        const REAL e0 = v.elev[ci] + a.pd[ci], e1 = v.elev[di] + a.pd[di];
+        if (v.is_levee[ci] && (e0 < w0 || w0 != w0)) Ld0 = Lp0;
+        else Lw0 += Lp0;
+        if (v.is_levee[di] && (e1 < w1 || w1 != w1)) Ld1 = Lp1;
+        else Lw1 += Lp1;
+        a.p_d0[p] = (REAL)Ld0;
+        a.p_d1[p] = (REAL)Ld1;
+    }
     a.p_w0[p] = (REAL)Lw0;
     a.p_w1[p] = (REAL)Lw1;
     a.p_t0[p] = (REAL)Lts0;
@@ -766,14 +975,21 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
         Lre -= Lupd_rcsd;
     }
     if (o.slope_in) Lslope += Lfslope;
+    double Lpd = 0.0;
     if (a.bif) {
-        // the paths of i read its water surface and total storage (B2)
+        // the paths of i read its water surface, total storage and (levee
+        // catchment) protected depth (B2)
         for (int32_t e = v.ev_ptr[i]; e < v.ev_ptr[i + 1]; ++e) {
             const int32_t code = v.ev_code[e];
             const int64_t p = code >> 1;
             Lwse += (double)((code & 1) ? a.p_w1[p] : a.p_w0[p]);
             Ltot += (double)((code & 1) ? a.p_t1[p] : a.p_t0[p]);
+            if (a.lev) Lpd += (double)((code & 1) ? a.p_d1[p] : a.p_d0[p]);
         }
+    }
+    if (a.lev) {
+        a.lin_ps[i] = (REAL)((double)a.lin_ps[i] + Ltot);
+        a.lin_pd[i] = (REAL)Lpd;
     }
     const double dist = (double)v.dist[i];
     Lwse += Lslope / dist;

@@ -87,7 +87,7 @@ struct cmf_network {
     // adjoint scratch (cmf_hip_substep_adjoint), allocated on first use:
     // 14 arrays of N elements of up to 8 bytes
     void* adj_scratch = nullptr;
-    void* adj_path_scratch = nullptr;   // with paths: [P, LB] + [P] x 7
+    void* adj_path_scratch = nullptr;   // with paths: [P, LB] + [P] x 9
 };
 
 namespace {
@@ -814,8 +814,8 @@ int run_substep_taped(cmf_network* net, const cmf_fields* f, int substep, int st
     if (rc) return rc;
     if ((rc = need_base(f, "substep_taped")) || (rc = need_features(f, "substep_taped"))) return rc;
     if (!tp) return set_error(CMF_ERR_INVALID, "substep_taped: null tape");
-    if (v.T != 1 || net->G > 0 || (f->flags & (CMF_HAS_LEVEE | CMF_HAS_RESERVOIR)))
-        return set_error(CMF_ERR_UNSUPPORTED, "substep_taped: base physics and bifurcation, one trial, ungrouped");
+    if (v.T != 1 || net->G > 0 || (f->flags & CMF_HAS_RESERVOIR))
+        return set_error(CMF_ERR_UNSUPPORTED, "substep_taped: no reservoir module, one trial, ungrouped");
     void* ptrs[] = {tp->river_outflow, tp->flood_outflow, tp->river_cross_section_depth,
                     tp->flood_cross_section_depth, tp->flood_cross_section_area, tp->river_storage,
                     tp->flood_storage, tp->river_depth, tp->flood_depth, tp->qraw_r, tp->qraw_f, tp->lim};
@@ -833,6 +833,13 @@ int run_substep_taped(cmf_network* net, const cmf_fields* f, int substep, int st
     v.qraw_r = static_cast<REAL*>(tp->qraw_r);
     v.qraw_f = static_cast<REAL*>(tp->qraw_f);
     v.lim = static_cast<REAL*>(tp->lim);
+    if (v.has(CMF_HAS_LEVEE)) {
+        if (!tp->protected_storage || !tp->protected_depth)
+            return set_error(CMF_ERR_INVALID, "substep_taped: the protected storage and depth slots are needed "
+                                              "with levees");
+        v.tape_ps = static_cast<STO*>(tp->protected_storage);
+        v.tape_pd = static_cast<REAL*>(tp->protected_depth);
+    }
     if (v.has(CMF_HAS_BIFURCATION) && v.P > 0) {
         if (!tp->bifurcation_outflow || !tp->bifurcation_cross_section_depth)
             return set_error(CMF_ERR_INVALID, "substep_taped: the path state slots are needed with bifurcation");
@@ -856,8 +863,8 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
     int rc = make_view<REAL, STO>(net, f, v);
     if (rc) return rc;
     if (!adj) return set_error(CMF_ERR_INVALID, "substep_adjoint: null adjoint table");
-    if (f->flags & (CMF_HAS_LEVEE | CMF_HAS_RESERVOIR))
-        return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: levee and reservoir are not differentiated");
+    if (f->flags & CMF_HAS_RESERVOIR)
+        return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: the reservoir module is not differentiated");
     if (v.T != 1 || net->G > 0 || f->batched)
         return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: one trial, ungrouped, no per-trial parameters");
     if ((rc = require({NEED(downstream_idx), NEED(river_manning), NEED(flood_manning), NEED(river_width),
@@ -890,11 +897,22 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
             return set_error(CMF_ERR_INVALID, "substep_adjoint: the path state and its adjoints are needed with "
                                               "bifurcation");
     }
+    const bool lev = v.has(CMF_HAS_LEVEE);
+    if (lev) {
+        if ((rc = require({NEED(is_levee), NEED(levee_index), NEED(levee_base_height), NEED(levee_crown_height),
+                           NEED(levee_fraction), NEED(protected_depth)}, "substep_adjoint")))
+            return rc;
+        if (!(adj->mode & CMF_ADJ_FLUSH) &&
+            (!adj->protected_storage || !adj->protected_depth || !adj->adj_out_levee[0] || !adj->adj_out_levee[1] ||
+             !adj->adj_in_levee[0] || !adj->adj_in_levee[1]))
+            return set_error(CMF_ERR_INVALID, "substep_adjoint: the protected storage and depth and their adjoints "
+                                              "are needed with levees");
+    }
     if (!net->adj_scratch && (rc = dev_alloc(&net->adj_scratch, (size_t)n * 8 * 14))) return rc;
     if (bif && !net->adj_path_scratch &&
-        (rc = dev_alloc(&net->adj_path_scratch, (size_t)v.P * 8 * (size_t)(v.LB + 7))))
+        (rc = dev_alloc(&net->adj_path_scratch, (size_t)v.P * 8 * (size_t)(v.LB + 9))))
         return rc;
-    AdjView<REAL> a;
+    AdjView<REAL> a{};
     a.q_r = static_cast<const REAL*>(adj->river_outflow);
     a.q_f = static_cast<const REAL*>(adj->flood_outflow);
     a.rcsd = static_cast<const REAL*>(adj->river_cross_section_depth);
@@ -918,6 +936,15 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
     a.a_rout = w[5]; a.a_fout = w[6]; a.a_ri = w[7]; a.a_fi = w[8]; a.a_qr = w[9]; a.a_qf = w[10];
     a.a_og = w[11]; a.d_wd = w[12]; a.a_gb = w[13];
     a.bif = bif;
+    a.lev = lev;
+    if (lev) {
+        a.ps_raw = adj->protected_storage;
+        a.pd = static_cast<const REAL*>(adj->protected_depth);
+        a.lout_ps = static_cast<const REAL*>(adj->adj_out_levee[0]);
+        a.lout_pd = static_cast<const REAL*>(adj->adj_out_levee[1]);
+        a.lin_ps = static_cast<REAL*>(adj->adj_in_levee[0]);
+        a.lin_pd = static_cast<REAL*>(adj->adj_in_levee[1]);
+    }
     if (bif) {
         a.bq = static_cast<const REAL*>(adj->bifurcation_outflow);
         a.bcsd = static_cast<const REAL*>(adj->bifurcation_cross_section_depth);
@@ -929,8 +956,8 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
         const int64_t pst = v.P * (int64_t)(8 / sizeof(REAL));
         a.p_qlim = pr;
         REAL* q = pr + v.LB * pst;
-        REAL** one[] = {&a.p_raw, &a.p_pos, &a.p_neg, &a.p_w0, &a.p_w1, &a.p_t0, &a.p_t1};
-        for (int k = 0; k < 7; ++k) *one[k] = q + k * pst;
+        REAL** one[] = {&a.p_raw, &a.p_pos, &a.p_neg, &a.p_w0, &a.p_w1, &a.p_t0, &a.p_t1, &a.p_d0, &a.p_d1};
+        for (int k = 0; k < 9; ++k) *one[k] = q + k * pst;
         if (adj->path_terms) {
             // the forward's own limited level flows, level sums and terms
             REAL* pt = const_cast<REAL*>(static_cast<const REAL*>(adj->path_terms));

@@ -170,6 +170,8 @@ struct View {
     STO *tape_rs, *tape_fs;
     REAL *tape_bq, *tape_bcsd;    // the path pass: input path flows and depths
     REAL* tape_bqlim;             // and its limited level flows
+    STO* tape_ps;                 // the outflow pass: input protected storage
+    REAL* tape_pd;                // and depth (levee module)
     // output statistics: sum over sub-steps of (river + flood outflow) * dt
     // per element (nullptr: off)
     double* acc_outflow;

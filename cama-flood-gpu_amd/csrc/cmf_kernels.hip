@@ -177,6 +177,10 @@ __device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int subste
         v.tape_fd[e] = f_dep;
         v.tape_rs[e] = v.river_storage[e];
         v.tape_fs[e] = v.flood_storage[e];
+        if (v.tape_ps) {
+            v.tape_ps[e] = v.protected_storage[e];
+            v.tape_pd[e] = v.protected_depth[e];
+        }
     }
     if (FUSED) {
         v.qraw_r[e] = q_r;
@@ -818,12 +822,24 @@ struct LeveeOut {
     REAL r_dph, f_dph, p_dph, f_frc;
 };
 
+// What the reverse of the levee stage needs of one evaluation: the case, the
+// storages the cases compare against, the case-3 operands, and whether the
+// case-3 fill storage moves with the bankfull storage.
+template <class REAL>
+struct LeveeTrace {
+    bool case2, case3, case4, none, found_B, fil_moves;
+    REAL total, rs_c, base_sto, s_top, levee_distance, dsto_fil_B, dwth_fil_B, gradient_B, term_B;
+    REAL r_cand, f_cand, f_cand_raw;
+};
+
 // Levee stage of catchment c (levee row lv) from its post-flood-stage state
-// (triton/levee.py:62-304 with the partition of :18-59).
-template <class REAL, class STO, int LFIX>
+// (triton/levee.py:62-304 with the partition of :18-59).  TR: also record
+// the evaluation in *tr (the reverse's recompute; the arithmetic is the same).
+template <class REAL, class STO, int LFIX, bool TR = false>
 __device__ __forceinline__ LeveeOut<REAL, STO> levee_one_p(
     const View<REAL, STO>& v, REAL rl, REAL rw, REAL rh, REAL ca, REAL crown, REAL lfrac, REAL base,
-    STO rs_hp, STO fs_hp, REAL fdep_curr, REAL rdep_curr, REAL ffr_curr, const TableRow<REAL, LFIX>& row) {
+    STO rs_hp, STO fs_hp, REAL fdep_curr, REAL rdep_curr, REAL ffr_curr, const TableRow<REAL, LFIX>& row,
+    LeveeTrace<REAL>* tr = nullptr) {
     const int L = LFIX > 0 ? LFIX : (int)v.L;
     const STO total_hp = rs_hp + fs_hp;
     const REAL rs_c = (REAL)rs_hp, fs_c = (REAL)fs_hp, total = (REAL)total_hp;
@@ -846,6 +862,12 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one_p(
         o.r_hp = rs_hp;
         o.f_hp = fs_hp;
         o.p_hp = vmax(remaining - fs_hp, (STO)0.0);
+        if (TR) {
+            tr->case2 = tr->case3 = tr->case4 = false;
+            tr->none = true;
+            tr->total = total;
+            tr->rs_c = rs_c;
+        }
         return o;
     }
     const REAL dwth_inc = (ca / rl) / (REAL)L;
@@ -865,7 +887,7 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one_p(
     bool ib_found = false;                                                         // base found by level ilev
     REAL ib_s = (REAL)0.0, ib_pre = (REAL)0.0, ib_seg = (REAL)1.0, ib_w = (REAL)0.0;
     REAL dsto_fil_B = (REAL)0.0, dwth_fil_B = (REAL)0.0, ddph_fil_B = (REAL)0.0, seg_B = (REAL)0.0;
-    bool found_B = false, B_at_ilev = false;
+    bool found_B = false, B_at_ilev = false, fil_moves = false;
 #pragma unroll
     for (int k = 0; k < (LFIX > 0 ? LFIX : 64); ++k) {
         if (LFIX == 0 && k >= L) break;
@@ -897,6 +919,7 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one_p(
                 dsto_fil_B = threshold;
                 dwth_fil_B = dwth_inc * (REAL)(k + 1) - levee_distance;
                 ddph_fil_B = depth_val - base;
+                fil_moves = true;
             }
         }
         s_curr = s_next;
@@ -918,6 +941,7 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one_p(
     if (B_at_ilev) {
         const REAL base_sto_ilev = ib_found ? interp_sto(base, ib_s, ib_pre, ib_seg, ib_w) : river_max_storage;
         dsto_fil_B = base_sto_ilev + (levee_distance + rw) * (crown - base) * rl;
+        fil_moves = true;
     }
     const REAL dhgt_dif = crown - base;
     const REAL s_top = base_sto + (levee_distance + rw) * dhgt_dif * rl;
@@ -926,7 +950,8 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one_p(
     const bool case2 = !case4 && !case3 && (total >= base_sto);
 
     LeveeOut<REAL, STO> o;
-    REAL r_cand = rs_c, f_cand = fs_c;
+    REAL r_cand = rs_c, f_cand = fs_c, f_cand_raw = fs_c;
+    REAL term_B_rec = (REAL)0;
     o.r_dph = rdep_curr;
     o.f_dph = fdep_curr;
     o.p_dph = (REAL)0.0;
@@ -937,7 +962,8 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one_p(
         const REAL f_dph_c2 = base + dsto_add_c2 / dwth_add_c2 / rl;
         const REAL r_sto_c2 = river_max_storage + rl * rw * f_dph_c2;
         r_cand = r_sto_c2;
-        f_cand = vmax(total - r_sto_c2, (REAL)0.0);
+        f_cand_raw = total - r_sto_c2;
+        f_cand = vmax(f_cand_raw, (REAL)0.0);
         o.r_dph = r_sto_c2 / rl / rw;
         o.f_dph = f_dph_c2;
         o.f_frc = lfrac;
@@ -946,6 +972,7 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one_p(
         REAL p_dph_B, f_frc_B;
         if (found_B) {
             const REAL term_B = dwth_fil_B * dwth_fil_B + (REAL)2.0 * dsto_add_B / rl / (gradient_B + (REAL)1e-9);
+            term_B_rec = term_B;
             const REAL dwth_add_B = -dwth_fil_B + dsqrt(vmax(term_B, (REAL)0.0));
             const REAL ddph_add_B = dwth_add_B * gradient_B;
             p_dph_B = base + ddph_fil_B + ddph_add_B;
@@ -957,7 +984,8 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one_p(
         }
         const REAL r_sto_c3 = river_max_storage + rl * rw * crown;
         r_cand = r_sto_c3;
-        f_cand = vmax(s_top - r_sto_c3, (REAL)0.0);
+        f_cand_raw = s_top - r_sto_c3;
+        f_cand = vmax(f_cand_raw, (REAL)0.0);
         o.r_dph = r_sto_c3 / rl / rw;
         o.f_dph = crown;
         o.p_dph = p_dph_B;
@@ -965,10 +993,31 @@ __device__ __forceinline__ LeveeOut<REAL, STO> levee_one_p(
     } else if (case4) {
         const REAL dsto_add_c4 = (fdep_curr - crown) * (levee_distance + rw) * rl;
         r_cand = rs_c;
-        f_cand = vmax(s_top + dsto_add_c4 - rs_c, (REAL)0.0);
+        f_cand_raw = s_top + dsto_add_c4 - rs_c;
+        f_cand = vmax(f_cand_raw, (REAL)0.0);
         o.p_dph = fdep_curr;
     }
     const bool none = !(case2 || case3 || case4);
+    if (TR) {
+        tr->case2 = case2;
+        tr->case3 = case3;
+        tr->case4 = case4;
+        tr->none = none;
+        tr->found_B = found_B;
+        tr->fil_moves = fil_moves;
+        tr->total = total;
+        tr->rs_c = rs_c;
+        tr->base_sto = base_sto;
+        tr->s_top = s_top;
+        tr->levee_distance = levee_distance;
+        tr->dsto_fil_B = dsto_fil_B;
+        tr->dwth_fil_B = dwth_fil_B;
+        tr->gradient_B = gradient_B;
+        tr->term_B = term_B_rec;
+        tr->r_cand = r_cand;
+        tr->f_cand = f_cand;
+        tr->f_cand_raw = f_cand_raw;
+    }
     STO r_hp = vmin((STO)r_cand, total_hp);
     r_hp = (case4 || none) ? rs_hp : r_hp;
     const STO remaining = vmax(total_hp - r_hp, (STO)0.0);

@@ -389,10 +389,12 @@ int cmf_network_set_outflow_accumulator(cmf_network* net, double* accumulator);
  * the flood table.  With the bifurcation module open the path state
  * (bifurcation_outflow, bifurcation_cross_section_depth: REAL [P, levels],
  * at most 8 levels) is part of the state: its saved input values and its
- * adjoints after (adj_out_path) and before (adj_in_path) the sub-step.
- * Levee and reservoir modules, trials, groups and per-trial parameters
- * return CMF_ERR_UNSUPPORTED.  No reference counterpart (the reference has
- * no differentiation). */
+ * adjoints after (adj_out_path) and before (adj_in_path) the sub-step; with
+ * the levee module open (cmfgpu/phys/triton/levee.py:62-304, and the path
+ * variant :565-684) so are the protected storage and depth, and the field
+ * table must hold levee_index.  The reservoir module, trials, groups and
+ * per-trial parameters return CMF_ERR_UNSUPPORTED.  No reference counterpart
+ * (the reference has no differentiation). */
 typedef struct cmf_adjoint {
     const void* river_outflow;
     const void* flood_outflow;
@@ -418,6 +420,13 @@ typedef struct cmf_adjoint {
     const void* adj_out_path[2];
     void* adj_in_path[2];
     const void* path_terms;   /* the forward's path intermediates (cmf_tape), or NULL */
+    /* levees: protected_storage (STO) and protected_depth (REAL) [N] join the
+     * state, with their adjoints after (adj_out_levee) and before
+     * (adj_in_levee) the sub-step */
+    const void* protected_storage;
+    const void* protected_depth;
+    const void* adj_out_levee[2];
+    void* adj_in_levee[2];
 } cmf_adjoint;
 /* mode bits for a sweep over consecutive sub-steps on one network:
  * DEFER leaves the last pass (the gather of downstream water-surface
@@ -432,8 +441,8 @@ int cmf_hip_substep_adjoint(cmf_network* net, const cmf_fields* fields, const cm
 /* One fused sub-step (as cmf_hip_substep) that records what the adjoint needs:
  * its input state (same nine fields and order as cmf_adjoint) and the
  * unlimited flows and limiters it forms; with the bifurcation module open
- * also the input path state [P, levels].  Base physics (and bifurcation),
- * one trial. */
+ * also the input path state [P, levels], with the levee module the input
+ * protected storage and depth.  No reservoir module; one trial. */
 typedef struct cmf_tape {
     void* river_outflow;
     void* flood_outflow;
@@ -453,6 +462,9 @@ typedef struct cmf_tape {
      * sums, and the outgoing terms at the upstream and downstream ends, as the
      * sub-step formed them (NULL: the adjoint recomputes them) */
     void* path_terms;
+    /* with levees: the input protected storage (STO) and depth (REAL) */
+    void* protected_storage;
+    void* protected_depth;
 } cmf_tape;
 int cmf_hip_substep_taped(cmf_network* net, const cmf_fields* fields, int32_t substep, int32_t store_inflow,
                           const cmf_tape* tape, void* stream);

@@ -13,11 +13,16 @@ Follows the same reference arithmetic as cmf_oracle.cpp, with vector ops:
              quadratic or linear inversion, river/flood split)
   paths      cmfgpu/phys/triton/bifurcation.py:13-104 (per-level flow between
              the end water surfaces, 5 % storage limiter, outgoing terms) and
-             :106-151 (level flows and the path net limited by the ends)
+             :106-151 (level flows and the path net limited by the ends); with
+             levees cmfgpu/phys/triton/levee.py:565-684 (level 0 on the river
+             surface, upper levels on the protected surface)
+  levee      cmfgpu/phys/triton/levee.py:62-304 with the partition :18-59 (the
+             floodplain scan for the storages at levee base and crown, cases
+             2-4, river / flood / protected split)
 Branch picks follow the forward kernels (max/min pick the first operand only
 when it is strictly larger/smaller), and every expression that is not taken
 is evaluated on a harmless stand-in so that its gradient is zero, not NaN.
-No levee or reservoir.
+No reservoir.
 """
 from __future__ import annotations
 
@@ -50,7 +55,100 @@ def _sqrt0(x):
     return _pick(pos, torch.sqrt(_safe(pos, x)), torch.zeros_like(x))
 
 
-def _paths(st, prm, wse, tot, dt, g):
+def _levee(prm, lidx, rs, fs, fdep_c, rdep_c, ffr_c):
+    """Levee stage of the levee catchments `lidx` from their flood-stage state:
+    river, flood and protected storage, river / flood / protected depth and
+    flood fraction.  Every case is formed and the forward's pick selects."""
+    rl, rw, rh = prm["river_length"][lidx], prm["river_width"][lidx], prm["river_height"][lidx]
+    ca = prm["catchment_area"][lidx]
+    crown, lfrac, base = prm["levee_crown_height"], prm["levee_fraction"], prm["levee_base_height"]
+    table = prm["flood_depth_table"][lidx]
+    L = table.shape[1]
+    total = rs + fs
+    rms = rl * rw * rh
+    dwth_inc = (ca / rl) / L
+    ld = lfrac * (ca / rl)
+    with torch.no_grad():
+        ilev = (lfrac * L).long()
+    s_curr = rms
+    dhgt_pre = torch.zeros_like(rms)
+    dwth_pre = rw
+    base_sto, fill_sto = rms, rms
+    found_base = torch.zeros_like(rms, dtype=torch.bool)
+    found_fill = torch.zeros_like(found_base)
+    found_b = torch.zeros_like(found_base)
+    dsto_fil_b = torch.zeros_like(rms)
+    dwth_fil_b = torch.zeros_like(rms)
+    ddph_fil_b = torch.zeros_like(rms)
+    gradient_b = torch.zeros_like(rms)
+    for k in range(L):
+        depth = table[:, k]
+        seg = _pick(depth - dhgt_pre > 1e-6, depth - dhgt_pre, torch.full_like(depth, 1e-6))
+        s_next = s_curr + rl * (dwth_pre + 0.5 * dwth_inc) * seg
+        cond_base = (base > dhgt_pre) & (base <= depth)
+        rb = (base - dhgt_pre) / seg
+        base_sto = _pick(cond_base, s_curr + rl * (dwth_pre + 0.5 * rb * dwth_inc) * (rb * seg), base_sto)
+        found_base = found_base | cond_base
+        cond_fill = (crown > dhgt_pre) & (crown <= depth)
+        rf = (crown - dhgt_pre) / seg
+        fill_sto = _pick(cond_fill, s_curr + rl * (dwth_pre + 0.5 * rf * dwth_inc) * (rf * seg), fill_sto)
+        found_fill = found_fill | cond_fill
+        s_top_loop = base_sto + (ld + rw) * (crown - base) * rl
+        threshold = s_next + (ld + rw) * (crown - depth) * rl
+        check = (ilev <= k) & ~found_b
+        found_now = check & (total < threshold)
+        lower = _pick(ilev == k, s_top_loop, dsto_fil_b)
+        advance = check & ~found_now
+        dsto_fil_b = _pick(advance, threshold, lower)
+        dwth_fil_b = _pick(advance, dwth_inc * (k + 1) - ld, dwth_fil_b)
+        ddph_fil_b = _pick(advance, depth - base, ddph_fil_b)
+        gradient_b = _pick(found_now, seg / dwth_inc, gradient_b)
+        found_b = found_b | found_now
+        s_curr = s_next
+        dhgt_pre = depth
+        dwth_pre = dwth_pre + dwth_inc
+    base_sto = _pick(found_base, base_sto,
+                     _pick(base > dhgt_pre, s_curr + rl * dwth_pre * (base - dhgt_pre), rms))
+    fill_sto = _pick(found_fill, fill_sto,
+                     _pick(crown > dhgt_pre, s_curr + rl * dwth_pre * (crown - dhgt_pre), rms))
+    s_top = base_sto + (ld + rw) * (crown - base) * rl
+    case4 = total >= fill_sto
+    case3 = ~case4 & (total >= s_top)
+    case2 = ~case4 & ~case3 & (total >= base_sto)
+    none = ~(case2 | case3 | case4)
+    zero = torch.zeros_like(rms)
+    # case 2
+    f_dph_c2 = base + (total - base_sto) / (ld + rw) / rl
+    r_sto_c2 = rms + rl * rw * f_dph_c2
+    f_sto_c2 = _maxz(total - r_sto_c2)
+    # case 3
+    add_b = total - dsto_fil_b
+    term_b = dwth_fil_b * dwth_fil_b + 2.0 * add_b / rl / (gradient_b + 1e-9)
+    p_dph_found = base + ddph_fil_b + (-dwth_fil_b + _sqrt0(term_b)) * gradient_b
+    p_dph_extra = base + ddph_fil_b + add_b / (dwth_fil_b * rl + 1e-9)
+    p_dph_b = _pick(found_b, p_dph_found, p_dph_extra)
+    f_frc_b = _pick(found_b, (dwth_fil_b + ld) / (dwth_inc * L), torch.ones_like(rms))
+    r_sto_c3 = rms + rl * rw * crown
+    f_sto_c3 = _maxz(s_top - r_sto_c3)
+    # case 4
+    f_sto_c4 = _maxz(s_top + (fdep_c - crown) * (ld + rw) * rl - rs)
+    r_cand = _pick(case2, r_sto_c2, _pick(case3, r_sto_c3, rs))
+    f_cand = _pick(case2, f_sto_c2, _pick(case3, f_sto_c3, _pick(case4, f_sto_c4, fs)))
+    # partition (levee.py:18-59)
+    r_hp = _pick(case4 | none, rs, _pick(r_cand < total, r_cand, total))
+    remaining = _maxz(total - r_hp)
+    f_cl = _maxz(f_cand)
+    f_hp = _pick(case2, remaining, _pick(none, fs, _pick(f_cl < remaining, f_cl, remaining)))
+    p_hp = _maxz(remaining - f_hp)
+    r_dph = _pick(case2, r_sto_c2 / rl / rw, _pick(case3, r_sto_c3 / rl / rw, rdep_c))
+    f_dph = _pick(case2, f_dph_c2, _pick(case3, crown.expand_as(rms), fdep_c))
+    p_dph = _pick(case3, p_dph_b, _pick(case4, fdep_c, zero))
+    with torch.no_grad():
+        f_frc = _pick(case2, lfrac.expand_as(rms), _pick(case3, torch.clamp(f_frc_b, 0.0, 1.0), ffr_c))
+    return r_hp, f_hp, p_hp, r_dph, f_dph, p_dph, f_frc
+
+
+def _paths(st, prm, wse, tot, dt, g, pwse=None):
     """Per-level path flows limited at 5 % of the smaller end storage, and the
     outgoing terms at the upstream (pos) and downstream (neg) ends."""
     ci, di = prm["bifurcation_catchment_idx"], prm["bifurcation_downstream_idx"]
@@ -63,8 +161,19 @@ def _paths(st, prm, wse, tot, dt, g):
     slope = _pick(inside, sx, torch.clamp(sx.detach(), -0.005, 0.005))
     upd = _maxz(max_w[:, None] - bel)
     a = _sqrt0(upd * bcsd)
-    b = _sqrt0(upd * 0.01)
-    sifd = _pick(a > b, a, b)
+    if pwse is None:
+        b = _sqrt0(upd * 0.01)
+        sifd = _pick(a > b, a, b)
+    else:
+        # levee variant: level 0 semi-implicit on the river surface (the
+        # explicit depth where the product is not positive), upper levels
+        # explicit on the protected surface
+        p0, p1 = pwse[ci], pwse[di]
+        max_pw = _pick(p0 > p1, p0, p1)
+        upd = torch.cat([upd[:, :1], _maxz(max_pw[:, None] - bel[:, 1:])], dim=1)
+        a = _sqrt0(upd * bcsd)
+        semi0 = _pick(a[:, :1] <= 0, upd[:, :1], a[:, :1])
+        sifd = torch.cat([semi0, upd[:, 1:]], dim=1)
     cond = sifd > 1e-5
     s_s = _safe(cond, sifd)
     unit = bq / bw
@@ -93,7 +202,10 @@ def substep(st: dict, prm: dict, dt: float, gravity: float = 9.8) -> dict:
     flood_depth_table [N, L], downstream_idx (int64).  With paths, prm also
     holds bifurcation_catchment_idx / bifurcation_downstream_idx (int64),
     bifurcation_length [P] and bifurcation_elevation / _width / _manning
-    [P, LB], and st bifurcation_outflow / bifurcation_cross_section_depth."""
+    [P, LB], and st bifurcation_outflow / bifurcation_cross_section_depth.
+    With levees, prm holds levee_catchment_idx (int64) and
+    levee_crown_height / levee_fraction / levee_base_height [V], and st
+    protected_storage and protected_depth."""
     dn = prm["downstream_idx"]
     n = dn.shape[0]
     idx = torch.arange(n, device=dn.device)
@@ -111,7 +223,9 @@ def substep(st: dict, prm: dict, dt: float, gravity: float = 9.8) -> dict:
     # ---------------------------------------------------------- outflow --
     river_elevation = c_elv - r_hgt
     wse = r_dep + river_elevation
-    tot = rs + fs
+    levee = "levee_catchment_idx" in prm
+    ps = st["protected_storage"] if levee else torch.zeros_like(rs)
+    tot = rs + fs + ps
     wse_dn = _pick(mouth, c_elv, r_dep[dn] + (c_elv[dn] - r_hgt[dn]))
     max_wse = _pick(wse > wse_dn, wse, wse_dn)
     slope = (wse - wse_dn) / dist
@@ -151,7 +265,12 @@ def substep(st: dict, prm: dict, dt: float, gravity: float = 9.8) -> dict:
 
     paths = "bifurcation_catchment_idx" in prm
     if paths:
-        qlim, b_upd, b_pos, b_neg = _paths(st, prm, wse, tot, dt, g)
+        pwse = None
+        if levee:
+            lidx = prm["levee_catchment_idx"]
+            pw_l = c_elv[lidx] + st["protected_depth"][lidx]
+            pwse = wse.index_put((lidx,), _pick(pw_l < wse[lidx], pw_l, wse[lidx]))
+        qlim, b_upd, b_pos, b_neg = _paths(st, prm, wse, tot, dt, g, pwse)
         ci, di = prm["bifurcation_catchment_idx"], prm["bifurcation_downstream_idx"]
 
     # ------------------------------------------------- limiter and inflow --
@@ -187,7 +306,7 @@ def substep(st: dict, prm: dict, dt: float, gravity: float = 9.8) -> dict:
     rsu1 = _maxz(sa)
     rsu2 = _pick(sb < 0, _maxz(rsu1 + sb), rsu1)
     fsu1 = _maxz(sb)
-    total_next = rsu2 + fsu1 + prm["runoff"] * dt
+    total_next = rsu2 + fsu1 + ps + prm["runoff"] * dt
     total = _maxz(total_next)
 
     # ------------------------------------------------------------ stage --
@@ -223,6 +342,16 @@ def substep(st: dict, prm: dict, dt: float, gravity: float = 9.8) -> dict:
     rsf = _pick(wet & (cap < total), cap, total)
     rdep = rsf / (r_len * r_wid)
     fsf = _maxz(total - rsf)
+    if levee:
+        lidx = prm["levee_catchment_idx"]
+        r_hp, f_hp, p_hp, r_dph, f_dph, p_dph, _ = _levee(prm, lidx, rsf[lidx], fsf[lidx], fdep[lidx], rdep[lidx],
+                                                          zero[lidx])
+        pdep = fdep.index_put((lidx,), p_dph)
+        rsf = rsf.index_put((lidx,), r_hp)
+        fsf = fsf.index_put((lidx,), f_hp)
+        rdep = rdep.index_put((lidx,), r_dph)
+        fdep = fdep.index_put((lidx,), f_dph)
+        psf = zero.index_put((lidx,), p_hp)
     out = {"river_outflow": rout, "flood_outflow": fout,
            "river_cross_section_depth": upd_rcsd, "flood_cross_section_depth": upd_fcsd,
            "flood_cross_section_area": upd_fcsa, "river_storage": rsf, "flood_storage": fsf,
@@ -230,4 +359,7 @@ def substep(st: dict, prm: dict, dt: float, gravity: float = 9.8) -> dict:
     if paths:
         out["bifurcation_outflow"] = b_next
         out["bifurcation_cross_section_depth"] = b_upd
+    if levee:
+        out["protected_storage"] = psf
+        out["protected_depth"] = pdep
     return out

@@ -7,8 +7,10 @@ Problem: plain network (3,000 catchments; the autograd check on a
 squared log-discharge error over 8 outer steps against a run whose Manning's
 n and river height were scaled by lognormal(0, 0.2) factors; checkpoints
 every 3 outer steps (so the backward re-runs segments).  The same checks run
-with bifurcation paths open (100 paths on 3,000 catchments, 600 on 20,000);
-the central differences then take half their parameters at path ends.
+with bifurcation paths open (100 paths on 3,000 catchments, 600 on 20,000),
+with levees on 30 % of the catchments (their initial river storage scaled
+by U(1, 100), so that every levee case occurs), and with both; the central
+differences then take half their parameters at path ends and levees.
 
 Tolerances: FP64 adjoint vs FP64 autograd, relative L2 of each gradient 1e-6
 (the two sum floats in different orders); central differences (h = 1e-3
@@ -30,15 +32,25 @@ PLAIN = ("base", "adaptive_time")
 STEPS = 8
 
 
-def _setup(precision, n=3000, paths=0):
+KINDS = {"plain": (0, 0.0), "paths": (1, 0.0), "levees": (0, 0.3), "both": (1, 0.3)}
+
+
+def _setup(precision, n=3000, kind="plain"):
     from cmfgpu_amd import synthetic
     from cmfgpu_amd.adjoint import GaugeProblem
     from cmfgpu_amd.models import CaMaFlood
-    net = synthetic.generate_network(n, 1003, paths=paths)
+    has_paths, levee = KINDS[kind]
+    paths = (n // 30 if n <= 3000 else n * 3 // 100) if has_paths else 0
+    net = synthetic.generate_network(n, 1003, paths=paths, levee=levee)
     src = dict(net.fields)
     src["runoff"] = synthetic.Runoff(src["catchment_area"], 2003).next_day() * np.float32(4.0)
+    if levee:
+        rs = np.array(src["river_storage"], dtype=np.float64)
+        lid = np.asarray(src["levee_catchment_idx"])
+        rs[lid] *= np.random.default_rng(3).uniform(1.0, 100.0, lid.size)
+        src["river_storage"] = rs.astype(np.asarray(src["river_storage"]).dtype)
     gauges = np.argsort(net.upstream_area)[-16:]
-    mods = PLAIN + (("bifurcation",) if paths else ())
+    mods = PLAIN + (("bifurcation",) if paths else ()) + (("levee",) if levee else ())
     model = CaMaFlood(src, opened_modules=mods, device="cuda:0", step_duration=1800.0, precision=precision)
     rng = np.random.default_rng(7)
     n0, h0 = src["river_manning"].astype(np.float64), src["river_height"].astype(np.float64)
@@ -67,6 +79,12 @@ def _torch_gradient(model, problem, gauges, n0, h0):
         "flood_manning", "river_width", "river_length", "catchment_elevation", "catchment_area",
         "downstream_distance", "runoff", "flood_depth_table")}
     prm["downstream_idx"] = torch.tensor(model.get("downstream_idx"), dtype=torch.int64)
+    if problem.levees:
+        for k in ("protected_storage", "protected_depth"):
+            st[k] = torch.tensor(model.get(k), dtype=torch.float64)
+        for k in ("levee_crown_height", "levee_fraction", "levee_base_height"):
+            prm[k] = torch.tensor(model.get(k), dtype=torch.float64)
+        prm["levee_catchment_idx"] = torch.tensor(model.get("levee_catchment_idx"), dtype=torch.int64)
     if problem.paths:
         for k in ("bifurcation_outflow", "bifurcation_cross_section_depth"):
             st[k] = torch.tensor(model.get(k), dtype=torch.float64)
@@ -97,9 +115,9 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("paths", [0, 600])
-def test_adjoint_matches_fp64_autograd(paths):
-    net, src, model, problem, gauges = _setup("float64", n=20000, paths=paths)
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_adjoint_matches_fp64_autograd(kind):
+    net, src, model, problem, gauges = _setup("float64", n=20000, kind=kind)
     loss = problem.run()
     assert loss > 0 and len(problem.checkpoints) == 3
     g_n, g_h = (x.cpu().numpy() for x in problem.gradient())
@@ -114,8 +132,8 @@ def test_adjoint_matches_fp64_autograd(paths):
 
 
 def _pick_checks(rng, g, ends, count):
-    """`count` indices whose gradient exceeds 1e-4 of the largest; with path
-    ends given, half of them among the path ends."""
+    """`count` indices whose gradient exceeds 1e-4 of the largest; with
+    `ends` given (path ends, levee catchments), half of them among those."""
     cand = np.flatnonzero(np.abs(g) > 1e-4 * np.abs(g).max())
     if ends is None:
         return list(rng.choice(cand, min(count, cand.size), replace=False))
@@ -125,17 +143,21 @@ def _pick_checks(rng, g, ends, count):
     return list(first) + list(rng.choice(rest, count - first.size, replace=False))
 
 
-@pytest.mark.parametrize("paths", [0, 100])
-def test_adjoint_against_central_differences(paths):
-    net, src, model, problem, gauges = _setup("float64", paths=paths)
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_adjoint_against_central_differences(kind):
+    net, src, model, problem, gauges = _setup("float64", kind=kind)
     problem.run()
     counts = list(problem.counts)
     g_n, g_h = (x.cpu().numpy() for x in problem.gradient())
     n0, h0 = (x.astype(np.float64) for x in problem.parameters())
     rng = np.random.default_rng(11)
     ends = None
-    if paths:
-        ends = np.union1d(src["bifurcation_catchment_idx"], src["bifurcation_downstream_idx"])
+    if kind != "plain":
+        ends = np.zeros(0, np.int64)
+        if "bifurcation_catchment_idx" in src:
+            ends = np.union1d(src["bifurcation_catchment_idx"], src["bifurcation_downstream_idx"])
+        if "levee_catchment_idx" in src:
+            ends = np.union1d(ends, src["levee_catchment_idx"])
     checks = [("n", i) for i in _pick_checks(rng, g_n, ends, 32)] + \
              [("h", i) for i in _pick_checks(rng, g_h, ends, 32)]
     assert len(checks) == 64, len(checks)
@@ -172,11 +194,11 @@ def test_adjoint_against_central_differences(paths):
     model.close()
 
 
-@pytest.mark.parametrize("paths", [0, 100])
-def test_fp32_adjoint_close_to_fp64(paths):
+@pytest.mark.parametrize("kind", ["plain", "both"])
+def test_fp32_adjoint_close_to_fp64(kind):
     grads = {}
     for prec in ("float64", "float32"):
-        net, src, model, problem, gauges = _setup(prec, paths=paths)
+        net, src, model, problem, gauges = _setup(prec, kind=kind)
         problem.run()
         grads[prec] = [x.cpu().numpy() for x in problem.gradient()]
         model.close()
@@ -190,11 +212,13 @@ def test_recomputed_intermediates_match_the_tape(precision):
     """One sub-step's adjoint from the taped forward intermediates (unlimited
     flows, limiters, path terms) and again with them recomputed from the saved
     state (passes A0, A1, B0): bit-identical adjoints and gradients, so the
-    reverse re-forms the forward's values exactly (network with paths)."""
+    reverse re-forms the forward's values exactly (network with paths and
+    levees)."""
     import ctypes
     from cmfgpu_amd import _lib
-    from cmfgpu_amd.adjoint import (CmfAdjoint, CmfTape, PATH_STATE, STATE, TAPE_REAL, TAPE_STO, _declare)
-    net, src, model, problem, gauges = _setup(precision, paths=100)
+    from cmfgpu_amd.adjoint import (CmfAdjoint, CmfTape, LEVEE_STATE, PATH_STATE, STATE, TAPE_REAL, TAPE_STO,
+                                    _declare)
+    net, src, model, problem, gauges = _setup(precision, kind="both")
     for _ in range(3):
         model.step_advance()
     _declare()
@@ -210,6 +234,9 @@ def test_recomputed_intermediates_match_the_tape(precision):
     ptrs.update({name: ts[x].data_ptr() for x, name in enumerate(TAPE_STO)})
     ptrs.update({name: tp[x].data_ptr() for x, name in enumerate(PATH_STATE)})
     ptrs["path_terms"] = tt.data_ptr()
+    tls = torch.empty(n, dtype=sto_t, device=dev)
+    tld = torch.empty(n, dtype=real_t, device=dev)
+    ptrs["protected_storage"], ptrs["protected_depth"] = tls.data_ptr(), tld.data_ptr()
     count = 6
     model.t["time_step"].fill_(model.step_duration / count)
     _lib.check(lib.cmf_hip_substep_taped(model.network_handle, model.field_table_ref(), 2, 0,
@@ -217,10 +244,12 @@ def test_recomputed_intermediates_match_the_tape(precision):
     gen = torch.Generator(device="cpu").manual_seed(5)
     lout = [torch.randn(n, generator=gen, dtype=torch.float64).to(dev, real_t) for _ in STATE]
     lout_p = [torch.randn(pe, generator=gen, dtype=torch.float64).to(dev, real_t) for _ in PATH_STATE]
+    lout_l = [torch.randn(n, generator=gen, dtype=torch.float64).to(dev, real_t) for _ in LEVEE_STATE]
     results = []
     for taped in (True, False):
         lin = [torch.zeros(n, dtype=real_t, device=dev) for _ in STATE]
         lin_p = [torch.zeros(pe, dtype=real_t, device=dev) for _ in PATH_STATE]
+        lin_l = [torch.zeros(n, dtype=real_t, device=dev) for _ in LEVEE_STATE]
         g_n = torch.zeros(n, dtype=torch.float64, device=dev)
         g_h = torch.zeros(n, dtype=torch.float64, device=dev)
         p = dict(ptrs)
@@ -234,11 +263,14 @@ def test_recomputed_intermediates_match_the_tape(precision):
         for x in range(len(PATH_STATE)):
             a.adj_out_path[x] = lout_p[x].data_ptr()
             a.adj_in_path[x] = lin_p[x].data_ptr()
+        for x in range(len(LEVEE_STATE)):
+            a.adj_out_levee[x] = lout_l[x].data_ptr()
+            a.adj_in_levee[x] = lin_l[x].data_ptr()
         a.grad_river_manning, a.grad_river_height = g_n.data_ptr(), g_h.data_ptr()
         a.time_step = model.step_duration / count
         _lib.check(lib.cmf_hip_substep_adjoint(model.network_handle, model.field_table_ref(), ctypes.byref(a),
                                                _lib.current_stream()), "adjoint")
-        results.append([x.cpu().numpy() for x in lin + lin_p + [g_n, g_h]])
+        results.append([x.cpu().numpy() for x in lin + lin_p + lin_l + [g_n, g_h]])
     for x, y in zip(*results):
         assert np.array_equal(x, y)
     assert np.count_nonzero(results[0][len(STATE)]) > 10   # the path flows' adjoint

@@ -2,6 +2,7 @@
 adjoint (oracle/torch_step.py) against the FP64 CPU oracle: same states after
 several sub-steps, to the rounding of different summation orders."""
 import numpy as np
+import pytest
 import torch
 
 from helpers import oracle_for
@@ -40,6 +41,20 @@ def test_torch_transcription_matches_fp64_oracle():
 PATH_FIELDS = ("bifurcation_outflow", "bifurcation_cross_section_depth")
 
 
+def levee_flooded(net, seed=3):
+    """The network's fields with runoff x4 and every levee catchment's
+    initial river storage scaled by U(1, 100), so that floods reach the
+    levees within a few outer steps."""
+    from cmfgpu_amd import synthetic
+    src = dict(net.fields)
+    src["runoff"] = synthetic.Runoff(src["catchment_area"], 2003).next_day() * np.float32(4.0)
+    rs = np.array(src["river_storage"], dtype=np.float64)
+    lid = np.asarray(src["levee_catchment_idx"])
+    rs[lid] *= np.random.default_rng(seed).uniform(1.0, 100.0, lid.size)
+    src["river_storage"] = rs.astype(np.asarray(src["river_storage"]).dtype)
+    return src
+
+
 def test_torch_transcription_with_paths_matches_fp64_oracle():
     """The same with 150 bifurcation paths open: path flows and depths and
     every catchment field after 12 outer steps of 6 sub-steps."""
@@ -65,6 +80,59 @@ def test_torch_transcription_with_paths_matches_fp64_oracle():
         moving = max(moving, int((np.abs(orc.state["bifurcation_outflow"]).sum(axis=1) > 0).sum()))
     assert moving > 50
     for k in FIELDS + PATH_FIELDS:
+        ref = orc.state[k].astype(np.float64)
+        got = st[k].numpy()
+        err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-30)
+        assert err < 1e-9, (k, err)
+
+
+LEVEE_FIELDS = ("protected_storage", "protected_depth")
+
+
+@pytest.mark.parametrize("paths", [0, 150])
+def test_torch_transcription_with_levees_matches_fp64_oracle(paths):
+    """The same with levees on 30 % of the catchments (and with 150 paths,
+    whose upper levels then run on the protected surface), the levee
+    catchments starting at 1-100 times their initial river storage: every
+    levee case (2: river side between base and crown, 3: protected side
+    filling, 4: overtopped) occurs, and all state fields agree after 12 outer
+    steps."""
+    from cmfgpu_amd import synthetic
+    from oracle.torch_step import substep
+    net = synthetic.generate_network(2000, 1003, levee=0.3, paths=paths)
+    src = levee_flooded(net)
+    mods = ("base", "levee") + (("bifurcation",) if paths else ())
+    orc, host = oracle_for(src, mods, precision="float64")
+    extra = PATH_FIELDS if paths else ()
+    st = {k: torch.tensor(orc.state[k], dtype=torch.float64) for k in FIELDS + LEVEE_FIELDS + extra}
+    names = ["river_manning", "river_height", "flood_manning", "river_width", "river_length",
+             "catchment_elevation", "catchment_area", "downstream_distance", "runoff", "flood_depth_table",
+             "levee_crown_height", "levee_fraction", "levee_base_height"]
+    idx = ["downstream_idx", "levee_catchment_idx"]
+    if paths:
+        names += ["bifurcation_length", "bifurcation_elevation", "bifurcation_width", "bifurcation_manning"]
+        idx += ["bifurcation_catchment_idx", "bifurcation_downstream_idx"]
+    prm = {k: torch.tensor(orc.state[k], dtype=torch.float64) for k in names}
+    for k in idx:
+        prm[k] = torch.tensor(orc.state[k], dtype=torch.int64)
+    lidx = orc.state["levee_catchment_idx"]
+    count = 6
+    cases = set()
+    for _ in range(12):
+        orc.advance(1800.0, 1, count)
+        for _ in range(count):
+            st = substep(st, prm, 1800.0 / count)
+        s = orc.state
+        frac = s["flood_fraction"][lidx]
+        ps = s["protected_storage"][lidx]
+        if np.any((ps == 0) & (np.abs(frac - s["levee_fraction"]) == 0) & (s["flood_depth"][lidx] > 0)):
+            cases.add(2)
+        if np.any((ps > 0) & (s["flood_depth"][lidx] == s["levee_crown_height"])):
+            cases.add(3)
+        if np.any((ps > 0) & (s["protected_depth"][lidx] == s["flood_depth"][lidx])):
+            cases.add(4)
+    assert cases == {2, 3, 4}, cases
+    for k in FIELDS + LEVEE_FIELDS + extra:
         ref = orc.state[k].astype(np.float64)
         got = st[k].numpy()
         err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-30)
