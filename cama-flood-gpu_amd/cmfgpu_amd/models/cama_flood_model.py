@@ -183,7 +183,8 @@ class CaMaFlood:
         for name in CATCHMENT_STATE:
             if name in hf:
                 self.t[name] = self._to_catchment(name, hf[name], per_trial=True, replicate=True)
-        if self.P:
+        if "bifurcation_catchment_idx" in hf:
+            # (with the module open and no paths the arrays are empty, still bound)
             ci = self.inv[hf["bifurcation_catchment_idx"]].astype(np.int32)
             di = self.inv[hf["bifurcation_downstream_idx"]].astype(np.int32)
             self.t["bifurcation_catchment_idx"] = torch.from_numpy(ci).to(self.device)

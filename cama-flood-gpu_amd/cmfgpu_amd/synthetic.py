@@ -126,7 +126,12 @@ class Network:
 
 
 def generate_network(n: int, seed: int, *, levee: float = 0.0, paths: int = 0,
-                     reservoirs: int = 0, gauges: int = 0, name: str = "") -> Network:
+                     reservoirs: int = 0, gauges: int = 0, name: str = "",
+                     downstream: np.ndarray | None = None) -> Network:
+    """The seeded random forest above, or with `downstream` a prescribed
+    topology in generation order (entry g: the catchment g drains into, an
+    index below g, or -1 for an outlet; n is then its length), with the same
+    parameter draws."""
     rng = np.random.default_rng(seed)
     g = np.arange(n, dtype=np.int64)
     outlet = rng.random(n) < 0.01
@@ -136,6 +141,11 @@ def generate_network(n: int, seed: int, *, levee: float = 0.0, paths: int = 0,
     k = ((k - 1) % window) + 1
     dn_gen = g - k
     dn_gen[outlet] = -1
+    if downstream is not None:
+        dn_gen = np.asarray(downstream, dtype=np.int64).copy()
+        if dn_gen.shape != (n,) or np.any(dn_gen >= g) or np.any(dn_gen < -1):
+            raise ValueError("downstream: n entries, each -1 or an index below its own")
+        outlet = dn_gen < 0
 
     length = rng.lognormal(np.log(15_000.0), 0.5, n)
     width = rng.lognormal(np.log(60.0), 0.8, n)

@@ -290,11 +290,15 @@ int need_features(const cmf_fields* f, const char* op) {
         if ((rc = require({NEED(is_levee), NEED(levee_catchment_idx), NEED(levee_base_height),
                            NEED(levee_crown_height), NEED(levee_fraction)}, op))) return rc;
     if (f->flags & CMF_HAS_BIFURCATION) {
-        if ((rc = require({NEED(bifurcation_catchment_idx), NEED(bifurcation_downstream_idx),
+        if ((rc = require({NEED(global_bifurcation_outflow), NEED(limit_rate), NEED(water_surface_elevation),
+                           NEED(total_storage)}, op)))
+            return rc;
+        // the per-path arrays (empty, so possibly NULL, without paths)
+        if (f->num_bifurcation_paths > 0 &&
+            (rc = require({NEED(bifurcation_catchment_idx), NEED(bifurcation_downstream_idx),
                            NEED(bifurcation_manning), NEED(bifurcation_outflow), NEED(bifurcation_width),
                            NEED(bifurcation_length), NEED(bifurcation_elevation),
-                           NEED(bifurcation_cross_section_depth), NEED(global_bifurcation_outflow),
-                           NEED(limit_rate), NEED(water_surface_elevation), NEED(total_storage)}, op)))
+                           NEED(bifurcation_cross_section_depth)}, op)))
             return rc;
         if (f->num_bifurcation_levels < 1)
             return set_error(CMF_ERR_INVALID, "%s: num_bifurcation_levels must be >= 1", op);
