@@ -340,6 +340,82 @@ def test_levee_partition_conserves_the_total(total):
         assert s["protected_storage"][0] == 0.0
 
 
+def levee_outputs(total, precision=1):
+    s = levee_state(total)
+    if precision == 1:
+        s = {k: (v.astype(np.float64) if v.dtype == np.float32 else v) for k, v in s.items()}
+    m = O.OracleModel(s, flags=O.HAS_LEVEE, num_flood_levels=10, precision=precision)
+    m.kernel("flood_stage", time_step=1.0)
+    m.kernel("levee_stage")
+    return (float(s["river_storage"][0]), float(s["flood_storage"][0]), float(s["protected_storage"][0]),
+            float(s["flood_depth"][0]), float(s["protected_depth"][0]), float(s["flood_fraction"][0]))
+
+
+def profile_storage(h, rl=10_000.0, rw=50.0, rh=4.0, ca=5e8, layer=0.5, levels=10):
+    """Storage at flood depth h over the whole floodplain of levee_state's
+    catchment, integrated layer by layer from the geometry (channel full,
+    each 0.5 m layer a trapezoid between the widths at its bottom and top)."""
+    winc = ca / rl / levels
+    s = rl * rw * rh
+    for k in range(levels):
+        lo, hi = k * layer, (k + 1) * layer
+        if h <= lo:
+            break
+        top = min(h, hi)
+        w_lo = rw + k * winc
+        w_top = w_lo + winc * (top - lo) / layer
+        s += rl * 0.5 * (w_lo + w_top) * (top - lo)
+    return s
+
+
+@pytest.mark.parametrize("total", [1.0e8, 2.0e8, 3.0e8])
+def test_levee_river_side_fill_closed_form(total):
+    """Case 2 (water between levee base and crown on the river side): above
+    the base the water spreads only over the channel and the floodplain up
+    to the levee, so the depth is base + (total - S(base)) / ((ld + W) L)
+    with S(base) integrated from the profile; the protected side stays
+    dry, the channel holds its column up to that depth, and the flooded
+    fraction is the levee's position."""
+    base, crown, rl, rw = 1.0, 3.0, 10_000.0, 50.0
+    frac = float(np.float32(0.3))          # the fraction as the state stores it
+    ld = frac * 5e8 / rl
+    s_base = profile_storage(base)
+    assert s_base < total < s_base + (ld + rw) * (crown - base) * rl
+    r, f, p, fdep, pdep, ffr = levee_outputs(total)
+    depth = base + (total - s_base) / ((ld + rw) * rl)
+    assert fdep == pytest.approx(depth, rel=1e-12)
+    assert p == 0.0 and pdep == 0.0
+    assert r == pytest.approx(rl * rw * (4.0 + depth), rel=1e-12)
+    assert r + f == pytest.approx(total, rel=1e-14)
+    assert ffr == pytest.approx(frac, rel=1e-6)
+
+
+def test_levee_cases_are_continuous_and_monotone():
+    """Sweeping the total through all four cases (below bankfull, river side
+    filling, protected side filling, overtopped): the flood depth, the
+    protected storage and the river + flood storage never decrease, none of
+    them jumps, the total is conserved, and the protected side first holds
+    water exactly above S(base) + (ld + W)(crown - base) L."""
+    totals = np.linspace(1e6, 1.2e9, 4001)
+    out = np.array([levee_outputs(t) for t in totals])
+    r, f, p, fdep = out[:, 0], out[:, 1], out[:, 2], out[:, 3]
+    assert np.allclose(r + f + p, totals, rtol=1e-13)
+    step = totals[1] - totals[0]
+    for x in (fdep, p, r + f):
+        d = np.diff(x)
+        assert np.all(d >= -1e-9 * np.abs(x[1:]).max()), "not monotone"
+    # no jumps: a step of the total moves a storage by at most that step, and
+    # the flood depth by at most the step over the narrowest wetted width
+    assert np.all(np.abs(np.diff(p)) <= step * (1 + 1e-9))
+    assert np.all(np.abs(np.diff(r + f)) <= step * (1 + 1e-9))
+    assert np.all(np.abs(np.diff(fdep)) <= step / (50.0 * 10_000.0) * (1 + 1e-9))
+    s_top = profile_storage(1.0) + (float(np.float32(0.3)) * 5e8 / 10_000.0 + 50.0) * 2.0 * 10_000.0
+    wet_protected = totals[p > 0]
+    assert wet_protected.size and wet_protected.min() >= s_top * (1 - 1e-12)
+    assert wet_protected.min() <= s_top + 2 * step
+    assert fdep.max() > 3.0                     # the sweep reaches past the crown
+
+
 # -------------------------------------------------------------- reservoir --
 def reservoir_state(total, inflow, cv=4e7, ev=9.5e7, qn=50.0, qf=150.0):
     n = 3
