@@ -1,0 +1,82 @@
+"""The oracle pins catch plausible mistakes: each mistake below is planted in a
+scratch copy of the C++ oracle (built with its own Makefile), and the pins of
+tests/test_oracle_pins.py run against that copy must fail.  CPU only; the
+repository's oracle is not touched.
+"""
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+# name: (line of oracle/cmf_oracle.cpp, the same line with the mistake)
+MISTAKES = {
+    "friction exponent 4/3 instead of 7/3": (
+        "* ((REAL)1.0 / (r_sifd * r_sifd * cbrt_real(r_sifd)));",
+        "* ((REAL)1.0 / (r_sifd * cbrt_real(r_sifd)));"),
+    "backflow limit 10 % instead of 5 %": (
+        "const REAL limit = fmin_(negative ? (REAL)0.05 * total_storage / total_neg : (REAL)1.0, (REAL)1.0);",
+        "const REAL limit = fmin_(negative ? (REAL)0.10 * total_storage / total_neg : (REAL)1.0, (REAL)1.0);"),
+    "storage limiter inverted": (
+        "lim[i] = (og > (REAL)1e-8) ? fmin_(rate / og, (REAL)1.0) : (REAL)1.0;",
+        "lim[i] = (og > (REAL)1e-8) ? fmin_(og / rate, (REAL)1.0) : (REAL)1.0;"),
+    "path net added at both ends": (
+        "gbif[di] += (STO)(-net);",
+        "gbif[di] += (STO)(net);"),
+    "river deficit not carried into flood storage": (
+        "STO fsu = fs + ((rsu < (STO)0.0) ? rsu : (STO)0.0) + (STO)((fin - fout - gbif) * time_step);",
+        "STO fsu = fs + (STO)((fin - fout - gbif) * time_step);"),
+    "floodplain width index off by one": (
+        "const REAL W = rw + (REAL)(k + 1) * width_increment;",
+        "const REAL W = rw + (REAL)(k) * width_increment;"),
+    "CFL offset sign": (
+        "const REAL nf = (REAL)std::floor(outer_time_step / dtc - (REAL)0.01) + (REAL)1.0;",
+        "const REAL nf = (REAL)std::floor(outer_time_step / dtc + (REAL)0.01) + (REAL)1.0;"),
+    "CFL depth floor 0.1 m": (
+        "const REAL depth = fmax_(river_depth[i], (REAL)0.01);",
+        "const REAL depth = fmax_(river_depth[i], (REAL)0.1);"),
+    "reservoir conservation release without the square root": (
+        "out = qn * (REAL)std::sqrt(total / cv);",
+        "out = qn * (total / cv);"),
+    "path limit on the larger end storage": (
+        "const REAL limit = fmin_((REAL)0.05 * fmin_(ts0, ts1) / (std::fabs(sum) * time_step), (REAL)1.0);",
+        "const REAL limit = fmin_((REAL)0.05 * fmax_(ts0, ts1) / (std::fabs(sum) * time_step), (REAL)1.0);"),
+    "levee case 2 without the channel width": (
+        "const REAL dwth_add_c2 = levee_distance + rw;",
+        "const REAL dwth_add_c2 = levee_distance;"),
+    "protected-side threshold sign": (
+        "const REAL s_top = base_sto + (levee_distance + rw) * dhgt_dif * rl;",
+        "const REAL s_top = base_sto - (levee_distance + rw) * dhgt_dif * rl;"),
+    "levee case 4 measured from the base": (
+        "const REAL dsto_add_c4 = (f_dph_c4 - crown) * (levee_distance + rw) * rl;",
+        "const REAL dsto_add_c4 = (f_dph_c4 - base) * (levee_distance + rw) * rl;"),
+    "levee fill scan on the base height": (
+        "const REAL ratio_fill = (crown - dhgt_pre) / dhgt_seg;",
+        "const REAL ratio_fill = (base - dhgt_pre) / dhgt_seg;"),
+}
+
+
+@pytest.mark.parametrize("name", list(MISTAKES))
+def test_pins_catch_a_planted_mistake(name, tmp_path):
+    if shutil.which("make") is None or shutil.which(os.environ.get("CXX", "g++")) is None:
+        pytest.skip("no C++ toolchain")
+    old, new = MISTAKES[name]
+    src = (ROOT / "oracle" / "cmf_oracle.cpp").read_text()
+    assert src.count(old) == 1, "the line to alter is no longer in the oracle"
+    work = tmp_path / "copy"
+    shutil.copytree(ROOT / "oracle", work / "oracle", ignore=shutil.ignore_patterns("*.so", "__pycache__"))
+    (work / "oracle" / "cmf_oracle.cpp").write_text(src.replace(old, new))
+    subprocess.run(["make", "-s", "-C", str(work / "oracle")], check=True)
+    (work / "tests").mkdir()
+    for f in ("conftest.py", "test_oracle_pins.py"):
+        shutil.copy(ROOT / "tests" / f, work / "tests" / f)
+    # one thread: the pins are tiny, and parallel regions would only spin
+    env = dict(os.environ, PYTHONPATH=f"{work}{os.pathsep}{ROOT / 'cama-flood-gpu_amd'}", OMP_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider", "-m", "not gpu",
+                        str(work / "tests" / "test_oracle_pins.py")],
+                       capture_output=True, text=True, env=env, cwd=work)
+    assert r.returncode == 1 and " failed" in r.stdout, r.stdout[-2000:]

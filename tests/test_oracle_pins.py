@@ -134,6 +134,25 @@ def test_outflow_sign_follows_slope_and_outgoing_is_positive_part():
             assert og[1] >= -q0 * dt * (1 - 1e-6)
 
 
+def test_backflow_is_limited_to_five_percent_of_the_storage():
+    """Water surface higher downstream: the flow turns upstream, and at most
+    5 % of the catchment's storage may leave backwards in one sub-step
+    (triton/outflow.py:235-245), so a backflow that would move more is scaled
+    to exactly 0.05 S / dt."""
+    h, dx, W, dt = 2.0, 10_000.0, 50.0, 600.0
+    s0 = 1e5
+    s = tiny_state([1, 1], river_depth=np.full(2, h, np.float32),
+                   catchment_elevation=np.array([10.0, 11.0], np.float32),
+                   river_cross_section_depth=np.full(2, h, np.float32),
+                   river_storage=np.array([s0, 1e8]))
+    # unlimited, from rest: q = W g dt h_s S (S = -1e-4, h_s = sqrt((h + 1) h))
+    q_free = W * G * dt * math.sqrt((h + 1.0) * h) * (-1.0 / dx)
+    assert -q_free * dt > 0.05 * s0
+    model(s).kernel("outflow", time_step=dt)
+    assert float(s["river_outflow"][0]) == pytest.approx(-0.05 * s0 / dt, rel=1e-6)
+    assert float(s["flood_outflow"][0]) == 0.0
+
+
 # ----------------------------------------------------------------- inflow --
 def test_inflow_is_the_sum_of_upstream_flows():
     rng = np.random.default_rng(3)
@@ -240,6 +259,20 @@ def test_storage_update_global_water_balance():
     assert after == pytest.approx(expect, rel=1e-12)
 
 
+def test_storage_update_takes_a_river_deficit_from_the_flood_storage():
+    """A river outflow larger than the river storage: the deficit is taken
+    from the floodplain storage (storage.py:81-89), so the catchment keeps
+    exactly old storage - outflow dt, and the river storage never goes
+    negative."""
+    dt = 1.0
+    s = tiny_state([0], river_storage=np.array([100.0]), flood_storage=np.array([1000.0]),
+                   river_outflow=np.array([150.0], np.float32))
+    s = {k: (v.astype(np.float64) if v.dtype == np.float32 else v) for k, v in s.items()}
+    model(s, precision=1).kernel("flood_stage", time_step=dt)
+    assert s["river_storage"][0] + s["flood_storage"][0] == pytest.approx(950.0, rel=1e-15)
+    assert s["river_storage"][0] >= 0.0
+
+
 # -------------------------------------------------------------------- CFL --
 def test_cfl_closed_form():
     depth = np.array([0.0, 0.5, 3.0, 12.0], np.float32)
@@ -252,6 +285,22 @@ def test_cfl_closed_form():
         dt = 0.7 * float(dx) / math.sqrt(9.8 * max(float(h), 0.01))
         expect = max(expect, math.floor(T / min(dt, T) - 0.01) + 1)
     assert m.adaptive_steps(T) == expect
+
+
+def test_cfl_offset_and_depth_floor():
+    """n = floor(T / dt - 0.01) + 1 (adaptive_time.py:46): a ratio T / dt of
+    6.004 gives 6 sub-steps, not 7; and the depth is floored at 0.01 m
+    (:39): a 1 mm deep catchment counts as 1 cm deep."""
+    T = 1800.0
+    h = 1.0
+    dx = T / 6.004 * math.sqrt(9.8 * h) / 0.7
+    s = tiny_state([0], river_depth=np.array([h], np.float32), downstream_distance=np.array([dx], np.float32))
+    assert O.OracleModel(s, flags=O.HAS_ADAPTIVE, num_flood_levels=10).adaptive_steps(T) == 6
+    s = tiny_state([0], river_depth=np.array([0.001], np.float32),
+                   downstream_distance=np.array([100.0], np.float32))
+    dt = 0.7 * 100.0 / math.sqrt(9.8 * 0.01)
+    assert O.OracleModel(s, flags=O.HAS_ADAPTIVE, num_flood_levels=10).adaptive_steps(T) == \
+        math.floor(T / dt - 0.01) + 1 == 9
 
 
 # ------------------------------------------------------------ bifurcation --
@@ -312,6 +361,31 @@ def test_bifurcation_first_step_closed_form():
         assert s["bifurcation_cross_section_depth"][0, lev] == pytest.approx(d, rel=1e-5, abs=1e-6)
         assert abs(s["bifurcation_outflow"][0, lev]) <= abs(q) * (1 + 1e-4) + 1e-6
         del lim
+
+
+def test_bifurcation_limited_by_the_smaller_end_storage():
+    """A path may move at most 5 % of the smaller of its two end storages in
+    one sub-step (bifurcation.py:93-99): with the ends holding x and 100 x,
+    a path flow between those two bounds is scaled to 0.05 x / dt."""
+    flags = O.HAS_BIFURCATION | O.HAS_TOTAL_STORAGE | O.HAS_WATER_SURFACE
+    dt = 300.0
+
+    def path_sum(store_ci, store_di):
+        s, n, P, Lb = bif_state(P=1)
+        ci, di = int(s["bifurcation_catchment_idx"][0]), int(s["bifurcation_downstream_idx"][0])
+        rs = s["river_storage"].copy()
+        rs[ci], rs[di] = store_ci, store_di
+        s["river_storage"] = rs
+        m = O.OracleModel(s, flags=flags, num_flood_levels=10, num_bifurcation_levels=Lb)
+        m.kernel("outflow", time_step=dt)
+        m.kernel("bifurcation_outflow", time_step=dt)
+        return float(s["bifurcation_outflow"][0].astype(np.float64).sum())
+
+    free = path_sum(1e12, 1e12)
+    assert abs(free) > 0
+    x = abs(free) * dt / 0.05 / 10.0          # 0.05 x < |free| dt < 0.05 (100 x)
+    for a, b in ((x, 100 * x), (100 * x, x)):
+        assert abs(path_sum(a, b)) * dt == pytest.approx(0.05 * x, rel=1e-5)
 
 
 # ------------------------------------------------------------------ levee --
