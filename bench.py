@@ -132,9 +132,11 @@ def fp64_errors(net, src, device: str, layout: str, days: int) -> dict:
 def basins_line(args, device: str) -> dict:
     """Uneven basins (log-uniform sizes 500..400,000, plain physics): every
     basin advances with its own CFL sub-step count and dt in one launch per
-    pass (groups), against the same network with one shared sub-step count
-    (the reference's semantics).  `value` counts the sub-steps each basin
-    really takes."""
+    pass (groups), eagerly (each sub-step launching only the blocks of basins
+    still sub-stepping) and with each outer step's sub-step loop replayed from
+    a captured HIP graph, against the same network with one shared sub-step
+    count (the reference's semantics).  `value` counts the sub-steps each
+    basin really takes."""
     import numpy as np
     import torch
     from cmfgpu_amd import synthetic
@@ -147,9 +149,9 @@ def basins_line(args, device: str) -> dict:
     log(f"basins: {args.basins} basins, {n} catchments ({time.perf_counter() - t0:.1f} s)")
     sizes = np.diff(bs.basin_ptr)
     out = {"basins": args.basins, "catchments": int(n)}
-    for grouped in (True, False):
+    for grouped, graph in ((True, False), (True, True), (False, False)):
         m = CaMaFlood(src, opened_modules=("base", "adaptive_time"), device=device, step_duration=1800.0,
-                      groups=bs.group if grouped else None)
+                      groups=bs.group if grouped else None, use_graph=graph)
         for _ in range(max(args.warmup // 4, 2)):
             m.step_advance()
         torch.cuda.synchronize()
@@ -164,7 +166,7 @@ def basins_line(args, device: str) -> dict:
                 work += n * cnt
         torch.cuda.synchronize()
         el = time.perf_counter() - t1
-        key = "per_basin_dt" if grouped else "shared_dt"
+        key = ("per_basin_dt_graph" if graph else "per_basin_dt") if grouped else "shared_dt"
         # whole-run HBM rate: the fused passes' algorithmic bytes per
         # catchment-sub-step times the measured rate (wall clock, CFL pass and
         # launch gaps included)
@@ -180,7 +182,8 @@ def basins_line(args, device: str) -> dict:
     out["value"] = out["per_basin_dt"]["catchment_steps_per_s"]
     out["unit"] = "catchment-steps/s"
     out["speedup_vs_shared_dt"] = round(out["per_basin_dt"]["outer_steps_per_s"] / out["shared_dt"]["outer_steps_per_s"], 3)
-    log(f"basins: per-basin dt {out['per_basin_dt']['outer_steps_per_s']:.1f} outer steps/s, shared dt "
+    log(f"basins: per-basin dt {out['per_basin_dt']['outer_steps_per_s']:.1f} outer steps/s "
+        f"({out['per_basin_dt_graph']['outer_steps_per_s']:.1f} from graphs), shared dt "
         f"{out['shared_dt']['outer_steps_per_s']:.1f} ({out['speedup_vs_shared_dt']}x); HBM "
         f"{out['per_basin_dt']['GBps']:.0f} / {out['shared_dt']['GBps']:.0f} GB/s algorithmic")
     return out

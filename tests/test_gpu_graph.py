@@ -65,3 +65,31 @@ def test_graph_replays_match_eager_with_groups_and_log():
         assert np.array_equal(eager.last_group_log, graph.last_group_log), step
     eager.close()
     graph.close()
+
+
+@pytest.mark.parametrize("trial_layout", ["inner", "major"])
+def test_graph_replays_match_eager_with_members(trial_layout):
+    """Eight runoff members with their own sub-step counts, full physics:
+    each outer step one graph launch (the loop for the largest count, the
+    members past their count sitting out), bit-identical to the eager run."""
+    import torch
+    from cmfgpu_amd import synthetic
+    net, src = small_source()
+    M = 8
+    runoff = np.stack([synthetic.Runoff(src["catchment_area"], 4000 + m).next_day() * np.float32(1 + m)
+                       for m in range(M)])
+    eager, graph = _pair(src, FULL, num_trials=M, batched={"runoff": runoff}, per_trial_time_step=True,
+                         trial_layout=trial_layout)
+    for step in range(8):
+        eager.step_advance()
+        graph.step_advance()
+        torch.cuda.synchronize()
+        counts = eager.t["max_sub_steps"].cpu().numpy()
+        assert np.array_equal(counts, graph.t["max_sub_steps"].cpu().numpy()), step
+        for name in STATE_FIELDS:
+            if name in eager.t:
+                msg = mismatch(graph.get(name), eager.get(name))
+                assert msg is None, f"step {step} {name}: {msg}"
+    assert len(set(counts.tolist())) > 1    # the members really took different counts
+    eager.close()
+    graph.close()
