@@ -4,6 +4,8 @@
 #include "cmf_kernels.hip"
 #include "cmf_adjoint.hip"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -330,26 +332,29 @@ struct Timing {
 };
 thread_local Timing* g_timing = nullptr;
 
-inline void timing_mark(const char* name, hipStream_t s, bool begin) {
-    Timing* t = g_timing;
-    if (!t || t->count >= Timing::kMax) return;
-    if (begin) {
-        t->name[t->count] = name;
-        (void)hipEventRecord(t->ev[2 * t->count], s);
-    } else {
-        (void)hipEventRecord(t->ev[2 * t->count + 1], s);
-        t->count++;
-    }
-}
+// Launch `kernel`.  While a Timing is installed the launch goes through
+// hipExtLaunchKernelGGL with a pair of the Timing's events, which the dispatch
+// itself stamps when the kernel starts and ends: the duration is the
+// kernel's own, without the dispatch latency before it.
+#define CMF_DISPATCH(label, kernel, grid, block, smem, stream, ...)                           \
+    do {                                                                                      \
+        Timing* _t = g_timing;                                                                \
+        if (_t && _t->count < Timing::kMax) {                                                 \
+            _t->name[_t->count] = label;                                                      \
+            hipExtLaunchKernelGGL(kernel, grid, block, smem, stream, _t->ev[2 * _t->count],   \
+                                  _t->ev[2 * _t->count + 1], 0, __VA_ARGS__);                 \
+            _t->count++;                                                                      \
+        } else {                                                                              \
+            hipLaunchKernelGGL(kernel, grid, block, smem, stream, __VA_ARGS__);               \
+        }                                                                                     \
+    } while (0)
 
 #define LAUNCH(kernel, count, stream, ...)                                                   \
     do {                                                                                     \
         const int64_t _cnt = (count);                                                        \
         if (_cnt > 0) {                                                                      \
-            timing_mark(#kernel, (hipStream_t)(stream), true);                               \
-            hipLaunchKernelGGL(kernel, dim3((unsigned)blocks_for(_cnt)), dim3(kBlock), 0,    \
-                               (hipStream_t)(stream), __VA_ARGS__);                          \
-            timing_mark(#kernel, (hipStream_t)(stream), false);                              \
+            CMF_DISPATCH(#kernel, kernel, dim3((unsigned)blocks_for(_cnt)), dim3(kBlock), 0,  \
+                         (hipStream_t)(stream), __VA_ARGS__);                                \
             hipError_t _e = hipGetLastError();                                               \
             if (_e != hipSuccess)                                                            \
                 return set_error(CMF_ERR_HIP, "launch of %s failed: %s", #kernel,            \
@@ -364,10 +369,8 @@ constexpr uint32_t kLogStage = (1u << CMF_LOG_STORAGE_STAGE) | (1u << CMF_LOG_RI
 
 template <class REAL, class STO>
 int log_final(const View<REAL, STO>& v, int64_t nblocks, uint32_t mask, void* stream) {
-    timing_mark("k_log_final", (hipStream_t)stream, true);
-    hipLaunchKernelGGL((k_log_final<REAL, STO>), dim3(1), dim3(kBlock), 0, (hipStream_t)stream, v,
-                       nblocks, mask);
-    timing_mark("k_log_final", (hipStream_t)stream, false);
+    CMF_DISPATCH("k_log_final", (k_log_final<REAL, STO>), dim3(1), dim3(kBlock), 0, (hipStream_t)stream, v,
+                 nblocks, mask);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(CMF_ERR_HIP, "launch of k_log_final failed: %s", hipGetErrorString(e));
     return CMF_OK;
@@ -379,10 +382,8 @@ int log_final(const View<REAL, STO>& v, int64_t nblocks, uint32_t mask, void* st
         const int64_t _cnt = (count);                                                        \
         const int64_t _blk = (block);                                                        \
         if (_cnt > 0) {                                                                      \
-            timing_mark(#kernel, (hipStream_t)(stream), true);                               \
-            hipLaunchKernelGGL(kernel, dim3((unsigned)((_cnt + _blk - 1) / _blk)), dim3((unsigned)_blk), 0, \
-                               (hipStream_t)(stream), __VA_ARGS__);                          \
-            timing_mark(#kernel, (hipStream_t)(stream), false);                              \
+            CMF_DISPATCH(#kernel, kernel, dim3((unsigned)((_cnt + _blk - 1) / _blk)), dim3((unsigned)_blk), 0, \
+                         (hipStream_t)(stream), __VA_ARGS__);                                \
             hipError_t _e = hipGetLastError();                                               \
             if (_e != hipSuccess)                                                            \
                 return set_error(CMF_ERR_HIP, "launch of %s failed: %s", #kernel,            \
@@ -603,10 +604,8 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
     if (rc) return rc;
     if (log) {
         if (net->G > 0) {
-            timing_mark("k_log_final_grouped", (hipStream_t)stream, true);
-            hipLaunchKernelGGL((k_log_final_grouped<REAL, STO>), dim3((unsigned)net->G), dim3(kBlock), 0,
-                               (hipStream_t)stream, v, net->gblk_ptr, kLogAll, substep);
-            timing_mark("k_log_final_grouped", (hipStream_t)stream, false);
+            CMF_DISPATCH("k_log_final_grouped", (k_log_final_grouped<REAL, STO>), dim3((unsigned)net->G),
+                         dim3(kBlock), 0, (hipStream_t)stream, v, net->gblk_ptr, kLogAll, substep);
             CMF_HIP_CHECK(hipGetLastError());
         } else {
             // partials: one per block of the stage pass
