@@ -173,9 +173,11 @@ def basins_line(args, device: str) -> dict:
         f_wet = float((m.t["flood_depth"] > 0).float().mean().item())
         per = sum(algorithmic_bytes(k, m, f_wet) for k in ("k_outflow", "k_limiter_pair", "k_stage_fused")) / n
         gbs = work / el * per / 1e9
+        cgbs = work / el * fused_substep_bytes(n, 0, 0, f_wet) / n / 1e9
         out[key] = {"outer_steps_per_s": steps / el, "catchment_steps_per_s": work / el,
                     "bytes_per_catchment_step": round(per, 1), "GBps": round(gbs, 1),
-                    "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4), "f_wet": round(f_wet, 4)}
+                    "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4), "f_wet": round(f_wet, 4),
+                    "GBps_compulsory_fused": round(cgbs, 1), "frac_compulsory_fused": round(cgbs / HBM_PEAK_GBS, 4)}
         m.close()
         del m
         torch.cuda.empty_cache()
@@ -187,6 +189,38 @@ def basins_line(args, device: str) -> dict:
         f"{out['shared_dt']['outer_steps_per_s']:.1f} ({out['speedup_vs_shared_dt']}x); HBM "
         f"{out['per_basin_dt']['GBps']:.0f} / {out['shared_dt']['GBps']:.0f} GB/s algorithmic")
     return out
+
+
+def fused_substep_bytes(N: int, V: int, P: int, f_wet: float, LB: int = 5) -> float:
+    """Compulsory HBM traffic of one fully fused sub-step (every field read
+    or written once, gathers as cache hits): 128 B per catchment plus 44 B
+    per wet catchment (area and floodplain row); levees add the protected
+    storage and depth read and written (24 B per catchment) and 16 B of
+    parameters per levee; paths add 16 B per catchment (water surface, total
+    storage and limiter) and per path LB levels of Manning, width, elevation
+    (read) and flow and depth (read and written), its length and two ends."""
+    b = N * (128.0 + 44.0 * f_wet)
+    if V:
+        b += 24.0 * N + 16.0 * V
+    if P:
+        b += 16.0 * N + P * (LB * (4 + 4 + 4 + 8 + 8) + 4 + 8)
+    return b
+
+
+def fused_batched_bytes(N: int, M: int, V: int, P: int, f_wet: float, LB: int = 5) -> float:
+    """The same for M members sharing the static parameters: per
+    member-catchment runoff, state read and written and flood fraction (96 B),
+    with levees the protected state (24 B), with paths the surface, total
+    storage and limiter (16 B); per catchment the eight static fields (32 B)
+    and, where any member is wet, area and floodplain row (44 B); per levee
+    16 B of parameters; per path-member the level flows and depths read and
+    written (LB x 16 B); per path the level parameters, length and ends."""
+    b = N * M * (96.0 + (24.0 if V else 0.0) + (16.0 if P else 0.0)) + N * (32.0 + 44.0 * f_wet)
+    if V:
+        b += 16.0 * V
+    if P:
+        b += P * M * LB * 16.0 + P * (LB * 12.0 + 12.0)
+    return b
 
 
 def batched_full_bytes(N: int, M: int, P: int, f_wet: float) -> float:
@@ -575,11 +609,18 @@ def main():
     bytes_all = sum(algorithmic_bytes(k, model, f_wet) for k in ktimes)
     log("kernels (ms): " + ", ".join(f"{k}={v:.4f}" for k, v in sorted(ktimes.items(), key=lambda x: -x[1])))
     traffic = read_traffic(dominant)
+    # the whole sub-step against the compulsory traffic of a fully fused one
+    # (wall clock of the timed run, CFL pass and host sync included)
+    fused_b = fused_substep_bytes(model.N, model.V, model.P, f_wet)
+    us_sub = elapsed / substeps * 1e6
+    fused_gbs = fused_b / (us_sub * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dominant,
                 "algorithmic_bytes": int(bytes_dom), "kernel_ms": round(ktimes[dominant], 5),
                 "substep_kernels_ms": round(substep_ms, 5),
                 "substep_frac": round(bytes_all / (substep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "fused_substep": {"compulsory_bytes": int(fused_b), "us_per_substep": round(us_sub, 2),
+                                  "GBps": round(fused_gbs, 1), "frac": round(fused_gbs / HBM_PEAK_GBS, 4)},
                 "f_wet": round(f_wet, 4)}
 
     cpu = None
@@ -609,9 +650,11 @@ def main():
         # whole-run algorithmic rate, shared parameters counted once per catchment
         bf_wet = float((bm.t["flood_depth"] > 0).float().mean().item())
         bgbs = batched_full_bytes(bm.N, M, bm.P, bf_wet) * (bwork / (bm.N * M)) / belapsed / 1e9
+        cgbs = fused_batched_bytes(bm.N, M, bm.V, bm.P, bf_wet) * (bwork / (bm.N * M)) / belapsed / 1e9
         batched = {"members": M, "value": bvalue, "unit": "member-catchment-steps/s", "steps": bsteps,
                    "vs_plain": round(bvalue / value, 3), "GBps_algorithmic": round(bgbs, 1),
-                   "frac_of_peak": round(bgbs / HBM_PEAK_GBS, 4)}
+                   "frac_of_peak": round(bgbs / HBM_PEAK_GBS, 4),
+                   "GBps_compulsory_fused": round(cgbs, 1), "frac_compulsory_fused": round(cgbs / HBM_PEAK_GBS, 4)}
         log(f"batched: {M} members, {bsteps} outer steps, {belapsed:.3f} s -> {bvalue:.4g} "
             f"member-catchment-steps/s ({bvalue / value:.2f}x plain), {bgbs:.0f} GB/s algorithmic")
         bm.close()
