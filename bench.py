@@ -508,10 +508,19 @@ def cpu_baseline(model, net, budget_s: float):
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = time.perf_counter() - t0
+    # one thread: one outer step more from where the sample ended
+    from oracle.oracle import set_threads
+    prev = set_threads(1)
+    t1 = time.perf_counter()
+    one_sub = int(orc.advance(1800.0, 1)[0])
+    dt1 = time.perf_counter() - t1
+    set_threads(prev)
     return {"value": net.num_catchments * substeps / dt, "unit": "catchment-steps/s", "cores": threads,
             "kind": "port",
             "sample": f"{steps} outer steps ({substeps} CFL sub-steps) of the same network from the GPU "
-                      f"run's state, CPU oracle (C++ restatement, OpenMP, {threads} threads), {dt:.1f} s"}
+                      f"run's state, CPU oracle (C++ restatement, OpenMP, {threads} threads), {dt:.1f} s",
+            "one_thread": {"value": net.num_catchments * one_sub / dt1, "unit": "catchment-steps/s",
+                           "sample": f"the next outer step ({one_sub} CFL sub-steps) on one thread, {dt1:.1f} s"}}
 
 
 def read_traffic(kernel: str):

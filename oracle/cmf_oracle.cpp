@@ -50,6 +50,7 @@
 #include <vector>
 #include <algorithm>
 #include <limits>
+#include <omp.h>
 
 namespace {
 
@@ -901,6 +902,10 @@ int run_op(int op, void* const* ptrs, const int64_t* dims, const double* scalars
 }  // namespace
 
 extern "C" {
+
+// Threads of the OpenMP loops (the CPU baseline times one thread and all).
+void cmf_oracle_set_threads(int threads) { omp_set_num_threads(threads > 0 ? threads : 1); }
+int cmf_oracle_max_threads() { return omp_get_max_threads(); }
 
 // Advance `num_outer` outer steps of `outer_seconds` each.  With HAS_ADAPTIVE
 // the sub-step count comes from the CFL rule; otherwise `fixed_count` is used.

@@ -63,6 +63,10 @@ def lib():
         h.cmf_oracle_advance.restype = ctypes.c_int
         h.cmf_oracle_advance.argtypes = [ctypes.c_int, vpp, i64p, f64p, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
+        h.cmf_oracle_set_threads.restype = None
+        h.cmf_oracle_set_threads.argtypes = [ctypes.c_int]
+        h.cmf_oracle_max_threads.restype = ctypes.c_int
+        h.cmf_oracle_max_threads.argtypes = []
         h.cmf_oracle_adaptive_steps.restype = ctypes.c_int
         h.cmf_oracle_adaptive_steps.argtypes = [ctypes.c_int, vpp, i64p, f64p, ctypes.c_int, ctypes.c_double]
         h.cmf_oracle_kernel.restype = ctypes.c_int
@@ -80,6 +84,14 @@ def lib():
             raise RuntimeError("oracle field table out of sync")
         _LIB = h
     return _LIB
+
+
+def set_threads(threads: int) -> int:
+    """Set the oracle's OpenMP thread count; returns the previous one."""
+    h = lib()
+    prev = int(h.cmf_oracle_max_threads())
+    h.cmf_oracle_set_threads(int(threads))
+    return prev
 
 
 class OracleModel:
