@@ -17,9 +17,10 @@ The initial state and every other parameter are held fixed.
 
 ``GaugeLoss.apply(manning, height, problem)`` wraps this in a
 torch.autograd.Function.  The base physics, the bifurcation paths (whose
-per-level flows and depths join the state) and the levee stage (whose
-protected storage and depth join the state) are differentiated; the
-reservoir module, trials and grouped networks are refused.
+per-level flows and depths join the state), the levee stage (whose protected
+storage and depth join the state) and the reservoirs (whose gathered inflow
+joins the state) are differentiated; trials and grouped networks are
+refused.
 """
 from __future__ import annotations
 
@@ -34,6 +35,7 @@ STATE = ("river_outflow", "flood_outflow", "river_cross_section_depth", "flood_c
          "flood_cross_section_area", "river_storage", "flood_storage", "river_depth", "flood_depth")
 PATH_STATE = ("bifurcation_outflow", "bifurcation_cross_section_depth")
 LEVEE_STATE = ("protected_storage", "protected_depth")
+RES_STATE = ("reservoir_total_inflow",)
 QR, QF = 0, 1
 
 
@@ -62,6 +64,9 @@ class CmfAdjoint(ctypes.Structure):
     ] + [(name, ctypes.c_void_p) for name in LEVEE_STATE] + [
         ("adj_out_levee", ctypes.c_void_p * 2),
         ("adj_in_levee", ctypes.c_void_p * 2),
+        ("reservoir_total_inflow", ctypes.c_void_p),
+        ("adj_out_reservoir", ctypes.c_void_p),
+        ("adj_in_reservoir", ctypes.c_void_p),
     ]
 
 
@@ -72,7 +77,7 @@ class CmfTape(ctypes.Structure):
     """Mirror of struct cmf_tape."""
 
     _fields_ = [(name, ctypes.c_void_p) for name in STATE + INTERMEDIATES + PATH_STATE + ("path_terms",)
-                + LEVEE_STATE]
+                + LEVEE_STATE + RES_STATE]
 
 
 _SIG_DONE = False
@@ -104,16 +109,17 @@ class GaugeProblem:
     callable step -> array) and the checkpoint interval K."""
 
     def __init__(self, model, gauges, observed, runoff=None, checkpoint_every: int = 48, eps: float = 1e-3):
-        extra = set(model.opened_modules) - {"base", "adaptive_time", "bifurcation", "levee"}
+        extra = set(model.opened_modules) - {"base", "adaptive_time", "bifurcation", "levee", "reservoir"}
         if extra:
-            raise ValueError(f"the adjoint covers the base physics, bifurcation and levee; modules {sorted(extra)} "
-                             "are open")
+            raise ValueError(f"the adjoint covers the base physics, bifurcation, levee and reservoir; modules "
+                             f"{sorted(extra)} are open")
         if model.T != 1 or model.G or model.batched_names or model.mode != "fused":
             raise ValueError("the adjoint needs one trial, an ungrouped network, no per-trial parameters and "
                              "the fused schedule")
         self.model = model
         self.paths = "bifurcation" in model.opened_modules and model.P > 0
         self.levees = "levee" in model.opened_modules
+        self.reservoirs = "reservoir" in model.opened_modules
         self.gauges = np.asarray(gauges, dtype=np.int64)
         self.obs = torch.as_tensor(np.asarray(observed, dtype=np.float64), device=model.device)
         if self.obs.ndim != 2 or self.obs.shape[1] != self.gauges.shape[0]:
@@ -131,7 +137,8 @@ class GaugeProblem:
 
     # ------------------------------------------------------------ state --
     def _state(self):
-        names = STATE + (PATH_STATE if self.paths else ()) + (LEVEE_STATE if self.levees else ())
+        names = (STATE + (PATH_STATE if self.paths else ()) + (LEVEE_STATE if self.levees else ())
+                 + (RES_STATE if self.reservoirs else ()))
         return [self.model.t[name] for name in names]
 
     def _snapshot(self):
@@ -226,6 +233,7 @@ class GaugeProblem:
         lam_p = [[torch.zeros(pe, dtype=real_t, device=dev) for _ in PATH_STATE] for _ in range(2)]
         lam_l = [[torch.zeros(n if self.levees else 0, dtype=real_t, device=dev) for _ in LEVEE_STATE]
                  for _ in range(2)]
+        lam_r = [torch.zeros(n if self.reservoirs else 0, dtype=real_t, device=dev) for _ in range(2)]
         g_n = torch.zeros(n, dtype=torch.float64, device=dev)
         g_h = torch.zeros(n, dtype=torch.float64, device=dev)
         starts = sorted(self.checkpoints)
@@ -239,6 +247,8 @@ class GaugeProblem:
         # with levees the input protected storage (STO) and depth (REAL)
         tape_ls = torch.empty((slots, n), dtype=sto_t, device=dev) if self.levees else None
         tape_ld = torch.empty((slots, n), dtype=real_t, device=dev) if self.levees else None
+        # with reservoirs the inflow each gathered in the previous sub-step (STO)
+        tape_ri = torch.empty((slots, n), dtype=sto_t, device=dev) if self.reservoirs else None
         tapes, adjs = [], []
         for k in range(slots):
             ptrs = {name: tape_r[k, x].data_ptr() for x, name in enumerate(TAPE_REAL)}
@@ -249,6 +259,8 @@ class GaugeProblem:
             if self.levees:
                 ptrs["protected_storage"] = tape_ls[k].data_ptr()
                 ptrs["protected_depth"] = tape_ld[k].data_ptr()
+            if self.reservoirs:
+                ptrs["reservoir_total_inflow"] = tape_ri[k].data_ptr()
             tp = CmfTape(**ptrs)
             pair = []
             for parity in range(2):
@@ -264,6 +276,9 @@ class GaugeProblem:
                     for x in range(len(LEVEE_STATE)):
                         a.adj_out_levee[x] = lam_l[parity][x].data_ptr()
                         a.adj_in_levee[x] = lam_l[1 - parity][x].data_ptr()
+                if self.reservoirs:
+                    a.adj_out_reservoir = lam_r[parity].data_ptr()
+                    a.adj_in_reservoir = lam_r[1 - parity].data_ptr()
                 a.grad_river_manning = g_n.data_ptr()
                 a.grad_river_height = g_h.data_ptr()
                 pair.append(a)
@@ -308,7 +323,7 @@ class GaugeProblem:
         fl.mode = ADJ_FLUSH
         _lib.check(lib.cmf_hip_substep_adjoint(m.network_handle, m.field_table_ref(), ctypes.byref(fl),
                                                _lib.current_stream()), "cmf_hip_substep_adjoint")
-        del tape_r, tape_s, tape_p, tape_pt, tape_ls, tape_ld
+        del tape_r, tape_s, tape_p, tape_pt, tape_ls, tape_ld, tape_ri
         self._restore(self.final)
         inv = torch.as_tensor(m.inv, device=dev)
         return g_n[inv], g_h[inv]

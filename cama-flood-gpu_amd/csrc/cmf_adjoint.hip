@@ -7,8 +7,10 @@
 // the 5 % storage limiter of a path, the path terms of the outgoing totals and
 // the path nets limited by the end catchments; with levees the variant of
 // cmfgpu/phys/triton/levee.py:565-684) and of the levee stage
-// (cmfgpu/phys/triton/levee.py:62-304 with the partition :18-59).  The
-// reservoir module is not differentiated.
+// (cmfgpu/phys/triton/levee.py:62-304 with the partition :18-59) and of the
+// reservoirs (the kinematic flow upstream of a dam, outflow.py:251-272, and
+// the release by storage regime, reservoir.py:17-184, from the inflow the
+// previous sub-step gathered).
 //
 // The sub-step maps the state x = (river/flood outflow, river/flood/flood-area
 // cross-section depth, river/flood storage, river/flood depth) and the
@@ -75,6 +77,13 @@ struct AdjView {
     const void* ps_raw;
     const REAL *pd, *lout_ps, *lout_pd;
     REAL *lin_ps, *lin_pd;
+    // reservoirs (res: the module is open): the inflow each reservoir
+    // gathered in the previous sub-step (STO) joins the state
+    bool res;
+    const void* ri_raw;
+    const REAL* lout_ri;
+    REAL* lin_ri;
+    int32_t* res_row;     // catchment -> reservoir row (reservoir catchments only)
 };
 
 constexpr int kAdjMaxLevels = 8;
@@ -91,6 +100,13 @@ struct OutTrace {
     REAL wse, wse_dn, slope, fslope, upd_rcsd, r_sifd_raw, r_sifd, fsurf, upd_fcsd, f_sifd_raw, f_sifd;
     REAL upd_fcsa_raw, upd_fcsa, f_csa_m, f_imp_raw, f_imp, unit_r, den_r, qr0, den_f, qf0, qf1, tot, total_neg;
     REAL ratio, limit, qr, qf;
+    // upstream of a dam: the kinematic flows (raw, and which clamps held)
+    bool dam, kr_cap, kr_pos, kf_cap, kf_pos;
+    REAL kr_raw, kf_raw, kf_area_raw, kf_coef;
+    // a reservoir catchment: the release (regime 1-4, picks and clamps)
+    bool reservoir, rel_flood, rel_oflood, rel_spill, rel_pos, rel_cap;
+    int regime;
+    REAL rel_total, rel_inflow, rel_frac, rel_raw;
 };
 
 template <class REAL, class STO>
@@ -182,6 +198,82 @@ __device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v
         o.limit = vmin(o.ratio, (REAL)1.0);
         q_r = q_r * o.limit;
         q_f = q_f * o.limit;
+    }
+    o.dam = false;
+    if (a.res && v.is_dam_up[i]) {
+        // kinematic flow upstream of a dam (outflow.py:251-272)
+        o.dam = true;
+        REAL bed_slope = (c_elv - c_elv_dn) / dn_dist;
+        bed_slope = vmax(bed_slope, v.min_kin);
+        const REAL kin_riv_vel = ((REAL)1.0 / r_man) * dsqrt(bed_slope) * dcbrt(r_dep * r_dep);
+        o.kr_raw = r_wid * r_dep * kin_riv_vel;
+        o.kr_cap = !(o.kr_raw < rs / time_step);
+        REAL kin_riv = vmin(o.kr_raw, rs / time_step);
+        o.kr_pos = kin_riv > (REAL)0.0;
+        kin_riv = vmax(kin_riv, (REAL)0.0);
+        const REAL bed_slope_f = vmin(bed_slope, (REAL)0.005);
+        o.kf_coef = ((REAL)1.0 / f_man) * dsqrt(bed_slope_f);
+        const REAL kin_fld_vel = o.kf_coef * dcbrt(f_dep * f_dep);
+        o.kf_area_raw = fs / r_len - f_dep * r_wid;
+        const REAL kin_fld_area = vmax(o.kf_area_raw, (REAL)0.0);
+        o.kf_raw = kin_fld_area * kin_fld_vel;
+        o.kf_cap = !(o.kf_raw < fs / time_step);
+        REAL kin_fld = vmin(o.kf_raw, fs / time_step);
+        o.kf_pos = kin_fld > (REAL)0.0;
+        kin_fld = vmax(kin_fld, (REAL)0.0);
+        q_r = kin_riv;
+        q_f = kin_fld;
+    }
+    o.reservoir = a.res && v.is_reservoir[i];
+    if (o.reservoir) {
+        // release by storage regime (k_reservoir, reservoir.py:114-163)
+        const int64_t r = a.res_row[i];
+        const REAL total = rs + fs;
+        const REAL inflow = (REAL)adj_sto<STO>(a.ri_raw, i) + v.runoff[i];
+        const REAL cv = v.res_cons[r], ev = v.res_emer[r], av = v.res_adj[r];
+        const REAL qn = v.res_qn[r], qa = v.res_qa[r], qf = v.res_qf[r];
+        o.rel_total = total;
+        o.rel_inflow = inflow;
+        REAL out = (REAL)0;
+        o.regime = 0;
+        o.rel_flood = o.rel_oflood = o.rel_spill = false;
+        o.rel_frac = (REAL)0;
+        if (total <= cv) {
+            out = qn * dsqrt(total / cv);
+            o.regime = 1;
+            o.rel_frac = total / cv;
+        }
+        if ((total > cv) && (total <= av)) {
+            const REAL frac2 = (total - cv) / (av - cv);
+            out = qn + dexp((REAL)3.0 * dlog(frac2)) * (qa - qn);
+            o.regime = 2;
+            o.rel_frac = frac2;
+        }
+        if ((total > av) && (total <= ev)) {
+            o.rel_flood = inflow >= qf;
+            const REAL frac3 = (total - av) / (ev - av);
+            const REAL tmp = qa + dexp((REAL)0.1 * dlog(frac3)) * (qf - qa);
+            o.rel_frac = frac3;
+            if (o.rel_flood) {
+                const REAL o_flood = qn + ((total - cv) / (ev - cv)) * (inflow - qn);
+                o.rel_oflood = (o_flood > tmp || tmp != tmp);
+                out = vmax(o_flood, tmp);
+            } else {
+                out = tmp;
+            }
+            o.regime = 3;
+        }
+        if (total > ev) {
+            o.rel_spill = inflow >= qf;
+            out = o.rel_spill ? inflow : qf;
+            o.regime = 4;
+        }
+        o.rel_raw = out;
+        o.rel_pos = out > (REAL)0.0;
+        const REAL lo = vmax(out, (REAL)0.0);
+        o.rel_cap = !(lo < total / time_step);
+        q_r = vclamp(out, (REAL)0.0, total / time_step);
+        q_f = (REAL)0.0;
     }
     o.qr = q_r;
     o.qf = q_f;
@@ -414,6 +506,14 @@ __device__ __forceinline__ void levee_reverse(const LeveeTrace<REAL>& t, STO rs_
     Lfd = Lfdep;
 }
 
+// reservoir row of each reservoir catchment
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_adj_res_rows(View<REAL, STO> v, AdjView<REAL> a) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= v.R) return;
+    a.res_row[v.res_idx[r]] = (int32_t)r;
+}
+
 // A0: unlimited outflows and outgoing terms of the saved state
 template <class REAL, class STO>
 __global__ __launch_bounds__(kBlock) void k_adj_outflow_fwd(View<REAL, STO> v, AdjView<REAL> a) {
@@ -627,8 +727,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_route(View<REAL, STO> v, AdjView
     const bool mouth = (d == k);
     const REAL qr = a.qr[k], qf = a.qf[k];
     const REAL l_own = a.lim[k], l_dn = a.lim[d];
-    const double LR = (double)a.a_rout[k] + (mouth ? 0.0 : (double)a.a_ri[d]);
-    const double LF = (double)a.a_fout[k] + (mouth ? 0.0 : (double)a.a_fi[d]);
+    // a reservoir downstream also gathers the limited flows into its inflow
+    const double Lres = (a.res && !mouth && v.is_reservoir[d]) ? (double)a.lout_ri[d] : 0.0;
+    const double LR = (double)a.a_rout[k] + (mouth ? 0.0 : (double)a.a_ri[d]) + Lres;
+    const double LF = (double)a.a_fout[k] + (mouth ? 0.0 : (double)a.a_fi[d]) + Lres;
     a.a_qr[k] = (REAL)(LR * (double)((qr >= (REAL)0.0) ? l_own : l_dn));
     a.a_qf[k] = (REAL)(LF * (double)((qf >= (REAL)0.0) ? l_own : l_dn));
     double Llim = 0.0;
@@ -892,6 +994,64 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
     if (o.qf > (REAL)0.0) Lqf += og_own * dt;
     if (o.qr < (REAL)0.0) Lqr -= og_dn * dt;
     if (o.qf < (REAL)0.0) Lqf -= og_dn * dt;
+    // a reservoir's release or the kinematic flow upstream of a dam replaces
+    // the momentum update's flows, whose reverse then gets no adjoint
+    double Lrs_x = 0.0, Lfs_x = 0.0, Lrd_x = 0.0, Lfd_x = 0.0, gn_x = 0.0, Lri = 0.0;
+    if (o.reservoir) {
+        const int64_t r = a.res_row[i];
+        const double cv = (double)v.res_cons[r], ev = (double)v.res_emer[r], av = (double)v.res_adj[r];
+        const double qn = (double)v.res_qn[r], qa = (double)v.res_qa[r], qf = (double)v.res_qf[r];
+        const double total = (double)o.rel_total, fr = (double)o.rel_frac;
+        double Ltotal = 0.0, Lraw = 0.0;
+        // q = clamp(raw, 0, total / dt)
+        if (o.rel_cap) Ltotal += Lqr / dt;
+        else if (o.rel_pos) Lraw = Lqr;
+        if (o.regime == 1) {
+            if (fr > 0.0) Ltotal += Lraw * qn * 0.5 / sqrt(fr) / cv;
+        } else if (o.regime == 2) {
+            Ltotal += Lraw * 3.0 * fr * fr * (qa - qn) / (av - cv);
+        } else if (o.regime == 3) {
+            if (o.rel_flood && o.rel_oflood) {
+                Ltotal += Lraw * ((double)o.rel_inflow - qn) / (ev - cv);
+                Lri += Lraw * (total - cv) / (ev - cv);
+            } else if (fr > 0.0) {
+                Ltotal += Lraw * 0.1 * pow(fr, -0.9) * (qf - qa) / (ev - av);
+            }
+        } else if (o.regime == 4) {
+            if (o.rel_spill) Lri += Lraw;
+        }
+        Lrs_x += Ltotal;
+        Lfs_x += Ltotal;
+        Lqr = 0.0;
+        Lqf = 0.0;
+    } else if (o.dam) {
+        // q_r = max(min(W h sqrt(S) / n h^(2/3), rs / dt), 0), q_f likewise
+        const double h = (double)a.rd[i], fdp = (double)a.fd[i];
+        if (o.kr_pos) {
+            if (o.kr_cap) {
+                Lrs_x += Lqr / dt;
+            } else {
+                const double kr = (double)o.kr_raw;
+                gn_x -= Lqr * kr / (double)v.river_manning[i];
+                if (h > 0.0) Lrd_x += Lqr * (5.0 / 3.0) * kr / h;
+            }
+        }
+        if (o.kf_pos) {
+            if (o.kf_cap) {
+                Lfs_x += Lqf / dt;
+            } else {
+                const double kf = (double)o.kf_raw;
+                if (o.kf_area_raw > (REAL)0.0) {
+                    const double Larea = Lqf * kf / (double)o.kf_area_raw;
+                    Lfs_x += Larea / (double)v.river_length[i];
+                    Lfd_x -= Larea * (double)v.river_width[i];
+                }
+                if (fdp > 0.0) Lfd_x += Lqf * (2.0 / 3.0) * kf / fdp;
+            }
+        }
+        Lqr = 0.0;
+        Lqf = 0.0;
+    }
     // negative-flow limiter at 5 % of the storage
     double Lqr0, Lqf1, Ltot = 0.0;
     if (o.negative) {
@@ -991,6 +1151,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
         a.lin_ps[i] = (REAL)((double)a.lin_ps[i] + Ltot);
         a.lin_pd[i] = (REAL)Lpd;
     }
+    if (a.res) a.lin_ri[i] = (REAL)Lri;
     const double dist = (double)v.dist[i];
     Lwse += Lslope / dist;
     Lwse_dn -= Lslope / dist;
@@ -1003,11 +1164,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
     a.lin[ADJ_RCSD][i] = (REAL)Lrcsd;
     a.lin[ADJ_FCSD][i] = (REAL)Lfcsd;
     a.lin[ADJ_FCSA][i] = (REAL)Lfcsa;
-    a.lin[ADJ_RS][i] = (REAL)((double)a.lin[ADJ_RS][i] + Ltot);
-    a.lin[ADJ_FS][i] = (REAL)((double)a.lin[ADJ_FS][i] + Ltot + Lfs);
-    a.lin[ADJ_RD][i] = (REAL)Lrd;
-    a.lin[ADJ_FD][i] = (REAL)Lfd;
-    a.g_n[i] += gn;
+    a.lin[ADJ_RS][i] = (REAL)((double)a.lin[ADJ_RS][i] + Ltot + Lrs_x);
+    a.lin[ADJ_FS][i] = (REAL)((double)a.lin[ADJ_FS][i] + Ltot + Lfs + Lfs_x);
+    a.lin[ADJ_RD][i] = (REAL)(Lrd + Lrd_x);
+    a.lin[ADJ_FD][i] = (REAL)(Lfd + Lfd_x);
+    a.g_n[i] += gn + gn_x;
     a.g_h[i] += -Lre;
     a.d_wd[i] = o.mouth ? (REAL)0.0 : (REAL)Lwse_dn;
 }

@@ -90,6 +90,7 @@ struct cmf_network {
     // 14 arrays of N elements of up to 8 bytes
     void* adj_scratch = nullptr;
     void* adj_path_scratch = nullptr;   // with paths: [P, LB] + [P] x 9
+    void* adj_res_rows = nullptr;       // with reservoirs: catchment -> reservoir row
 };
 
 namespace {
@@ -109,6 +110,7 @@ void free_network(cmf_network* net) {
     if (net->capture_stream) (void)hipStreamDestroy(net->capture_stream);
     if (net->adj_scratch) (void)hipFree(net->adj_scratch);
     if (net->adj_path_scratch) (void)hipFree(net->adj_path_scratch);
+    if (net->adj_res_rows) (void)hipFree(net->adj_res_rows);
     delete net;
 }
 
@@ -817,8 +819,8 @@ int run_substep_taped(cmf_network* net, const cmf_fields* f, int substep, int st
     if (rc) return rc;
     if ((rc = need_base(f, "substep_taped")) || (rc = need_features(f, "substep_taped"))) return rc;
     if (!tp) return set_error(CMF_ERR_INVALID, "substep_taped: null tape");
-    if (v.T != 1 || net->G > 0 || (f->flags & CMF_HAS_RESERVOIR))
-        return set_error(CMF_ERR_UNSUPPORTED, "substep_taped: no reservoir module, one trial, ungrouped");
+    if (v.T != 1 || net->G > 0)
+        return set_error(CMF_ERR_UNSUPPORTED, "substep_taped: one trial, ungrouped");
     void* ptrs[] = {tp->river_outflow, tp->flood_outflow, tp->river_cross_section_depth,
                     tp->flood_cross_section_depth, tp->flood_cross_section_area, tp->river_storage,
                     tp->flood_storage, tp->river_depth, tp->flood_depth, tp->qraw_r, tp->qraw_f, tp->lim};
@@ -843,6 +845,11 @@ int run_substep_taped(cmf_network* net, const cmf_fields* f, int substep, int st
         v.tape_ps = static_cast<STO*>(tp->protected_storage);
         v.tape_pd = static_cast<REAL*>(tp->protected_depth);
     }
+    if (v.has(CMF_HAS_RESERVOIR)) {
+        if (!tp->reservoir_total_inflow)
+            return set_error(CMF_ERR_INVALID, "substep_taped: the reservoir inflow slot is needed with reservoirs");
+        v.tape_ri = static_cast<STO*>(tp->reservoir_total_inflow);
+    }
     if (v.has(CMF_HAS_BIFURCATION) && v.P > 0) {
         if (!tp->bifurcation_outflow || !tp->bifurcation_cross_section_depth)
             return set_error(CMF_ERR_INVALID, "substep_taped: the path state slots are needed with bifurcation");
@@ -866,8 +873,7 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
     int rc = make_view<REAL, STO>(net, f, v);
     if (rc) return rc;
     if (!adj) return set_error(CMF_ERR_INVALID, "substep_adjoint: null adjoint table");
-    if (f->flags & CMF_HAS_RESERVOIR)
-        return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: the reservoir module is not differentiated");
+
     if (v.T != 1 || net->G > 0 || f->batched)
         return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: one trial, ungrouped, no per-trial parameters");
     if ((rc = require({NEED(downstream_idx), NEED(river_manning), NEED(flood_manning), NEED(river_width),
@@ -911,7 +917,21 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
             return set_error(CMF_ERR_INVALID, "substep_adjoint: the protected storage and depth and their adjoints "
                                               "are needed with levees");
     }
+    const bool resv = v.has(CMF_HAS_RESERVOIR);
+    if (resv) {
+        if ((rc = require({NEED(reservoir_catchment_idx), NEED(conservation_volume), NEED(emergency_volume),
+                           NEED(adjustment_volume), NEED(effective_normal_outflow), NEED(adjustment_outflow),
+                           NEED(flood_control_outflow), NEED(is_reservoir), NEED(is_dam_upstream)},
+                          "substep_adjoint")))
+            return rc;
+        if (!(adj->mode & CMF_ADJ_FLUSH) &&
+            (!adj->reservoir_total_inflow || !adj->adj_out_reservoir || !adj->adj_in_reservoir || !adj->qraw_r ||
+             !adj->qraw_f || !adj->lim))
+            return set_error(CMF_ERR_INVALID, "substep_adjoint: with reservoirs the gathered inflow, its adjoints "
+                                              "and the taped intermediates are needed");
+    }
     if (!net->adj_scratch && (rc = dev_alloc(&net->adj_scratch, (size_t)n * 8 * 14))) return rc;
+    if (resv && !net->adj_res_rows && (rc = dev_alloc(&net->adj_res_rows, (size_t)n * 4))) return rc;
     if (bif && !net->adj_path_scratch &&
         (rc = dev_alloc(&net->adj_path_scratch, (size_t)v.P * 8 * (size_t)(v.LB + 9))))
         return rc;
@@ -940,6 +960,13 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
     a.a_og = w[11]; a.d_wd = w[12]; a.a_gb = w[13];
     a.bif = bif;
     a.lev = lev;
+    a.res = resv;
+    if (resv) {
+        a.ri_raw = adj->reservoir_total_inflow;
+        a.lout_ri = static_cast<const REAL*>(adj->adj_out_reservoir);
+        a.lin_ri = static_cast<REAL*>(adj->adj_in_reservoir);
+        a.res_row = static_cast<int32_t*>(net->adj_res_rows);
+    }
     if (lev) {
         a.ps_raw = adj->protected_storage;
         a.pd = static_cast<const REAL*>(adj->protected_depth);
@@ -982,6 +1009,7 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
         else LAUNCH((k_adj_gather<REAL, STO, false>), n, stream, v, a);
         return CMF_OK;
     }
+    if (resv) LAUNCH((k_adj_res_rows<REAL, STO>), v.R, stream, v, a);
     if (bif && !adj->path_terms) LAUNCH_B((k_adj_path_fwd<REAL, STO>), v.P, 64, stream, v, a);
     if (!taped) {
         LAUNCH((k_adj_outflow_fwd<REAL, STO>), n, stream, v, a);

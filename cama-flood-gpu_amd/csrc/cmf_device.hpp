@@ -172,6 +172,7 @@ struct View {
     REAL* tape_bqlim;             // and its limited level flows
     STO* tape_ps;                 // the outflow pass: input protected storage
     REAL* tape_pd;                // and depth (levee module)
+    STO* tape_ri;                 // and the reservoirs' gathered inflow (reservoir module)
     // output statistics: sum over sub-steps of (river + flood outflow) * dt
     // per element (nullptr: off)
     double* acc_outflow;

@@ -181,6 +181,7 @@ __device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int subste
             v.tape_ps[e] = v.protected_storage[e];
             v.tape_pd[e] = v.protected_depth[e];
         }
+        if (v.tape_ri) v.tape_ri[e] = v.res_inflow[e];
     }
     if (FUSED) {
         v.qraw_r[e] = q_r;

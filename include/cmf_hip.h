@@ -392,9 +392,11 @@ int cmf_network_set_outflow_accumulator(cmf_network* net, double* accumulator);
  * adjoints after (adj_out_path) and before (adj_in_path) the sub-step; with
  * the levee module open (cmfgpu/phys/triton/levee.py:62-304, and the path
  * variant :565-684) so are the protected storage and depth, and the field
- * table must hold levee_index.  The reservoir module, trials, groups and
- * per-trial parameters return CMF_ERR_UNSUPPORTED.  No reference counterpart
- * (the reference has no differentiation). */
+ * table must hold levee_index; with the reservoir module (the kinematic flow
+ * upstream of dams, outflow.py:251-272, and the release,
+ * reservoir.py:17-184) the reservoirs' gathered inflow is.  Trials, groups
+ * and per-trial parameters return CMF_ERR_UNSUPPORTED.  No reference
+ * counterpart (the reference has no differentiation). */
 typedef struct cmf_adjoint {
     const void* river_outflow;
     const void* flood_outflow;
@@ -427,6 +429,12 @@ typedef struct cmf_adjoint {
     const void* protected_depth;
     const void* adj_out_levee[2];
     void* adj_in_levee[2];
+    /* reservoirs: reservoir_total_inflow (STO [N], the inflow each reservoir
+     * gathered in the previous sub-step) joins the state; needs the taped
+     * intermediates (qraw_r, qraw_f, lim) */
+    const void* reservoir_total_inflow;
+    const void* adj_out_reservoir;
+    void* adj_in_reservoir;
 } cmf_adjoint;
 /* mode bits for a sweep over consecutive sub-steps on one network:
  * DEFER leaves the last pass (the gather of downstream water-surface
@@ -442,7 +450,8 @@ int cmf_hip_substep_adjoint(cmf_network* net, const cmf_fields* fields, const cm
  * its input state (same nine fields and order as cmf_adjoint) and the
  * unlimited flows and limiters it forms; with the bifurcation module open
  * also the input path state [P, levels], with the levee module the input
- * protected storage and depth.  No reservoir module; one trial. */
+ * protected storage and depth, with the reservoir module the reservoirs'
+ * gathered inflow.  One trial. */
 typedef struct cmf_tape {
     void* river_outflow;
     void* flood_outflow;
@@ -465,6 +474,8 @@ typedef struct cmf_tape {
     /* with levees: the input protected storage (STO) and depth (REAL) */
     void* protected_storage;
     void* protected_depth;
+    /* with reservoirs: the input reservoir_total_inflow (STO) */
+    void* reservoir_total_inflow;
 } cmf_tape;
 int cmf_hip_substep_taped(cmf_network* net, const cmf_fields* fields, int32_t substep, int32_t store_inflow,
                           const cmf_tape* tape, void* stream);

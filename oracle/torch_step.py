@@ -19,10 +19,13 @@ Follows the same reference arithmetic as cmf_oracle.cpp, with vector ops:
   levee      cmfgpu/phys/triton/levee.py:62-304 with the partition :18-59 (the
              floodplain scan for the storages at levee base and crown, cases
              2-4, river / flood / protected split)
+  reservoir  cmfgpu/phys/triton/outflow.py:251-272 (kinematic flow upstream of
+             dams), cmfgpu/phys/triton/reservoir.py:17-184 (release by storage
+             regime from the inflow the previous sub-step gathered) and the
+             reservoir inflow sum outflow.py:380-383
 Branch picks follow the forward kernels (max/min pick the first operand only
 when it is strictly larger/smaller), and every expression that is not taken
 is evaluated on a harmless stand-in so that its gradient is zero, not NaN.
-No reservoir.
 """
 from __future__ import annotations
 
@@ -148,6 +151,53 @@ def _levee(prm, lidx, rs, fs, fdep_c, rdep_c, ffr_c):
     return r_hp, f_hp, p_hp, r_dph, f_dph, p_dph, f_frc
 
 
+def _cbrt_sq(x):
+    """(x * x)^(1/3) with a zero (not infinite) gradient at x = 0."""
+    pos = x > 0
+    xs = _safe(pos, x)
+    return _pick(pos, torch.pow(xs * xs, 1.0 / 3.0), torch.zeros_like(x))
+
+
+def _kinematic(prm, dn, c_elv, dist, r_man, f_man, r_wid, r_len, r_dep, f_dep, rs, fs, dt):
+    """Kinematic river and floodplain flow of the catchments upstream of a
+    dam (outflow.py:251-272)."""
+    mk = prm.get("min_kinematic_slope", 1e-5)
+    bed = (c_elv - c_elv[dn]) / dist
+    bed = _pick(bed > mk, bed, torch.full_like(bed, mk))
+    kin_riv = r_wid * r_dep * ((1.0 / r_man) * _sqrt0(bed) * _cbrt_sq(r_dep))
+    kin_riv = _maxz(_pick(kin_riv < rs / dt, kin_riv, rs / dt))
+    bed_f = _pick(bed < 0.005, bed, torch.full_like(bed, 0.005))
+    area = _maxz(fs / r_len - f_dep * r_wid)
+    kin_fld = area * ((1.0 / f_man) * _sqrt0(bed_f) * _cbrt_sq(f_dep))
+    kin_fld = _maxz(_pick(kin_fld < fs / dt, kin_fld, fs / dt))
+    return kin_riv, kin_fld
+
+
+def _release(prm, ridx, rs, fs, inflow_state, runoff, dt):
+    """Reservoir release by storage regime (reservoir.py:114-163): the
+    conservation, adjustment and flood-control zones and the emergency
+    spill, clamped to [0, S / dt]."""
+    total = rs[ridx] + fs[ridx]
+    inflow = inflow_state[ridx] + runoff[ridx]
+    cv, ev, av = prm["conservation_volume"], prm["emergency_volume"], prm["adjustment_volume"]
+    qn, qa, qf = prm["effective_normal_outflow"], prm["adjustment_outflow"], prm["flood_control_outflow"]
+    r1 = total <= cv
+    r2 = (total > cv) & (total <= av)
+    r3 = (total > av) & (total <= ev)
+    out1 = qn * _sqrt0(_safe(r1, total / cv))
+    f2 = _safe(r2 & (total > cv), (total - cv) / (av - cv))
+    out2 = qn + f2 ** 3 * (qa - qn)
+    f3 = _safe(r3 & (total > av), (total - av) / (ev - av))
+    tmp = qa + torch.pow(f3, 0.1) * (qf - qa)
+    o_flood = qn + ((total - cv) / (ev - cv)) * (inflow - qn)
+    out3 = _pick(inflow >= qf, _pick(o_flood > tmp, o_flood, tmp), tmp)
+    out4 = _pick(inflow >= qf, inflow, qf)
+    out = _pick(r1, out1, _pick(r2, out2, _pick(r3, out3, out4)))
+    out = _maxz(out)
+    cap = total / dt
+    return _pick(out < cap, out, cap)
+
+
 def _paths(st, prm, wse, tot, dt, g, pwse=None):
     """Per-level path flows limited at 5 % of the smaller end storage, and the
     outgoing terms at the upstream (pos) and downstream (neg) ends."""
@@ -262,6 +312,16 @@ def substep(st: dict, prm: dict, dt: float, gravity: float = 9.8) -> dict:
     limit = _pick(negative & (ratio < 1.0), ratio, torch.ones_like(ratio))
     q_r = q_r * limit
     q_f = q_f * limit
+    reservoirs = "reservoir_catchment_idx" in prm
+    if reservoirs:
+        du = prm["is_dam_upstream"]
+        kr, kf = _kinematic(prm, dn, c_elv, dist, r_man, f_man, r_wid, r_len, r_dep, f_dep, rs, fs, dt)
+        q_r = _pick(du, kr, q_r)
+        q_f = _pick(du, kf, q_f)
+        ridx = prm["reservoir_catchment_idx"]
+        out = _release(prm, ridx, rs, fs, st["reservoir_total_inflow"], prm["runoff"], dt)
+        q_r = q_r.index_put((ridx,), out)
+        q_f = q_f.index_put((ridx,), torch.zeros_like(out))
 
     paths = "bifurcation_catchment_idx" in prm
     if paths:
@@ -362,4 +422,7 @@ def substep(st: dict, prm: dict, dt: float, gravity: float = 9.8) -> dict:
     if levee:
         out["protected_storage"] = psf
         out["protected_depth"] = pdep
+    if reservoirs:
+        # the limited inflow this sub-step gathers at each reservoir
+        out["reservoir_total_inflow"] = _pick(prm["is_reservoir"], rin + fin, zero)
     return out

@@ -9,8 +9,11 @@ n and river height were scaled by lognormal(0, 0.2) factors; checkpoints
 every 3 outer steps (so the backward re-runs segments).  The same checks run
 with bifurcation paths open (100 paths on 3,000 catchments, 600 on 20,000),
 with levees on 30 % of the catchments (their initial river storage scaled
-by U(1, 100), so that every levee case occurs), and with both; the central
-differences then take half their parameters at path ends and levees.
+by U(1, 100), so that every levee case occurs), with both, and with both plus
+reservoirs (one per 50 catchments, each started at U(0.1, 1.1) of its
+capacity so that every release regime occurs); the central differences then
+take half their parameters at path ends, levees, reservoirs and the
+catchments draining into them.
 
 Tolerances: FP64 adjoint vs FP64 autograd, relative L2 of each gradient 1e-6
 (the two sum floats in different orders); central differences (h = 1e-3
@@ -32,16 +35,17 @@ PLAIN = ("base", "adaptive_time")
 STEPS = 8
 
 
-KINDS = {"plain": (0, 0.0), "paths": (1, 0.0), "levees": (0, 0.3), "both": (1, 0.3)}
+KINDS = {"plain": (0, 0.0, 0), "paths": (1, 0.0, 0), "levees": (0, 0.3, 0), "both": (1, 0.3, 0),
+         "all": (1, 0.3, 1)}
 
 
 def _setup(precision, n=3000, kind="plain"):
     from cmfgpu_amd import synthetic
     from cmfgpu_amd.adjoint import GaugeProblem
     from cmfgpu_amd.models import CaMaFlood
-    has_paths, levee = KINDS[kind]
+    has_paths, levee, has_res = KINDS[kind]
     paths = (n // 30 if n <= 3000 else n * 3 // 100) if has_paths else 0
-    net = synthetic.generate_network(n, 1003, paths=paths, levee=levee)
+    net = synthetic.generate_network(n, 1003, paths=paths, levee=levee, reservoirs=n // 50 if has_res else 0)
     src = dict(net.fields)
     src["runoff"] = synthetic.Runoff(src["catchment_area"], 2003).next_day() * np.float32(4.0)
     if levee:
@@ -49,8 +53,15 @@ def _setup(precision, n=3000, kind="plain"):
         lid = np.asarray(src["levee_catchment_idx"])
         rs[lid] *= np.random.default_rng(3).uniform(1.0, 100.0, lid.size)
         src["river_storage"] = rs.astype(np.asarray(src["river_storage"]).dtype)
+    if has_res:
+        rs = np.array(src["river_storage"], dtype=np.float64)
+        rid = np.asarray(src["reservoir_catchment_idx"])
+        cap = np.asarray(src["reservoir_capacity"], dtype=np.float64)
+        rs[rid] = cap * np.random.default_rng(5).uniform(0.1, 1.1, rid.size)
+        src["river_storage"] = rs.astype(np.asarray(src["river_storage"]).dtype)
     gauges = np.argsort(net.upstream_area)[-16:]
-    mods = PLAIN + (("bifurcation",) if paths else ()) + (("levee",) if levee else ())
+    mods = (PLAIN + (("bifurcation",) if paths else ()) + (("levee",) if levee else ())
+            + (("reservoir",) if has_res else ()))
     model = CaMaFlood(src, opened_modules=mods, device="cuda:0", step_duration=1800.0, precision=precision)
     rng = np.random.default_rng(7)
     n0, h0 = src["river_manning"].astype(np.float64), src["river_height"].astype(np.float64)
@@ -79,6 +90,14 @@ def _torch_gradient(model, problem, gauges, n0, h0):
         "flood_manning", "river_width", "river_length", "catchment_elevation", "catchment_area",
         "downstream_distance", "runoff", "flood_depth_table")}
     prm["downstream_idx"] = torch.tensor(model.get("downstream_idx"), dtype=torch.int64)
+    if problem.reservoirs:
+        st["reservoir_total_inflow"] = torch.tensor(model.get("reservoir_total_inflow"), dtype=torch.float64)
+        for k in ("conservation_volume", "emergency_volume", "adjustment_volume", "effective_normal_outflow",
+                  "adjustment_outflow", "flood_control_outflow"):
+            prm[k] = torch.tensor(model.get(k), dtype=torch.float64)
+        prm["reservoir_catchment_idx"] = torch.tensor(model.get("reservoir_catchment_idx"), dtype=torch.int64)
+        for k in ("is_dam_upstream", "is_reservoir"):
+            prm[k] = torch.tensor(model.get(k), dtype=torch.bool)
     if problem.levees:
         for k in ("protected_storage", "protected_depth"):
             st[k] = torch.tensor(model.get(k), dtype=torch.float64)
@@ -158,6 +177,10 @@ def test_adjoint_against_central_differences(kind):
             ends = np.union1d(src["bifurcation_catchment_idx"], src["bifurcation_downstream_idx"])
         if "levee_catchment_idx" in src:
             ends = np.union1d(ends, src["levee_catchment_idx"])
+        if "reservoir_catchment_idx" in src:
+            rid = np.asarray(src["reservoir_catchment_idx"])
+            ends = np.union1d(ends, rid)
+            ends = np.union1d(ends, np.flatnonzero(np.isin(np.asarray(src["downstream_idx"]), rid)))
     checks = [("n", i) for i in _pick_checks(rng, g_n, ends, 32)] + \
              [("h", i) for i in _pick_checks(rng, g_h, ends, 32)]
     assert len(checks) == 64, len(checks)
@@ -194,7 +217,7 @@ def test_adjoint_against_central_differences(kind):
     model.close()
 
 
-@pytest.mark.parametrize("kind", ["plain", "both"])
+@pytest.mark.parametrize("kind", ["plain", "all"])
 def test_fp32_adjoint_close_to_fp64(kind):
     grads = {}
     for prec in ("float64", "float32"):

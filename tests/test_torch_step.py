@@ -137,3 +137,66 @@ def test_torch_transcription_with_levees_matches_fp64_oracle(paths):
         got = st[k].numpy()
         err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-30)
         assert err < 1e-9, (k, err)
+
+
+RES_FIELDS = ("reservoir_total_inflow",)
+
+
+def reservoirs_filled(net, seed=5):
+    """The network's fields with runoff x4 and every reservoir catchment's
+    initial river storage set to U(0.1, 1.1) of its capacity, so that the
+    release runs through all four storage regimes."""
+    from cmfgpu_amd import synthetic
+    src = dict(net.fields)
+    src["runoff"] = synthetic.Runoff(src["catchment_area"], 2003).next_day() * np.float32(4.0)
+    rs = np.array(src["river_storage"], dtype=np.float64)
+    rid = np.asarray(src["reservoir_catchment_idx"])
+    cap = np.asarray(src["reservoir_capacity"], dtype=np.float64)
+    rs[rid] = cap * np.random.default_rng(seed).uniform(0.1, 1.1, rid.size)
+    src["river_storage"] = rs.astype(np.asarray(src["river_storage"]).dtype)
+    return src
+
+
+def test_torch_transcription_with_reservoirs_matches_fp64_oracle():
+    """The same with 60 reservoirs (the kinematic flow upstream of each dam,
+    the release by storage regime, the reservoir inflow carried between
+    sub-steps), levees and paths: every regime occurs, and all state fields
+    agree after 12 outer steps."""
+    from cmfgpu_amd import synthetic
+    from oracle.torch_step import substep
+    net = synthetic.generate_network(3000, 1003, levee=0.05, paths=100, reservoirs=60)
+    src = reservoirs_filled(net)
+    mods = ("base", "levee", "bifurcation", "reservoir")
+    orc, host = oracle_for(src, mods, precision="float64")
+    st = {k: torch.tensor(orc.state[k], dtype=torch.float64) for k in FIELDS + LEVEE_FIELDS + PATH_FIELDS
+          + RES_FIELDS}
+    names = ["river_manning", "river_height", "flood_manning", "river_width", "river_length",
+             "catchment_elevation", "catchment_area", "downstream_distance", "runoff", "flood_depth_table",
+             "levee_crown_height", "levee_fraction", "levee_base_height", "bifurcation_length",
+             "bifurcation_elevation", "bifurcation_width", "bifurcation_manning", "conservation_volume",
+             "emergency_volume", "adjustment_volume", "effective_normal_outflow", "adjustment_outflow",
+             "flood_control_outflow"]
+    prm = {k: torch.tensor(orc.state[k], dtype=torch.float64) for k in names}
+    for k in ("downstream_idx", "levee_catchment_idx", "bifurcation_catchment_idx", "bifurcation_downstream_idx",
+              "reservoir_catchment_idx"):
+        prm[k] = torch.tensor(orc.state[k], dtype=torch.int64)
+    for k in ("is_dam_upstream", "is_reservoir"):
+        prm[k] = torch.tensor(orc.state[k], dtype=torch.bool)
+    rid = orc.state["reservoir_catchment_idx"]
+    cv, av, ev = (orc.state[k] for k in ("conservation_volume", "adjustment_volume", "emergency_volume"))
+    regimes = set()
+    count = 6
+    for _ in range(12):
+        s = orc.state
+        total = s["river_storage"][rid] + s["flood_storage"][rid]
+        regimes |= {int(r) for r in np.where(total <= cv, 1, np.where(total <= av, 2, np.where(total <= ev, 3, 4)))}
+        orc.advance(1800.0, 1, count)
+        for _ in range(count):
+            st = substep(st, prm, 1800.0 / count)
+    assert regimes == {1, 2, 3, 4}, regimes
+    assert prm["is_dam_upstream"].any()
+    for k in FIELDS + LEVEE_FIELDS + PATH_FIELDS + RES_FIELDS:
+        ref = orc.state[k].astype(np.float64)
+        got = st[k].numpy()
+        err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-30)
+        assert err < 1e-9, (k, err)
