@@ -100,33 +100,60 @@ def fp64_errors(net, src, device: str, layout: str, days: int) -> dict:
     against the (FP64, FP64) run of the same kernels, after `days` simulated
     days (48 outer steps each, fresh daily runoff) from the same state.  The
     FP64 profile is bit-identical to the FP64 CPU oracle (parity tests), so
-    this is the error against an FP64 oracle run at full size."""
+    this is the error against an FP64 oracle run at full size.
+
+    A third run, FP32 with water surfaces formed in double (precise_surfaces),
+    gives the same errors under "precise_surfaces", and then that run and the
+    FP32 run are timed from their final states (alternating blocks of 12
+    outer steps, same launch shapes) for the cost of the flag."""
+    import time
     import numpy as np
+    import torch
     from cmfgpu_amd import synthetic
     from cmfgpu_amd.models import CaMaFlood
     mods = ["base", "adaptive_time"] + (["bifurcation"] if net.num_paths else []) \
         + (["levee"] if net.num_levees else [])
     models = [CaMaFlood(src, opened_modules=tuple(mods), device=device, step_duration=1800.0, layout=layout,
-                        precision=p) for p in ("float32", "float64")]
+                        precision=p, precise_surfaces=x)
+              for p, x in (("float32", False), ("float64", False), ("float32", True))]
     forcing = synthetic.Runoff(src["catchment_area"], 2003)
-    count_mismatch = 0
+    count_mismatch = [0, 0]
     for _ in range(days):
         r = forcing.next_day()
         for m in models:
             m.set_inputs(r)
         for _ in range(48):
-            a, b = (m.step_advance() for m in models)
-            count_mismatch += int(a != b)
-    out = {}
-    for name in ERROR_FIELDS:
-        x, y = (m.get(name).astype(np.float64) for m in models)
-        out[name] = float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
-    q32, q64 = ((m.get("river_outflow").astype(np.float64) + m.get("flood_outflow")) for m in models)
-    for m in models:
-        m.close()
-    return {"days": days, "outer_steps": 48 * days, "substep_count_mismatches": count_mismatch,
-            "discharge_rel_l2": float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64)),
-            "fields_rel_l2": {k: float(f"{v:.4g}") for k, v in out.items()}}
+            a, b, c = (m.step_advance() for m in models)
+            count_mismatch[0] += int(a != b)
+            count_mismatch[1] += int(c != b)
+    res = []
+    for k in (0, 2):
+        out = {}
+        for name in ERROR_FIELDS:
+            x, y = (models[j].get(name).astype(np.float64) for j in (k, 1))
+            out[name] = float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+        q, q64 = ((models[j].get("river_outflow").astype(np.float64) + models[j].get("flood_outflow"))
+                  for j in (k, 1))
+        res.append({"days": days, "outer_steps": 48 * days, "substep_count_mismatches": count_mismatch[k // 2],
+                    "discharge_rel_l2": float(np.linalg.norm(q - q64) / np.linalg.norm(q64)),
+                    "fields_rel_l2": {k2: float(f"{v:.4g}") for k2, v in out.items()}})
+    models[1].close()
+    rate = {0: [0, 0.0], 2: [0, 0.0]}
+    for _ in range(4):
+        for k in (0, 2):
+            m = models[k]
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(12):
+                rate[k][0] += m.step_advance() * m.N
+            torch.cuda.synchronize()
+            rate[k][1] += time.perf_counter() - t0
+    for k in (0, 2):
+        models[k].close()
+    out = res[0]
+    out["precise_surfaces"] = dict(res[1], catchment_steps_per_s=rate[2][0] / rate[2][1],
+                                   vs_reference_surfaces=round((rate[2][0] / rate[2][1]) / (rate[0][0] / rate[0][1]), 4))
+    return out
 
 
 def basins_line(args, device: str) -> dict:
@@ -678,7 +705,9 @@ def main():
     errors = None
     if args.error_days > 0:
         errors = fp64_errors(net, src, device, args.layout, args.error_days)
-        log(f"FP32 vs FP64 after {args.error_days} day(s): discharge rel L2 {errors['discharge_rel_l2']:.3g}")
+        log(f"FP32 vs FP64 after {args.error_days} day(s): discharge rel L2 {errors['discharge_rel_l2']:.3g}; "
+            f"precise surfaces {errors['precise_surfaces']['discharge_rel_l2']:.3g} at "
+            f"{errors['precise_surfaces']['vs_reference_surfaces']:.3f}x the rate")
 
     line = {
         "metric": METRIC, "value": value, "unit": "catchment-steps/s", "n_gpus": 1, "steps": args.steps,

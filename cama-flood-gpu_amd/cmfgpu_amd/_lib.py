@@ -35,6 +35,7 @@ HAS_WATER_SURFACE = 1 << 5
 HAS_PROTECTED_WATER_SURFACE = 1 << 6
 HAS_ADAPTIVE_TIME = 1 << 7
 PER_TRIAL_DT = 1 << 8
+PRECISE_SURFACE = 1 << 9
 
 BATCHED_BITS = {
     "river_manning": 1 << 0,

@@ -91,7 +91,8 @@ class CaMaFlood:
                  log_buffer_size: int = 20000, step_duration: float = 1800.0, num_sub_steps: int = 1,
                  start_time: datetime = datetime(2000, 1, 1), log_path: str | Path | None = None,
                  mode: str = "fused", gravity: float = 9.8, adaptive_time_factor: float = 0.7,
-                 min_kinematic_slope: float = 1e-5, groups=None, use_graph: bool = False):
+                 min_kinematic_slope: float = 1e-5, groups=None, use_graph: bool = False,
+                 precise_surfaces: bool = False):
         if mode not in ("fused", "ops"):
             raise ValueError("mode must be 'fused' or 'ops'")
         if layout not in ("basin", "level", "network"):
@@ -124,7 +125,8 @@ class CaMaFlood:
         self.log_path = Path(log_path) if log_path is not None else None
         self.host = assemble(src, opened, precision=precision, log_buffer_size=log_buffer_size,
                              gravity=gravity, adaptive_time_factor=adaptive_time_factor,
-                             min_kinematic_slope=min_kinematic_slope)
+                             min_kinematic_slope=min_kinematic_slope, precise_surfaces=precise_surfaces)
+        self.precise_surfaces = bool(precise_surfaces)
         self.flags = self.host.flags | (_lib.PER_TRIAL_DT if self.per_trial else 0)
         self.real = self.host.real
         hf = self.host.fields

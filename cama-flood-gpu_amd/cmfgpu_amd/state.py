@@ -46,7 +46,10 @@ def real_dtype(precision) -> type:
 
 def assemble(src: dict, opened_modules=("base", "adaptive_time"), *, precision="float32",
              log_buffer_size: int = 0, gravity: float = 9.8, adaptive_time_factor: float = 0.7,
-             min_kinematic_slope: float = 1e-5, cold_start: bool = True) -> HostState:
+             min_kinematic_slope: float = 1e-5, cold_start: bool = True,
+             precise_surfaces: bool = False) -> HostState:
+    """``precise_surfaces``: in the FP32 profile form water surfaces and their
+    differences in double (the PRECISE_SURFACE flag); no effect in FP64."""
     opened = set(opened_modules) | {"base"}
     unknown = opened - set(KNOWN_MODULES)
     if unknown:
@@ -56,7 +59,7 @@ def assemble(src: dict, opened_modules=("base", "adaptive_time"), *, precision="
     base = BaseModule(src, opened, real, gravity=gravity, min_kinematic_slope=min_kinematic_slope)
     mods["base"] = base
     fields = dict(base.fields)
-    flags = 0
+    flags = _lib.PRECISE_SURFACE if precise_surfaces else 0
     nb_levels = 0
     if "adaptive_time" in opened:
         at = AdaptiveTimeModule(adaptive_time_factor)

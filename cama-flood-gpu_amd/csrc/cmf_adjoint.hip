@@ -109,7 +109,9 @@ struct OutTrace {
     REAL rel_total, rel_inflow, rel_frac, rel_raw;
 };
 
-template <class REAL, class STO>
+// RES: the reservoir module is open (a compile-time switch, so that the
+// kernels of networks without reservoirs carry none of its registers)
+template <class REAL, class STO, bool RES>
 __device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v, const AdjView<REAL>& a,
                                                         int64_t i, REAL time_step) {
     OutTrace<REAL> o;
@@ -200,7 +202,7 @@ __device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v
         q_f = q_f * o.limit;
     }
     o.dam = false;
-    if (a.res && v.is_dam_up[i]) {
+    if (RES && v.is_dam_up[i]) {
         // kinematic flow upstream of a dam (outflow.py:251-272)
         o.dam = true;
         REAL bed_slope = (c_elv - c_elv_dn) / dn_dist;
@@ -224,7 +226,7 @@ __device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v
         q_r = kin_riv;
         q_f = kin_fld;
     }
-    o.reservoir = a.res && v.is_reservoir[i];
+    o.reservoir = RES && v.is_reservoir[i];
     if (o.reservoir) {
         // release by storage regime (k_reservoir, reservoir.py:114-163)
         const int64_t r = a.res_row[i];
@@ -515,11 +517,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_res_rows(View<REAL, STO> v, AdjV
 }
 
 // A0: unlimited outflows and outgoing terms of the saved state
-template <class REAL, class STO>
+template <class REAL, class STO, bool RES>
 __global__ __launch_bounds__(kBlock) void k_adj_outflow_fwd(View<REAL, STO> v, AdjView<REAL> a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.N) return;
-    const OutTrace<REAL> o = outflow_trace(v, a, i, a.dt);
+    const OutTrace<REAL> o = outflow_trace<REAL, STO, RES>(v, a, i, a.dt);
     a.w_qr[i] = o.qr;
     a.w_qf[i] = o.qf;
     const REAL pos = vmax(o.qr, (REAL)0.0) + vmax(o.qf, (REAL)0.0);
@@ -980,12 +982,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
 // flows enter the limited flows (a_qr/a_qf) and the outgoing terms (own total
 // for the positive part, the downstream total for the negative part); the
 // cross-section depths enter the next sub-step.
-template <class REAL, class STO>
+template <class REAL, class STO, bool RES>
 __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjView<REAL> a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.N) return;
     const REAL dtr = a.dt;
-    const OutTrace<REAL> o = outflow_trace(v, a, i, dtr);
+    const OutTrace<REAL> o = outflow_trace<REAL, STO, RES>(v, a, i, dtr);
     const double dt = (double)dtr, g = (double)v.gravity;
     const int64_t d = v.dn[i];
     const double og_own = (double)a.a_og[i], og_dn = o.mouth ? 0.0 : (double)a.a_og[d];
@@ -1151,7 +1153,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
         a.lin_ps[i] = (REAL)((double)a.lin_ps[i] + Ltot);
         a.lin_pd[i] = (REAL)Lpd;
     }
-    if (a.res) a.lin_ri[i] = (REAL)Lri;
+    if (RES) a.lin_ri[i] = (REAL)Lri;
     const double dist = (double)v.dist[i];
     Lwse += Lslope / dist;
     Lwse_dn -= Lslope / dist;

@@ -252,4 +252,11 @@ struct View {
     }
 };
 
+// CMF_PRECISE_SURFACE in the FP32 profile: water surfaces formed in double
+// (the FP64 profile forms them in double anyway)
+template <class REAL, class STO>
+__device__ __forceinline__ bool precise_surfaces(const View<REAL, STO>& v) {
+    return sizeof(REAL) < sizeof(double) && v.has(CMF_PRECISE_SURFACE);
+}
+
 }  // namespace cmf

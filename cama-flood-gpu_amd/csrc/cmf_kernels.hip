@@ -100,26 +100,45 @@ __device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int subste
     const REAL ps = (FUSED && substep > 0 && !lev_i) ? (REAL)0.0 : (REAL)v.protected_storage[e];
     const REAL r_csd = v.rcsd[e], f_csd = v.fcsd[e], f_csa = v.fcsa[e];
 
-    const REAL river_elevation = c_elv - r_hgt;
-    const REAL wse = r_dep + river_elevation;
-    REAL pwse = wse;
-    if (lev_i) pwse = vmin(c_elv + v.protected_depth[e], wse);
     const REAL total_storage = rs + fs + ps;
     const REAL r_dep_dn = v.river_depth[ed];
     const REAL r_hgt_dn = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, ed, d);
     const REAL c_elv_dn = v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, ed, d);
-    const REAL river_elevation_dn = c_elv_dn - r_hgt_dn;
-    REAL wse_dn = r_dep_dn + river_elevation_dn;
-    if (mouth) wse_dn = c_elv;
-    const REAL max_wse = vmax(wse, wse_dn);
+    // wse: water surface; dwse: its drop to the downstream surface;
+    // upd_rcsd / fsd: the higher of the two above the river bed / the bank
+    REAL wse, pwse, dwse, upd_rcsd, fsd;
+    if (precise_surfaces(v)) {
+        // in double from the REAL depth, elevation and height (sums of two
+        // or three floats: exact), each difference rounded once
+        const double rel = (double)c_elv - (double)r_hgt;
+        const double w = (double)r_dep + rel;
+        const double w_dn = mouth ? (double)c_elv : (double)r_dep_dn + ((double)c_elv_dn - (double)r_hgt_dn);
+        const double mw = vmax(w, w_dn);
+        wse = (REAL)w;
+        pwse = lev_i ? (REAL)vmin((double)c_elv + (double)v.protected_depth[e], w) : wse;
+        dwse = (REAL)(w - w_dn);
+        upd_rcsd = mouth ? r_dep : (REAL)(mw - rel);
+        fsd = (REAL)((mouth ? w : mw) - (double)c_elv);
+    } else {
+        const REAL river_elevation = c_elv - r_hgt;
+        wse = r_dep + river_elevation;
+        pwse = wse;
+        if (lev_i) pwse = vmin(c_elv + v.protected_depth[e], wse);
+        const REAL river_elevation_dn = c_elv_dn - r_hgt_dn;
+        REAL wse_dn = r_dep_dn + river_elevation_dn;
+        if (mouth) wse_dn = c_elv;
+        const REAL max_wse = vmax(wse, wse_dn);
+        dwse = wse - wse_dn;
+        upd_rcsd = mouth ? r_dep : (max_wse - river_elevation);
+        const REAL flood_surface = mouth ? wse : max_wse;
+        fsd = flood_surface - c_elv;
+    }
 
-    const REAL river_slope = (wse - wse_dn) / dn_dist;
+    const REAL river_slope = dwse / dn_dist;
     const REAL flood_slope = vclamp(river_slope, (REAL)-0.005, (REAL)0.005);
 
-    const REAL upd_rcsd = mouth ? r_dep : (max_wse - river_elevation);
     const REAL r_sifd = vmax(dsqrt(upd_rcsd * r_csd), (REAL)1e-6);
-    const REAL flood_surface = mouth ? wse : max_wse;
-    const REAL upd_fcsd = vmax(flood_surface - c_elv, (REAL)0.0);
+    const REAL upd_fcsd = vmax(fsd, (REAL)0.0);
     const REAL f_sifd = vmax(dsqrt(upd_fcsd * f_csd), (REAL)1e-6);
     const REAL upd_fcsa = vmax(fs / r_len - f_dep * r_wid, (REAL)0.0);
 
@@ -315,6 +334,17 @@ __global__ __launch_bounds__(kBlock) void k_res_regather(View<REAL, STO> v) {
 // pass stores in the water_surface_elevation, protected_water_surface_elevation
 // and total_storage fields.
 template <class REAL, class STO>
+__device__ __forceinline__ double surface_of_d(const View<REAL, STO>& v, int64_t e, int64_t i) {
+    const REAL c_elv = v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, e, i);
+    const REAL r_hgt = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, i);
+    return (double)v.river_depth[e] + ((double)c_elv - (double)r_hgt);
+}
+template <class REAL, class STO>
+__device__ __forceinline__ double protected_surface_of_d(const View<REAL, STO>& v, int64_t e, int64_t i, double w) {
+    if (!(v.has(CMF_HAS_LEVEE) && v.is_levee[i])) return w;
+    return vmin((double)v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, e, i) + (double)v.protected_depth[e], w);
+}
+template <class REAL, class STO>
 __device__ __forceinline__ REAL surface_of(const View<REAL, STO>& v, int64_t e, int64_t i) {
     const REAL c_elv = v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, e, i);
     const REAL r_hgt = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, i);
@@ -351,6 +381,16 @@ __device__ __forceinline__ void bif_path(const View<REAL, STO>& v, int substep, 
     const REAL gravity = v.gravity;
     const REAL length = v.prm(v.bif_length, CMF_BATCHED_BIFURCATION_LENGTH, pe, p);
     REAL w0, w1, ts0, ts1, max_pw = (REAL)0;
+    // precise surfaces: formed in double from the state as the outflow pass
+    // forms them (the river depth is unchanged since that pass)
+    const bool precise = precise_surfaces(v);
+    double w0d = 0.0, w1d = 0.0, max_wd = 0.0, max_pwd = 0.0;
+    if (precise) {
+        w0d = surface_of_d(v, eci, ci);
+        w1d = surface_of_d(v, edi, di);
+        max_wd = vmax(w0d, w1d);
+        if (LEVEE) max_pwd = vmax(protected_surface_of_d(v, eci, ci, w0d), protected_surface_of_d(v, edi, di, w1d));
+    }
     if (FROM_STATE) {
         w0 = surface_of(v, eci, ci);
         w1 = surface_of(v, edi, di);
@@ -365,7 +405,7 @@ __device__ __forceinline__ void bif_path(const View<REAL, STO>& v, int substep, 
         ts1 = (REAL)v.total_storage[edi];
     }
     const REAL max_w = vmax(w0, w1);
-    REAL slope = (w0 - w1) / length;
+    REAL slope = (precise ? (REAL)(w0d - w1d) : (w0 - w1)) / length;
     slope = vclamp(slope, (REAL)-0.005, (REAL)0.005);
     const int LB = LBFIX > 0 ? LBFIX : (int)v.LB;
     REAL qs[LBFIX > 0 ? LBFIX : 1];
@@ -378,15 +418,16 @@ __device__ __forceinline__ void bif_path(const View<REAL, STO>& v, int substep, 
         const REAL csd_old = v.bif_csd[k];
         const REAL bel = v.prm(v.bif_elev, CMF_BATCHED_BIFURCATION_ELEVATION, k, ks);
         REAL upd_csd, sifd;
+        const REAL above = precise ? (REAL)(max_wd - (double)bel) : max_w - bel;
         if (!LEVEE) {
-            upd_csd = vmax(max_w - bel, (REAL)0.0);
+            upd_csd = vmax(above, (REAL)0.0);
             sifd = vmax(dsqrt(upd_csd * csd_old), dsqrt(upd_csd * (REAL)0.01));
         } else if (l == 0) {
-            upd_csd = vmax(max_w - bel, (REAL)0.0);
+            upd_csd = vmax(above, (REAL)0.0);
             const REAL semi = dsqrt(upd_csd * csd_old);
             sifd = (semi <= (REAL)0.0) ? upd_csd : semi;
         } else {
-            upd_csd = vmax(max_pw - bel, (REAL)0.0);
+            upd_csd = vmax(precise ? (REAL)(max_pwd - (double)bel) : max_pw - bel, (REAL)0.0);
             sifd = upd_csd;
         }
         const REAL w = v.prm(v.bif_width, CMF_BATCHED_BIFURCATION_WIDTH, k, ks);

@@ -53,7 +53,16 @@ enum {
     /* time_step (and max_sub_steps) hold one entry per trial, so every trial
      * advances with its own CFL sub-step (an extension of the reference, whose
      * trials share one time_step, cmfgpu/phys/triton/outflow.py:454). */
-    CMF_PER_TRIAL_DT = 1u << 8
+    CMF_PER_TRIAL_DT = 1u << 8,
+    /* FP32 profile: water surfaces and their differences (slopes, depths
+     * above the bed, the bank and path sills) formed in double from the FP32
+     * depth, elevation and height, each difference rounded to FP32 once.  An
+     * extension of the reference, whose FP32 surfaces
+     * (cmfgpu/phys/triton/outflow.py:111-134, bifurcation.py:49-69) lose the
+     * centimetres their differences are made of; discharge then follows the
+     * FP64 run about 50 times closer.  No effect in the FP64 profile; the
+     * adjoint and the taped forward return CMF_ERR_UNSUPPORTED with it. */
+    CMF_PRECISE_SURFACE = 1u << 9
 };
 
 /* Per-trial parameter flags: the reference's batched_* constants

@@ -17,8 +17,10 @@
 //   cmfgpu/phys/triton/reservoir.py    compute_reservoir_outflow_kernel :17-184
 // and the sub-step sequence of cmfgpu/models/cama_flood_model.py:104-173.
 //
-// Precision is a template pair (REAL, STO): (float, double) is the north-star
-// profile, (double, double) the FP64 oracle run.
+// Precision is a template pair (REAL, STO): (float, double) is the FP32
+// profile, (double, double) the FP64 oracle run.  With the PRECISE_SURFACE
+// flag the FP32 profile forms water surfaces in double (see outflow()); the
+// flag changes nothing in the FP64 profile.
 //
 // Accumulation order.  The reference accumulates with float atomics, whose
 // order is unspecified.  The oracle fixes a canonical order, which is the order
@@ -156,7 +158,7 @@ enum Scalar { S_GRAVITY = 0, S_ADAPTIVE_FACTOR, S_MIN_KIN_SLOPE, S_COUNT };
 enum Flag {
     HAS_BIFURCATION = 1, HAS_LEVEE = 2, HAS_RESERVOIR = 4, HAS_LOG = 8,
     HAS_TOTAL_STORAGE = 16, HAS_WATER_SURFACE = 32, HAS_PROTECTED_WATER_SURFACE = 64,
-    HAS_ADAPTIVE = 128
+    HAS_ADAPTIVE = 128, PRECISE_SURFACE = 512
 };
 // order of the 11 water-balance sums in F_LOG_SUMS (LogModule field order,
 // cmfgpu/modules/log.py:207-326)
@@ -236,6 +238,7 @@ struct Model {
         const uint8_t* is_dam_up = f<uint8_t>(F_IS_DAM_UPSTREAM);
         const bool has_levee = flags & HAS_LEVEE, has_res = flags & HAS_RESERVOIR;
         const bool has_bif = flags & HAS_BIFURCATION;
+        const bool precise = (sizeof(REAL) < sizeof(double)) && (flags & PRECISE_SURFACE);
 
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; ++i) {
@@ -252,24 +255,45 @@ struct Model {
             const REAL ps = (REAL)protected_storage[i];
             const REAL r_csd = rcsd[i], f_csd = fcsd[i], f_csa = fcsa[i];
 
-            const REAL river_elevation = c_elv - r_hgt;
-            const REAL wse = r_dep + river_elevation;
-            REAL pwse = wse;
-            if (has_levee && is_levee[i]) pwse = fmin_(c_elv + p_dep, wse);
             const REAL total_storage = rs + fs + ps;
             const REAL r_dep_dn = river_depth[dn], r_hgt_dn = river_height[dn], c_elv_dn = elev[dn];
-            const REAL river_elevation_dn = c_elv_dn - r_hgt_dn;
-            REAL wse_dn = r_dep_dn + river_elevation_dn;
-            if (mouth) wse_dn = c_elv;
-            const REAL max_wse = fmax_(wse, wse_dn);
+            // wse: water surface; dwse: its drop to the downstream surface;
+            // upd_rcsd / fsd: the higher surface above the river bed / the bank
+            REAL wse, pwse, dwse, upd_rcsd, fsd;
+            if (precise) {
+                // in double from the REAL depth, elevation and height (the
+                // sums of two or three floats are exact), each difference
+                // rounded once: the REAL surfaces (hundreds of metres) lose
+                // the centimetres their differences are made of
+                const double rel = (double)c_elv - (double)r_hgt;
+                const double w = (double)r_dep + rel;
+                const double w_dn = mouth ? (double)c_elv : (double)r_dep_dn + ((double)c_elv_dn - (double)r_hgt_dn);
+                const double mw = fmax_(w, w_dn);
+                wse = (REAL)w;
+                pwse = (has_levee && is_levee[i]) ? (REAL)fmin_((double)c_elv + (double)p_dep, w) : wse;
+                dwse = (REAL)(w - w_dn);
+                upd_rcsd = mouth ? r_dep : (REAL)(mw - rel);
+                fsd = (REAL)((mouth ? w : mw) - (double)c_elv);
+            } else {
+                const REAL river_elevation = c_elv - r_hgt;
+                wse = r_dep + river_elevation;
+                pwse = wse;
+                if (has_levee && is_levee[i]) pwse = fmin_(c_elv + p_dep, wse);
+                const REAL river_elevation_dn = c_elv_dn - r_hgt_dn;
+                REAL wse_dn = r_dep_dn + river_elevation_dn;
+                if (mouth) wse_dn = c_elv;
+                const REAL max_wse = fmax_(wse, wse_dn);
+                dwse = wse - wse_dn;
+                upd_rcsd = mouth ? r_dep : (max_wse - river_elevation);
+                const REAL flood_surface = mouth ? wse : max_wse;
+                fsd = flood_surface - c_elv;
+            }
 
-            const REAL river_slope = (wse - wse_dn) / dn_dist;
+            const REAL river_slope = dwse / dn_dist;
             const REAL flood_slope = clamp_(river_slope, (REAL)-0.005, (REAL)0.005);
 
-            const REAL upd_rcsd = mouth ? r_dep : (max_wse - river_elevation);
             const REAL r_sifd = fmax_((REAL)std::sqrt(upd_rcsd * r_csd), (REAL)1e-6);
-            const REAL flood_surface = mouth ? wse : max_wse;
-            const REAL upd_fcsd = fmax_(flood_surface - c_elv, (REAL)0.0);
+            const REAL upd_fcsd = fmax_(fsd, (REAL)0.0);
             const REAL f_sifd = fmax_((REAL)std::sqrt(upd_fcsd * f_csd), (REAL)1e-6);
 
             const REAL upd_fcsa = fmax_(fs / r_len - f_dep * r_wid, (REAL)0.0);
@@ -419,14 +443,34 @@ struct Model {
         const REAL* pwse = f<REAL>(F_PWSE);
         const STO* total_storage = f<STO>(F_TOTAL_STORAGE);
         STO* outgoing = f<STO>(F_OUTGOING_STORAGE);
+        // with PRECISE_SURFACE the end surfaces are formed in double from the
+        // state, as the outflow pass forms them; the river depth has not
+        // changed since that pass
+        const bool precise = (sizeof(REAL) < sizeof(double)) && (flags & PRECISE_SURFACE);
+        const REAL* rdep = f<REAL>(F_RIVER_DEPTH);
+        const REAL* rhgt = f<REAL>(F_RIVER_HEIGHT);
+        const REAL* celv = f<REAL>(F_CATCHMENT_ELEVATION);
+        const REAL* pdep = f<REAL>(F_PROTECTED_DEPTH);
+        const uint8_t* is_levee = f<uint8_t>(F_IS_LEVEE);
+        auto surf = [&](int64_t c) { return (double)rdep[c] + ((double)celv[c] - (double)rhgt[c]); };
+        auto psurf = [&](int64_t c, double w) {
+            return ((flags & HAS_LEVEE) && is_levee[c]) ? fmin_((double)celv[c] + (double)pdep[c], w) : w;
+        };
         for (int64_t p = 0; p < P; ++p) {
             const int64_t ci = ci_p[p], di = di_p[p];
             const REAL length = len[p];
+            double w0d = 0.0, w1d = 0.0, max_wd = 0.0, max_pwd = 0.0;
+            if (precise) {
+                w0d = surf(ci);
+                w1d = surf(di);
+                max_wd = fmax_(w0d, w1d);
+                if (levee_variant) max_pwd = fmax_(psurf(ci, w0d), psurf(di, w1d));
+            }
             const REAL w0 = wse[ci], w1 = wse[di];
             const REAL max_w = fmax_(w0, w1);
             REAL max_pw = (REAL)0;
             if (levee_variant) max_pw = fmax_(pwse[ci], pwse[di]);
-            REAL slope = (w0 - w1) / length;
+            REAL slope = (precise ? (REAL)(w0d - w1d) : (w0 - w1)) / length;
             slope = clamp_(slope, (REAL)-0.005, (REAL)0.005);
             const REAL ts0 = (REAL)total_storage[ci];
             const REAL ts1 = (REAL)total_storage[di];
@@ -435,15 +479,16 @@ struct Model {
                 const int64_t k = p * LB + l;
                 const REAL csd_old = bcsd[k];
                 REAL upd_csd, sifd;
+                const REAL above = precise ? (REAL)(max_wd - (double)bel[k]) : max_w - bel[k];
                 if (!levee_variant) {
-                    upd_csd = fmax_(max_w - bel[k], (REAL)0.0);
+                    upd_csd = fmax_(above, (REAL)0.0);
                     sifd = fmax_((REAL)std::sqrt(upd_csd * csd_old), (REAL)std::sqrt(upd_csd * (REAL)0.01));
                 } else if (l == 0) {
-                    upd_csd = fmax_(max_w - bel[k], (REAL)0.0);
+                    upd_csd = fmax_(above, (REAL)0.0);
                     const REAL semi = (REAL)std::sqrt(upd_csd * csd_old);
                     sifd = (semi <= (REAL)0.0) ? upd_csd : semi;
                 } else {
-                    upd_csd = fmax_(max_pw - bel[k], (REAL)0.0);
+                    upd_csd = fmax_(precise ? (REAL)(max_pwd - (double)bel[k]) : max_pw - bel[k], (REAL)0.0);
                     sifd = upd_csd;
                 }
                 const REAL w = wid[k];

@@ -41,6 +41,7 @@ FIELDS = (
 
 HAS_BIFURCATION, HAS_LEVEE, HAS_RESERVOIR, HAS_LOG = 1, 2, 4, 8
 HAS_TOTAL_STORAGE, HAS_WATER_SURFACE, HAS_PROTECTED_WATER_SURFACE, HAS_ADAPTIVE = 16, 32, 64, 128
+PRECISE_SURFACE = 512   # FP32 profile: water surfaces formed in double
 
 KERNELS = {
     "outflow": 0, "inflow": 1, "flood_stage": 2, "bifurcation_outflow": 3,

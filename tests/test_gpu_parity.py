@@ -55,6 +55,32 @@ def test_full_physics_bitexact(layout, mode):
                                          "reservoir_total_inflow"))
 
 
+@pytest.mark.parametrize("layout", ["basin", "network"])
+@pytest.mark.parametrize("mode", ["fused", "ops"])
+def test_precise_surfaces_bitexact(layout, mode):
+    """FP32 with water surfaces formed in double (CMF_PRECISE_SURFACE)
+    against the oracle with the same flag; levee catchments started at 1-100
+    times their river storage so that the protected surfaces are in play."""
+    net, src = small_source()
+    rs = np.array(src["river_storage"], dtype=np.float64)
+    lid = np.asarray(src["levee_catchment_idx"])
+    rs[lid] *= np.random.default_rng(3).uniform(1.0, 100.0, lid.size)
+    src["river_storage"] = rs
+    model = _model(src, FULL, layout=layout, mode=mode, precise_surfaces=True)
+    orc, _ = oracle_for(src, FULL, precise_surfaces=True)
+    ref, _ = oracle_for(src, FULL)
+    for step in range(6):
+        n = model.step_advance()
+        assert n == int(orc.advance(1800.0, 1)[0])
+        ref.advance(1800.0, 1, n)
+    _compare(model, orc, STATE_FIELDS + ("bifurcation_outflow", "bifurcation_cross_section_depth",
+                                         "global_bifurcation_outflow", "limit_rate", "total_storage",
+                                         "water_surface_elevation", "protected_water_surface_elevation",
+                                         "reservoir_total_inflow"))
+    # the flag is in effect: the reference's FP32 surfaces give other flows
+    assert not np.array_equal(orc.state["river_outflow"], ref.state["river_outflow"])
+
+
 def test_fp64_profile_bitexact():
     net, src = small_source()
     model = _model(src, FULL, precision="float64")

@@ -54,6 +54,12 @@ MISTAKES = {
     "levee case 4 measured from the base": (
         "const REAL dsto_add_c4 = (f_dph_c4 - crown) * (levee_distance + rw) * rl;",
         "const REAL dsto_add_c4 = (f_dph_c4 - base) * (levee_distance + rw) * rl;"),
+    "precise cross-section depth measured from the bank": (
+        "upd_rcsd = mouth ? r_dep : (REAL)(mw - rel);",
+        "upd_rcsd = mouth ? r_dep : (REAL)(mw - (double)c_elv);"),
+    "precise surface without the river height": (
+        "const double rel = (double)c_elv - (double)r_hgt;",
+        "const double rel = (double)c_elv;"),
     "levee fill scan on the base height": (
         "const REAL ratio_fill = (crown - dhgt_pre) / dhgt_seg;",
         "const REAL ratio_fill = (base - dhgt_pre) / dhgt_seg;"),

@@ -134,6 +134,42 @@ def test_outflow_sign_follows_slope_and_outgoing_is_positive_part():
             assert og[1] >= -q0 * dt * (1 - 1e-6)
 
 
+def test_precise_surfaces_keep_the_centimetres():
+    """High in the mountains (elevation 3,000 m, where FP32 spacing is
+    0.24 mm) a 1.23 cm surface drop over 10 km: from rest q = W g dt h_s S,
+    with S and h_s from the exact (float64) surfaces of the FP32 inputs.
+    With PRECISE_SURFACE the FP32 profile forms the surfaces in double and
+    meets the closed form to FP32 rounding; the reference's FP32 surfaces
+    (triton/outflow.py:111-134) carry their rounding into S at the per-cent
+    level.  The flag changes nothing in the FP64 profile."""
+    h, dx, W, dt = 2.0, 10_000.0, 50.0, 60.0
+    elev = np.array([3000.0123, 3000.0, 2990.0], np.float32)
+    depth = np.array([2.00037, 1.99961, 2.0], np.float32)
+    height = np.array([4.0, 4.0, 4.0], np.float32)
+    wse = elev.astype(np.float64) - height + depth
+    S = (wse[0] - wse[1]) / dx
+    h_new = max(wse[0], wse[1]) - (float(elev[0]) - 4.0)
+    q_exact = W * G * dt * math.sqrt(h_new * h) * S
+    got = {}
+    for tag, flags, prec in (("default", 0, 0), ("precise", O.PRECISE_SURFACE, 0), ("fp64", 0, 1),
+                             ("fp64 precise", O.PRECISE_SURFACE, 1)):
+        real = np.float32 if prec == 0 else np.float64
+        over = dict(river_depth=depth, catchment_elevation=elev, river_height=height,
+                    river_cross_section_depth=np.full(3, h, np.float32), river_storage=np.full(3, h * W * dx))
+        s = tiny_state([1, 2, 2], **over)
+        for k, v in s.items():
+            if v.dtype == np.float32:
+                s[k] = v.astype(real)
+        m = model(s, flags=flags, precision=prec)
+        m.kernel("outflow", time_step=dt)
+        got[tag] = (float(s["river_outflow"][0]), float(s["river_cross_section_depth"][0]))
+    assert got["precise"][0] == pytest.approx(q_exact, rel=2e-6)
+    assert got["precise"][1] == pytest.approx(h_new, rel=1e-6)
+    assert abs(got["default"][0] / q_exact - 1.0) > 1e-3
+    assert got["fp64"] == got["fp64 precise"]
+    assert got["fp64"][0] == pytest.approx(q_exact, rel=1e-12)
+
+
 def test_backflow_is_limited_to_five_percent_of_the_storage():
     """Water surface higher downstream: the flow turns upstream, and at most
     5 % of the catchment's storage may leave backwards in one sub-step
