@@ -207,6 +207,17 @@ class CaMaFlood:
             for name in LEVEE_PARAMS:
                 src_arr = np.asarray(batched.get(name, hf[name]), dtype=hf[name].dtype)
                 self.t[name] = torch.from_numpy(np.ascontiguousarray(src_arr)).to(self.device)
+            if batched.keys() & {"levee_crown_height", "levee_base_height", "levee_fraction"}:
+                # per-trial levee parameters obey the module's rules in every trial
+                rows = {k: np.broadcast_to(np.asarray(batched.get(k, hf[k])), (self.T, self.V))
+                        for k in ("levee_crown_height", "levee_base_height", "levee_fraction")}
+                frac, crown, base = rows["levee_fraction"], rows["levee_crown_height"], rows["levee_base_height"]
+                if (~np.isfinite(frac) | (frac < 0) | (frac >= 1)).any():
+                    raise ValueError("levee_fraction must be finite and lie within [0, 1) in every trial")
+                bad = int((~np.isfinite(crown) | ~np.isfinite(base) | (base >= crown)).sum())
+                if bad:
+                    raise ValueError(f"{bad} trial levees have non-finite heights or a base height greater than or "
+                                     f"equal to the crown height")
         if self.R:
             self.t["reservoir_catchment_idx"] = torch.from_numpy(self.inv[hf["reservoir_catchment_idx"]].astype(np.int32)).to(self.device)
             for name in RESERVOIR_PARAMS:

@@ -150,6 +150,56 @@ def test_batched_members_match_standalone_runs(trial_layout, modules):
     del _lib
 
 
+@pytest.mark.parametrize("precise", [False, True])
+def test_batched_geometry_matches_standalone_runs(precise):
+    """Members with their own river Manning's n, river height, catchment
+    elevation (member m raised by 1,000 m per m, where FP32 surfaces are
+    coarser), levee crown height and path sill elevations, and their own
+    runoff, full physics, per-member CFL steps: each member is bit-exact with
+    a standalone oracle run given that member's parameters, with the
+    reference's FP32 surfaces and with precise surfaces."""
+    net, src = small_source()
+    M = 3
+    rng = np.random.default_rng(9)
+    n = src["river_height"].shape[0]
+    batched = {
+        "river_manning": np.stack([src["river_manning"] * rng.lognormal(0.0, 0.2, n) for _ in range(M)]),
+        "river_height": np.stack([src["river_height"] * rng.uniform(0.8, 1.2, n) for _ in range(M)]),
+        "catchment_elevation": np.stack([src["catchment_elevation"] + 1000.0 * m for m in range(M)]),
+        "levee_crown_height": np.stack([src["levee_crown_height"] * rng.uniform(1.0, 1.2, src["levee_crown_height"].shape)
+                                        for _ in range(M)]),
+        "bifurcation_elevation": np.stack([src["bifurcation_elevation"] + 1000.0 * m for m in range(M)]),
+        "runoff": np.stack(_member_sources(src, M)),
+    }
+    batched = {k: v.astype(np.float32) for k, v in batched.items()}
+    model = _model(src, FULL, num_trials=M, batched=batched, per_trial_time_step=True, trial_layout="inner",
+                   precise_surfaces=precise)
+    oracles = [oracle_for(dict(src, **{k: v[m] for k, v in batched.items()}), FULL, precise_surfaces=precise)[0]
+               for m in range(M)]
+    for step in range(5):
+        model.step_advance()
+        counts = model.t["max_sub_steps"].cpu().numpy()
+        for m in range(M):
+            assert int(oracles[m].advance(1800.0, 1)[0]) == counts[m], (step, m)
+    import torch
+    torch.cuda.synchronize()
+    for name in STATE_FIELDS + ("bifurcation_outflow", "bifurcation_cross_section_depth"):
+        got = model.get(name)
+        for m in range(M):
+            msg = mismatch(got[m], oracles[m].state[name])
+            assert msg is None, f"member {m} {name}: {msg}"
+
+
+def test_batched_levee_heights_are_validated():
+    """A trial whose levee crown falls to its base height is refused, as the
+    levee module refuses it for a single run (levee.py:233-244)."""
+    net, src = small_source()
+    crown = np.stack([src["levee_crown_height"], src["levee_crown_height"]]).astype(np.float32)
+    crown[1, 0] = src["levee_base_height"][0]
+    with pytest.raises(ValueError, match="base height"):
+        _model(src, FULL, num_trials=2, batched={"levee_crown_height": crown})
+
+
 def test_batched_shared_time_step():
     """Trials sharing one time step (the reference's batched semantics): every
     member advances the maximum sub-step count; compare with fixed-count runs."""
