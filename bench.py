@@ -310,32 +310,36 @@ def batched_full_line(args, device: str) -> dict:
         f"{M} members ({time.perf_counter() - t0:.1f} s)")
     mods = ("base", "adaptive_time", "bifurcation", "levee", "reservoir")
 
-    def build(precision):
+    def build(precision, precise=False):
         return CaMaFlood(src, opened_modules=mods, device=device, step_duration=1800.0, num_trials=M,
                          batched={"runoff": runoff}, per_trial_time_step=True, trial_layout="inner",
-                         precision=precision)
+                         precision=precision, precise_surfaces=precise)
 
     out = {"workload": f"{N} catchments, {P} paths, {net.num_levees} levees, {net.num_reservoirs} reservoirs, "
                        f"{M} members with their own CFL counts, trial-inner", "members": M}
     bm = build("float32")
     if args.full_days > 0:
-        # FP32 and FP64 profiles from the same state
+        # FP32 and FP64 profiles from the same state, and FP32 with precise
+        # surfaces
         ref = build("float64")
+        pr = build("float32", precise=True)
         for day in range(args.full_days):
             if day:
                 r = np.stack([f.next_day() for f in forcing])
-                for m in (bm, ref):
+                for m in (bm, ref, pr):
                     m.set_inputs(r)
             for _ in range(48):
                 bm.step_advance()
                 ref.step_advance()
+                pr.step_advance()
         errors = {}
         for name in ("river_storage", "flood_storage", "flood_depth"):
             x, y = bm.get(name).astype(np.float64), ref.get(name).astype(np.float64)
             errors[name] = [float(f"{np.linalg.norm(x[t] - y[t]) / max(np.linalg.norm(y[t]), 1e-300):.4g}")
                             for t in range(M)]
-        q32, q64 = ((m.get("river_outflow").astype(np.float64) + m.get("flood_outflow")) for m in (bm, ref))
+        q32, q64, qpr = ((m.get("river_outflow").astype(np.float64) + m.get("flood_outflow")) for m in (bm, ref, pr))
         ref.close()
+        pr.close()
         errors["discharge"] = [float(f"{np.linalg.norm(q32[t] - q64[t]) / np.linalg.norm(q64[t]):.4g}")
                                for t in range(M)]
         # the same outside the reaches below reservoirs: a release switches
@@ -345,8 +349,15 @@ def batched_full_line(args, device: str) -> dict:
         errors["discharge_outside_reservoir_reaches"] = [
             float(f"{np.linalg.norm((q32[t] - q64[t])[keep]) / np.linalg.norm(q64[t][keep]):.4g}")
             for t in range(M)]
-        del q32, q64
-        out.update({"error_days": args.full_days, "rel_l2_per_member": errors,
+        precise = {"discharge": [float(f"{np.linalg.norm(qpr[t] - q64[t]) / np.linalg.norm(q64[t]):.4g}")
+                                 for t in range(M)],
+                   "discharge_outside_reservoir_reaches": [
+                       float(f"{np.linalg.norm((qpr[t] - q64[t])[keep]) / np.linalg.norm(q64[t][keep]):.4g}")
+                       for t in range(M)]}
+        precise["max_discharge_rel_l2"] = max(precise["discharge"])
+        precise["max_discharge_rel_l2_outside_reservoir_reaches"] = max(precise["discharge_outside_reservoir_reaches"])
+        del q32, q64, qpr
+        out.update({"error_days": args.full_days, "rel_l2_per_member": errors, "precise_surfaces": precise,
                     "max_discharge_rel_l2": max(errors["discharge"]),
                     "reservoir_reach_catchments": int((~keep).sum()),
                     "max_discharge_rel_l2_outside_reservoir_reaches":
@@ -369,7 +380,9 @@ def batched_full_line(args, device: str) -> dict:
     msg = f"batched full: {work / el:.4g} member-catchment-steps/s, {gbs:.0f} GB/s algorithmic"
     if "max_discharge_rel_l2" in out:
         msg += (f"; discharge error after {args.full_days} day(s) up to {out['max_discharge_rel_l2']:.3g}, "
-                f"{out['max_discharge_rel_l2_outside_reservoir_reaches']:.3g} outside reservoir reaches")
+                f"{out['max_discharge_rel_l2_outside_reservoir_reaches']:.3g} outside reservoir reaches; precise "
+                f"surfaces {out['precise_surfaces']['max_discharge_rel_l2']:.3g}, "
+                f"{out['precise_surfaces']['max_discharge_rel_l2_outside_reservoir_reaches']:.3g}")
     log(msg)
     return out
 
