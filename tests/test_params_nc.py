@@ -110,3 +110,21 @@ def test_hdf5_and_broken_files_are_refused_with_a_reason(src, tmp_path):
         read_parameters(p)
     with pytest.raises(ParameterFileError, match="CDF-5"):
         write_parameters(tmp_path / "v2.nc", bad, version=2)
+
+
+def test_a_file_without_state_starts_cold(src, tmp_path):
+    """The reference's parameters.nc holds parameters only: the model starts
+    from zero storage outside the reservoirs (whose cold start fills them)
+    and runs (oracle, full physics)."""
+    params = {k: v for k, v in src.items() if k not in ("river_storage", "river_depth")}
+    p = tmp_path / "parameters.nc"
+    write_parameters(p, params)
+    got = read_parameters(p)
+    assert "river_storage" not in got
+    got["runoff"] = synthetic.Runoff(got["catchment_area"], 2007).next_day()
+    orc, host = oracle_for(got, FULL)
+    rs = np.array(host.fields["river_storage"])
+    rs[got["reservoir_catchment_idx"]] = 0.0
+    assert not np.any(rs)
+    orc.advance(1800.0, 2)
+    assert np.all(np.isfinite(orc.state["river_outflow"])) and np.any(orc.state["river_storage"] > 0)
