@@ -63,6 +63,42 @@ MISTAKES = {
     "levee fill scan on the base height": (
         "const REAL ratio_fill = (crown - dhgt_pre) / dhgt_seg;",
         "const REAL ratio_fill = (base - dhgt_pre) / dhgt_seg;"),
+    "levee path levels above the channel on the river surface": (
+        "upd_csd = fmax_(precise ? (REAL)(max_pwd - (double)bel[k]) : max_pw - bel[k], (REAL)0.0);",
+        "upd_csd = fmax_(precise ? (REAL)(max_pwd - (double)bel[k]) : max_w - bel[k], (REAL)0.0);"),
+    "levee path channel level falls back to the 1 cm floor": (
+        "sifd = (semi <= (REAL)0.0) ? upd_csd : semi;",
+        "sifd = (semi <= (REAL)0.0) ? (REAL)std::sqrt(upd_csd * (REAL)0.01) : semi;"),
+    "levee path levels above the channel semi-implicit": (
+        "sifd = upd_csd;",
+        "sifd = (REAL)std::sqrt(upd_csd * csd_old);"),
+    "kinematic flow with depth to the 4/3": (
+        "const REAL kin_riv_vel = ((REAL)1.0 / r_man) * (REAL)std::sqrt(bed_slope) * cbrt_real(r_dep * r_dep);",
+        "const REAL kin_riv_vel = ((REAL)1.0 / r_man) * (REAL)std::sqrt(bed_slope) * cbrt_real(r_dep);"),
+    "kinematic floodplain slope not capped": (
+        "const REAL bed_slope_f = fmin_(bed_slope, (REAL)0.005);",
+        "const REAL bed_slope_f = bed_slope;"),
+    "kinematic bed slope not floored": (
+        "bed_slope = fmax_(bed_slope, min_kin);",
+        "bed_slope = fabs(bed_slope);"),
+    "log inflow sum without the flood inflow": (
+        "q[L_INFLOW] = (double)((rin + fin + inflow_extra) * time_step);",
+        "q[L_INFLOW] = (double)((rin + inflow_extra) * time_step);"),
+    "log stage sums include levee catchments": (
+        "if (non_levee) {",
+        "if (non_levee || true) {"),
+    "log slot overwritten instead of accumulated": (
+        "sums[k * stride + current_step] += acc * 1e-9;",
+        "sums[k * stride + current_step] = acc * 1e-9;"),
+    "log inflow error with the sign reversed": (
+        "q[L_INFLOW_ERROR] = (double)(total_pre - total_next",
+        "q[L_INFLOW_ERROR] = (double)(total_next - total_pre"),
+    "path levels rescaled by the opposite ends": (
+        "bout[k] = (raw >= (REAL)0.0) ? raw * l0 : raw * l1;",
+        "bout[k] = (raw >= (REAL)0.0) ? raw * l1 : raw * l0;"),
+    "path net always scaled by the upstream end": (
+        "const REAL net = (raw_sum >= (REAL)0.0) ? raw_sum * l0 : raw_sum * l1;",
+        "const REAL net = raw_sum * l0;"),
 }
 
 

@@ -588,3 +588,273 @@ def test_reservoir_rule_is_continuous_at_the_thresholds():
         m.kernel("reservoir", time_step=600.0)
         outs.append(float(s["river_outflow"][1]))
     assert outs[0] == pytest.approx(outs[1], rel=1e-3)
+
+
+# ------------------------------------------- levee bifurcation variant --
+def levee_path_state(csd_prev, w=(20.0, 19.0), pw=(18.5, 18.2), bel=(15.0, 18.3, 18.4, 19.5, 30.0)):
+    """One path between catchments 0 and 1 with water surfaces `w`,
+    protected surfaces `pw` (a protected side lower than the river) and
+    huge storages, so that the path limiter stays at 1."""
+    s = tiny_state([1, 1], river_storage=np.array([1e12, 1e12]))
+    Lb = len(bel)
+    s.update({
+        "water_surface_elevation": np.array(w, np.float32),
+        "protected_water_surface_elevation": np.array(pw, np.float32),
+        "total_storage": np.array([1e12, 1e12]),
+        "is_levee": np.array([True, True]),
+        "bifurcation_catchment_idx": np.array([0], np.int32),
+        "bifurcation_downstream_idx": np.array([1], np.int32),
+        "bifurcation_manning": np.full((1, Lb), 0.03, np.float32),
+        "bifurcation_width": np.array([[50.0, 60.0, 70.0, 80.0, 90.0]], np.float32),
+        "bifurcation_length": np.array([4000.0], np.float32),
+        "bifurcation_elevation": np.array([bel], np.float32),
+        "bifurcation_outflow": np.zeros((1, Lb), np.float32),
+        "bifurcation_cross_section_depth": np.array([csd_prev], np.float32),
+        "global_bifurcation_outflow": np.zeros(2), "limit_rate": np.ones(2, np.float32),
+    })
+    return s
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_levee_path_depths(prec):
+    """With levees the path levels split (triton/levee.py:631-653): level 0
+    flows on the river surfaces with the semi-implicit depth, falling back to
+    the explicit depth when the previous depth is zero; levels above flow on
+    the higher protected surface with the explicit depth, whatever their
+    previous depth.  From rest q = W g dt h S, S from the river surfaces."""
+    dt = 20.0
+    w, pw = (20.0, 19.0), (18.5, 18.2)
+    bel = (15.0, 18.3, 18.4, 19.5, 30.0)
+    S = (w[0] - w[1]) / 4000.0
+    widths = (50.0, 60.0, 70.0, 80.0, 90.0)
+    for c0 in (0.0, 2.0):
+        prev = [c0, 5.0, 5.0, 5.0, 5.0]
+        s = levee_path_state(prev, w, pw, bel)
+        if prec == 1:
+            s = {k: (v.astype(np.float64) if v.dtype == np.float32 else v) for k, v in s.items()}
+        m = O.OracleModel(s, flags=O.HAS_BIFURCATION | O.HAS_LEVEE, num_flood_levels=10,
+                          num_bifurcation_levels=5, precision=prec)
+        m.kernel("levee_bifurcation_outflow", time_step=dt)
+        # the inputs as stored (float32 values, widened in the FP64 profile)
+        w32 = [float(np.float32(x)) for x in w]
+        pw32 = [float(np.float32(x)) for x in pw]
+        b32 = [float(np.float32(x)) for x in bel]
+        S = (w32[0] - w32[1]) / 4000.0
+        for lev in range(5):
+            top = max(w32) if lev == 0 else max(pw32)
+            upd = max(top - b32[lev], 0.0)
+            h = upd if lev > 0 or c0 == 0.0 else math.sqrt(upd * c0)
+            q = widths[lev] * G * dt * h * S if h > 1e-5 else 0.0
+            rel = 2e-6 if prec == 0 else 1e-12
+            assert float(s["bifurcation_cross_section_depth"][0, lev]) == pytest.approx(upd, rel=rel, abs=1e-6)
+            assert float(s["bifurcation_outflow"][0, lev]) == pytest.approx(q, rel=rel, abs=1e-9), (c0, lev)
+        # level 2 lies below the river surfaces but above the protected ones:
+        # it carries no flow in the levee variant
+        assert float(s["bifurcation_outflow"][0, 3]) == 0.0
+        assert float(s["bifurcation_outflow"][0, 2]) > 0.0
+
+
+# ----------------------------------------- kinematic flow above a dam --
+def dam_upstream_state(elev0, elev1, rs, fs, r_dep=2.0, f_dep=0.4, dist=8000.0):
+    s = tiny_state([1, 1], river_storage=np.array([rs, 1e9]), flood_storage=np.array([fs, 0.0]),
+                   river_depth=np.array([r_dep, 3.0], np.float32),
+                   flood_depth=np.array([f_dep, 0.0], np.float32),
+                   catchment_elevation=np.array([elev0, elev1], np.float32),
+                   downstream_distance=np.array([dist, dist], np.float32),
+                   river_cross_section_depth=np.array([r_dep, 3.0], np.float32),
+                   river_outflow=np.array([123.0, 0.0], np.float32),
+                   flood_outflow=np.array([45.0, 0.0], np.float32))
+    s["is_dam_upstream"] = np.array([True, False])
+    return s
+
+
+def kinematic_expect(elev0, elev1, rs, fs, dt, r_dep=2.0, f_dep=0.4, dist=8000.0,
+                     W=50.0, L=10_000.0, n_r=0.03, n_f=0.1, min_kin=1e-5):
+    S = max((elev0 - elev1) / dist, min_kin)
+    q_r = W * r_dep * (1.0 / n_r) * math.sqrt(S) * r_dep ** (2.0 / 3.0)
+    q_r = max(min(q_r, rs / dt), 0.0)
+    Sf = min(S, 0.005)
+    area = max(fs / L - f_dep * W, 0.0)
+    q_f = area * (1.0 / n_f) * math.sqrt(Sf) * f_dep ** (2.0 / 3.0)
+    q_f = max(min(q_f, fs / dt), 0.0)
+    return q_r, q_f
+
+
+@pytest.mark.parametrize("case", ["moderate", "uphill", "steep", "capped"])
+def test_kinematic_flow_upstream_of_a_dam(case):
+    """Upstream of a dam the momentum update is replaced by Manning's
+    kinematic flow on the bed slope (triton/outflow.py:251-272):
+    Q = W d n^-1 S^(1/2) d^(2/3) with S the bed slope floored at the minimum
+    kinematic slope, the floodplain on the same slope capped at 0.005, each
+    flow capped at storage / dt and non-negative; the previous flows and the
+    water-surface slope play no part."""
+    elev0, elev1, rs, fs, f_dep, dt = {
+        "moderate": (18.0, 10.0, 1e9, 5e7, 0.4, 600.0),     # bed slope 1e-3
+        "uphill": (10.0, 12.0, 1e9, 5e7, 0.4, 600.0),       # negative: floored at 1e-5
+        "steep": (170.0, 10.0, 1e9, 5e7, 0.4, 600.0),       # 0.02: floodplain on 0.005
+        "capped": (170.0, 10.0, 2e4, 1e8, 8.0, 7200.0),     # storage / dt binds
+    }[case]
+    s = dam_upstream_state(elev0, elev1, rs, fs, f_dep=f_dep)
+    m = model(s, flags=O.HAS_RESERVOIR)
+    m.kernel("outflow", time_step=dt)
+    e32 = [float(np.float32(x)) for x in (elev0, elev1)]
+    q_r, q_f = kinematic_expect(e32[0], e32[1], rs, fs, dt, f_dep=f_dep)
+    assert float(s["river_outflow"][0]) == pytest.approx(q_r, rel=1e-5)
+    assert float(s["flood_outflow"][0]) == pytest.approx(q_f, rel=1e-5)
+    if case == "capped":
+        assert float(s["river_outflow"][0]) == pytest.approx(rs / dt, rel=1e-6)
+        assert float(s["flood_outflow"][0]) == pytest.approx(fs / dt, rel=1e-6)
+    # not above a dam: the same catchment runs the momentum update
+    s2 = dam_upstream_state(elev0, elev1, rs, fs, f_dep=f_dep)
+    s2["is_dam_upstream"] = np.array([False, False])
+    model(s2, flags=O.HAS_RESERVOIR).kernel("outflow", time_step=dt)
+    assert float(s2["river_outflow"][0]) != pytest.approx(q_r, rel=1e-3)
+
+
+# ---------------------------------------------- water-balance log sums --
+def log_network(n=60, seed=11, levee_every=0, clip=False):
+    rng = np.random.default_rng(seed)
+    dn = np.array([min(n - 1, i + 1 + int(rng.integers(0, 4))) for i in range(n)], np.int32)
+    dn[n - 1] = n - 1
+    dn[n - 5] = n - 5
+    rs = rng.uniform(2e6, 4e6, n)
+    if levee_every:
+        rs = rng.uniform(1e6, 8e8, n)       # every levee case, and dry catchments
+    qr = rng.uniform(0.0, 20.0, n).astype(np.float32)
+    qf = rng.uniform(0.0, 5.0, n).astype(np.float32)
+    if clip:
+        qr[3] = np.float32(1e6)             # far more than the catchment holds
+    rin = np.zeros(n)
+    fin = np.zeros(n)
+    for i in range(n):
+        if dn[i] != i:
+            rin[dn[i]] += qr[i]
+            fin[dn[i]] += qf[i]
+    s = tiny_state(dn, river_outflow=qr, flood_outflow=qf, river_inflow=rin, flood_inflow=fin,
+                   river_storage=rs, flood_storage=rng.uniform(0.0, 1e5, n),
+                   protected_storage=np.where(np.arange(n) % 7 == 0, 1e4, 0.0),
+                   runoff=rng.uniform(0.0, 3.0, n).astype(np.float32),
+                   catchment_area=rng.uniform(1e8, 1e9, n).astype(np.float32))
+    s["log_sums"] = np.zeros((11, 4))
+    flags = O.HAS_LOG
+    if levee_every:
+        lv = np.arange(0, n, levee_every).astype(np.int32)
+        isl = np.zeros(n, bool)
+        isl[lv] = True
+        s.update({"is_levee": isl, "levee_catchment_idx": lv,
+                  "levee_base_height": np.full(lv.size, 1.0, np.float32),
+                  "levee_crown_height": np.full(lv.size, 3.0, np.float32),
+                  "levee_fraction": np.full(lv.size, 0.3, np.float32)})
+        flags |= O.HAS_LEVEE
+    s = {k: (v.astype(np.float64) if v.dtype == np.float32 else v) for k, v in s.items()}
+    return s, flags
+
+
+# log slots (cmfgpu/modules/log.py:207-326 field order)
+PRE, NEXT, NEW, IN_ERR, INFLOW, OUTFLOW, STAGE, STAGE_ERR, RIVER, FLOOD, AREA = range(11)
+
+
+def test_log_sums_against_brute_force():
+    """The eleven FP64 sums of compute_flood_stage_log (triton/storage.py:
+    276-305, 395-414), each against a brute-force float64 sum of the state
+    before and after, in units of 1e9 m3, added into slot current_step."""
+    s, flags = log_network()
+    old = {k: v.copy() for k, v in s.items()}
+    s["log_sums"][:, 2] = 7.0                # the slot accumulates
+    dt = 300.0
+    m = O.OracleModel(s, flags=flags, num_flood_levels=10, log_buffer_size=4, precision=1)
+    m.kernel("flood_stage", time_step=dt, current_step=2)
+    got = s["log_sums"][:, 2] - 7.0
+    assert np.all(s["log_sums"][:, [0, 1, 3]] == 0.0)
+    pre = old["river_storage"] + old["flood_storage"] + old["protected_storage"]
+    new = s["river_storage"] + s["flood_storage"] + s["protected_storage"]
+    inflow = (old["river_inflow"] + old["flood_inflow"]) * dt
+    outflow = (old["river_outflow"] + old["flood_outflow"]) * dt
+    runoff = old["runoff"] * dt
+    tol = dict(rel=1e-12)
+    assert got[PRE] == pytest.approx(pre.sum() * 1e-9, **tol)
+    assert got[INFLOW] == pytest.approx(inflow.sum() * 1e-9, **tol)
+    assert got[OUTFLOW] == pytest.approx(outflow.sum() * 1e-9, **tol)
+    # no clipping: next = pre + inflow + runoff - outflow = new
+    expect_next = (pre + inflow + runoff - outflow).sum() * 1e-9
+    assert got[NEXT] == pytest.approx(expect_next, **tol)
+    assert got[NEW] == pytest.approx(new.sum() * 1e-9, **tol)
+    assert got[STAGE] == pytest.approx(new.sum() * 1e-9, **tol)
+    assert got[RIVER] == pytest.approx(s["river_storage"].sum() * 1e-9, **tol)
+    assert got[FLOOD] == pytest.approx(s["flood_storage"].sum() * 1e-9, **tol)
+    assert got[AREA] == pytest.approx((s["flood_fraction"] * s["catchment_area"]).sum() * 1e-9, **tol)
+    scale = pre.sum() * 1e-9
+    assert abs(got[IN_ERR]) <= 1e-13 * scale and abs(got[STAGE_ERR]) <= 1e-13 * scale
+
+
+def test_log_inflow_error_holds_the_clipped_deficit():
+    """A river outflow far beyond the catchment's storage is clipped at zero
+    storage (storage.py:81-94): the inflow-error sum is then pre - next +
+    net inflow dt, i.e. minus the water that was asked for but not there."""
+    s, flags = log_network(clip=True)
+    old = {k: v.copy() for k, v in s.items()}
+    dt = 300.0
+    m = O.OracleModel(s, flags=flags, num_flood_levels=10, log_buffer_size=4, precision=1)
+    m.kernel("flood_stage", time_step=dt, current_step=0)
+    c = 3
+    net = (old["river_inflow"][c] + old["flood_inflow"][c] + old["runoff"][c]
+           - old["river_outflow"][c] - old["flood_outflow"][c]) * dt
+    pre_c = old["river_storage"][c] + old["flood_storage"][c] + old["protected_storage"][c]
+    # the clipped catchment keeps its protected storage and receives its runoff
+    next_c = old["protected_storage"][c] + old["runoff"][c] * dt
+    deficit = pre_c - next_c + net           # < 0: the water that was not there
+    assert deficit < -1e8
+    assert s["log_sums"][IN_ERR, 0] == pytest.approx(deficit * 1e-9, rel=1e-9)
+
+
+def test_log_stage_sums_count_levee_catchments_once():
+    """compute_flood_stage_log leaves levee catchments out of the stage sums
+    (storage.py:251-255) and compute_levee_stage_log adds them after the
+    partition (levee.py:549-553): over both, the stage sum equals the new
+    total storage of every catchment, its error is zero, and the river and
+    flood storage sums leave exactly the protected storage out."""
+    s, flags = log_network(levee_every=3)
+    dt = 300.0
+    m = O.OracleModel(s, flags=flags, num_flood_levels=10, log_buffer_size=4, precision=1)
+    m.kernel("flood_stage", time_step=dt, current_step=1)
+    lv = s["levee_catchment_idx"]
+    non = np.ones(s["river_storage"].size, bool)
+    non[lv] = False
+    after_stage = s["log_sums"][:, 1].copy()
+    assert after_stage[STAGE] == pytest.approx((s["river_storage"] + s["flood_storage"])[non].sum() * 1e-9,
+                                               rel=1e-12)
+    m.kernel("levee_stage", current_step=1)
+    got = s["log_sums"][:, 1]
+    new = s["river_storage"] + s["flood_storage"] + s["protected_storage"]
+    assert s["protected_storage"][lv].sum() > 0            # some levee holds water behind it
+    assert got[STAGE] == pytest.approx(new.sum() * 1e-9, rel=1e-12)
+    assert got[STAGE] == pytest.approx(got[NEW], rel=1e-12)
+    assert abs(got[STAGE_ERR]) <= 1e-13 * got[NEW]
+    assert got[STAGE] - got[RIVER] - got[FLOOD] == pytest.approx(s["protected_storage"].sum() * 1e-9, rel=1e-9)
+    assert got[AREA] == pytest.approx((s["flood_fraction"] * s["catchment_area"]).sum() * 1e-9, rel=1e-12)
+
+
+# --------------------------------------------- path nets and rescale --
+def test_path_levels_rescaled_by_the_end_limiters():
+    """compute_bifurcation_inflow (triton/bifurcation.py:128-151): a level
+    flowing from the upstream end is scaled by that end's storage limiter,
+    one flowing back by the downstream end's; the net of the raw level sum
+    likewise, added at the upstream end and taken at the downstream end."""
+    s, n, P, Lb = bif_state(P=2)
+    s["bifurcation_catchment_idx"] = np.array([2, 6], np.int32)
+    s["bifurcation_downstream_idx"] = np.array([5, 9], np.int32)
+    raw = np.array([[3.0, -1.0, 2.0, -4.0, 0.5], [-6.0, 1.0, -0.25, 2.0, 0.0]], np.float32)
+    s["bifurcation_outflow"] = raw.copy()
+    lim = np.ones(n, np.float32)
+    lim[2], lim[5], lim[6], lim[9] = 0.5, 0.25, 0.75, 0.125
+    s["limit_rate"] = lim
+    m = O.OracleModel(s, flags=O.HAS_BIFURCATION, num_flood_levels=10, num_bifurcation_levels=Lb)
+    m.kernel("bifurcation_inflow")
+    for p, (a, b) in enumerate(((2, 5), (6, 9))):
+        want = np.where(raw[p] >= 0, raw[p] * lim[a], raw[p] * lim[b])
+        assert np.array_equal(s["bifurcation_outflow"][p], want.astype(np.float32))
+        tot = float(raw[p].astype(np.float64).sum())
+        net = tot * (lim[a] if tot >= 0 else lim[b])
+        assert s["global_bifurcation_outflow"][a] == net
+        assert s["global_bifurcation_outflow"][b] == -net
+    assert s["global_bifurcation_outflow"].sum() == 0.0
