@@ -59,6 +59,7 @@ BATCHED_BITS = {
 
 PRECISION_F32_F64 = 0
 PRECISION_F64_F64 = 1
+PRECISION_F32_F32 = 2
 
 LOG_FIELDS = (
     "total_storage_pre_sum",

@@ -104,7 +104,8 @@ def _declare():
 
 
 class GaugeProblem:
-    """A calibration problem on a built model: gauges (network order), the
+    """A calibration problem on a built model: gauges (catchment indices in
+    the model's input order, the reference's order that ``get`` returns), the
     observed discharge [S, G], the runoff of each outer step (one array, or a
     callable step -> array) and the checkpoint interval K."""
 
@@ -159,7 +160,8 @@ class GaugeProblem:
             self.model.set_inputs(r)
 
     def set_parameters(self, manning=None, height=None):
-        """Per-catchment river Manning's n and river height, network order."""
+        """Per-catchment river Manning's n and river height, in the model's
+        input order (the reference's catchment order, as ``get`` returns)."""
         m = self.model
         perm = torch.as_tensor(m.perm, device=m.device)
         for name, x in (("river_manning", manning), ("river_height", height)):
@@ -222,8 +224,8 @@ class GaugeProblem:
         return self.last_loss
 
     def gradient(self):
-        """dL/d(river_manning), dL/d(river_height) (float64, network order)
-        for the last `run`."""
+        """dL/d(river_manning), dL/d(river_height) (float64 device tensors, in
+        the model's input order) for the last `run`."""
         if not self.counts:
             raise RuntimeError("run() first")
         m = self.model
@@ -334,17 +336,23 @@ class GaugeProblem:
 
 class GaugeLoss(torch.autograd.Function):
     """loss = GaugeLoss.apply(manning, height, problem): the gauge-discharge
-    loss of `problem` as a function of per-catchment river Manning's n and
-    river height (network order); backward runs the adjoint."""
+    loss of `problem` (a float64 scalar on the inputs' device) as a function
+    of per-catchment river Manning's n and river height, both in the model's
+    input order; backward runs the adjoint and returns each gradient in its
+    input's dtype and on its input's device."""
 
     @staticmethod
     def forward(ctx, manning, height, problem):
         problem.set_parameters(manning.detach(), height.detach())
         loss = problem.run()
         ctx.problem = problem
+        ctx.meta = ((manning.dtype, manning.device), (height.dtype, height.device))
         return torch.tensor(loss, dtype=torch.float64, device=manning.device)
 
     @staticmethod
     def backward(ctx, grad_out):
         g_n, g_h = ctx.problem.gradient()
-        return (grad_out * g_n).to(grad_out.device), (grad_out * g_h).to(grad_out.device), None
+        (dn, devn), (dh, devh) = ctx.meta
+        gn = (grad_out.to(g_n.device) * g_n).to(device=devn, dtype=dn)
+        gh = (grad_out.to(g_h.device) * g_h).to(device=devh, dtype=dh)
+        return gn, gh, None

@@ -92,7 +92,7 @@ class CaMaFlood:
                  start_time: datetime = datetime(2000, 1, 1), log_path: str | Path | None = None,
                  mode: str = "fused", gravity: float = 9.8, adaptive_time_factor: float = 0.7,
                  min_kinematic_slope: float = 1e-5, groups=None, use_graph: bool = False,
-                 precise_surfaces: bool = False):
+                 precise_surfaces: bool = False, mixed_precision: bool = True):
         if mode not in ("fused", "ops"):
             raise ValueError("mode must be 'fused' or 'ops'")
         if layout not in ("basin", "level", "network"):
@@ -125,7 +125,8 @@ class CaMaFlood:
         self.log_path = Path(log_path) if log_path is not None else None
         self.host = assemble(src, opened, precision=precision, log_buffer_size=log_buffer_size,
                              gravity=gravity, adaptive_time_factor=adaptive_time_factor,
-                             min_kinematic_slope=min_kinematic_slope, precise_surfaces=precise_surfaces)
+                             min_kinematic_slope=min_kinematic_slope, precise_surfaces=precise_surfaces,
+                             mixed_precision=mixed_precision)
         self.precise_surfaces = bool(precise_surfaces)
         self.flags = self.host.flags | (_lib.PER_TRIAL_DT if self.per_trial else 0)
         self.real = self.host.real
@@ -167,6 +168,12 @@ class CaMaFlood:
         self.layout = layout
 
         batched = dict(batched or {})
+        if (self.V and "levee_base_height" not in batched
+                and batched.keys() & {"levee_fraction", "flood_depth_table"}):
+            # the base height is derived from the fraction over the floodplain
+            # table (levee.py:140-217), so a trial with its own fraction or
+            # table gets its own base height
+            batched["levee_base_height"] = self._trial_levee_base(batched)
         self.batched_names = set(batched)
         bits = 0
         for name in batched:
@@ -247,6 +254,28 @@ class CaMaFlood:
         self._table = self._build_table()
         self.step_count = 0
         self.last_sub_steps = 0
+        self._perm_dev = None
+        # with the log module: keep each outer step's sums in last_log_sums
+        self.keep_log_sums = False
+        self.last_log_sums = None
+
+    def _trial_levee_base(self, batched: dict) -> np.ndarray:
+        """[T, V] levee base heights, each trial's from its own fraction and
+        floodplain table (LeveeModule._interp_lookup, levee.py:172-217)."""
+        from ..modules.levee import interp_lookup
+        hf = self.host.fields
+        real = hf["levee_fraction"].dtype.type
+        lidx = hf["levee_catchment_idx"]
+        frac = np.broadcast_to(np.asarray(batched.get("levee_fraction", hf["levee_fraction"]), dtype=real),
+                               (self.T, self.V))
+        table = np.asarray(batched.get("flood_depth_table", hf["flood_depth_table"]),
+                           dtype=hf["flood_depth_table"].dtype)
+        L = table.shape[-1]
+        rows = []
+        for t in range(self.T):
+            tab = table[t] if table.ndim == 3 else table
+            rows.append(interp_lookup(tab, lidx, (frac[t] * real(L)).astype(real)))
+        return np.stack(rows).astype(real)
 
     # ------------------------------------------------------------ layout --
     def _to_catchment(self, name, arr, per_trial=False, replicate=False):
@@ -296,7 +325,10 @@ class CaMaFlood:
         f.num_reservoirs = self.R
         f.log_buffer_size = self.host.log_buffer_size
         f.trial_stride, f.catchment_stride, f.path_trial_stride, f.path_stride = self._strides()
-        f.precision = _lib.PRECISION_F32_F64 if self.real == np.float32 else _lib.PRECISION_F64_F64
+        if self.real == np.float64:
+            f.precision = _lib.PRECISION_F64_F64
+        else:
+            f.precision = _lib.PRECISION_F32_F64 if self.host.sto == np.float64 else _lib.PRECISION_F32_F32
         f.flags = self.flags
         f.batched = self.batched_bits
         f.gravity = self.host.gravity
@@ -378,11 +410,29 @@ class CaMaFlood:
         return self.t["max_sub_steps"].cpu().numpy()
 
     def set_inputs(self, runoff) -> None:
-        """Stage runoff forcing (cama_flood_model.py:74-102): [N], or [T, N] when runoff is batched."""
-        r = runoff.detach().cpu().numpy() if isinstance(runoff, torch.Tensor) else np.asarray(runoff)
-        r = r.astype(self.real, copy=False)
+        """Stage runoff forcing (cama_flood_model.py:74-102): [N], or [T, N] when runoff is batched.
+
+        A torch tensor is permuted into the device layout on its own device
+        (an index gather on the GPU when it lives there, no host round trip);
+        a numpy array is permuted on the host and copied once."""
         per_trial = "runoff" in self.batched_names
         expect = (self.T, self.N) if (per_trial and self.T > 1) else (self.N,)
+        if isinstance(runoff, torch.Tensor):
+            if tuple(runoff.shape) != expect:
+                raise ValueError(f"runoff must have shape {expect}, got {tuple(runoff.shape)}")
+            dst = self.t["runoff"]
+            r = runoff.detach().to(device=dst.device, dtype=dst.dtype)
+            if self._perm_dev is None:
+                self._perm_dev = torch.as_tensor(self.perm, device=dst.device)
+            if per_trial and self.T > 1:
+                r = r.index_select(1, self._perm_dev)
+                if self.trial_layout == "inner":
+                    r = r.t()
+            else:
+                r = r.index_select(0, self._perm_dev)
+            dst.copy_(r)
+            return
+        r = np.asarray(runoff).astype(self.real, copy=False)
         if r.shape != expect:
             raise ValueError(f"runoff must have shape {expect}, got {r.shape}")
         if per_trial and self.T > 1:
@@ -436,8 +486,13 @@ class CaMaFlood:
             self.t["log_sums"].zero_()
         elif log is not None:
             log.set_time(self.step_duration / n, n, self.current_time)
-            if self.log_path is not None:
-                log.write_step(self.log_path, self.log_sums())
+            if self.log_path is not None or self.keep_log_sums:
+                sums = self.log_sums()
+                if self.log_path is not None:
+                    log.write_step(self.log_path, sums)
+                if self.keep_log_sums:
+                    # this outer step's sums, one column per sub-step: [11, n]
+                    self.last_log_sums = sums[:, :n].copy()
             self.t["log_sums"].zero_()
         if "acc_outflow" in self.t:
             self._acc_seconds += self.step_duration

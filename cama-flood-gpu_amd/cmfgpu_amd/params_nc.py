@@ -44,6 +44,39 @@ ID_FIELDS = {
 _RESERVOIR_FIELDS = ("conservation_volume", "emergency_volume", "adjustment_volume", "normal_outflow",
                      "effective_normal_outflow", "adjustment_outflow", "flood_control_outflow")
 
+# Dimensions of the schema's fields by name: the map metadata
+# (cmfgpu/params/schema.py:44-73) and the module fields (schema.py:13-22,
+# modules/base.py:92-270, bifurcation.py:75-150, levee.py:84-110,
+# reservoir.py:76-120).  A name not listed here is placed by its shape.
+_C, _P, _PL = ("catchment",), ("bifurcation_path",), ("bifurcation_path", "bifurcation_level")
+FIELD_DIMS: dict[str, tuple[str, ...]] = {
+    "nx": (), "ny": (), "num_basins": (),
+    "basin_sizes": ("basin",),
+    "output_catchment_id": ("saved_points",),
+    "flood_depth_table": ("catchment", "flood_level"),
+    **{k: _C for k in (
+        "catchment_id", "catchment_basin_id", "downstream_id", "river_width", "river_length", "river_height",
+        "catchment_elevation", "catchment_area", "runoff", "downstream_distance", "river_manning", "flood_manning",
+        "river_storage", "flood_storage", "protected_storage", "protected_depth", "river_depth", "flood_depth",
+        "river_outflow", "flood_outflow", "river_cross_section_depth", "flood_cross_section_depth",
+        "flood_cross_section_area", "flood_fraction", "catchment_x", "catchment_y", "catchment_mainstem_basin_id",
+        "upstream_area", "river_mouth_id", "longitude", "latitude", "satellite_width")},
+    **{k: _P for k in ("bifurcation_path_id", "bifurcation_catchment_id", "bifurcation_downstream_id",
+                       "bifurcation_length", "bifurcation_catchment_x", "bifurcation_downstream_x",
+                       "bifurcation_catchment_y", "bifurcation_downstream_y")},
+    **{k: _PL for k in ("bifurcation_manning", "bifurcation_width", "bifurcation_elevation", "bifurcation_outflow",
+                        "bifurcation_cross_section_depth")},
+    **{k: ("levee",) for k in ("levee_id", "levee_catchment_id", "levee_crown_height", "levee_fraction",
+                               "levee_base_height", "levee_catchment_x", "levee_catchment_y")},
+    **{k: ("reservoir",) for k in ("reservoir_id", "reservoir_catchment_id", "reservoir_capacity",
+                                   "reservoir_area") + _RESERVOIR_FIELDS},
+    **{k: ("gauge",) for k in ("gauge_catchment_id", "gauge_station_id", "gauge_reported_area_km2",
+                               "gauge_allocated_area_km2", "gauge_alloc_error")},
+}
+# dimension order of a written file
+_DIM_ORDER = ("catchment", "flood_level", "bifurcation_path", "bifurcation_level", "levee", "reservoir", "basin",
+              "gauge", "saved_points")
+
 
 class ParameterFileError(ValueError):
     pass
@@ -137,6 +170,9 @@ def read_nc(path) -> tuple[dict, dict, dict]:
             raise ParameterFileError(f"{name}: record variables are not part of the parameter schema")
         if code not in _TYPES or code == 2:
             raise ParameterFileError(f"{name}: unsupported type code {code}")
+        if code > 6 and r.version != 5:
+            raise ParameterFileError(f"{name}: type code {code} exists only in CDF-5 files (CDF-{r.version} "
+                                     f"allows codes 1-6)")
         dt = np.dtype(_TYPES[code])
         count = int(np.prod(shape, dtype=np.int64)) if shape else 1
         if begin + count * dt.itemsize > len(buf):
@@ -179,8 +215,11 @@ def read_parameters(path) -> dict:
 
 
 def _dims_of(name: str, a: np.ndarray, n: int) -> tuple[str, ...]:
-    if name == "flood_depth_table":
-        return ("catchment", "flood_level")
+    if name in FIELD_DIMS:
+        dims = FIELD_DIMS[name]
+        if len(dims) != a.ndim:
+            raise ParameterFileError(f"{name}: the schema gives it dimensions {dims}, got shape {a.shape}")
+        return dims
     if name.startswith("bifurcation_"):
         return ("bifurcation_path", "bifurcation_level")[:a.ndim]
     if name.startswith("levee_"):
@@ -207,7 +246,12 @@ def write_parameters(path, fields: dict, version: int = 5, title: str = "CaMa-Fl
             continue
         a = np.asarray(a)
         if a.dtype == np.bool_:
-            a = a.astype(np.uint8)
+            # NC_UBYTE (code 7) exists only in CDF-5: CDF-1/2 files get NC_BYTE
+            a = a.astype(np.uint8 if version == 5 else np.int8)
+        if version != 5 and a.dtype == np.uint8:
+            if a.size and a.max() > 127:
+                raise ParameterFileError(f"{name}: uint8 values above 127 need CDF-5 (version=5)")
+            a = a.astype(np.int8)
         if version != 5 and a.dtype == np.int64:
             if a.size and (a.min() < -2 ** 31 or a.max() >= 2 ** 31):
                 raise ParameterFileError(f"{name}: int64 values need CDF-5 (version=5)")
@@ -221,8 +265,7 @@ def write_parameters(path, fields: dict, version: int = 5, title: str = "CaMa-Fl
             if dim_len.setdefault(d, size) != size:
                 raise ParameterFileError(f"{name}: dimension {d} is {size}, elsewhere {dim_len[d]}")
         items.append((name, dn, a))
-    order = [d for d in ("catchment", "flood_level", "bifurcation_path", "bifurcation_level", "levee", "reservoir")
-             if d in dim_len]
+    order = [d for d in _DIM_ORDER if d in dim_len]
     dim_id = {d: k for k, d in enumerate(order)}
 
     def count(v: int) -> bytes:

@@ -16,6 +16,9 @@ from .modules import (AdaptiveTimeModule, BaseModule, BifurcationModule, LeveeMo
                       ReservoirModule)
 
 KNOWN_MODULES = ("base", "adaptive_time", "bifurcation", "levee", "reservoir", "log")
+# the hpfloat fields (modules/base.py:197-216,325-410, bifurcation.py, reservoir.py)
+STORAGE_FIELDS = ("river_storage", "flood_storage", "protected_storage", "outgoing_storage", "total_storage",
+                  "river_inflow", "flood_inflow", "global_bifurcation_outflow", "reservoir_total_inflow")
 
 
 @dataclass
@@ -30,6 +33,7 @@ class HostState:
     min_kinematic_slope: float = 1e-5
     real: type = np.float32
     modules: dict = field(default_factory=dict)
+    sto: type = np.float64
 
     @property
     def num_catchments(self) -> int:
@@ -47,9 +51,13 @@ def real_dtype(precision) -> type:
 def assemble(src: dict, opened_modules=("base", "adaptive_time"), *, precision="float32",
              log_buffer_size: int = 0, gravity: float = 9.8, adaptive_time_factor: float = 0.7,
              min_kinematic_slope: float = 1e-5, cold_start: bool = True,
-             precise_surfaces: bool = False) -> HostState:
+             precise_surfaces: bool = False, mixed_precision: bool = True) -> HostState:
     """``precise_surfaces``: in the FP32 profile form water surfaces and their
-    differences in double (the PRECISE_SURFACE flag); no effect in FP64."""
+    differences in double (the PRECISE_SURFACE flag); no effect in FP64.
+    ``mixed_precision``: storages and accumulators (the hpfloat fields) in
+    FP64 under ``precision="float32"``; with False they are FP32 as well, the
+    reference's float32 profile without mixed precision
+    (cmfgpu/phys/triton/utils.py:18-30).  The water-balance sums stay FP64."""
     opened = set(opened_modules) | {"base"}
     unknown = opened - set(KNOWN_MODULES)
     if unknown:
@@ -96,7 +104,13 @@ def assemble(src: dict, opened_modules=("base", "adaptive_time"), *, precision="
         flags |= _lib.HAS_WATER_SURFACE
     if base.has_protected_water_surface:
         flags |= _lib.HAS_PROTECTED_WATER_SURFACE
+    sto = np.float64
+    if not mixed_precision and real == np.float32:
+        sto = np.float32
+        for name in STORAGE_FIELDS:
+            if name in fields:
+                fields[name] = np.ascontiguousarray(fields[name], dtype=np.float32)
     return HostState(fields=fields, flags=flags, num_flood_levels=base.num_flood_levels,
                      num_bifurcation_levels=nb_levels, log_buffer_size=log_buffer_size if "log" in opened else 0,
                      gravity=gravity, adaptive_time_factor=adaptive_time_factor,
-                     min_kinematic_slope=min_kinematic_slope, real=real, modules=mods)
+                     min_kinematic_slope=min_kinematic_slope, real=real, modules=mods, sto=sto)

@@ -536,6 +536,7 @@ int dispatch_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
     if (!f) return set_error(CMF_ERR_INVALID, "null field table");
     if (f->precision == CMF_PRECISION_F32_F64) return run_op<float, double>(net, f, op, stream);
     if (f->precision == CMF_PRECISION_F64_F64) return run_op<double, double>(net, f, op, stream);
+    if (f->precision == CMF_PRECISION_F32_F32) return run_op<float, float>(net, f, op, stream);
     return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
 }
 
@@ -1286,6 +1287,7 @@ int cmf_hip_substep(cmf_network* net, const cmf_fields* f, int32_t substep, int3
     if (!f) return set_error(CMF_ERR_INVALID, "null field table");
     if (f->precision == CMF_PRECISION_F32_F64) return run_substep<float, double>(net, f, substep, store_inflow, s);
     if (f->precision == CMF_PRECISION_F64_F64) return run_substep<double, double>(net, f, substep, store_inflow, s);
+    if (f->precision == CMF_PRECISION_F32_F32) return run_substep<float, float>(net, f, substep, store_inflow, s);
     return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
 }
 
@@ -1330,6 +1332,8 @@ int cmf_hip_advance(cmf_network* net, const cmf_fields* f, double outer_seconds,
         return run_advance<float, double>(net, f, outer_seconds, fixed_count, out_count, s, false);
     if (f->precision == CMF_PRECISION_F64_F64)
         return run_advance<double, double>(net, f, outer_seconds, fixed_count, out_count, s, false);
+    if (f->precision == CMF_PRECISION_F32_F32)
+        return run_advance<float, float>(net, f, outer_seconds, fixed_count, out_count, s, false);
     return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
 }
 
@@ -1384,6 +1388,8 @@ int cmf_hip_advance_graph(cmf_network* net, const cmf_fields* f, double outer_se
         return run_advance<float, double>(net, f, outer_seconds, fixed_count, out_count, s, true);
     if (f->precision == CMF_PRECISION_F64_F64)
         return run_advance<double, double>(net, f, outer_seconds, fixed_count, out_count, s, true);
+    if (f->precision == CMF_PRECISION_F32_F32)
+        return run_advance<float, float>(net, f, outer_seconds, fixed_count, out_count, s, true);
     return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
 }
 

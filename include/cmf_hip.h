@@ -88,8 +88,12 @@ enum {
 };
 
 /* Precision profiles: REAL is the flux/stage dtype, STO the hpfloat storage
- * dtype (cmfgpu/phys/cuda/outflow.cu:301-366 dispatches the same pairs). */
-enum { CMF_PRECISION_F32_F64 = 0, CMF_PRECISION_F64_F64 = 1 };
+ * dtype (cmfgpu/phys/cuda/outflow.cu:301-366 dispatches the same three pairs;
+ * F32_F32 is the reference's precision float32 without mixed precision,
+ * cmfgpu/phys/triton/utils.py:18-30).  F32_F32 runs the forward entry points
+ * (per-op, sub-step, advance, graph); the taped and adjoint entry points take
+ * the other two.  The water-balance sums stay FP64 in every profile. */
+enum { CMF_PRECISION_F32_F64 = 0, CMF_PRECISION_F64_F64 = 1, CMF_PRECISION_F32_F32 = 2 };
 
 /* Tunable kernels of the fused schedule (block sizes: multiples of 64 up to
  * 512, see cmf_network_set_block_sizes). */

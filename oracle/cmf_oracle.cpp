@@ -955,14 +955,14 @@ int cmf_oracle_max_threads() { return omp_get_max_threads(); }
 // Advance `num_outer` outer steps of `outer_seconds` each.  With HAS_ADAPTIVE
 // the sub-step count comes from the CFL rule; otherwise `fixed_count` is used.
 // `counts_out[k]` receives the sub-step count of outer step k.  precision 0 is
-// (float, double), 1 is (double, double).
+// (float, double), 1 is (double, double), 2 is (float, float).
 int cmf_oracle_advance(int precision, void* const* ptrs, const int64_t* dims,
                        const double* scalars, int flags, double outer_seconds,
                        int fixed_count, int num_outer, int32_t* counts_out) {
     for (int k = 0; k < num_outer; ++k) {
-        int c = precision == 0
-            ? run_op<float, double>(0, ptrs, dims, scalars, flags, outer_seconds, fixed_count)
-            : run_op<double, double>(0, ptrs, dims, scalars, flags, outer_seconds, fixed_count);
+        int c = precision == 0 ? run_op<float, double>(0, ptrs, dims, scalars, flags, outer_seconds, fixed_count)
+              : precision == 2 ? run_op<float, float>(0, ptrs, dims, scalars, flags, outer_seconds, fixed_count)
+                               : run_op<double, double>(0, ptrs, dims, scalars, flags, outer_seconds, fixed_count);
         if (counts_out) counts_out[k] = c;
     }
     return 0;
@@ -970,9 +970,9 @@ int cmf_oracle_advance(int precision, void* const* ptrs, const int64_t* dims,
 
 int cmf_oracle_adaptive_steps(int precision, void* const* ptrs, const int64_t* dims,
                               const double* scalars, int flags, double outer_seconds) {
-    return precision == 0
-        ? run_op<float, double>(1, ptrs, dims, scalars, flags, outer_seconds, 0)
-        : run_op<double, double>(1, ptrs, dims, scalars, flags, outer_seconds, 0);
+    return precision == 0 ? run_op<float, double>(1, ptrs, dims, scalars, flags, outer_seconds, 0)
+         : precision == 2 ? run_op<float, float>(1, ptrs, dims, scalars, flags, outer_seconds, 0)
+                          : run_op<double, double>(1, ptrs, dims, scalars, flags, outer_seconds, 0);
 }
 
 // Single-kernel entry points, for pinning each kernel on its own.
@@ -1012,6 +1012,10 @@ int cmf_oracle_kernel(int precision, int op, void* const* ptrs, const int64_t* d
     };
     if (precision == 0) {
         Model<float, double> m{ptrs, dims, scalars, flags, {}, {}, {}, {}};
+        return go(m, (float)time_step);
+    }
+    if (precision == 2) {
+        Model<float, float> m{ptrs, dims, scalars, flags, {}, {}, {}, {}};
         return go(m, (float)time_step);
     }
     Model<double, double> m{ptrs, dims, scalars, flags, {}, {}, {}, {}};

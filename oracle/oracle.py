@@ -98,7 +98,8 @@ def set_threads(threads: int) -> int:
 class OracleModel:
     """Sequential CPU model over a dict of numpy arrays.
 
-    precision 0: REAL float32 / STO float64; precision 1: float64 / float64.
+    precision 0: REAL float32 / STO float64; 1: float64 / float64; 2: float32 /
+    float32 (the water-balance sums stay float64).
     """
 
     def __init__(self, state: dict, *, flags: int, num_flood_levels: int,
@@ -118,7 +119,8 @@ class OracleModel:
         self._check_dtypes()
 
     def _check_dtypes(self):
-        real = np.float32 if self.precision == 0 else np.float64
+        real = np.float64 if self.precision == 1 else np.float32
+        sto = np.float32 if self.precision == 2 else np.float64
         for name, arr in self.state.items():
             if name not in FIELDS:
                 continue
@@ -130,7 +132,9 @@ class OracleModel:
                 want = np.bool_
             elif name in ("river_inflow", "flood_inflow", "river_storage", "flood_storage",
                           "protected_storage", "outgoing_storage", "total_storage",
-                          "global_bifurcation_outflow", "reservoir_total_inflow", "log_sums"):
+                          "global_bifurcation_outflow", "reservoir_total_inflow"):
+                want = sto
+            elif name == "log_sums":
                 want = np.float64
             else:
                 want = real

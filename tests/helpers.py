@@ -26,7 +26,7 @@ def oracle_for(src, modules, precision="float32", log_buffer_size=0, **kw):
     state = {k: v.copy() for k, v in host.fields.items()}
     return OracleModel(state, flags=host.flags, num_flood_levels=host.num_flood_levels,
                        num_bifurcation_levels=host.num_bifurcation_levels, log_buffer_size=log_buffer_size,
-                       precision=0 if host.real == np.float32 else 1,
+                       precision=(0 if host.sto == np.float64 else 2) if host.real == np.float32 else 1,
                        gravity=host.gravity, adaptive_time_factor=host.adaptive_time_factor,
                        min_kinematic_slope=host.min_kinematic_slope), host
 

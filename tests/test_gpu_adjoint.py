@@ -298,3 +298,35 @@ def test_recomputed_intermediates_match_the_tape(precision):
         assert np.array_equal(x, y)
     assert np.count_nonzero(results[0][len(STATE)]) > 10   # the path flows' adjoint
     model.close()
+
+
+@pytest.mark.parametrize("kind", ["plain", "all"])
+def test_gauge_loss_function_matches_the_problem_gradient(kind):
+    """GaugeLoss.apply, the torch.autograd.Function the adjoint is delivered
+    through: on float32 parameter tensors in the model's input order (the
+    basin layout, where the device order is another permutation), the loss
+    is problem.run()'s and torch.autograd.grad returns problem.gradient(),
+    as float32 tensors on the parameters' device, scaled by the incoming
+    gradient."""
+    from cmfgpu_amd.adjoint import GaugeLoss
+    net, src, model, problem, gauges = _setup("float32", kind=kind)
+    assert not np.array_equal(model.perm, np.arange(model.N))
+    n0 = torch.tensor(src["river_manning"], dtype=torch.float32, device="cuda:0", requires_grad=True)
+    h0 = torch.tensor(src["river_height"], dtype=torch.float32, device="cuda:0", requires_grad=True)
+    loss = GaugeLoss.apply(n0, h0, problem)
+    assert loss.dtype == torch.float64 and loss.device == n0.device
+    gn, gh = torch.autograd.grad(3.0 * loss, (n0, h0))
+    assert gn.dtype == torch.float32 and gh.dtype == torch.float32 and gn.device == n0.device
+    ref_loss = problem.run()
+    ref_n, ref_h = problem.gradient()
+    assert float(loss) == ref_loss
+    want_n = (3.0 * ref_n).to(torch.float32)
+    want_h = (3.0 * ref_h).to(torch.float32)
+    assert torch.equal(gn, want_n) and torch.equal(gh, want_h)
+    assert float(want_n.abs().max()) > 0 and float(want_h.abs().max()) > 0
+    # the same parameters permuted into device order would give another loss:
+    # the function takes them in input order
+    p = torch.as_tensor(model.perm, device="cuda:0")
+    other = GaugeLoss.apply(n0.detach()[p], h0.detach()[p], problem)
+    assert float(other) != float(loss)
+    model.close()

@@ -94,3 +94,32 @@ def test_outflow_mean_is_the_substep_time_integral():
     assert np.all(model.outflow_mean() == 0.0)       # reset
     model.close()
     ref.close()
+
+
+@pytest.mark.parametrize("trial_layout", ["inner", "major"])
+def test_set_inputs_from_a_device_tensor(trial_layout):
+    """Runoff given as a device tensor is permuted into the device layout on
+    the GPU and lands exactly where the host path puts the same values, for
+    one run and for [T, N] members in both trial layouts."""
+    import torch
+    from cmfgpu_amd import synthetic
+    from cmfgpu_amd.models import CaMaFlood
+    net, src = small_source(paths=0, levee=0.0, reservoirs=0)
+    n = net.num_catchments
+    T = 3
+    r = np.stack([synthetic.Runoff(src["catchment_area"], 2100 + t).next_day() for t in range(T)])
+    kw = dict(opened_modules=("base", "adaptive_time"), device="cuda:0")
+    one = CaMaFlood(src, **kw)
+    many = CaMaFlood(src, num_trials=T, batched={"runoff": r}, trial_layout=trial_layout, **kw)
+    for model, x in ((one, r[1]), (many, r[::-1].copy())):
+        model.set_inputs(x)
+        host = model.t["runoff"].clone()
+        model.t["runoff"].zero_()
+        model.set_inputs(torch.as_tensor(x, device="cuda:0"))
+        assert torch.equal(model.t["runoff"], host)
+        # float64 input: converted on the device
+        model.set_inputs(torch.as_tensor(x, dtype=torch.float64, device="cuda:0"))
+        assert torch.equal(model.t["runoff"], host)
+    assert np.array_equal(many.get("runoff"), r[::-1])
+    with pytest.raises(ValueError, match="shape"):
+        one.set_inputs(torch.zeros(n + 1, device="cuda:0"))

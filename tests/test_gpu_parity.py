@@ -221,3 +221,62 @@ def test_batched_shared_time_step():
     got = model.get("river_outflow")
     for m in range(M):
         assert mismatch(got[m], oracles[m].state["river_outflow"]) is None
+
+
+@pytest.mark.parametrize("what", ["levee_fraction", "flood_depth_table"])
+def test_batched_levee_fraction_derives_per_trial_base(what):
+    """Batching only the levee fraction (or only the floodplain table): each
+    trial's levee base height is derived from its own fraction over its own
+    table, as the levee module derives a single run's (levee.py:140-217), so
+    every member is bit-exact with a standalone oracle run on its own
+    fraction or table.  River storage is raised so that the levees are
+    engaged."""
+    net, src = small_source()
+    rs = np.array(src["river_storage"], dtype=np.float64)
+    lid = np.asarray(src["levee_catchment_idx"])
+    rs[lid] *= np.random.default_rng(5).uniform(1.0, 60.0, lid.size)
+    src["river_storage"] = rs
+    M = 3
+    rng = np.random.default_rng(12)
+    # fractions and tables no larger than the source's keep every base height
+    # below its crown (the base rises with both)
+    if what == "levee_fraction":
+        frac = np.asarray(src["levee_fraction"], np.float32)
+        vals = np.stack([frac * rng.uniform(0.3, 1.0, lid.size) for _ in range(M)]).astype(np.float32)
+    else:
+        base = np.asarray(src["flood_depth_table"], np.float32)
+        vals = np.stack([base * np.float32(0.6 + 0.2 * m) for m in range(M)]).astype(np.float32)
+    model = _model(src, FULL, num_trials=M, batched={what: vals}, per_trial_time_step=True)
+    oracles = [oracle_for(dict(src, **{what: vals[m]}), FULL)[0] for m in range(M)]
+    # the members' base heights differ, and each is its standalone run's
+    bases = model.t["levee_base_height"].cpu().numpy().reshape(M, -1)
+    for m in range(M):
+        assert np.array_equal(bases[m], oracles[m].state["levee_base_height"])
+    assert not np.array_equal(bases[0], bases[1])
+    for step in range(4):
+        model.step_advance()
+        counts = model.t["max_sub_steps"].cpu().numpy()
+        for m in range(M):
+            assert int(oracles[m].advance(1800.0, 1)[0]) == counts[m], (step, m)
+    import torch
+    torch.cuda.synchronize()
+    for name in STATE_FIELDS:
+        got = model.get(name)
+        for m in range(M):
+            msg = mismatch(got[m], oracles[m].state[name])
+            assert msg is None, f"member {m} {name}: {msg}"
+
+
+@pytest.mark.parametrize("mode", ["fused", "ops"])
+def test_fp32_storage_profile_bitexact(mode):
+    """The float32 profile without mixed precision (storages and accumulators
+    in float32 too) against the oracle's (float, float) instantiation, full
+    physics, both schedules."""
+    net, src = small_source()
+    model = _model(src, FULL, mixed_precision=False, mode=mode)
+    assert model.t["river_storage"].dtype == __import__("torch").float32
+    orc, host = oracle_for(src, FULL, mixed_precision=False)
+    assert orc.precision == 2
+    for step in range(4):
+        assert model.step_advance() == int(orc.advance(1800.0, 1)[0])
+    _compare(model, orc, STATE_FIELDS + ("bifurcation_outflow", "global_bifurcation_outflow", "total_storage"))

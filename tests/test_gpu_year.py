@@ -1,0 +1,126 @@
+"""Full physics at 15-arcmin scale over a simulated year, FP32 against FP64.
+
+The workload: 270,000 catchments generated as the 60,000-catchment plain
+network, levees on 5 % of them, 12,000 bifurcation paths with five levels,
+daily lognormal runoff with AR(1) correlation 0.8 in time, 365 days of 48
+outer steps (17,520 outer steps, adaptive CFL sub-steps).
+
+Four runs of the same kernels advance side by side:
+  fp64     (float64, float64): bit-identical to the FP64 CPU oracle
+           (tests/test_gpu_parity.py::test_fp64_profile_bitexact), the reference
+  fp32     (float32 fluxes, float64 storage): the reference's mixed profile
+  precise  the same with water surfaces formed in double
+  fp32sto  (float32, float32): the reference's float32 profile without mixed
+           precision
+each with the water-balance log module open.  Per outer step the eleven FP64
+sums are read back; the year's inflow-error sum (triton/storage.py:305: old
+storage - new storage + net inflow x dt, summed over catchments and
+sub-steps) is the mass the storage update created or destroyed.
+
+Stated bounds (relative L2 over all catchments against fp64; the runs are
+deterministic, measured values on MI355X in brackets):
+  discharge, fp32      1e-4 on days 1, 30 and 365   (3.1e-5, 7.7e-5, 8.5e-5)
+  discharge, precise   1e-5 on days 1, 30 and 365   (4.4e-7, 2.2e-6, 7.7e-6)
+  year mass error |sum inflow error| / year runoff volume:
+      fp64, fp32, precise <= 1e-9                   (1.1e-16, 1.3e-10, 9.1e-11)
+      fp32sto >= 100 x the fp32 run's                (2.4e-6: 18,000 x)
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from cmfgpu_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("river_outflow", "flood_outflow", "river_storage", "flood_storage", "protected_storage", "river_depth",
+          "flood_depth", "flood_fraction", "bifurcation_outflow")
+DISCHARGE_BOUND = {"fp32": {1: 1e-4, 30: 1e-4, 365: 1e-4}, "precise": {1: 1e-5, 30: 1e-5, 365: 1e-5}}
+IN_ERR, INFLOW = 3, 4
+
+
+def _rel(x, y):
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    return float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+
+
+def _say(*a):
+    # progress on the real stderr (pytest captures sys.stderr)
+    print(*a, file=sys.__stderr__, flush=True)
+
+
+def test_fp32_against_fp64_over_a_year():
+    import torch
+    from cmfgpu_amd.models import CaMaFlood
+
+    net = synthetic.build_config(1)
+    src = dict(net.fields)
+    mods = ("base", "adaptive_time", "bifurcation", "levee", "log")
+    runs = {
+        "fp64": dict(precision="float64"),
+        "fp32": dict(precision="float32"),
+        "precise": dict(precision="float32", precise_surfaces=True),
+        "fp32sto": dict(precision="float32", mixed_precision=False),
+    }
+    models = {}
+    for name, kw in runs.items():
+        m = CaMaFlood(src, opened_modules=mods, device="cuda", step_duration=1800.0, log_buffer_size=64, **kw)
+        m.keep_log_sums = True
+        models[name] = m
+    in_err = {k: 0.0 for k in runs}
+    inflow = {k: 0.0 for k in runs}
+    runoff_volume = 0.0
+    count_diff = {k: 0 for k in runs}
+    area = src["catchment_area"]
+    forcing = synthetic.Runoff(area, 2001, rho=0.8)
+    report = {}
+    for day in range(365):
+        r = forcing.next_day()
+        for m in models.values():
+            m.set_inputs(r)
+        runoff_volume += float(r.astype(np.float64).sum()) * 86400.0
+        for _ in range(48):
+            counts = {}
+            for name, m in models.items():
+                counts[name] = m.step_advance()
+                ls = m.last_log_sums
+                in_err[name] += float(ls[IN_ERR].sum()) * 1e9
+                inflow[name] += float(ls[INFLOW].sum()) * 1e9
+            for name in runs:
+                count_diff[name] += int(counts[name] != counts["fp64"])
+        d = day + 1
+        if d in (1, 30, 365):
+            ref = models["fp64"]
+            q_ref = ref.get("river_outflow") + ref.get("flood_outflow")
+            row = {}
+            for name in ("fp32", "precise", "fp32sto"):
+                m = models[name]
+                q = m.get("river_outflow").astype(np.float64) + m.get("flood_outflow")
+                row[name] = {"discharge": _rel(q, q_ref)}
+                for f in FIELDS:
+                    row[name][f] = _rel(m.get(f), ref.get(f))
+            report[d] = row
+            for name in row:
+                _say(f"day {d} {name}: " + ", ".join(f"{k} {v:.2e}" for k, v in row[name].items()))
+            _say(f"day {d}: " + "; ".join(f"{n} discharge {row[n]['discharge']:.2e}" for n in row)
+                 + "; outer steps with another sub-step count: "
+                 + ", ".join(f"{n} {count_diff[n]}" for n in ("fp32", "precise", "fp32sto")))
+        elif d % 60 == 0:
+            _say(f"day {d}")
+    torch.cuda.synchronize()
+    rel_mass = {k: abs(v) / runoff_volume for k, v in in_err.items()}
+    _say("year mass error |sum inflow error| / runoff volume: "
+         + ", ".join(f"{k} {v:.2e}" for k, v in rel_mass.items()))
+    for d, row in report.items():
+        for name in ("fp32", "precise"):
+            got = row[name]["discharge"]
+            bound = DISCHARGE_BOUND[name][d]
+            assert got <= bound, f"day {d} {name}: discharge relative L2 error {got:.3e} > {bound}"
+    for name in ("fp64", "fp32", "precise"):
+        assert rel_mass[name] <= 1e-9, f"{name}: year mass error {rel_mass[name]:.3e}"
+    # FP32 storage cannot hold the balance that FP64 storage holds
+    assert rel_mass["fp32sto"] >= 100.0 * max(rel_mass["fp32"], 1e-16), rel_mass
+    for m in models.values():
+        m.close()

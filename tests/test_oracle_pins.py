@@ -858,3 +858,25 @@ def test_path_levels_rescaled_by_the_end_limiters():
         assert s["global_bifurcation_outflow"][a] == net
         assert s["global_bifurcation_outflow"][b] == -net
     assert s["global_bifurcation_outflow"].sum() == 0.0
+
+
+def test_fp32_storage_profile_rounds_the_storage_update():
+    """Without mixed precision the storages are float32 (the oracle's
+    precision 2): above 2^24 m3 a 1 m3 gain is lost to rounding, which the
+    FP64 storage of the mixed profile keeps; with exactly representable
+    values both profiles give the same storage."""
+    out = {}
+    for prec in (0, 2):
+        for start, q in ((16777216.0, 1.0), (1024.0, 3.0)):
+            # a 40 m deep channel: 2e7 m3 bankfull, all the water in the river
+            s = tiny_state([0], river_storage=np.array([start]), river_inflow=np.array([q]),
+                           river_height=np.array([40.0], np.float32))
+            if prec == 2:
+                for k in ("river_storage", "flood_storage", "protected_storage", "outgoing_storage",
+                          "river_inflow", "flood_inflow"):
+                    s[k] = s[k].astype(np.float32)
+            model(s, precision=prec).kernel("flood_stage", time_step=1.0)
+            out[prec, start] = float(s["river_storage"][0]) + float(s["flood_storage"][0])
+    assert out[0, 16777216.0] == 16777217.0
+    assert out[2, 16777216.0] == 16777216.0
+    assert out[0, 1024.0] == out[2, 1024.0] == 1027.0

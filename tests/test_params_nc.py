@@ -128,3 +128,73 @@ def test_a_file_without_state_starts_cold(src, tmp_path):
     assert not np.any(rs)
     orc.advance(1800.0, 2)
     assert np.all(np.isfinite(orc.state["river_outflow"])) and np.any(orc.state["river_storage"] > 0)
+
+
+def test_map_metadata_fields_keep_their_schema_dimensions(src, tmp_path):
+    """The map-construction metadata of the schema (schema.py:44-73) round-trip
+    on their own dimensions: scalars nx, ny, num_basins; basin_sizes on
+    'basin' (here as long as the catchment list, which must not make it a
+    catchment field); gauge fields on 'gauge'; output_catchment_id on
+    'saved_points'."""
+    n = src["river_width"].shape[0]
+    rng = np.random.default_rng(2)
+    meta = dict(src)
+    meta.update({
+        "nx": np.int64(1440), "ny": np.int64(720), "num_basins": np.int64(n),
+        "basin_sizes": np.ones(n, np.int64),
+        "gauge_catchment_id": src["catchment_id"][:7].copy(), "gauge_station_id": np.arange(7, dtype=np.int64),
+        "gauge_reported_area_km2": rng.uniform(1e3, 1e5, 7).astype(np.float32),
+        "output_catchment_id": src["catchment_id"][::50].copy(),
+    })
+    for version in (2, 5):
+        p = tmp_path / f"meta_v{version}.nc"
+        write_parameters(p, meta, version=version)
+        arrays, dims, _ = read_nc(p)
+        assert dims["basin"] == n and dims["gauge"] == 7 and dims["saved_points"] == meta["output_catchment_id"].size
+        assert arrays["nx"].shape == () and int(arrays["nx"]) == 1440 and int(arrays["num_basins"]) == n
+        got = read_parameters(p)
+        for k in ("basin_sizes", "gauge_catchment_id", "gauge_reported_area_km2", "output_catchment_id",
+                  "river_width"):
+            assert _same(got[k], meta[k]), (version, k)
+        # written back after reading (index fields derived by the reader are left out)
+        write_parameters(tmp_path / f"again_v{version}.nc", got, version=version)
+    with pytest.raises(ParameterFileError, match="dimensions"):
+        write_parameters(tmp_path / "bad.nc", dict(meta, flood_depth_table=np.zeros(n, np.float32)))
+
+
+def test_bool_fields_in_cdf2_are_nc_byte(src, tmp_path):
+    """CDF-1 and CDF-2 have type codes 1-6 only: a bool field is written as
+    NC_BYTE, which scipy's independent reader takes."""
+    from scipy.io import netcdf_file
+    n = src["river_width"].shape[0]
+    flag = np.zeros(n, np.bool_)
+    flag[::3] = True
+    p = tmp_path / "bool_v2.nc"
+    write_parameters(p, dict(src, is_outlet=flag), version=2)
+    with netcdf_file(p, "r", mmap=False) as f:
+        v = f.variables["is_outlet"]
+        assert v.typecode() == "b"
+        assert np.array_equal(v[:].astype(bool), flag)
+    assert np.array_equal(read_parameters(p)["is_outlet"].astype(bool), flag)
+
+
+def _cdf2_with_type(code: int, n: int = 4) -> bytes:
+    import struct
+    name = b"x"
+    h = b"CDF\x02" + struct.pack(">i", 0)
+    h += struct.pack(">ii", 10, 1) + struct.pack(">i", 1) + b"c\0\0\0" + struct.pack(">i", n)
+    h += struct.pack(">ii", 0, 0)                            # no global attributes
+    h += struct.pack(">ii", 11, 1) + struct.pack(">i", 1) + name + b"\0\0\0"
+    h += struct.pack(">ii", 1, 0) + struct.pack(">ii", 0, 0)   # one dimension (id 0), no attributes
+    h += struct.pack(">ii", code, n) + struct.pack(">q", len(h) + 4 + 4 + 8)
+    return h + bytes(n)
+
+
+def test_reader_refuses_cdf5_types_in_cdf1_and_cdf2(tmp_path):
+    p = tmp_path / "ubyte_v2.nc"
+    p.write_bytes(_cdf2_with_type(7))
+    with pytest.raises(ParameterFileError, match="only in CDF-5"):
+        read_nc(p)
+    p.write_bytes(_cdf2_with_type(1))
+    arrays, dims, _ = read_nc(p)
+    assert arrays["x"].dtype == np.int8 and dims == {"c": 4}
