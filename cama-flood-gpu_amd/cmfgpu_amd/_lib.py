@@ -178,6 +178,7 @@ def _declare(lib):
     }
     for op in OP_SYMBOLS.values():
         sig[op] = (ctypes.c_int, [_vp, p(CmfFields), _vp])
+    sig["cmf_hip_flood_stage_log_noalloc"] = (ctypes.c_int, [p(CmfFields), _vp])
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -134,17 +135,13 @@ int upload(void** p, const void* host, size_t bytes) {
 int64_t blocks_for(int64_t count) { return (count + kBlock - 1) / kBlock; }
 
 // --------------------------------------------------------------- view ---
+// The field part of a view: everything a cmf_fields table holds; the
+// network part (upstream ranges, events, scratch) stays null.
 template <class REAL, class STO>
-int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
-    if (!net || !f) return set_error(CMF_ERR_INVALID, "null network or field table");
-    if (f->num_catchments != net->n)
-        return set_error(CMF_ERR_INVALID, "num_catchments %lld does not match the network (%lld)",
-                         (long long)f->num_catchments, (long long)net->n);
-    if (f->num_trials < 1 || f->num_trials > net->T)
-        return set_error(CMF_ERR_INVALID, "num_trials %lld outside [1, %lld] of the network",
-                         (long long)f->num_trials, (long long)net->T);
-    if ((f->flags & CMF_HAS_BIFURCATION) && f->num_bifurcation_paths != net->P)
-        return set_error(CMF_ERR_INVALID, "num_bifurcation_paths does not match the network");
+int view_of_fields(const cmf_fields* f, View<REAL, STO>& v) {
+    if (!f) return set_error(CMF_ERR_INVALID, "null field table");
+    if (f->num_catchments < 0 || f->num_trials < 1)
+        return set_error(CMF_ERR_INVALID, "num_catchments must be >= 0 and num_trials >= 1");
     std::memset(&v, 0, sizeof(v));
     v.N = f->num_catchments;
     v.T = f->num_trials;
@@ -237,6 +234,22 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.is_dam_related = f->is_dam_related;
     v.levee_index = f->levee_index;
     for (int k = 0; k < CMF_LOG_COUNT; ++k) v.log_sums[k] = f->log_sums[k];
+    return CMF_OK;
+}
+
+template <class REAL, class STO>
+int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
+    if (!net || !f) return set_error(CMF_ERR_INVALID, "null network or field table");
+    if (f->num_catchments != net->n)
+        return set_error(CMF_ERR_INVALID, "num_catchments %lld does not match the network (%lld)",
+                         (long long)f->num_catchments, (long long)net->n);
+    if (f->num_trials < 1 || f->num_trials > net->T)
+        return set_error(CMF_ERR_INVALID, "num_trials %lld outside [1, %lld] of the network",
+                         (long long)f->num_trials, (long long)net->T);
+    if ((f->flags & CMF_HAS_BIFURCATION) && f->num_bifurcation_paths != net->P)
+        return set_error(CMF_ERR_INVALID, "num_bifurcation_paths does not match the network");
+    const int rc = view_of_fields<REAL, STO>(f, v);
+    if (rc) return rc;
     v.acc_outflow = net->acc_outflow;
     v.G = net->G;
     v.cgroup = net->cgroup;
@@ -530,6 +543,68 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
             return CMF_OK;
     }
     return set_error(CMF_ERR_INVALID, "unknown op");
+}
+
+// ------------------------------------------- allocation-free variant ----
+// cmf_hip_flood_stage_log_noalloc keeps its FP64 per-block log partials in
+// this file-scope device array instead of a network's scratch: room for the
+// partials of kNoAllocMaxElements catchment elements (kBlock per block, 11
+// sums each), 5.8 MB held on every GPU the library runs on until the process
+// ends.  Host code never passes `scratch` itself to a kernel: on the host the
+// name stands for a host-side symbol, so each device's own address is taken
+// with hipGetSymbolAddress, once per device, and kept in g_scratch_dev.
+constexpr int64_t kNoAllocMaxElements = int64_t(1) << 24;
+constexpr int64_t kNoAllocBlocks = kNoAllocMaxElements / kBlock;
+static __device__ __align__(256) double scratch[kNoAllocBlocks * CMF_LOG_COUNT];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (its address looked up on the first
+// call on that device).
+int scratch_of_current_device(double** out) {
+    int dev = -1;
+    CMF_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(CMF_ERR_UNSUPPORTED, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(CMF_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                             hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return CMF_OK;
+}
+
+// compute_flood_stage_log without a network context and without any
+// allocation: k_stage with its partials in `scratch`, then k_log_final.
+template <class REAL, class STO>
+int run_stage_log_noalloc(const cmf_fields* f, void* stream) {
+    const char* name = "cmf_hip_flood_stage_log_noalloc";
+    View<REAL, STO> v;
+    int rc = view_of_fields<REAL, STO>(f, v);
+    if (rc) return rc;
+    if ((rc = need_base(f, name))) return rc;
+    if (!f->outgoing_storage) return set_error(CMF_ERR_INVALID, "%s: outgoing_storage is NULL", name);
+    if ((f->flags & CMF_HAS_BIFURCATION) && !f->global_bifurcation_outflow)
+        return set_error(CMF_ERR_INVALID, "%s: global_bifurcation_outflow is NULL", name);
+    if ((f->flags & CMF_HAS_LEVEE) && !f->is_levee) return set_error(CMF_ERR_INVALID, "%s: is_levee is NULL", name);
+    cmf_fields g = *f;
+    g.flags |= CMF_HAS_LOG;
+    if ((rc = need_features(&g, name))) return rc;
+    const int64_t NT = v.N * v.T;
+    if (NT > kNoAllocMaxElements)
+        return set_error(CMF_ERR_UNSUPPORTED, "%s: %lld catchment elements exceed the %lld its static scratch holds",
+                         name, (long long)NT, (long long)kNoAllocMaxElements);
+    if ((rc = scratch_of_current_device(&v.partials))) return rc;
+    if (v.L == 10) LAUNCH((k_stage<REAL, STO, true, 10>), NT, stream, v);
+    else LAUNCH((k_stage<REAL, STO, true, 0>), NT, stream, v);
+    if (NT == 0) return CMF_OK;
+    return log_final(v, blocks_for(NT), kLogAll, stream);
 }
 
 int dispatch_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
@@ -1282,6 +1357,14 @@ int cmf_hip_levee_stage(cmf_network* net, const cmf_fields* f, void* s) { return
 int cmf_hip_levee_stage_log(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_LEVEE_LOG, s); }
 int cmf_hip_levee_bifurcation_outflow(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_LEVEE_BIF_OUT, s); }
 int cmf_hip_reservoir_outflow(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_RESERVOIR, s); }
+
+int cmf_hip_flood_stage_log_noalloc(const cmf_fields* f, void* s) {
+    if (!f) return set_error(CMF_ERR_INVALID, "null field table");
+    if (f->precision == CMF_PRECISION_F32_F64) return run_stage_log_noalloc<float, double>(f, s);
+    if (f->precision == CMF_PRECISION_F64_F64) return run_stage_log_noalloc<double, double>(f, s);
+    if (f->precision == CMF_PRECISION_F32_F32) return run_stage_log_noalloc<float, float>(f, s);
+    return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", f->precision);
+}
 
 int cmf_hip_substep(cmf_network* net, const cmf_fields* f, int32_t substep, int32_t store_inflow, void* s) {
     if (!f) return set_error(CMF_ERR_INVALID, "null field table");

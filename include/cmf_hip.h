@@ -333,6 +333,18 @@ int cmf_hip_levee_bifurcation_outflow(cmf_network* net, const cmf_fields* f, voi
 /* compute_reservoir_outflow: cuda/reservoir.cu:96-106, triton/reservoir.py:17-184 */
 int cmf_hip_reservoir_outflow(cmf_network* net, const cmf_fields* f, void* stream);
 
+/* No counterpart in the reference: compute_flood_stage_log (the results of
+ * cmf_hip_flood_stage_log, bit for bit) for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  It takes no network context and calls
+ * no hipMalloc, hipFree or allocating library routine: the FP64 per-block
+ * partial sums live in a static device array of the library (5.8 MB per
+ * GPU, held until the process ends), which limits num_catchments *
+ * num_trials to 2^24 (larger inputs return CMF_ERR_UNSUPPORTED).  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (concurrent
+ * streams, threads) race and must be serialised by the caller. */
+int cmf_hip_flood_stage_log_noalloc(const cmf_fields* f, void* stream);
+
 /* ------------------------------------------------------ fused schedule ---
  * One sub-step of CaMaFlood.step_advance (cmfgpu/models/cama_flood_model.py:
  * 134-162) with the ops fused into the fewest passes the dependencies allow:
