@@ -61,6 +61,8 @@ PRECISION_F32_F64 = 0
 PRECISION_F64_F64 = 1
 PRECISION_F32_F32 = 2
 
+SCHEDULES = {"three_pass": 0, "block_local": 1}
+
 LOG_FIELDS = (
     "total_storage_pre_sum",
     "total_storage_next_sum",
@@ -179,6 +181,9 @@ def _declare(lib):
     for op in OP_SYMBOLS.values():
         sig[op] = (ctypes.c_int, [_vp, p(CmfFields), _vp])
     sig["cmf_hip_flood_stage_log_noalloc"] = (ctypes.c_int, [p(CmfFields), _vp])
+    sig["cmf_network_set_schedule"] = (ctypes.c_int, [_vp, ctypes.c_int32])
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    sig["cmf_network_local_counts"] = (ctypes.c_int, [_vp, i64p, i64p, i64p])
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res

@@ -405,6 +405,26 @@ class CaMaFlood:
         from ..autotune import tune
         return tune(self, **kw)
 
+    def set_schedule(self, name: str) -> None:
+        """Schedule of the fused sub-step, both with the same results:
+        "three_pass" (the default: outflow, limiter and stage passes over the
+        whole network) or "block_local" (one trial, breadth-first layout, no
+        reservoirs): one pass in which each block of 512 catchments completes
+        the limiters and stages it holds whole, then list passes over the
+        rest; measured slower on MI355X, kept as an option."""
+        if name not in _lib.SCHEDULES:
+            raise ValueError(f"schedule must be one of {sorted(_lib.SCHEDULES)}")
+        _lib.check(_lib.lib().cmf_network_set_schedule(self.network_handle, _lib.SCHEDULES[name]),
+                   "cmf_network_set_schedule")
+
+    def local_schedule_counts(self) -> tuple[int, int, int]:
+        """(block size, catchments of the limiter list, of the stage list) of
+        the block-local schedule; block size 0 where it does not apply."""
+        b, l, s = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(_lib.lib().cmf_network_local_counts(self.network_handle, ctypes.byref(b), ctypes.byref(l),
+                                                       ctypes.byref(s)), "cmf_network_local_counts")
+        return int(b.value), int(l.value), int(s.value)
+
     def group_sub_steps(self) -> np.ndarray:
         """Sub-step count of each group in the last outer step (group order = sorted labels)."""
         return self.t["max_sub_steps"].cpu().numpy()

@@ -87,6 +87,16 @@ struct cmf_network {
     std::vector<std::pair<int, hipGraphExec_t>> graphs;
     // output statistics accumulator (cmf_network_set_outflow_accumulator)
     double* acc_outflow = nullptr;
+    // block-local schedule (k_fused_local; one trial, ungrouped, contiguous
+    // ranges): its block size (0: not available), the per-catchment flags and
+    // the catchments left to the list passes; `schedule` 1 uses it
+    int64_t local_block = 0;
+    uint8_t* lflags = nullptr;
+    int32_t* lim_list = nullptr;
+    int64_t lim_count = 0;
+    int32_t* stage_list = nullptr;
+    int64_t stage_count = 0;
+    int schedule = CMF_SCHEDULE_THREE_PASS;
     // adjoint scratch (cmf_hip_substep_adjoint), allocated on first use:
     // 14 arrays of N elements of up to 8 bytes
     void* adj_scratch = nullptr;
@@ -101,7 +111,8 @@ void free_network(cmf_network* net) {
     void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->ev_other, net->ev1, net->pos_term,
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
                     net->path_neg, net->path_net, net->path_raw, net->partials, net->cgroup,
-                    net->blk_rec_id, net->blk_rec_ord, net->gblk_ptr};
+                    net->blk_rec_id, net->blk_rec_ord, net->gblk_ptr, net->lflags, net->lim_list,
+                    net->stage_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (net->host_counts) (void)hipHostFree(net->host_counts);
@@ -133,6 +144,56 @@ int upload(void** p, const void* host, size_t bytes) {
 }
 
 int64_t blocks_for(int64_t count) { return (count + kBlock - 1) / kBlock; }
+
+// Block size of the block-local schedule (k_fused_local).
+constexpr int64_t kLocalBlock = 512;
+
+// Flags of the block-local schedule for blocks of B catchments (contiguous
+// upstream ranges `up_ptr`, bifurcation events `ev_ptr`), and the catchments
+// it leaves to the limiter and stage list passes (ascending).
+//   LL(j): j's upstream range lies in j's block and j is no path end: its
+//          limiter comes out of the block's LDS;
+//   SL(j): LL(j), LL of every upstream catchment, and j is a mouth or its
+//          downstream catchment is in the block with LL: its stage too.
+// The scratch each catchment writes is what the list passes read: its
+// positive term when not LL (the limiter list), its negative term when its
+// downstream catchment is not LL, its unlimited flows when it or its
+// downstream catchment is not SL (the stage list), its limiter when LL and
+// it, its downstream catchment or one of its upstream catchments is not SL.
+void local_tables(int64_t n, const int32_t* dn, const std::vector<int32_t>& up_ptr,
+                  const std::vector<int32_t>& ev_ptr, int64_t B, std::vector<uint8_t>& flags,
+                  std::vector<int32_t>& lim_list, std::vector<int32_t>& stage_list) {
+    std::vector<uint8_t> ll((size_t)n), sl((size_t)n);
+    for (int64_t j = 0; j < n; ++j) {
+        const int64_t b0 = up_ptr[j], b1 = up_ptr[j + 1];
+        const bool in_block = (b0 == b1) || (b0 / B == j / B && (b1 - 1) / B == j / B);
+        ll[j] = in_block && ev_ptr[j] == ev_ptr[j + 1];
+    }
+    for (int64_t j = 0; j < n; ++j) {
+        bool ok = ll[j] && (dn[j] == j || ((int64_t)dn[j] / B == j / B && ll[dn[j]]));
+        for (int64_t u = up_ptr[j]; ok && u < up_ptr[j + 1]; ++u)
+            if (u != j && !ll[u]) ok = false;
+        sl[j] = ok;
+    }
+    flags.assign((size_t)n, 0);
+    for (int64_t j = 0; j < n; ++j) {
+        const bool mouth = dn[j] == j;
+        uint8_t f = 0;
+        if (ll[j]) f |= kLocalLim;
+        if (sl[j]) f |= kLocalStage;
+        if (!ll[j]) f |= kPosG;
+        if (!mouth && !ll[dn[j]]) f |= kNegG;
+        const bool stage_reader = !sl[j] || (!mouth && !sl[dn[j]]);
+        if (stage_reader) f |= kQrawG;
+        bool up_not_sl = false;
+        for (int64_t u = up_ptr[j]; u < up_ptr[j + 1]; ++u)
+            if (u != j && !sl[u]) up_not_sl = true;
+        if (ll[j] && (stage_reader || up_not_sl)) f |= kLimG;
+        flags[j] = f;
+        if (!ll[j]) lim_list.push_back((int32_t)j);
+        if (!sl[j]) stage_list.push_back((int32_t)j);
+    }
+}
 
 // --------------------------------------------------------------- view ---
 // The field part of a view: everything a cmf_fields table holds; the
@@ -619,26 +680,63 @@ int dispatch_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
 // The stage pass k_stage_fused, specialised on the table level count and the
 // fused levee stage.  `nt`: threads of a catchment kernel (N*T, or the group
 // block table).
-template <class REAL, class STO, bool CONTIG, bool LOG>
-int launch_stage(const cmf_network* net, const View<REAL, STO>& v, int64_t nt, bool lev, int substep,
+template <class REAL, class STO, bool CONTIG, bool LOG, class VT>
+int launch_stage(const cmf_network* net, const VT& v, int64_t nt, bool lev, int substep,
                   int store_inflow, void* stream) {
     const int64_t b = block_of(net, CMF_TUNE_STAGE, true);
     if (v.L == 10) {
-        if (lev) LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, 10, true>), nt, b, stream, v, substep, store_inflow);
-        else LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, 10, false>), nt, b, stream, v, substep, store_inflow);
+        if (lev) LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, 10, true, VT>), nt, b, stream, v, substep, store_inflow);
+        else LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, 10, false, VT>), nt, b, stream, v, substep, store_inflow);
     } else {
-        if (lev) LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, 0, true>), nt, b, stream, v, substep, store_inflow);
-        else LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, 0, false>), nt, b, stream, v, substep, store_inflow);
+        if (lev) LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, 0, true, VT>), nt, b, stream, v, substep, store_inflow);
+        else LAUNCH_B((k_stage_fused<REAL, STO, CONTIG, LOG, 0, false, VT>), nt, b, stream, v, substep, store_inflow);
     }
     return CMF_OK;
 }
 
-// One fused sub-step: outflow [+ bifurcation paths], [reservoir], limiters,
-// stage (inflow sums, limited flows, bifurcation nets, storage, flood stage
-// [, levee stage]).
+// The block-local schedule of one sub-step: k_fused_local, the limiter list,
+// the stage list (see k_fused_local).
+template <class REAL, class STO, bool LOG, bool LEV, bool PATHS, bool PLEVEE>
+int local_substep_t(cmf_network* net, const View<REAL, STO>& v, int substep, int store_inflow, void* stream) {
+    const int64_t B = net->local_block;
+    const int64_t pb = PATHS ? (v.P + B - 1) / B : 0;
+    const int64_t nf = (v.N + B - 1) / B;
+    LAUNCH_B((k_fused_local<REAL, STO, true, LOG, 10, LEV, PATHS, PLEVEE, 5>), (pb + nf) * B, B, stream, v, substep,
+             pb, store_inflow, (const uint8_t*)net->lflags);
+    if (net->lim_count > 0)
+        LAUNCH_B((k_limiter_list<REAL, STO, true>), net->lim_count, block_of(net, CMF_TUNE_LIMITER, true), stream, v,
+                 store_inflow, (const int32_t*)net->lim_list, net->lim_count);
+    const int64_t bs = block_of(net, CMF_TUNE_STAGE, true);
+    if (net->stage_count > 0)
+        LAUNCH_B((k_stage_list<REAL, STO, true, LOG, 10, LEV>), net->stage_count, bs, stream, v, substep,
+                 store_inflow, (const int32_t*)net->stage_list, net->stage_count, nf);
+    if (LOG) {
+        const int64_t nblocks = nf + (net->stage_count + bs - 1) / bs;
+        if (nblocks > net->partial_blocks) return set_error(CMF_ERR_INTERNAL, "local sub-step: partial buffer too small");
+        const int rc = log_final(v, nblocks, kLogAll, stream);
+        if (rc) return rc;
+    }
+    return CMF_OK;
+}
+
 template <class REAL, class STO>
-int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int substep,
-                  int store_inflow, void* stream) {
+int local_substep(cmf_network* net, const View<REAL, STO>& v, int substep, int store_inflow, bool lev_fused,
+                  void* stream) {
+    const bool log = v.has(CMF_HAS_LOG), paths = v.has(CMF_HAS_BIFURCATION), levee = v.has(CMF_HAS_LEVEE);
+#define CMF_LOCAL(LOGV, LEVV, PV, PLV) local_substep_t<REAL, STO, LOGV, LEVV, PV, PLV>(net, v, substep, store_inflow, stream)
+#define CMF_LOCAL_P(LOGV)                                                                 \
+    (!paths ? (lev_fused ? CMF_LOCAL(LOGV, true, false, false) : CMF_LOCAL(LOGV, false, false, false)) \
+            : (lev_fused ? CMF_LOCAL(LOGV, true, true, true)                              \
+                         : (levee ? CMF_LOCAL(LOGV, false, true, true) : CMF_LOCAL(LOGV, false, true, false))))
+    return log ? CMF_LOCAL_P(true) : CMF_LOCAL_P(false);
+#undef CMF_LOCAL_P
+#undef CMF_LOCAL
+}
+
+// The three passes of one fused sub-step, on view type VT (PlainView for one
+// plain trial, where the per-trial and per-group indexing folds away).
+template <class REAL, class STO, class VT>
+int three_pass_substep(cmf_network* net, const VT& v, int substep, int store_inflow, void* stream) {
     // catchment kernels: N*T threads, or one block per group table entry
     const bool ordered = net->G > 0 && v.ordered && substep >= 0 && substep < (int)net->sub_blocks.size();
     const int64_t NT = net->G > 0 ? (ordered ? net->sub_blocks[substep] : net->nblk) * kBlock : v.N * v.T;
@@ -653,14 +751,14 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
         const int64_t pb = (v.P * v.T + b_out - 1) / b_out;
         const int64_t cnt = (pb + (NT + b_out - 1) / b_out) * b_out;
         if (levee) {
-            if (v.LB == 5) LAUNCH_B((k_outflow_paths<REAL, STO, true, 5>), cnt, b_out, stream, v, substep, pb, store_inflow);
-            else LAUNCH_B((k_outflow_paths<REAL, STO, true, 0>), cnt, b_out, stream, v, substep, pb, store_inflow);
+            if (v.LB == 5) LAUNCH_B((k_outflow_paths<REAL, STO, true, 5, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
+            else LAUNCH_B((k_outflow_paths<REAL, STO, true, 0, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
         } else {
-            if (v.LB == 5) LAUNCH_B((k_outflow_paths<REAL, STO, false, 5>), cnt, b_out, stream, v, substep, pb, store_inflow);
-            else LAUNCH_B((k_outflow_paths<REAL, STO, false, 0>), cnt, b_out, stream, v, substep, pb, store_inflow);
+            if (v.LB == 5) LAUNCH_B((k_outflow_paths<REAL, STO, false, 5, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
+            else LAUNCH_B((k_outflow_paths<REAL, STO, false, 0, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
         }
     } else {
-        LAUNCH_B((k_outflow<REAL, STO, true>), NT, b_out, stream, v, substep, store_inflow);
+        LAUNCH_B((k_outflow<REAL, STO, true, VT>), NT, b_out, stream, v, substep, store_inflow);
     }
     if (v.has(CMF_HAS_RESERVOIR))
         LAUNCH_B((k_reservoir<REAL, STO, true>), v.R * v.T, block_of(net, CMF_TUNE_RESERVOIR, false), stream, v, substep);
@@ -672,13 +770,13 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
         else LAUNCH_B((k_limiter<REAL, STO, false>), NT, b_lim, stream, v, substep, store_inflow);
     } else {
         const int64_t threads = ((NT + 2 * b_lim - 1) / (2 * b_lim)) * b_lim;
-        if (contig) LAUNCH_B((k_limiter_pair<REAL, STO, true>), threads, b_lim, stream, v, substep, store_inflow);
-        else LAUNCH_B((k_limiter_pair<REAL, STO, false>), threads, b_lim, stream, v, substep, store_inflow);
+        if (contig) LAUNCH_B((k_limiter_pair<REAL, STO, true, VT>), threads, b_lim, stream, v, substep, store_inflow);
+        else LAUNCH_B((k_limiter_pair<REAL, STO, false, VT>), threads, b_lim, stream, v, substep, store_inflow);
     }
-    int rc = log ? (contig ? launch_stage<REAL, STO, true, true>(net, v, NT, lev_fused, substep, store_inflow, stream)
-                           : launch_stage<REAL, STO, false, true>(net, v, NT, lev_fused, substep, store_inflow, stream))
-                 : (contig ? launch_stage<REAL, STO, true, false>(net, v, NT, lev_fused, substep, store_inflow, stream)
-                           : launch_stage<REAL, STO, false, false>(net, v, NT, lev_fused, substep, store_inflow, stream));
+    int rc = log ? (contig ? launch_stage<REAL, STO, true, true, VT>(net, v, NT, lev_fused, substep, store_inflow, stream)
+                           : launch_stage<REAL, STO, false, true, VT>(net, v, NT, lev_fused, substep, store_inflow, stream))
+                 : (contig ? launch_stage<REAL, STO, true, false, VT>(net, v, NT, lev_fused, substep, store_inflow, stream)
+                           : launch_stage<REAL, STO, false, false, VT>(net, v, NT, lev_fused, substep, store_inflow, stream));
     if (rc) return rc;
     if (log) {
         if (net->G > 0) {
@@ -704,8 +802,43 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
             else LAUNCH((k_levee<REAL, STO, false, 0>), v.V * v.T, stream, v, substep);
         }
     }
-    (void)f;
     return CMF_OK;
+}
+
+// One fused sub-step: outflow [+ bifurcation paths], [reservoir], limiters,
+// stage (inflow sums, limited flows, bifurcation nets, storage, flood stage
+// [, levee stage]).
+template <class REAL, class STO>
+int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int substep,
+                  int store_inflow, void* stream) {
+    if (net->local_block > 0 && net->schedule == CMF_SCHEDULE_BLOCK_LOCAL && net->G == 0 && v.T == 1 && v.R == 0 && v.L == 10 &&
+        (v.P == 0 || v.LB == 5) && !v.tape_q_r && !v.has(CMF_PER_TRIAL_DT) && net->contiguous) {
+        const bool levee = v.has(CMF_HAS_LEVEE);
+        const bool lev_fused = levee && v.levee_index != nullptr;
+        int rc = local_substep<REAL, STO>(net, v, substep, store_inflow, lev_fused, stream);
+        if (rc) return rc;
+        if (levee && !lev_fused) {
+            const bool log = v.has(CMF_HAS_LOG);
+            if (log) {
+                LAUNCH((k_levee<REAL, STO, true, 10>), v.V * v.T, stream, v, substep);
+                if (v.V * v.T > 0) {
+                    rc = log_final(v, blocks_for(v.V * v.T), kLogStage, stream);
+                    if (rc) return rc;
+                }
+            } else {
+                LAUNCH((k_levee<REAL, STO, false, 10>), v.V * v.T, stream, v, substep);
+            }
+        }
+        return CMF_OK;
+    }
+    const bool plain = net->G == 0 && v.T == 1 && v.batched == 0 && !v.has(CMF_PER_TRIAL_DT);
+    (void)f;
+    if (plain) {
+        PlainView<REAL, STO> pv;
+        static_cast<View<REAL, STO>&>(pv) = v;
+        return three_pass_substep<REAL, STO, PlainView<REAL, STO>>(net, pv, substep, store_inflow, stream);
+    }
+    return three_pass_substep<REAL, STO, View<REAL, STO>>(net, v, substep, store_inflow, stream);
 }
 
 // Grouped networks: one trial; with the log module the levee stage must run
@@ -1234,6 +1367,11 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
             ev1[i].y = (e - b > 1) ? -(ev_other[b] + 1) : ev_other[b];
         }
     }
+    // block-local schedule tables for blocks of kLocalBlock catchments
+    std::vector<uint8_t> lflags;
+    std::vector<int32_t> lim_list, stage_list;
+    const bool local = contiguous && num_groups == 0 && num_trials == 1;
+    if (local) local_tables(n, downstream_idx, up_ptr, ev_ptr, kLocalBlock, lflags, lim_list, stage_list);
     cmf_network* net = new cmf_network();
     net->n = n;
     net->T = num_trials;
@@ -1244,8 +1382,9 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     const size_t pel = (size_t)std::max<int64_t>(num_paths, 1) * (size_t)num_trials * 8;
     net->G = num_groups;
     net->nblk = (int64_t)rec.size();
-    // partials for the smallest block size a tunable pass may use (64)
-    net->partial_blocks = std::max<int64_t>((n * num_trials + 63) / 64, net->nblk) + 1;
+    // partials for the smallest block size a tunable pass may use (64), and
+    // for the block-local pass ahead of its stage list
+    net->partial_blocks = std::max<int64_t>((n * num_trials + 63) / 64, net->nblk) + (n + kLocalBlock - 1) / kLocalBlock + 2;
     if ((rc = upload((void**)&net->up_ptr, up_ptr.data(), up_ptr.size() * 4)) ||
         (rc = upload((void**)&net->up_idx, up_idx.data(), up_idx.size() * 4)) ||
         (rc = upload((void**)&net->ev_ptr, ev_ptr.data(), ev_ptr.size() * 4)) ||
@@ -1260,6 +1399,17 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
         (rc = dev_alloc((void**)&net->partials, (size_t)net->partial_blocks * CMF_LOG_COUNT * 8))) {
         free_network(net);
         return rc;
+    }
+    if (local) {
+        net->local_block = kLocalBlock;
+        net->lim_count = (int64_t)lim_list.size();
+        net->stage_count = (int64_t)stage_list.size();
+        if ((rc = upload((void**)&net->lflags, lflags.data(), lflags.size())) ||
+            (rc = upload((void**)&net->lim_list, lim_list.data(), lim_list.size() * 4)) ||
+            (rc = upload((void**)&net->stage_list, stage_list.data(), stage_list.size() * 4))) {
+            free_network(net);
+            return rc;
+        }
     }
     if (num_groups > 0 &&
         ((rc = upload((void**)&net->cgroup, cgroup.data(), cgroup.size() * 4)) ||
@@ -1345,6 +1495,26 @@ int cmf_network_destroy(cmf_network* net) {
 }
 
 int cmf_network_is_contiguous(const cmf_network* net) { return net ? net->contiguous : 0; }
+
+int cmf_network_set_schedule(cmf_network* net, int32_t schedule) {
+    if (!net) return set_error(CMF_ERR_INVALID, "null network");
+    if (schedule != CMF_SCHEDULE_THREE_PASS && schedule != CMF_SCHEDULE_BLOCK_LOCAL)
+        return set_error(CMF_ERR_INVALID, "unknown schedule %d", schedule);
+    net->schedule = schedule;
+    // captured graphs hold the launches of the other schedule
+    for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
+    net->graphs.clear();
+    net->graph_fields_set = false;
+    return CMF_OK;
+}
+
+int cmf_network_local_counts(const cmf_network* net, int64_t* block, int64_t* lim_count, int64_t* stage_count) {
+    if (!net || !block || !lim_count || !stage_count) return set_error(CMF_ERR_INVALID, "null argument");
+    *block = net->local_block;
+    *lim_count = net->lim_count;
+    *stage_count = net->stage_count;
+    return CMF_OK;
+}
 
 int cmf_hip_outflow(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_OUTFLOW, s); }
 int cmf_hip_inflow(cmf_network* net, const cmf_fields* f, void* s) { return dispatch_op(net, f, OP_INFLOW, s); }

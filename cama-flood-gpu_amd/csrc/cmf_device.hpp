@@ -252,6 +252,38 @@ struct View {
     }
 };
 
+// The view of a one-trial, ungrouped run without per-trial parameters or
+// time steps: every per-trial and per-group index folded away at compile
+// time (element = catchment, shared parameters, the one time-step slot).
+// Same layout as View; kernels templated on the view type take either.
+template <class REAL, class STO>
+struct PlainView : View<REAL, STO> {
+    __device__ __forceinline__ int64_t el(int64_t, int64_t i) const { return i; }
+    __device__ __forceinline__ int64_t pel(int64_t, int64_t p) const { return p; }
+    __host__ __device__ __forceinline__ bool bat(uint32_t) const { return false; }
+    template <class X>
+    __device__ __forceinline__ X prm(const X* p, uint32_t, int64_t, int64_t i) const { return p[i]; }
+    __device__ __forceinline__ int64_t slot(int64_t, int64_t) const { return 0; }
+    __device__ __forceinline__ bool per_slot() const { return false; }
+    __device__ __forceinline__ REAL dt(int64_t, int64_t) const { return this->time_step[0]; }
+    __device__ __forceinline__ bool active(int64_t, int64_t, int) const { return true; }
+    __device__ __forceinline__ bool active_in_block(int64_t, int64_t, int) const { return true; }
+    __device__ __forceinline__ bool last_substep(int64_t, int64_t, int, int shared_last) const {
+        return shared_last != 0;
+    }
+    __device__ __forceinline__ void split_c(int64_t g, int64_t& t, int64_t& i) const { t = 0; i = g; }
+    __device__ __forceinline__ void split_p(int64_t g, int64_t& t, int64_t& p) const { t = 0; p = g; }
+    __device__ __forceinline__ int64_t partial_slot() const { return (int64_t)blockIdx.x; }
+    __device__ __forceinline__ bool catchment_of(int64_t g, int64_t& t, int64_t& i) const {
+        t = 0;
+        i = g;
+        return g < this->N;
+    }
+    __device__ __forceinline__ bool catchment_in(int64_t blk, int64_t& t, int64_t& i) const {
+        return catchment_of(blk * blockDim.x + threadIdx.x, t, i);
+    }
+};
+
 // CMF_PRECISE_SURFACE in the FP32 profile: water surfaces formed in double
 // (the FP64 profile forms them in double anyway)
 template <class REAL, class STO>

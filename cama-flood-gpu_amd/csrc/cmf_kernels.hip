@@ -68,8 +68,24 @@ __device__ __forceinline__ void store_if_changed(X* p, X old, X val) {
 // of the fused schedule reads, on the catchment's last sub-step only
 // (`store_last`: that flag for one shared count).  `blk`: the block's index
 // in the catchment grid.
-template <class REAL, class STO, bool FUSED>
-__device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int substep, int64_t blk, int store_last) {
+// Block-local schedule (k_fused_local): per catchment flags of a network,
+// for one block size, saying which parts of the sub-step the block holding
+// the catchment completes itself (LL: the storage limiter, SL: the stage) and
+// which scratch values other blocks or the list passes read (the *_G bits).
+enum : uint8_t {
+    kLocalLim = 1, kLocalStage = 2, kNegG = 4, kPosG = 8, kQrawG = 16, kLimG = 32
+};
+
+// The outflow of one catchment as the block-local schedule keeps it.
+template <class REAL, class STO>
+struct OutflowLocal {
+    REAL q_r, q_f, pos, neg;
+    STO rs, fs;
+};
+
+template <class REAL, class STO, bool FUSED, class VT>
+__device__ __forceinline__ void outflow_one(const VT& v, int substep, int64_t blk, int store_last,
+                                            OutflowLocal<REAL, STO>* local = nullptr, uint8_t lflags = 0) {
     int64_t t, i;
     if (!v.catchment_in(blk, t, i)) return;
     if (!v.active_in_block(t, i, substep)) return;
@@ -91,8 +107,9 @@ __device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int subste
     const REAL f_dep = v.flood_depth[e];
     const REAL c_elv = v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, e, i);
     const REAL dn_dist = v.prm(v.dist, CMF_BATCHED_DOWNSTREAM_DISTANCE, e, i);
-    const REAL rs = (REAL)v.river_storage[e];
-    const REAL fs = (REAL)v.flood_storage[e];
+    const STO rs_s = v.river_storage[e], fs_s = v.flood_storage[e];
+    const REAL rs = (REAL)rs_s;
+    const REAL fs = (REAL)fs_s;
     // A stage pass folds protected storage into the total and leaves +0.0 in
     // every catchment without a levee, so after the first sub-step of an
     // outer step only levee catchments hold any.
@@ -202,7 +219,14 @@ __device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int subste
         }
         if (v.tape_ri) v.tape_ri[e] = v.res_inflow[e];
     }
-    if (FUSED) {
+    if (FUSED && local) {
+        // block-local schedule: the scratch only where another block or a
+        // list pass reads it
+        if (lflags & kQrawG) {
+            v.qraw_r[e] = q_r;
+            v.qraw_f[e] = q_f;
+        }
+    } else if (FUSED) {
         v.qraw_r[e] = q_r;
         v.qraw_f[e] = q_f;
     } else {
@@ -221,19 +245,32 @@ __device__ __forceinline__ void outflow_one(const View<REAL, STO>& v, int subste
     if (fin && v.has(CMF_HAS_TOTAL_STORAGE)) v.total_storage[e] = (STO)total_storage;
     const REAL pos = vmax(q_r, (REAL)0.0) + vmax(q_f, (REAL)0.0);
     const REAL neg = vmin(q_r, (REAL)0.0) + vmin(q_f, (REAL)0.0);
-    v.pos_term[e] = pos * time_step;
-    v.neg_term[e] = mouth ? (REAL)0.0 : -neg * time_step;
+    const REAL pos_t = pos * time_step;
+    const REAL neg_t = mouth ? (REAL)0.0 : -neg * time_step;
+    if (FUSED && local) {
+        if (lflags & kPosG) v.pos_term[e] = pos_t;
+        if (lflags & kNegG) v.neg_term[e] = neg_t;
+        local->q_r = q_r;
+        local->q_f = q_f;
+        local->pos = pos_t;
+        local->neg = neg_t;
+        local->rs = rs_s;
+        local->fs = fs_s;
+        return;
+    }
+    v.pos_term[e] = pos_t;
+    v.neg_term[e] = neg_t;
 }
 
-template <class REAL, class STO, bool FUSED>
-__global__ __launch_bounds__(kMaxBlock) void k_outflow(View<REAL, STO> v, int substep, int store_last) {
+template <class REAL, class STO, bool FUSED, class VT = View<REAL, STO>>
+__global__ __launch_bounds__(kMaxBlock) void k_outflow(VT v, int substep, int store_last) {
     outflow_one<REAL, STO, FUSED>(v, substep, xcd_block(), store_last);
 }
 
 // Outgoing total of catchment j from the catchment terms: upstream negative
 // terms in ascending network index, then j's own positive term.
-template <class REAL, class STO, bool CONTIG>
-__device__ __forceinline__ STO outgoing_terms(const View<REAL, STO>& v, int64_t t, int64_t j) {
+template <class REAL, class STO, bool CONTIG, class VT>
+__device__ __forceinline__ STO outgoing_terms(const VT& v, int64_t t, int64_t j) {
     STO acc = (STO)0;
     const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
     for (int32_t k = b; k < en; ++k) {
@@ -333,25 +370,25 @@ __global__ __launch_bounds__(kBlock) void k_res_regather(View<REAL, STO> v) {
 // (outflow.py:185-200, levee.py:565-600); bit-identical to the values that
 // pass stores in the water_surface_elevation, protected_water_surface_elevation
 // and total_storage fields.
-template <class REAL, class STO>
-__device__ __forceinline__ double surface_of_d(const View<REAL, STO>& v, int64_t e, int64_t i) {
+template <class REAL, class STO, class VT>
+__device__ __forceinline__ double surface_of_d(const VT& v, int64_t e, int64_t i) {
     const REAL c_elv = v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, e, i);
     const REAL r_hgt = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, i);
     return (double)v.river_depth[e] + ((double)c_elv - (double)r_hgt);
 }
-template <class REAL, class STO>
-__device__ __forceinline__ double protected_surface_of_d(const View<REAL, STO>& v, int64_t e, int64_t i, double w) {
+template <class REAL, class STO, class VT>
+__device__ __forceinline__ double protected_surface_of_d(const VT& v, int64_t e, int64_t i, double w) {
     if (!(v.has(CMF_HAS_LEVEE) && v.is_levee[i])) return w;
     return vmin((double)v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, e, i) + (double)v.protected_depth[e], w);
 }
-template <class REAL, class STO>
-__device__ __forceinline__ REAL surface_of(const View<REAL, STO>& v, int64_t e, int64_t i) {
+template <class REAL, class STO, class VT>
+__device__ __forceinline__ REAL surface_of(const VT& v, int64_t e, int64_t i) {
     const REAL c_elv = v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, e, i);
     const REAL r_hgt = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, i);
     return v.river_depth[e] + (c_elv - r_hgt);
 }
-template <class REAL, class STO>
-__device__ __forceinline__ REAL protected_surface_of(const View<REAL, STO>& v, int64_t e, int64_t i, REAL wse) {
+template <class REAL, class STO, class VT>
+__device__ __forceinline__ REAL protected_surface_of(const VT& v, int64_t e, int64_t i, REAL wse) {
     if (!(v.has(CMF_HAS_LEVEE) && v.is_levee[i])) return wse;
     return vmin(v.prm(v.elev, CMF_BATCHED_CATCHMENT_ELEVATION, e, i) + v.protected_depth[e], wse);
 }
@@ -368,8 +405,8 @@ __device__ __forceinline__ REAL total_of(const View<REAL, STO>& v, int64_t e) {
 // wrote.  Besides the per-level state and the outgoing terms, writes path_raw:
 // the sum of the limited per-level flows, in level order, as
 // compute_bifurcation_inflow forms it (bifurcation.py:132-137).
-template <class REAL, class STO, bool LEVEE, int LBFIX, bool FROM_STATE>
-__device__ __forceinline__ void bif_path(const View<REAL, STO>& v, int substep, int64_t g) {
+template <class REAL, class STO, bool LEVEE, int LBFIX, bool FROM_STATE, class VT>
+__device__ __forceinline__ void bif_path(const VT& v, int substep, int64_t g) {
     if (g >= v.P * v.T) return;
     int64_t t, p;
     v.split_p(g, t, p);
@@ -386,15 +423,15 @@ __device__ __forceinline__ void bif_path(const View<REAL, STO>& v, int substep, 
     const bool precise = precise_surfaces(v);
     double w0d = 0.0, w1d = 0.0, max_wd = 0.0, max_pwd = 0.0;
     if (precise) {
-        w0d = surface_of_d(v, eci, ci);
-        w1d = surface_of_d(v, edi, di);
+        w0d = surface_of_d<REAL, STO>(v, eci, ci);
+        w1d = surface_of_d<REAL, STO>(v, edi, di);
         max_wd = vmax(w0d, w1d);
-        if (LEVEE) max_pwd = vmax(protected_surface_of_d(v, eci, ci, w0d), protected_surface_of_d(v, edi, di, w1d));
+        if (LEVEE) max_pwd = vmax(protected_surface_of_d<REAL, STO>(v, eci, ci, w0d), protected_surface_of_d<REAL, STO>(v, edi, di, w1d));
     }
     if (FROM_STATE) {
-        w0 = surface_of(v, eci, ci);
-        w1 = surface_of(v, edi, di);
-        if (LEVEE) max_pw = vmax(protected_surface_of(v, eci, ci, w0), protected_surface_of(v, edi, di, w1));
+        w0 = surface_of<REAL, STO>(v, eci, ci);
+        w1 = surface_of<REAL, STO>(v, edi, di);
+        if (LEVEE) max_pw = vmax(protected_surface_of<REAL, STO>(v, eci, ci, w0), protected_surface_of<REAL, STO>(v, edi, di, w1));
         ts0 = total_of(v, eci);
         ts1 = total_of(v, edi);
     } else {
@@ -478,8 +515,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_bif_outflow(View<REAL, STO> v, in
 // fused, with bifurcation: the catchment outflow pass and the path pass in
 // one launch.  Blocks [0, path_blocks) run the paths, first, so that their
 // longer threads overlap the catchment blocks; the rest run the catchments.
-template <class REAL, class STO, bool LEVEE, int LBFIX>
-__global__ __launch_bounds__(kMaxBlock) void k_outflow_paths(View<REAL, STO> v, int substep, int64_t path_blocks,
+template <class REAL, class STO, bool LEVEE, int LBFIX, class VT = View<REAL, STO>>
+__global__ __launch_bounds__(kMaxBlock) void k_outflow_paths(VT v, int substep, int64_t path_blocks,
                                                          int store_last) {
     const int64_t b = blockIdx.x;
     if (b < path_blocks) bif_path<REAL, STO, LEVEE, LBFIX, true>(v, substep, b * blockDim.x + threadIdx.x);
@@ -490,14 +527,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_outflow_paths(View<REAL, STO> v, 
 // bifurcation.py:128-151) in ascending path order, each path net formed from
 // the path's level sum and the limiters of its two ends (l_own: j's); the
 // upstream end also writes the path's limited per-level flows.
-template <class REAL, class STO>
-__device__ __forceinline__ STO bif_nets(const View<REAL, STO>& v, int64_t t, int64_t j, REAL l_own) {
+template <class REAL, class STO, class Src, class VT>
+__device__ __forceinline__ STO bif_nets(const VT& v, int64_t t, int64_t j, REAL l_own, const Src& src) {
     STO acc = (STO)0;
     const int32_t b = v.ev_ptr[j], en = v.ev_ptr[j + 1];
     for (int32_t k = b; k < en; ++k) {
         const int32_t code = v.ev_code[k];
         const int64_t pe = v.pel(t, code >> 1);
-        const REAL l_other = v.lim[v.el(t, v.ev_other[k])];
+        const int64_t other = v.ev_other[k];
+        const REAL l_other = src.lim(v, v.el(t, other), other);
         const bool down_end = (code & 1) != 0;
         const REAL l0 = down_end ? l_other : l_own;   // limiter of the upstream end
         const REAL l1 = down_end ? l_own : l_other;   // of the downstream end
@@ -543,8 +581,8 @@ __device__ __forceinline__ STO bif_event_sum(const View<REAL, STO>& v, int64_t t
 
 // The same from the first-event record `ev` of j (ev.x >= 0), the further
 // events (if any) from the list.
-template <class REAL, class STO>
-__device__ __forceinline__ STO bif_terms_from(const View<REAL, STO>& v, int64_t t, int64_t j, int2 ev, STO acc) {
+template <class REAL, class STO, class VT>
+__device__ __forceinline__ STO bif_terms_from(const VT& v, int64_t t, int64_t j, int2 ev, STO acc) {
     const int64_t pe = v.pel(t, ev.x >> 1);
     acc += (STO)((ev.x & 1) ? v.path_neg[pe] : v.path_pos[pe]);
     if (ev.y < 0) {
@@ -758,9 +796,9 @@ __device__ __forceinline__ bool stage_is_wet(const View<REAL, STO>& v, int64_t e
 // Flood-stage diagnosis of one catchment from its new total storage
 // (storage.py:105-181), with LOG the sums of the stage (non-levee catchments).
 // `ca_pre` >= 0 is the catchment area already read (else read when needed).
-template <class REAL, class STO, int LFIX, bool LOG>
+template <class REAL, class STO, int LFIX, bool LOG, class VT>
 __device__ __forceinline__ StageOut<REAL, STO> stage_diag_p(
-    const View<REAL, STO>& v, int64_t e, int64_t i, STO total_hp, REAL rl, REAL rw, REAL rh, REAL ca_pre,
+    const VT& v, int64_t e, int64_t i, STO total_hp, REAL rl, REAL rw, REAL rh, REAL ca_pre,
     bool non_levee, TableRow<REAL, LFIX>& row, bool& row_loaded, LogPart& lp) {
     StageOut<REAL, STO> o;
     const REAL total = (REAL)total_hp;
@@ -1179,8 +1217,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter(View<REAL, STO> v, int su
 // blockDim.x of a block's 2 * blockDim.x) with their loads interleaved: the
 // pass is bound by dependent round trips (upstream range, then its terms), so
 // two independent chains per thread keep twice the loads in flight.
-template <class REAL, class STO, bool CONTIG>
-__global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(View<REAL, STO> v, int substep, int store_last) {
+template <class REAL, class STO, bool CONTIG, class VT = View<REAL, STO>>
+__global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(VT v, int substep, int store_last) {
     const int64_t NT = v.N * v.T;
     const int64_t g0 = (int64_t)blockIdx.x * (2 * (int64_t)blockDim.x) + threadIdx.x;
     const int64_t g1 = g0 + blockDim.x;
@@ -1250,17 +1288,138 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(View<REAL, STO> v, i
     }
 }
 
-// fused, after k_limiter: limited outflow, inflow sums, bifurcation totals,
-// storage update and flood stage, and with LEV the levee stage of levee
-// catchments, in one pass.
+// Where the stage pass reads the limiters and the unlimited flows: the
+// network scratch (StageGlobal), or the LDS rows of the block-local schedule
+// (StageLds: entries of the block's catchments [base, base + blockDim.x)).
+struct StageGlobal {
+    template <class REAL, class STO>
+    __device__ __forceinline__ REAL lim(const View<REAL, STO>& v, int64_t e, int64_t) const { return v.lim[e]; }
+    template <class REAL, class STO>
+    __device__ __forceinline__ REAL qr(const View<REAL, STO>& v, int64_t e, int64_t) const { return v.qraw_r[e]; }
+    template <class REAL, class STO>
+    __device__ __forceinline__ REAL qf(const View<REAL, STO>& v, int64_t e, int64_t) const { return v.qraw_f[e]; }
+};
+template <class REAL>
+struct StageLds {
+    const REAL *l, *r, *f;
+    int64_t base;
+    template <class STO>
+    __device__ __forceinline__ REAL lim(const View<REAL, STO>&, int64_t, int64_t i) const { return l[i - base]; }
+    template <class STO>
+    __device__ __forceinline__ REAL qr(const View<REAL, STO>&, int64_t, int64_t i) const { return r[i - base]; }
+    template <class STO>
+    __device__ __forceinline__ REAL qf(const View<REAL, STO>&, int64_t, int64_t i) const { return f[i - base]; }
+};
+
+// Stage of one catchment in the fused schedule: limited outflow, inflow sums,
+// bifurcation totals, storage update and flood stage, and with LEV the levee
+// stage of levee catchments.
 //
-// The pass is bound by memory latency: every field of the catchment that
-// does not depend on another load is read up front (one round trip), then
-// the downstream limiter, the upstream ranges, the bifurcation events and,
-// for catchments near bankfull or with a levee, the floodplain row and levee
-// parameters (a second round trip); the arithmetic follows.
-template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
-__global__ __launch_bounds__(kMaxBlock) void k_stage_fused(View<REAL, STO> v, int substep, int store_inflow) {
+// Bound by memory latency: every field of the catchment that does not depend
+// on another load is read up front (one round trip), then the downstream
+// limiter, the upstream ranges, the bifurcation events and, for catchments
+// near bankfull or with a levee, the floodplain row and levee parameters (a
+// second round trip); the arithmetic follows.
+template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV, class Src, class VT>
+__device__ __forceinline__ void stage_fused_one(const VT& v, int64_t t, int64_t j, int substep,
+                                                int store_inflow, LogPart& lp, const Src& src) {
+    const int64_t e = v.el(t, j);
+    const REAL dt = v.dt(t, j);
+    // round trip 1
+    const int64_t d = v.dn[j];
+    const REAL l_own = src.lim(v, e, j);
+    const REAL qr = src.qr(v, e, j), qf = src.qf(v, e, j);
+    const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
+    const STO rs = v.river_storage[e], fs = v.flood_storage[e];
+    // without the levee module every catchment holds +0.0 protected
+    // storage after the first sub-step's stage pass
+    const STO ps = (!v.has(CMF_HAS_LEVEE) && substep > 0) ? (STO)0.0 : v.protected_storage[e];
+    const REAL ro = v.prm(v.runoff, CMF_BATCHED_RUNOFF, e, j);
+    const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, j);
+    const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, j);
+    const REAL rl = v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, j);
+    const int32_t lv = LEV ? v.levee_index[j] : -1;
+    const bool lev_cell = LEV ? (lv >= 0) : (v.has(CMF_HAS_LEVEE) && v.is_levee[j]);
+    const bool res = v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
+    const bool events = v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[j] != v.ev_ptr[j + 1];
+    // round trip 2
+    const int64_t ed = v.el(t, d);
+    const REAL l_dn = src.lim(v, ed, d);
+    TableRow<REAL, LFIX> row;
+    bool row_loaded = false;
+    REAL ca = (REAL)-1;
+    const bool near = lev_cell || (REAL)(rs + fs + ps) > (REAL)0.9 * (rl * rw * rh);
+    if (LFIX > 0 && near) {
+        row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
+        row_loaded = true;
+        ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, j);
+    }
+    REAL crown = (REAL)0, lfrac = (REAL)0, lbase = (REAL)0;
+    if (LEV && lev_cell) {
+        const int64_t le = t * v.V + lv;
+        crown = v.prm(v.levee_crown, CMF_BATCHED_LEVEE_CROWN_HEIGHT, le, lv);
+        lfrac = v.prm(v.levee_frac, CMF_BATCHED_LEVEE_FRACTION, le, lv);
+        lbase = v.prm(v.levee_base, CMF_BATCHED_LEVEE_BASE_HEIGHT, le, lv);
+    }
+    const REAL rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
+    const REAL fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
+    v.river_outflow[e] = rout;
+    v.flood_outflow[e] = fout;
+    if (v.acc_outflow) v.acc_outflow[e] += (double)(rout + fout) * (double)dt;
+    // upstream u has downstream j: its negative flow is limited by j
+    STO ri = (STO)0, fi = (STO)0;
+    STO rt = res ? v.res_inflow[e] : (STO)0;
+    for (int32_t k = b; k < en; ++k) {
+        const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+        if (CONTIG && u == j) continue;
+        const int64_t eu = v.el(t, u);
+        const REAL lu = src.lim(v, eu, u);
+        const REAL ur0 = src.qr(v, eu, u), uf0 = src.qf(v, eu, u);
+        const REAL ur1 = (ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own;
+        const REAL uf1 = (uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own;
+        ri += (STO)ur1;
+        fi += (STO)uf1;
+        if (res) rt += (STO)(ur1 + uf1);
+    }
+    if (res) v.res_inflow[e] = rt;
+    STO gsum = (STO)0;
+    REAL gb = (REAL)0.0;
+    if (v.has(CMF_HAS_BIFURCATION)) {
+        if (events) gsum = bif_nets<REAL, STO>(v, t, j, l_own, src);
+        gb = (REAL)gsum;
+    }
+    const bool last = v.last_substep(t, j, substep, store_inflow);
+    if (last) {
+        v.river_inflow[e] = ri;
+        v.flood_inflow[e] = fi;
+        if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = gsum;
+    }
+    const StagePre<REAL, STO> pre = stage_update_p<REAL, STO, LOG>(rs, fs, ps, ro, (REAL)ri, (REAL)fi, rout, fout,
+                                                                   gb, dt, lp);
+    const StageOut<REAL, STO> so = stage_diag_p<REAL, STO, LFIX, LOG>(v, e, j, pre.total_hp, rl, rw, rh, ca,
+                                                                     !lev_cell, row, row_loaded, lp);
+    if (LEV && lev_cell) {
+        if (!row_loaded) row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
+        const REAL cav = ca >= (REAL)0 ? ca : v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, j);
+        const LeveeOut<REAL, STO> lo = levee_one_p<REAL, STO, LFIX>(
+            v, rl, rw, rh, cav, crown, lfrac, lbase, so.rsf_hp, so.fsf_hp, so.fdep, so.rdep, so.ffr, row);
+        if (LOG) levee_log(lo, so.rsf_hp + so.fsf_hp, cav, lp);
+        v.river_storage[e] = lo.r_hp;
+        v.flood_storage[e] = lo.f_hp;
+        v.protected_storage[e] = lo.p_hp;
+        v.river_depth[e] = lo.r_dph;
+        v.flood_depth[e] = lo.f_dph;
+        v.protected_depth[e] = lo.p_dph;
+        v.flood_fraction[e] = lo.f_frc;
+    } else {
+        // a separate levee pass (k_levee) reads the flood fraction
+        store_state(v, e, so, pre.fs_old, pre.ps_old, last || (v.has(CMF_HAS_LEVEE) && !LEV));
+    }
+}
+
+// fused, after k_limiter: the stage of every catchment from the scratch.
+template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV, class VT = View<REAL, STO>>
+__global__ __launch_bounds__(kMaxBlock) void k_stage_fused(VT v, int substep, int store_inflow) {
     const int64_t g = xcd_block() * blockDim.x + threadIdx.x;
     LogPart lp;
     if (LOG) {
@@ -1268,100 +1427,113 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_fused(View<REAL, STO> v, in
         for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
     }
     int64_t t, j;
-    if (v.catchment_of(g, t, j) && v.active_in_block(t, j, substep)) {
-        const int64_t e = v.el(t, j);
-        const REAL dt = v.dt(t, j);
-        // round trip 1
-        const int64_t d = v.dn[j];
-        const REAL l_own = v.lim[e];
-        const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
-        const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
-        const STO rs = v.river_storage[e], fs = v.flood_storage[e];
-        // without the levee module every catchment holds +0.0 protected
-        // storage after the first sub-step's stage pass
-        const STO ps = (!v.has(CMF_HAS_LEVEE) && substep > 0) ? (STO)0.0 : v.protected_storage[e];
-        const REAL ro = v.prm(v.runoff, CMF_BATCHED_RUNOFF, e, j);
-        const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, j);
-        const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, j);
-        const REAL rl = v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, j);
-        const int32_t lv = LEV ? v.levee_index[j] : -1;
-        const bool lev_cell = LEV ? (lv >= 0) : (v.has(CMF_HAS_LEVEE) && v.is_levee[j]);
-        const bool res = v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
-        const bool events = v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[j] != v.ev_ptr[j + 1];
-        // round trip 2
-        const REAL l_dn = v.lim[v.el(t, d)];
-        TableRow<REAL, LFIX> row;
-        bool row_loaded = false;
-        REAL ca = (REAL)-1;
-        const bool near = lev_cell || (REAL)(rs + fs + ps) > (REAL)0.9 * (rl * rw * rh);
-        if (LFIX > 0 && near) {
-            row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
-            row_loaded = true;
-            ca = v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, j);
-        }
-        REAL crown = (REAL)0, lfrac = (REAL)0, lbase = (REAL)0;
-        if (LEV && lev_cell) {
-            const int64_t le = t * v.V + lv;
-            crown = v.prm(v.levee_crown, CMF_BATCHED_LEVEE_CROWN_HEIGHT, le, lv);
-            lfrac = v.prm(v.levee_frac, CMF_BATCHED_LEVEE_FRACTION, le, lv);
-            lbase = v.prm(v.levee_base, CMF_BATCHED_LEVEE_BASE_HEIGHT, le, lv);
-        }
-        const REAL rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
-        const REAL fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
-        v.river_outflow[e] = rout;
-        v.flood_outflow[e] = fout;
-        if (v.acc_outflow) v.acc_outflow[e] += (double)(rout + fout) * (double)dt;
-        // upstream u has downstream j: its negative flow is limited by j
-        STO ri = (STO)0, fi = (STO)0;
-        STO rt = res ? v.res_inflow[e] : (STO)0;
-        for (int32_t k = b; k < en; ++k) {
-            const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
-            if (CONTIG && u == j) continue;
-            const int64_t eu = v.el(t, u);
-            const REAL lu = v.lim[eu];
-            const REAL ur0 = v.qraw_r[eu], uf0 = v.qraw_f[eu];
-            const REAL ur1 = (ur0 >= (REAL)0.0) ? ur0 * lu : ur0 * l_own;
-            const REAL uf1 = (uf0 >= (REAL)0.0) ? uf0 * lu : uf0 * l_own;
-            ri += (STO)ur1;
-            fi += (STO)uf1;
-            if (res) rt += (STO)(ur1 + uf1);
-        }
-        if (res) v.res_inflow[e] = rt;
-        STO gsum = (STO)0;
-        REAL gb = (REAL)0.0;
-        if (v.has(CMF_HAS_BIFURCATION)) {
-            if (events) gsum = bif_nets<REAL, STO>(v, t, j, l_own);
-            gb = (REAL)gsum;
-        }
-        const bool last = v.last_substep(t, j, substep, store_inflow);
-        if (last) {
-            v.river_inflow[e] = ri;
-            v.flood_inflow[e] = fi;
-            if (v.has(CMF_HAS_BIFURCATION)) v.gbif[e] = gsum;
-        }
-        const StagePre<REAL, STO> pre = stage_update_p<REAL, STO, LOG>(rs, fs, ps, ro, (REAL)ri, (REAL)fi, rout, fout,
-                                                                       gb, dt, lp);
-        const StageOut<REAL, STO> so = stage_diag_p<REAL, STO, LFIX, LOG>(v, e, j, pre.total_hp, rl, rw, rh, ca,
-                                                                         !lev_cell, row, row_loaded, lp);
-        if (LEV && lev_cell) {
-            if (!row_loaded) row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
-            const REAL cav = ca >= (REAL)0 ? ca : v.prm(v.catchment_area, CMF_BATCHED_CATCHMENT_AREA, e, j);
-            const LeveeOut<REAL, STO> lo = levee_one_p<REAL, STO, LFIX>(
-                v, rl, rw, rh, cav, crown, lfrac, lbase, so.rsf_hp, so.fsf_hp, so.fdep, so.rdep, so.ffr, row);
-            if (LOG) levee_log(lo, so.rsf_hp + so.fsf_hp, cav, lp);
-            v.river_storage[e] = lo.r_hp;
-            v.flood_storage[e] = lo.f_hp;
-            v.protected_storage[e] = lo.p_hp;
-            v.river_depth[e] = lo.r_dph;
-            v.flood_depth[e] = lo.f_dph;
-            v.protected_depth[e] = lo.p_dph;
-            v.flood_fraction[e] = lo.f_frc;
-        } else {
-            // a separate levee pass (k_levee) reads the flood fraction
-            store_state(v, e, so, pre.fs_old, pre.ps_old, last || (v.has(CMF_HAS_LEVEE) && !LEV));
-        }
-    }
+    if (v.catchment_of(g, t, j) && v.active_in_block(t, j, substep))
+        stage_fused_one<REAL, STO, CONTIG, LOG, LFIX, LEV>(v, t, j, substep, store_inflow, lp, StageGlobal{});
     if (LOG) block_log_partials(lp, v.partials + v.partial_slot() * CMF_LOG_COUNT);
+}
+
+// ------------------------------------------------ block-local schedule ---
+// One trial, ungrouped, contiguous upstream ranges, no reservoirs.  A sub-step
+// is k_fused_local over the whole network, then k_limiter_list over the
+// catchments whose limiter it left (list `lim_list`), then k_stage_list over
+// those whose stage it left (`stage_list`).  Block k of the catchment grid
+// holds catchments [k B, (k + 1) B) (B = blockDim.x):
+//   1. the outflow of every catchment (and, in leading blocks of their own,
+//      the bifurcation paths, as k_outflow_paths); flows and negative
+//      outgoing terms into LDS;
+//   2. after a barrier, the storage limiter of each catchment whose upstream
+//      range lies in the block and that is no path end (kLocalLim): the same
+//      ordered sum as k_limiter, from LDS;
+//   3. after a barrier, the stage of each catchment whose limiter, upstream
+//      catchments' limiters and downstream limiter all came out of step 2
+//      (kLocalStage).  No other block reads such a catchment's state in this
+//      launch (its upstream catchments are in the block, it is no path end),
+//      so its new state can be written here.
+// Every value is the one the three-pass schedule computes, by the same
+// operations in the same order.
+template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV, bool PATHS, bool PLEVEE, int LBFIX>
+__global__ __launch_bounds__(kMaxBlock) void k_fused_local(View<REAL, STO> v, int substep, int64_t path_blocks,
+                                                         int store_last, const uint8_t* __restrict__ lflags) {
+    const int64_t b = blockIdx.x;
+    if (PATHS && b < path_blocks) {
+        bif_path<REAL, STO, PLEVEE, LBFIX, true>(v, substep, b * blockDim.x + threadIdx.x);
+        return;
+    }
+    __shared__ REAL sh_qr[kMaxBlock], sh_qf[kMaxBlock], sh_neg[kMaxBlock], sh_lim[kMaxBlock];
+    const int64_t blk = PATHS ? b - path_blocks : b;
+    const int64_t base = blk * (int64_t)blockDim.x;
+    const int64_t j = base + threadIdx.x;
+    const bool valid = j < v.N;
+    const uint8_t fl = valid ? lflags[j] : (uint8_t)0;
+    OutflowLocal<REAL, STO> o;
+    o.q_r = o.q_f = o.pos = o.neg = (REAL)0;
+    o.rs = o.fs = (STO)0;
+    if (valid) outflow_one<REAL, STO, true>(v, substep, blk, store_last, &o, fl);
+    sh_qr[threadIdx.x] = o.q_r;
+    sh_qf[threadIdx.x] = o.q_f;
+    sh_neg[threadIdx.x] = o.neg;
+    __syncthreads();
+    if (fl & kLocalLim) {
+        // outgoing_terms (upstream negative terms in ascending index, then
+        // the own positive term), then limit_of
+        STO acc = (STO)0;
+        const int32_t b0 = v.up_ptr[j], b1 = v.up_ptr[j + 1];
+        for (int32_t k = b0; k < b1; ++k) {
+            if (k == j) continue;   // a mouth's range ends with the mouth
+            acc += (STO)sh_neg[k - base];
+        }
+        acc += (STO)o.pos;
+        const REAL og = (REAL)acc;
+        const REAL rate = (REAL)(o.rs + o.fs);
+        const REAL l = (og > (REAL)1e-8) ? vmin(rate / og, (REAL)1.0) : (REAL)1.0;
+        sh_lim[threadIdx.x] = l;
+        if (fl & kLimG) v.lim[j] = l;
+        if (v.has(CMF_HAS_BIFURCATION) && store_last) v.limit_rate[j] = l;
+    }
+    __syncthreads();
+    LogPart lp;
+    if (LOG) {
+#pragma unroll
+        for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
+    }
+    if (fl & kLocalStage)
+        stage_fused_one<REAL, STO, CONTIG, LOG, LFIX, LEV>(v, 0, j, substep, store_last, lp,
+                                                           StageLds<REAL>{sh_lim, sh_qr, sh_qf, base});
+    if (LOG) block_log_partials(lp, v.partials + blk * CMF_LOG_COUNT);
+}
+
+// The storage limiter of the listed catchments (one trial, ungrouped), from
+// the terms in the network scratch: k_limiter's arithmetic.
+template <class REAL, class STO, bool CONTIG>
+__global__ __launch_bounds__(kMaxBlock) void k_limiter_list(View<REAL, STO> v, int store_last,
+                                                          const int32_t* __restrict__ list, int64_t count) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= count) return;
+    const int64_t j = list[g];
+    const int2 ev = v.has(CMF_HAS_BIFURCATION) ? v.ev1[j] : make_int2(-1, 0);
+    STO og = outgoing_terms<REAL, STO, CONTIG>(v, 0, j);
+    if (ev.x >= 0) og = bif_terms_from<REAL, STO>(v, 0, j, ev, og);
+    const REAL l = limit_of(v, og, j);
+    v.lim[j] = l;
+    if (v.has(CMF_HAS_BIFURCATION) && store_last) v.limit_rate[j] = l;
+}
+
+// The stage of the listed catchments from the scratch; log partials in slots
+// [slot0, slot0 + gridDim.x).
+template <class REAL, class STO, bool CONTIG, bool LOG, int LFIX, bool LEV>
+__global__ __launch_bounds__(kMaxBlock) void k_stage_list(View<REAL, STO> v, int substep, int store_inflow,
+                                                        const int32_t* __restrict__ list, int64_t count,
+                                                        int64_t slot0) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    LogPart lp;
+    if (LOG) {
+#pragma unroll
+        for (int k = 0; k < CMF_LOG_COUNT; ++k) lp.q[k] = 0.0;
+    }
+    if (g < count)
+        stage_fused_one<REAL, STO, CONTIG, LOG, LFIX, LEV>(v, 0, (int64_t)list[g], substep, store_inflow, lp,
+                                                           StageGlobal{});
+    if (LOG) block_log_partials(lp, v.partials + (slot0 + blockIdx.x) * CMF_LOG_COUNT);
 }
 
 // Fold the per-block partials into log slot current_step: one block, each

@@ -302,6 +302,21 @@ int cmf_network_destroy(cmf_network* net);
 /* 1 when every upstream range is the identity (breadth-first layout). */
 int cmf_network_is_contiguous(const cmf_network* net);
 
+/* Schedule of the fused sub-step (both give the same bits; no counterpart in
+ * the reference).  THREE_PASS (the default): outflow (+ paths), limiter and
+ * stage passes, each over the whole network.  BLOCK_LOCAL (one trial,
+ * ungrouped, contiguous upstream ranges, no reservoirs, 10 flood levels, 5
+ * path levels, not taped; otherwise the three passes run): one pass in which
+ * each block of 512 catchments completes the limiters and stages it holds
+ * whole, then limiter and stage passes over the catchments left (lists fixed
+ * at network creation).  Measured slower on MI355X than THREE_PASS; kept as
+ * an option. */
+enum { CMF_SCHEDULE_THREE_PASS = 0, CMF_SCHEDULE_BLOCK_LOCAL = 1 };
+int cmf_network_set_schedule(cmf_network* net, int32_t schedule);
+/* The block-local schedule's block size (0: not available for this network)
+ * and the sizes of its limiter and stage lists. */
+int cmf_network_local_counts(const cmf_network* net, int64_t* block, int64_t* lim_count, int64_t* stage_count);
+
 /* ------------------------------------------------- per-op entry points ---
  * Each replaces one reference launcher (the .cu files of cmfgpu/phys/cuda, routed at
  * cmfgpu/phys/cuda/__init__.py:68-155) and one Triton kernel; the fields read

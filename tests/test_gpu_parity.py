@@ -280,3 +280,68 @@ def test_fp32_storage_profile_bitexact(mode):
     for step in range(4):
         assert model.step_advance() == int(orc.advance(1800.0, 1)[0])
     _compare(model, orc, STATE_FIELDS + ("bifurcation_outflow", "global_bifurcation_outflow", "total_storage"))
+
+
+NO_RES = ("base", "adaptive_time", "bifurcation", "levee")
+
+
+@pytest.mark.parametrize("case", ["plain", "paths_levees", "log", "precise", "fp64", "fp32sto", "network_levee",
+                                  "reservoirs"])
+def test_schedules_bitexact(case):
+    """The block-local schedule (one pass completing whole blocks, then the
+    limiter and stage lists) and the three-pass schedule against the oracle,
+    bit for bit; levee catchments started at 1-60 times their river storage so
+    that the levee cases and floodplains are in play."""
+    import torch
+    kw, okw = {}, {}
+    mods = NO_RES
+    net, src = small_source(n=6000, seed=21, reservoirs=40 if case == "reservoirs" else 0, paths=150,
+                            levee=0.08)
+    if case == "plain":
+        mods = ("base", "adaptive_time")
+    if case == "log":
+        mods = NO_RES + ("log",)
+        kw = okw = dict(log_buffer_size=64)
+    if case == "precise":
+        kw = okw = dict(precise_surfaces=True)
+    if case == "fp64":
+        kw = okw = dict(precision="float64")
+    if case == "fp32sto":
+        kw = okw = dict(mixed_precision=False)
+    if case == "reservoirs":
+        mods = FULL
+    rs = np.array(src["river_storage"], dtype=np.float64)
+    lid = np.asarray(src["levee_catchment_idx"])
+    rs[lid] *= np.random.default_rng(3).uniform(1.0, 60.0, lid.size)
+    src["river_storage"] = rs
+    models = {name: _model(src, mods, **kw) for name in ("block_local", "three_pass")}
+    for name, m in models.items():
+        m.set_schedule(name)
+    if case == "network_levee":
+        # a separate levee pass after the stage (no levee_index)
+        for m in models.values():
+            m.t.pop("levee_index")
+            m._table = m._build_table()
+    B, nl, ns = models["block_local"].local_schedule_counts()
+    assert B == 512 and 0 < nl < ns < models["block_local"].N
+    orc, _ = oracle_for(src, mods, **okw)
+    for m in models.values():
+        m.keep_log_sums = True
+    for step in range(5):
+        counts = {name: m.step_advance() for name, m in models.items()}
+        assert set(counts.values()) == {int(orc.advance(1800.0, 1)[0])}, counts
+        if case == "log":
+            # FP64 sums folded in another block order: relative 1e-12
+            c = models["three_pass"].last_log_sums
+            for name in ("block_local",):
+                a = models[name].last_log_sums
+                assert np.all(np.abs(a - c) <= 1e-12 * np.maximum(np.abs(c), 1e-9) + 1e-15), (name, step)
+    torch.cuda.synchronize()
+    fields = STATE_FIELDS + (("bifurcation_outflow", "bifurcation_cross_section_depth", "global_bifurcation_outflow",
+                              "limit_rate") if case != "plain" else ())
+    for name, m in models.items():
+        for f in fields:
+            if f not in orc.state:
+                continue
+            msg = mismatch(m.get(f), orc.state[f])
+            assert msg is None, f"{name} {f}: {msg}"
