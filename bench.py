@@ -74,21 +74,23 @@ def algorithmic_bytes(kernel: str, model, f_wet: float) -> float:
     return 0.0
 
 
-def build_model(net, src, members: int, device: str, layout: str):
+def build_model(net, src, members: int, device: str, layout: str, precise: bool = False,
+                precision: str = "float32"):
     from cmfgpu_amd.models import CaMaFlood
     mods = ["base", "adaptive_time"]
     if net.num_paths:
         mods.append("bifurcation")
     if net.num_levees:
         mods.append("levee")
+    kw = dict(opened_modules=tuple(mods), device=device, step_duration=1800.0, layout=layout,
+              precise_surfaces=precise, precision=precision)
     if members <= 1:
-        return CaMaFlood(src, opened_modules=tuple(mods), device=device, step_duration=1800.0, layout=layout)
+        return CaMaFlood(src, **kw)
     import numpy as np
     from cmfgpu_amd import synthetic
     runoff = np.stack([synthetic.Runoff(src["catchment_area"], 4000 + m).next_day() for m in range(members)])
-    return CaMaFlood(src, opened_modules=tuple(mods), device=device, step_duration=1800.0,
-                     num_trials=members, batched={"runoff": runoff}, per_trial_time_step=True,
-                     trial_layout="inner", layout=layout)
+    return CaMaFlood(src, num_trials=members, batched={"runoff": runoff}, per_trial_time_step=True,
+                     trial_layout="inner", **kw)
 
 
 ERROR_FIELDS = ("river_outflow", "flood_outflow", "river_storage", "flood_storage", "river_depth",
@@ -153,6 +155,83 @@ def fp64_errors(net, src, device: str, layout: str, days: int) -> dict:
     out = res[0]
     out["precise_surfaces"] = dict(res[1], catchment_steps_per_s=rate[2][0] / rate[2][1],
                                    vs_reference_surfaces=round((rate[2][0] / rate[2][1]) / (rate[0][0] / rate[0][1]), 4))
+    return out
+
+
+def _rel_rows(x, y):
+    """Relative L2 error of each member row of x against y."""
+    import numpy as np
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    return [float(f"{np.linalg.norm(x[t] - y[t]) / max(np.linalg.norm(y[t]), 1e-300):.4g}") for t in range(x.shape[0])]
+
+
+def batched_line(args, net, src, device: str, plain_value: float, plain_ref_value: float, tuned) -> dict:
+    """64 members of the same network (runoff seeds 4000 + m), each with its
+    own CFL count, trial-inner, with precise surfaces (the headline profile)
+    and with the reference's FP32 surfaces: one simulated day of warm-up
+    (48 outer steps), then `--batched-steps` timed outer steps each.  Then the
+    per-member error of one simulated day against the FP64 profile of the same
+    batch, from the same initial state, for both surface modes."""
+    import numpy as np
+    import torch
+    M = args.members
+    out = {"members": M, "unit": "member-catchment-steps/s"}
+    bsteps = args.batched_steps
+    for precise in (True, False):
+        bm = build_model(net, src, M, device, args.layout, precise=precise)
+        if args.autotune:
+            bm.autotune(use_cache=False)
+        run_steps(bm, 48, per_trial=True)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        bwork = run_steps(bm, bsteps, per_trial=True)
+        torch.cuda.synchronize()
+        belapsed = time.perf_counter() - t2
+        bvalue = bwork / belapsed
+        bf_wet = float((bm.t["flood_depth"] > 0).float().mean().item())
+        bgbs = batched_full_bytes(bm.N, M, bm.P, bf_wet) * (bwork / (bm.N * M)) / belapsed / 1e9
+        cgbs = fused_batched_bytes(bm.N, M, bm.V, bm.P, bf_wet) * (bwork / (bm.N * M)) / belapsed / 1e9
+        base = plain_value if precise else plain_ref_value
+        rec = {"value": bvalue, "steps": bsteps, "warmup": "one simulated day (48 outer steps)",
+               "vs_plain": round(bvalue / base, 3), "GBps_algorithmic": round(bgbs, 1),
+               "frac_of_peak": round(bgbs / HBM_PEAK_GBS, 4), "GBps_compulsory_fused": round(cgbs, 1),
+               "frac_compulsory_fused": round(cgbs / HBM_PEAK_GBS, 4)}
+        if precise:
+            out.update(rec)
+        else:
+            out["reference_surfaces"] = rec
+        log(f"batched ({'precise' if precise else 'reference'} surfaces): {M} members, {bsteps} outer steps, "
+            f"{belapsed:.3f} s -> {bvalue:.4g} member-catchment-steps/s ({bvalue / base:.2f}x plain)")
+        bm.close()
+        del bm
+        torch.cuda.empty_cache()
+    if args.error_days > 0:
+        models = {"fp64": build_model(net, src, M, device, args.layout, precision="float64"),
+                  "precise": build_model(net, src, M, device, args.layout, precise=True),
+                  "reference": build_model(net, src, M, device, args.layout, precise=False)}
+        mismatch = {"precise": 0, "reference": 0}
+        for _ in range(48 * args.error_days):
+            n64 = None
+            for name, m in models.items():
+                m.step_advance()
+                c = m.t["max_sub_steps"].cpu().numpy()
+                if name == "fp64":
+                    n64 = c
+                else:
+                    mismatch[name] += int((c != n64).sum())
+        q = {k: m.get("river_outflow").astype(np.float64) + m.get("flood_outflow") for k, m in models.items()}
+        for k in ("precise", "reference"):
+            rows = _rel_rows(q[k], q["fp64"])
+            rec = out if k == "precise" else out["reference_surfaces"]
+            rec["discharge_rel_l2_per_member"] = rows
+            rec["max_discharge_rel_l2"] = max(rows)
+            rec["member_substep_count_mismatches"] = mismatch[k]
+        out["error_days"] = args.error_days
+        log(f"batched errors after {args.error_days} day(s): discharge up to {out['max_discharge_rel_l2']:.3g} "
+            f"(precise), {out['reference_surfaces']['max_discharge_rel_l2']:.3g} (reference surfaces)")
+        for m in models.values():
+            m.close()
     return out
 
 
@@ -323,29 +402,49 @@ def batched_full_line(args, device: str) -> dict:
         # surfaces
         ref = build("float64")
         pr = build("float32", precise=True)
+        # the FP64 profile again, from the initial river storage scaled by
+        # 1 +- 1e-7 (one FP32 rounding): the scheme's own sensitivity
+        pert = build("float64")
+        sign = torch.from_numpy(np.random.default_rng(1).choice([-1.0, 1.0], pert.t["river_storage"].numel())).to(
+            pert.t["river_storage"].device).reshape(pert.t["river_storage"].shape)
+        pert.t["river_storage"].mul_(1.0 + 1e-7 * sign)
+        del sign
         for day in range(args.full_days):
             if day:
                 r = np.stack([f.next_day() for f in forcing])
-                for m in (bm, ref, pr):
+                for m in (bm, ref, pr, pert):
                     m.set_inputs(r)
             for _ in range(48):
                 bm.step_advance()
                 ref.step_advance()
                 pr.step_advance()
+                pert.step_advance()
         errors = {}
         for name in ("river_storage", "flood_storage", "flood_depth"):
             x, y = bm.get(name).astype(np.float64), ref.get(name).astype(np.float64)
             errors[name] = [float(f"{np.linalg.norm(x[t] - y[t]) / max(np.linalg.norm(y[t]), 1e-300):.4g}")
                             for t in range(M)]
-        q32, q64, qpr = ((m.get("river_outflow").astype(np.float64) + m.get("flood_outflow")) for m in (bm, ref, pr))
+        q32, q64, qpr, qpt = ((m.get("river_outflow").astype(np.float64) + m.get("flood_outflow"))
+                              for m in (bm, ref, pr, pert))
         ref.close()
         pr.close()
+        pert.close()
         errors["discharge"] = [float(f"{np.linalg.norm(q32[t] - q64[t]) / np.linalg.norm(q64[t]):.4g}")
                                for t in range(M)]
         # the same outside the reaches below reservoirs: a release switches
         # regime at storage and inflow thresholds, and a regime that flips
         # between the two profiles changes the flow of the whole reach below
-        keep = ~_reaches_below(src, src["reservoir_catchment_idx"])
+        reach = _reaches_below(src, src["reservoir_catchment_idx"])
+        keep = ~reach
+        # and without the catchments coupled to a reach within a sub-step:
+        # those draining into it (their slope and backflow follow the reach's
+        # water surface) and the other ends of paths touching it
+        dn = np.asarray(src["downstream_idx"], np.int64)
+        coupled = reach | reach[dn]
+        ci, di = np.asarray(src["bifurcation_catchment_idx"]), np.asarray(src["bifurcation_downstream_idx"])
+        coupled[ci[reach[di]]] = True
+        coupled[di[reach[ci]]] = True
+        keep2 = ~coupled
         errors["discharge_outside_reservoir_reaches"] = [
             float(f"{np.linalg.norm((q32[t] - q64[t])[keep]) / np.linalg.norm(q64[t][keep]):.4g}")
             for t in range(M)]
@@ -354,10 +453,21 @@ def batched_full_line(args, device: str) -> dict:
                    "discharge_outside_reservoir_reaches": [
                        float(f"{np.linalg.norm((qpr[t] - q64[t])[keep]) / np.linalg.norm(q64[t][keep]):.4g}")
                        for t in range(M)]}
+        precise["discharge_outside_reservoir_coupled"] = [
+            float(f"{np.linalg.norm((qpr[t] - q64[t])[keep2]) / np.linalg.norm(q64[t][keep2]):.4g}")
+            for t in range(M)]
+        precise["max_discharge_rel_l2_outside_reservoir_coupled"] = max(precise["discharge_outside_reservoir_coupled"])
+        sens = {"discharge": [float(f"{np.linalg.norm(qpt[t] - q64[t]) / np.linalg.norm(q64[t]):.4g}")
+                              for t in range(M)],
+                "discharge_outside_reservoir_reaches": [
+                    float(f"{np.linalg.norm((qpt[t] - q64[t])[keep]) / np.linalg.norm(q64[t][keep]):.4g}")
+                    for t in range(M)]}
+        sens["max_discharge_rel_l2"] = max(sens["discharge"])
         precise["max_discharge_rel_l2"] = max(precise["discharge"])
         precise["max_discharge_rel_l2_outside_reservoir_reaches"] = max(precise["discharge_outside_reservoir_reaches"])
-        del q32, q64, qpr
+        del q32, q64, qpr, qpt
         out.update({"error_days": args.full_days, "rel_l2_per_member": errors, "precise_surfaces": precise,
+                    "fp64_perturbed_1e-7": sens, "reservoir_coupled_catchments": int(coupled.sum()),
                     "max_discharge_rel_l2": max(errors["discharge"]),
                     "reservoir_reach_catchments": int((~keep).sum()),
                     "max_discharge_rel_l2_outside_reservoir_reaches":
@@ -382,7 +492,9 @@ def batched_full_line(args, device: str) -> dict:
         msg += (f"; discharge error after {args.full_days} day(s) up to {out['max_discharge_rel_l2']:.3g}, "
                 f"{out['max_discharge_rel_l2_outside_reservoir_reaches']:.3g} outside reservoir reaches; precise "
                 f"surfaces {out['precise_surfaces']['max_discharge_rel_l2']:.3g}, "
-                f"{out['precise_surfaces']['max_discharge_rel_l2_outside_reservoir_reaches']:.3g}")
+                f"{out['precise_surfaces']['max_discharge_rel_l2_outside_reservoir_reaches']:.3g} outside reaches, "
+                f"{out['precise_surfaces']['max_discharge_rel_l2_outside_reservoir_coupled']:.3g} outside reaches and "
+                f"their neighbours; FP64 perturbed by 1e-7: {out['fp64_perturbed_1e-7']['max_discharge_rel_l2']:.3g}")
     log(msg)
     return out
 
@@ -419,12 +531,15 @@ def graph_line(args, device: str) -> dict:
 
 
 def adjoint_line(args, device: str) -> dict:
-    """Cost of the adjoint on the 6-arcmin network with all its modules: the
-    base physics, 40,000 bifurcation paths and levees on 5 % of the
-    catchments: 2,000 gauges with upstream area > 5,000 km^2,
-    `--adjoint-steps` outer steps, checkpoints every 48.  Forward time per
-    sub-step against backward time (segment re-run + reverse sweep) per
-    sub-step."""
+    """The calibration step on the 6-arcmin network with all its modules
+    (base physics, 40,000 bifurcation paths, levees on 5 %): 2,000 gauges with
+    upstream area > 5,000 km^2, observations from a forward run with Manning's
+    n and river height scaled by lognormal(0, 0.2) factors, loss = MSE of log
+    discharge over `--adjoint-steps` outer steps (720: 15 days), checkpoints
+    every 48.  Forward time per sub-step against backward time (segment
+    re-runs + reverse sweep) per sub-step, after one untimed forward and
+    backward of one segment (first use of every kernel, and the tape of a
+    full segment allocated once by the caching allocator)."""
     import numpy as np
     import torch
     from cmfgpu_amd import synthetic
@@ -437,17 +552,26 @@ def adjoint_line(args, device: str) -> dict:
                       step_duration=1800.0)
     gauges = net.gauges
     steps = args.adjoint_steps
-    problem = GaugeProblem(model, gauges, np.ones((steps, gauges.shape[0])), checkpoint_every=48)
-    # warm-up: the same forward and backward once (first use of every kernel,
-    # and the tape of a full checkpoint segment, tens of GB at this size, is
-    # allocated once by the caching allocator, as in a calibration loop that
-    # takes many gradients); then the timed pass from the same initial state
-    problem.run()
-    problem.gradient()
+    K = 48
+    rng = np.random.default_rng(2003)
+    problem = GaugeProblem(model, gauges, np.ones((steps, gauges.shape[0])), checkpoint_every=K)
+    n0, h0 = src["river_manning"].astype(np.float64), src["river_height"].astype(np.float64)
+    problem.set_parameters(n0 * rng.lognormal(0.0, 0.2, n0.size), h0 * rng.lognormal(0.0, 0.2, h0.size))
+    obs = []
+    for _ in range(steps):
+        model.step_advance()
+        obs.append(problem._gauge_q().cpu().numpy())
+    problem.set_parameters(n0, h0)
+    problem._restore(problem.initial)
+    warm = GaugeProblem(model, gauges, np.array(obs[:min(K, steps)]), checkpoint_every=K)
+    warm.run()
+    warm.gradient()
+    del warm
+    problem.obs = torch.as_tensor(np.array(obs), dtype=torch.float64, device=model.device)
     problem._restore(problem.initial)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    problem.run()
+    loss = problem.run()
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
@@ -456,12 +580,14 @@ def adjoint_line(args, device: str) -> dict:
     t_bwd = time.perf_counter() - t0
     sub = sum(problem.counts)
     out = {"workload": f"{net.num_catchments} catchments, {net.num_paths} bifurcation paths, {net.num_levees} "
-                       f"levees, {gauges.shape[0]} gauges",
-           "outer_steps": steps, "substeps": sub, "forward_ms_per_substep": t_fwd / sub * 1e3,
-           "backward_ms_per_substep": t_bwd / sub * 1e3, "ratio": round(t_bwd / t_fwd, 3),
-           "gradient_finite": bool(torch.isfinite(g_n).all().item() and torch.isfinite(g_h).all().item())}
-    log(f"adjoint: forward {out['forward_ms_per_substep']:.3f} ms/sub-step, backward "
-        f"{out['backward_ms_per_substep']:.3f} ms/sub-step ({out['ratio']}x)")
+                       f"levees, {gauges.shape[0]} gauges, observations from lognormal(0, 0.2)-perturbed n and H",
+           "outer_steps": steps, "checkpoint_every": K, "substeps": sub, "loss": loss,
+           "forward_ms_per_substep": t_fwd / sub * 1e3, "backward_ms_per_substep": t_bwd / sub * 1e3,
+           "ratio": round(t_bwd / t_fwd, 3),
+           "gradient_finite": bool(torch.isfinite(g_n).all().item() and torch.isfinite(g_h).all().item()),
+           "gradient_nonzero_catchments": int(((g_n != 0) | (g_h != 0)).sum().item())}
+    log(f"adjoint: {steps} outer steps ({sub} sub-steps), forward {out['forward_ms_per_substep']:.3f} ms/sub-step, "
+        f"backward {out['backward_ms_per_substep']:.3f} ms/sub-step ({out['ratio']}x)")
     model.close()
     return out
 
@@ -521,14 +647,14 @@ def kernel_times(model, reps: int):
     return kt(model, reps)
 
 
-def cpu_baseline(model, net, budget_s: float):
-    """CPU oracle on the GPU model's current state; bounded sample of outer steps."""
+def _oracle_from_model(model, net):
+    """The CPU oracle on the GPU model's current state (reference order)."""
     import numpy as np
     from cmfgpu_amd.state import assemble
     from oracle.oracle import OracleModel
     src = dict(net.fields)
     src["runoff"] = model.get("runoff")
-    host = assemble(src, model.opened_modules, cold_start=False)
+    host = assemble(src, model.opened_modules, cold_start=False, precise_surfaces=model.precise_surfaces)
     state = {k: v.copy() for k, v in host.fields.items()}
     for name in ("river_storage", "flood_storage", "protected_storage", "protected_depth", "river_depth",
                  "flood_depth", "river_outflow", "flood_outflow", "river_cross_section_depth",
@@ -536,45 +662,111 @@ def cpu_baseline(model, net, budget_s: float):
                  "bifurcation_outflow", "bifurcation_cross_section_depth"):
         if name in state and name in model.t:
             state[name] = np.ascontiguousarray(model.get(name), dtype=state[name].dtype)
-    orc = OracleModel(state, flags=host.flags, num_flood_levels=host.num_flood_levels,
-                      num_bifurcation_levels=host.num_bifurcation_levels)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    substeps = 0
-    steps = 0
+    return OracleModel(state, flags=host.flags, num_flood_levels=host.num_flood_levels,
+                       num_bifurcation_levels=host.num_bifurcation_levels)
+
+
+def _timed_reps(orc, n, outer_steps: int, reps: int = 3):
+    """Median catchment-sub-steps/s of `reps` runs of `outer_steps` outer steps."""
+    import statistics
+    rates = []
+    subs = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        k = int(orc.advance(1800.0, outer_steps).sum())
+        rates.append(n * k / (time.perf_counter() - t0))
+        subs.append(k)
+    return statistics.median(rates), subs
+
+
+def _cpu_config(config: int, threads: int, all_steps: int, one_steps: int) -> dict:
+    """A benchmark configuration on the CPU oracle: one simulated day of
+    warm-up (48 outer steps, all threads), then the median of 3 runs of
+    `all_steps` outer steps on all threads and of 3 runs of `one_steps` on
+    one thread."""
+    import numpy as np
+    from cmfgpu_amd import synthetic
+    from oracle.oracle import OracleModel, set_threads
+    from cmfgpu_amd.state import assemble
+    net = synthetic.build_config(config)
+    src = dict(net.fields)
+    rho = synthetic.CONFIGS[config]["rho"]
+    forcing = synthetic.Runoff(src["catchment_area"], 2000 + config, rho=rho)
+    src["runoff"] = forcing.next_day()
+    mods = ("base", "adaptive_time") + (("bifurcation",) if net.num_paths else ()) + \
+        (("levee",) if net.num_levees else ())
+    host = assemble(src, mods)
+    orc = OracleModel({k: v.copy() for k, v in host.fields.items()}, flags=host.flags,
+                      num_flood_levels=host.num_flood_levels, num_bifurcation_levels=host.num_bifurcation_levels)
+    prev = set_threads(threads)
     t0 = time.perf_counter()
-    while True:
-        substeps += int(orc.advance(1800.0, 1)[0])
-        steps += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
-    dt = time.perf_counter() - t0
-    # one thread: one outer step more from where the sample ended
-    from oracle.oracle import set_threads
-    prev = set_threads(1)
-    t1 = time.perf_counter()
-    one_sub = int(orc.advance(1800.0, 1)[0])
-    dt1 = time.perf_counter() - t1
+    orc.advance(1800.0, 48)
+    warm = time.perf_counter() - t0
+    orc.state["runoff"][:] = forcing.next_day()
+    rate_all, subs_all = _timed_reps(orc, net.num_catchments, all_steps)
+    set_threads(1)
+    rate_one, subs_one = _timed_reps(orc, net.num_catchments, one_steps)
     set_threads(prev)
-    return {"value": net.num_catchments * substeps / dt, "unit": "catchment-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{steps} outer steps ({substeps} CFL sub-steps) of the same network from the GPU "
-                      f"run's state, CPU oracle (C++ restatement, OpenMP, {threads} threads), {dt:.1f} s",
-            "one_thread": {"value": net.num_catchments * one_sub / dt1, "unit": "catchment-steps/s",
-                           "sample": f"the next outer step ({one_sub} CFL sub-steps) on one thread, {dt1:.1f} s"}}
+    return {"catchments": net.num_catchments, "paths": net.num_paths, "levees": net.num_levees,
+            "warmup": f"one simulated day (48 outer steps, {warm:.1f} s)",
+            "all_threads": {"value": rate_all, "threads": threads,
+                            "sample": f"median of 3 runs of {all_steps} outer steps (sub-steps {subs_all})"},
+            "one_thread": {"value": rate_one,
+                           "sample": f"median of 3 runs of {one_steps} outer step(s) (sub-steps {subs_one})"},
+            "unit": "catchment-steps/s"}
+
+
+def cpu_baseline(model, net, budget_s: float):
+    """The CPU oracle (C++/OpenMP restatement of the reference step) timed on
+    the host cores.  `value`: the same workload as the GPU line, from the GPU
+    run's state, median of 3 runs sized to about budget_s / 3 each; one
+    thread likewise on one outer step.  `configs`: the 60,000-catchment plain
+    configuration and the 270,000-catchment full-physics one, each after a
+    simulated day of warm-up, medians of 3 runs on all threads and on one."""
+    from oracle.oracle import set_threads
+    orc = _oracle_from_model(model, net)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    prev = set_threads(threads)
+    t0 = time.perf_counter()
+    k0 = int(orc.advance(1800.0, 1)[0])
+    per_step = time.perf_counter() - t0
+    steps = max(1, int(budget_s / 3 / per_step))
+    rate, subs = _timed_reps(orc, net.num_catchments, steps)
+    set_threads(1)
+    rate1, subs1 = _timed_reps(orc, net.num_catchments, 1)
+    set_threads(prev)
+    out = {"value": rate, "unit": "catchment-steps/s", "cores": threads, "kind": "port",
+           "sample": f"median of 3 runs of {steps} outer steps (sub-steps {subs}) of the same network from "
+                     f"the GPU run's state, CPU oracle (C++ restatement, OpenMP, {threads} threads), after one "
+                     f"untimed outer step ({k0} sub-steps)",
+           "one_thread": {"value": rate1, "unit": "catchment-steps/s",
+                          "sample": f"median of 3 runs of 1 outer step (sub-steps {subs1}) on one thread"}}
+    out["configs"] = {"60k_plain": _cpu_config(0, threads, 12, 2),
+                      "270k_full_physics": _cpu_config(1, threads, 6, 1)}
+    return out
 
 
 def read_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (rocprofv3 FETCH_SIZE + WRITE_SIZE runs, calibrated), with where that
+    figure comes from: the file and the source commit it records."""
     path = ROOT / "profiles" / "hbm_traffic.txt"
     if not path.exists():
-        return None
+        return None, None
+    commit = None
+    value = None
     for line in path.read_text().splitlines():
+        if line.startswith("# source commit:"):
+            commit = line.split(":", 1)[1].strip()
         parts = line.split()
         if len(parts) >= 2 and not line.startswith("#") and kernel.split("<")[0] == parts[0]:
             try:
-                return float(parts[1])
+                value = float(parts[1])
             except ValueError:
-                return None
-    return None
+                value = None
+    return value, {"file": "profiles/hbm_traffic.txt", "measured_at_commit": commit,
+                   "how": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of the same workload, "
+                          "scaled by a device copy of known size"}
 
 
 def main():
@@ -583,7 +775,7 @@ def main():
     ap.add_argument("--steps", type=int, default=48, help="timed outer steps (1800 s each)")
     ap.add_argument("--warmup", type=int, default=48, help="untimed outer steps before timing")
     ap.add_argument("--members", type=int, default=64, help="members of the batched line (0: skip)")
-    ap.add_argument("--batched-steps", type=int, default=0, help="timed outer steps of the batched run")
+    ap.add_argument("--batched-steps", type=int, default=12, help="timed outer steps of the batched run")
     ap.add_argument("--scale", type=float, default=1.0, help="network size factor (1.0 = 1.6M catchments)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--kernel-reps", type=int, default=20)
@@ -595,12 +787,14 @@ def main():
     ap.add_argument("--full-steps", type=int, default=4, help="timed outer steps of that line")
     ap.add_argument("--full-days", type=int, default=1, help="simulated days of its FP32-vs-FP64 error run")
     ap.add_argument("--autotune", type=int, default=1, help="pick kernel block sizes by timing first (0: 256)")
-    ap.add_argument("--adjoint-steps", type=int, default=48, help="outer steps of the adjoint line (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=720, help="outer steps of the adjoint line (0: skip)")
     ap.add_argument("--graph", type=int, default=1, help="eager vs HIP-graph line on the small network (0: skip)")
     ap.add_argument("--error-days", type=int, default=1,
                     help="simulated days of the FP32-vs-FP64 error run (0: skip)")
     ap.add_argument("--layout", default="basin", choices=("basin", "level", "network"),
                     help="device catchment order")
+    ap.add_argument("--reference-surfaces", type=int, default=1,
+                    help="also time the plain run with the reference's FP32 surfaces (0: skip)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
@@ -635,7 +829,10 @@ def main():
         f"({time.perf_counter() - t0:.1f} s)")
 
     # ---------------------------------------------------------- plain --
-    model = build_model(net, src, 1, device, args.layout)
+    # the headline run forms water surfaces in double (precise_surfaces), the
+    # FP32 profile that meets a 1e-5 discharge error against FP64; the same
+    # run with the reference's FP32 surfaces is timed beside it
+    model = build_model(net, src, 1, device, args.layout, precise=True)
     tuned = model.autotune(use_cache=False) if args.autotune else None
     if tuned:
         log("autotune: " + ", ".join(f"{k}={v['block']}" for k, v in sorted(tuned.items())))
@@ -648,6 +845,23 @@ def main():
     value = work / elapsed
     substeps = work // model.N
     log(f"plain: {args.steps} outer steps, {substeps} sub-steps, {elapsed:.3f} s -> {value:.4g} catchment-steps/s")
+    reference_surfaces = None
+    ref_model = build_model(net, src, 1, device, args.layout, precise=False) if args.reference_surfaces else None
+    if ref_model is not None:
+        if tuned:
+            from cmfgpu_amd.autotune import get_block_sizes, set_block_sizes
+            set_block_sizes(ref_model, get_block_sizes(model))
+        run_steps(ref_model, args.warmup, per_trial=False)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rwork = run_steps(ref_model, args.steps, per_trial=False)
+        torch.cuda.synchronize()
+        relapsed = time.perf_counter() - t1
+        reference_surfaces = {"value": rwork / relapsed, "unit": "catchment-steps/s",
+                              "ms_per_step": relapsed / args.steps * 1e3, "substeps_timed": int(rwork // model.N)}
+        log(f"plain, reference surfaces: {rwork / relapsed:.4g} catchment-steps/s")
+        ref_model.close()
+        del ref_model
 
     f_wet = float((model.t["flood_depth"] > 0).float().mean().item())
     ktimes = kernel_times(model, args.kernel_reps)
@@ -657,14 +871,15 @@ def main():
     substep_ms = sum(ktimes.values())
     bytes_all = sum(algorithmic_bytes(k, model, f_wet) for k in ktimes)
     log("kernels (ms): " + ", ".join(f"{k}={v:.4f}" for k, v in sorted(ktimes.items(), key=lambda x: -x[1])))
-    traffic = read_traffic(dominant)
+    traffic, traffic_source = read_traffic(dominant)
     # the whole sub-step against the compulsory traffic of a fully fused one
     # (wall clock of the timed run, CFL pass and host sync included)
     fused_b = fused_substep_bytes(model.N, model.V, model.P, f_wet)
     us_sub = elapsed / substeps * 1e6
     fused_gbs = fused_b / (us_sub * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dominant,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_source,
+                "kernel": dominant,
                 "algorithmic_bytes": int(bytes_dom), "kernel_ms": round(ktimes[dominant], 5),
                 "substep_kernels_ms": round(substep_ms, 5),
                 "substep_frac": round(bytes_all / (substep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -684,29 +899,8 @@ def main():
     # -------------------------------------------------------- batched --
     batched = None
     if args.members > 1:
-        M = args.members
-        bm = build_model(net, src, M, device, args.layout)
-        if args.autotune:
-            bm.autotune(use_cache=False)
-        run_steps(bm, max(args.warmup // 4, 1), per_trial=True)
-        bsteps = args.batched_steps or max(args.steps // 4, 2)
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        bwork = run_steps(bm, bsteps, per_trial=True)
-        torch.cuda.synchronize()
-        belapsed = time.perf_counter() - t2
-        bvalue = bwork / belapsed
-        # whole-run algorithmic rate, shared parameters counted once per catchment
-        bf_wet = float((bm.t["flood_depth"] > 0).float().mean().item())
-        bgbs = batched_full_bytes(bm.N, M, bm.P, bf_wet) * (bwork / (bm.N * M)) / belapsed / 1e9
-        cgbs = fused_batched_bytes(bm.N, M, bm.V, bm.P, bf_wet) * (bwork / (bm.N * M)) / belapsed / 1e9
-        batched = {"members": M, "value": bvalue, "unit": "member-catchment-steps/s", "steps": bsteps,
-                   "vs_plain": round(bvalue / value, 3), "GBps_algorithmic": round(bgbs, 1),
-                   "frac_of_peak": round(bgbs / HBM_PEAK_GBS, 4),
-                   "GBps_compulsory_fused": round(cgbs, 1), "frac_compulsory_fused": round(cgbs / HBM_PEAK_GBS, 4)}
-        log(f"batched: {M} members, {bsteps} outer steps, {belapsed:.3f} s -> {bvalue:.4g} "
-            f"member-catchment-steps/s ({bvalue / value:.2f}x plain), {bgbs:.0f} GB/s algorithmic")
-        bm.close()
+        batched = batched_line(args, net, src, device, value,
+                               reference_surfaces["value"] if reference_surfaces else value, tuned)
 
     basins = None
     if args.basins > 0:
@@ -726,11 +920,13 @@ def main():
         "metric": METRIC, "value": value, "unit": "catchment-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 storage)", "data": "synthetic",
-        "config": {"workload": "6-arcmin synthetic network, plain method, levees + bifurcation, adaptive CFL",
+        "config": {"workload": "6-arcmin synthetic network, plain method, levees + bifurcation, adaptive CFL, "
+                               "water surfaces formed in double (precise_surfaces)",
                    "catchments": net.num_catchments, "bifurcation_paths": net.num_paths,
                    "levees": net.num_levees, "flood_levels": 10, "outer_step_s": 1800.0,
                    "substeps_timed": int(substeps), "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "basins": basins, "graph": graph,
+        "roofline": roofline, "cpu_baseline": cpu, "reference_surfaces": reference_surfaces,
+        "batched": batched, "basins": basins, "graph": graph,
         "adjoint": adjoint, "forcing": forcing, "batched_full": batched_full,
         "block_sizes": {k: v["block"] for k, v in tuned.items()} if tuned else None,
         "error_vs_fp64": errors,
