@@ -345,3 +345,27 @@ def test_schedules_bitexact(case):
                 continue
             msg = mismatch(m.get(f), orc.state[f])
             assert msg is None, f"{name} {f}: {msg}"
+
+
+def test_fp32_storage_batched_members_match_standalone_runs():
+    """Members in the float32 profile without mixed precision, trial-inner,
+    per-member CFL counts, full physics: each bit-exact with a standalone
+    (float, float) oracle run on its own runoff."""
+    net, src = small_source()
+    M = 3
+    runoffs = _member_sources(src, M)
+    model = _model(src, FULL, num_trials=M, batched={"runoff": np.stack(runoffs)}, per_trial_time_step=True,
+                   trial_layout="inner", mixed_precision=False)
+    oracles = [oracle_for(dict(src, runoff=runoffs[m]), FULL, mixed_precision=False)[0] for m in range(M)]
+    for step in range(4):
+        model.step_advance()
+        counts = model.t["max_sub_steps"].cpu().numpy()
+        for m in range(M):
+            assert int(oracles[m].advance(1800.0, 1)[0]) == counts[m], (step, m)
+    import torch
+    torch.cuda.synchronize()
+    for name in STATE_FIELDS:
+        got = model.get(name)
+        for m in range(M):
+            msg = mismatch(got[m], oracles[m].state[name])
+            assert msg is None, f"member {m} {name}: {msg}"

@@ -507,7 +507,11 @@ def test_levee_cases_are_continuous_and_monotone():
     them jumps, the total is conserved, and the protected side first holds
     water exactly above S(base) + (ld + W)(crown - base) L."""
     totals = np.linspace(1e6, 1.2e9, 4001)
-    out = np.array([levee_outputs(t) for t in totals])
+    prev = O.set_threads(1)          # 8,002 one-catchment kernels: no parallel regions
+    try:
+        out = np.array([levee_outputs(t) for t in totals])
+    finally:
+        O.set_threads(prev)
     r, f, p, fdep = out[:, 0], out[:, 1], out[:, 2], out[:, 3]
     assert np.allclose(r + f + p, totals, rtol=1e-13)
     step = totals[1] - totals[0]
