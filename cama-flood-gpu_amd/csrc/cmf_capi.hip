@@ -67,18 +67,23 @@ struct cmf_network {
     int64_t G = 0;
     int64_t nblk = 0;
     int32_t* cgroup = nullptr;
-    // block records {first catchment, end, table entry, group}, in table
-    // order (rec_id) and, for an adaptive advance, ordered by group count
+    // block records {first catchment, end, table entry, group} in memory
+    // order (rec_id); group g holds records [gblk_ptr[g], gblk_ptr[g + 1])
     int4* blk_rec_id = nullptr;
-    int4* blk_rec_ord = nullptr;
-    std::vector<int4> rec_host;
-    int32_t* gblk_ptr = nullptr;      // first block of each group, G + 1 entries
-    // adaptive grouped advance: per sub-step the number of leading ordered
-    // records whose group is still sub-stepping
+    int32_t* gblk_ptr = nullptr;
     std::vector<int32_t> gblk_host;
-    int4* host_rec = nullptr;         // pinned staging of blk_rec_ord
+    // adaptive grouped advance: one table per sub-step k of the outer step,
+    // the records of the groups with count > k in memory order, at
+    // level_rec + level_off[k], sub_blocks[k] records long; built on the
+    // device from level_pos[k * G + g] (where group g's records start in
+    // level k's table), staged through host_level_pos
+    int4* level_rec = nullptr;
+    int64_t level_cap = 0;
+    int32_t* level_pos = nullptr;
+    int32_t* host_level_pos = nullptr;
+    int64_t level_pos_cap = 0;
     hipEvent_t order_copied = nullptr;
-    std::vector<int64_t> sub_blocks;
+    std::vector<int64_t> sub_blocks, level_off;
     // captured sub-step loops (cmf_hip_advance_graph): one executable graph
     // per sub-step count, valid for the field table they were captured with
     hipStream_t capture_stream = nullptr;
@@ -111,12 +116,12 @@ void free_network(cmf_network* net) {
     void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->ev_other, net->ev1, net->pos_term,
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
                     net->path_neg, net->path_net, net->path_raw, net->partials, net->cgroup,
-                    net->blk_rec_id, net->blk_rec_ord, net->gblk_ptr, net->lflags, net->lim_list,
+                    net->blk_rec_id, net->level_rec, net->level_pos, net->gblk_ptr, net->lflags, net->lim_list,
                     net->stage_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (net->host_counts) (void)hipHostFree(net->host_counts);
-    if (net->host_rec) (void)hipHostFree(net->host_rec);
+    if (net->host_level_pos) (void)hipHostFree(net->host_level_pos);
     if (net->order_copied) (void)hipEventDestroy(net->order_copied);
     for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
     if (net->capture_stream) (void)hipStreamDestroy(net->capture_stream);
@@ -468,9 +473,9 @@ int log_final(const View<REAL, STO>& v, int64_t nblocks, uint32_t mask, void* st
     } while (0)
 
 // Block size of a tunable kernel; catchment kernels of a grouped network run
-// the group block table (kBlock threads per entry).
+// the group block table (kGroupRecord threads per entry).
 inline int64_t block_of(const cmf_network* net, int id, bool catchment_kernel) {
-    return (catchment_kernel && net->G > 0) ? kBlock : net->blocks[id];
+    return (catchment_kernel && net->G > 0) ? kGroupRecord : net->blocks[id];
 }
 
 // ----------------------------------------------------------- per-op -----
@@ -739,7 +744,7 @@ template <class REAL, class STO, class VT>
 int three_pass_substep(cmf_network* net, const VT& v, int substep, int store_inflow, void* stream) {
     // catchment kernels: N*T threads, or one block per group table entry
     const bool ordered = net->G > 0 && v.ordered && substep >= 0 && substep < (int)net->sub_blocks.size();
-    const int64_t NT = net->G > 0 ? (ordered ? net->sub_blocks[substep] : net->nblk) * kBlock : v.N * v.T;
+    const int64_t NT = net->G > 0 ? (ordered ? net->sub_blocks[substep] : net->nblk) * kGroupRecord : v.N * v.T;
     const bool contig = net->contiguous != 0;
     const bool bif = v.has(CMF_HAS_BIFURCATION), levee = v.has(CMF_HAS_LEVEE);
     const bool log = v.has(CMF_HAS_LOG);
@@ -764,7 +769,13 @@ int three_pass_substep(cmf_network* net, const VT& v, int substep, int store_inf
         LAUNCH_B((k_reservoir<REAL, STO, true>), v.R * v.T, block_of(net, CMF_TUNE_RESERVOIR, false), stream, v, substep);
     // limiters of every catchment, then the stage pass (which also forms the
     // path nets from the limiters of both path ends)
-    if (net->G > 0) {
+    if constexpr (VT::kGrouped) {
+        // one block of kGroupRecord / 2 threads per group table entry
+        if (contig)
+            LAUNCH_B((k_limiter_pair<REAL, STO, true, VT>), NT / 2, kGroupRecord / 2, stream, v, substep, store_inflow);
+        else
+            LAUNCH_B((k_limiter_pair<REAL, STO, false, VT>), NT / 2, kGroupRecord / 2, stream, v, substep, store_inflow);
+    } else if (net->G > 0) {
         // the group block table maps one catchment per thread
         if (contig) LAUNCH_B((k_limiter<REAL, STO, true>), NT, b_lim, stream, v, substep, store_inflow);
         else LAUNCH_B((k_limiter<REAL, STO, false>), NT, b_lim, stream, v, substep, store_inflow);
@@ -805,6 +816,16 @@ int three_pass_substep(cmf_network* net, const VT& v, int substep, int store_inf
     return CMF_OK;
 }
 
+// The group block table of sub-step `substep` of an ordered (adaptive,
+// eager) grouped advance: the records of the groups still sub-stepping, in
+// memory order; else the view's own table.
+template <class REAL, class STO>
+const int4* level_table(const cmf_network* net, const View<REAL, STO>& v, int substep) {
+    if (net->G > 0 && v.ordered && substep >= 0 && substep < (int)net->sub_blocks.size())
+        return net->level_rec + net->level_off[(size_t)substep];
+    return v.blk_rec;
+}
+
 // One fused sub-step: outflow [+ bifurcation paths], [reservoir], limiters,
 // stage (inflow sums, limited flows, bifurcation nets, storage, flood stage
 // [, levee stage]).
@@ -837,6 +858,19 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
         PlainView<REAL, STO> pv;
         static_cast<View<REAL, STO>&>(pv) = v;
         return three_pass_substep<REAL, STO, PlainView<REAL, STO>>(net, pv, substep, store_inflow, stream);
+    }
+    // grouped, without the path pass (whose blocks are no group records)
+    if (net->G > 0 && v.T == 1 && v.batched == 0 && !v.has(CMF_PER_TRIAL_DT) && !v.has(CMF_HAS_BIFURCATION) &&
+        !v.tape_q_r) {
+        GroupView<REAL, STO> gv;
+        static_cast<View<REAL, STO>&>(gv) = v;
+        gv.blk_rec = level_table(net, v, substep);
+        return three_pass_substep<REAL, STO, GroupView<REAL, STO>>(net, gv, substep, store_inflow, stream);
+    }
+    if (net->G > 0) {
+        View<REAL, STO> w = v;
+        w.blk_rec = level_table(net, v, substep);
+        return three_pass_substep<REAL, STO, View<REAL, STO>>(net, w, substep, store_inflow, stream);
     }
     return three_pass_substep<REAL, STO, View<REAL, STO>>(net, v, substep, store_inflow, stream);
 }
@@ -935,6 +969,74 @@ int substep_loop_graph(cmf_network* net, const View<REAL, STO>& v, const cmf_fie
     return CMF_OK;
 }
 
+// The per-sub-step group block tables of an adaptive grouped advance whose
+// group counts are in host_counts and, on the device, `counts` (count: their
+// maximum).  Level k lists the
+// records of the groups with count > k in memory order, so a sub-step
+// launches only those groups' blocks and walks memory as an ungrouped launch
+// does.  The host places each group in each level (count x G positions); a
+// device pass copies the records.  Called after the stream has drained (the
+// count read): the old tables are free to replace, and the staging buffer is
+// rewritten only after the previous copy from it has completed.
+int build_level_tables(cmf_network* net, const int32_t* counts, int count, hipStream_t s) {
+    const int64_t G = net->G;
+    const int32_t* hc = net->host_counts;
+    net->sub_blocks.assign((size_t)count, 0);
+    net->level_off.assign((size_t)count, 0);
+    for (int64_t g = 0; g < G; ++g) {
+        const int64_t nb = net->gblk_host[g + 1] - net->gblk_host[g];
+        for (int k = 0; k < hc[g] && k < count; ++k) net->sub_blocks[(size_t)k] += nb;
+    }
+    int64_t total = 0;
+    for (int k = 0; k < count; ++k) {
+        net->level_off[(size_t)k] = total;
+        total += net->sub_blocks[(size_t)k];
+    }
+    const int64_t npos = (int64_t)count * G;
+    if (total > net->level_cap) {
+        if (net->level_rec) CMF_HIP_CHECK(hipFree(net->level_rec));
+        net->level_rec = nullptr;
+        net->level_cap = 0;
+        const int64_t cap = std::max<int64_t>(total, net->nblk * 4);
+        int rc = dev_alloc((void**)&net->level_rec, (size_t)cap * sizeof(int4));
+        if (rc) return rc;
+        net->level_cap = cap;
+    }
+    CMF_HIP_CHECK(hipEventSynchronize(net->order_copied));
+    if (npos > net->level_pos_cap) {
+        if (net->level_pos) CMF_HIP_CHECK(hipFree(net->level_pos));
+        if (net->host_level_pos) CMF_HIP_CHECK(hipHostFree(net->host_level_pos));
+        net->level_pos = nullptr;
+        net->host_level_pos = nullptr;
+        net->level_pos_cap = 0;
+        const int64_t cap = std::max<int64_t>(npos, G * 32);
+        int rc = dev_alloc((void**)&net->level_pos, (size_t)cap * sizeof(int32_t));
+        if (rc) return rc;
+        hipError_t e = hipHostMalloc((void**)&net->host_level_pos, (size_t)cap * sizeof(int32_t), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            net->host_level_pos = nullptr;
+            return set_error(CMF_ERR_NOMEM, "hipHostMalloc failed: %s", hipGetErrorString(e));
+        }
+        net->level_pos_cap = cap;
+    }
+    std::vector<int64_t> run(net->level_off);
+    for (int64_t g = 0; g < G; ++g) {
+        const int64_t nb = net->gblk_host[g + 1] - net->gblk_host[g];
+        for (int k = 0; k < hc[g] && k < count; ++k) {
+            net->host_level_pos[(int64_t)k * G + g] = (int32_t)run[(size_t)k];
+            run[(size_t)k] += nb;
+        }
+    }
+    CMF_HIP_CHECK(hipMemcpyAsync(net->level_pos, net->host_level_pos, (size_t)npos * sizeof(int32_t),
+                                 hipMemcpyHostToDevice, s));
+    CMF_HIP_CHECK(hipEventRecord(net->order_copied, s));
+    hipLaunchKernelGGL(k_level_tables, dim3((unsigned)blocks_for(net->nblk)), dim3(kBlock), 0, s,
+                       (const int4*)net->blk_rec_id, net->nblk, (const int32_t*)net->gblk_ptr, counts,
+                       (const int32_t*)net->level_pos, G, count, (int4*)net->level_rec);
+    CMF_HIP_CHECK(hipGetLastError());
+    return CMF_OK;
+}
+
 template <class REAL, class STO>
 int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_count,
                 int32_t* out_count, void* stream, bool use_graph) {
@@ -955,7 +1057,8 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
         hipLaunchKernelGGL((k_prepare<REAL>), dim3(1), dim3(64), 0, s, static_cast<REAL*>(f->outer_time_step), outer);
         hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, f->max_sub_steps, nslots);
         if (net->G > 0) {
-            hipLaunchKernelGGL((k_cfl_grouped<REAL, STO>), dim3((unsigned)net->nblk), dim3(kBlock), 0, s, v);
+            hipLaunchKernelGGL((k_cfl_grouped<REAL, STO>), dim3((unsigned)((net->nblk + kCflRecords - 1) / kCflRecords)),
+                               dim3(kGroupRecord), 0, s, v, (int64_t)net->nblk);
         } else if (v.N > 0) {
             const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 512);
             hipLaunchKernelGGL((k_cfl<REAL, STO>), dim3((unsigned)chunks, (unsigned)v.T), dim3(kBlock), 0, s, v);
@@ -982,33 +1085,7 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
         CMF_HIP_CHECK(hipGetLastError());
     }
     if (net->G > 0 && v.has(CMF_HAS_ADAPTIVE_TIME) && !use_graph) {
-        // Order the block table by group sub-step count, descending, so that
-        // sub-step k launches only the leading blocks of the groups with
-        // count > k.  The staging buffer is rewritten only after the previous
-        // copy from it has completed.
-        const int64_t G = net->G;
-        std::vector<int32_t> gs((size_t)G);
-        std::iota(gs.begin(), gs.end(), 0);
-        const int32_t* hc = net->host_counts;
-        std::stable_sort(gs.begin(), gs.end(), [hc](int32_t a, int32_t b) { return hc[a] > hc[b]; });
-        CMF_HIP_CHECK(hipEventSynchronize(net->order_copied));
-        std::vector<int64_t> lead((size_t)G + 1, 0);
-        int64_t nb = 0;
-        for (int64_t q = 0; q < G; ++q) {
-            const int32_t g = gs[q];
-            for (int32_t b = net->gblk_host[g]; b < net->gblk_host[g + 1]; ++b) net->host_rec[nb++] = net->rec_host[b];
-            lead[q + 1] = nb;
-        }
-        net->sub_blocks.assign((size_t)count, 0);
-        int64_t active = G;
-        for (int k = 0; k < count; ++k) {
-            while (active > 0 && hc[gs[active - 1]] <= k) --active;
-            net->sub_blocks[k] = lead[active];
-        }
-        CMF_HIP_CHECK(hipMemcpyAsync(net->blk_rec_ord, net->host_rec, (size_t)nb * sizeof(int4),
-                                     hipMemcpyHostToDevice, s));
-        CMF_HIP_CHECK(hipEventRecord(net->order_copied, s));
-        v.blk_rec = net->blk_rec_ord;
+        if ((rc = build_level_tables(net, f->max_sub_steps, count, s))) return rc;
         v.ordered = true;
     }
     rc = use_graph ? substep_loop_graph(net, v, f, count, s) : substep_loop(net, v, f, count, s);
@@ -1295,10 +1372,10 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
                                  (long long)g);
             gblk_ptr.push_back((int32_t)rec.size());
             for (int64_t i = group_ptr[g]; i < group_ptr[g + 1]; ++i) cgroup[i] = (int32_t)g;
-            for (int64_t b = group_ptr[g]; b < group_ptr[g + 1]; b += kBlock) {
+            for (int64_t b = group_ptr[g]; b < group_ptr[g + 1]; b += kGroupRecord) {
                 int4 r;
                 r.x = (int32_t)b;
-                r.y = (int32_t)std::min<int64_t>(b + kBlock, group_ptr[g + 1]);
+                r.y = (int32_t)std::min<int64_t>(b + kGroupRecord, group_ptr[g + 1]);
                 r.z = (int32_t)rec.size();
                 r.w = (int32_t)g;
                 rec.push_back(r);
@@ -1426,12 +1503,9 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     }
     if (num_groups > 0) {
         net->gblk_host = gblk_ptr;
-        net->rec_host = rec;
-        if ((rc = dev_alloc((void**)&net->blk_rec_ord, (size_t)net->nblk * sizeof(int4))) ||
-            (e = hipHostMalloc((void**)&net->host_rec, (size_t)net->nblk * sizeof(int4), hipHostMallocDefault)) != hipSuccess ||
-            (e = hipEventCreateWithFlags(&net->order_copied, hipEventDisableTiming)) != hipSuccess) {
+        if ((e = hipEventCreateWithFlags(&net->order_copied, hipEventDisableTiming)) != hipSuccess) {
             free_network(net);
-            return rc ? rc : set_error(CMF_ERR_NOMEM, "block order buffers: %s", hipGetErrorString(e));
+            return set_error(CMF_ERR_NOMEM, "hipEventCreateWithFlags: %s", hipGetErrorString(e));
         }
     }
     *out = net;

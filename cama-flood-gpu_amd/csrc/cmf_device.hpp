@@ -250,6 +250,8 @@ struct View {
     __device__ __forceinline__ void split_p(int64_t g, int64_t& t, int64_t& p) const {
         split(g, P, pts == 1, t, p);
     }
+    // GroupView: the group of a launch's catchments is that of its block record
+    static constexpr bool kGrouped = false;
 };
 
 // The view of a one-trial, ungrouped run without per-trial parameters or
@@ -281,6 +283,50 @@ struct PlainView : View<REAL, STO> {
     }
     __device__ __forceinline__ bool catchment_in(int64_t blk, int64_t& t, int64_t& i) const {
         return catchment_of(blk * blockDim.x + threadIdx.x, t, i);
+    }
+};
+
+// The view of a grouped network with one trial, shared parameters and no
+// bifurcation paths, for the catchment kernels, which run one block per
+// record of blk_rec.  A group is a union of whole basins, so every catchment
+// a thread reaches through the river network (itself, its downstream and
+// upstream catchments) is in the group of the block's record: the time-step
+// slot, dt and sub-step count are the record's, read once per block instead
+// of per catchment, and the per-trial indexing folds away as in PlainView.
+template <class REAL, class STO>
+struct GroupView : View<REAL, STO> {
+    static constexpr bool kGrouped = true;
+    __device__ __forceinline__ int4 rec() const { return this->blk_rec[blockIdx.x]; }
+    __device__ __forceinline__ int64_t el(int64_t, int64_t i) const { return i; }
+    __device__ __forceinline__ int64_t pel(int64_t, int64_t p) const { return p; }
+    __host__ __device__ __forceinline__ bool bat(uint32_t) const { return false; }
+    template <class X>
+    __device__ __forceinline__ X prm(const X* p, uint32_t, int64_t, int64_t i) const { return p[i]; }
+    __device__ __forceinline__ int64_t slot(int64_t, int64_t) const { return (int64_t)rec().w; }
+    __device__ __forceinline__ bool per_slot() const { return true; }
+    __device__ __forceinline__ REAL dt(int64_t, int64_t) const { return this->time_step[rec().w]; }
+    __device__ __forceinline__ bool active(int64_t, int64_t, int substep) const {
+        return substep < 0 || substep < this->max_sub_steps[rec().w];
+    }
+    __device__ __forceinline__ bool active_in_block(int64_t t, int64_t c, int substep) const {
+        return this->ordered || active(t, c, substep);
+    }
+    __device__ __forceinline__ bool last_substep(int64_t, int64_t, int substep, int shared_last) const {
+        return substep >= 0 ? substep == this->max_sub_steps[rec().w] - 1 : shared_last != 0;
+    }
+    __device__ __forceinline__ void split_c(int64_t g, int64_t& t, int64_t& i) const { t = 0; i = g; }
+    __device__ __forceinline__ void split_p(int64_t g, int64_t& t, int64_t& p) const { t = 0; p = g; }
+    __device__ __forceinline__ int64_t partial_slot() const { return (int64_t)rec().z; }
+    __device__ __forceinline__ bool catchment_of(int64_t, int64_t& t, int64_t& i) const {
+        const int4 r = rec();
+        t = 0;
+        i = (int64_t)r.x + threadIdx.x;
+        return i < r.y;
+    }
+    // block b of a launch is record b, whatever renumbering the caller applied
+    // (the record must be the one slot() reads)
+    __device__ __forceinline__ bool catchment_in(int64_t, int64_t& t, int64_t& i) const {
+        return catchment_of(0, t, i);
     }
 };
 

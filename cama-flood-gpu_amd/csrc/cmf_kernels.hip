@@ -27,6 +27,13 @@ namespace cmf {
 // block size is a launch parameter, a multiple of 64).
 constexpr int kBlock = 256;
 constexpr int kMaxBlock = 512;
+// Width of a group block table entry (catchments of one group per block of
+// a grouped network's catchment kernels, which run kGroupRecord threads).
+#ifndef CMF_GROUP_RECORD
+#define CMF_GROUP_RECORD 512
+#endif
+constexpr int kGroupRecord = CMF_GROUP_RECORD;
+static_assert(kGroupRecord % 128 == 0 && kGroupRecord <= kMaxBlock, "group records: 128..512 catchments");
 
 // Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
 // With CMF_XCD_REMAP the blocks are renumbered so that every XCD walks one
@@ -1213,24 +1220,36 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter(View<REAL, STO> v, int su
     if (v.has(CMF_HAS_BIFURCATION) && v.last_substep(t, j, substep, store_last)) v.limit_rate[e] = l;
 }
 
-// The same for ungrouped networks, two catchments per thread (g and g +
-// blockDim.x of a block's 2 * blockDim.x) with their loads interleaved: the
-// pass is bound by dependent round trips (upstream range, then its terms), so
-// two independent chains per thread keep twice the loads in flight.
+// The same, two catchments per thread (g and g + blockDim.x of a block's
+// 2 * blockDim.x) with their loads interleaved: the pass is bound by
+// dependent round trips (upstream range, then its terms), so two independent
+// chains per thread keep twice the loads in flight.  With GroupView the
+// block is one record of the group block table, blockDim.x half its width.
 template <class REAL, class STO, bool CONTIG, class VT = View<REAL, STO>>
 __global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(VT v, int substep, int store_last) {
-    const int64_t NT = v.N * v.T;
-    const int64_t g0 = (int64_t)blockIdx.x * (2 * (int64_t)blockDim.x) + threadIdx.x;
-    const int64_t g1 = g0 + blockDim.x;
     int64_t t0 = 0, j0 = 0, t1 = 0, j1 = 0;
-    bool a0 = g0 < NT, a1 = g1 < NT;
-    if (a0) {
-        v.split_c(g0, t0, j0);
-        a0 = v.active(t0, j0, substep);
-    }
-    if (a1) {
-        v.split_c(g1, t1, j1);
-        a1 = v.active(t1, j1, substep);
+    bool a0, a1;
+    if constexpr (VT::kGrouped) {
+        const int4 r = v.rec();
+        j0 = (int64_t)r.x + threadIdx.x;
+        j1 = j0 + blockDim.x;
+        const bool on = v.active_in_block(0, j0, substep);
+        a0 = on && j0 < r.y;
+        a1 = on && j1 < r.y;
+    } else {
+        const int64_t NT = v.N * v.T;
+        const int64_t g0 = (int64_t)blockIdx.x * (2 * (int64_t)blockDim.x) + threadIdx.x;
+        const int64_t g1 = g0 + blockDim.x;
+        a0 = g0 < NT;
+        a1 = g1 < NT;
+        if (a0) {
+            v.split_c(g0, t0, j0);
+            a0 = v.active(t0, j0, substep);
+        }
+        if (a1) {
+            v.split_c(g1, t1, j1);
+            a1 = v.active(t1, j1, substep);
+        }
     }
     if (!a0 && !a1) return;
     const int64_t e0 = v.el(t0, j0), e1 = v.el(t1, j1);
@@ -1622,16 +1641,20 @@ __global__ __launch_bounds__(kBlock) void k_levee(View<REAL, STO> v, int substep
 // =============================================================== CFL ====
 // Sub-step count of one catchment (adaptive_time.py:30-48): dam-related
 // catchments count 1.
+// CFL sub-step count of a catchment from its river depth and downstream distance
 template <class REAL, class STO>
-__device__ __forceinline__ int cfl_steps(const View<REAL, STO>& v, int64_t t, int64_t i, REAL outer) {
-    if (v.has(CMF_HAS_RESERVOIR) && v.is_dam_related[i]) return 1;
-    const int64_t e = v.el(t, i);
-    const REAL depth = vmax(v.river_depth[e], (REAL)0.01);
-    const REAL dist = v.prm(v.dist, CMF_BATCHED_DOWNSTREAM_DISTANCE, e, i);
+__device__ __forceinline__ int cfl_count(const View<REAL, STO>& v, REAL river_depth, REAL dist, REAL outer) {
+    const REAL depth = vmax(river_depth, (REAL)0.01);
     const REAL dt = v.adaptive_factor * dist / dsqrt(v.gravity * depth);
     const REAL dtc = vmin(dt, outer);
     const REAL nf = floor(outer / dtc - (REAL)0.01) + (REAL)1.0;
     return (int)nf;
+}
+template <class REAL, class STO>
+__device__ __forceinline__ int cfl_steps(const View<REAL, STO>& v, int64_t t, int64_t i, REAL outer) {
+    if (v.has(CMF_HAS_RESERVOIR) && v.is_dam_related[i]) return 1;
+    const int64_t e = v.el(t, i);
+    return cfl_count(v, v.river_depth[e], v.prm(v.dist, CMF_BATCHED_DOWNSTREAM_DISTANCE, e, i), outer);
 }
 
 __device__ __forceinline__ int block_max_int(int best) {
@@ -1665,15 +1688,80 @@ __global__ __launch_bounds__(kBlock) void k_cfl(View<REAL, STO> v) {
     if (threadIdx.x == 0) atomicMax(&v.max_sub_steps[v.has(CMF_PER_TRIAL_DT) ? t : 0], m);
 }
 
-// grouped networks: one block per table entry (a single group), one
-// integer atomic max per block into the group's count
+// grouped networks: kCflRecords consecutive table entries (each a single
+// group's catchments) per block of kGroupRecord threads, one catchment of
+// each entry per thread; the records are read first and every catchment load
+// is issued before any is used.  The entries' maxima are merged over runs of
+// one group, then one integer atomic max per run into the group's count (a
+// large group's entries otherwise queue on one address).
+constexpr int kCflRecords = 16;
 template <class REAL, class STO>
-__global__ __launch_bounds__(kBlock) void k_cfl_grouped(View<REAL, STO> v) {
+__global__ __launch_bounds__(kGroupRecord) void k_cfl_grouped(View<REAL, STO> v, int64_t nrec) {
+    __shared__ int sh[kCflRecords][kGroupRecord / 64];
+    __shared__ int rec_max[kCflRecords], rec_group[kCflRecords];
     const REAL outer = v.outer_time_step[0];
-    int64_t t, i;
-    const int best = v.catchment_of(0, t, i) ? cfl_steps(v, t, i, outer) : 0;
-    const int m = block_max_int(best);
-    if (threadIdx.x == 0) atomicMax(&v.max_sub_steps[v.blk_rec[blockIdx.x].w], m);
+    int4 r[kCflRecords];
+#pragma unroll
+    for (int k = 0; k < kCflRecords; ++k) {
+        const int64_t q = (int64_t)blockIdx.x * kCflRecords + k;
+        r[k] = q < nrec ? v.blk_rec[q] : make_int4(0, 0, 0, -1);
+    }
+    REAL depth[kCflRecords], dist[kCflRecords];
+    bool dam[kCflRecords];
+#pragma unroll
+    for (int k = 0; k < kCflRecords; ++k) {
+        // an index inside the record, or catchment 0 for lanes past its end
+        const int64_t i = (int64_t)r[k].x + threadIdx.x < r[k].y ? (int64_t)r[k].x + threadIdx.x : 0;
+        depth[k] = v.river_depth[v.el(0, i)];
+        dist[k] = v.prm(v.dist, CMF_BATCHED_DOWNSTREAM_DISTANCE, v.el(0, i), i);
+        dam[k] = v.has(CMF_HAS_RESERVOIR) && v.is_dam_related[i];
+    }
+#pragma unroll
+    for (int k = 0; k < kCflRecords; ++k) {
+        const bool in = (int64_t)r[k].x + threadIdx.x < r[k].y;
+        const int c = in ? (dam[k] ? 1 : cfl_count(v, depth[k], dist[k], outer)) : 0;
+        const int m = wave_max_int(c);
+        if ((threadIdx.x & 63) == 0) sh[k][threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x < kCflRecords) {
+        int m = sh[threadIdx.x][0];
+        for (int w = 1; w < kGroupRecord / 64; ++w) m = max(m, sh[threadIdx.x][w]);
+        int g = -1;
+#pragma unroll
+        for (int k = 0; k < kCflRecords; ++k)
+            if ((int)threadIdx.x == k) g = r[k].w;   // static indices: r stays in registers
+        rec_max[threadIdx.x] = m;
+        rec_group[threadIdx.x] = g;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int g = rec_group[0], m = rec_max[0];
+        for (int k = 1; k <= kCflRecords; ++k) {
+            if (k == kCflRecords || rec_group[k] != g) {
+                if (g >= 0) atomicMax(&v.max_sub_steps[g], m);
+                if (k == kCflRecords) break;
+                g = rec_group[k];
+                m = rec_max[k];
+            } else {
+                m = max(m, rec_max[k]);
+            }
+        }
+    }
+}
+
+// The per-sub-step group block tables (level k: the records of the groups
+// with count > k, group g's from level_pos[k * G + g] on): record r of group
+// g is copied into the first min(count, counts[g]) levels.
+__global__ void k_level_tables(const int4* rec, int64_t nrec, const int32_t* gblk, const int32_t* counts,
+                               const int32_t* level_pos, int64_t G, int count, int4* out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrec) return;
+    const int4 x = rec[r];
+    const int64_t g = x.w;
+    const int64_t within = r - gblk[g];
+    const int c = min(count, counts[g]);
+    for (int k = 0; k < c; ++k) out[(int64_t)level_pos[(int64_t)k * G + g] + within] = x;
 }
 
 // time_step[t] = outer / count[t], in double then rounded (cama_flood_model.py:127,132)
