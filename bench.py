@@ -791,7 +791,7 @@ def main():
     ap.add_argument("--graph", type=int, default=1, help="eager vs HIP-graph line on the small network (0: skip)")
     ap.add_argument("--error-days", type=int, default=1,
                     help="simulated days of the FP32-vs-FP64 error run (0: skip)")
-    ap.add_argument("--layout", default="basin", choices=("basin", "level", "network"),
+    ap.add_argument("--layout", default="auto", choices=("auto", "basin", "level", "network", "keyed"),
                     help="device catchment order")
     ap.add_argument("--reference-surfaces", type=int, default=1,
                     help="also time the plain run with the reference's FP32 surfaces (0: skip)")
@@ -833,6 +833,7 @@ def main():
     # FP32 profile that meets a 1e-5 discharge error against FP64; the same
     # run with the reference's FP32 surfaces is timed beside it
     model = build_model(net, src, 1, device, args.layout, precise=True)
+    layout_used = model.layout
     tuned = model.autotune(use_cache=False) if args.autotune else None
     if tuned:
         log("autotune: " + ", ".join(f"{k}={v['block']}" for k, v in sorted(tuned.items())))
@@ -924,7 +925,7 @@ def main():
                                "water surfaces formed in double (precise_surfaces)",
                    "catchments": net.num_catchments, "bifurcation_paths": net.num_paths,
                    "levees": net.num_levees, "flood_levels": 10, "outer_step_s": 1800.0,
-                   "substeps_timed": int(substeps), "parallelism": "single GPU"},
+                   "substeps_timed": int(substeps), "layout": layout_used, "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "reference_surfaces": reference_surfaces,
         "batched": batched, "basins": basins, "graph": graph,
         "adjoint": adjoint, "forcing": forcing, "batched_full": batched_full,

@@ -161,6 +161,7 @@ def _declare(lib):
         "cmf_accumulate_downstream": (ctypes.c_int, [_i64, i32p, f64p, f64p]),
         "cmf_level_order": (ctypes.c_int, [_i64, i32p, i32p, i32p, i32p, i32p, i64p, _i64, i64p]),
         "cmf_basin_level_order": (ctypes.c_int, [_i64, i32p, i32p, i32p, i32p, i32p, i64p, _i64, i64p]),
+        "cmf_keyed_level_order": (ctypes.c_int, [_i64, i32p, p(ctypes.c_uint8), i32p, i32p, i32p, i64p, _i64, i64p]),
         "cmf_network_create": (ctypes.c_int, [_i64, i32p, _i64, i32p, i32p, _i64, p(_vp)]),
         "cmf_network_create_grouped": (ctypes.c_int, [_i64, i32p, _i64, i32p, i32p, _i64, _i64, i64p, p(_vp)]),
         "cmf_network_num_groups": (_i64, [_vp]),

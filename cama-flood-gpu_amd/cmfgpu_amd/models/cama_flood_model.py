@@ -6,12 +6,17 @@ context, and advances outer steps either through the fused C-ABI schedule
 per-op sequence (``mode="ops"``, the zero-argument registered ops called in
 the order of step_advance, cama_flood_model.py:104-173).
 
-Device layout.  With ``layout="basin"`` (the default) catchments are
+Device layout.  With ``layout="basin"`` catchments are
 renumbered breadth-first from the mouths inside each basin, basin after basin
 (network.basin_level_order), so that the upstream catchments of every
 catchment are one contiguous range next to it; ``layout="level"`` uses the
 breadth-first levels of the whole forest (network.level_order); with
-``layout="network"`` the reference's own order is kept.  With trials,
+``layout="network"`` the reference's own order is kept; ``layout="keyed"``
+uses the forest-wide levels with each level's sibling groups arranged by the
+work they carry (levee catchments, bifurcation path ends, both, neither:
+network.keyed_level_order), so that the levee partition and the path nets run
+in few waves.  ``layout="auto"`` (the default) is "keyed" for one trial with
+levees or bifurcation paths and no groups, "basin" otherwise.  With trials,
 ``trial_layout="inner"`` stores element (trial t, catchment i) at i*T + t
 (all members of a catchment adjacent), ``"major"`` at t*N + i (the
 reference's [T, N]).  ``get``/``set_inputs`` always speak the reference's
@@ -87,7 +92,7 @@ def _group_basins(order, groups: np.ndarray, dn: np.ndarray):
 class CaMaFlood:
     def __init__(self, src: dict, *, opened_modules=("base", "adaptive_time"), precision="float32",
                  device="cuda:0", num_trials: int = 1, batched: dict | None = None,
-                 per_trial_time_step: bool = False, layout: str = "basin", trial_layout: str = "inner",
+                 per_trial_time_step: bool = False, layout: str = "auto", trial_layout: str = "inner",
                  log_buffer_size: int = 20000, step_duration: float = 1800.0, num_sub_steps: int = 1,
                  start_time: datetime = datetime(2000, 1, 1), log_path: str | Path | None = None,
                  mode: str = "fused", gravity: float = 9.8, adaptive_time_factor: float = 0.7,
@@ -95,8 +100,8 @@ class CaMaFlood:
                  precise_surfaces: bool = False, mixed_precision: bool = True):
         if mode not in ("fused", "ops"):
             raise ValueError("mode must be 'fused' or 'ops'")
-        if layout not in ("basin", "level", "network"):
-            raise ValueError("layout must be 'basin', 'level' or 'network'")
+        if layout not in ("auto", "basin", "level", "network", "keyed"):
+            raise ValueError("layout must be 'auto', 'basin', 'level', 'network' or 'keyed'")
         if trial_layout not in ("inner", "major"):
             raise ValueError("trial_layout must be 'inner' or 'major'")
         if num_trials < 1:
@@ -106,7 +111,7 @@ class CaMaFlood:
             raise ValueError("the log module does not support trials (cama_flood_model.py:65-67)")
         if mode == "ops" and per_trial_time_step:
             raise ValueError("per-trial time steps need the fused schedule")
-        if groups is not None and (mode != "fused" or num_trials != 1 or layout != "basin"):
+        if groups is not None and (mode != "fused" or num_trials != 1 or layout not in ("basin", "auto")):
             raise ValueError("groups need the fused schedule, one trial and the basin layout")
         if groups is not None and log_path is not None:
             raise ValueError("the text log is global; read per-group sums with log_sums()")
@@ -137,6 +142,8 @@ class CaMaFlood:
         self.V = int(hf["levee_catchment_idx"].shape[0]) if "levee_catchment_idx" in hf else 0
         self.R = int(hf["reservoir_catchment_idx"].shape[0]) if "reservoir_catchment_idx" in hf else 0
         self.L = self.host.num_flood_levels
+        if layout == "auto":
+            layout = "keyed" if (groups is None and self.T == 1 and (self.V or self.P)) else "basin"
 
         dn = hf["downstream_idx"]
         self.G = 0
@@ -156,6 +163,17 @@ class CaMaFlood:
                 dn_int = self.order.dn.astype(np.int32)
         elif layout == "level":
             self.order = netmod.level_order(dn)
+            self.perm = self.order.perm.astype(np.int64)
+            self.inv = self.order.inv.astype(np.int64)
+            dn_int = self.order.dn.astype(np.int32)
+        elif layout == "keyed":
+            key = np.zeros(self.N, np.uint8)
+            if self.V:
+                key[np.asarray(hf["levee_catchment_idx"], np.int64)] |= 1
+            if self.P:
+                key[np.asarray(hf["bifurcation_catchment_idx"], np.int64)] |= 2
+                key[np.asarray(hf["bifurcation_downstream_idx"], np.int64)] |= 2
+            self.order = netmod.keyed_level_order(dn, key)
             self.perm = self.order.perm.astype(np.int64)
             self.inv = self.order.inv.astype(np.int64)
             dn_int = self.order.dn.astype(np.int32)

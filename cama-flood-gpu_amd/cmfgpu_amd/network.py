@@ -169,6 +169,34 @@ def basin_level_order(downstream_idx) -> BasinLevelOrder:
     return BasinLevelOrder(perm, inv, up_ptr, dn, basin_ptr[: nb.value + 1].copy())
 
 
+def keyed_level_order(downstream_idx, key=None) -> LevelOrder:
+    """Breadth-first levels of the whole forest with each level's sibling
+    groups arranged by work class (cmf_keyed_level_order): `key` holds one
+    small class code per catchment (network order), a group's class is the OR
+    of its members'.  `up_ptr` is None: the upstream ranges stay contiguous
+    but do not follow their owners' order."""
+    d = _i32(downstream_idx)
+    n = d.shape[0]
+    k = None
+    if key is not None:
+        k = np.ascontiguousarray(key, dtype=np.uint8)
+        if k.shape != (n,):
+            raise ValueError(f"key must have one entry per catchment ({n}), got shape {k.shape}")
+    perm = np.empty(n, np.int32)
+    inv = np.empty(n, np.int32)
+    dn = np.empty(n, np.int32)
+    nl = ctypes.c_int64(0)
+    cap = n + 2
+    level_ptr = np.empty(cap, np.int64)
+    _lib.check(_lib.lib().cmf_keyed_level_order(n, _p(d, ctypes.c_int32),
+                                                _p(k, ctypes.c_uint8) if k is not None else None,
+                                                _p(perm, ctypes.c_int32), _p(inv, ctypes.c_int32),
+                                                _p(dn, ctypes.c_int32), ctypes.byref(nl), cap,
+                                                _p(level_ptr, ctypes.c_int64)),
+               "keyed_level_order")
+    return LevelOrder(perm, inv, None, dn, level_ptr[: nl.value + 1].copy())
+
+
 @dataclass
 class Connectivity:
     """Result of finalize_connectivity: the reference order of the catchments."""

@@ -537,7 +537,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_limiter_fwd(View<REAL, STO> v, A
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= v.N) return;
     STO acc = (STO)0;
-    for (int32_t k = v.up_ptr[j]; k < v.up_ptr[j + 1]; ++k) {
+    const int2 rg = v.up_range(j);
+    for (int32_t k = rg.x; k < rg.y; ++k) {
         const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
         if (CONTIG && u == j) continue;
         acc += (STO)a.neg[u];
@@ -566,7 +567,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     const REAL fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
     STO ri = (STO)0, fi = (STO)0;
     double pend = 0.0;   // PENDING: downstream water-surface adjoints of the later sub-step
-    for (int32_t k = v.up_ptr[j]; k < v.up_ptr[j + 1]; ++k) {
+    const int2 rg = v.up_range(j);
+    for (int32_t k = rg.x; k < rg.y; ++k) {
         const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
         if (CONTIG && u == j) continue;
         const REAL lu = a.lim[u];
@@ -743,7 +745,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_route(View<REAL, STO> v, AdjView
     // terms in order, then k's own positive term
     const REAL dt = a.dt;
     STO og_acc = (STO)0;
-    for (int32_t p = v.up_ptr[k]; p < v.up_ptr[k + 1]; ++p) {
+    const int2 rg = v.up_range(k);
+    for (int32_t p = rg.x; p < rg.y; ++p) {
         const int64_t u = CONTIG ? (int64_t)p : (int64_t)v.up_idx[p];
         if (CONTIG && u == k) continue;
         const REAL uq = a.qr[u], uf = a.qf[u];
@@ -1182,7 +1185,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(View<REAL, STO> v, AdjVie
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= v.N) return;
     double s = 0.0;
-    for (int32_t p = v.up_ptr[k]; p < v.up_ptr[k + 1]; ++p) {
+    const int2 rg = v.up_range(k);
+    for (int32_t p = rg.x; p < rg.y; ++p) {
         const int64_t u = CONTIG ? (int64_t)p : (int64_t)v.up_idx[p];
         if (CONTIG && u == k) continue;
         s += (double)a.d_wd[u];

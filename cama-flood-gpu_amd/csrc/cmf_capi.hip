@@ -43,6 +43,7 @@ struct cmf_network {
     int contiguous = 0;
     int device = 0;
     int32_t* up_ptr = nullptr;
+    int2* up_rng = nullptr;           // {start, end} where ranges do not follow their owners
     int32_t* up_idx = nullptr;
     int32_t* ev_ptr = nullptr;
     int32_t* ev_code = nullptr;
@@ -113,7 +114,7 @@ namespace {
 
 void free_network(cmf_network* net) {
     if (!net) return;
-    void* ptrs[] = {net->up_ptr, net->up_idx, net->ev_ptr, net->ev_code, net->ev_other, net->ev1, net->pos_term,
+    void* ptrs[] = {net->up_ptr, net->up_rng, net->up_idx, net->ev_ptr, net->ev_code, net->ev_other, net->ev1, net->pos_term,
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
                     net->path_neg, net->path_net, net->path_raw, net->partials, net->cgroup,
                     net->blk_rec_id, net->level_rec, net->level_pos, net->gblk_ptr, net->lflags, net->lim_list,
@@ -154,7 +155,7 @@ int64_t blocks_for(int64_t count) { return (count + kBlock - 1) / kBlock; }
 constexpr int64_t kLocalBlock = 512;
 
 // Flags of the block-local schedule for blocks of B catchments (contiguous
-// upstream ranges `up_ptr`, bifurcation events `ev_ptr`), and the catchments
+// upstream ranges [up_ptr, up_end), bifurcation events `ev_ptr`), and the catchments
 // it leaves to the limiter and stage list passes (ascending).
 //   LL(j): j's upstream range lies in j's block and j is no path end: its
 //          limiter comes out of the block's LDS;
@@ -166,17 +167,18 @@ constexpr int64_t kLocalBlock = 512;
 // downstream catchment is not SL (the stage list), its limiter when LL and
 // it, its downstream catchment or one of its upstream catchments is not SL.
 void local_tables(int64_t n, const int32_t* dn, const std::vector<int32_t>& up_ptr,
-                  const std::vector<int32_t>& ev_ptr, int64_t B, std::vector<uint8_t>& flags,
+                  const std::vector<int32_t>& up_end, const std::vector<int32_t>& ev_ptr, int64_t B,
+                  std::vector<uint8_t>& flags,
                   std::vector<int32_t>& lim_list, std::vector<int32_t>& stage_list) {
     std::vector<uint8_t> ll((size_t)n), sl((size_t)n);
     for (int64_t j = 0; j < n; ++j) {
-        const int64_t b0 = up_ptr[j], b1 = up_ptr[j + 1];
+        const int64_t b0 = up_ptr[j], b1 = up_end[j];
         const bool in_block = (b0 == b1) || (b0 / B == j / B && (b1 - 1) / B == j / B);
         ll[j] = in_block && ev_ptr[j] == ev_ptr[j + 1];
     }
     for (int64_t j = 0; j < n; ++j) {
         bool ok = ll[j] && (dn[j] == j || ((int64_t)dn[j] / B == j / B && ll[dn[j]]));
-        for (int64_t u = up_ptr[j]; ok && u < up_ptr[j + 1]; ++u)
+        for (int64_t u = up_ptr[j]; ok && u < up_end[j]; ++u)
             if (u != j && !ll[u]) ok = false;
         sl[j] = ok;
     }
@@ -191,7 +193,7 @@ void local_tables(int64_t n, const int32_t* dn, const std::vector<int32_t>& up_p
         const bool stage_reader = !sl[j] || (!mouth && !sl[dn[j]]);
         if (stage_reader) f |= kQrawG;
         bool up_not_sl = false;
-        for (int64_t u = up_ptr[j]; u < up_ptr[j + 1]; ++u)
+        for (int64_t u = up_ptr[j]; u < up_end[j]; ++u)
             if (u != j && !sl[u]) up_not_sl = true;
         if (ll[j] && (stage_reader || up_not_sl)) f |= kLimG;
         flags[j] = f;
@@ -323,6 +325,7 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.ordered = false;
     v.up_ptr = net->up_ptr;
     v.up_idx = net->contiguous ? nullptr : net->up_idx;
+    v.up_rng = net->up_rng;
     v.ev_ptr = net->ev_ptr;
     v.ev_code = net->ev_code;
     v.ev_other = net->ev_other;
@@ -1401,23 +1404,48 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
         for (int64_t i = 0; i < n; ++i)
             if (downstream_idx[i] != i) up_idx[fill[downstream_idx[i]]++] = (int32_t)i;
     }
-    // Contiguous layout: walking the catchments in order, the upstream
-    // catchments of each are the next ones in index order.  A mouth whose own
-    // index comes next claims itself as well (the basin-by-basin level order
-    // puts each mouth right after its upstream range); kernels skip the owner.
-    std::vector<int32_t> cup((size_t)n + 1, 0);
-    bool contiguous = true;
+    // Contiguous layout: the upstream catchments of every catchment are
+    // consecutive indices.  Monotone (the level orders): walking the
+    // catchments in order, the upstream catchments of each are the next ones,
+    // so one array of starts holds every range (k's ends at the start of
+    // k + 1); a mouth whose own index comes next claims itself as well (the
+    // basin-by-basin level order puts each mouth right after its upstream
+    // range) and kernels skip the owner.  Otherwise (the keyed level order,
+    // whose ranges are not in the order of their owners) starts and ends.
+    std::vector<int32_t> cup((size_t)n + 1, 0), cend;
+    bool contiguous = true, monotone = true;
     {
         int64_t pos = 0;
-        for (int64_t k = 0; k < n && contiguous; ++k) {
+        for (int64_t k = 0; k < n && monotone; ++k) {
             cup[k] = (int32_t)pos;
             for (int32_t q = up_ptr[k]; q < up_ptr[k + 1]; ++q) {
-                if (up_idx[q] != pos) { contiguous = false; break; }
+                if (up_idx[q] != pos) { monotone = false; break; }
                 ++pos;
             }
             if (downstream_idx[k] == k && pos == k) ++pos;
         }
         cup[n] = (int32_t)pos;
+    }
+    if (!monotone) {
+        cup.assign((size_t)n + 1, 0);
+        cend.assign((size_t)n, 0);
+        for (int64_t k = 0; k < n && contiguous; ++k) {
+            const int32_t b = up_ptr[k], e = up_ptr[k + 1];
+            const int32_t first = e > b ? up_idx[b] : 0;
+            for (int32_t q = b; q < e; ++q)
+                if (up_idx[q] != first + (q - b)) { contiguous = false; break; }
+            cup[k] = first;
+            cend[k] = first + (e - b);
+        }
+    }
+    // the end of each range on the host, in every representation
+    std::vector<int32_t> range_end((size_t)n);
+    if (contiguous && !monotone) range_end = cend;
+    else for (int64_t k = 0; k < n; ++k) range_end[k] = (contiguous ? cup : up_ptr)[k + 1];
+    std::vector<int2> rng;
+    if (contiguous && !monotone) {
+        rng.resize((size_t)n);
+        for (int64_t k = 0; k < n; ++k) rng[(size_t)k] = make_int2(cup[k], cend[k]);
     }
     if (contiguous) up_ptr.swap(cup);
     // bifurcation events per catchment, ascending path, upstream end first
@@ -1448,7 +1476,7 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     std::vector<uint8_t> lflags;
     std::vector<int32_t> lim_list, stage_list;
     const bool local = contiguous && num_groups == 0 && num_trials == 1;
-    if (local) local_tables(n, downstream_idx, up_ptr, ev_ptr, kLocalBlock, lflags, lim_list, stage_list);
+    if (local) local_tables(n, downstream_idx, up_ptr, range_end, ev_ptr, kLocalBlock, lflags, lim_list, stage_list);
     cmf_network* net = new cmf_network();
     net->n = n;
     net->T = num_trials;
@@ -1463,6 +1491,7 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     // for the block-local pass ahead of its stage list
     net->partial_blocks = std::max<int64_t>((n * num_trials + 63) / 64, net->nblk) + (n + kLocalBlock - 1) / kLocalBlock + 2;
     if ((rc = upload((void**)&net->up_ptr, up_ptr.data(), up_ptr.size() * 4)) ||
+        (contiguous && !monotone && (rc = upload((void**)&net->up_rng, rng.data(), rng.size() * sizeof(int2)))) ||
         (rc = upload((void**)&net->up_idx, up_idx.data(), up_idx.size() * 4)) ||
         (rc = upload((void**)&net->ev_ptr, ev_ptr.data(), ev_ptr.size() * 4)) ||
         (rc = upload((void**)&net->ev_code, ev_code.data(), ev_code.size() * 4)) ||

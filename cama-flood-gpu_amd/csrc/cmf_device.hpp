@@ -138,10 +138,13 @@ struct View {
     const uint8_t *is_reservoir, *is_dam_up, *is_dam_related;
     double* log_sums[CMF_LOG_COUNT];
     // network
-    // Upstream catchments of j: [up_ptr[j], up_ptr[j+1]) directly (up_idx ==
+    // Upstream catchments of j: the range up_range(j) directly (up_idx ==
     // nullptr: contiguous ranges, where a mouth's range may end with the mouth
-    // itself) or through up_idx.
+    // itself) or through up_idx.  Where the ranges follow their owners (and
+    // for up_idx) they are [up_ptr[j], up_ptr[j+1]); otherwise up_rng[j]
+    // holds {start, end}.
     const int32_t *up_ptr, *up_idx;
+    const int2* up_rng;
     // bifurcation events per catchment: ev_code = 2 path + role (0: the
     // catchment is the path's upstream end, 1: its downstream end), ev_other
     // the path's other end
@@ -180,6 +183,9 @@ struct View {
     __device__ __forceinline__ int64_t el(int64_t t, int64_t i) const { return t * ts + i * cs; }
     __device__ __forceinline__ int64_t pel(int64_t t, int64_t p) const { return t * pts + p * pps; }
     __host__ __device__ __forceinline__ bool has(uint32_t bit) const { return (flags & bit) != 0; }
+    __device__ __forceinline__ int2 up_range(int64_t j) const {
+        return up_rng ? up_rng[j] : make_int2(up_ptr[j], up_ptr[j + 1]);
+    }
     __host__ __device__ __forceinline__ bool bat(uint32_t bit) const { return (batched & bit) != 0; }
     // parameter read: per-trial element when batched, shared otherwise
     template <class X>

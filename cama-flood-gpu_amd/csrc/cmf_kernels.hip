@@ -279,7 +279,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_outflow(VT v, int substep, int st
 template <class REAL, class STO, bool CONTIG, class VT>
 __device__ __forceinline__ STO outgoing_terms(const VT& v, int64_t t, int64_t j) {
     STO acc = (STO)0;
-    const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
+    const int2 rg = v.up_range(j);
+    const int32_t b = rg.x, en = rg.y;
     for (int32_t k = b; k < en; ++k) {
         const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
         if (CONTIG && u == j) continue;  // a mouth's range ends with the mouth
@@ -680,7 +681,8 @@ __global__ __launch_bounds__(kBlock) void k_inflow_sum(View<REAL, STO> v) {
     if (g >= v.N * v.T) return;
     int64_t t, j;
     v.split_c(g, t, j);
-    const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
+    const int2 rg = v.up_range(j);
+    const int32_t b = rg.x, en = rg.y;
     if (b == en) return;
     const int64_t e = v.el(t, j);
     STO ri = v.river_inflow[e], fi = v.flood_inflow[e];
@@ -1262,15 +1264,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(VT v, int substep, i
         if (a1) ev1 = v.ev1[j1];
     }
     if (a0) {
-        b0 = v.up_ptr[j0];
-        n0 = v.up_ptr[j0 + 1] - b0;
+        const int2 r0 = v.up_range(j0);
+        b0 = r0.x;
+        n0 = r0.y - r0.x;
         p0 = v.pos_term[e0];
         rs0 = v.river_storage[e0];
         fs0 = v.flood_storage[e0];
     }
     if (a1) {
-        b1 = v.up_ptr[j1];
-        n1 = v.up_ptr[j1 + 1] - b1;
+        const int2 r1 = v.up_range(j1);
+        b1 = r1.x;
+        n1 = r1.y - r1.x;
         p1 = v.pos_term[e1];
         rs1 = v.river_storage[e1];
         fs1 = v.flood_storage[e1];
@@ -1348,7 +1352,8 @@ __device__ __forceinline__ void stage_fused_one(const VT& v, int64_t t, int64_t 
     const int64_t d = v.dn[j];
     const REAL l_own = src.lim(v, e, j);
     const REAL qr = src.qr(v, e, j), qf = src.qf(v, e, j);
-    const int32_t b = v.up_ptr[j], en = v.up_ptr[j + 1];
+    const int2 rg = v.up_range(j);
+    const int32_t b = rg.x, en = rg.y;
     const STO rs = v.river_storage[e], fs = v.flood_storage[e];
     // without the levee module every catchment holds +0.0 protected
     // storage after the first sub-step's stage pass
@@ -1496,7 +1501,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_fused_local(View<REAL, STO> v, in
         // outgoing_terms (upstream negative terms in ascending index, then
         // the own positive term), then limit_of
         STO acc = (STO)0;
-        const int32_t b0 = v.up_ptr[j], b1 = v.up_ptr[j + 1];
+        const int2 rg = v.up_range(j);
+        const int32_t b0 = rg.x, b1 = rg.y;
         for (int32_t k = b0; k < b1; ++k) {
             if (k == j) continue;   // a mouth's range ends with the mouth
             acc += (STO)sh_neg[k - base];

@@ -11,6 +11,8 @@
 //   cmf_level_order            breadth-first levels from the mouths; see the
 //                              comment on the function.
 //   cmf_basin_level_order      the same levels laid out basin by basin.
+//   cmf_keyed_level_order      the forest-wide levels with each level's
+//                              sibling groups arranged by the work they carry.
 
 #include "cmf_internal.hpp"
 
@@ -357,6 +359,105 @@ int cmf_basin_level_order(int64_t n, const int32_t* downstream_idx, int32_t* out
         for (int64_t b = 0; b < nb; ++b) out_basin_ptr[b] = bstart[b];
         out_basin_ptr[nb] = n;
     }
+    return CMF_OK;
+}
+
+// Breadth-first levels of the whole forest, each level's sibling groups (the
+// upstream catchments of one catchment, in ascending network index) arranged
+// by the class of work they carry.  key[i] (network order; NULL: all 0) is a
+// small class code of catchment i (the model passes bit 0 for a levee, bit 1
+// for a bifurcation path end); a group's class is the OR of its members'
+// (a mouth is a group of its own).  Each level lists its groups class by
+// class, stable inside a class, in the order
+//     1, 3, 2, 0, then every other class value ascending    (even depth)
+// and the reverse of that on odd depths, so that the class a level ends with
+// is the one the next level in memory (one nearer the mouths: deepest level
+// first) starts with.  A wave of consecutive catchments then mostly holds one
+// class, and the kernels' per-class work (the levee partition, the path
+// nets) runs in few waves.  Level d + 1 is formed from level d's order as in
+// cmf_level_order, so every catchment's upstream catchments are still one
+// contiguous range in ascending network index, but the ranges no longer
+// follow the order of their owners (there is no monotone up_ptr: the device
+// network keeps a start and an end per catchment).
+//
+//   out_perm, out_inv, out_dn, out_num_levels, out_level_ptr: as cmf_level_order
+int cmf_keyed_level_order(int64_t n, const int32_t* downstream_idx, const uint8_t* key, int32_t* out_perm,
+                          int32_t* out_inv, int32_t* out_dn, int64_t* out_num_levels,
+                          int64_t level_ptr_capacity, int64_t* out_level_ptr) {
+    if (n < 0 || n > INT32_MAX - 1 || (n > 0 && (!downstream_idx || !out_perm || !out_inv || !out_dn)) ||
+        !out_num_levels)
+        return set_error(CMF_ERR_INVALID, "cmf_keyed_level_order: bad arguments");
+    int rc = cmf_check_flow_direction(n, downstream_idx);
+    if (rc != CMF_OK) return rc;
+    std::vector<int32_t> cptr((size_t)n + 1, 0), cidx;
+    for (int64_t i = 0; i < n; ++i) if (downstream_idx[i] != i) cptr[downstream_idx[i] + 1]++;
+    for (int64_t i = 0; i < n; ++i) cptr[i + 1] += cptr[i];
+    cidx.resize((size_t)cptr[n]);
+    {
+        std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
+        for (int64_t i = 0; i < n; ++i)
+            if (downstream_idx[i] != i) cidx[fill[downstream_idx[i]]++] = (int32_t)i;
+    }
+    // class rank: 1, 3, 2, 0, then 4..255 ascending
+    int rank[256];
+    {
+        const int lead[4] = {1, 3, 2, 0};
+        int r = 0;
+        for (int c : lead) rank[c] = r++;
+        for (int c = 4; c < 256; ++c) rank[c] = r++;
+    }
+    auto cls = [&](int32_t i) -> int { return key ? (int)key[i] : 0; };
+    // a group: [begin, end) in `members`; sorted by (rank, position) per level
+    struct Grp { int64_t begin, end; int c; };
+    std::vector<std::vector<int32_t>> levels;
+    std::vector<int32_t> members;
+    std::vector<Grp> groups;
+    // level 0: each mouth a group of its own, in ascending network index
+    for (int64_t i = 0; i < n; ++i) {
+        if (downstream_idx[i] != i) continue;
+        groups.push_back({(int64_t)members.size(), (int64_t)members.size() + 1, cls((int32_t)i)});
+        members.push_back((int32_t)i);
+    }
+    int64_t placed = 0;
+    for (int64_t d = 0; !groups.empty(); ++d) {
+        const bool odd = (d & 1) != 0;
+        std::stable_sort(groups.begin(), groups.end(), [&](const Grp& a, const Grp& b) {
+            return odd ? rank[a.c] > rank[b.c] : rank[a.c] < rank[b.c];
+        });
+        std::vector<int32_t> level;
+        level.reserve((size_t)(members.size()));
+        for (const Grp& g : groups)
+            for (int64_t k = g.begin; k < g.end; ++k) level.push_back(members[(size_t)k]);
+        placed += (int64_t)level.size();
+        if (placed > n) return set_error(CMF_ERR_INVALID, "cmf_keyed_level_order: network is not a forest");
+        // the next level's groups: the upstream catchments of this level's, in its order
+        members.clear();
+        groups.clear();
+        for (int32_t u : level) {
+            if (cptr[u + 1] == cptr[u]) continue;
+            int c = 0;
+            const int64_t b = (int64_t)members.size();
+            for (int32_t q = cptr[u]; q < cptr[u + 1]; ++q) {
+                members.push_back(cidx[q]);
+                c |= cls(cidx[q]);
+            }
+            groups.push_back({b, (int64_t)members.size(), c});
+        }
+        levels.push_back(std::move(level));
+    }
+    if (placed != n) return set_error(CMF_ERR_INVALID, "cmf_keyed_level_order: network is not a forest");
+    const int64_t num_levels = (int64_t)levels.size();
+    *out_num_levels = num_levels;
+    if (out_level_ptr && level_ptr_capacity < num_levels + 1)
+        return set_error(CMF_ERR_INVALID, "cmf_keyed_level_order: level_ptr buffer too small");
+    int64_t pos = 0;
+    for (int64_t d = num_levels - 1; d >= 0; --d) {
+        if (out_level_ptr) out_level_ptr[num_levels - 1 - d] = pos;
+        for (int32_t c : levels[(size_t)d]) out_perm[pos++] = c;
+    }
+    if (out_level_ptr) out_level_ptr[num_levels] = n;
+    for (int64_t k = 0; k < n; ++k) out_inv[out_perm[k]] = (int32_t)k;
+    for (int64_t k = 0; k < n; ++k) out_dn[k] = out_inv[downstream_idx[out_perm[k]]];
     return CMF_OK;
 }
 

@@ -263,6 +263,17 @@ int cmf_basin_level_order(int64_t n, const int32_t* downstream_idx, int32_t* out
                           int32_t* out_inv, int32_t* out_up_ptr, int32_t* out_dn,
                           int64_t* out_num_basins, int64_t basin_ptr_capacity,
                           int64_t* out_basin_ptr);
+/* Breadth-first levels of the whole forest with each level's sibling groups
+ * arranged by work class (`key`, one byte per catchment in network order,
+ * NULL for none: the model passes bit 0 for a levee catchment and bit 1 for
+ * a bifurcation path end), classes alternating direction level by level, so
+ * that waves of consecutive catchments mostly hold one class.  Upstream
+ * ranges stay contiguous but are not monotone (no up_ptr output).  No
+ * reference counterpart. */
+int cmf_keyed_level_order(int64_t n, const int32_t* downstream_idx, const uint8_t* key,
+                          int32_t* out_perm, int32_t* out_inv, int32_t* out_dn,
+                          int64_t* out_num_levels, int64_t level_ptr_capacity,
+                          int64_t* out_level_ptr);
 
 /* -------------------------------------------------------------- network */
 /* Build the device network context from host copies of the indices.

@@ -9,6 +9,7 @@ TEST INFRASTRUCTURE ONLY (the checker for cmfgpu_amd.network).  Small inputs.
                          default kind leaves that tie order unspecified)
   level_order            breadth-first levels from the mouths (the device layout)
   basin_level_order      the same levels laid out basin by basin
+  keyed_level_order      the forest-wide levels, sibling groups by work class
 """
 from __future__ import annotations
 
@@ -127,3 +128,49 @@ def basin_level_order(downstream_idx):
         up_ptr.append(end)
     return (np.array(perm, np.int32), np.array(inv, np.int32), np.array(up_ptr, np.int32),
             np.array(dn, np.int32), np.array(basin_ptr, np.int64))
+
+
+_CLASS_LEAD = (1, 3, 2, 0)
+
+
+def _class_rank(c):
+    return _CLASS_LEAD.index(c) if c in _CLASS_LEAD else c
+
+
+def keyed_level_order(downstream_idx, key=None):
+    """Forest-wide breadth-first levels (the layout of cmf_keyed_level_order):
+    level 0 lists the mouths (each its own group) in ascending index, level
+    d + 1 the children of level d's catchments in that level's order, one group
+    per parent (children ascending).  A group's class is the OR of its
+    members' keys; each level lists its groups by class rank (1, 3, 2, 0, then
+    the other values ascending), ties in formation order, reversed on odd
+    depths; levels are emitted deepest first."""
+    n = len(downstream_idx)
+    key = [0] * n if key is None else [int(k) for k in key]
+    children = [[] for _ in range(n)]
+    for i, d in enumerate(downstream_idx):
+        if d != i:
+            children[d].append(i)
+    groups = [[m] for m in range(n) if downstream_idx[m] == m]
+    levels = []
+    depth = 0
+    while groups:
+        def cls(g):
+            c = 0
+            for i in g:
+                c |= key[i]
+            return c
+        ranked = sorted(range(len(groups)), key=lambda q: (_class_rank(cls(groups[q])), q))
+        if depth % 2 == 1:
+            # reverse class order, formation order kept inside a class
+            ranked = sorted(range(len(groups)), key=lambda q: (-_class_rank(cls(groups[q])), q))
+        level = [i for q in ranked for i in groups[q]]
+        levels.append(level)
+        groups = [children[u] for u in level if children[u]]
+        depth += 1
+    perm = [i for lev in reversed(levels) for i in lev]
+    inv = [0] * n
+    for k, i in enumerate(perm):
+        inv[i] = k
+    dn = [inv[downstream_idx[perm[k]]] for k in range(n)]
+    return np.array(perm, np.int32), np.array(inv, np.int32), np.array(dn, np.int32), len(levels)

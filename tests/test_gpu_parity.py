@@ -27,7 +27,7 @@ def _compare(model, orc, fields):
         assert msg is None, f"{name}: {msg}"
 
 
-@pytest.mark.parametrize("layout", ["basin", "level", "network"])
+@pytest.mark.parametrize("layout", ["basin", "level", "network", "keyed"])
 @pytest.mark.parametrize("mode", ["fused", "ops"])
 def test_plain_bitexact(layout, mode):
     net, src = small_source(paths=0, levee=0.0, reservoirs=0)
@@ -41,7 +41,7 @@ def test_plain_bitexact(layout, mode):
     _compare(model, orc, STATE_FIELDS)
 
 
-@pytest.mark.parametrize("layout", ["basin", "level", "network"])
+@pytest.mark.parametrize("layout", ["basin", "level", "network", "keyed"])
 @pytest.mark.parametrize("mode", ["fused", "ops"])
 def test_full_physics_bitexact(layout, mode):
     net, src = small_source()
@@ -55,7 +55,7 @@ def test_full_physics_bitexact(layout, mode):
                                          "reservoir_total_inflow"))
 
 
-@pytest.mark.parametrize("layout", ["basin", "network"])
+@pytest.mark.parametrize("layout", ["basin", "network", "keyed"])
 @pytest.mark.parametrize("mode", ["fused", "ops"])
 def test_precise_surfaces_bitexact(layout, mode):
     """FP32 with water surfaces formed in double (CMF_PRECISE_SURFACE)
@@ -286,7 +286,7 @@ NO_RES = ("base", "adaptive_time", "bifurcation", "levee")
 
 
 @pytest.mark.parametrize("case", ["plain", "paths_levees", "log", "precise", "fp64", "fp32sto", "network_levee",
-                                  "reservoirs"])
+                                  "reservoirs", "keyed"])
 def test_schedules_bitexact(case):
     """The block-local schedule (one pass completing whole blocks, then the
     limiter and stage lists) and the three-pass schedule against the oracle,
@@ -310,11 +310,16 @@ def test_schedules_bitexact(case):
         kw = okw = dict(mixed_precision=False)
     if case == "reservoirs":
         mods = FULL
+    if case == "keyed":
+        # ranges in another order than their owners (start and end arrays)
+        kw = dict(layout="keyed")
     rs = np.array(src["river_storage"], dtype=np.float64)
     lid = np.asarray(src["levee_catchment_idx"])
     rs[lid] *= np.random.default_rng(3).uniform(1.0, 60.0, lid.size)
     src["river_storage"] = rs
-    models = {name: _model(src, mods, **kw) for name in ("block_local", "three_pass")}
+    # the basin layout unless the case names another (the block-local
+    # schedule completes most catchments inside their block there)
+    models = {name: _model(src, mods, **{"layout": "basin", **kw}) for name in ("block_local", "three_pass")}
     for name, m in models.items():
         m.set_schedule(name)
     if case == "network_levee":

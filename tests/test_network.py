@@ -147,3 +147,75 @@ def test_find_indices_and_accumulate():
     d = np.array([2, 2, 3, 3], np.int32)
     out = network.accumulate_downstream(d, np.array([1.0, 2.0, 4.0, 8.0]))
     assert out.tolist() == [1.0, 2.0, 7.0, 15.0]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_keyed_level_order_matches_restatement(seed):
+    ids, down = random_forest(400, seed, p_outlet=0.05)
+    c = network.finalize_connectivity(ids, down)
+    d = c.downstream_idx
+    rng = np.random.default_rng(100 + seed)
+    key = rng.choice(np.array([0, 0, 0, 1, 2, 3, 6], np.uint8), size=d.shape[0])
+    ko = network.keyed_level_order(d, key)
+    perm, inv, dn, levels = ordering.keyed_level_order(d.tolist(), key.tolist())
+    assert np.array_equal(ko.perm, perm)
+    assert np.array_equal(ko.inv, inv)
+    assert np.array_equal(ko.dn, dn)
+    assert ko.num_levels == levels
+    # no key: the plain forest-wide level order
+    lo = network.level_order(d)
+    assert np.array_equal(network.keyed_level_order(d).perm, lo.perm)
+
+
+def test_keyed_level_order_on_a_small_tree():
+    """Mouth 6 with upstream catchments 3 (levee), 4, 5 (path end); 3 drains
+    0 and 1, 5 drains 2 (levee).  Depth 1 (odd, classes in the order 0, 2, 3,
+    1) holds one group, {3, 4, 5}, of class 1 | 2 = 3, kept whole and in
+    network order.  Depth 2 (even: 1, 3, 2, 0) is formed as {0, 1} (from 3)
+    then {2} (from 5), and lists {2} (class 1) before {0, 1} (class 0).
+    Levels deepest first."""
+    d = np.array([3, 3, 5, 6, 6, 6, 6], np.int32)
+    key = np.array([0, 0, 1, 1, 0, 2, 0], np.uint8)
+    ko = network.keyed_level_order(d, key)
+    assert ko.perm.tolist() == [2, 0, 1, 3, 4, 5, 6]
+    assert ko.level_ptr.tolist() == [0, 3, 6, 7]
+    # without keys: breadth-first formation order
+    assert network.keyed_level_order(d).perm.tolist() == [0, 1, 2, 3, 4, 5, 6]
+
+
+def test_keyed_level_order_properties_at_config1_size():
+    """Every catchment's upstream catchments are one contiguous range in
+    ascending network index (the kernels' summation order), downstream(k) > k,
+    and the levee catchments fill far fewer 64-catchment waves than in the
+    plain level order."""
+    net = synthetic.build_config(1)
+    f = net.fields
+    d = f["downstream_idx"]
+    n = d.shape[0]
+    key = np.zeros(n, np.uint8)
+    key[f["levee_catchment_idx"]] |= 1
+    key[f["bifurcation_catchment_idx"]] |= 2
+    key[f["bifurcation_downstream_idx"]] |= 2
+    ko = network.keyed_level_order(d, key)
+    k = np.arange(n)
+    mouth = ko.dn == k
+    assert np.all(ko.dn[~mouth] > k[~mouth])
+    assert np.array_equal(ko.inv[ko.perm], k)
+    up = k[~mouth]
+    owner = ko.dn[up]
+    order = np.lexsort((up, owner))
+    up, owner = up[order], owner[order]
+    first = np.r_[True, owner[1:] != owner[:-1]]
+    gid = np.cumsum(first) - 1
+    rank_in_group = np.arange(up.size) - np.flatnonzero(first)[gid]
+    assert np.array_equal(up, up[first][gid] + rank_in_group)        # consecutive internal indices
+    net_order = ko.perm[up]
+    same = ~first[1:]
+    assert np.all(net_order[1:][same] > net_order[:-1][same])         # ascending network index
+
+    def levee_waves(perm):
+        lev = (key[perm] & 1).astype(bool)[: (n // 64) * 64].reshape(-1, 64)
+        return lev.any(1).mean()
+    plain = levee_waves(network.level_order(d).perm)
+    keyed = levee_waves(ko.perm)
+    assert plain > 0.9 and keyed < 0.3, (plain, keyed)
