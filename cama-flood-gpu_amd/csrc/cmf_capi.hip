@@ -85,6 +85,9 @@ struct cmf_network {
     int64_t level_pos_cap = 0;
     hipEvent_t order_copied = nullptr;
     std::vector<int64_t> sub_blocks, level_off;
+    // plain adaptive advance: the event that marks the sub-step count's copy
+    // to host_counts (the first sub-step is enqueued before it is waited on)
+    hipEvent_t count_ready = nullptr;
     // captured sub-step loops (cmf_hip_advance_graph): one executable graph
     // per sub-step count, valid for the field table they were captured with
     hipStream_t capture_stream = nullptr;
@@ -124,6 +127,7 @@ void free_network(cmf_network* net) {
     if (net->host_counts) (void)hipHostFree(net->host_counts);
     if (net->host_level_pos) (void)hipHostFree(net->host_level_pos);
     if (net->order_copied) (void)hipEventDestroy(net->order_copied);
+    if (net->count_ready) (void)hipEventDestroy(net->count_ready);
     for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
     if (net->capture_stream) (void)hipStreamDestroy(net->capture_stream);
     if (net->adj_scratch) (void)hipFree(net->adj_scratch);
@@ -832,9 +836,18 @@ const int4* level_table(const cmf_network* net, const View<REAL, STO>& v, int su
 // One fused sub-step: outflow [+ bifurcation paths], [reservoir], limiters,
 // stage (inflow sums, limited flows, bifurcation nets, storage, flood stage
 // [, levee stage]).
+//
+// dev_last: a plain run's first sub-step, enqueued before the host has read
+// the outer step's count (early_advance): whether it is also the last is
+// read on the device.
 template <class REAL, class STO>
 int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int substep,
-                  int store_inflow, void* stream) {
+                  int store_inflow, void* stream, bool dev_last = false) {
+    if (dev_last) {
+        PlainView<REAL, STO, true> pv;
+        static_cast<View<REAL, STO>&>(pv) = v;
+        return three_pass_substep<REAL, STO, PlainView<REAL, STO, true>>(net, pv, substep, 0, stream);
+    }
     if (net->local_block > 0 && net->schedule == CMF_SCHEDULE_BLOCK_LOCAL && net->G == 0 && v.T == 1 && v.R == 0 && v.L == 10 &&
         (v.P == 0 || v.LB == 5) && !v.tape_q_r && !v.has(CMF_PER_TRIAL_DT) && net->contiguous) {
         const bool levee = v.has(CMF_HAS_LEVEE);
@@ -905,6 +918,13 @@ int run_substep(cmf_network* net, const cmf_fields* f, int substep, int store_in
 template <class REAL>
 __global__ void k_prepare(REAL* outer_time_step, double outer) {
     if (threadIdx.x == 0 && blockIdx.x == 0) outer_time_step[0] = (REAL)outer;
+}
+// the same and the n sub-step counts cleared, in one launch
+template <class REAL>
+__global__ void k_prepare_counts(REAL* outer_time_step, double outer, int32_t* counts, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) outer_time_step[0] = (REAL)outer;
+    if (t < n) counts[t] = 0;
 }
 template <class REAL>
 __global__ void k_set_dt_fixed(REAL* time_step, int64_t n, double value) {
@@ -1040,6 +1060,40 @@ int build_level_tables(cmf_network* net, const int32_t* counts, int count, hipSt
     return CMF_OK;
 }
 
+// A plain adaptive advance need not wait for its sub-step count before
+// launching its first sub-step: one trial, ungrouped, shared parameters, the
+// three-pass schedule, and no pass that runs on the generic view (reservoirs,
+// the separate levee pass, the log's per-sub-step partial folding).
+template <class REAL, class STO>
+bool early_ok(const cmf_network* net, const View<REAL, STO>& v) {
+    const bool plain = net->G == 0 && v.T == 1 && v.batched == 0 && !v.has(CMF_PER_TRIAL_DT);
+    const bool lev_separate = v.has(CMF_HAS_LEVEE) && !v.levee_index;
+    const bool local = net->local_block > 0 && net->schedule == CMF_SCHEDULE_BLOCK_LOCAL;
+    return plain && !local && v.R == 0 && !lev_separate && !v.has(CMF_HAS_LOG) && !v.tape_q_r;
+}
+
+// The sub-step loop of a plain adaptive advance whose count is on its way to
+// the host (the copy is enqueued on `s`): the first sub-step, which every
+// outer step has, is enqueued at once (whether it is also the last is read
+// on the device, where its stores happen), then the host waits for the count,
+// which arrives while the GPU runs that sub-step, and enqueues the rest.  The
+// GPU does not idle on the count's round trip.  current_step was set to
+// count - 1 with the time step (k_set_dt).
+template <class REAL, class STO>
+int early_advance(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int32_t* out_count,
+                  hipStream_t s) {
+    if (!net->count_ready) CMF_HIP_CHECK(hipEventCreateWithFlags(&net->count_ready, hipEventDisableTiming));
+    CMF_HIP_CHECK(hipEventRecord(net->count_ready, s));
+    int rc = fused_substep(net, v, f, 0, 0, s, true);
+    if (rc) return rc;
+    CMF_HIP_CHECK(hipEventSynchronize(net->count_ready));
+    const int count = std::max(1, (int)net->host_counts[0]);
+    for (int k = 1; k < count; ++k)
+        if ((rc = fused_substep(net, v, f, k, (k == count - 1) ? 1 : 0, s))) return rc;
+    if (out_count) *out_count = count;
+    return CMF_OK;
+}
+
 template <class REAL, class STO>
 int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_count,
                 int32_t* out_count, void* stream, bool use_graph) {
@@ -1057,8 +1111,8 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
         return set_error(CMF_ERR_INTERNAL, "advance: partial buffer too small");
     if (v.has(CMF_HAS_ADAPTIVE_TIME)) {
         if ((rc = require({NEED(max_sub_steps), NEED(outer_time_step)}, "advance"))) return rc;
-        hipLaunchKernelGGL((k_prepare<REAL>), dim3(1), dim3(64), 0, s, static_cast<REAL*>(f->outer_time_step), outer);
-        hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, f->max_sub_steps, nslots);
+        hipLaunchKernelGGL((k_prepare_counts<REAL>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
+                           static_cast<REAL*>(f->outer_time_step), outer, f->max_sub_steps, nslots);
         if (net->G > 0) {
             hipLaunchKernelGGL((k_cfl_grouped<REAL, STO>), dim3((unsigned)((net->nblk + kCflRecords - 1) / kCflRecords)),
                                dim3(kGroupRecord), 0, s, v, (int64_t)net->nblk);
@@ -1066,11 +1120,15 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
             const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 512);
             hipLaunchKernelGGL((k_cfl<REAL, STO>), dim3((unsigned)chunks, (unsigned)v.T), dim3(kBlock), 0, s, v);
         }
+        // the early advance leaves current_step at count - 1 here (one slot)
+        const bool early = !use_graph && early_ok(net, v);
         hipLaunchKernelGGL((k_set_dt<REAL>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
-                           static_cast<REAL*>(f->time_step), f->max_sub_steps, nslots, outer);
+                           static_cast<REAL*>(f->time_step), f->max_sub_steps, nslots, outer,
+                           early ? f->current_step : nullptr);
         CMF_HIP_CHECK(hipGetLastError());
         CMF_HIP_CHECK(hipMemcpyAsync(net->host_counts, f->max_sub_steps, sizeof(int32_t) * nslots,
                                      hipMemcpyDeviceToHost, s));
+        if (early) return early_advance(net, v, f, out_count, s);
         CMF_HIP_CHECK(hipStreamSynchronize(s));
         count = 1;
         for (int64_t t = 0; t < nslots; ++t) count = std::max(count, (int)net->host_counts[t]);

@@ -264,7 +264,10 @@ struct View {
 // time steps: every per-trial and per-group index folded away at compile
 // time (element = catchment, shared parameters, the one time-step slot).
 // Same layout as View; kernels templated on the view type take either.
-template <class REAL, class STO>
+// DEVLAST: whether the sub-step is the outer step's last is read on the
+// device (max_sub_steps[0]), for the first sub-step, which is enqueued before
+// the host has read the count (every outer step has at least one sub-step).
+template <class REAL, class STO, bool DEVLAST = false>
 struct PlainView : View<REAL, STO> {
     __device__ __forceinline__ int64_t el(int64_t, int64_t i) const { return i; }
     __device__ __forceinline__ int64_t pel(int64_t, int64_t p) const { return p; }
@@ -276,8 +279,8 @@ struct PlainView : View<REAL, STO> {
     __device__ __forceinline__ REAL dt(int64_t, int64_t) const { return this->time_step[0]; }
     __device__ __forceinline__ bool active(int64_t, int64_t, int) const { return true; }
     __device__ __forceinline__ bool active_in_block(int64_t, int64_t, int) const { return true; }
-    __device__ __forceinline__ bool last_substep(int64_t, int64_t, int, int shared_last) const {
-        return shared_last != 0;
+    __device__ __forceinline__ bool last_substep(int64_t, int64_t, int substep, int shared_last) const {
+        return (DEVLAST && substep >= 0) ? substep == this->max_sub_steps[0] - 1 : shared_last != 0;
     }
     __device__ __forceinline__ void split_c(int64_t g, int64_t& t, int64_t& i) const { t = 0; i = g; }
     __device__ __forceinline__ void split_p(int64_t g, int64_t& t, int64_t& p) const { t = 0; p = g; }

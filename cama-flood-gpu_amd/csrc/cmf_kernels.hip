@@ -1770,13 +1770,15 @@ __global__ void k_level_tables(const int4* rec, int64_t nrec, const int32_t* gbl
     for (int k = 0; k < c; ++k) out[(int64_t)level_pos[(int64_t)k * G + g] + within] = x;
 }
 
-// time_step[t] = outer / count[t], in double then rounded (cama_flood_model.py:127,132)
+// time_step[t] = outer / count[t], in double then rounded (cama_flood_model.py:127,132);
+// with one slot and `current_step` given, also current_step = count - 1
 template <class REAL>
-__global__ void k_set_dt(REAL* time_step, const int32_t* counts, int64_t n, double outer) {
+__global__ void k_set_dt(REAL* time_step, const int32_t* counts, int64_t n, double outer, int32_t* current_step) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int c = counts[t] < 1 ? 1 : counts[t];
     time_step[t] = (REAL)(outer / (double)c);
+    if (current_step && n == 1) current_step[0] = c - 1;
 }
 
 __global__ void k_zero_int(int32_t* p, int64_t n) {
