@@ -109,6 +109,8 @@ struct cmf_network {
     // adjoint scratch (cmf_hip_substep_adjoint), allocated on first use:
     // 14 arrays of N elements of up to 8 bytes
     void* adj_scratch = nullptr;
+    // per-block, per-trial CFL maxima of k_cfl_inner (trial counts dividing 64)
+    int32_t* cfl_part = nullptr;
     void* adj_path_scratch = nullptr;   // with paths: [P, LB] + [P] x 9
     void* adj_res_rows = nullptr;       // with reservoirs: catchment -> reservoir row
 };
@@ -131,6 +133,7 @@ void free_network(cmf_network* net) {
     for (auto& g : net->graphs) (void)hipGraphExecDestroy(g.second);
     if (net->capture_stream) (void)hipStreamDestroy(net->capture_stream);
     if (net->adj_scratch) (void)hipFree(net->adj_scratch);
+    if (net->cfl_part) (void)hipFree(net->cfl_part);
     if (net->adj_path_scratch) (void)hipFree(net->adj_path_scratch);
     if (net->adj_res_rows) (void)hipFree(net->adj_res_rows);
     delete net;
@@ -485,6 +488,28 @@ inline int64_t block_of(const cmf_network* net, int id, bool catchment_kernel) {
     return (catchment_kernel && net->G > 0) ? kGroupRecord : net->blocks[id];
 }
 
+// The CFL sub-step count(s) into max_sub_steps (cleared before): the
+// trial-inner kernel for trials in the inner layout whose count divides 64,
+// else k_cfl (chunks of the catchments per trial).
+template <class REAL, class STO>
+int launch_cfl(const cmf_network* net, const View<REAL, STO>& v, hipStream_t s) {
+    const bool inner = v.T > 1 && v.ts == 1 && v.cs == v.T && net->cfl_part && v.T <= 64 && 64 % v.T == 0;
+    if (inner) {
+        int log2T = 0;
+        while ((int64_t(1) << log2T) < v.T) ++log2T;
+        const int64_t nb = std::min<int64_t>(kCflInnerBlocks, (v.N * v.T + 8 * kBlock - 1) / (8 * kBlock));
+        hipLaunchKernelGGL((k_cfl_inner<REAL, STO>), dim3((unsigned)nb), dim3(kBlock), 0, s, v, log2T, net->cfl_part);
+        hipLaunchKernelGGL(k_cfl_fold, dim3((unsigned)v.T), dim3(kBlock), 0, s, (const int32_t*)net->cfl_part, nb, v.T,
+                           v.max_sub_steps, v.has(CMF_PER_TRIAL_DT) ? 1 : 0);
+    } else {
+        const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 512);
+        hipLaunchKernelGGL((k_cfl<REAL, STO>), dim3((unsigned)chunks, (unsigned)v.T), dim3(kBlock), 0, s, v);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(CMF_ERR_HIP, "launch of the CFL pass failed: %s", hipGetErrorString(e));
+    return CMF_OK;
+}
+
 // ----------------------------------------------------------- per-op -----
 enum Op {
     OP_OUTFLOW, OP_INFLOW, OP_STAGE, OP_STAGE_LOG, OP_CFL, OP_BIF_OUT, OP_LEVEE_BIF_OUT,
@@ -553,12 +578,7 @@ int run_op(cmf_network* net, const cmf_fields* f, int op, void* stream) {
             if ((f->flags & CMF_HAS_RESERVOIR) && !f->is_dam_related)
                 return set_error(CMF_ERR_INVALID, "compute_adaptive_time_step: is_dam_related is NULL");
             if (v.N == 0) return CMF_OK;
-            const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 512);
-            hipLaunchKernelGGL((k_cfl<REAL, STO>), dim3((unsigned)chunks, (unsigned)v.T), dim3(kBlock), 0,
-                               (hipStream_t)stream, v);
-            hipError_t e = hipGetLastError();
-            if (e != hipSuccess) return set_error(CMF_ERR_HIP, "launch of k_cfl failed: %s", hipGetErrorString(e));
-            return CMF_OK;
+            return launch_cfl(net, v, (hipStream_t)stream);
         }
         case OP_BIF_OUT:
         case OP_LEVEE_BIF_OUT: {
@@ -1117,8 +1137,7 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
             hipLaunchKernelGGL((k_cfl_grouped<REAL, STO>), dim3((unsigned)((net->nblk + kCflRecords - 1) / kCflRecords)),
                                dim3(kGroupRecord), 0, s, v, (int64_t)net->nblk);
         } else if (v.N > 0) {
-            const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 512);
-            hipLaunchKernelGGL((k_cfl<REAL, STO>), dim3((unsigned)chunks, (unsigned)v.T), dim3(kBlock), 0, s, v);
+            if ((rc = launch_cfl(net, v, s))) return rc;
         }
         // the early advance leaves current_step at count - 1 here (one slot)
         const bool early = !use_graph && early_ok(net, v);
@@ -1548,6 +1567,11 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     // partials for the smallest block size a tunable pass may use (64), and
     // for the block-local pass ahead of its stage list
     net->partial_blocks = std::max<int64_t>((n * num_trials + 63) / 64, net->nblk) + (n + kLocalBlock - 1) / kLocalBlock + 2;
+    if (num_trials > 1 && num_trials <= 64 && 64 % num_trials == 0 &&
+        (rc = dev_alloc((void**)&net->cfl_part, (size_t)kCflInnerBlocks * (size_t)num_trials * sizeof(int32_t)))) {
+        free_network(net);
+        return rc;
+    }
     if ((rc = upload((void**)&net->up_ptr, up_ptr.data(), up_ptr.size() * 4)) ||
         (contiguous && !monotone && (rc = upload((void**)&net->up_rng, rng.data(), rng.size() * sizeof(int2)))) ||
         (rc = upload((void**)&net->up_idx, up_idx.data(), up_idx.size() * 4)) ||

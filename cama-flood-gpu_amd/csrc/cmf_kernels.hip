@@ -1694,6 +1694,59 @@ __global__ __launch_bounds__(kBlock) void k_cfl(View<REAL, STO> v) {
     if (threadIdx.x == 0) atomicMax(&v.max_sub_steps[v.has(CMF_PER_TRIAL_DT) ? t : 0], m);
 }
 
+// Trial-inner layout with T dividing 64 (element e = catchment * T + trial):
+// threads walk the elements contiguously with a stride that is a multiple of
+// 64, so every element a thread meets belongs to the trial lane % T; eight
+// elements' loads are issued before any is used.  Lanes of one trial are
+// folded with shuffles over lane distances that are multiples of T, each
+// block leaves one maximum per trial in `part` [gridDim.x, T], and
+// k_cfl_fold takes the maximum over the blocks (no atomics on a few hot
+// addresses from every block).
+constexpr int kCflInnerBlocks = 1024;
+template <class REAL, class STO>
+__global__ __launch_bounds__(kBlock) void k_cfl_inner(View<REAL, STO> v, int log2T, int32_t* part) {
+    __shared__ int sh[kBlock / 64][64];
+    const int64_t NT = v.N * v.T;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    const REAL outer = v.outer_time_step[0];
+    int best = 0;
+    for (int64_t e0 = (int64_t)blockIdx.x * kBlock + threadIdx.x; e0 < NT; e0 += 8 * stride) {
+        REAL depth[8], dist[8];
+        bool dam[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int64_t e = e0 + k * stride < NT ? e0 + k * stride : e0;
+            const int64_t i = e >> log2T;
+            depth[k] = v.river_depth[e];
+            dist[k] = v.prm(v.dist, CMF_BATCHED_DOWNSTREAM_DISTANCE, e, i);
+            dam[k] = v.has(CMF_HAS_RESERVOIR) && v.is_dam_related[i];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (e0 + k * stride < NT) best = max(best, dam[k] ? 1 : cfl_count(v, depth[k], dist[k], outer));
+    }
+    const int T = 1 << log2T;
+    for (int off = 32; off >= T; off >>= 1) best = max(best, __shfl_xor(best, off, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane < T) sh[wave][lane] = best;
+    __syncthreads();
+    if ((int)threadIdx.x < T) {
+        int m = sh[0][threadIdx.x];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = max(m, sh[w][threadIdx.x]);
+        part[(int64_t)blockIdx.x * T + threadIdx.x] = m;
+    }
+}
+// one block per trial: the maximum of its blocks' entries into its count
+// (slot 0 without per-trial time steps)
+__global__ __launch_bounds__(kBlock) void k_cfl_fold(const int32_t* part, int64_t nblocks, int64_t T,
+                                                     int32_t* counts, int per_trial) {
+    const int64_t t = blockIdx.x;
+    int m = 0;
+    for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) m = max(m, part[b * T + t]);
+    m = block_max_int(m);
+    if (threadIdx.x == 0) atomicMax(&counts[per_trial ? t : 0], m);
+}
+
 // grouped networks: kCflRecords consecutive table entries (each a single
 // group's catchments) per block of kGroupRecord threads, one catchment of
 // each entry per thread; the records are read first and every catchment load

@@ -374,3 +374,34 @@ def test_fp32_storage_batched_members_match_standalone_runs():
         for m in range(M):
             msg = mismatch(got[m], oracles[m].state[name])
             assert msg is None, f"member {m} {name}: {msg}"
+
+
+@pytest.mark.parametrize("members,per_trial", [(2, True), (16, True), (64, True), (16, False)])
+def test_member_cfl_counts_in_the_inner_layout(members, per_trial):
+    """The CFL pass of trial-inner runs whose member count divides 64 (one
+    trial per lane class, block maxima folded per trial): with per-member dt
+    every member's count and state equal its standalone oracle run's over
+    three outer steps, full physics with reservoirs; with one shared dt the
+    first count is the largest standalone count."""
+    import torch
+    net, src = small_source(n=1500, seed=5)
+    runoffs = _member_sources(src, members, seed0=7000)
+    model = _model(src, FULL, num_trials=members, batched={"runoff": np.stack(runoffs)},
+                   per_trial_time_step=per_trial, trial_layout="inner")
+    oracles = [oracle_for(dict(src, runoff=r), FULL)[0] for r in runoffs]
+    steps = 3 if per_trial else 1
+    for step in range(steps):
+        model.step_advance()
+        counts = model.t["max_sub_steps"].cpu().numpy()
+        want = [int(o.advance(1800.0, 1)[0]) for o in oracles]
+        if per_trial:
+            assert counts.tolist() == want, (step, counts.tolist(), want)
+        else:
+            assert int(counts[0]) == max(want), (int(counts[0]), want)
+    if per_trial:
+        torch.cuda.synchronize()
+        for name in ("river_storage", "flood_storage", "river_outflow"):
+            got = model.get(name)
+            for m in (0, members // 2, members - 1):
+                msg = mismatch(got[m], oracles[m].state[name])
+                assert msg is None, f"member {m} {name}: {msg}"
