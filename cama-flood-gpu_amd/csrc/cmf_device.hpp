@@ -295,6 +295,53 @@ struct PlainView : View<REAL, STO> {
     }
 };
 
+// The view of a trial-inner run (element = catchment * T + member, path
+// element = path * T + member) whose member count T is a power of two and a
+// multiple of 64, ungrouped: a wave's 64 lanes are members of one catchment
+// (or path), so its index is the same in every lane.  It is formed once per
+// wave (readfirstlane, exact because blocks are whole waves and T is a
+// multiple of the wave), and every catchment-indexed address, shared
+// parameter and upstream range becomes a scalar; element indices are a
+// shift and an add.
+template <class REAL, class STO>
+struct MemberView : View<REAL, STO> {
+    int log2T;
+    __device__ __forceinline__ int64_t el(int64_t t, int64_t i) const { return (i << log2T) + t; }
+    __device__ __forceinline__ int64_t pel(int64_t t, int64_t p) const { return (p << log2T) + t; }
+    __device__ __forceinline__ void split_c(int64_t g, int64_t& t, int64_t& i) const {
+        i = (int64_t)__builtin_amdgcn_readfirstlane((int)(g >> log2T));
+        t = g - (i << log2T);
+    }
+    __device__ __forceinline__ void split_p(int64_t g, int64_t& t, int64_t& p) const {
+        p = (int64_t)__builtin_amdgcn_readfirstlane((int)(g >> log2T));
+        t = g - (p << log2T);
+    }
+    __device__ __forceinline__ int64_t slot(int64_t t, int64_t) const {
+        return this->has(CMF_PER_TRIAL_DT) ? t : 0;
+    }
+    __device__ __forceinline__ bool per_slot() const { return this->has(CMF_PER_TRIAL_DT); }
+    __device__ __forceinline__ REAL dt(int64_t t, int64_t c) const { return this->time_step[slot(t, c)]; }
+    __device__ __forceinline__ bool active(int64_t t, int64_t c, int substep) const {
+        if (substep < 0 || !per_slot()) return true;
+        return substep < this->max_sub_steps[slot(t, c)];
+    }
+    __device__ __forceinline__ bool active_in_block(int64_t t, int64_t c, int substep) const {
+        return active(t, c, substep);
+    }
+    __device__ __forceinline__ bool last_substep(int64_t t, int64_t c, int substep, int shared_last) const {
+        return (per_slot() && substep >= 0) ? (substep == this->max_sub_steps[slot(t, c)] - 1) : (shared_last != 0);
+    }
+    __device__ __forceinline__ int64_t partial_slot() const { return (int64_t)blockIdx.x; }
+    __device__ __forceinline__ bool catchment_of(int64_t g, int64_t& t, int64_t& i) const {
+        if (g >= this->N * this->T) return false;
+        split_c(g, t, i);
+        return true;
+    }
+    __device__ __forceinline__ bool catchment_in(int64_t blk, int64_t& t, int64_t& i) const {
+        return catchment_of(blk * blockDim.x + threadIdx.x, t, i);
+    }
+};
+
 // The view of a grouped network with one trial, shared parameters and no
 // bifurcation paths, for the catchment kernels, which run one block per
 // record of blk_rec.  A group is a union of whole basins, so every catchment
