@@ -489,11 +489,12 @@ inline int64_t block_of(const cmf_network* net, int id, bool catchment_kernel) {
 }
 
 // The CFL sub-step count(s) into max_sub_steps (cleared before): the
-// trial-inner kernel for trials in the inner layout whose count divides 64,
-// else k_cfl (chunks of the catchments per trial).
+// contiguous kernel for one trial or trials in the inner layout whose count
+// divides 64, else k_cfl (chunks of the catchments per trial).
 template <class REAL, class STO>
 int launch_cfl(const cmf_network* net, const View<REAL, STO>& v, hipStream_t s) {
-    const bool inner = v.T > 1 && v.ts == 1 && v.cs == v.T && net->cfl_part && v.T <= 64 && 64 % v.T == 0;
+    const bool contiguous = (v.T == 1 && v.cs == 1) || (v.ts == 1 && v.cs == v.T);
+    const bool inner = contiguous && net->cfl_part && v.T == net->T && v.T <= 64 && 64 % v.T == 0;
     if (inner) {
         int log2T = 0;
         while ((int64_t(1) << log2T) < v.T) ++log2T;
@@ -1576,7 +1577,7 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     // partials for the smallest block size a tunable pass may use (64), and
     // for the block-local pass ahead of its stage list
     net->partial_blocks = std::max<int64_t>((n * num_trials + 63) / 64, net->nblk) + (n + kLocalBlock - 1) / kLocalBlock + 2;
-    if (num_trials > 1 && num_trials <= 64 && 64 % num_trials == 0 &&
+    if (num_trials <= 64 && 64 % num_trials == 0 &&
         (rc = dev_alloc((void**)&net->cfl_part, (size_t)kCflInnerBlocks * (size_t)num_trials * sizeof(int32_t)))) {
         free_network(net);
         return rc;

@@ -1694,7 +1694,8 @@ __global__ __launch_bounds__(kBlock) void k_cfl(View<REAL, STO> v) {
     if (threadIdx.x == 0) atomicMax(&v.max_sub_steps[v.has(CMF_PER_TRIAL_DT) ? t : 0], m);
 }
 
-// Trial-inner layout with T dividing 64 (element e = catchment * T + trial):
+// One trial, or the trial-inner layout with T dividing 64 (element e =
+// catchment * T + trial):
 // threads walk the elements contiguously with a stride that is a multiple of
 // 64, so every element a thread meets belongs to the trial lane % T; eight
 // elements' loads are issued before any is used.  Lanes of one trial are
