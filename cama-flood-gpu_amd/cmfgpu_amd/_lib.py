@@ -167,7 +167,8 @@ def _declare(lib):
         "cmf_network_num_groups": (_i64, [_vp]),
         "cmf_network_set_block_sizes": (ctypes.c_int, [_vp, i32p, ctypes.c_int32]),
         "cmf_network_set_outflow_accumulator": (ctypes.c_int, [_vp, _vp]),
-        "cmf_hip_map_runoff": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int32, _vp, _vp]),
+        "cmf_hip_map_runoff": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, ctypes.c_int32, _vp,
+                                              _vp]),
         "cmf_network_get_block_sizes": (ctypes.c_int, [_vp, i32p, ctypes.c_int32]),
         "cmf_network_destroy": (ctypes.c_int, [_vp]),
         "cmf_network_is_contiguous": (ctypes.c_int, [_vp]),

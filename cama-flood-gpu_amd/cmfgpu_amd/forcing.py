@@ -8,7 +8,8 @@ hydroforge's ``generate_mapping_table``: ``catchment_ids``, ``sparse_data``,
 hydroforge's ``shard_forcing`` before ``set_inputs``
 (scripts/run_daily_bin.py:98-119).  Here the product runs in one HIP pass
 (cmf_hip_map_runoff) that writes the model's runoff field directly in the
-device layout.  The unit factor is the caller's: ``scale`` multiplies every
+device layout (rows in the mapping's order, each sum scattered to its
+catchment's place).  The unit factor is the caller's: ``scale`` multiplies every
 row sum (hydroforge's conversion is not available to pin, so it is a
 parameter here).
 """
@@ -72,21 +73,19 @@ class RunoffMapping:
         return cls(indptr, indices, weights, shape[1], scale)
 
     def bind(self, model) -> "RunoffMapping":
-        """Reorder the rows into the model's device layout and upload."""
+        """Upload the matrix in its own (network) row order with each row's
+        place in the model's device layout: consecutive rows gather nearby
+        grid cells whatever the device layout, and the kernel scatters the
+        row sums to their catchments."""
         if model.N != self.rows:
             raise ValueError(f"mapping has {self.rows} rows, model {model.N} catchments")
         if model.T > 1 and "runoff" in model.batched_names:
             raise ValueError("the mapped runoff is shared by all trials")
-        perm = model.perm
-        counts = np.diff(self.indptr)[perm]
-        indptr = np.zeros(model.N + 1, np.int64)
-        indptr[1:] = np.cumsum(counts)
-        # entry e of new row k comes from old entry indptr[perm[k]] + (e - new start of k)
-        src = np.repeat(self.indptr[:-1][perm] - indptr[:-1], counts) + np.arange(indptr[-1], dtype=np.int64)
         dev = model.device
-        self.d_indptr = torch.from_numpy(indptr).to(dev)
-        self.d_indices = torch.from_numpy(self.indices[src]).to(dev)
-        self.d_weights = torch.from_numpy(self.weights[src]).to(dev)
+        self.d_indptr = torch.from_numpy(self.indptr).to(dev)
+        self.d_indices = torch.from_numpy(self.indices).to(dev)
+        self.d_weights = torch.from_numpy(self.weights).to(dev)
+        self.d_place = torch.from_numpy(np.asarray(model.inv, dtype=np.int32)).to(dev)
         self.model = model
         return self
 
@@ -106,6 +105,7 @@ class RunoffMapping:
         out = m.t["runoff"]
         prec = _lib.PRECISION_F32_F64 if out.dtype == torch.float32 else _lib.PRECISION_F64_F64
         _lib.check(_lib.lib().cmf_hip_map_runoff(self.rows, _lib.ptr(self.d_indptr), _lib.ptr(self.d_indices),
-                                                 _lib.ptr(self.d_weights), _lib.ptr(g), self.scale, prec,
-                                                 _lib.ptr(out), _lib.current_stream()), "cmf_hip_map_runoff")
+                                                 _lib.ptr(self.d_weights), _lib.ptr(g), self.scale,
+                                                 _lib.ptr(self.d_place), prec, _lib.ptr(out),
+                                                 _lib.current_stream()), "cmf_hip_map_runoff")
         self._keep = g

@@ -1650,14 +1650,17 @@ int cmf_network_set_outflow_accumulator(cmf_network* net, double* acc) {
 }
 
 int cmf_hip_map_runoff(int64_t rows, const int64_t* indptr, const int32_t* indices, const float* weights,
-                       const float* grid, double scale, int32_t precision, void* out, void* stream) {
+                       const float* grid, double scale, const int32_t* out_index, int32_t precision, void* out,
+                       void* stream) {
     if (rows < 0 || (rows > 0 && (!indptr || !indices || !weights || !grid || !out)))
         return set_error(CMF_ERR_INVALID, "cmf_hip_map_runoff: bad arguments");
     if (rows == 0) return CMF_OK;
-    if (precision == CMF_PRECISION_F32_F64)
-        LAUNCH((k_map_runoff<float>), rows, stream, rows, indptr, indices, weights, grid, scale, static_cast<float*>(out));
+    if (precision == CMF_PRECISION_F32_F64 || precision == CMF_PRECISION_F32_F32)
+        LAUNCH((k_map_runoff<float>), rows, stream, rows, indptr, indices, weights, grid, scale, out_index,
+               static_cast<float*>(out));
     else if (precision == CMF_PRECISION_F64_F64)
-        LAUNCH((k_map_runoff<double>), rows, stream, rows, indptr, indices, weights, grid, scale, static_cast<double*>(out));
+        LAUNCH((k_map_runoff<double>), rows, stream, rows, indptr, indices, weights, grid, scale, out_index,
+               static_cast<double*>(out));
     else
         return set_error(CMF_ERR_UNSUPPORTED, "unknown precision %d", precision);
     return CMF_OK;

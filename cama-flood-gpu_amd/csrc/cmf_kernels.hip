@@ -1843,18 +1843,21 @@ __global__ void k_zero_int(int32_t* p, int64_t n) {
 // ==================================================== runoff mapping ===
 // Catchment runoff from a gridded field through the catchment x grid-cell
 // area-weight matrix (CSR, rows in the caller's catchment order):
-// out[r] = scale * sum_k w[k] * grid[idx[k]], summed in double in index order.
+// out[r] = scale * sum_k w[k] * grid[idx[k]], summed in double in index order
+// (out[out_index[r]] when out_index is given: the rows in an order whose grid
+// gathers stay close, the results scattered to the catchments' elements).
 template <class REAL>
 __global__ __launch_bounds__(kBlock) void k_map_runoff(int64_t rows, const int64_t* __restrict__ indptr,
                                                        const int32_t* __restrict__ idx,
                                                        const float* __restrict__ w,
                                                        const float* __restrict__ grid, double scale,
+                                                       const int32_t* __restrict__ out_index,
                                                        REAL* __restrict__ out) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     double acc = 0.0;
     for (int64_t k = indptr[r], e = indptr[r + 1]; k < e; ++k) acc += (double)w[k] * (double)grid[idx[k]];
-    out[r] = (REAL)(scale * acc);
+    out[out_index ? (int64_t)out_index[r] : r] = (REAL)(scale * acc);
 }
 
 // ============================================= explicit instantiation ===

@@ -412,9 +412,13 @@ int cmf_hip_advance(cmf_network* net, const cmf_fields* f, double outer_seconds,
  * runoff mapping (catchments x grid cells, area weights; the arrays of the
  * mapping .npz read at cmfgpu/params/export_bin.py:963-975), applied where
  * the reference applies it in hydroforge's shard_forcing
- * (scripts/run_daily_bin.py:114-119); rows in the caller's catchment order. */
+ * (scripts/run_daily_bin.py:114-119).  With `out_index` (device, one entry
+ * per row; NULL: out[r]) row r lands in out[out_index[r]]: the rows can stay
+ * in the mapping's own (spatially ordered) catchment order while the results
+ * go to the catchments' places in the device layout. */
 int cmf_hip_map_runoff(int64_t rows, const int64_t* indptr, const int32_t* indices, const float* weights,
-                       const float* grid, double scale, int32_t precision, void* out, void* stream);
+                       const float* grid, double scale, const int32_t* out_index, int32_t precision, void* out,
+                       void* stream);
 /* With a non-NULL accumulator (double, one entry per catchment element) the
  * fused stage pass adds (river_outflow + flood_outflow) * dt every sub-step:
  * the time integral behind the reference's "mean" statistics of
