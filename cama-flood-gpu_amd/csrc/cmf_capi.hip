@@ -896,14 +896,22 @@ int fused_substep(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* 
         static_cast<View<REAL, STO>&>(pv) = v;
         return three_pass_substep<REAL, STO, PlainView<REAL, STO>>(net, pv, substep, store_inflow, stream);
     }
-    // members in the trial-inner layout, a multiple of 64 and a power of two
-    if (net->G == 0 && v.T >= 64 && (v.T & (v.T - 1)) == 0 && v.ts == 1 && v.cs == v.T &&
+    // members in the trial-inner layout, a power of two (one catchment per
+    // wave from 64 on)
+    if (net->G == 0 && v.T >= 2 && (v.T & (v.T - 1)) == 0 && v.ts == 1 && v.cs == v.T &&
         (v.P == 0 || (v.pts == 1 && v.pps == v.T)) && !v.tape_q_r) {
-        MemberView<REAL, STO> mv;
+        int log2T = 0;
+        while ((int64_t(1) << log2T) < v.T) ++log2T;
+        if (v.T >= 64) {
+            MemberView<REAL, STO, true> mv;
+            static_cast<View<REAL, STO>&>(mv) = v;
+            mv.log2T = log2T;
+            return three_pass_substep<REAL, STO, MemberView<REAL, STO, true>>(net, mv, substep, store_inflow, stream);
+        }
+        MemberView<REAL, STO, false> mv;
         static_cast<View<REAL, STO>&>(mv) = v;
-        mv.log2T = 0;
-        while ((int64_t(1) << mv.log2T) < v.T) ++mv.log2T;
-        return three_pass_substep<REAL, STO, MemberView<REAL, STO>>(net, mv, substep, store_inflow, stream);
+        mv.log2T = log2T;
+        return three_pass_substep<REAL, STO, MemberView<REAL, STO, false>>(net, mv, substep, store_inflow, stream);
     }
     // grouped, without the path pass (whose blocks are no group records)
     if (net->G > 0 && v.T == 1 && v.batched == 0 && !v.has(CMF_PER_TRIAL_DT) && !v.has(CMF_HAS_BIFURCATION) &&

@@ -296,24 +296,28 @@ struct PlainView : View<REAL, STO> {
 };
 
 // The view of a trial-inner run (element = catchment * T + member, path
-// element = path * T + member) whose member count T is a power of two and a
-// multiple of 64, ungrouped: a wave's 64 lanes are members of one catchment
-// (or path), so its index is the same in every lane.  It is formed once per
-// wave (readfirstlane, exact because blocks are whole waves and T is a
-// multiple of the wave), and every catchment-indexed address, shared
-// parameter and upstream range becomes a scalar; element indices are a
-// shift and an add.
-template <class REAL, class STO>
+// element = path * T + member) whose member count T is a power of two,
+// ungrouped: element indices are a shift and an add, and splitting a thread
+// index into (member, catchment) a shift and a mask instead of a 64-bit
+// division.  WAVE (T a multiple of 64): a wave's 64 lanes are members of one
+// catchment (or path), so its index is the same in every lane; it is formed
+// once per wave (readfirstlane, exact because blocks are whole waves and T
+// is a multiple of the wave), and every catchment-indexed address, shared
+// parameter and upstream range becomes a scalar.
+template <class REAL, class STO, bool WAVE = true>
 struct MemberView : View<REAL, STO> {
     int log2T;
     __device__ __forceinline__ int64_t el(int64_t t, int64_t i) const { return (i << log2T) + t; }
     __device__ __forceinline__ int64_t pel(int64_t t, int64_t p) const { return (p << log2T) + t; }
+    __device__ __forceinline__ int64_t item_of(int64_t g) const {
+        return WAVE ? (int64_t)__builtin_amdgcn_readfirstlane((int)(g >> log2T)) : (g >> log2T);
+    }
     __device__ __forceinline__ void split_c(int64_t g, int64_t& t, int64_t& i) const {
-        i = (int64_t)__builtin_amdgcn_readfirstlane((int)(g >> log2T));
+        i = item_of(g);
         t = g - (i << log2T);
     }
     __device__ __forceinline__ void split_p(int64_t g, int64_t& t, int64_t& p) const {
-        p = (int64_t)__builtin_amdgcn_readfirstlane((int)(g >> log2T));
+        p = item_of(g);
         t = g - (p << log2T);
     }
     __device__ __forceinline__ int64_t slot(int64_t t, int64_t) const {
