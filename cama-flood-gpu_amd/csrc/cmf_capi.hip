@@ -1036,9 +1036,9 @@ int substep_loop_graph(cmf_network* net, const View<REAL, STO>& v, const cmf_fie
 // records of the groups with count > k in memory order, so a sub-step
 // launches only those groups' blocks and walks memory as an ungrouped launch
 // does.  The host places each group in each level (count x G positions); a
-// device pass copies the records.  Called after the stream has drained (the
-// count read): the old tables are free to replace, and the staging buffer is
-// rewritten only after the previous copy from it has completed.
+// device pass copies the records.  Growing a table waits for the device
+// first (a launch on another stream may still read the old one); the staging
+// buffer is rewritten only after the previous copy from it has completed.
 int build_level_tables(cmf_network* net, const int32_t* counts, int count, hipStream_t s) {
     const int64_t G = net->G;
     const int32_t* hc = net->host_counts;
@@ -1055,7 +1055,11 @@ int build_level_tables(cmf_network* net, const int32_t* counts, int count, hipSt
     }
     const int64_t npos = (int64_t)count * G;
     if (total > net->level_cap) {
-        if (net->level_rec) CMF_HIP_CHECK(hipFree(net->level_rec));
+        // the old tables may still be read by launches on another stream
+        if (net->level_rec) {
+            CMF_HIP_CHECK(hipDeviceSynchronize());
+            CMF_HIP_CHECK(hipFree(net->level_rec));
+        }
         net->level_rec = nullptr;
         net->level_cap = 0;
         const int64_t cap = std::max<int64_t>(total, net->nblk * 4);
@@ -1065,7 +1069,10 @@ int build_level_tables(cmf_network* net, const int32_t* counts, int count, hipSt
     }
     CMF_HIP_CHECK(hipEventSynchronize(net->order_copied));
     if (npos > net->level_pos_cap) {
-        if (net->level_pos) CMF_HIP_CHECK(hipFree(net->level_pos));
+        if (net->level_pos) {
+            CMF_HIP_CHECK(hipDeviceSynchronize());
+            CMF_HIP_CHECK(hipFree(net->level_pos));
+        }
         if (net->host_level_pos) CMF_HIP_CHECK(hipHostFree(net->host_level_pos));
         net->level_pos = nullptr;
         net->host_level_pos = nullptr;
