@@ -759,9 +759,10 @@ def read_traffic(kernel: str):
         if line.startswith("# source commit:"):
             commit = line.split(":", 1)[1].strip()
         parts = line.split()
-        if len(parts) >= 2 and not line.startswith("#") and kernel.split("<")[0] == parts[0]:
+        if (value is None and len(parts) >= 2 and not line.startswith("#")
+                and kernel.split("<")[0] == parts[0]):
             try:
-                value = float(parts[1])
+                value = float(parts[1])        # the kernel's own row, not a "(first sub-step)" one
             except ValueError:
                 value = None
     return value, {"file": "profiles/hbm_traffic.txt", "measured_at_commit": commit,
