@@ -123,3 +123,34 @@ def test_set_inputs_from_a_device_tensor(trial_layout):
     assert np.array_equal(many.get("runoff"), r[::-1])
     with pytest.raises(ValueError, match="shape"):
         one.set_inputs(torch.zeros(n + 1, device="cuda:0"))
+
+
+@pytest.mark.parametrize("placed", [False, True])
+def test_map_runoff_entry_point_with_and_without_out_index(placed):
+    """cmf_hip_map_runoff through the C ABI: rows summed in index order in
+    FP64 and written to out[r] (out_index NULL) or to out[out_index[r]] (a
+    permutation), against the FP64 CSR product of scipy."""
+    import scipy.sparse as sp
+    import torch
+    from cmfgpu_amd import _lib
+    n, cells = 4000, 64 * 64
+    indptr, indices, weights = _mapping(n, cells, seed=11)
+    grid = np.random.default_rng(4).lognormal(0.0, 1.0, cells).astype(np.float32)
+    scale = 2.5e-4
+    want = (scale * sp.csr_matrix((weights.astype(np.float64), indices, indptr), shape=(n, cells))
+            .dot(grid.astype(np.float64))).astype(np.float32)
+    dev = "cuda:0"
+    t = {k: torch.from_numpy(v).to(dev) for k, v in
+         (("indptr", indptr), ("indices", indices), ("weights", weights), ("grid", grid))}
+    out = torch.full((n,), -1.0, dtype=torch.float32, device=dev)
+    place = np.random.default_rng(2).permutation(n).astype(np.int32) if placed else None
+    d_place = torch.from_numpy(place).to(dev) if placed else None
+    _lib.check(_lib.lib().cmf_hip_map_runoff(n, _lib.ptr(t["indptr"]), _lib.ptr(t["indices"]),
+                                             _lib.ptr(t["weights"]), _lib.ptr(t["grid"]), scale,
+                                             _lib.ptr(d_place) if placed else None, _lib.PRECISION_F32_F64,
+                                             _lib.ptr(out), _lib.current_stream()), "cmf_hip_map_runoff")
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    if placed:
+        got = got[place]          # row r was written at place[r]
+    assert np.array_equal(got, want)
