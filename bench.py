@@ -55,8 +55,26 @@ def algorithmic_bytes(kernel: str, model, f_wet: float) -> float:
     # total storage, flood fraction and protected depth, written on the last
     # one only, are not counted)
     if kernel.startswith("k_outflow"):
-        per = 84 + 28 + (5 if lev else 0)
-        return per * N * T + (paths if kernel.startswith("k_outflow_paths") else 0)
+        # writes: two flows, three cross-section fields and either the two
+        # outgoing terms or, with provisional limiters (_prov), the limiter
+        base = kernel.split("<")[0]
+        per = 84 + (24 if base.endswith("_prov") else 28) + (5 if lev else 0)
+        return per * N * T + (paths if base.startswith("k_outflow_paths") else 0)
+    if kernel.startswith("k_limiter_fix"):
+        # the flags (1 B per element), and per path end and trial its record
+        # (16 B), storages (16 B), unlimited flows (8 B) and limiter (4 B),
+        # the flows of its upstream catchments (8 B each) and per event the
+        # code and the path term (8 B); backflow catchments are not counted
+        if not bif:
+            return 1.0 * N * T
+        import numpy as np
+        hf = model.host.fields
+        ci = np.asarray(hf["bifurcation_catchment_idx"])
+        di = np.asarray(hf["bifurcation_downstream_idx"])
+        ends = np.unique(np.concatenate([ci, di]))
+        dn = np.asarray(hf["downstream_idx"])
+        ups = np.bincount(dn[dn != np.arange(dn.size)], minlength=dn.size)[ends].sum()
+        return 1.0 * N * T + T * (ends.size * 44.0 + 8.0 * ups + 8.0 * 2 * P)
     if kernel.startswith("k_limiter"):
         per = 4 + 4 + 4 + 16 + 4 + (8 if bif else 0)
         return per * N * T
@@ -74,8 +92,19 @@ def algorithmic_bytes(kernel: str, model, f_wet: float) -> float:
     return 0.0
 
 
+# schedule of the fused sub-step for every model the bench builds (--schedule)
+SCHEDULE = "three_pass"
+
+
 def build_model(net, src, members: int, device: str, layout: str, precise: bool = False,
                 precision: str = "float32"):
+    m = _build_model(net, src, members, device, layout, precise, precision)
+    if SCHEDULE != "three_pass":
+        m.set_schedule(SCHEDULE)
+    return m
+
+
+def _build_model(net, src, members: int, device: str, layout: str, precise: bool, precision: str):
     from cmfgpu_amd.models import CaMaFlood
     mods = ["base", "adaptive_time"]
     if net.num_paths:
@@ -796,10 +825,14 @@ def main():
                     help="device catchment order")
     ap.add_argument("--reference-surfaces", type=int, default=1,
                     help="also time the plain run with the reference's FP32 surfaces (0: skip)")
+    ap.add_argument("--schedule", default="three_pass", choices=("three_pass", "limiter_pass", "block_local"),
+                    help="schedule of the fused sub-step (same results; for A/B timing)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
 
+    global SCHEDULE
+    SCHEDULE = args.schedule
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU basin sharding is design-only)", file=sys.stderr)
@@ -926,7 +959,7 @@ def main():
                                "water surfaces formed in double (precise_surfaces)",
                    "catchments": net.num_catchments, "bifurcation_paths": net.num_paths,
                    "levees": net.num_levees, "flood_levels": 10, "outer_step_s": 1800.0,
-                   "substeps_timed": int(substeps), "layout": layout_used, "parallelism": "single GPU"},
+                   "substeps_timed": int(substeps), "layout": layout_used, "schedule": SCHEDULE, "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "reference_surfaces": reference_surfaces,
         "batched": batched, "basins": basins, "graph": graph,
         "adjoint": adjoint, "forcing": forcing, "batched_full": batched_full,

@@ -61,7 +61,7 @@ PRECISION_F32_F64 = 0
 PRECISION_F64_F64 = 1
 PRECISION_F32_F32 = 2
 
-SCHEDULES = {"three_pass": 0, "block_local": 1}
+SCHEDULES = {"three_pass": 0, "block_local": 1, "limiter_pass": 2}
 
 LOG_FIELDS = (
     "total_storage_pre_sum",

@@ -23,8 +23,8 @@ from . import _lib
 
 CANDIDATES = (64, 128, 256, 512)
 KERNELS = {
-    "k_outflow": 0, "k_outflow_paths": 0, "k_reservoir": 1, "k_limiter": 2, "k_limiter_pair": 2,
-    "k_stage_fused": 3,
+    "k_outflow": 0, "k_outflow_paths": 0, "k_outflow_prov": 0, "k_outflow_paths_prov": 0, "k_reservoir": 1, "k_limiter": 2, "k_limiter_pair": 2,
+    "k_limiter_fix": 2, "k_stage_fused": 3,
     # the block-local schedule: its list passes take the limiter and stage
     # sizes (k_fused_local runs fixed 512-catchment blocks)
     "k_limiter_list": 2, "k_stage_list": 3,

@@ -424,12 +424,17 @@ class CaMaFlood:
         return tune(self, **kw)
 
     def set_schedule(self, name: str) -> None:
-        """Schedule of the fused sub-step, both with the same results:
-        "three_pass" (the default: outflow, limiter and stage passes over the
-        whole network) or "block_local" (one trial, breadth-first layout, no
-        reservoirs): one pass in which each block of 512 catchments completes
-        the limiters and stages it holds whole, then list passes over the
-        rest; measured slower on MI355X, kept as an option."""
+        """Schedule of the fused sub-step, all with the same results:
+        "three_pass" (the default): an outflow pass that also leaves each
+        storage limiter its own outgoing term fixes, a fix-up pass over the
+        path ends and the catchments with backflow from upstream, and the
+        stage pass (grouped networks, reservoirs and taped runs take
+        "limiter_pass"); "limiter_pass": outflow, limiter and stage passes
+        over the whole network; "block_local" (one trial, breadth-first
+        layout, no reservoirs): one pass in which each block of 512
+        catchments completes the limiters and stages it holds whole, then
+        list passes over the rest; measured slower on MI355X, kept as an
+        option."""
         if name not in _lib.SCHEDULES:
             raise ValueError(f"schedule must be one of {sorted(_lib.SCHEDULES)}")
         _lib.check(_lib.lib().cmf_network_set_schedule(self.network_handle, _lib.SCHEDULES[name]),

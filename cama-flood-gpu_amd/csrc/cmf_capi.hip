@@ -106,6 +106,14 @@ struct cmf_network {
     int32_t* stage_list = nullptr;
     int64_t stage_count = 0;
     int schedule = CMF_SCHEDULE_THREE_PASS;
+    // three-pass schedule: the flags of the provisional limiters (one byte
+    // per element, whole 32-bit words, zero between sub-steps) and the
+    // path-end catchments, whose limiters the fix-up pass always forms:
+    // {catchment, upstream range start, end, first event} in ascending
+    // catchment order, and a last record whose .w ends the events
+    uint8_t* bflag = nullptr;
+    int4* pend_list = nullptr;
+    int64_t pend_count = 0;
     // adjoint scratch (cmf_hip_substep_adjoint), allocated on first use:
     // 14 arrays of N elements of up to 8 bytes
     void* adj_scratch = nullptr;
@@ -123,7 +131,7 @@ void free_network(cmf_network* net) {
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
                     net->path_neg, net->path_net, net->path_raw, net->partials, net->cgroup,
                     net->blk_rec_id, net->level_rec, net->level_pos, net->gblk_ptr, net->lflags, net->lim_list,
-                    net->stage_list};
+                    net->stage_list, net->bflag, net->pend_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (net->host_counts) (void)hipHostFree(net->host_counts);
@@ -230,6 +238,8 @@ int view_of_fields(const cmf_fields* f, View<REAL, STO>& v) {
     v.cs = f->catchment_stride ? f->catchment_stride : 1;
     v.pts = f->path_trial_stride ? f->path_trial_stride : v.P;
     v.pps = f->path_stride ? f->path_stride : 1;
+    if (v.T == 1 && (v.cs != 1 || (v.P > 0 && v.pps != 1)))
+        return set_error(CMF_ERR_INVALID, "one trial: catchment_stride and path_stride must be 1");
     if (v.T > 1) {
         const bool major = (v.cs == 1 && v.ts == v.N);
         const bool inner = (v.ts == 1 && v.cs == v.T);
@@ -347,6 +357,7 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.path_net = static_cast<REAL*>(net->path_net);
     v.path_raw = static_cast<REAL*>(net->path_raw);
     v.partials = net->partials;
+    v.bflag = net->bflag;
     return CMF_OK;
 }
 
@@ -766,8 +777,41 @@ int local_substep(cmf_network* net, const View<REAL, STO>& v, int substep, int s
 #undef CMF_LOCAL
 }
 
+// The outflow launch of a fused sub-step: the catchment outflow pass, with
+// bifurcation paths in leading blocks of its own.  PROV: provisional
+// limiters instead of outgoing terms (k_limiter_fix).
+template <class REAL, class STO, class VT, bool PROV>
+int launch_outflow(const cmf_network* net, const VT& v, int64_t NT, int substep, int store_inflow, void* stream) {
+    const int64_t b_out = block_of(net, CMF_TUNE_OUTFLOW, true);
+    if (v.has(CMF_HAS_BIFURCATION)) {
+        // the path pass rides in the outflow launch, in blocks of its own first
+        const int64_t pb = (v.P * v.T + b_out - 1) / b_out;
+        const int64_t cnt = (pb + (NT + b_out - 1) / b_out) * b_out;
+        const bool levee = v.has(CMF_HAS_LEVEE), l5 = v.LB == 5;
+        if constexpr (PROV) {
+            if (levee && l5) LAUNCH_B((k_outflow_paths_prov<REAL, STO, true, 5, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
+            else if (levee) LAUNCH_B((k_outflow_paths_prov<REAL, STO, true, 0, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
+            else if (l5) LAUNCH_B((k_outflow_paths_prov<REAL, STO, false, 5, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
+            else LAUNCH_B((k_outflow_paths_prov<REAL, STO, false, 0, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
+        } else {
+            if (levee && l5) LAUNCH_B((k_outflow_paths<REAL, STO, true, 5, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
+            else if (levee) LAUNCH_B((k_outflow_paths<REAL, STO, true, 0, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
+            else if (l5) LAUNCH_B((k_outflow_paths<REAL, STO, false, 5, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
+            else LAUNCH_B((k_outflow_paths<REAL, STO, false, 0, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
+        }
+    } else if constexpr (PROV) {
+        LAUNCH_B((k_outflow_prov<REAL, STO, VT>), NT, b_out, stream, v, substep, store_inflow);
+    } else {
+        LAUNCH_B((k_outflow<REAL, STO, true, VT>), NT, b_out, stream, v, substep, store_inflow);
+    }
+    return CMF_OK;
+}
+
 // The three passes of one fused sub-step, on view type VT (PlainView for one
 // plain trial, where the per-trial and per-group indexing folds away).
+// THREE_PASS on an ungrouped network without reservoirs, not taped: outflow
+// with provisional limiters, the limiter fix-up, stage; otherwise outflow,
+// [reservoir], limiter, stage.
 template <class REAL, class STO, class VT>
 int three_pass_substep(cmf_network* net, const VT& v, int substep, int store_inflow, void* stream) {
     // catchment kernels: N*T threads, or one block per group table entry
@@ -777,22 +821,12 @@ int three_pass_substep(cmf_network* net, const VT& v, int substep, int store_inf
     const bool bif = v.has(CMF_HAS_BIFURCATION), levee = v.has(CMF_HAS_LEVEE);
     const bool log = v.has(CMF_HAS_LOG);
     const bool lev_fused = levee && v.levee_index != nullptr;
-    const int64_t b_out = block_of(net, CMF_TUNE_OUTFLOW, true);
     const int64_t b_lim = block_of(net, CMF_TUNE_LIMITER, true), b_stage = block_of(net, CMF_TUNE_STAGE, true);
-    if (bif) {
-        // the path pass rides in the outflow launch, in blocks of its own first
-        const int64_t pb = (v.P * v.T + b_out - 1) / b_out;
-        const int64_t cnt = (pb + (NT + b_out - 1) / b_out) * b_out;
-        if (levee) {
-            if (v.LB == 5) LAUNCH_B((k_outflow_paths<REAL, STO, true, 5, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
-            else LAUNCH_B((k_outflow_paths<REAL, STO, true, 0, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
-        } else {
-            if (v.LB == 5) LAUNCH_B((k_outflow_paths<REAL, STO, false, 5, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
-            else LAUNCH_B((k_outflow_paths<REAL, STO, false, 0, VT>), cnt, b_out, stream, v, substep, pb, store_inflow);
-        }
-    } else {
-        LAUNCH_B((k_outflow<REAL, STO, true, VT>), NT, b_out, stream, v, substep, store_inflow);
-    }
+    const bool prov = !VT::kGrouped && net->schedule == CMF_SCHEDULE_THREE_PASS && net->G == 0 &&
+                      !v.has(CMF_HAS_RESERVOIR) && !v.tape_q_r && net->bflag;
+    int rc = prov ? launch_outflow<REAL, STO, VT, true>(net, v, NT, substep, store_inflow, stream)
+                  : launch_outflow<REAL, STO, VT, false>(net, v, NT, substep, store_inflow, stream);
+    if (rc) return rc;
     if (v.has(CMF_HAS_RESERVOIR))
         LAUNCH_B((k_reservoir<REAL, STO, true>), v.R * v.T, block_of(net, CMF_TUNE_RESERVOIR, false), stream, v, substep);
     // limiters of every catchment, then the stage pass (which also forms the
@@ -803,6 +837,15 @@ int three_pass_substep(cmf_network* net, const VT& v, int substep, int store_inf
             LAUNCH_B((k_limiter_pair<REAL, STO, true, VT>), NT / 2, kGroupRecord / 2, stream, v, substep, store_inflow);
         else
             LAUNCH_B((k_limiter_pair<REAL, STO, false, VT>), NT / 2, kGroupRecord / 2, stream, v, substep, store_inflow);
+    } else if (prov) {
+        // path ends (per trial) in leading blocks, then four flags per thread
+        const int64_t list_n = bif ? net->pend_count : 0;
+        const int64_t lb = (list_n * v.T + b_lim - 1) / b_lim;
+        const int64_t words = (v.N * v.T + 3) / 4;
+        const int64_t cnt = (lb + (words + b_lim - 1) / b_lim) * b_lim;
+        const int4* list = net->pend_list;
+        if (contig) LAUNCH_B((k_limiter_fix<REAL, STO, true, VT>), cnt, b_lim, stream, v, substep, store_inflow, list, list_n, lb);
+        else LAUNCH_B((k_limiter_fix<REAL, STO, false, VT>), cnt, b_lim, stream, v, substep, store_inflow, list, list_n, lb);
     } else if (net->G > 0) {
         // the group block table maps one catchment per thread
         if (contig) LAUNCH_B((k_limiter<REAL, STO, true>), NT, b_lim, stream, v, substep, store_inflow);
@@ -812,7 +855,7 @@ int three_pass_substep(cmf_network* net, const VT& v, int substep, int store_inf
         if (contig) LAUNCH_B((k_limiter_pair<REAL, STO, true, VT>), threads, b_lim, stream, v, substep, store_inflow);
         else LAUNCH_B((k_limiter_pair<REAL, STO, false, VT>), threads, b_lim, stream, v, substep, store_inflow);
     }
-    int rc = log ? (contig ? launch_stage<REAL, STO, true, true, VT>(net, v, NT, lev_fused, substep, store_inflow, stream)
+    rc = log ? (contig ? launch_stage<REAL, STO, true, true, VT>(net, v, NT, lev_fused, substep, store_inflow, stream)
                            : launch_stage<REAL, STO, false, true, VT>(net, v, NT, lev_fused, substep, store_inflow, stream))
                  : (contig ? launch_stage<REAL, STO, true, false, VT>(net, v, NT, lev_fused, substep, store_inflow, stream)
                            : launch_stage<REAL, STO, false, false, VT>(net, v, NT, lev_fused, substep, store_inflow, stream));
@@ -1574,6 +1617,13 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
             ev1[i].y = (e - b > 1) ? -(ev_other[b] + 1) : ev_other[b];
         }
     }
+    // path-end records of the limiter fix-up pass, ascending catchment
+    std::vector<int4> pend;
+    for (int64_t i = 0; i < n; ++i)
+        if (ev_ptr[i + 1] != ev_ptr[i])
+            pend.push_back(make_int4((int32_t)i, up_ptr[i], range_end[i], ev_ptr[i]));
+    const int64_t pend_n = (int64_t)pend.size();
+    pend.push_back(make_int4(-1, 0, 0, ev_ptr[n]));
     // block-local schedule tables for blocks of kLocalBlock catchments
     std::vector<uint8_t> lflags;
     std::vector<int32_t> lim_list, stage_list;
@@ -1609,9 +1659,20 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
         (rc = dev_alloc(&net->lim, el)) || (rc = dev_alloc(&net->path_pos, pel)) ||
         (rc = dev_alloc(&net->path_neg, pel)) || (rc = dev_alloc(&net->path_net, pel)) ||
         (rc = dev_alloc(&net->path_raw, pel)) ||
-        (rc = dev_alloc((void**)&net->partials, (size_t)net->partial_blocks * CMF_LOG_COUNT * 8))) {
+        (rc = dev_alloc((void**)&net->partials, (size_t)net->partial_blocks * CMF_LOG_COUNT * 8)) ||
+        (rc = dev_alloc((void**)&net->bflag, (((size_t)n * (size_t)num_trials + 3) / 4) * 4)) ||
+        (rc = upload((void**)&net->pend_list, pend.data(), pend.size() * sizeof(int4)))) {
         free_network(net);
         return rc;
+    }
+    net->pend_count = pend_n;
+    {
+        const hipError_t me = hipMemset(net->bflag, 0, (((size_t)n * (size_t)num_trials + 3) / 4) * 4);
+        if (me == hipSuccess) (void)hipDeviceSynchronize();
+        if (me != hipSuccess) {
+            free_network(net);
+            return set_error(CMF_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(me));
+        }
     }
     if (local) {
         net->local_block = kLocalBlock;
@@ -1711,7 +1772,8 @@ int cmf_network_is_contiguous(const cmf_network* net) { return net ? net->contig
 
 int cmf_network_set_schedule(cmf_network* net, int32_t schedule) {
     if (!net) return set_error(CMF_ERR_INVALID, "null network");
-    if (schedule != CMF_SCHEDULE_THREE_PASS && schedule != CMF_SCHEDULE_BLOCK_LOCAL)
+    if (schedule != CMF_SCHEDULE_THREE_PASS && schedule != CMF_SCHEDULE_BLOCK_LOCAL &&
+        schedule != CMF_SCHEDULE_LIMITER_PASS)
         return set_error(CMF_ERR_INVALID, "unknown schedule %d", schedule);
     net->schedule = schedule;
     // captured graphs hold the launches of the other schedule

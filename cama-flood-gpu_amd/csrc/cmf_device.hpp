@@ -157,6 +157,10 @@ struct View {
     REAL *pos_term, *neg_term, *qraw_r, *qraw_f, *lim;
     REAL *path_pos, *path_neg, *path_net, *path_raw;
     double* partials;
+    // provisional-limiter schedule: nonzero at the elements whose limiter
+    // the outflow pass could not form exactly (an upstream catchment flows
+    // back into them); set by the outflow pass, cleared by the fix-up pass
+    uint8_t* bflag;
     // groups (G > 0): independent sets of basins, each with its own sub-step
     // count and time step (slot = group of the catchment).  Catchment kernels
     // then run one block per record of blk_rec: {first catchment, end, block
@@ -256,6 +260,18 @@ struct View {
     __device__ __forceinline__ void split_p(int64_t g, int64_t& t, int64_t& p) const {
         split(g, P, pts == 1, t, p);
     }
+    // element index -> (trial, catchment) (one trial: the element is the
+    // catchment, unit stride)
+    __device__ __forceinline__ void elem_split(int64_t e, int64_t& t, int64_t& i) const {
+        if (T == 1) { t = 0; i = e; }
+        else if (ts == 1) { i = e / cs; t = e - i * cs; }
+        else { t = e / ts; i = e - t * ts; }
+    }
+    // entry g of a per-catchment list run once per trial -> (trial, entry)
+    __device__ __forceinline__ void list_split(int64_t g, int64_t& t, int64_t& k) const {
+        if (T == 1) { t = 0; k = g; }
+        else { k = g / T; t = g - k * T; }
+    }
     // GroupView: the group of a launch's catchments is that of its block record
     static constexpr bool kGrouped = false;
 };
@@ -284,6 +300,8 @@ struct PlainView : View<REAL, STO> {
     }
     __device__ __forceinline__ void split_c(int64_t g, int64_t& t, int64_t& i) const { t = 0; i = g; }
     __device__ __forceinline__ void split_p(int64_t g, int64_t& t, int64_t& p) const { t = 0; p = g; }
+    __device__ __forceinline__ void elem_split(int64_t e, int64_t& t, int64_t& i) const { t = 0; i = e; }
+    __device__ __forceinline__ void list_split(int64_t g, int64_t& t, int64_t& k) const { t = 0; k = g; }
     __device__ __forceinline__ int64_t partial_slot() const { return (int64_t)blockIdx.x; }
     __device__ __forceinline__ bool catchment_of(int64_t g, int64_t& t, int64_t& i) const {
         t = 0;
@@ -319,6 +337,16 @@ struct MemberView : View<REAL, STO> {
     __device__ __forceinline__ void split_p(int64_t g, int64_t& t, int64_t& p) const {
         p = item_of(g);
         t = g - (p << log2T);
+    }
+    // element / list entry -> (member, catchment / entry), per lane (a
+    // lane's element need not share the wave's catchment)
+    __device__ __forceinline__ void elem_split(int64_t e, int64_t& t, int64_t& i) const {
+        i = e >> log2T;
+        t = e - (i << log2T);
+    }
+    __device__ __forceinline__ void list_split(int64_t g, int64_t& t, int64_t& k) const {
+        k = g >> log2T;
+        t = g - (k << log2T);
     }
     __device__ __forceinline__ int64_t slot(int64_t t, int64_t) const {
         return this->has(CMF_PER_TRIAL_DT) ? t : 0;

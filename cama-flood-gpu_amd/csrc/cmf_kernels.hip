@@ -90,7 +90,11 @@ struct OutflowLocal {
     STO rs, fs;
 };
 
-template <class REAL, class STO, bool FUSED, class VT>
+// PROV (fused, not block-local): instead of the outgoing terms, the storage
+// limiter formed from the catchment's own positive term alone, and a flag
+// at the downstream catchment when this one flows back into it (see
+// k_limiter_fix).
+template <class REAL, class STO, bool FUSED, class VT, bool PROV = false>
 __device__ __forceinline__ void outflow_one(const VT& v, int substep, int64_t blk, int store_last,
                                             OutflowLocal<REAL, STO>* local = nullptr, uint8_t lflags = 0) {
     int64_t t, i;
@@ -265,6 +269,23 @@ __device__ __forceinline__ void outflow_one(const VT& v, int substep, int64_t bl
         local->fs = fs_s;
         return;
     }
+    if constexpr (PROV) {
+        // The limiter (outflow.py:346-352) sums the negative terms of the
+        // upstream catchments, then the own positive term, then the path
+        // terms.  Without backflow from upstream those terms are zeros, the
+        // sum is the own term, and the limiter is known here.  A catchment
+        // with backflow flags its downstream catchment, whose limiter the
+        // fix-up pass forms in full, as it does for every path end.
+        STO acc = (STO)0;
+        acc += (STO)pos_t;
+        const REAL og = (REAL)acc;
+        const REAL rate = (REAL)(rs_s + fs_s);
+        const REAL l = (og > (REAL)1e-8) ? vmin(rate / og, (REAL)1.0) : (REAL)1.0;
+        v.lim[e] = l;
+        if (v.has(CMF_HAS_BIFURCATION) && fin) v.limit_rate[e] = l;
+        if (neg_t != (REAL)0.0) v.bflag[ed] = 1;
+        return;
+    }
     v.pos_term[e] = pos_t;
     v.neg_term[e] = neg_t;
 }
@@ -272,6 +293,13 @@ __device__ __forceinline__ void outflow_one(const VT& v, int substep, int64_t bl
 template <class REAL, class STO, bool FUSED, class VT = View<REAL, STO>>
 __global__ __launch_bounds__(kMaxBlock) void k_outflow(VT v, int substep, int store_last) {
     outflow_one<REAL, STO, FUSED>(v, substep, xcd_block(), store_last);
+}
+
+// fused, provisional limiters instead of outgoing terms (the three-pass
+// schedule, before k_limiter_fix)
+template <class REAL, class STO, class VT>
+__global__ __launch_bounds__(kMaxBlock) void k_outflow_prov(VT v, int substep, int store_last) {
+    outflow_one<REAL, STO, true, VT, true>(v, substep, xcd_block(), store_last);
 }
 
 // Outgoing total of catchment j from the catchment terms: upstream negative
@@ -529,6 +557,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_outflow_paths(VT v, int substep, 
     const int64_t b = blockIdx.x;
     if (b < path_blocks) bif_path<REAL, STO, LEVEE, LBFIX, true>(v, substep, b * blockDim.x + threadIdx.x);
     else outflow_one<REAL, STO, true>(v, substep, b - path_blocks, store_last);
+}
+
+// the same with provisional limiters (the three-pass schedule)
+template <class REAL, class STO, bool LEVEE, int LBFIX, class VT>
+__global__ __launch_bounds__(kMaxBlock) void k_outflow_paths_prov(VT v, int substep, int64_t path_blocks,
+                                                              int store_last) {
+    const int64_t b = blockIdx.x;
+    if (b < path_blocks) bif_path<REAL, STO, LEVEE, LBFIX, true>(v, substep, b * blockDim.x + threadIdx.x);
+    else outflow_one<REAL, STO, true, VT, true>(v, substep, b - path_blocks, store_last);
 }
 
 // fused: bifurcation total of catchment j (compute_bifurcation_inflow,
@@ -1311,6 +1348,120 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(VT v, int substep, i
     }
 }
 
+// The storage limiter of element (t, x) in full (outflow.py:306-352): the
+// negative terms of its upstream catchments (range rg) in ascending index,
+// its own positive term, then its path terms (events [eb, ee) of the event
+// list, in list order as bif_terms_from), each catchment term formed from the
+// unlimited flows by the outflow pass's own expressions, so that every sum is
+// the limiter pass's.  An upstream catchment u of x drains into x != u, so
+// it is no mouth.  Bound by dependent round trips: with contiguous ranges
+// the flows of the first kHoist upstream catchments and the first two event
+// codes are loaded with the catchment's own fields, before any sum.
+template <class REAL, class STO, bool CONTIG, class VT>
+__device__ __forceinline__ REAL limiter_exact(const VT& v, int64_t t, int64_t x, int2 rg, int32_t eb, int32_t ee) {
+    constexpr int kHoist = 4;
+    const int64_t e = v.el(t, x);
+    const REAL dt = v.dt(t, x);
+    const STO rs = v.river_storage[e], fs = v.flood_storage[e];
+    const REAL qr = v.qraw_r[e], qf = v.qraw_f[e];
+    REAL hr[kHoist], hf[kHoist];
+    if (CONTIG) {
+#pragma unroll
+        for (int k = 0; k < kHoist; ++k) {
+            const int64_t u = (int64_t)rg.x + k;
+            const bool in = u < (int64_t)rg.y && u != x;
+            const int64_t eu = v.el(t, in ? u : x);
+            hr[k] = in ? v.qraw_r[eu] : (REAL)0.0;
+            hf[k] = in ? v.qraw_f[eu] : (REAL)0.0;
+        }
+    }
+    const int32_t c0 = (eb < ee) ? v.ev_code[eb] : 0;
+    const int32_t c1 = (eb + 1 < ee) ? v.ev_code[eb + 1] : 0;
+    STO acc = (STO)0;
+    for (int32_t k = rg.x; k < rg.y; ++k) {
+        const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
+        if (CONTIG && u == x) continue;
+        REAL ur, uf;
+        if (CONTIG && k - rg.x < kHoist) {
+            // registers of the hoisted loads (k - rg.x is the slot)
+            ur = (REAL)0.0;
+            uf = (REAL)0.0;
+#pragma unroll
+            for (int h = 0; h < kHoist; ++h)
+                if (h == k - rg.x) { ur = hr[h]; uf = hf[h]; }
+        } else {
+            const int64_t eu = v.el(t, u);
+            ur = v.qraw_r[eu];
+            uf = v.qraw_f[eu];
+        }
+        const REAL neg = vmin(ur, (REAL)0.0) + vmin(uf, (REAL)0.0);
+        acc += (STO)(-neg * dt);
+    }
+    const REAL pos = vmax(qr, (REAL)0.0) + vmax(qf, (REAL)0.0);
+    acc += (STO)(pos * dt);
+    for (int32_t k = eb; k < ee; ++k) {
+        const int32_t code = (k == eb) ? c0 : (k == eb + 1) ? c1 : v.ev_code[k];
+        const int64_t pe = v.pel(t, code >> 1);
+        acc += (STO)((code & 1) ? v.path_neg[pe] : v.path_pos[pe]);
+    }
+    const REAL og = (REAL)acc;
+    const REAL rate = (REAL)(rs + fs);
+    return (og > (REAL)1e-8) ? vmin(rate / og, (REAL)1.0) : (REAL)1.0;
+}
+
+template <class REAL, class STO, class VT>
+__device__ __forceinline__ void store_limiter(const VT& v, int64_t t, int64_t x, REAL l, int substep,
+                                              int store_last) {
+    const int64_t e = v.el(t, x);
+    v.lim[e] = l;
+    if (v.has(CMF_HAS_BIFURCATION) && v.last_substep(t, x, substep, store_last)) v.limit_rate[e] = l;
+}
+
+// Fix-up pass of the provisional-limiter schedule, after the outflow pass
+// (PROV), before the stage pass: the full limiter of every path end and of
+// every element an upstream backflow flagged; every other limiter is exact
+// as the outflow pass left it.
+//   blocks [0, list_blocks): the path ends, once per trial, from the static
+//     records `list` ({catchment, upstream range start, end, first event};
+//     entry list_n holds the end of the last one's events in .w), so that
+//     the catchment's terms and its event codes are one round trip away;
+//   the other blocks: four flags per thread (one 32-bit load, cleared after
+//     use); path ends among them are left to the list blocks (the limiter is
+//     formed before that test so that its loads issue with the test's).
+template <class REAL, class STO, bool CONTIG, class VT>
+__global__ __launch_bounds__(kMaxBlock) void k_limiter_fix(VT v, int substep, int store_last,
+                                                        const int4* __restrict__ list, int64_t list_n,
+                                                        int64_t list_blocks) {
+    const int64_t b = blockIdx.x;
+    if (b < list_blocks) {
+        const int64_t g = b * blockDim.x + threadIdx.x;
+        if (g >= list_n * v.T) return;
+        int64_t t, k;
+        v.list_split(g, t, k);
+        const int4 r = list[k];
+        const int32_t ee = list[k + 1].w;
+        if (!v.active(t, r.x, substep)) return;
+        const REAL l = limiter_exact<REAL, STO, CONTIG>(v, t, r.x, make_int2(r.y, r.z), r.w, ee);
+        store_limiter<REAL, STO>(v, t, r.x, l, substep, store_last);
+        return;
+    }
+    const int64_t w = (b - list_blocks) * blockDim.x + threadIdx.x;
+    if (4 * w >= v.N * v.T) return;
+    uint32_t* words = reinterpret_cast<uint32_t*>(v.bflag);
+    const uint32_t f = words[w];
+    if (f == 0u) return;
+    words[w] = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (((f >> (8 * k)) & 0xffu) == 0u) continue;
+        int64_t t, x;
+        v.elem_split(4 * w + k, t, x);
+        const bool pend = list_n > 0 && v.ev1[x].x >= 0;
+        const REAL l = limiter_exact<REAL, STO, CONTIG>(v, t, x, v.up_range(x), 0, 0);
+        if (!pend) store_limiter<REAL, STO>(v, t, x, l, substep, store_last);
+    }
+}
+
 // Where the stage pass reads the limiters and the unlimited flows: the
 // network scratch (StageGlobal), or the LDS rows of the block-local schedule
 // (StageLds: entries of the block's catchments [base, base + blockDim.x)).
@@ -1355,9 +1506,6 @@ __device__ __forceinline__ void stage_fused_one(const VT& v, int64_t t, int64_t 
     const int2 rg = v.up_range(j);
     const int32_t b = rg.x, en = rg.y;
     const STO rs = v.river_storage[e], fs = v.flood_storage[e];
-    // without the levee module every catchment holds +0.0 protected
-    // storage after the first sub-step's stage pass
-    const STO ps = (!v.has(CMF_HAS_LEVEE) && substep > 0) ? (STO)0.0 : v.protected_storage[e];
     const REAL ro = v.prm(v.runoff, CMF_BATCHED_RUNOFF, e, j);
     const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, j);
     const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, j);
@@ -1367,8 +1515,13 @@ __device__ __forceinline__ void stage_fused_one(const VT& v, int64_t t, int64_t 
     const bool res = v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
     const bool events = v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[j] != v.ev_ptr[j + 1];
     // round trip 2
+    // after the first sub-step's stage pass every catchment without a levee
+    // holds +0.0 protected storage
+    const STO ps = (substep > 0 && !lev_cell) ? (STO)0.0 : v.protected_storage[e];
+    // the downstream limiter scales only the catchment's own negative flows
     const int64_t ed = v.el(t, d);
-    const REAL l_dn = src.lim(v, ed, d);
+    const bool own_back = !(qr >= (REAL)0.0) || !(qf >= (REAL)0.0);
+    const REAL l_dn = own_back ? src.lim(v, ed, d) : (REAL)1.0;
     TableRow<REAL, LFIX> row;
     bool row_loaded = false;
     REAL ca = (REAL)-1;

@@ -313,16 +313,22 @@ int cmf_network_destroy(cmf_network* net);
 /* 1 when every upstream range is the identity (breadth-first layout). */
 int cmf_network_is_contiguous(const cmf_network* net);
 
-/* Schedule of the fused sub-step (both give the same bits; no counterpart in
- * the reference).  THREE_PASS (the default): outflow (+ paths), limiter and
- * stage passes, each over the whole network.  BLOCK_LOCAL (one trial,
+/* Schedule of the fused sub-step (all give the same bits; no counterpart in
+ * the reference).  THREE_PASS (the default): an outflow pass (+ paths) that
+ * also leaves each catchment's storage limiter as its own outgoing term
+ * fixes it, a fix-up pass that forms in full the limiters of path ends and
+ * of catchments an upstream catchment flows back into (flagged by the
+ * outflow pass), and the stage pass; on grouped networks, with reservoirs
+ * or while taping, LIMITER_PASS runs instead.  LIMITER_PASS:
+ * outflow (+ paths), limiter and stage passes, each over the whole network.
+ * BLOCK_LOCAL (one trial,
  * ungrouped, contiguous upstream ranges, no reservoirs, 10 flood levels, 5
  * path levels, not taped; otherwise the three passes run): one pass in which
  * each block of 512 catchments completes the limiters and stages it holds
  * whole, then limiter and stage passes over the catchments left (lists fixed
  * at network creation).  Measured slower on MI355X than THREE_PASS; kept as
  * an option. */
-enum { CMF_SCHEDULE_THREE_PASS = 0, CMF_SCHEDULE_BLOCK_LOCAL = 1 };
+enum { CMF_SCHEDULE_THREE_PASS = 0, CMF_SCHEDULE_BLOCK_LOCAL = 1, CMF_SCHEDULE_LIMITER_PASS = 2 };
 int cmf_network_set_schedule(cmf_network* net, int32_t schedule);
 /* The block-local schedule's block size (0: not available for this network)
  * and the sizes of its limiter and stage lists. */

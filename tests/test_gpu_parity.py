@@ -289,9 +289,12 @@ NO_RES = ("base", "adaptive_time", "bifurcation", "levee")
                                   "reservoirs", "keyed"])
 def test_schedules_bitexact(case):
     """The block-local schedule (one pass completing whole blocks, then the
-    limiter and stage lists) and the three-pass schedule against the oracle,
-    bit for bit; levee catchments started at 1-60 times their river storage so
-    that the levee cases and floodplains are in play."""
+    limiter and stage lists), the three-pass schedule (provisional limiters
+    from the outflow pass, fixed up at path ends and below backflow) and the
+    limiter-pass schedule against the oracle, bit for bit; levee catchments
+    started at 1-60 times their river storage so that the levee cases and
+    floodplains are in play.  Backflow must occur (non-mouth catchments with
+    negative outflow), so that the fix-up of flagged limiters is exercised."""
     import torch
     kw, okw = {}, {}
     mods = NO_RES
@@ -319,7 +322,8 @@ def test_schedules_bitexact(case):
     src["river_storage"] = rs
     # the basin layout unless the case names another (the block-local
     # schedule completes most catchments inside their block there)
-    models = {name: _model(src, mods, **{"layout": "basin", **kw}) for name in ("block_local", "three_pass")}
+    models = {name: _model(src, mods, **{"layout": "basin", **kw})
+              for name in ("block_local", "three_pass", "limiter_pass")}
     for name, m in models.items():
         m.set_schedule(name)
     if case == "network_levee":
@@ -338,7 +342,7 @@ def test_schedules_bitexact(case):
         if case == "log":
             # FP64 sums folded in another block order: relative 1e-12
             c = models["three_pass"].last_log_sums
-            for name in ("block_local",):
+            for name in ("block_local", "limiter_pass"):
                 a = models[name].last_log_sums
                 assert np.all(np.abs(a - c) <= 1e-12 * np.maximum(np.abs(c), 1e-9) + 1e-15), (name, step)
     torch.cuda.synchronize()
@@ -350,6 +354,10 @@ def test_schedules_bitexact(case):
                 continue
             msg = mismatch(m.get(f), orc.state[f])
             assert msg is None, f"{name} {f}: {msg}"
+    dn = np.asarray(src["downstream_idx"])
+    inland = dn != np.arange(dn.size)
+    back = (orc.state["river_outflow"] < 0) | (orc.state["flood_outflow"] < 0)
+    assert np.count_nonzero(back & inland) > 0, "no backflow: the limiter fix-up was not exercised"
 
 
 def test_fp32_storage_batched_members_match_standalone_runs():
