@@ -96,12 +96,16 @@ def algorithmic_bytes(kernel: str, model, f_wet: float) -> float:
 SCHEDULE = "three_pass"
 
 
-def build_model(net, src, members: int, device: str, layout: str, precise: bool = False,
-                precision: str = "float32"):
-    m = _build_model(net, src, members, device, layout, precise, precision)
+def scheduled(m):
+    """The model with the bench's sub-step schedule applied."""
     if SCHEDULE != "three_pass":
         m.set_schedule(SCHEDULE)
     return m
+
+
+def build_model(net, src, members: int, device: str, layout: str, precise: bool = False,
+                precision: str = "float32"):
+    return scheduled(_build_model(net, src, members, device, layout, precise, precision))
 
 
 def _build_model(net, src, members: int, device: str, layout: str, precise: bool, precision: str):
@@ -285,8 +289,8 @@ def basins_line(args, device: str) -> dict:
     sizes = np.diff(bs.basin_ptr)
     out = {"basins": args.basins, "catchments": int(n)}
     for grouped, graph in ((True, False), (True, True), (False, False)):
-        m = CaMaFlood(src, opened_modules=("base", "adaptive_time"), device=device, step_duration=1800.0,
-                      groups=bs.group if grouped else None, use_graph=graph)
+        m = scheduled(CaMaFlood(src, opened_modules=("base", "adaptive_time"), device=device, step_duration=1800.0,
+                                groups=bs.group if grouped else None, use_graph=graph))
         for _ in range(max(args.warmup // 4, 2)):
             m.step_advance()
         torch.cuda.synchronize()
@@ -539,8 +543,8 @@ def graph_line(args, device: str) -> dict:
     src["runoff"] = synthetic.Runoff(src["catchment_area"], 2000).next_day()
     out = {"workload": "60,000-catchment plain network", "catchments": net.num_catchments}
     for use_graph in (False, True):
-        m = CaMaFlood(src, opened_modules=("base", "adaptive_time"), device=device, step_duration=1800.0,
-                      use_graph=use_graph)
+        m = scheduled(CaMaFlood(src, opened_modules=("base", "adaptive_time"), device=device, step_duration=1800.0,
+                                use_graph=use_graph))
         for _ in range(8):
             m.step_advance()
         torch.cuda.synchronize()

@@ -106,14 +106,18 @@ struct cmf_network {
     int32_t* stage_list = nullptr;
     int64_t stage_count = 0;
     int schedule = CMF_SCHEDULE_THREE_PASS;
-    // three-pass schedule: the flags of the provisional limiters (one byte
-    // per element, whole 32-bit words, zero between sub-steps) and the
-    // path-end catchments, whose limiters the fix-up pass always forms:
-    // {catchment, upstream range start, end, first event} in ascending
-    // catchment order, and a last record whose .w ends the events
-    uint8_t* bflag = nullptr;
+    // three-pass schedule: the flags of the provisional limiters (one bit
+    // per element, zero between sub-steps), the path-end catchments, whose
+    // limiters the fix-up pass always forms ({catchment, upstream range
+    // start, end, first event} in ascending catchment order, and a last
+    // record whose .w ends the events), the event slots of each path's two
+    // ends and the path terms at those slots (2 P per trial)
+    uint32_t* bmask = nullptr;
+    int64_t bmask_words = 0;
     int4* pend_list = nullptr;
     int64_t pend_count = 0;
+    int2* pev = nullptr;
+    void* eterm = nullptr;
     // adjoint scratch (cmf_hip_substep_adjoint), allocated on first use:
     // 14 arrays of N elements of up to 8 bytes
     void* adj_scratch = nullptr;
@@ -131,7 +135,7 @@ void free_network(cmf_network* net) {
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
                     net->path_neg, net->path_net, net->path_raw, net->partials, net->cgroup,
                     net->blk_rec_id, net->level_rec, net->level_pos, net->gblk_ptr, net->lflags, net->lim_list,
-                    net->stage_list, net->bflag, net->pend_list};
+                    net->stage_list, net->bmask, net->pend_list, net->pev, net->eterm};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (net->host_counts) (void)hipHostFree(net->host_counts);
@@ -357,7 +361,9 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.path_net = static_cast<REAL*>(net->path_net);
     v.path_raw = static_cast<REAL*>(net->path_raw);
     v.partials = net->partials;
-    v.bflag = net->bflag;
+    v.bmask = net->bmask;
+    v.pev = net->pev;
+    v.eterm = nullptr;
     return CMF_OK;
 }
 
@@ -809,11 +815,15 @@ int launch_outflow(const cmf_network* net, const VT& v, int64_t NT, int substep,
 
 // The three passes of one fused sub-step, on view type VT (PlainView for one
 // plain trial, where the per-trial and per-group indexing folds away).
-// THREE_PASS on an ungrouped network without reservoirs, not taped: outflow
-// with provisional limiters, the limiter fix-up, stage; otherwise outflow,
-// [reservoir], limiter, stage.
+// THREE_PASS without reservoirs, not taped: outflow with provisional
+// limiters, the limiter fix-up, stage; otherwise outflow, [reservoir],
+// limiter, stage.
 template <class REAL, class STO, class VT>
-int three_pass_substep(cmf_network* net, const VT& v, int substep, int store_inflow, void* stream) {
+int three_pass_substep(cmf_network* net, const VT& v_in, int substep, int store_inflow, void* stream) {
+    const bool prov = net->schedule == CMF_SCHEDULE_THREE_PASS && !v_in.has(CMF_HAS_RESERVOIR) && !v_in.tape_q_r &&
+                      net->bmask;
+    VT v = v_in;
+    if (prov) v.eterm = static_cast<REAL*>(net->eterm);
     // catchment kernels: N*T threads, or one block per group table entry
     const bool ordered = net->G > 0 && v.ordered && substep >= 0 && substep < (int)net->sub_blocks.size();
     const int64_t NT = net->G > 0 ? (ordered ? net->sub_blocks[substep] : net->nblk) * kGroupRecord : v.N * v.T;
@@ -822,8 +832,6 @@ int three_pass_substep(cmf_network* net, const VT& v, int substep, int store_inf
     const bool log = v.has(CMF_HAS_LOG);
     const bool lev_fused = levee && v.levee_index != nullptr;
     const int64_t b_lim = block_of(net, CMF_TUNE_LIMITER, true), b_stage = block_of(net, CMF_TUNE_STAGE, true);
-    const bool prov = !VT::kGrouped && net->schedule == CMF_SCHEDULE_THREE_PASS && net->G == 0 &&
-                      !v.has(CMF_HAS_RESERVOIR) && !v.tape_q_r && net->bflag;
     int rc = prov ? launch_outflow<REAL, STO, VT, true>(net, v, NT, substep, store_inflow, stream)
                   : launch_outflow<REAL, STO, VT, false>(net, v, NT, substep, store_inflow, stream);
     if (rc) return rc;
@@ -831,21 +839,33 @@ int three_pass_substep(cmf_network* net, const VT& v, int substep, int store_inf
         LAUNCH_B((k_reservoir<REAL, STO, true>), v.R * v.T, block_of(net, CMF_TUNE_RESERVOIR, false), stream, v, substep);
     // limiters of every catchment, then the stage pass (which also forms the
     // path nets from the limiters of both path ends)
-    if constexpr (VT::kGrouped) {
+    if (prov) {
+        // path ends (per trial) in leading blocks, then the flag words: the
+        // whole network, or one wave per group record of the sub-step
+        const int64_t list_n = bif ? net->pend_count : 0;
+        const int4* list = net->pend_list;
+        if (net->G > 0) {
+            const View<REAL, STO> gv = v;
+            const int64_t lb = (list_n * v.T + 63) / 64, nrec = NT / kGroupRecord;
+            if (contig) LAUNCH_B((k_limiter_fix<REAL, STO, true, View<REAL, STO>>), (lb + nrec) * 64, 64, stream, gv,
+                                 substep, store_inflow, list, list_n, lb, (const int4*)gv.blk_rec);
+            else LAUNCH_B((k_limiter_fix<REAL, STO, false, View<REAL, STO>>), (lb + nrec) * 64, 64, stream, gv,
+                          substep, store_inflow, list, list_n, lb, (const int4*)gv.blk_rec);
+        } else {
+            const int64_t lb = (list_n * v.T + b_lim - 1) / b_lim;
+            const int64_t words = (v.N * v.T + 31) / 32;
+            const int64_t cnt = (lb + (words + b_lim - 1) / b_lim) * b_lim;
+            if (contig) LAUNCH_B((k_limiter_fix<REAL, STO, true, VT>), cnt, b_lim, stream, v, substep, store_inflow, list,
+                                 list_n, lb, (const int4*)nullptr);
+            else LAUNCH_B((k_limiter_fix<REAL, STO, false, VT>), cnt, b_lim, stream, v, substep, store_inflow, list,
+                          list_n, lb, (const int4*)nullptr);
+        }
+    } else if constexpr (VT::kGrouped) {
         // one block of kGroupRecord / 2 threads per group table entry
         if (contig)
             LAUNCH_B((k_limiter_pair<REAL, STO, true, VT>), NT / 2, kGroupRecord / 2, stream, v, substep, store_inflow);
         else
             LAUNCH_B((k_limiter_pair<REAL, STO, false, VT>), NT / 2, kGroupRecord / 2, stream, v, substep, store_inflow);
-    } else if (prov) {
-        // path ends (per trial) in leading blocks, then four flags per thread
-        const int64_t list_n = bif ? net->pend_count : 0;
-        const int64_t lb = (list_n * v.T + b_lim - 1) / b_lim;
-        const int64_t words = (v.N * v.T + 3) / 4;
-        const int64_t cnt = (lb + (words + b_lim - 1) / b_lim) * b_lim;
-        const int4* list = net->pend_list;
-        if (contig) LAUNCH_B((k_limiter_fix<REAL, STO, true, VT>), cnt, b_lim, stream, v, substep, store_inflow, list, list_n, lb);
-        else LAUNCH_B((k_limiter_fix<REAL, STO, false, VT>), cnt, b_lim, stream, v, substep, store_inflow, list, list_n, lb);
     } else if (net->G > 0) {
         // the group block table maps one catchment per thread
         if (contig) LAUNCH_B((k_limiter<REAL, STO, true>), NT, b_lim, stream, v, substep, store_inflow);
@@ -1597,11 +1617,14 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     std::vector<int32_t> ev_ptr((size_t)n + 1, 0), ev_code((size_t)(2 * num_paths)), ev_other((size_t)(2 * num_paths));
     for (int64_t p = 0; p < num_paths; ++p) { ev_ptr[bif_ci[p] + 1]++; ev_ptr[bif_di[p] + 1]++; }
     for (int64_t i = 0; i < n; ++i) ev_ptr[i + 1] += ev_ptr[i];
+    std::vector<int2> pev((size_t)std::max<int64_t>(num_paths, 1));
     {
         std::vector<int32_t> fill(ev_ptr.begin(), ev_ptr.end() - 1);
         for (int64_t p = 0; p < num_paths; ++p) {
+            pev[(size_t)p].x = fill[bif_ci[p]];
             ev_other[fill[bif_ci[p]]] = bif_di[p];
             ev_code[fill[bif_ci[p]]++] = (int32_t)(2 * p);
+            pev[(size_t)p].y = fill[bif_di[p]];
             ev_other[fill[bif_di[p]]] = bif_ci[p];
             ev_code[fill[bif_di[p]]++] = (int32_t)(2 * p + 1);
         }
@@ -1660,14 +1683,17 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
         (rc = dev_alloc(&net->path_neg, pel)) || (rc = dev_alloc(&net->path_net, pel)) ||
         (rc = dev_alloc(&net->path_raw, pel)) ||
         (rc = dev_alloc((void**)&net->partials, (size_t)net->partial_blocks * CMF_LOG_COUNT * 8)) ||
-        (rc = dev_alloc((void**)&net->bflag, (((size_t)n * (size_t)num_trials + 3) / 4) * 4)) ||
-        (rc = upload((void**)&net->pend_list, pend.data(), pend.size() * sizeof(int4)))) {
+        (rc = dev_alloc((void**)&net->bmask, (((size_t)n * (size_t)num_trials + 31) / 32) * 4)) ||
+        (rc = upload((void**)&net->pend_list, pend.data(), pend.size() * sizeof(int4))) ||
+        (rc = upload((void**)&net->pev, pev.data(), pev.size() * sizeof(int2))) ||
+        (rc = dev_alloc(&net->eterm, 2 * pel))) {
         free_network(net);
         return rc;
     }
     net->pend_count = pend_n;
     {
-        const hipError_t me = hipMemset(net->bflag, 0, (((size_t)n * (size_t)num_trials + 3) / 4) * 4);
+        net->bmask_words = ((int64_t)n * num_trials + 31) / 32;
+        const hipError_t me = hipMemset(net->bmask, 0, (size_t)net->bmask_words * 4);
         if (me == hipSuccess) (void)hipDeviceSynchronize();
         if (me != hipSuccess) {
             free_network(net);

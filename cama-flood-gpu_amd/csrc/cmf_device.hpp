@@ -157,10 +157,14 @@ struct View {
     REAL *pos_term, *neg_term, *qraw_r, *qraw_f, *lim;
     REAL *path_pos, *path_neg, *path_net, *path_raw;
     double* partials;
-    // provisional-limiter schedule: nonzero at the elements whose limiter
-    // the outflow pass could not form exactly (an upstream catchment flows
-    // back into them); set by the outflow pass, cleared by the fix-up pass
-    uint8_t* bflag;
+    // provisional-limiter schedule: one bit per element, set where the
+    // outflow pass could not form the limiter exactly (an upstream catchment
+    // flows back into it), cleared by the fix-up pass; and each path's two
+    // outgoing terms at its ends' event slots (eterm[evel(t, pev[p].x)]: the
+    // upstream end's, .y the downstream end's; eterm == nullptr: not kept)
+    uint32_t* bmask;
+    REAL* eterm;
+    const int2* pev;
     // groups (G > 0): independent sets of basins, each with its own sub-step
     // count and time step (slot = group of the catchment).  Catchment kernels
     // then run one block per record of blk_rec: {first catchment, end, block
@@ -186,6 +190,10 @@ struct View {
 
     __device__ __forceinline__ int64_t el(int64_t t, int64_t i) const { return t * ts + i * cs; }
     __device__ __forceinline__ int64_t pel(int64_t t, int64_t p) const { return t * pts + p * pps; }
+    // element of event slot k (2 P slots per trial, laid out as the paths)
+    __device__ __forceinline__ int64_t evel(int64_t t, int64_t k) const {
+        return t * (pts == 1 ? 1 : 2 * P) + k * pps;
+    }
     __host__ __device__ __forceinline__ bool has(uint32_t bit) const { return (flags & bit) != 0; }
     __device__ __forceinline__ int2 up_range(int64_t j) const {
         return up_rng ? up_rng[j] : make_int2(up_ptr[j], up_ptr[j + 1]);
@@ -287,6 +295,7 @@ template <class REAL, class STO, bool DEVLAST = false>
 struct PlainView : View<REAL, STO> {
     __device__ __forceinline__ int64_t el(int64_t, int64_t i) const { return i; }
     __device__ __forceinline__ int64_t pel(int64_t, int64_t p) const { return p; }
+    __device__ __forceinline__ int64_t evel(int64_t, int64_t k) const { return k; }
     __host__ __device__ __forceinline__ bool bat(uint32_t) const { return false; }
     template <class X>
     __device__ __forceinline__ X prm(const X* p, uint32_t, int64_t, int64_t i) const { return p[i]; }
@@ -327,6 +336,7 @@ struct MemberView : View<REAL, STO> {
     int log2T;
     __device__ __forceinline__ int64_t el(int64_t t, int64_t i) const { return (i << log2T) + t; }
     __device__ __forceinline__ int64_t pel(int64_t t, int64_t p) const { return (p << log2T) + t; }
+    __device__ __forceinline__ int64_t evel(int64_t t, int64_t k) const { return (k << log2T) + t; }
     __device__ __forceinline__ int64_t item_of(int64_t g) const {
         return WAVE ? (int64_t)__builtin_amdgcn_readfirstlane((int)(g >> log2T)) : (g >> log2T);
     }

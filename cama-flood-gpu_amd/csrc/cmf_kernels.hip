@@ -283,7 +283,8 @@ __device__ __forceinline__ void outflow_one(const VT& v, int substep, int64_t bl
         const REAL l = (og > (REAL)1e-8) ? vmin(rate / og, (REAL)1.0) : (REAL)1.0;
         v.lim[e] = l;
         if (v.has(CMF_HAS_BIFURCATION) && fin) v.limit_rate[e] = l;
-        if (neg_t != (REAL)0.0) v.bflag[ed] = 1;
+        if (neg_t != (REAL)0.0)
+            __hip_atomic_fetch_or(v.bmask + (ed >> 5), 1u << (ed & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
     v.pos_term[e] = pos_t;
@@ -537,9 +538,17 @@ __device__ __forceinline__ void bif_path(const VT& v, int substep, int64_t g) {
     }
     const REAL pos = vmax(sum, (REAL)0.0);
     const REAL neg = vmin(sum, (REAL)0.0);
-    v.path_pos[pe] = pos * time_step;
-    v.path_neg[pe] = -neg * time_step;
+    const REAL pos_t = pos * time_step, neg_t = -neg * time_step;
+    v.path_pos[pe] = pos_t;
+    v.path_neg[pe] = neg_t;
     v.path_raw[pe] = raw;
+    if (v.eterm) {
+        // the same terms at the two ends' event slots, where the limiter
+        // fix-up reads each path end's terms in event order
+        const int2 sl = v.pev[p];
+        v.eterm[v.evel(t, sl.x)] = pos_t;
+        v.eterm[v.evel(t, sl.y)] = neg_t;
+    }
 }
 
 // per-op: compute_bifurcation_outflow
@@ -1351,12 +1360,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter_pair(VT v, int substep, i
 // The storage limiter of element (t, x) in full (outflow.py:306-352): the
 // negative terms of its upstream catchments (range rg) in ascending index,
 // its own positive term, then its path terms (events [eb, ee) of the event
-// list, in list order as bif_terms_from), each catchment term formed from the
-// unlimited flows by the outflow pass's own expressions, so that every sum is
-// the limiter pass's.  An upstream catchment u of x drains into x != u, so
-// it is no mouth.  Bound by dependent round trips: with contiguous ranges
-// the flows of the first kHoist upstream catchments and the first two event
-// codes are loaded with the catchment's own fields, before any sum.
+// list, in list order as bif_terms_from, read from the event slots the path
+// pass filled: eterm), each catchment term formed from the unlimited flows by
+// the outflow pass's own expressions, so that every sum is the limiter
+// pass's.  An upstream catchment u of x drains into x != u, so it is no
+// mouth.  Bound by dependent round trips: with contiguous ranges the flows
+// of the first kHoist upstream catchments and the first two path terms are
+// loaded with the catchment's own fields, before any sum.
 template <class REAL, class STO, bool CONTIG, class VT>
 __device__ __forceinline__ REAL limiter_exact(const VT& v, int64_t t, int64_t x, int2 rg, int32_t eb, int32_t ee) {
     constexpr int kHoist = 4;
@@ -1375,8 +1385,8 @@ __device__ __forceinline__ REAL limiter_exact(const VT& v, int64_t t, int64_t x,
             hf[k] = in ? v.qraw_f[eu] : (REAL)0.0;
         }
     }
-    const int32_t c0 = (eb < ee) ? v.ev_code[eb] : 0;
-    const int32_t c1 = (eb + 1 < ee) ? v.ev_code[eb + 1] : 0;
+    const REAL p0 = (eb < ee) ? v.eterm[v.evel(t, eb)] : (REAL)0.0;
+    const REAL p1 = (eb + 1 < ee) ? v.eterm[v.evel(t, eb + 1)] : (REAL)0.0;
     STO acc = (STO)0;
     for (int32_t k = rg.x; k < rg.y; ++k) {
         const int64_t u = CONTIG ? (int64_t)k : (int64_t)v.up_idx[k];
@@ -1399,11 +1409,7 @@ __device__ __forceinline__ REAL limiter_exact(const VT& v, int64_t t, int64_t x,
     }
     const REAL pos = vmax(qr, (REAL)0.0) + vmax(qf, (REAL)0.0);
     acc += (STO)(pos * dt);
-    for (int32_t k = eb; k < ee; ++k) {
-        const int32_t code = (k == eb) ? c0 : (k == eb + 1) ? c1 : v.ev_code[k];
-        const int64_t pe = v.pel(t, code >> 1);
-        acc += (STO)((code & 1) ? v.path_neg[pe] : v.path_pos[pe]);
-    }
+    for (int32_t k = eb; k < ee; ++k) acc += (STO)((k == eb) ? p0 : (k == eb + 1) ? p1 : v.eterm[v.evel(t, k)]);
     const REAL og = (REAL)acc;
     const REAL rate = (REAL)(rs + fs);
     return (og > (REAL)1e-8) ? vmin(rate / og, (REAL)1.0) : (REAL)1.0;
@@ -1424,14 +1430,21 @@ __device__ __forceinline__ void store_limiter(const VT& v, int64_t t, int64_t x,
 //   blocks [0, list_blocks): the path ends, once per trial, from the static
 //     records `list` ({catchment, upstream range start, end, first event};
 //     entry list_n holds the end of the last one's events in .w), so that
-//     the catchment's terms and its event codes are one round trip away;
-//   the other blocks: four flags per thread (one 32-bit load, cleared after
-//     use); path ends among them are left to the list blocks (the limiter is
-//     formed before that test so that its loads issue with the test's).
+//     the catchment's fields, its upstream flows and its path terms (eterm)
+//     are one round trip away;
+//   the other blocks: 32-element words of the flag bits, one per lane (the
+//     whole network, or with `rec` the words of one catchment record per
+//     block of one wave: the group records of the sub-step), cleared after
+//     use, the flagged elements dealt out one per lane of the wave; path
+//     ends among them are left to the list blocks (the limiter is formed
+//     before that test so that its loads issue with the test's).  A word two
+//     records share may be read by both: each of its flags is then formed
+//     twice, to the same value.  VT must give any element its own time-step
+//     slot (not GroupView, whose slot is the block's record's).
 template <class REAL, class STO, bool CONTIG, class VT>
 __global__ __launch_bounds__(kMaxBlock) void k_limiter_fix(VT v, int substep, int store_last,
                                                         const int4* __restrict__ list, int64_t list_n,
-                                                        int64_t list_blocks) {
+                                                        int64_t list_blocks, const int4* __restrict__ rec) {
     const int64_t b = blockIdx.x;
     if (b < list_blocks) {
         const int64_t g = b * blockDim.x + threadIdx.x;
@@ -1445,17 +1458,47 @@ __global__ __launch_bounds__(kMaxBlock) void k_limiter_fix(VT v, int substep, in
         store_limiter<REAL, STO>(v, t, r.x, l, substep, store_last);
         return;
     }
-    const int64_t w = (b - list_blocks) * blockDim.x + threadIdx.x;
-    if (4 * w >= v.N * v.T) return;
-    uint32_t* words = reinterpret_cast<uint32_t*>(v.bflag);
-    const uint32_t f = words[w];
-    if (f == 0u) return;
-    words[w] = 0u;
+    // the flagged elements of the wave's words are dealt out one per lane
+    // (wave-wide prefix of the bit counts, then each lane finds the word and
+    // bit of its flag), so that a word with several flags costs no longer
+    // chain than one with a single flag.  Every lane takes part in the
+    // shuffles: no lane leaves before them.
+    const int lane = (int)(threadIdx.x & 63);
+    int64_t w;
+    if (rec) {
+        const int4 r = rec[b - list_blocks];
+        const int64_t wa = (int64_t)r.x >> 5, wb = ((int64_t)r.y - 1) >> 5;
+        w = (lane <= wb - wa) ? wa + lane : -1;
+    } else {
+        w = (b - list_blocks) * blockDim.x + threadIdx.x;
+        if (32 * w >= v.N * v.T) w = -1;
+    }
+    const uint32_t f = (w >= 0) ? v.bmask[w] : 0u;
+    if (f != 0u) v.bmask[w] = 0u;
+    const int cnt = __popc(f);
+    int incl = cnt;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (((f >> (8 * k)) & 0xffu) == 0u) continue;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    const int excl = incl - cnt;
+    const int total = __shfl(incl, 63, 64);
+    const int wlo = (int)(w & 0xffffffff), whi = (int)(w >> 32);
+    for (int base = 0; base < total; base += 64) {
+        const int k = base + lane;
+        // the last lane s whose flags start at or before flag k
+        int s = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1)
+            if (__shfl(excl, s + step, 64) <= k) s += step;
+        uint32_t fw = __shfl(f, s, 64);
+        const int r = k - __shfl(excl, s, 64);
+        const int64_t ws = (int64_t)(uint32_t)__shfl(wlo, s, 64) | ((int64_t)__shfl(whi, s, 64) << 32);
+        if (k >= total) continue;
+        for (int i = 0; i < r; ++i) fw &= fw - 1u;
         int64_t t, x;
-        v.elem_split(4 * w + k, t, x);
+        v.elem_split(32 * ws + __builtin_ctz(fw), t, x);
         const bool pend = list_n > 0 && v.ev1[x].x >= 0;
         const REAL l = limiter_exact<REAL, STO, CONTIG>(v, t, x, v.up_range(x), 0, 0);
         if (!pend) store_limiter<REAL, STO>(v, t, x, l, substep, store_last);
