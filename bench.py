@@ -58,23 +58,19 @@ def algorithmic_bytes(kernel: str, model, f_wet: float) -> float:
         # writes: two flows, three cross-section fields and either the two
         # outgoing terms or, with provisional limiters (_prov), the limiter
         base = kernel.split("<")[0]
-        per = 84 + (24 if base.endswith("_prov") else 28) + (5 if lev else 0)
-        return per * N * T + (paths if base.startswith("k_outflow_paths") else 0)
+        prov = base.endswith("_prov")
+        per = 84 + (24 if prov else 28) + (5 if lev else 0)
+        # with _prov the path pass also writes its two terms at the ends' event slots
+        return per * N * T + ((paths + (8 * P * T if prov else 0)) if base.startswith("k_outflow_paths") else 0)
     if kernel.startswith("k_limiter_fix"):
-        # the flags (1 B per element), and per path end and trial its record
-        # (16 B), storages (16 B), unlimited flows (8 B) and limiter (4 B),
-        # the flows of its upstream catchments (8 B each) and per event the
-        # code and the path term (8 B); backflow catchments are not counted
+        # the flag bits (1/8 B per element), and per path end and trial its
+        # record (16 B), storages (16 B), unlimited flows (8 B) and limiter
+        # (4 B), the flows of its upstream catchments (8 B each) and per event
+        # the path term (4 B); backflow catchments are not counted
         if not bif:
-            return 1.0 * N * T
-        import numpy as np
-        hf = model.host.fields
-        ci = np.asarray(hf["bifurcation_catchment_idx"])
-        di = np.asarray(hf["bifurcation_downstream_idx"])
-        ends = np.unique(np.concatenate([ci, di]))
-        dn = np.asarray(hf["downstream_idx"])
-        ups = np.bincount(dn[dn != np.arange(dn.size)], minlength=dn.size)[ends].sum()
-        return 1.0 * N * T + T * (ends.size * 44.0 + 8.0 * ups + 8.0 * 2 * P)
+            return N * T / 8.0
+        ends, ups = path_end_counts(model)
+        return N * T / 8.0 + T * (ends * 44.0 + 8.0 * ups + 4.0 * 2 * P)
     if kernel.startswith("k_limiter"):
         per = 4 + 4 + 4 + 16 + 4 + (8 if bif else 0)
         return per * N * T
@@ -90,6 +86,18 @@ def algorithmic_bytes(kernel: str, model, f_wet: float) -> float:
     if kernel.startswith("k_bif_inflow"):
         return (8 + 8 + 5 * 8 + 4) * P * T
     return 0.0
+
+
+def path_end_counts(model):
+    """(catchments that end a bifurcation path, their upstream catchments)."""
+    import numpy as np
+    hf = model.host.fields
+    ci = np.asarray(hf["bifurcation_catchment_idx"])
+    di = np.asarray(hf["bifurcation_downstream_idx"])
+    ends = np.unique(np.concatenate([ci, di]))
+    dn = np.asarray(hf["downstream_idx"])
+    ups = np.bincount(dn[dn != np.arange(dn.size)], minlength=dn.size)[ends].sum()
+    return int(ends.size), int(ups)
 
 
 # schedule of the fused sub-step for every model the bench builds (--schedule)
@@ -223,7 +231,8 @@ def batched_line(args, net, src, device: str, plain_value: float, plain_ref_valu
         belapsed = time.perf_counter() - t2
         bvalue = bwork / belapsed
         bf_wet = float((bm.t["flood_depth"] > 0).float().mean().item())
-        bgbs = batched_full_bytes(bm.N, M, bm.P, bf_wet) * (bwork / (bm.N * M)) / belapsed / 1e9
+        prov = path_end_counts(bm) if SCHEDULE == "three_pass" else None
+        bgbs = batched_full_bytes(bm.N, M, bm.P, bf_wet, prov) * (bwork / (bm.N * M)) / belapsed / 1e9
         cgbs = fused_batched_bytes(bm.N, M, bm.V, bm.P, bf_wet) * (bwork / (bm.N * M)) / belapsed / 1e9
         base = plain_value if precise else plain_ref_value
         rec = {"value": bvalue, "steps": bsteps, "warmup": "one simulated day (48 outer steps)",
@@ -310,7 +319,9 @@ def basins_line(args, device: str) -> dict:
         # catchment-sub-step times the measured rate (wall clock, CFL pass and
         # launch gaps included)
         f_wet = float((m.t["flood_depth"] > 0).float().mean().item())
-        per = sum(algorithmic_bytes(k, m, f_wet) for k in ("k_outflow", "k_limiter_pair", "k_stage_fused")) / n
+        kern = (("k_outflow_prov", "k_limiter_fix", "k_stage_fused") if SCHEDULE == "three_pass"
+                else ("k_outflow", "k_limiter_pair", "k_stage_fused"))
+        per = sum(algorithmic_bytes(k, m, f_wet) for k in kern) / n
         gbs = work / el * per / 1e9
         cgbs = work / el * fused_substep_bytes(n, 0, 0, f_wet) / n / 1e9
         out[key] = {"outer_steps_per_s": steps / el, "catchment_steps_per_s": work / el,
@@ -362,7 +373,7 @@ def fused_batched_bytes(N: int, M: int, V: int, P: int, f_wet: float, LB: int = 
     return b
 
 
-def batched_full_bytes(N: int, M: int, P: int, f_wet: float) -> float:
+def batched_full_bytes(N: int, M: int, P: int, f_wet: float, prov=None) -> float:
     """Algorithmic bytes of one fused sub-step of M trial-inner members with
     full physics, per-member data counted once per member and the shared
     parameters once per catchment (a sub-step after the first: the write-only
@@ -379,8 +390,20 @@ def batched_full_bytes(N: int, M: int, P: int, f_wet: float) -> float:
       per path-member 112 B: five levels of flow and depth (60 B), three path
       terms (12 B), the nets' rescale of the five flows (40 B);
       per path 72 B: five levels of Manning's n, width and elevation, length
-      and the two ends."""
-    return N * M * 184.0 + N * (69.0 + 44.0 * f_wet) + P * M * 112.0 + P * 72.0
+      and the two ends.
+    prov = (path ends, their upstream catchments): the provisional-limiter
+    schedule, whose outflow writes the limiter (4 B) instead of the two terms
+    and which has no limiter pass (152 B per member-catchment); its fix-up
+    reads the flag bits (1/8 B per member-catchment), per path end its record
+    (16 B) and per path end and member the storages, flows and limiter (28 B)
+    and the flows of its upstream catchments (8 B each); the path pass writes
+    its two terms at the ends' event slots and the fix-up reads them (16 B
+    per path-member)."""
+    if prov is None:
+        return N * M * 184.0 + N * (69.0 + 44.0 * f_wet) + P * M * 112.0 + P * 72.0
+    ends, ups = prov
+    return (N * M * (152.0 + 0.125) + N * (69.0 + 44.0 * f_wet) + P * M * 128.0 + P * 72.0
+            + ends * 16.0 + M * (ends * 28.0 + ups * 8.0))
 
 
 def _reaches_below(src: dict, start: np.ndarray) -> np.ndarray:

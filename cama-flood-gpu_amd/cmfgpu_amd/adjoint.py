@@ -120,6 +120,15 @@ class GaugeProblem:
         if model.T != 1 or model.G or model.batched_names or model.mode != "fused":
             raise ValueError("the adjoint needs one trial, an ungrouped network, no per-trial parameters and "
                              "the fused schedule")
+        # the adjoint takes the protected storage of a catchment without a
+        # levee as zero, which is what every stage pass leaves there
+        ps = model.t.get("protected_storage")
+        if ps is not None:
+            lev = model.t.get("is_levee") if "levee" in model.opened_modules else None
+            outside = ps if lev is None else ps[~lev.bool()]
+            if bool((outside != 0).any()):
+                raise ValueError("the adjoint needs zero protected storage outside levee catchments in the "
+                                 "initial state")
         self.model = model
         self.paths = "bifurcation" in model.opened_modules and model.P > 0
         self.levees = "levee" in model.opened_modules
@@ -185,9 +194,11 @@ class GaugeProblem:
     def _log_obs(self, step):
         return torch.log(self.obs[step].clamp_min(0.0) + self.eps)
 
-    def _loss_seed(self, step, q):
+    def _loss_seeds(self, q):
+        """dL/dQ of every outer step and gauge at once ([S, G], from the
+        forward's gauge discharge q)."""
         qc = q.clamp_min(0.0)
-        r = torch.log(qc + self.eps) - self._log_obs(step)
+        r = torch.log(qc + self.eps) - torch.log(self.obs.clamp_min(0.0) + self.eps)
         return torch.where(q > 0, 2.0 * r / (qc + self.eps), torch.zeros_like(q)) / (self.steps * self.obs.shape[1])
 
     def _substep(self, k: int, last: bool):
@@ -208,6 +219,8 @@ class GaugeProblem:
         self.counts = []
         self.checkpoints = {}
         total = torch.zeros((), dtype=torch.float64, device=self.model.device)
+        # the gauge discharge after every outer step, for the backward's seeds
+        self.q_hist = torch.empty(self.obs.shape, dtype=torch.float64, device=self.model.device)
         for step in range(self.steps):
             if step % self.K == 0:
                 self.checkpoints[step] = self._snapshot()
@@ -218,7 +231,9 @@ class GaugeProblem:
                 n = int(fixed_counts[step])
                 self._step_fixed(n)
             self.counts.append(n)
-            total += self._loss_term(step, self._gauge_q())
+            q = self._gauge_q()
+            self.q_hist[step] = q
+            total += self._loss_term(step, q)
         self.final = self._snapshot()
         self.last_loss = float(total.item()) / (self.steps * self.obs.shape[1])
         return self.last_loss
@@ -290,6 +305,9 @@ class GaugeProblem:
             tapes.append(tp)
             adjs.append(pair)
         lib = _lib.lib()
+        # the taped re-runs reproduce the forward bit for bit, so the seeds of
+        # every outer step come from the forward's gauge discharge at once
+        seeds = self._loss_seeds(self.q_hist).to(real_t)
         parity = 0
         first_done = False
         for s0 in reversed(starts):
@@ -304,10 +322,10 @@ class GaugeProblem:
                 _lib.check(lib.cmf_hip_advance_taped(m.network_handle, m.field_table_ref(), m.step_duration, cnt,
                                                      arr, _lib.current_stream()), "cmf_hip_advance_taped")
                 slot += cnt
-                marks.append((step, slot, self._gauge_q()))
-            for step, end, q in reversed(marks):
+                marks.append((step, slot))
+            for step, end in reversed(marks):
                 cnt = self.counts[step]
-                seed = self._loss_seed(step, q).to(real_t)
+                seed = seeds[step]
                 lam[parity][QR].index_add_(0, self.g_dev, seed)
                 lam[parity][QF].index_add_(0, self.g_dev, seed)
                 self._apply_runoff(step)

@@ -91,6 +91,15 @@ constexpr int kAdjMaxLevels = 8;
 template <class STO>
 __device__ __forceinline__ STO adj_sto(const void* p, int64_t i) { return static_cast<const STO*>(p)[i]; }
 
+// Protected storage of catchment i in the saved state: the taped value at a
+// levee catchment, zero elsewhere (a stage pass leaves zero outside levee
+// catchments, and the adjoint's driver refuses an initial state that holds
+// any there).
+template <class REAL, class STO>
+__device__ __forceinline__ REAL adj_ps(const View<REAL, STO>& v, const AdjView<REAL>& a, int64_t i) {
+    return (a.lev && v.is_levee[i]) ? (REAL)adj_sto<STO>(a.ps_raw, i) : (REAL)0.0;
+}
+
 // ---------------------------------------------------- outflow trace ----
 // The arithmetic of k_outflow (FUSED, no reservoir override), same
 // operations in the same order, keeping what the reverse needs.
@@ -124,15 +133,15 @@ __device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v
     const REAL r_wid = v.river_width[i];
     const REAL r_len = v.river_length[i];
     const REAL r_hgt = v.river_height[i];
-    const REAL f_out = a.q_f[i];
-    const REAL f_man = v.flood_manning[i];
+    // the floodplain flow, Manning's n and cross-section area are read where
+    // the floodplain flows (flood_cond) or below a dam
     const REAL f_dep = a.fd[i];
     const REAL c_elv = v.elev[i];
     const REAL dn_dist = v.dist[i];
     const REAL rs = (REAL)adj_sto<STO>(a.rs_raw, i);
     const REAL fs = (REAL)adj_sto<STO>(a.fs_raw, i);
-    const REAL ps = a.lev ? (REAL)adj_sto<STO>(a.ps_raw, i) : (REAL)v.protected_storage[i];
-    const REAL r_csd = a.rcsd[i], f_csd = a.fcsd[i], f_csa = a.fcsa[i];
+    const REAL ps = adj_ps(v, a, i);
+    const REAL r_csd = a.rcsd[i], f_csd = a.fcsd[i];
 
     const REAL river_elevation = c_elv - r_hgt;
     o.wse = r_dep + river_elevation;
@@ -172,11 +181,14 @@ __device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v
     o.qr0 = q_r;
     o.flood_cond = (o.f_sifd > (REAL)1e-5) && (o.upd_fcsa > (REAL)1e-5);
     REAL q_f = (REAL)0.0;
-    o.f_csa_m = vmax(f_csa, (REAL)1e-6);
+    o.f_csa_m = (REAL)1e-6;
     o.f_imp_raw = (REAL)0;
     o.f_imp = (REAL)1;
     o.den_f = (REAL)1;
     if (o.flood_cond) {
+        const REAL f_out = a.q_f[i];
+        const REAL f_man = v.flood_manning[i];
+        o.f_csa_m = vmax(a.fcsa[i], (REAL)1e-6);
         o.f_imp_raw = dsqrt(o.upd_fcsa * o.f_csa_m);
         o.f_imp = vmax(o.f_imp_raw, (REAL)1e-6);
         const REAL num_f = f_out + gravity * time_step * o.f_imp * o.fslope;
@@ -214,7 +226,7 @@ __device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v
         o.kr_pos = kin_riv > (REAL)0.0;
         kin_riv = vmax(kin_riv, (REAL)0.0);
         const REAL bed_slope_f = vmin(bed_slope, (REAL)0.005);
-        o.kf_coef = ((REAL)1.0 / f_man) * dsqrt(bed_slope_f);
+        o.kf_coef = ((REAL)1.0 / v.flood_manning[i]) * dsqrt(bed_slope_f);
         const REAL kin_fld_vel = o.kf_coef * dcbrt(f_dep * f_dep);
         o.kf_area_raw = fs / r_len - f_dep * r_wid;
         const REAL kin_fld_area = vmax(o.kf_area_raw, (REAL)0.0);
@@ -282,12 +294,6 @@ __device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v
     return o;
 }
 
-// protected storage of catchment i in the saved state (with levees) or the
-// field (without: +0.0 after every stage pass)
-template <class REAL, class STO>
-__device__ __forceinline__ REAL adj_ps(const View<REAL, STO>& v, const AdjView<REAL>& a, int64_t i) {
-    return a.lev ? (REAL)adj_sto<STO>(a.ps_raw, i) : (REAL)v.protected_storage[i];
-}
 // protected water surface of catchment i from its river surface wse, as
 // protected_surface_of forms it (levee.py:565-600)
 template <class REAL, class STO>
@@ -561,8 +567,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     if (j >= v.N) return;
     const REAL dt = a.dt;
     const int64_t d = v.dn[j];
-    const REAL l_own = a.lim[j], l_dn = a.lim[d];
+    const REAL l_own = a.lim[j];
     const REAL qr = a.qr[j], qf = a.qf[j];
+    // the downstream limiter scales only the catchment's own negative flows
+    const REAL l_dn = (!(qr >= (REAL)0.0) || !(qf >= (REAL)0.0)) ? a.lim[d] : (REAL)1.0;
     const REAL rout = (qr >= (REAL)0.0) ? qr * l_own : qr * l_dn;
     const REAL fout = (qf >= (REAL)0.0) ? qf * l_own : qf * l_dn;
     STO ri = (STO)0, fi = (STO)0;
@@ -593,7 +601,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     }
     const REAL gb = (REAL)gsum;
     const STO rs = adj_sto<STO>(a.rs_raw, j), fs = adj_sto<STO>(a.fs_raw, j);
-    const STO ps = a.lev ? adj_sto<STO>(a.ps_raw, j) : v.protected_storage[j];
+    const int32_t lv = a.lev ? v.levee_index[j] : -1;
+    const STO ps = (lv >= 0) ? adj_sto<STO>(a.ps_raw, j) : (STO)0.0;
     const REAL ro = v.runoff[j];
     // storage update (storage.py:81-94)
     const STO sa = rs + (STO)((rin - rout) * dt);
@@ -659,7 +668,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
         cap_pick = ((STO)cap < total_hp);
         rsf = cap_pick ? (STO)cap : total_hp;
     }
-    const int32_t lv = a.lev ? v.levee_index[j] : -1;
     if (lv >= 0) {
         // levee stage on the stage's output, then its reverse: the adjoints of
         // the stage's river / flood storage and depths, and of the bankfull
@@ -696,7 +704,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     }
     // total_hp = max(total_next, 0); total_next = rsu2 + fsu1 + ...
     const double Ltn = (total_next > (STO)0.0) ? Ltot : 0.0;
-    if (a.lev) a.lin_ps[j] = (REAL)Ltn;
+    // the protected storage of a catchment without a levee stays zero: its
+    // adjoint is never read
+    if (lv >= 0) a.lin_ps[j] = (REAL)Ltn;
     double Lrsu1 = 0.0, Lb = 0.0, La = 0.0;
     if (sb > (STO)0.0) Lb += Ltn;
     if (b_neg) {
@@ -712,7 +722,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     a.a_fout[j] = (REAL)(Lfout + (double)a.lout[ADJ_QF][j]);
     a.a_ri[j] = (REAL)Lrin;
     a.a_fi[j] = (REAL)Lfin;
-    if (a.bif) a.a_gb[j] = (REAL)(-Lb * (double)dt);
+    // read by the path reverse and the limiter reverse at path ends only
+    if (a.bif && v.ev_ptr[j] != v.ev_ptr[j + 1]) a.a_gb[j] = (REAL)(-Lb * (double)dt);
     a.lin[ADJ_RS][j] = (REAL)Lrs;
     a.lin[ADJ_FS][j] = (REAL)Lfs;
     a.g_h[j] += LH - pend;
@@ -730,7 +741,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_route(View<REAL, STO> v, AdjView
     const int64_t d = v.dn[k];
     const bool mouth = (d == k);
     const REAL qr = a.qr[k], qf = a.qf[k];
-    const REAL l_own = a.lim[k], l_dn = a.lim[d];
+    const REAL l_own = a.lim[k];
+    const REAL l_dn = (!(qr >= (REAL)0.0) || !(qf >= (REAL)0.0)) ? a.lim[d] : (REAL)1.0;
     // a reservoir downstream also gathers the limited flows into its inflow
     const double Lres = (a.res && !mouth && v.is_reservoir[d]) ? (double)a.lout_ri[d] : 0.0;
     const double LR = (double)a.a_rout[k] + (mouth ? 0.0 : (double)a.a_ri[d]) + Lres;
@@ -740,6 +752,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_route(View<REAL, STO> v, AdjView
     double Llim = 0.0;
     if (qr >= (REAL)0.0 || mouth) Llim += LR * (double)qr;
     if (qf >= (REAL)0.0 || mouth) Llim += LF * (double)qf;
+    // The limiter binds (og > 1e-8 and rate / og < 1) exactly where the
+    // forward's limiter min(rate / og, 1) is below 1; elsewhere its adjoint
+    // is zero and nothing below is needed.
+    if (!(l_own < (REAL)1.0)) {
+        a.a_og[k] = (REAL)0.0;
+        return;
+    }
     const double Lri_k = (double)a.a_ri[k], Lfi_k = (double)a.a_fi[k];
     // the outgoing total of k, as the forward formed it: upstream negative
     // terms in order, then k's own positive term
@@ -1076,8 +1095,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
     const double Lqf0 = o.killed ? 0.0 : Lqf1;
     double Lro = 0.0, Lfo = 0.0, Lr_sifd = 0.0, Lf_sifd = 0.0, Lf_imp = 0.0, Lslope = 0.0, Lfslope = 0.0;
     double gn = 0.0;
-    const double fo = (double)a.q_f[i];
     if (o.flood_cond) {
+        const double fo = (double)a.q_f[i];
         const double fm = (double)v.flood_manning[i];
         const double Lnum = Lqf0 / (double)o.den_f, Lden = -Lqf0 * (double)o.qf0 / (double)o.den_f;
         Lfo += Lnum;
@@ -1152,7 +1171,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
             if (a.lev) Lpd += (double)((code & 1) ? a.p_d1[p] : a.p_d0[p]);
         }
     }
-    if (a.lev) {
+    if (a.lev && v.is_levee[i]) {
         a.lin_ps[i] = (REAL)((double)a.lin_ps[i] + Ltot);
         a.lin_pd[i] = (REAL)Lpd;
     }

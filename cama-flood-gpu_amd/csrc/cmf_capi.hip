@@ -815,13 +815,13 @@ int launch_outflow(const cmf_network* net, const VT& v, int64_t NT, int substep,
 
 // The three passes of one fused sub-step, on view type VT (PlainView for one
 // plain trial, where the per-trial and per-group indexing folds away).
-// THREE_PASS without reservoirs, not taped: outflow with provisional
-// limiters, the limiter fix-up, stage; otherwise outflow, [reservoir],
-// limiter, stage.
+// THREE_PASS without reservoirs: outflow with provisional limiters, the
+// limiter fix-up, stage (taped: the unlimited flows and limiters land in the
+// tape, the same values as from the limiter pass); otherwise outflow,
+// [reservoir], limiter, stage.
 template <class REAL, class STO, class VT>
 int three_pass_substep(cmf_network* net, const VT& v_in, int substep, int store_inflow, void* stream) {
-    const bool prov = net->schedule == CMF_SCHEDULE_THREE_PASS && !v_in.has(CMF_HAS_RESERVOIR) && !v_in.tape_q_r &&
-                      net->bmask;
+    const bool prov = net->schedule == CMF_SCHEDULE_THREE_PASS && !v_in.has(CMF_HAS_RESERVOIR) && net->bmask;
     VT v = v_in;
     if (prov) v.eterm = static_cast<REAL*>(net->eterm);
     // catchment kernels: N*T threads, or one block per group table entry

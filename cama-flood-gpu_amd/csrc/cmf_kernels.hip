@@ -224,7 +224,9 @@ __device__ __forceinline__ void outflow_one(const VT& v, int substep, int64_t bl
         v.tape_fd[e] = f_dep;
         v.tape_rs[e] = v.river_storage[e];
         v.tape_fs[e] = v.flood_storage[e];
-        if (v.tape_ps) {
+        // levee catchments only: every other catchment holds zero protected
+        // storage (the adjoint takes it as zero there)
+        if (v.tape_ps && lev_i) {
             v.tape_ps[e] = v.protected_storage[e];
             v.tape_pd[e] = v.protected_depth[e];
         }
