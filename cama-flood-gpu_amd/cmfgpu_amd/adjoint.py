@@ -114,9 +114,6 @@ class GaugeProblem:
         if extra:
             raise ValueError(f"the adjoint covers the base physics, bifurcation, levee and reservoir; modules "
                              f"{sorted(extra)} are open")
-        if getattr(model, "precise_surfaces", False):
-            raise ValueError("the adjoint forms water surfaces as the reference does; build the model without "
-                             "precise_surfaces")
         if model.T != 1 or model.G or model.batched_names or model.mode != "fused":
             raise ValueError("the adjoint needs one trial, an ungrouped network, no per-trial parameters and "
                              "the fused schedule")

@@ -106,7 +106,7 @@ __device__ __forceinline__ REAL adj_ps(const View<REAL, STO>& v, const AdjView<R
 template <class REAL>
 struct OutTrace {
     bool mouth, river_cond, flood_cond, killed, negative, limited, mx_own, slope_in;
-    REAL wse, wse_dn, slope, fslope, upd_rcsd, r_sifd_raw, r_sifd, fsurf, upd_fcsd, f_sifd_raw, f_sifd;
+    REAL wse, wse_dn, slope, fslope, upd_rcsd, r_sifd_raw, r_sifd, fsd, upd_fcsd, f_sifd_raw, f_sifd;
     REAL upd_fcsa_raw, upd_fcsa, f_csa_m, f_imp_raw, f_imp, unit_r, den_r, qr0, den_f, qf0, qf1, tot, total_neg;
     REAL ratio, limit, qr, qf;
     // upstream of a dam: the kinematic flows (raw, and which clamps held)
@@ -143,25 +143,44 @@ __device__ __forceinline__ OutTrace<REAL> outflow_trace(const View<REAL, STO>& v
     const REAL ps = adj_ps(v, a, i);
     const REAL r_csd = a.rcsd[i], f_csd = a.fcsd[i];
 
-    const REAL river_elevation = c_elv - r_hgt;
-    o.wse = r_dep + river_elevation;
     o.tot = rs + fs + ps;
     const REAL r_dep_dn = a.rd[d];
     const REAL r_hgt_dn = v.river_height[d];
     const REAL c_elv_dn = v.elev[d];
-    const REAL river_elevation_dn = c_elv_dn - r_hgt_dn;
-    o.wse_dn = r_dep_dn + river_elevation_dn;
-    if (o.mouth) o.wse_dn = c_elv;
-    o.mx_own = (o.wse > o.wse_dn || o.wse_dn != o.wse_dn);
-    const REAL max_wse = vmax(o.wse, o.wse_dn);
-    o.slope = (o.wse - o.wse_dn) / dn_dist;
+    // the surfaces, their drop and the depths above bed and bank as the
+    // forward forms them: in FP32 as the reference does, or with precise
+    // surfaces in double, each difference rounded once (the derivatives are
+    // the same; only the values that pick the branches differ)
+    REAL dwse;
+    if (precise_surfaces(v)) {
+        const double rel = (double)c_elv - (double)r_hgt;
+        const double w = (double)r_dep + rel;
+        const double w_dn = o.mouth ? (double)c_elv : (double)r_dep_dn + ((double)c_elv_dn - (double)r_hgt_dn);
+        const double mw = vmax(w, w_dn);
+        o.wse = (REAL)w;
+        o.wse_dn = (REAL)w_dn;
+        o.mx_own = (w > w_dn || w_dn != w_dn);
+        dwse = (REAL)(w - w_dn);
+        o.upd_rcsd = o.mouth ? r_dep : (REAL)(mw - rel);
+        o.fsd = (REAL)((o.mouth ? w : mw) - (double)c_elv);
+    } else {
+        const REAL river_elevation = c_elv - r_hgt;
+        o.wse = r_dep + river_elevation;
+        const REAL river_elevation_dn = c_elv_dn - r_hgt_dn;
+        o.wse_dn = r_dep_dn + river_elevation_dn;
+        if (o.mouth) o.wse_dn = c_elv;
+        o.mx_own = (o.wse > o.wse_dn || o.wse_dn != o.wse_dn);
+        const REAL max_wse = vmax(o.wse, o.wse_dn);
+        dwse = o.wse - o.wse_dn;
+        o.upd_rcsd = o.mouth ? r_dep : (max_wse - river_elevation);
+        o.fsd = (o.mouth ? o.wse : max_wse) - c_elv;
+    }
+    o.slope = dwse / dn_dist;
     o.fslope = vclamp(o.slope, (REAL)-0.005, (REAL)0.005);
     o.slope_in = (o.slope > (REAL)-0.005) && (o.slope < (REAL)0.005);
-    o.upd_rcsd = o.mouth ? r_dep : (max_wse - river_elevation);
     o.r_sifd_raw = dsqrt(o.upd_rcsd * r_csd);
     o.r_sifd = vmax(o.r_sifd_raw, (REAL)1e-6);
-    o.fsurf = o.mouth ? o.wse : max_wse;
-    o.upd_fcsd = vmax(o.fsurf - c_elv, (REAL)0.0);
+    o.upd_fcsd = vmax(o.fsd, (REAL)0.0);
     o.f_sifd_raw = dsqrt(o.upd_fcsd * f_csd);
     o.f_sifd = vmax(o.f_sifd_raw, (REAL)1e-6);
     o.upd_fcsa_raw = fs / r_len - f_dep * r_wid;
@@ -301,6 +320,17 @@ __device__ __forceinline__ REAL adj_pwse(const View<REAL, STO>& v, const AdjView
     if (!v.is_levee[i]) return wse;
     return vmin(v.elev[i] + a.pd[i], wse);
 }
+// the same two in double, as surface_of_d / protected_surface_of_d form them
+// with precise surfaces
+template <class REAL, class STO>
+__device__ __forceinline__ double adj_surface_d(const View<REAL, STO>& v, const AdjView<REAL>& a, int64_t i) {
+    return (double)a.rd[i] + ((double)v.elev[i] - (double)v.river_height[i]);
+}
+template <class REAL, class STO>
+__device__ __forceinline__ double adj_pwse_d(const View<REAL, STO>& v, const AdjView<REAL>& a, int64_t i, double w) {
+    if (!v.is_levee[i]) return w;
+    return vmin((double)v.elev[i] + (double)a.pd[i], w);
+}
 
 // ------------------------------------------------------- path trace ----
 // The arithmetic of bif_path (non-levee variant, end surfaces and totals
@@ -328,11 +358,23 @@ __device__ __forceinline__ void path_trace(const View<REAL, STO>& v, const AdjVi
     o.w1 = a.rd[di] + (v.elev[di] - v.river_height[di]);
     o.ts0 = (REAL)adj_sto<STO>(a.rs_raw, ci) + (REAL)adj_sto<STO>(a.fs_raw, ci) + adj_ps(v, a, ci);
     o.ts1 = (REAL)adj_sto<STO>(a.rs_raw, di) + (REAL)adj_sto<STO>(a.fs_raw, di) + adj_ps(v, a, di);
-    o.max_own = (o.w0 > o.w1 || o.w1 != o.w1);
     o.max_w = vmax(o.w0, o.w1);
     REAL max_pw = (REAL)0;
     if (a.lev) max_pw = vmax(adj_pwse(v, a, ci, o.w0), adj_pwse(v, a, di, o.w1));
-    const REAL sx = (o.w0 - o.w1) / length;
+    // with precise surfaces the forward forms the surfaces, their drop and
+    // the depths above the sills in double (bif_path)
+    const bool prec = precise_surfaces(v);
+    double w0d = 0.0, w1d = 0.0, mwd = 0.0, mpwd = 0.0;
+    if (prec) {
+        w0d = adj_surface_d(v, a, ci);
+        w1d = adj_surface_d(v, a, di);
+        mwd = vmax(w0d, w1d);
+        if (a.lev) mpwd = vmax(adj_pwse_d(v, a, ci, w0d), adj_pwse_d(v, a, di, w1d));
+        o.max_own = (w0d > w1d || w1d != w1d);
+    } else {
+        o.max_own = (o.w0 > o.w1 || o.w1 != o.w1);
+    }
+    const REAL sx = (prec ? (REAL)(w0d - w1d) : (o.w0 - o.w1)) / length;
     o.slope_in = (sx > (REAL)-0.005) && (sx < (REAL)0.005);
     o.slope = vclamp(sx, (REAL)-0.005, (REAL)0.005);
     REAL sum = (REAL)0;
@@ -341,18 +383,19 @@ __device__ __forceinline__ void path_trace(const View<REAL, STO>& v, const AdjVi
         if (l >= LB) break;
         const int64_t k = p * LB + l;
         const REAL csd_old = a.bcsd[k];
+        const REAL bel = v.bif_elev[k];
         REAL upd, sa, sb = (REAL)0, sifd;
         if (!a.lev) {
-            upd = vmax(o.max_w - v.bif_elev[k], (REAL)0.0);
+            upd = vmax(prec ? (REAL)(mwd - (double)bel) : o.max_w - bel, (REAL)0.0);
             sa = dsqrt(upd * csd_old);
             sb = dsqrt(upd * (REAL)0.01);
             sifd = vmax(sa, sb);
         } else if (l == 0) {
-            upd = vmax(o.max_w - v.bif_elev[k], (REAL)0.0);
+            upd = vmax(prec ? (REAL)(mwd - (double)bel) : o.max_w - bel, (REAL)0.0);
             sa = dsqrt(upd * csd_old);
             sifd = (sa <= (REAL)0.0) ? upd : sa;
         } else {
-            upd = vmax(max_pw - v.bif_elev[k], (REAL)0.0);
+            upd = vmax(prec ? (REAL)(mpwd - (double)bel) : max_pw - bel, (REAL)0.0);
             sa = (REAL)0;
             sifd = upd;
         }
@@ -838,7 +881,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
     const REAL w1 = a.rd[di] + (v.elev[di] - v.river_height[di]);
     const REAL ts0 = (REAL)adj_sto<STO>(a.rs_raw, ci) + (REAL)adj_sto<STO>(a.fs_raw, ci) + adj_ps(v, a, ci);
     const REAL ts1 = (REAL)adj_sto<STO>(a.rs_raw, di) + (REAL)adj_sto<STO>(a.fs_raw, di) + adj_ps(v, a, di);
-    const bool max_own = (w0 > w1 || w1 != w1);
     const REAL max_w = vmax(w0, w1);
     // with levees the upper levels run on the protected surfaces
     REAL pw0 = w0, pw1 = w1, max_pw = (REAL)0;
@@ -847,27 +889,46 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
         pw1 = adj_pwse(v, a, di, w1);
         max_pw = vmax(pw0, pw1);
     }
+    // with precise surfaces the forward forms the surfaces, their drop and
+    // the depths above the sills in double (bif_path): every pick below
+    // follows those values
+    const bool prec = precise_surfaces(v);
+    double w0d = 0.0, w1d = 0.0, pw0d = 0.0, pw1d = 0.0, mwd = 0.0, mpwd = 0.0;
+    if (prec) {
+        w0d = adj_surface_d(v, a, ci);
+        w1d = adj_surface_d(v, a, di);
+        mwd = vmax(w0d, w1d);
+        if (a.lev) {
+            pw0d = adj_pwse_d(v, a, ci, w0d);
+            pw1d = adj_pwse_d(v, a, di, w1d);
+            mpwd = vmax(pw0d, pw1d);
+        }
+    }
+    const bool max_own = prec ? (w0d > w1d || w1d != w1d) : (w0 > w1 || w1 != w1);
     const bool upper = a.lev && l > 0;   // this lane's level on the protected surface
-    const REAL sx = (w0 - w1) / length;
+    const REAL sx = (prec ? (REAL)(w0d - w1d) : (w0 - w1)) / length;
     const bool slope_in = (sx > (REAL)-0.005) && (sx < (REAL)0.005);
     const REAL slope = vclamp(sx, (REAL)-0.005, (REAL)0.005);
     // level l of the forward
     const int64_t k = p * LB + (lv ? l : 0);
     REAL upd = (REAL)0, sa = (REAL)0, sb = (REAL)0, sifd = (REAL)0, unit = (REAL)0, den = (REAL)1, q = (REAL)0;
-    REAL csd_old = (REAL)0;
+    REAL csd_old = (REAL)0, above = (REAL)0;
     if (lv) {
         csd_old = a.bcsd[k];
+        const REAL bel = v.bif_elev[k];
+        // the depth above the sill: of the higher river surface, or of the
+        // higher protected surface on the upper levels with levees
+        if (!upper) above = prec ? (REAL)(mwd - (double)bel) : max_w - bel;
+        else above = prec ? (REAL)(mpwd - (double)bel) : max_pw - bel;
+        upd = vmax(above, (REAL)0.0);
         if (!a.lev) {
-            upd = vmax(max_w - v.bif_elev[k], (REAL)0.0);
             sa = dsqrt(upd * csd_old);
             sb = dsqrt(upd * (REAL)0.01);
             sifd = vmax(sa, sb);
         } else if (!upper) {
-            upd = vmax(max_w - v.bif_elev[k], (REAL)0.0);
             sa = dsqrt(upd * csd_old);
             sifd = (sa <= (REAL)0.0) ? upd : sa;
         } else {
-            upd = vmax(max_pw - v.bif_elev[k], (REAL)0.0);
             sifd = upd;
         }
         const REAL w = v.bif_width[k], man = v.bif_manning[k];
@@ -956,10 +1017,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
             Lupd += Lsifd * 0.5 * 0.01 / (double)sb;
         }
         // upd = max(surface - elevation, 0)
-        if (upper) {
-            if (max_pw - v.bif_elev[k] > (REAL)0.0) Lmax_pw += Lupd;
-        } else if (max_w - v.bif_elev[k] > (REAL)0.0) {
-            Lmax_w += Lupd;
+        if (above > (REAL)0.0) {
+            if (upper) Lmax_pw += Lupd;
+            else Lmax_w += Lupd;
         }
         a.lin_bq[k] = (REAL)Lqold;
         a.lin_bcsd[k] = (REAL)Lcsd;
@@ -984,12 +1044,23 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
         // max_pw = max(pw0, pw1); pw = min(elevation + protected depth, wse)
         // at a levee end, wse elsewhere
         double Lp0 = 0.0, Lp1 = 0.0, Ld0 = 0.0, Ld1 = 0.0;
-        if (pw0 > pw1 || pw1 != pw1) Lp0 = Lxp;
+        bool p0_max, lev0, lev1;
+        if (prec) {
+            const double e0 = (double)v.elev[ci] + (double)a.pd[ci], e1 = (double)v.elev[di] + (double)a.pd[di];
+            p0_max = (pw0d > pw1d || pw1d != pw1d);
+            lev0 = v.is_levee[ci] && (e0 < w0d || w0d != w0d);
+            lev1 = v.is_levee[di] && (e1 < w1d || w1d != w1d);
+        } else {
+            const REAL e0 = v.elev[ci] + a.pd[ci], e1 = v.elev[di] + a.pd[di];
+            p0_max = (pw0 > pw1 || pw1 != pw1);
+            lev0 = v.is_levee[ci] && (e0 < w0 || w0 != w0);
+            lev1 = v.is_levee[di] && (e1 < w1 || w1 != w1);
+        }
+        if (p0_max) Lp0 = Lxp;
         else Lp1 = Lxp;
-        const REAL e0 = v.elev[ci] + a.pd[ci], e1 = v.elev[di] + a.pd[di];
-        if (v.is_levee[ci] && (e0 < w0 || w0 != w0)) Ld0 = Lp0;
+        if (lev0) Ld0 = Lp0;
         else Lw0 += Lp0;
-        if (v.is_levee[di] && (e1 < w1 || w1 != w1)) Ld1 = Lp1;
+        if (lev1) Ld1 = Lp1;
         else Lw1 += Lp1;
         a.p_d0[p] = (REAL)Ld0;
         a.p_d1[p] = (REAL)Ld1;
@@ -1147,8 +1218,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
         Lfs += Lupd_fcsa / (double)v.river_length[i];
         Lfd -= Lupd_fcsa * (double)v.river_width[i];
     }
-    const REAL c_elv = v.elev[i];
-    if (o.fsurf - c_elv > (REAL)0.0) {
+    if (o.fsd > (REAL)0.0) {
         if (o.mouth) Lwse += Lupd_fcsd;
         else Lmx += Lupd_fcsd;
     }

@@ -1275,9 +1275,6 @@ int run_substep_taped(cmf_network* net, const cmf_fields* f, int substep, int st
     if (!tp) return set_error(CMF_ERR_INVALID, "substep_taped: null tape");
     if (v.T != 1 || net->G > 0)
         return set_error(CMF_ERR_UNSUPPORTED, "substep_taped: one trial, ungrouped");
-    if (f->flags & CMF_PRECISE_SURFACE)
-        return set_error(CMF_ERR_UNSUPPORTED, "substep_taped: the adjoint forms surfaces as the reference does; "
-                                              "clear CMF_PRECISE_SURFACE");
     void* ptrs[] = {tp->river_outflow, tp->flood_outflow, tp->river_cross_section_depth,
                     tp->flood_cross_section_depth, tp->flood_cross_section_area, tp->river_storage,
                     tp->flood_storage, tp->river_depth, tp->flood_depth, tp->qraw_r, tp->qraw_f, tp->lim};
@@ -1330,9 +1327,6 @@ int run_adjoint(cmf_network* net, const cmf_fields* f, const cmf_adjoint* adj, v
     int rc = make_view<REAL, STO>(net, f, v);
     if (rc) return rc;
     if (!adj) return set_error(CMF_ERR_INVALID, "substep_adjoint: null adjoint table");
-    if (f->flags & CMF_PRECISE_SURFACE)
-        return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: the adjoint forms surfaces as the reference does; "
-                                              "clear CMF_PRECISE_SURFACE");
     if (v.T != 1 || net->G > 0 || f->batched)
         return set_error(CMF_ERR_UNSUPPORTED, "substep_adjoint: one trial, ungrouped, no per-trial parameters");
     if ((rc = require({NEED(downstream_idx), NEED(river_manning), NEED(flood_manning), NEED(river_width),

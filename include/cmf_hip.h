@@ -61,7 +61,7 @@ enum {
      * (cmfgpu/phys/triton/outflow.py:111-134, bifurcation.py:49-69) lose the
      * centimetres their differences are made of; discharge then follows the
      * FP64 run about 50 times closer.  No effect in the FP64 profile; the
-     * adjoint and the taped forward return CMF_ERR_UNSUPPORTED with it. */
+     * adjoint follows the forward's values with it. */
     CMF_PRECISE_SURFACE = 1u << 9
 };
 
@@ -455,9 +455,13 @@ int cmf_network_set_outflow_accumulator(cmf_network* net, double* accumulator);
  * variant :565-684) so are the protected storage and depth, and the field
  * table must hold levee_index; with the reservoir module (the kinematic flow
  * upstream of dams, outflow.py:251-272, and the release,
- * reservoir.py:17-184) the reservoirs' gathered inflow is.  Trials, groups
- * and per-trial parameters return CMF_ERR_UNSUPPORTED.  No reference
- * counterpart (the reference has no differentiation). */
+ * reservoir.py:17-184) the reservoirs' gathered inflow is.  The protected
+ * storage of catchments without a levee is taken as zero (what every stage
+ * pass leaves there).  With CMF_PRECISE_SURFACE the reverse re-forms the
+ * surfaces as the forward did (in double), so its branches follow the
+ * forward's.  Trials, groups and per-trial parameters return
+ * CMF_ERR_UNSUPPORTED.  No reference counterpart (the reference has no
+ * differentiation). */
 typedef struct cmf_adjoint {
     const void* river_outflow;
     const void* flood_outflow;

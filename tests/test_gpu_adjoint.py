@@ -39,7 +39,7 @@ KINDS = {"plain": (0, 0.0, 0), "paths": (1, 0.0, 0), "levees": (0, 0.3, 0), "bot
          "all": (1, 0.3, 1)}
 
 
-def _setup(precision, n=3000, kind="plain"):
+def _setup(precision, n=3000, kind="plain", precise=False):
     from cmfgpu_amd import synthetic
     from cmfgpu_amd.adjoint import GaugeProblem
     from cmfgpu_amd.models import CaMaFlood
@@ -62,7 +62,8 @@ def _setup(precision, n=3000, kind="plain"):
     gauges = np.argsort(net.upstream_area)[-16:]
     mods = (PLAIN + (("bifurcation",) if paths else ()) + (("levee",) if levee else ())
             + (("reservoir",) if has_res else ()))
-    model = CaMaFlood(src, opened_modules=mods, device="cuda:0", step_duration=1800.0, precision=precision)
+    model = CaMaFlood(src, opened_modules=mods, device="cuda:0", step_duration=1800.0, precision=precision,
+                      precise_surfaces=precise)
     rng = np.random.default_rng(7)
     n0, h0 = src["river_manning"].astype(np.float64), src["river_height"].astype(np.float64)
     n_true = n0 * rng.lognormal(0.0, 0.2, n0.shape[0])
@@ -230,18 +231,43 @@ def test_fp32_adjoint_close_to_fp64(kind):
         assert _rel(a, b) <= 5e-2, _rel(a, b)
 
 
-@pytest.mark.parametrize("precision", ["float32", "float64"])
+@pytest.mark.parametrize("kind", ["plain", "both"])
+def test_precise_fp32_adjoint_close_to_fp64(kind):
+    """The FP32 adjoint with water surfaces formed in double (precise_surfaces)
+    against the FP64 one: as close as the FP32 adjoint with the reference's
+    FP32 surfaces (within three times its error), and within the FP32 bound
+    above.  Over these 8 outer steps both errors come from the FP32 arithmetic
+    of the reverse sweep itself (about 3e-6 on the plain network), not from
+    the forward trajectory that precise surfaces bring closer to FP64."""
+    grads = {}
+    for name, prec, precise in (("fp64", "float64", False), ("fp32", "float32", False),
+                                ("precise", "float32", True)):
+        net, src, model, problem, gauges = _setup(prec, kind=kind, precise=precise)
+        problem.run()
+        grads[name] = [x.cpu().numpy() for x in problem.gradient()]
+        model.close()
+    for k, label in ((0, "n"), (1, "H")):
+        e32 = _rel(grads["fp32"][k], grads["fp64"][k])
+        ep = _rel(grads["precise"][k], grads["fp64"][k])
+        print(f"{kind} d/d{label}: FP32 {e32:.2e}, FP32 precise surfaces {ep:.2e} against FP64", flush=True)
+        assert ep <= 3.0 * max(e32, 1e-6) and ep <= 5e-2, (e32, ep)
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64", "float32-precise"])
 def test_recomputed_intermediates_match_the_tape(precision):
     """One sub-step's adjoint from the taped forward intermediates (unlimited
     flows, limiters, path terms) and again with them recomputed from the saved
     state (passes A0, A1, B0): bit-identical adjoints and gradients, so the
     reverse re-forms the forward's values exactly (network with paths and
-    levees)."""
+    levees; "float32-precise": FP32 with water surfaces formed in double,
+    which the reverse re-forms the same way)."""
     import ctypes
     from cmfgpu_amd import _lib
     from cmfgpu_amd.adjoint import (CmfAdjoint, CmfTape, LEVEE_STATE, PATH_STATE, STATE, TAPE_REAL, TAPE_STO,
                                     _declare)
-    net, src, model, problem, gauges = _setup(precision, kind="both")
+    precise = precision.endswith("-precise")
+    net, src, model, problem, gauges = _setup(precision.split("-")[0], kind="both", precise=precise)
+    assert model.precise_surfaces == precise
     for _ in range(3):
         model.step_advance()
     _declare()
