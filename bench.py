@@ -586,26 +586,15 @@ def graph_line(args, device: str) -> dict:
     return out
 
 
-def adjoint_line(args, device: str) -> dict:
-    """The calibration step on the 6-arcmin network with all its modules
-    (base physics, 40,000 bifurcation paths, levees on 5 %): 2,000 gauges with
-    upstream area > 5,000 km^2, observations from a forward run with Manning's
-    n and river height scaled by lognormal(0, 0.2) factors, loss = MSE of log
-    discharge over `--adjoint-steps` outer steps (720: 15 days), checkpoints
-    every 48.  Forward time per sub-step against backward time (segment
-    re-runs + reverse sweep) per sub-step, after one untimed forward and
-    backward of one segment (first use of every kernel, and the tape of a
-    full segment allocated once by the caching allocator)."""
+def _adjoint_run(args, net, src, device: str, precise: bool) -> dict:
+    """One timed forward and backward of the calibration problem (see
+    adjoint_line), with the reference's FP32 surfaces or precise ones."""
     import numpy as np
     import torch
-    from cmfgpu_amd import synthetic
     from cmfgpu_amd.adjoint import GaugeProblem
     from cmfgpu_amd.models import CaMaFlood
-    net = synthetic.build_config(3, scale=args.scale)
-    src = dict(net.fields)
-    src["runoff"] = synthetic.Runoff(src["catchment_area"], 2003).next_day()
     model = CaMaFlood(src, opened_modules=("base", "adaptive_time", "bifurcation", "levee"), device=device,
-                      step_duration=1800.0)
+                      step_duration=1800.0, precise_surfaces=precise)
     gauges = net.gauges
     steps = args.adjoint_steps
     K = 48
@@ -635,16 +624,41 @@ def adjoint_line(args, device: str) -> dict:
     torch.cuda.synchronize()
     t_bwd = time.perf_counter() - t0
     sub = sum(problem.counts)
-    out = {"workload": f"{net.num_catchments} catchments, {net.num_paths} bifurcation paths, {net.num_levees} "
-                       f"levees, {gauges.shape[0]} gauges, observations from lognormal(0, 0.2)-perturbed n and H",
-           "outer_steps": steps, "checkpoint_every": K, "substeps": sub, "loss": loss,
+    out = {"outer_steps": steps, "checkpoint_every": K, "substeps": sub, "loss": loss,
            "forward_ms_per_substep": t_fwd / sub * 1e3, "backward_ms_per_substep": t_bwd / sub * 1e3,
            "ratio": round(t_bwd / t_fwd, 3),
            "gradient_finite": bool(torch.isfinite(g_n).all().item() and torch.isfinite(g_h).all().item()),
            "gradient_nonzero_catchments": int(((g_n != 0) | (g_h != 0)).sum().item())}
-    log(f"adjoint: {steps} outer steps ({sub} sub-steps), forward {out['forward_ms_per_substep']:.3f} ms/sub-step, "
-        f"backward {out['backward_ms_per_substep']:.3f} ms/sub-step ({out['ratio']}x)")
+    log(f"adjoint{' (precise surfaces)' if precise else ''}: {steps} outer steps ({sub} sub-steps), forward "
+        f"{out['forward_ms_per_substep']:.3f} ms/sub-step, backward {out['backward_ms_per_substep']:.3f} "
+        f"ms/sub-step ({out['ratio']}x)")
+    del problem
     model.close()
+    del model
+    torch.cuda.empty_cache()
+    return out
+
+
+def adjoint_line(args, device: str) -> dict:
+    """The calibration step on the 6-arcmin network with all its modules
+    (base physics, 40,000 bifurcation paths, levees on 5 %): 2,000 gauges with
+    upstream area > 5,000 km^2, observations from a forward run with Manning's
+    n and river height scaled by lognormal(0, 0.2) factors, loss = MSE of log
+    discharge over `--adjoint-steps` outer steps (720: 15 days), checkpoints
+    every 48.  Forward time per sub-step against backward time (segment
+    re-runs + reverse sweep) per sub-step, after one untimed forward and
+    backward of one segment (first use of every kernel, and the tape of a
+    full segment allocated once by the caching allocator); with the
+    reference's FP32 surfaces, then with water surfaces formed in double."""
+    from cmfgpu_amd import synthetic
+    net = synthetic.build_config(3, scale=args.scale)
+    src = dict(net.fields)
+    src["runoff"] = synthetic.Runoff(src["catchment_area"], 2003).next_day()
+    out = {"workload": f"{net.num_catchments} catchments, {net.num_paths} bifurcation paths, {net.num_levees} "
+                       f"levees, {net.gauges.shape[0]} gauges, observations from lognormal(0, 0.2)-perturbed n "
+                       f"and H"}
+    out.update(_adjoint_run(args, net, src, device, precise=False))
+    out["precise_surfaces"] = _adjoint_run(args, net, src, device, precise=True)
     return out
 
 
