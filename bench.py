@@ -446,9 +446,9 @@ def batched_full_line(args, device: str) -> dict:
     mods = ("base", "adaptive_time", "bifurcation", "levee", "reservoir")
 
     def build(precision, precise=False):
-        return CaMaFlood(src, opened_modules=mods, device=device, step_duration=1800.0, num_trials=M,
-                         batched={"runoff": runoff}, per_trial_time_step=True, trial_layout="inner",
-                         precision=precision, precise_surfaces=precise)
+        return scheduled(CaMaFlood(src, opened_modules=mods, device=device, step_duration=1800.0, num_trials=M,
+                                   batched={"runoff": runoff}, per_trial_time_step=True, trial_layout="inner",
+                                   precision=precision, precise_surfaces=precise))
 
     out = {"workload": f"{N} catchments, {P} paths, {net.num_levees} levees, {net.num_reservoirs} reservoirs, "
                        f"{M} members with their own CFL counts, trial-inner", "members": M}
@@ -538,7 +538,8 @@ def batched_full_line(args, device: str) -> dict:
     el = time.perf_counter() - t1
     subs = work / (N * M)                       # mean sub-steps per member over the timed steps
     f_wet = float((bm.t["flood_depth"] > 0).float().mean().item())
-    gbs = batched_full_bytes(N, M, P, f_wet) * subs / el / 1e9
+    prov = path_end_counts(bm) if SCHEDULE == "three_pass" else None
+    gbs = batched_full_bytes(N, M, P, f_wet, prov) * subs / el / 1e9
     bm.close()
     out.update({"value": work / el, "unit": "member-catchment-steps/s", "steps": steps,
                 "GBps_algorithmic": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),

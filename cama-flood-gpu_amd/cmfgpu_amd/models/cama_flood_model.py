@@ -428,8 +428,7 @@ class CaMaFlood:
         "three_pass" (the default): an outflow pass that also leaves each
         storage limiter its own outgoing term fixes, a fix-up pass over the
         path ends and the catchments with backflow from upstream, and the
-        stage pass (reservoirs take "limiter_pass");
-        "limiter_pass": outflow, limiter and stage passes
+        stage pass; "limiter_pass": outflow, limiter and stage passes
         over the whole network; "block_local" (one trial, breadth-first
         layout, no reservoirs): one pass in which each block of 512
         catchments completes the limiters and stages it holds whole, then

@@ -815,13 +815,14 @@ int launch_outflow(const cmf_network* net, const VT& v, int64_t NT, int substep,
 
 // The three passes of one fused sub-step, on view type VT (PlainView for one
 // plain trial, where the per-trial and per-group indexing folds away).
-// THREE_PASS without reservoirs: outflow with provisional limiters, the
-// limiter fix-up, stage (taped: the unlimited flows and limiters land in the
-// tape, the same values as from the limiter pass); otherwise outflow,
-// [reservoir], limiter, stage.
+// THREE_PASS: outflow with provisional limiters, [reservoir release], the
+// limiter fix-up (path ends, reservoir catchments, flagged catchments), stage
+// (taped: the unlimited flows and limiters land in the tape, the same values
+// as from the limiter pass); LIMITER_PASS: outflow, [reservoir], limiter,
+// stage.
 template <class REAL, class STO, class VT>
 int three_pass_substep(cmf_network* net, const VT& v_in, int substep, int store_inflow, void* stream) {
-    const bool prov = net->schedule == CMF_SCHEDULE_THREE_PASS && !v_in.has(CMF_HAS_RESERVOIR) && net->bmask;
+    const bool prov = net->schedule == CMF_SCHEDULE_THREE_PASS && net->bmask;
     VT v = v_in;
     if (prov) v.eterm = static_cast<REAL*>(net->eterm);
     // catchment kernels: N*T threads, or one block per group table entry
@@ -840,25 +841,27 @@ int three_pass_substep(cmf_network* net, const VT& v_in, int substep, int store_
     // limiters of every catchment, then the stage pass (which also forms the
     // path nets from the limiters of both path ends)
     if (prov) {
-        // path ends (per trial) in leading blocks, then the flag words: the
-        // whole network, or one wave per group record of the sub-step
+        // path ends (per trial) in leading blocks, reservoir catchments
+        // next, then the flag words: the whole network, or one wave per
+        // group record of the sub-step
         const int64_t list_n = bif ? net->pend_count : 0;
+        const int64_t nres = v.has(CMF_HAS_RESERVOIR) ? v.R * v.T : 0;
         const int4* list = net->pend_list;
         if (net->G > 0) {
             const View<REAL, STO> gv = v;
-            const int64_t lb = (list_n * v.T + 63) / 64, nrec = NT / kGroupRecord;
-            if (contig) LAUNCH_B((k_limiter_fix<REAL, STO, true, View<REAL, STO>>), (lb + nrec) * 64, 64, stream, gv,
-                                 substep, store_inflow, list, list_n, lb, (const int4*)gv.blk_rec);
-            else LAUNCH_B((k_limiter_fix<REAL, STO, false, View<REAL, STO>>), (lb + nrec) * 64, 64, stream, gv,
-                          substep, store_inflow, list, list_n, lb, (const int4*)gv.blk_rec);
+            const int64_t lb = (list_n * v.T + 63) / 64, rb = (nres + 63) / 64, nrec = NT / kGroupRecord;
+            if (contig) LAUNCH_B((k_limiter_fix<REAL, STO, true, View<REAL, STO>>), (lb + rb + nrec) * 64, 64, stream,
+                                 gv, substep, store_inflow, list, list_n, lb, rb, (const int4*)gv.blk_rec);
+            else LAUNCH_B((k_limiter_fix<REAL, STO, false, View<REAL, STO>>), (lb + rb + nrec) * 64, 64, stream,
+                          gv, substep, store_inflow, list, list_n, lb, rb, (const int4*)gv.blk_rec);
         } else {
-            const int64_t lb = (list_n * v.T + b_lim - 1) / b_lim;
+            const int64_t lb = (list_n * v.T + b_lim - 1) / b_lim, rb = (nres + b_lim - 1) / b_lim;
             const int64_t words = (v.N * v.T + 31) / 32;
-            const int64_t cnt = (lb + (words + b_lim - 1) / b_lim) * b_lim;
+            const int64_t cnt = (lb + rb + (words + b_lim - 1) / b_lim) * b_lim;
             if (contig) LAUNCH_B((k_limiter_fix<REAL, STO, true, VT>), cnt, b_lim, stream, v, substep, store_inflow, list,
-                                 list_n, lb, (const int4*)nullptr);
+                                 list_n, lb, rb, (const int4*)nullptr);
             else LAUNCH_B((k_limiter_fix<REAL, STO, false, VT>), cnt, b_lim, stream, v, substep, store_inflow, list,
-                          list_n, lb, (const int4*)nullptr);
+                          list_n, lb, rb, (const int4*)nullptr);
         }
     } else if constexpr (VT::kGrouped) {
         // one block of kGroupRecord / 2 threads per group table entry

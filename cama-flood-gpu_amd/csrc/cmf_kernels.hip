@@ -1434,6 +1434,9 @@ __device__ __forceinline__ void store_limiter(const VT& v, int64_t t, int64_t x,
 //     entry list_n holds the end of the last one's events in .w), so that
 //     the catchment's fields, its upstream flows and its path terms (eterm)
 //     are one round trip away;
+//   the next res_blocks: the reservoir catchments, once per trial, whose
+//     release k_reservoir wrote over the flows the outflow pass formed the
+//     provisional limiter from;
 //   the other blocks: 32-element words of the flag bits, one per lane (the
 //     whole network, or with `rec` the words of one catchment record per
 //     block of one wave: the group records of the sub-step), cleared after
@@ -1446,8 +1449,22 @@ __device__ __forceinline__ void store_limiter(const VT& v, int64_t t, int64_t x,
 template <class REAL, class STO, bool CONTIG, class VT>
 __global__ __launch_bounds__(kMaxBlock) void k_limiter_fix(VT v, int substep, int store_last,
                                                         const int4* __restrict__ list, int64_t list_n,
-                                                        int64_t list_blocks, const int4* __restrict__ rec) {
-    const int64_t b = blockIdx.x;
+                                                        int64_t list_blocks, int64_t res_blocks,
+                                                        const int4* __restrict__ rec) {
+    int64_t b = blockIdx.x;
+    if (b >= list_blocks && b < list_blocks + res_blocks) {
+        const int64_t g = (b - list_blocks) * blockDim.x + threadIdx.x;
+        if (g >= v.R * v.T) return;
+        const int64_t t = g / v.R, r = g - t * v.R;
+        const int64_t x = v.res_idx[r];
+        if (!v.active(t, x, substep)) return;
+        const bool ev = v.has(CMF_HAS_BIFURCATION) && v.eterm;
+        const int32_t eb = ev ? v.ev_ptr[x] : 0, ee = ev ? v.ev_ptr[x + 1] : 0;
+        const REAL l = limiter_exact<REAL, STO, CONTIG>(v, t, x, v.up_range(x), eb, ee);
+        store_limiter<REAL, STO>(v, t, x, l, substep, store_last);
+        return;
+    }
+    if (b >= list_blocks) b -= res_blocks;
     if (b < list_blocks) {
         const int64_t g = b * blockDim.x + threadIdx.x;
         if (g >= list_n * v.T) return;

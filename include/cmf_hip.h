@@ -318,8 +318,8 @@ int cmf_network_is_contiguous(const cmf_network* net);
  * also leaves each catchment's storage limiter as its own outgoing term
  * fixes it, a fix-up pass that forms in full the limiters of path ends and
  * of catchments an upstream catchment flows back into (flagged by the
- * outflow pass), and the stage pass; with reservoirs LIMITER_PASS runs
- * instead.  LIMITER_PASS:
+ * outflow pass) and of reservoir catchments (whose release replaces the
+ * outflow pass's flows), and the stage pass.  LIMITER_PASS:
  * outflow (+ paths), limiter and stage passes, each over the whole network.
  * BLOCK_LOCAL (one trial,
  * ungrouped, contiguous upstream ranges, no reservoirs, 10 flood levels, 5
