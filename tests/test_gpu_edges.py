@@ -4,7 +4,7 @@ that are all mouths (with five times as many paths as catchments, so pairs
 repeat), a single 3,000-catchment river (3,000 topological levels), a 4,000-way
 confluence (one mouth with 3,999 direct upstream catchments), and levees on
 every catchment.  Runoff x4 and the levee catchments started at 1-100 times
-their initial river storage so that floods reach the levees.  Both schedules,
+their initial river storage so that floods reach the levees.  Both modes (fused, per-op),
 four outer steps with the adaptive CFL count.
 """
 import numpy as np
