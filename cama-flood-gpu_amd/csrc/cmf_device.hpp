@@ -38,7 +38,22 @@ __device__ __forceinline__ double cbrt_any_d(double x, int steps) {
     if (x < 1e-290) return cbrt_pos_d(x * 0x1p300, steps) * 0x1p-100;
     return cbrt_pos_d(x, steps);
 }
-__device__ __forceinline__ float dcbrt(float x) { return (float)cbrt_any_d((double)x, 2); }
+// Float cube root without a division: Newton steps on the reciprocal cube
+// root, r <- r (4 - a r^3) / 3, from a bit-pattern guess, then a r^2.  Every
+// float is a normal double, and after four steps the result rounds to the
+// correctly rounded float cube root for every finite float.
+__device__ __forceinline__ float dcbrt(float x) {
+    if (x != x || x == 0.0f || __builtin_isinf(x)) return x;
+    const double a = x < 0.0f ? -(double)x : (double)x;
+    double r = __longlong_as_double(
+        (long long)(0x553EF0FF289DD796ull - (unsigned long long)__double_as_longlong(a) / 3ull));
+    for (int k = 0; k < 4; ++k) {
+        const double r3 = r * r * r;
+        r = r * ((4.0 - a * r3) * (1.0 / 3.0));
+    }
+    const double y = a * r * r;
+    return (float)(x < 0.0f ? -y : y);
+}
 __device__ __forceinline__ double dcbrt(double x) { return cbrt_any_d(x, 4); }
 
 // ------------------------------------------------------- log / exp -------
