@@ -7,10 +7,13 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstring>
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace cmf {
@@ -61,6 +64,7 @@ struct cmf_network {
     double* partials = nullptr;
     int64_t partial_blocks = 0;
     int32_t* host_counts = nullptr;   // pinned, max(T, G) entries
+    int32_t* host_counts_dev = nullptr;   // its device address (null: copied instead)
     // block size of each tunable kernel of the fused schedule (CMF_TUNE_*)
     int32_t blocks[CMF_TUNE_COUNT] = {256, 256, 256, 256};
     // groups (G > 0): group of each catchment and the block table of the
@@ -85,8 +89,9 @@ struct cmf_network {
     int64_t level_pos_cap = 0;
     hipEvent_t order_copied = nullptr;
     std::vector<int64_t> sub_blocks, level_off;
-    // plain adaptive advance: the event that marks the sub-step count's copy
-    // to host_counts (the first sub-step is enqueued before it is waited on)
+    // plain adaptive advance: the event that marks the sub-step count's arrival
+    // in host_counts, stored there by the CFL pass or copied (the first
+    // sub-step is enqueued before it is waited on)
     hipEvent_t count_ready = nullptr;
     // captured sub-step loops (cmf_hip_advance_graph): one executable graph
     // per sub-step count, valid for the field table they were captured with
@@ -505,20 +510,38 @@ inline int64_t block_of(const cmf_network* net, int id, bool catchment_kernel) {
     return (catchment_kernel && net->G > 0) ? kGroupRecord : net->blocks[id];
 }
 
-// The CFL sub-step count(s) into max_sub_steps (cleared before): the
-// contiguous kernel for one trial or trials in the inner layout whose count
-// divides 64, else k_cfl (chunks of the catchments per trial).
+// Whether the contiguous CFL kernel applies: one trial, or trials in the
+// inner layout whose count divides 64.
 template <class REAL, class STO>
-int launch_cfl(const cmf_network* net, const View<REAL, STO>& v, hipStream_t s) {
+bool cfl_inner_ok(const cmf_network* net, const View<REAL, STO>& v) {
     const bool contiguous = (v.T == 1 && v.cs == 1) || (v.ts == 1 && v.cs == v.T);
-    const bool inner = contiguous && net->cfl_part && v.T == net->T && v.T <= 64 && 64 % v.T == 0;
-    if (inner) {
+    return contiguous && net->cfl_part && v.T == net->T && v.T <= 64 && 64 % v.T == 0;
+}
+
+// The CFL sub-step count(s) into max_sub_steps (cleared before): the
+// contiguous kernel (cfl_inner_ok), else k_cfl (chunks of the catchments per
+// trial).  With `tail` (contiguous kernel only) the fold also writes the
+// counts whole, the time steps and the rest of CflTail: nothing is cleared
+// before and nothing follows.
+template <class REAL, class STO>
+int launch_cfl(const cmf_network* net, const View<REAL, STO>& v, hipStream_t s, const CflTail<REAL>* tail = nullptr) {
+    if (cfl_inner_ok(net, v)) {
         int log2T = 0;
         while ((int64_t(1) << log2T) < v.T) ++log2T;
         const int64_t nb = std::min<int64_t>(kCflInnerBlocks, (v.N * v.T + 8 * kBlock - 1) / (8 * kBlock));
-        hipLaunchKernelGGL((k_cfl_inner<REAL, STO>), dim3((unsigned)nb), dim3(kBlock), 0, s, v, log2T, net->cfl_part);
-        hipLaunchKernelGGL(k_cfl_fold, dim3((unsigned)v.T), dim3(kBlock), 0, s, (const int32_t*)net->cfl_part, nb, v.T,
-                           v.max_sub_steps, v.has(CMF_PER_TRIAL_DT) ? 1 : 0);
+        if (tail) {
+            hipLaunchKernelGGL((k_cfl_inner<REAL, STO, true>), dim3((unsigned)nb), dim3(kBlock), 0, s, v, log2T,
+                               net->cfl_part, tail->outer);
+            hipLaunchKernelGGL((k_cfl_fold_tail<REAL>), dim3(1), dim3(kBlock), 0, s, (const int32_t*)net->cfl_part,
+                               nb, log2T, *tail);
+        } else {
+            hipLaunchKernelGGL((k_cfl_inner<REAL, STO>), dim3((unsigned)nb), dim3(kBlock), 0, s, v, log2T,
+                               net->cfl_part, 0.0);
+            hipLaunchKernelGGL(k_cfl_fold, dim3((unsigned)v.T), dim3(kBlock), 0, s, (const int32_t*)net->cfl_part, nb,
+                               v.T, v.max_sub_steps, v.has(CMF_PER_TRIAL_DT) ? 1 : 0);
+        }
+    } else if (tail) {
+        return set_error(CMF_ERR_INTERNAL, "CFL pass: the one-launch form needs the contiguous kernel");
     } else {
         const int64_t chunks = std::min<int64_t>(blocks_for(v.N), 512);
         hipLaunchKernelGGL((k_cfl<REAL, STO>), dim3((unsigned)chunks, (unsigned)v.T), dim3(kBlock), 0, s, v);
@@ -1183,21 +1206,56 @@ bool early_ok(const cmf_network* net, const View<REAL, STO>& v) {
     return plain && !local && v.R == 0 && !lev_separate && !v.has(CMF_HAS_LOG) && !v.tape_q_r;
 }
 
+// Waits for the n sub-step counts that k_cfl_fold_tail stores in pinned host
+// memory (-1 until then): a busy read for about a millisecond, then reads
+// between stream queries, so that the wait ends with the stream even if a
+// store were never seen.  No event is recorded on the stream (its marker
+// costs the GPU several microseconds of idle time per outer step).
+int wait_counts(cmf_network* net, int64_t n, hipStream_t s) {
+    const volatile int32_t* hc = net->host_counts;
+    auto ready = [&]() {
+        for (int64_t t = 0; t < n; ++t)
+            if (hc[t] < 0) return false;
+        return true;
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int64_t k = 0;; ++k) {
+        if (ready()) break;
+        if ((k & 255) == 255 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(1)) {
+            const hipError_t q = hipStreamQuery(s);
+            if (q == hipSuccess) {
+                if (ready()) break;
+                return set_error(CMF_ERR_INTERNAL, "advance: the sub-step counts did not reach the host");
+            }
+            if (q != hipErrorNotReady) return set_error(CMF_ERR_HIP, "advance: %s", hipGetErrorString(q));
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return CMF_OK;
+}
+
 // The sub-step loop of a plain adaptive advance whose count is on its way to
-// the host (the copy is enqueued on `s`): the first sub-step, which every
-// outer step has, is enqueued at once (whether it is also the last is read
-// on the device, where its stores happen), then the host waits for the count,
-// which arrives while the GPU runs that sub-step, and enqueues the rest.  The
-// GPU does not idle on the count's round trip.  current_step was set to
-// count - 1 with the time step (k_set_dt).
+// the host (stored by the CFL pass or copied, enqueued on `s`): the first
+// sub-step, which every outer step has, is enqueued at once (whether it is
+// also the last is read on the device, where its stores happen), then the
+// host waits for the count, which arrives while the GPU runs that sub-step,
+// and enqueues the rest.  The GPU does not idle on the count's round trip.
+// current_step was set to count - 1 with the time step.
 template <class REAL, class STO>
 int early_advance(cmf_network* net, const View<REAL, STO>& v, const cmf_fields* f, int32_t* out_count,
-                  hipStream_t s) {
-    if (!net->count_ready) CMF_HIP_CHECK(hipEventCreateWithFlags(&net->count_ready, hipEventDisableTiming));
-    CMF_HIP_CHECK(hipEventRecord(net->count_ready, s));
+                  hipStream_t s, bool stored) {
+    if (!stored) {
+        if (!net->count_ready) CMF_HIP_CHECK(hipEventCreateWithFlags(&net->count_ready, hipEventDisableTiming));
+        CMF_HIP_CHECK(hipEventRecord(net->count_ready, s));
+    }
     int rc = fused_substep(net, v, f, 0, 0, s, true);
     if (rc) return rc;
-    CMF_HIP_CHECK(hipEventSynchronize(net->count_ready));
+    if (stored) {
+        if ((rc = wait_counts(net, 1, s))) return rc;
+    } else {
+        CMF_HIP_CHECK(hipEventSynchronize(net->count_ready));
+    }
     const int count = std::max(1, (int)net->host_counts[0]);
     for (int k = 1; k < count; ++k)
         if ((rc = fused_substep(net, v, f, k, (k == count - 1) ? 1 : 0, s))) return rc;
@@ -1222,24 +1280,41 @@ int run_advance(cmf_network* net, const cmf_fields* f, double outer, int fixed_c
         return set_error(CMF_ERR_INTERNAL, "advance: partial buffer too small");
     if (v.has(CMF_HAS_ADAPTIVE_TIME)) {
         if ((rc = require({NEED(max_sub_steps), NEED(outer_time_step)}, "advance"))) return rc;
-        hipLaunchKernelGGL((k_prepare_counts<REAL>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
-                           static_cast<REAL*>(f->outer_time_step), outer, f->max_sub_steps, nslots);
-        if (net->G > 0) {
-            hipLaunchKernelGGL((k_cfl_grouped<REAL, STO>), dim3((unsigned)((net->nblk + kCflRecords - 1) / kCflRecords)),
-                               dim3(kGroupRecord), 0, s, v, (int64_t)net->nblk);
-        } else if (v.N > 0) {
-            if ((rc = launch_cfl(net, v, s))) return rc;
-        }
         // the early advance leaves current_step at count - 1 here (one slot)
         const bool early = !use_graph && early_ok(net, v);
-        hipLaunchKernelGGL((k_set_dt<REAL>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
-                           static_cast<REAL*>(f->time_step), f->max_sub_steps, nslots, outer,
-                           early ? f->current_step : nullptr);
-        CMF_HIP_CHECK(hipGetLastError());
-        CMF_HIP_CHECK(hipMemcpyAsync(net->host_counts, f->max_sub_steps, sizeof(int32_t) * nslots,
-                                     hipMemcpyDeviceToHost, s));
-        if (early) return early_advance(net, v, f, out_count, s);
-        CMF_HIP_CHECK(hipStreamSynchronize(s));
+        const bool stored = net->G == 0 && v.N > 0 && net->host_counts_dev && cfl_inner_ok(net, v);
+        if (stored) {
+            // two launches: the CFL maxima, then one block that folds them,
+            // sets the time steps and stores the counts in pinned host memory,
+            // where the host waits for them to replace its -1s
+            for (int64_t t = 0; t < nslots; ++t) net->host_counts[t] = -1;
+            const CflTail<REAL> tail{f->max_sub_steps, static_cast<REAL*>(f->time_step),
+                                     static_cast<REAL*>(f->outer_time_step), early ? f->current_step : nullptr,
+                                     net->host_counts_dev, outer, v.has(CMF_PER_TRIAL_DT) ? 1 : 0};
+            if ((rc = launch_cfl(net, v, s, &tail))) return rc;
+        } else {
+            hipLaunchKernelGGL((k_prepare_counts<REAL>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
+                               static_cast<REAL*>(f->outer_time_step), outer, f->max_sub_steps, nslots);
+            if (net->G > 0) {
+                hipLaunchKernelGGL((k_cfl_grouped<REAL, STO>),
+                                   dim3((unsigned)((net->nblk + kCflRecords - 1) / kCflRecords)), dim3(kGroupRecord),
+                                   0, s, v, (int64_t)net->nblk);
+            } else if (v.N > 0) {
+                if ((rc = launch_cfl(net, v, s))) return rc;
+            }
+            hipLaunchKernelGGL((k_set_dt<REAL>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
+                               static_cast<REAL*>(f->time_step), f->max_sub_steps, nslots, outer,
+                               early ? f->current_step : nullptr);
+            CMF_HIP_CHECK(hipGetLastError());
+            CMF_HIP_CHECK(hipMemcpyAsync(net->host_counts, f->max_sub_steps, sizeof(int32_t) * nslots,
+                                         hipMemcpyDeviceToHost, s));
+        }
+        if (early) return early_advance(net, v, f, out_count, s, stored);
+        if (stored) {
+            if ((rc = wait_counts(net, nslots, s))) return rc;
+        } else {
+            CMF_HIP_CHECK(hipStreamSynchronize(s));
+        }
         count = 1;
         for (int64_t t = 0; t < nslots; ++t) count = std::max(count, (int)net->host_counts[t]);
     } else {
@@ -1720,6 +1795,12 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
     if (e != hipSuccess) {
         free_network(net);
         return set_error(CMF_ERR_NOMEM, "hipHostMalloc failed: %s", hipGetErrorString(e));
+    }
+    // kernels may store the sub-step counts there themselves; without a device
+    // address they are copied
+    if (hipHostGetDevicePointer((void**)&net->host_counts_dev, net->host_counts, 0) != hipSuccess) {
+        net->host_counts_dev = nullptr;
+        (void)hipGetLastError();
     }
     if (num_groups > 0) {
         net->gblk_host = gblk_ptr;

@@ -1918,13 +1918,17 @@ __global__ __launch_bounds__(kBlock) void k_cfl(View<REAL, STO> v) {
 // block leaves one maximum per trial in `part` [gridDim.x, T], and
 // k_cfl_fold takes the maximum over the blocks (no atomics on a few hot
 // addresses from every block).
+//
+// OUTER_ARG (the adaptive advance): the outer time step comes as an argument
+// instead of from outer_time_step, which k_cfl_fold_tail sets.
 constexpr int kCflInnerBlocks = 1024;
-template <class REAL, class STO>
-__global__ __launch_bounds__(kBlock) void k_cfl_inner(View<REAL, STO> v, int log2T, int32_t* part) {
+template <class REAL, class STO, bool OUTER_ARG = false>
+__global__ __launch_bounds__(kBlock) void k_cfl_inner(View<REAL, STO> v, int log2T, int32_t* part,
+                                                      double outer_arg) {
     __shared__ int sh[kBlock / 64][64];
     const int64_t NT = v.N * v.T;
     const int64_t stride = (int64_t)gridDim.x * kBlock;
-    const REAL outer = v.outer_time_step[0];
+    const REAL outer = OUTER_ARG ? (REAL)outer_arg : v.outer_time_step[0];
     int best = 0;
     for (int64_t e0 = (int64_t)blockIdx.x * kBlock + threadIdx.x; e0 < NT; e0 += 8 * stride) {
         REAL depth[8], dist[8];
@@ -1961,6 +1965,57 @@ __global__ __launch_bounds__(kBlock) void k_cfl_fold(const int32_t* part, int64_
     for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) m = max(m, part[b * T + t]);
     m = block_max_int(m);
     if (threadIdx.x == 0) atomicMax(&counts[per_trial ? t : 0], m);
+}
+
+// What the adaptive advance needs after k_cfl_inner<..., OUTER_ARG = true>,
+// in one block: the counts folded from the blocks' maxima (written whole: no
+// clearing before), the time steps outer / max(count, 1) in double, rounded
+// once (cama_flood_model.py:127,132), the outer time step, current_step =
+// count - 1 (one slot, when given) and the counts again at `host` (the device
+// address of pinned host memory, which the host reads without a copy).
+template <class REAL>
+struct CflTail {
+    int32_t* counts;        // max_sub_steps: nslots entries
+    REAL* time_step;        // nslots entries
+    REAL* outer_time_step;  // one entry
+    int32_t* current_step;  // or null
+    int32_t* host;          // nslots entries
+    double outer;
+    int per_trial;          // nslots = T, else 1 (every trial folds into slot 0)
+};
+template <class REAL>
+__global__ __launch_bounds__(kBlock) void k_cfl_fold_tail(const int32_t* part, int64_t nblocks, int log2T,
+                                                          CflTail<REAL> tail) {
+    __shared__ int sh[kBlock / 64][64];
+    const int T = 1 << log2T;
+    // thread k folds trial k % T over the blocks k / T, k / T + kBlock / T, ...
+    const int tt = (int)threadIdx.x & (T - 1);
+    int m = 0;
+    for (int64_t b = threadIdx.x >> log2T; b < nblocks; b += kBlock >> log2T) m = max(m, part[b * T + tt]);
+    // then over the lanes of one trial (distances that are multiples of T)
+    // and over the waves
+    for (int off = 32; off >= T; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) < (unsigned)T) sh[threadIdx.x >> 6][threadIdx.x & 63] = m;
+    __syncthreads();
+    if ((int)threadIdx.x < T) {
+        int c = sh[0][threadIdx.x];
+        for (int w = 1; w < kBlock / 64; ++w) c = max(c, sh[w][threadIdx.x]);
+        sh[0][threadIdx.x] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) tail.outer_time_step[0] = (REAL)tail.outer;
+    const int nslots = tail.per_trial ? T : 1;
+    if ((int)threadIdx.x < nslots) {
+        int c = sh[0][threadIdx.x];
+        if (!tail.per_trial)
+            for (int k = 1; k < T; ++k) c = max(c, sh[0][k]);
+        const int cc = c < 1 ? 1 : c;
+        tail.counts[threadIdx.x] = c;
+        tail.time_step[threadIdx.x] = (REAL)(tail.outer / (double)cc);
+        if (tail.current_step && nslots == 1) tail.current_step[0] = cc - 1;
+        __hip_atomic_store(tail.host + threadIdx.x, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    __threadfence_system();
 }
 
 // grouped networks: kCflRecords consecutive table entries (each a single

