@@ -224,13 +224,20 @@ class CaMaFlood:
         else:
             ci = di = np.zeros(0, np.int32)
         if self.V:
-            lidx_dev = self.inv[hf["levee_catchment_idx"]].astype(np.int32)
+            # levee rows in the device order of their catchments (row r holds
+            # the input's levee lperm[r]): a wave of levee catchments then
+            # reads consecutive rows of the levee parameters
+            lidx_in = self.inv[hf["levee_catchment_idx"]].astype(np.int64)
+            self.lperm = np.argsort(lidx_in, kind="stable")
+            self.linv = np.empty(self.V, np.int64)
+            self.linv[self.lperm] = np.arange(self.V)
+            lidx_dev = lidx_in[self.lperm].astype(np.int32)
             self.t["levee_catchment_idx"] = torch.from_numpy(lidx_dev).to(self.device)
             levee_index = np.full(self.N, -1, np.int32)
             levee_index[lidx_dev] = np.arange(self.V, dtype=np.int32)
             self.t["levee_index"] = torch.from_numpy(levee_index).to(self.device)
             for name in LEVEE_PARAMS:
-                src_arr = np.asarray(batched.get(name, hf[name]), dtype=hf[name].dtype)
+                src_arr = np.asarray(batched.get(name, hf[name]), dtype=hf[name].dtype)[..., self.lperm]
                 self.t[name] = torch.from_numpy(np.ascontiguousarray(src_arr)).to(self.device)
             if batched.keys() & {"levee_crown_height", "levee_base_height", "levee_fraction"}:
                 # per-trial levee parameters obey the module's rules in every trial
@@ -372,7 +379,11 @@ class CaMaFlood:
         if name in INDEX_FIELDS:
             if name == "downstream_idx":
                 return self.host.fields["downstream_idx"].copy()
+            if name == "levee_catchment_idx":
+                x = x[self.linv]
             return self.perm[x].astype(np.int32)
+        if name in LEVEE_PARAMS:
+            return np.ascontiguousarray(x[..., self.linv])
         if name in CATCHMENT_STATE or name in CATCHMENT_SHARED:
             per_trial = name in CATCHMENT_STATE or name in self.batched_names
             if per_trial and self.T > 1:

@@ -249,7 +249,7 @@ def test_batched_levee_fraction_derives_per_trial_base(what):
     model = _model(src, FULL, num_trials=M, batched={what: vals}, per_trial_time_step=True)
     oracles = [oracle_for(dict(src, **{what: vals[m]}), FULL)[0] for m in range(M)]
     # the members' base heights differ, and each is its standalone run's
-    bases = model.t["levee_base_height"].cpu().numpy().reshape(M, -1)
+    bases = model.get("levee_base_height").reshape(M, -1)
     for m in range(M):
         assert np.array_equal(bases[m], oracles[m].state["levee_base_height"])
     assert not np.array_equal(bases[0], bases[1])
@@ -265,6 +265,30 @@ def test_batched_levee_fraction_derives_per_trial_base(what):
         for m in range(M):
             msg = mismatch(got[m], oracles[m].state[name])
             assert msg is None, f"member {m} {name}: {msg}"
+
+
+@pytest.mark.parametrize("layout", ["keyed", "basin", "network"])
+def test_levee_rows_in_device_order(layout):
+    """Levee rows sit on the device in the order of their catchments' device
+    positions; get() gives the parameters and indices back in the input's
+    order, and a run matches the oracle (which keeps the input's rows)."""
+    net, src = small_source()
+    lid = np.asarray(src["levee_catchment_idx"])
+    # rows in an order unrelated to the catchments' (reversed input)
+    rev = np.arange(lid.size)[::-1].copy()
+    for k in ("levee_catchment_idx", "levee_crown_height", "levee_fraction", "levee_base_height"):
+        src[k] = np.ascontiguousarray(np.asarray(src[k])[rev])
+    model = _model(src, FULL, layout=layout)
+    dev = model.t["levee_catchment_idx"].cpu().numpy()
+    assert np.all(np.diff(dev) > 0)
+    for k in ("levee_catchment_idx", "levee_crown_height", "levee_fraction", "levee_base_height"):
+        assert np.array_equal(model.get(k), np.asarray(src[k])), k
+    orc, _ = oracle_for(src, FULL)
+    for _ in range(3):
+        assert model.step_advance() == int(orc.advance(1800.0, 1)[0])
+    for name in STATE_FIELDS:
+        msg = mismatch(model.get(name), orc.state[name])
+        assert msg is None, f"{name}: {msg}"
 
 
 @pytest.mark.parametrize("mode", ["fused", "ops"])
