@@ -89,6 +89,48 @@ def _group_basins(order, groups: np.ndarray, dn: np.ndarray):
     return perm, group_ptr, labels
 
 
+
+def _path_order(ci: np.ndarray, di: np.ndarray) -> np.ndarray:
+    """Device order of the bifurcation paths (entry q: the input path placed
+    q-th), from their ends' device positions ci, di (input order).
+
+    Paths are taken by ascending device position of their upstream end, so
+    that the path pass and the path ends of a wave touch consecutive paths,
+    subject to one rule: the paths of every catchment keep their input order
+    among themselves, since each catchment sums its paths' terms in ascending
+    path order (the reference's order).  That is a linear extension of the
+    per-catchment chains (edges run from lower to higher input index, so the
+    graph has no cycle), formed with a heap keyed by the upstream end."""
+    import heapq
+    P = int(ci.shape[0])
+    if P == 0:
+        return np.zeros(0, np.int64)
+    c = np.concatenate([ci, di]).astype(np.int64)
+    p = np.concatenate([np.arange(P), np.arange(P)])
+    o = np.lexsort((p, c))
+    c, p = c[o], p[o]
+    same = c[1:] == c[:-1]
+    src, dst = p[:-1][same], p[1:][same]
+    indeg = np.bincount(dst, minlength=P)
+    so = np.argsort(src, kind="stable")
+    src, dst = src[so], dst[so]
+    start = np.searchsorted(src, np.arange(P + 1))
+    key = ci.astype(np.int64)
+    heap = [(int(key[q]), q) for q in np.flatnonzero(indeg == 0).tolist()]
+    heapq.heapify(heap)
+    order = []
+    while heap:
+        _, q = heapq.heappop(heap)
+        order.append(q)
+        for r in dst[start[q]:start[q + 1]].tolist():
+            indeg[r] -= 1
+            if indeg[r] == 0:
+                heapq.heappush(heap, (int(key[r]), r))
+    if len(order) != P:
+        raise RuntimeError("bifurcation path order: the per-catchment chains do not cover every path")
+    return np.asarray(order, np.int64)
+
+
 class CaMaFlood:
     def __init__(self, src: dict, *, opened_modules=("base", "adaptive_time"), precision="float32",
                  device="cuda:0", num_trials: int = 1, batched: dict | None = None,
@@ -214,6 +256,11 @@ class CaMaFlood:
             # (with the module open and no paths the arrays are empty, still bound)
             ci = self.inv[hf["bifurcation_catchment_idx"]].astype(np.int32)
             di = self.inv[hf["bifurcation_downstream_idx"]].astype(np.int32)
+            self.pperm = _path_order(ci, di)
+            self.pinv = np.empty(self.P, np.int64)
+            self.pinv[self.pperm] = np.arange(self.P)
+            ci = np.ascontiguousarray(ci[self.pperm])
+            di = np.ascontiguousarray(di[self.pperm])
             self.t["bifurcation_catchment_idx"] = torch.from_numpy(ci).to(self.device)
             self.t["bifurcation_downstream_idx"] = torch.from_numpy(di).to(self.device)
             for name in PATH_SHARED:
@@ -323,11 +370,13 @@ class CaMaFlood:
         return x
 
     def _to_path(self, arr, per_trial=False, replicate=False):
+        """Host array in the input's path order ([P, ...], or [T, P, ...] per
+        trial) -> device tensor with the paths in the device order (pperm)."""
         a = np.asarray(arr)
         if per_trial and self.T > 1 and not replicate:
-            x = torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+            x = torch.from_numpy(np.ascontiguousarray(a[:, self.pperm])).to(self.device)
             return x.transpose(0, 1).contiguous() if self.trial_layout == "inner" else x
-        x = torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+        x = torch.from_numpy(np.ascontiguousarray(a[self.pperm])).to(self.device)
         if per_trial and self.T > 1:
             if self.trial_layout == "inner":
                 return x.unsqueeze(1).expand(x.shape[0], self.T, *x.shape[1:]).contiguous()
@@ -381,6 +430,8 @@ class CaMaFlood:
                 return self.host.fields["downstream_idx"].copy()
             if name == "levee_catchment_idx":
                 x = x[self.linv]
+            elif name in ("bifurcation_catchment_idx", "bifurcation_downstream_idx"):
+                x = x[self.pinv]
             return self.perm[x].astype(np.int32)
         if name in LEVEE_PARAMS:
             return np.ascontiguousarray(x[..., self.linv])
@@ -393,9 +444,11 @@ class CaMaFlood:
             return x[self.inv]
         if name in PATH_STATE or name in PATH_SHARED:
             per_trial = name in PATH_STATE or name in self.batched_names
-            if per_trial and self.T > 1 and self.trial_layout == "inner":
-                x = np.swapaxes(x, 0, 1)
-            return np.ascontiguousarray(x)
+            if per_trial and self.T > 1:
+                if self.trial_layout == "inner":
+                    x = np.swapaxes(x, 0, 1)
+                return np.ascontiguousarray(x[:, self.pinv])
+            return np.ascontiguousarray(x[self.pinv])
         return x
 
     def log_sums(self) -> np.ndarray:

@@ -291,6 +291,70 @@ def test_levee_rows_in_device_order(layout):
         assert msg is None, f"{name}: {msg}"
 
 
+def _hub_paths(src, seed=3):
+    """src with extra bifurcation paths on three hub catchments (several
+    paths each, in both roles) and every path row in a shuffled order."""
+    from cmfgpu_amd import synthetic
+    rng = np.random.default_rng(seed)
+    n = np.asarray(src["downstream_idx"]).size
+    hubs = rng.choice(n, 3, replace=False)
+    others = rng.choice(np.setdiff1d(np.arange(n), hubs), 12, replace=False)
+    pairs = ([(hubs[0], o) for o in others[:4]] + [(o, hubs[1]) for o in others[4:8]]
+             + [(hubs[2], o) for o in others[8:10]] + [(o, hubs[2]) for o in others[10:12]]
+             + [(hubs[0], hubs[1]), (hubs[1], hubs[2]), (hubs[2], hubs[0])])
+    ci = np.array([c for c, _ in pairs], np.int32)
+    di = np.array([d for _, d in pairs], np.int32)
+    L = synthetic.NUM_BIFURCATION_LEVELS
+    bed = src["catchment_elevation"][ci].astype(np.float64) - src["river_height"][ci].astype(np.float64)
+    extra = {
+        "bifurcation_catchment_idx": ci, "bifurcation_downstream_idx": di,
+        "bifurcation_length": (0.5 * (src["river_length"][ci].astype(np.float64)
+                                      + src["river_length"][di].astype(np.float64))).astype(np.float32),
+        "bifurcation_width": np.full((ci.size, L), 120.0, np.float32),
+        "bifurcation_elevation": (bed[:, None] + synthetic.BIFURCATION_LEVEL_HEIGHTS[None, :]).astype(np.float32),
+        "bifurcation_manning": np.broadcast_to(synthetic.BIFURCATION_MANNING, (ci.size, L)).copy(),
+    }
+    out = dict(src)
+    for k, v in extra.items():
+        out[k] = np.concatenate([np.asarray(src[k]), v])
+    P = out["bifurcation_catchment_idx"].size
+    shuf = rng.permutation(P)
+    for k in list(extra) :
+        out[k] = np.ascontiguousarray(out[k][shuf])
+    cid = np.asarray(src["catchment_id"])
+    out["bifurcation_catchment_id"] = cid[out["bifurcation_catchment_idx"]]
+    out["bifurcation_downstream_id"] = cid[out["bifurcation_downstream_idx"]]
+    out["bifurcation_path_id"] = np.arange(P, dtype=np.int64)
+    return out, hubs
+
+
+@pytest.mark.parametrize("layout", ["keyed", "basin"])
+def test_path_device_order_bitexact(layout):
+    """Paths sit on the device by their upstream ends' device positions while
+    each catchment keeps its paths' input order (its path terms are summed in
+    that order): hub catchments with several paths in both roles, path rows
+    shuffled; get() gives path fields and indices back in the input order,
+    and the run is bit-exact with the oracle."""
+    net, src = small_source()
+    src, hubs = _hub_paths(src)
+    model = _model(src, FULL, layout=layout)
+    assert not np.array_equal(model.pperm, np.arange(model.P))
+    for k in ("bifurcation_catchment_idx", "bifurcation_downstream_idx"):
+        assert np.array_equal(model.get(k), src[k]), k
+    for k in ("bifurcation_width", "bifurcation_elevation", "bifurcation_manning"):
+        assert np.array_equal(model.get(k), src[k]), k
+    orc, _ = oracle_for(src, FULL)
+    for _ in range(4):
+        assert model.step_advance() == int(orc.advance(1800.0, 1)[0])
+    for name in STATE_FIELDS + ("bifurcation_outflow", "bifurcation_cross_section_depth"):
+        msg = mismatch(model.get(name), orc.state[name])
+        assert msg is None, f"{name}: {msg}"
+    # the hubs' path terms are non-zero sums of several terms
+    gb = np.asarray(orc.state["global_bifurcation_outflow"])[hubs]
+    assert np.count_nonzero(gb) >= 2, gb
+    assert np.array_equal(model.get("global_bifurcation_outflow"), orc.state["global_bifurcation_outflow"])
+
+
 @pytest.mark.parametrize("mode", ["fused", "ops"])
 def test_fp32_storage_profile_bitexact(mode):
     """The float32 profile without mixed precision (storages and accumulators

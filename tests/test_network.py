@@ -219,3 +219,29 @@ def test_keyed_level_order_properties_at_config1_size():
     plain = levee_waves(network.level_order(d).perm)
     keyed = levee_waves(ko.perm)
     assert plain > 0.9 and keyed < 0.3, (plain, keyed)
+
+
+def test_path_device_order():
+    """The device order of bifurcation paths: a permutation in which every
+    catchment's paths keep their input order (the order its path terms are
+    summed in), otherwise ascending by the upstream end's device position;
+    without shared ends it is the sort by upstream end."""
+    from cmfgpu_amd.models.cama_flood_model import _path_order
+    rng = np.random.default_rng(11)
+    for trial in range(20):
+        n, P = int(rng.integers(5, 60)), int(rng.integers(1, 120))
+        ci = rng.integers(0, n, P)
+        di = (ci + rng.integers(1, n, P)) % n
+        o = _path_order(ci, di)
+        assert np.array_equal(np.sort(o), np.arange(P))
+        pos = np.empty(P, np.int64)
+        pos[o] = np.arange(P)
+        for c in range(n):
+            inc = np.flatnonzero((ci == c) | (di == c))
+            assert np.all(np.diff(pos[inc]) > 0), (trial, c)
+    # distinct ends everywhere: plain sort by upstream end
+    ci = rng.permutation(200)[:50]
+    di = 200 + rng.permutation(50)
+    o = _path_order(ci, di)
+    assert np.array_equal(o, np.argsort(ci, kind="stable"))
+    assert _path_order(np.zeros(0, np.int32), np.zeros(0, np.int32)).size == 0
