@@ -52,6 +52,7 @@ struct cmf_network {
     int32_t* ev_code = nullptr;
     int32_t* ev_other = nullptr;
     int2* ev1 = nullptr;
+    uint8_t* ev_flag = nullptr;   // 1 where the catchment has a bifurcation event
     void* pos_term = nullptr;
     void* neg_term = nullptr;
     void* qraw_r = nullptr;
@@ -140,7 +141,7 @@ void free_network(cmf_network* net) {
                     net->neg_term, net->qraw_r, net->qraw_f, net->lim, net->path_pos,
                     net->path_neg, net->path_net, net->path_raw, net->partials, net->cgroup,
                     net->blk_rec_id, net->level_rec, net->level_pos, net->gblk_ptr, net->lflags, net->lim_list,
-                    net->stage_list, net->bmask, net->pend_list, net->pev, net->eterm};
+                    net->stage_list, net->bmask, net->pend_list, net->pev, net->eterm, net->ev_flag};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (net->host_counts) (void)hipHostFree(net->host_counts);
@@ -356,6 +357,7 @@ int make_view(const cmf_network* net, const cmf_fields* f, View<REAL, STO>& v) {
     v.ev_code = net->ev_code;
     v.ev_other = net->ev_other;
     v.ev1 = net->ev1;
+    v.ev_flag = net->ev_flag;
     v.pos_term = static_cast<REAL*>(net->pos_term);
     v.neg_term = static_cast<REAL*>(net->neg_term);
     v.qraw_r = static_cast<REAL*>(net->qraw_r);
@@ -1712,6 +1714,8 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
             ev1[i].y = (e - b > 1) ? -(ev_other[b] + 1) : ev_other[b];
         }
     }
+    std::vector<uint8_t> ev_flag((size_t)std::max<int64_t>(n, 1), 0);
+    for (int64_t i = 0; i < n; ++i) ev_flag[(size_t)i] = ev_ptr[i + 1] != ev_ptr[i] ? 1 : 0;
     // path-end records of the limiter fix-up pass, ascending catchment
     std::vector<int4> pend;
     for (int64_t i = 0; i < n; ++i)
@@ -1749,6 +1753,7 @@ int cmf_network_create_grouped(int64_t n, const int32_t* downstream_idx, int64_t
         (rc = upload((void**)&net->ev_code, ev_code.data(), ev_code.size() * 4)) ||
         (rc = upload((void**)&net->ev_other, ev_other.data(), ev_other.size() * 4)) ||
         (rc = upload((void**)&net->ev1, ev1.data(), (size_t)n * sizeof(int2))) ||
+        (rc = upload((void**)&net->ev_flag, ev_flag.data(), ev_flag.size())) ||
         (rc = dev_alloc(&net->pos_term, el)) || (rc = dev_alloc(&net->neg_term, el)) ||
         (rc = dev_alloc(&net->qraw_r, el)) || (rc = dev_alloc(&net->qraw_f, el)) ||
         (rc = dev_alloc(&net->lim, el)) || (rc = dev_alloc(&net->path_pos, pel)) ||

@@ -168,6 +168,8 @@ struct View {
     // pass: {code, other end}; code -1: no event; other end -(other + 1):
     // more events follow (the ev_ptr list from its second entry)
     const int2* ev1;
+    // 1 byte per catchment: 1 where it has an event (ev_ptr[j] != ev_ptr[j + 1])
+    const uint8_t* ev_flag;
     // scratch
     REAL *pos_term, *neg_term, *qraw_r, *qraw_f, *lim;
     REAL *path_pos, *path_neg, *path_net, *path_raw;
@@ -297,6 +299,11 @@ struct View {
     }
     // GroupView: the group of a launch's catchments is that of its block record
     static constexpr bool kGrouped = false;
+    // whether a catchment's levee and event tests read the byte flags
+    // (is_levee, ev_flag) before any index: per-lane catchments, where a
+    // byte is a quarter of the index; a view whose catchment index is the
+    // same in every lane reads the index itself (one scalar load)
+    static constexpr bool kByteFlags = true;
 };
 
 // The view of a one-trial, ungrouped run without per-trial parameters or
@@ -348,6 +355,7 @@ struct PlainView : View<REAL, STO> {
 // parameter and upstream range becomes a scalar.
 template <class REAL, class STO, bool WAVE = true>
 struct MemberView : View<REAL, STO> {
+    static constexpr bool kByteFlags = !WAVE;
     int log2T;
     __device__ __forceinline__ int64_t el(int64_t t, int64_t i) const { return (i << log2T) + t; }
     __device__ __forceinline__ int64_t pel(int64_t t, int64_t p) const { return (p << log2T) + t; }

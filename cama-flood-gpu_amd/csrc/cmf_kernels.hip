@@ -1572,14 +1572,24 @@ __device__ __forceinline__ void stage_fused_one(const VT& v, int64_t t, int64_t 
     const REAL rh = v.prm(v.river_height, CMF_BATCHED_RIVER_HEIGHT, e, j);
     const REAL rw = v.prm(v.river_width, CMF_BATCHED_RIVER_WIDTH, e, j);
     const REAL rl = v.prm(v.river_length, CMF_BATCHED_RIVER_LENGTH, e, j);
-    const int32_t lv = LEV ? v.levee_index[j] : -1;
-    const bool lev_cell = LEV ? (lv >= 0) : (v.has(CMF_HAS_LEVEE) && v.is_levee[j]);
+    // with byte flags (VT::kByteFlags) a byte each says whether the catchment
+    // has a levee and bifurcation events, and the levee row index is read
+    // only where there is a levee; otherwise the index and the event range
+    // are read up front
+    constexpr bool BYTES = VT::kByteFlags;
+    const int32_t lv0 = (LEV && !BYTES) ? v.levee_index[j] : -1;
+    const bool lev_flag = v.has(CMF_HAS_LEVEE) &&
+                          ((LEV && !BYTES) ? lv0 >= 0
+                                           : (v.is_levee ? v.is_levee[j] != 0 : (LEV && v.levee_index[j] >= 0)));
     const bool res = v.has(CMF_HAS_RESERVOIR) && v.is_reservoir[j];
-    const bool events = v.has(CMF_HAS_BIFURCATION) && v.ev_ptr[j] != v.ev_ptr[j + 1];
+    const bool events = v.has(CMF_HAS_BIFURCATION) &&
+                        ((BYTES && v.ev_flag) ? v.ev_flag[j] != 0 : v.ev_ptr[j] != v.ev_ptr[j + 1]);
     // round trip 2
+    const int32_t lv = !LEV ? -1 : (BYTES ? (lev_flag ? v.levee_index[j] : -1) : lv0);
+    const bool lev_cell = LEV ? (lv >= 0) : lev_flag;
     // after the first sub-step's stage pass every catchment without a levee
     // holds +0.0 protected storage
-    const STO ps = (substep > 0 && !lev_cell) ? (STO)0.0 : v.protected_storage[e];
+    const STO ps = (substep > 0 && !lev_flag) ? (STO)0.0 : v.protected_storage[e];
     // the downstream limiter scales only the catchment's own negative flows
     const int64_t ed = v.el(t, d);
     const bool own_back = !(qr >= (REAL)0.0) || !(qf >= (REAL)0.0);
@@ -1587,7 +1597,7 @@ __device__ __forceinline__ void stage_fused_one(const VT& v, int64_t t, int64_t 
     TableRow<REAL, LFIX> row;
     bool row_loaded = false;
     REAL ca = (REAL)-1;
-    const bool near = lev_cell || (REAL)(rs + fs + ps) > (REAL)0.9 * (rl * rw * rh);
+    const bool near = lev_flag || (REAL)(rs + fs + ps) > (REAL)0.9 * (rl * rw * rh);
     if (LFIX > 0 && near) {
         row.load(v.bat(CMF_BATCHED_FLOOD_DEPTH_TABLE) ? v.table + e * v.L : v.table + j * v.L);
         row_loaded = true;
