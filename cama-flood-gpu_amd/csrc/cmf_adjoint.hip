@@ -631,7 +631,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     const REAL rin = (REAL)ri, fin = (REAL)fi;
     // bifurcation total: the path nets in path order, as bif_nets forms them
     STO gsum = (STO)0;
-    if (a.bif) {
+    // a byte says whether j has paths (and a levee) before any index is read
+    const bool events = a.bif && (v.ev_flag ? v.ev_flag[j] != 0 : v.ev_ptr[j] != v.ev_ptr[j + 1]);
+    if (events) {
         for (int32_t k = v.ev_ptr[j]; k < v.ev_ptr[j + 1]; ++k) {
             const int32_t code = v.ev_code[k];
             const REAL l_other = a.lim[v.ev_other[k]];
@@ -644,7 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     }
     const REAL gb = (REAL)gsum;
     const STO rs = adj_sto<STO>(a.rs_raw, j), fs = adj_sto<STO>(a.fs_raw, j);
-    const int32_t lv = a.lev ? v.levee_index[j] : -1;
+    const int32_t lv = (a.lev && v.is_levee[j]) ? v.levee_index[j] : -1;
     const STO ps = (lv >= 0) ? adj_sto<STO>(a.ps_raw, j) : (STO)0.0;
     const REAL ro = v.runoff[j];
     // storage update (storage.py:81-94)
@@ -766,7 +768,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_stage(View<REAL, STO> v, AdjView
     a.a_ri[j] = (REAL)Lrin;
     a.a_fi[j] = (REAL)Lfin;
     // read by the path reverse and the limiter reverse at path ends only
-    if (a.bif && v.ev_ptr[j] != v.ev_ptr[j + 1]) a.a_gb[j] = (REAL)(-Lb * (double)dt);
+    if (events) a.a_gb[j] = (REAL)(-Lb * (double)dt);
     a.lin[ADJ_RS][j] = (REAL)Lrs;
     a.lin[ADJ_FS][j] = (REAL)Lfs;
     a.g_h[j] += LH - pend;
@@ -983,8 +985,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
         if (sifd > (REAL)1e-5) {
             const double w = (double)v.bif_width[k], man = (double)v.bif_manning[k];
             const double s = (double)sifd, dd = (double)den;
-            const double Lnum = Lql / dd, Lden = -Lql * (double)q / dd;
-            const double Q = 1.0 / (s * s * cbrt(s));
+            const double rdd = 1.0 / dd;
+            const double Lnum = Lql * rdd, Lden = -Lql * (double)q * rdd;
+            const double r3 = rcbrt_pos_d(s);
+            const double rs3 = r3 * r3 * r3;     // 1 / s
+            const double Q = rs3 * rs3 * r3;     // s^(-7/3)
             const double u = (double)unit, au = fabs(u);
             const double sgn = u > 0.0 ? 1.0 : (u < 0.0 ? -1.0 : 0.0);
             // num = w (q_old / w + g dt sifd slope)
@@ -993,7 +998,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_path(View<REAL, STO> v, AdjView<
             Lslope += Lnum * w * g * dt * s;
             // den = 1 + g dt n^2 |q_old / w| sifd^(-7/3)
             Lqold += Lden * g * dt * man * man * sgn * Q / w;
-            Lsifd += Lden * g * dt * man * man * au * (-7.0 / 3.0) * Q / s;
+            Lsifd += Lden * g * dt * man * man * au * (-7.0 / 3.0) * (Q * rs3);
         }
         // the next depth is upd; sifd = max(sqrt(upd csd_old), sqrt(upd 0.01)),
         // with levees sqrt(upd csd_old) (upd where that is not positive) at
@@ -1166,52 +1171,59 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
     const double Lqf0 = o.killed ? 0.0 : Lqf1;
     double Lro = 0.0, Lfo = 0.0, Lr_sifd = 0.0, Lf_sifd = 0.0, Lf_imp = 0.0, Lslope = 0.0, Lfslope = 0.0;
     double gn = 0.0;
+    // (reciprocals formed once; the cube-root powers as powers of s^(-1/3))
     if (o.flood_cond) {
         const double fo = (double)a.q_f[i];
         const double fm = (double)v.flood_manning[i];
-        const double Lnum = Lqf0 / (double)o.den_f, Lden = -Lqf0 * (double)o.qf0 / (double)o.den_f;
+        const double rden = 1.0 / (double)o.den_f;
+        const double Lnum = Lqf0 * rden, Lden = -Lqf0 * (double)o.qf0 * rden;
         Lfo += Lnum;
         Lf_imp += Lnum * g * dt * (double)o.fslope;
         Lfslope += Lnum * g * dt * (double)o.f_imp;
         const double cf = g * dt * fm * fm;
-        const double fs_ = (double)o.f_sifd, fimp = (double)o.f_imp;
-        const double P = fs_ * cbrt(fs_);
+        const double fs_ = (double)o.f_sifd, rimp = 1.0 / (double)o.f_imp;
+        const double r3 = rcbrt_pos_d(fs_);
+        const double rfs = r3 * r3 * r3;   // 1 / fs
+        const double rP = rfs * r3;        // 1 / (fs cbrt(fs)) = fs^(-4/3)
         const double sgn = fo > 0.0 ? 1.0 : (fo < 0.0 ? -1.0 : 0.0);
-        Lfo += Lden * cf * sgn / (P * fimp);
-        Lf_sifd += Lden * (-cf * fabs(fo) / fimp) * (4.0 / 3.0) / (P * fs_);
-        Lf_imp += Lden * (-cf * fabs(fo) / (P * fimp * fimp));
+        Lfo += Lden * cf * sgn * (rP * rimp);
+        Lf_sifd += Lden * (-cf * fabs(fo) * rimp) * (4.0 / 3.0) * (rP * rfs);
+        Lf_imp += Lden * (-cf * fabs(fo) * (rP * rimp * rimp));
     }
     double Lupd_fcsa = (double)a.lout[ADJ_FCSA][i], Lfcsa = 0.0;
     if (o.flood_cond && o.f_imp_raw > (REAL)1e-6) {
-        const double fr = (double)o.f_imp_raw;
-        Lupd_fcsa += Lf_imp * 0.5 * (double)o.f_csa_m / fr;
-        if (a.fcsa[i] > (REAL)1e-6) Lfcsa += Lf_imp * 0.5 * (double)o.upd_fcsa / fr;
+        const double h = Lf_imp * 0.5 / (double)o.f_imp_raw;
+        Lupd_fcsa += h * (double)o.f_csa_m;
+        if (a.fcsa[i] > (REAL)1e-6) Lfcsa += h * (double)o.upd_fcsa;
     }
     if (o.river_cond) {
         const double rw = (double)v.river_width[i], n = (double)v.river_manning[i];
-        const double Lnum = Lqr0 / (double)o.den_r, Lden = -Lqr0 * (double)o.qr0 / (double)o.den_r;
+        const double rden = 1.0 / (double)o.den_r;
+        const double Lnum = Lqr0 * rden, Lden = -Lqr0 * (double)o.qr0 * rden;
         const double rsf = (double)o.r_sifd;
         Lro += Lnum;
         Lr_sifd += Lnum * rw * g * dt * (double)o.slope;
         Lslope += Lnum * rw * g * dt * rsf;
-        const double Q = 1.0 / (rsf * rsf * cbrt(rsf));
+        const double r3 = rcbrt_pos_d(rsf);
+        const double rrs = r3 * r3 * r3;     // 1 / rsf
+        const double Q = rrs * rrs * r3;     // 1 / (rsf^2 cbrt(rsf)) = rsf^(-7/3)
         const double unit = (double)o.unit_r, au = fabs(unit);
         const double sgn = unit > 0.0 ? 1.0 : (unit < 0.0 ? -1.0 : 0.0);
         gn += Lden * 2.0 * g * dt * n * au * Q;
         Lro += Lden * g * dt * n * n * sgn / rw * Q;
-        Lr_sifd += Lden * g * dt * n * n * au * (-7.0 / 3.0) * Q / rsf;
+        Lr_sifd += Lden * g * dt * n * n * au * (-7.0 / 3.0) * (Q * rrs);
     }
     double Lupd_rcsd = (double)a.lout[ADJ_RCSD][i], Lupd_fcsd = (double)a.lout[ADJ_FCSD][i], Lrcsd = 0.0,
            Lfcsd = 0.0;
     if (o.f_sifd_raw > (REAL)1e-6) {
-        const double fr = (double)o.f_sifd_raw;
-        Lupd_fcsd += Lf_sifd * 0.5 * (double)a.fcsd[i] / fr;
-        Lfcsd += Lf_sifd * 0.5 * (double)o.upd_fcsd / fr;
+        const double h = Lf_sifd * 0.5 / (double)o.f_sifd_raw;
+        Lupd_fcsd += h * (double)a.fcsd[i];
+        Lfcsd += h * (double)o.upd_fcsd;
     }
     if (o.r_sifd_raw > (REAL)1e-6) {
-        const double rr = (double)o.r_sifd_raw;
-        Lupd_rcsd += Lr_sifd * 0.5 * (double)a.rcsd[i] / rr;
-        Lrcsd += Lr_sifd * 0.5 * (double)o.upd_rcsd / rr;
+        const double h = Lr_sifd * 0.5 / (double)o.r_sifd_raw;
+        Lupd_rcsd += h * (double)a.rcsd[i];
+        Lrcsd += h * (double)o.upd_rcsd;
     }
     double Lfs = 0.0, Lfd = 0.0, Lrd = 0.0, Lwse = 0.0, Lwse_dn = 0.0, Lmx = 0.0, Lre = 0.0;
     if (o.upd_fcsa_raw > (REAL)0.0) {
@@ -1230,7 +1242,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
     }
     if (o.slope_in) Lslope += Lfslope;
     double Lpd = 0.0;
-    if (a.bif) {
+    if (a.bif && (v.ev_flag ? v.ev_flag[i] != 0 : v.ev_ptr[i] != v.ev_ptr[i + 1])) {
         // the paths of i read its water surface, total storage and (levee
         // catchment) protected depth (B2)
         for (int32_t e = v.ev_ptr[i]; e < v.ev_ptr[i + 1]; ++e) {
@@ -1246,9 +1258,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_outflow(View<REAL, STO> v, AdjVi
         a.lin_pd[i] = (REAL)Lpd;
     }
     if (RES) a.lin_ri[i] = (REAL)Lri;
-    const double dist = (double)v.dist[i];
-    Lwse += Lslope / dist;
-    Lwse_dn -= Lslope / dist;
+    const double Lsd = Lslope / (double)v.dist[i];
+    Lwse += Lsd;
+    Lwse_dn -= Lsd;
     if (o.mx_own) Lwse += Lmx;
     else Lwse_dn += Lmx;
     Lrd += Lwse;

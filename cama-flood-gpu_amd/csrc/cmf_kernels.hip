@@ -215,15 +215,18 @@ __device__ __forceinline__ void outflow_one(const VT& v, int substep, int64_t bl
     }
 
     if (v.tape_q_r) {
-        v.tape_q_r[e] = r_out;
-        v.tape_q_f[e] = f_out;
-        v.tape_rcsd[e] = r_csd;
-        v.tape_fcsd[e] = f_csd;
-        v.tape_fcsa[e] = f_csa;
-        v.tape_rd[e] = r_dep;
-        v.tape_fd[e] = f_dep;
-        v.tape_rs[e] = v.river_storage[e];
-        v.tape_fs[e] = v.flood_storage[e];
+        // the input state goes to the tape with non-temporal stores: the
+        // reverse sweep reads it long after, so it need not displace the
+        // fields the rest of the sub-step reads from the caches
+        __builtin_nontemporal_store(r_out, v.tape_q_r + e);
+        __builtin_nontemporal_store(f_out, v.tape_q_f + e);
+        __builtin_nontemporal_store(r_csd, v.tape_rcsd + e);
+        __builtin_nontemporal_store(f_csd, v.tape_fcsd + e);
+        __builtin_nontemporal_store(f_csa, v.tape_fcsa + e);
+        __builtin_nontemporal_store(r_dep, v.tape_rd + e);
+        __builtin_nontemporal_store(f_dep, v.tape_fd + e);
+        __builtin_nontemporal_store(rs_s, v.tape_rs + e);
+        __builtin_nontemporal_store(fs_s, v.tape_fs + e);
         // levee catchments only: every other catchment holds zero protected
         // storage (the adjoint takes it as zero there)
         if (v.tape_ps && lev_i) {
