@@ -42,7 +42,7 @@ def log(*a):
 
 # Algorithmic bytes per element of each fused kernel (f32 fluxes, f64
 # storage), counting each field the kernel must read or write once; gathers
-# from the downstream catchment are taken as cache hits.  See DESIGN.md.
+# from the downstream catchment are taken as cache hits.
 def algorithmic_bytes(kernel: str, model, f_wet: float) -> float:
     N, T, P, V = model.N, model.T, model.P, model.V
     lev = V > 0
